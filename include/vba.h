@@ -1,0 +1,264 @@
+/*
+ * vba.h — C ABI of the MI355X bundle-adjustment library (libvba.so).
+ *
+ * The library is a drop-in for the Levenberg–Marquardt Schur-complement
+ * bundle-adjustment path of visloc-rs: the `BaAccelerator` trait
+ * (pipelines/slam/src/ba_accel.rs:22-32) whose reference implementation is
+ * `GpuBundleAdjuster` (crates/ba-gpu/src/solver.rs:729-742), with the LM
+ * semantics of `BundleAdjustment::optimize` (pipelines/slam/src/bundle/
+ * optimize.rs:8-28 → lm.rs:162-1017) and of the matrix-free entry
+ * `optimize_matrix_free` (optimize.rs:124-139).
+ *
+ * Conventions (same as the reference's native modules,
+ * native/descriptor_gemm/descriptor_gemm.cu:103-128):
+ *   - every entry point is `extern "C"`, takes plain pointers and sizes;
+ *   - a context is created once per device and destroyed by the caller;
+ *   - errors are negative return codes plus a message from vba_last_error;
+ *   - the library copies all inputs to the device and keeps no pointer to
+ *     caller memory after a call returns.
+ *
+ * Ids are mapped to dense indices in ascending id order, exactly as the
+ * reference builds its variable layout from `BTreeMap` keys
+ * (lm.rs:234-249; solver.rs:256-280).
+ */
+#ifndef VBA_H
+#define VBA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VBA_ABI_VERSION 1u
+
+/* ---------------------------------------------------------------------------
+ * Status codes. VBA_DECLINED is the `None` of BaAccelerator::optimize
+ * (ba_accel.rs:24-31): the caller falls back to its CPU solver. Negative codes
+ * map one-to-one onto `BaError` (pipelines/slam/src/bundle/mod.rs:1021-1056);
+ * the offending id of MissingPose / MissingLandmark is returned in
+ * vba_result.error_id.
+ * ------------------------------------------------------------------------- */
+enum vba_status {
+    VBA_OK = 0,
+    VBA_DECLINED = 1,
+    VBA_ERR_NO_POSES = -1,                 /* BaError::NoPoses */
+    VBA_ERR_NO_LANDMARKS = -2,             /* BaError::NoLandmarks */
+    VBA_ERR_NO_OBSERVATIONS = -3,          /* BaError::NoObservations */
+    VBA_ERR_ALL_POSES_FIXED = -4,          /* BaError::AllPosesFixed */
+    VBA_ERR_MISSING_POSE = -5,             /* BaError::MissingPose(id) */
+    VBA_ERR_MISSING_LANDMARK = -6,         /* BaError::MissingLandmark(id) */
+    VBA_ERR_UNSUPPORTED_CAMERA_MODEL = -7, /* BaError::UnsupportedCameraModel */
+    VBA_ERR_MISSING_STEREO_BASELINE = -8,  /* BaError::MissingStereoBaseline */
+    VBA_ERR_SINGULAR_SYSTEM = -13,         /* BaError::SingularSystem */
+    /* Matrix-free entry errors (MatrixFreeBaError, mod.rs:893-910). */
+    VBA_ERR_INELIGIBLE = -20,              /* MatrixFreeBaError::Ineligible */
+    VBA_ERR_INVALID_CONFIGURATION = -21,   /* MatrixFreeBaError::InvalidConfiguration */
+    VBA_ERR_LINEAR_SOLVE = -22,            /* MatrixFreeBaError::LinearSolve */
+    /* Library errors with no BaError counterpart. */
+    VBA_ERR_INVALID_ARGUMENT = -100,
+    VBA_ERR_DEVICE = -101,
+    VBA_ERR_OUT_OF_MEMORY = -102
+};
+
+/* Camera models: the COLMAP pinhole family that `BundleAdjustment` admits on
+ * this path (crates/core/src/types/camera.rs:15-41). Parameter layouts:
+ *   PINHOLE        [fx, fy, cx, cy (, k1, k2)]
+ *   SIMPLE_PINHOLE [f, cx, cy]
+ *   SIMPLE_RADIAL  [f, cx, cy, k]
+ *   RADIAL         [f, cx, cy, k1, k2]
+ *   OPENCV         [fx, fy, cx, cy, k1, k2, p1, p2]                        */
+enum vba_camera_model {
+    VBA_CAMERA_PINHOLE = 0,
+    VBA_CAMERA_SIMPLE_PINHOLE = 1,
+    VBA_CAMERA_SIMPLE_RADIAL = 2,
+    VBA_CAMERA_RADIAL = 3,
+    VBA_CAMERA_OPENCV = 4
+};
+
+/* Robust kernels, `RobustKernel` (pipelines/slam/src/pose_graph.rs:87-140). */
+enum vba_robust_kernel {
+    VBA_KERNEL_NONE = 0,
+    VBA_KERNEL_HUBER = 1, /* robust_scale = delta */
+    VBA_KERNEL_CAUCHY = 2 /* robust_scale = c */
+};
+
+/* Reduced-camera solvers. DENSE is `LinearSolver::Dense` (dense Schur +
+ * Cholesky with LU fallback, normal_equations.rs:1403-1504); MATRIX_FREE is
+ * the implicit-Schur block-Jacobi PCG of `optimize_matrix_free`
+ * (implicit_schur.rs:133-824); PCG_EXPLICIT is the same PCG on an explicitly
+ * assembled block-sparse Schur matrix. */
+enum vba_linear_solver {
+    VBA_SOLVER_DENSE = 0,
+    VBA_SOLVER_MATRIX_FREE = 2,
+    VBA_SOLVER_PCG_EXPLICIT = 3
+};
+
+/* Arithmetic of the device solve. FP64 is the plain double-precision solver.
+ * MIXED keeps residual/Jacobian storage, Schur products and PCG matvecs in
+ * FP32, the 3x3 / 6x6 block inverses, cost sums, the outer PCG residual and
+ * the LM step in FP64 (iterative refinement of the step). */
+enum vba_precision {
+    VBA_PRECISION_FP64 = 0,
+    VBA_PRECISION_MIXED = 1
+};
+
+/* What a PCG solve that reaches max_pcg_iterations without meeting its
+ * target does. REJECT is the CPU matrix-free semantics
+ * (implicit_schur.rs:663-690: MaxIterations → failed solve → LM reject with
+ * lambda increase); ACCEPT_ITERATE is the reference GPU backend's
+ * (crates/ba-gpu/src/solver.rs:497-508: use the last iterate). */
+enum vba_pcg_maxit_policy {
+    VBA_PCG_MAXIT_REJECT = 0,
+    VBA_PCG_MAXIT_ACCEPT_ITERATE = 1
+};
+
+/* Terms of `BundleAdjustment` that this ABI does not carry. A caller sets the
+ * bit when its problem has them; the library then declines (as
+ * GpuBundleAdjuster::check does, solver.rs:212-251). */
+enum vba_declined_feature {
+    VBA_FEATURE_STEREO_OBSERVATIONS = 1u << 0,
+    VBA_FEATURE_GENERAL_STEREO = 1u << 1,
+    VBA_FEATURE_PRIORS = 1u << 2,          /* gravity / position / pairwise */
+    VBA_FEATURE_INERTIAL = 1u << 3,        /* velocities, biases, IMU, nav prior */
+    VBA_FEATURE_FIXED_ROTATIONS = 1u << 4, /* fixed_pose_rotations */
+    VBA_FEATURE_OBSERVATION_WEIGHTS = 1u << 5
+};
+
+typedef struct vba_camera {
+    int32_t model;       /* enum vba_camera_model */
+    uint32_t width;
+    uint32_t height;
+    uint32_t num_params;
+    double params[12];
+} vba_camera;
+
+/* A rig sensor: its lens and the fixed extrinsic T_sensor<-rig
+ * (BaRigObservation::{camera, sensor_from_rig}, bundle/mod.rs:244-251). */
+typedef struct vba_sensor {
+    vba_camera camera;
+    double sensor_from_rig[7]; /* qw qx qy qz tx ty tz */
+} vba_sensor;
+
+/* Caller-owned view of a `BundleAdjustment` (bundle/mod.rs:529-622).
+ * Poses are T_world->camera (T_rig<-world for rig observations) as a unit
+ * quaternion (w, x, y, z) and a translation, 7 doubles each. */
+typedef struct vba_problem {
+    vba_camera camera;               /* BundleAdjustment::camera */
+    uint64_t num_poses;
+    const uint64_t* pose_ids;        /* unique */
+    const double* poses;             /* 7 per pose */
+    const uint8_t* pose_fixed;       /* nullable; nonzero = in fixed_poses */
+    uint64_t num_landmarks;
+    const uint64_t* landmark_ids;    /* unique */
+    const double* landmarks;         /* 3 per landmark */
+    const uint8_t* landmark_fixed;   /* nullable; nonzero = in fixed_landmarks */
+    uint64_t num_observations;       /* BaObservation list (mod.rs:193-199) */
+    const uint64_t* obs_pose_ids;
+    const uint64_t* obs_landmark_ids;
+    const double* obs_xy;            /* 2 per observation */
+    uint64_t num_rig_observations;   /* BaRigObservation list (mod.rs:244-251) */
+    const uint64_t* rig_pose_ids;
+    const uint64_t* rig_landmark_ids;
+    const double* rig_xy;            /* 2 per rig observation */
+    const uint32_t* rig_sensor;      /* index into sensors */
+    uint64_t num_sensors;
+    const vba_sensor* sensors;
+    uint32_t declined_features;      /* enum vba_declined_feature bits */
+    uint32_t reserved;
+} vba_problem;
+
+/* `BaConfig` (bundle/mod.rs:625-752) plus the PCG settings of
+ * `MatrixFreeBaOptions` (mod.rs:780-798) and `GpuBaSettings`
+ * (crates/ba-gpu/src/solver.rs:24-50). vba_default_options gives
+ * BaConfig::default() and MatrixFreeBaOptions::default(). */
+typedef struct vba_options {
+    uint32_t max_iterations;
+    uint32_t has_initial_lambda;     /* 0 = None: pure Gauss-Newton */
+    double initial_lambda;
+    double lambda_increase_factor;
+    double lambda_decrease_factor;
+    double max_lambda;
+    double min_lambda;
+    double step_tolerance;
+    double cost_tolerance;
+    uint32_t has_relative_cost_tolerance;
+    uint32_t robust_kernel;          /* enum vba_robust_kernel */
+    double relative_cost_tolerance;
+    double robust_scale;
+    uint32_t linear_solver;          /* enum vba_linear_solver */
+    uint32_t precision;              /* enum vba_precision */
+    uint32_t max_pcg_iterations;
+    uint32_t pcg_maxit_policy;       /* enum vba_pcg_maxit_policy */
+    double pcg_relative_tolerance;
+    double pcg_absolute_tolerance;
+    uint32_t refine_intrinsics;      /* nonzero: declined (joint intrinsics) */
+    uint32_t reserved;
+} vba_options;
+
+/* `BaIterationStats` (mod.rs:754-763) plus the matrix-free diagnostics of
+ * `MatrixFreeBaIterationStats` (mod.rs:810-818). */
+typedef struct vba_iteration_stats {
+    uint32_t iteration;
+    uint32_t step_accepted;
+    double cost_before;
+    double cost_after;
+    double max_pose_step;
+    double max_landmark_step;
+    double lambda;                   /* post-update lambda, as the reference records */
+    int32_t pcg_iterations;          /* -1 when the solver is not PCG */
+    uint32_t nonprojectable_after;
+    double pcg_residual_norm;
+    double pcg_target;
+} vba_iteration_stats;
+
+/* `BaResult` (mod.rs:765-771) plus counters the harness reports. */
+typedef struct vba_result {
+    double initial_cost;
+    double final_cost;
+    uint32_t num_iterations;         /* entries written to the stats array (capped) */
+    uint32_t converged;
+    uint64_t error_id;               /* id for MISSING_POSE / MISSING_LANDMARK */
+    uint64_t num_observations_used;  /* N of the obs/s metric */
+    uint32_t total_pcg_iterations;
+    uint32_t linearizations;
+    double device_seconds;           /* device time of the solve (HIP events) */
+} vba_result;
+
+/* ---- lifecycle (the reference's *_abi_version/_create/_destroy/_last_error) */
+uint32_t vba_abi_version(void);
+/* Create a context on a HIP device ordinal. NULL on failure (see
+ * vba_last_error(NULL)). */
+struct vba_context* vba_create(int32_t device);
+void vba_destroy(struct vba_context* ctx);
+/* Last error message of ctx, or of the calling thread when ctx is NULL. */
+const char* vba_last_error(const struct vba_context* ctx);
+
+typedef struct vba_context vba_context;
+
+/* BaConfig::default() + MatrixFreeBaOptions::default()
+ * (mod.rs:729-752, 789-798). */
+void vba_default_options(vba_options* options);
+
+/* GpuBundleAdjuster::check (solver.rs:212-251), widened to the lens models
+ * and kernels this library implements: returns VBA_OK, or VBA_DECLINED with
+ * the reason in vba_last_error(NULL). */
+int32_t vba_check(const vba_problem* problem, const vba_options* options);
+
+/* BaAccelerator::optimize (ba_accel.rs:22-32; solver.rs:552-690, 729-742).
+ * On VBA_OK the final poses (7 per pose) and landmarks (3 per landmark), in
+ * the caller's input order, are written to poses_out / landmarks_out (which
+ * may alias the input arrays), and up to stats_capacity iteration records to
+ * stats. On an error nothing is written to poses_out / landmarks_out, as the
+ * reference leaves `ba` untouched when validation fails. */
+int32_t vba_optimize(vba_context* ctx, const vba_problem* problem,
+                     const vba_options* options, double* poses_out,
+                     double* landmarks_out, vba_iteration_stats* stats,
+                     uint32_t stats_capacity, vba_result* result);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* VBA_H */

@@ -1,0 +1,1813 @@
+// ba_oracle.cpp — CPU FP64 restatement of the reference bundle adjuster.
+//
+// TEST INFRASTRUCTURE ONLY (the checker and the timed CPU baseline): see
+// ba_oracle.h. It restates, function by function, the Levenberg–Marquardt
+// Schur-complement path of visloc-rs:
+//   pipelines/slam/src/bundle/lm.rs:162-1017              (LM loop)
+//   pipelines/slam/src/bundle/normal_equations.rs:352-651 (assembly)
+//   pipelines/slam/src/bundle/normal_equations.rs:1314-1592 (dense Schur)
+//   pipelines/slam/src/bundle/implicit_schur.rs:133-824   (matrix-free PCG)
+//   pipelines/slam/src/bundle/matrix_free.rs:1106-1240
+//   pipelines/slam/src/bundle/problem.rs:252-303, 497-711 (cost, feasibility)
+//   pipelines/slam/src/bundle/jacobians.rs:29-124         (projection)
+//   pipelines/slam/src/bundle/sqrt_qr.rs:445-463, 960-1005 (rig residual)
+//   pipelines/slam/src/pose_graph.rs:87-140, 2259-2270    (kernels, solve)
+//   crates/core/src/geometry/se3.rs, pose.rs
+//   crates/core/src/types/camera.rs:213-433, 844-941
+// Arithmetic follows the evaluation order of those sources and of the
+// nalgebra 0.33.3 routines they call (quaternion product and rotation,
+// 3x3 cofactor inverse, column Cholesky, partial-pivot LU, the 8-way
+// unrolled dynamic dot product), so that a change of operation order shows up
+// in the tests. Bit equality with the Rust binary itself cannot be checked
+// here (no Rust toolchain); the known-answer tests pin what they can.
+//
+// Extension (not in the reference): the matrix-free path accepts every lens
+// model of the dense path (the reference's matrix-free validation admits
+// only distortion-free PINHOLE / SIMPLE_PINHOLE, optimize.rs:462-485). For
+// those two models the behaviour is the reference's.
+
+#include "ba_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <thread>
+#include <vector>
+
+namespace {
+
+constexpr double kEps = 1e-12;  // camera.rs:5
+
+// ---------------------------------------------------------------------------
+// Fixed-size row-major matrices; products sum over k left to right, as
+// nalgebra's gemv/axpy column sweep does for statically sized operands.
+// ---------------------------------------------------------------------------
+template <int R, int C>
+struct Mat {
+    double a[R * C];
+    double& operator()(int r, int c) { return a[r * C + c]; }
+    double operator()(int r, int c) const { return a[r * C + c]; }
+    static Mat zeros() {
+        Mat m;
+        for (int i = 0; i < R * C; ++i) m.a[i] = 0.0;
+        return m;
+    }
+};
+using M3 = Mat<3, 3>;
+using M23 = Mat<2, 3>;
+using M26 = Mat<2, 6>;
+using M36 = Mat<3, 6>;
+using M63 = Mat<6, 3>;
+using M66 = Mat<6, 6>;
+
+template <int R, int K, int C>
+Mat<R, C> mul(const Mat<R, K>& A, const Mat<K, C>& B) {
+    Mat<R, C> out;
+    for (int r = 0; r < R; ++r)
+        for (int c = 0; c < C; ++c) {
+            double acc = A(r, 0) * B(0, c);
+            for (int k = 1; k < K; ++k) acc = A(r, k) * B(k, c) + acc;
+            out(r, c) = acc;
+        }
+    return out;
+}
+template <int R, int C>
+Mat<C, R> tr(const Mat<R, C>& A) {
+    Mat<C, R> out;
+    for (int r = 0; r < R; ++r)
+        for (int c = 0; c < C; ++c) out(c, r) = A(r, c);
+    return out;
+}
+template <int R, int C>
+Mat<R, C> scale(double s, const Mat<R, C>& A) {
+    Mat<R, C> out;
+    for (int i = 0; i < R * C; ++i) out.a[i] = s * A.a[i];
+    return out;
+}
+template <int R, int C>
+Mat<R, C> neg(const Mat<R, C>& A) {
+    Mat<R, C> out;
+    for (int i = 0; i < R * C; ++i) out.a[i] = -A.a[i];
+    return out;
+}
+template <int R, int C>
+Mat<R, C> add(const Mat<R, C>& A, const Mat<R, C>& B) {
+    Mat<R, C> out;
+    for (int i = 0; i < R * C; ++i) out.a[i] = A.a[i] + B.a[i];
+    return out;
+}
+template <int R, int C>
+void add_assign(Mat<R, C>& A, const Mat<R, C>& B) {
+    for (int i = 0; i < R * C; ++i) A.a[i] = A.a[i] + B.a[i];
+}
+template <int R, int C>
+void sub_assign(Mat<R, C>& A, const Mat<R, C>& B) {
+    for (int i = 0; i < R * C; ++i) A.a[i] = A.a[i] - B.a[i];
+}
+template <int R, int C>
+bool all_finite(const Mat<R, C>& A) {
+    for (int i = 0; i < R * C; ++i)
+        if (!std::isfinite(A.a[i])) return false;
+    return true;
+}
+// Matrix (R x K) times vector (K).
+template <int R, int K>
+void mulv(const Mat<R, K>& A, const double* x, double* y) {
+    for (int r = 0; r < R; ++r) {
+        double acc = A(r, 0) * x[0];
+        for (int k = 1; k < K; ++k) acc = A(r, k) * x[k] + acc;
+        y[r] = acc;
+    }
+}
+
+struct V3 {
+    double x, y, z;
+};
+inline V3 cross(const V3& a, const V3& b) {  // nalgebra Vector3::cross
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+inline double norm_sq3(const V3& v) { return v.x * v.x + v.y * v.y + v.z * v.z; }
+inline M3 skew(const V3& v) {  // se3.rs:102-104, sqrt_qr.rs:1003-1005
+    M3 m;
+    m(0, 0) = 0.0; m(0, 1) = -v.z; m(0, 2) = v.y;
+    m(1, 0) = v.z; m(1, 1) = 0.0; m(1, 2) = -v.x;
+    m(2, 0) = -v.y; m(2, 1) = v.x; m(2, 2) = 0.0;
+    return m;
+}
+inline M3 identity3() {
+    M3 m = M3::zeros();
+    m(0, 0) = m(1, 1) = m(2, 2) = 1.0;
+    return m;
+}
+
+// nalgebra DVector::dot: 8-accumulator unrolled sum (generic dotx path).
+double dyn_dot(const double* a, const double* b, size_t n) {
+    double res = 0.0;
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    size_t i = 0;
+    while (n - i >= 8) {
+        for (int k = 0; k < 8; ++k) acc[k] += a[i + k] * b[i + k];
+        i += 8;
+    }
+    res += acc[0] + acc[4];
+    res += acc[1] + acc[5];
+    res += acc[2] + acc[6];
+    res += acc[3] + acc[7];
+    for (size_t k = i; k < n; ++k) res += a[k] * b[k];
+    return res;
+}
+double dyn_norm(const std::vector<double>& v) { return std::sqrt(dyn_dot(v.data(), v.data(), v.size())); }
+bool all_finite(const std::vector<double>& v) {
+    for (double x : v)
+        if (!std::isfinite(x)) return false;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Unit quaternion and SE3 (crates/core/src/geometry/se3.rs).
+// ---------------------------------------------------------------------------
+struct Quat {
+    double w, i, j, k;
+};
+inline Quat qmul(const Quat& a, const Quat& b) {  // nalgebra Quaternion Mul
+    return {a.w * b.w - a.i * b.i - a.j * b.j - a.k * b.k,
+            a.w * b.i + a.i * b.w + a.j * b.k - a.k * b.j,
+            a.w * b.j - a.i * b.k + a.j * b.w + a.k * b.i,
+            a.w * b.k + a.i * b.j - a.j * b.i + a.k * b.w};
+}
+inline Quat qconj(const Quat& q) { return {q.w, -q.i, -q.j, -q.k}; }
+// UnitQuaternion × Vector3: t = 2 (v × p); v × t; (t w + v×t) + p.
+inline V3 qrot(const Quat& q, const V3& p) {
+    V3 v{q.i, q.j, q.k};
+    V3 c = cross(v, p);
+    V3 t{c.x * 2.0, c.y * 2.0, c.z * 2.0};
+    V3 c2 = cross(v, t);
+    return {t.x * q.w + c2.x + p.x, t.y * q.w + c2.y + p.y, t.z * q.w + c2.z + p.z};
+}
+inline M3 qmatrix(const Quat& q) {  // UnitQuaternion::to_rotation_matrix
+    double i = q.i, j = q.j, k = q.k, w = q.w;
+    double ww = w * w, ii = i * i, jj = j * j, kk = k * k;
+    double ij = i * j * 2.0, wk = w * k * 2.0, wj = w * j * 2.0;
+    double ik = i * k * 2.0, jk = j * k * 2.0, wi = w * i * 2.0;
+    M3 m;
+    m(0, 0) = ww + ii - jj - kk; m(0, 1) = ij - wk; m(0, 2) = wj + ik;
+    m(1, 0) = ij + wk; m(1, 1) = ww - ii + jj - kk; m(1, 2) = jk - wi;
+    m(2, 0) = ik - wj; m(2, 1) = wi + jk; m(2, 2) = ww - ii - jj + kk;
+    return m;
+}
+// UnitQuaternion::from_scaled_axis = Quaternion::from_imag(ω/2).exp()
+// (exp_eps with eps = f64::EPSILON).
+inline Quat from_scaled_axis(const V3& w) {
+    V3 v{w.x / 2.0, w.y / 2.0, w.z / 2.0};
+    double nn = norm_sq3(v);
+    const double eps = std::numeric_limits<double>::epsilon();
+    if (nn <= eps * eps) return {1.0, 0.0, 0.0, 0.0};
+    double w_exp = std::exp(0.0);
+    double n = std::sqrt(nn);
+    double s = w_exp * std::sin(n) / n;
+    return {w_exp * std::cos(n), v.x * s, v.y * s, v.z * s};
+}
+// UnitQuaternion::scaled_axis = axis() * angle(): the axis is the imaginary
+// part, negated when w < 0, normalized (None when its norm is 0); the angle
+// is 2 atan2(|v|, |w|).
+inline V3 scaled_axis(const Quat& q) {
+    V3 v = q.w >= 0.0 ? V3{q.i, q.j, q.k} : V3{-q.i, -q.j, -q.k};
+    double n = std::sqrt(norm_sq3(v));
+    if (n <= 0.0) return {0.0, 0.0, 0.0};
+    V3 axis{v.x / n, v.y / n, v.z / n};
+    double angle = std::atan2(std::sqrt(norm_sq3(V3{q.i, q.j, q.k})), std::fabs(q.w)) * 2.0;
+    return {axis.x * angle, axis.y * angle, axis.z * angle};
+}
+
+struct SE3 {
+    Quat q;
+    V3 t;
+};
+inline V3 se3_apply(const SE3& T, const V3& p) {
+    V3 r = qrot(T.q, p);
+    return {r.x + T.t.x, r.y + T.t.y, r.z + T.t.z};
+}
+inline SE3 se3_compose(const SE3& a, const SE3& b) {  // se3.rs:32-37
+    V3 r = qrot(a.q, b.t);
+    return {qmul(a.q, b.q), {r.x + a.t.x, r.y + a.t.y, r.z + a.t.z}};
+}
+inline SE3 se3_inverse(const SE3& a) {  // se3.rs:39-43
+    Quat qi = qconj(a.q);
+    V3 r = qrot(qi, a.t);
+    return {qi, {-r.x, -r.y, -r.z}};
+}
+M3 so3_left_jacobian(const V3& w) {  // se3.rs:107-120
+    double theta_sq = norm_sq3(w);
+    M3 S = skew(w);
+    if (theta_sq < 1e-10) {
+        return add(add(identity3(), scale(0.5, S)), mul(scale(1.0 / 6.0, S), S));
+    }
+    double theta = std::sqrt(theta_sq);
+    double a = (1.0 - std::cos(theta)) / theta_sq;
+    double b = (theta - std::sin(theta)) / (theta * theta_sq);
+    return add(add(identity3(), scale(a, S)), mul(scale(b, S), S));
+}
+M3 so3_left_jacobian_inverse(const V3& w) {  // se3.rs:122-134
+    double theta_sq = norm_sq3(w);
+    M3 S = skew(w);
+    if (theta_sq < 1e-10) {
+        M3 m = identity3();
+        sub_assign(m, scale(0.5, S));
+        return add(m, mul(scale(1.0 / 12.0, S), S));
+    }
+    double theta = std::sqrt(theta_sq);
+    double half = 0.5 * theta;
+    double c = (1.0 - half * std::cos(half) / std::sin(half)) / theta_sq;
+    M3 m = identity3();
+    sub_assign(m, scale(0.5, S));
+    return add(m, mul(scale(c, S), S));
+}
+SE3 se3_exp(const double xi[6]) {  // se3.rs:72-82
+    V3 rho{xi[0], xi[1], xi[2]};
+    V3 omega{xi[3], xi[4], xi[5]};
+    Quat q = from_scaled_axis(omega);
+    M3 V = so3_left_jacobian(omega);
+    double r[3] = {rho.x, rho.y, rho.z}, t[3];
+    mulv(V, r, t);
+    return {q, {t[0], t[1], t[2]}};
+}
+void se3_log(const SE3& T, double out[6]) {  // se3.rs:57-69
+    V3 omega = scaled_axis(T.q);
+    M3 Vi = so3_left_jacobian_inverse(omega);
+    double t[3] = {T.t.x, T.t.y, T.t.z}, rho[3];
+    mulv(Vi, t, rho);
+    out[0] = rho[0]; out[1] = rho[1]; out[2] = rho[2];
+    out[3] = omega.x; out[4] = omega.y; out[5] = omega.z;
+}
+
+// ---------------------------------------------------------------------------
+// Cameras (crates/core/src/types/camera.rs).
+// ---------------------------------------------------------------------------
+struct Cam {
+    int model;
+    int np;
+    double p[12];
+};
+Cam to_cam(const vba_camera& c) {
+    Cam k;
+    k.model = c.model;
+    k.np = (int)std::min<uint32_t>(c.num_params, 12);
+    for (int i = 0; i < 12; ++i) k.p[i] = c.params[i];
+    return k;
+}
+bool known_model(int m) { return m >= VBA_CAMERA_PINHOLE && m <= VBA_CAMERA_OPENCV; }
+struct Intr {
+    double fx, fy, cx, cy;
+};
+// Camera::intrinsics (camera.rs:241-271).
+bool intrinsics(const Cam& c, Intr* out) {
+    switch (c.model) {
+        case VBA_CAMERA_PINHOLE:
+        case VBA_CAMERA_OPENCV:
+            if (c.np < 4) return false;
+            *out = {c.p[0], c.p[1], c.p[2], c.p[3]};
+            return true;
+        case VBA_CAMERA_SIMPLE_PINHOLE:
+        case VBA_CAMERA_SIMPLE_RADIAL:
+        case VBA_CAMERA_RADIAL:
+            if (c.np < 3) return false;
+            *out = {c.p[0], c.p[0], c.p[1], c.p[2]};
+            return true;
+        default:
+            return false;
+    }
+}
+// Camera::radial_distortion (camera.rs:213-227).
+bool radial_distortion(const Cam& c, double* k1, double* k2) {
+    switch (c.model) {
+        case VBA_CAMERA_PINHOLE:
+        case VBA_CAMERA_OPENCV:
+            if (c.np <= 4) return false;
+            *k1 = c.p[4];
+            *k2 = c.np > 5 ? c.p[5] : 0.0;
+            return true;
+        case VBA_CAMERA_SIMPLE_RADIAL:
+            if (c.np <= 3) return false;
+            *k1 = c.p[3];
+            *k2 = 0.0;
+            return true;
+        case VBA_CAMERA_RADIAL:
+            if (c.np <= 3) return false;
+            *k1 = c.p[3];
+            *k2 = c.np > 4 ? c.p[4] : 0.0;
+            return true;
+        default:
+            return false;
+    }
+}
+// Camera::tangential_distortion (camera.rs:235-242).
+bool tangential_distortion(const Cam& c) {
+    if (c.model != VBA_CAMERA_OPENCV) return false;
+    double p1 = c.np > 6 ? c.p[6] : 0.0;
+    double p2 = c.np > 7 ? c.p[7] : 0.0;
+    return p1 != 0.0 || p2 != 0.0;
+}
+void full_opencv_coeffs(const Cam& c, double k[8]) {  // camera.rs:844-850
+    for (int i = 0; i < 8; ++i) k[i] = (4 + i < c.np) ? c.p[4 + i] : 0.0;
+}
+// project_full_opencv (camera.rs:852-877).
+bool project_full_opencv(double fx, double fy, double cx, double cy, const double k[8], const V3& P,
+                         double uv[2]) {
+    if (P.z <= kEps) return false;
+    double x = P.x / P.z, y = P.y / P.z;
+    double r2 = x * x + y * y;
+    double r2_2 = r2 * r2;
+    double r2_3 = r2_2 * r2;
+    double num = 1.0 + k[0] * r2 + k[1] * r2_2 + k[4] * r2_3;
+    double den = 1.0 + k[5] * r2 + k[6] * r2_2 + k[7] * r2_3;
+    if (std::fabs(den) <= kEps) return false;
+    double radial = num / den;
+    double xd = x * radial + 2.0 * k[2] * x * y + k[3] * (r2 + 2.0 * x * x);
+    double yd = y * radial + k[2] * (r2 + 2.0 * y * y) + 2.0 * k[3] * x * y;
+    uv[0] = fx * xd + cx;
+    uv[1] = fy * yd + cy;
+    return std::isfinite(uv[0]) && std::isfinite(uv[1]);
+}
+// full_opencv_point_jacobian + chain_normalized_jacobian (camera.rs:882-941).
+bool full_opencv_point_jacobian(double fx, double fy, const double k[8], const V3& P, M23* J) {
+    if (P.z <= 0.0) return false;
+    double x = P.x / P.z, y = P.y / P.z;
+    double r2 = x * x + y * y;
+    double r4 = r2 * r2;
+    double num = 1.0 + k[0] * r2 + k[1] * r4 + k[4] * r4 * r2;
+    double den = 1.0 + k[5] * r2 + k[6] * r4 + k[7] * r4 * r2;
+    if (std::fabs(den) <= kEps) return false;
+    double radial = num / den;
+    double dn = k[0] + 2.0 * k[1] * r2 + 3.0 * k[4] * r4;
+    double dd = k[5] + 2.0 * k[6] * r2 + 3.0 * k[7] * r4;
+    double g = (dn * den - num * dd) / (den * den);
+    double p1 = k[2], p2 = k[3];
+    double crs = 2.0 * x * y * g + 2.0 * p1 * x + 2.0 * p2 * y;
+    double a = radial + 2.0 * x * x * g + 2.0 * p1 * y + 6.0 * p2 * x;
+    double b = crs, c = crs;
+    double d = radial + 2.0 * y * y * g + 6.0 * p1 * y + 2.0 * p2 * x;
+    double z_inv = 1.0 / P.z;
+    (*J)(0, 0) = fx * a * z_inv;
+    (*J)(0, 1) = fx * b * z_inv;
+    (*J)(0, 2) = -fx * (a * x + b * y) * z_inv;
+    (*J)(1, 0) = fy * c * z_inv;
+    (*J)(1, 1) = fy * d * z_inv;
+    (*J)(1, 2) = -fy * (c * x + d * y) * z_inv;
+    return true;
+}
+// Camera::project (camera.rs:351-433), pinhole-family branches.
+bool camera_project(const Cam& c, const V3& P, double uv[2]) {
+    if (!(std::isfinite(P.x) && std::isfinite(P.y) && std::isfinite(P.z))) return false;
+    if (!known_model(c.model)) return false;
+    if (c.model == VBA_CAMERA_OPENCV && tangential_distortion(c)) {
+        if (c.np < 4) return false;
+        double k[8];
+        full_opencv_coeffs(c, k);
+        return project_full_opencv(c.p[0], c.p[1], c.p[2], c.p[3], k, P, uv);
+    }
+    if (P.z <= 0.0) return false;
+    Intr in;
+    if (!intrinsics(c, &in)) return false;
+    double x = P.x / P.z;
+    double y = P.y / P.z;
+    double k1, k2;
+    if (radial_distortion(c, &k1, &k2)) {
+        if (k1 != 0.0 || k2 != 0.0) {
+            double r2 = x * x + y * y;
+            double d = 1.0 + k1 * r2 + k2 * r2 * r2;
+            x *= d;
+            y *= d;
+        }
+    }
+    uv[0] = in.fx * x + in.cx;
+    uv[1] = in.fy * y + in.cy;
+    return true;
+}
+
+// MonoProjection (jacobians.rs:18-58).
+enum ProjKind { PROJ_PINHOLE, PROJ_RADIAL, PROJ_LENS };
+struct Proj {
+    ProjKind kind;
+    double k1, k2;
+};
+Proj mono_projection_for(const Cam& c) {
+    bool radial_family = (c.model == VBA_CAMERA_PINHOLE || c.model == VBA_CAMERA_SIMPLE_PINHOLE ||
+                          c.model == VBA_CAMERA_SIMPLE_RADIAL || c.model == VBA_CAMERA_RADIAL) ||
+                         (c.model == VBA_CAMERA_OPENCV && !tangential_distortion(c));
+    if (!radial_family) return {PROJ_LENS, 0.0, 0.0};
+    double k1, k2;
+    if (radial_distortion(c, &k1, &k2) && (k1 != 0.0 || k2 != 0.0)) return {PROJ_RADIAL, k1, k2};
+    return {PROJ_PINHOLE, 0.0, 0.0};
+}
+// project_pinhole (jacobians.rs:115-124).
+bool project_pinhole(const Intr& in, const V3& P, double uv[2]) {
+    if (P.z <= 0.0) return false;
+    uv[0] = in.fx * P.x / P.z + in.cx;
+    uv[1] = in.fy * P.y / P.z + in.cy;
+    return true;
+}
+// Camera::project_with_point_jacobian for the Brown–Conrady lens
+// (camera.rs:449-507; only the OPENCV-with-tangential branch reaches it).
+bool lens_project_with_jacobian(const Cam& c, const V3& P, double uv[2], M23* J) {
+    if (!camera_project(c, P, uv)) return false;
+    Intr in;
+    if (!intrinsics(c, &in)) return false;
+    double k[8];
+    if (c.model == VBA_CAMERA_OPENCV && tangential_distortion(c)) {
+        full_opencv_coeffs(c, k);
+    } else {
+        double k1 = 0.0, k2 = 0.0;
+        if (!(radial_distortion(c, &k1, &k2) && (k1 != 0.0 || k2 != 0.0))) k1 = k2 = 0.0;
+        for (int i = 0; i < 8; ++i) k[i] = 0.0;
+        k[0] = k1;
+        k[1] = k2;
+    }
+    if (!full_opencv_point_jacobian(in.fx, in.fy, k, P, J)) return false;
+    return all_finite(*J);
+}
+// mono_project_with_jacobian (jacobians.rs:63-113).
+bool mono_project_with_jacobian(const Proj& pr, const Cam& c, const Intr& in, const V3& P,
+                                double uv[2], M23* J) {
+    if (P.z <= 0.0) return false;
+    const double fx = in.fx, fy = in.fy, cx = in.cx, cy = in.cy;
+    switch (pr.kind) {
+        case PROJ_PINHOLE: {
+            if (!project_pinhole(in, P, uv)) return false;
+            double z_inv = 1.0 / P.z;
+            (*J)(0, 0) = fx * z_inv;
+            (*J)(0, 1) = 0.0;
+            (*J)(0, 2) = -fx * P.x * z_inv * z_inv;
+            (*J)(1, 0) = 0.0;
+            (*J)(1, 1) = fy * z_inv;
+            (*J)(1, 2) = -fy * P.y * z_inv * z_inv;
+            return true;
+        }
+        case PROJ_RADIAL: {
+            double k1 = pr.k1, k2 = pr.k2;
+            double x = P.x / P.z;
+            double y = P.y / P.z;
+            double r2 = x * x + y * y;
+            double d = 1.0 + k1 * r2 + k2 * r2 * r2;
+            uv[0] = fx * (x * d) + cx;
+            uv[1] = fy * (y * d) + cy;
+            double g = k1 + 2.0 * k2 * r2;
+            double d11 = d + 2.0 * x * x * g;
+            double d12 = 2.0 * x * y * g;
+            double d22 = d + 2.0 * y * y * g;
+            double z_inv = 1.0 / P.z;
+            (*J)(0, 0) = fx * d11 * z_inv;
+            (*J)(0, 1) = fx * d12 * z_inv;
+            (*J)(0, 2) = -fx * (d11 * x + d12 * y) * z_inv;
+            (*J)(1, 0) = fy * d12 * z_inv;
+            (*J)(1, 1) = fy * d22 * z_inv;
+            (*J)(1, 2) = -fy * (d12 * x + d22 * y) * z_inv;
+            return std::isfinite(uv[0]) && std::isfinite(uv[1]) && all_finite(*J);
+        }
+        default:
+            return lens_project_with_jacobian(c, P, uv, J);
+    }
+}
+// pinhole_projection_jacobian (sqrt_qr.rs:445-463).
+bool pinhole_projection_jacobian(const Intr& in, const V3& P, M23* J) {
+    if (P.z <= 0.0) return false;
+    double z_inv = 1.0 / P.z;
+    double z_inv2 = z_inv * z_inv;
+    (*J)(0, 0) = in.fx * z_inv;
+    (*J)(0, 1) = 0.0;
+    (*J)(0, 2) = -in.fx * P.x * z_inv2;
+    (*J)(1, 0) = 0.0;
+    (*J)(1, 1) = in.fy * z_inv;
+    (*J)(1, 2) = -in.fy * P.y * z_inv2;
+    return true;
+}
+
+// RobustKernel::{cost, weight} (pose_graph.rs:102-139).
+struct Kernel {
+    uint32_t kind;
+    double scale;
+    double cost(double s) const {
+        if (kind == VBA_KERNEL_HUBER) {
+            double d = scale, dsq = d * d;
+            return s <= dsq ? s : 2.0 * d * std::sqrt(s) - dsq;
+        }
+        if (kind == VBA_KERNEL_CAUCHY) {
+            double csq = scale * scale;
+            return csq * std::log(1.0 + s / csq);
+        }
+        return s;
+    }
+    double weight(double s) const {
+        if (kind == VBA_KERNEL_HUBER) {
+            double d = scale, dsq = d * d;
+            return s <= dsq ? 1.0 : d / std::sqrt(s);
+        }
+        if (kind == VBA_KERNEL_CAUCHY) {
+            double csq = scale * scale;
+            return 1.0 / (1.0 + s / csq);
+        }
+        return 1.0;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Problem (BundleAdjustment restricted to the terms of this path).
+// ---------------------------------------------------------------------------
+struct Sensor {
+    Cam cam;
+    SE3 sensor_from_rig;
+};
+struct Obs {
+    uint32_t pose;  // index into poses (id order)
+    uint32_t lm;    // index into landmarks (id order)
+    double x, y;
+    int32_t sensor;  // -1 = monocular observation with the shared camera
+};
+struct Problem {
+    Cam camera;
+    std::vector<SE3> poses;
+    std::vector<uint8_t> pose_fixed;
+    std::vector<V3> lms;
+    std::vector<uint8_t> lm_fixed;
+    std::vector<Obs> obs;  // mono observations first, then rig, input order
+    std::vector<Sensor> sensors;
+    size_t num_mono = 0;
+    // input position ↔ id-sorted slot
+    std::vector<uint32_t> pose_slot_of_input, lm_slot_of_input;
+    std::vector<int32_t> pose_var, lm_var;  // variable index or -1
+    uint32_t P = 0, L = 0;                   // variable counts
+};
+
+int32_t sorted_slots(uint64_t n, const uint64_t* ids, std::vector<uint32_t>* slot_of_input,
+                     std::vector<uint64_t>* sorted_ids) {
+    std::vector<uint32_t> order(n);
+    for (uint64_t i = 0; i < n; ++i) order[i] = (uint32_t)i;
+    std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return ids[a] < ids[b]; });
+    slot_of_input->assign(n, 0);
+    sorted_ids->resize(n);
+    for (uint64_t s = 0; s < n; ++s) {
+        (*slot_of_input)[order[s]] = (uint32_t)s;
+        (*sorted_ids)[s] = ids[order[s]];
+        if (s > 0 && (*sorted_ids)[s] == (*sorted_ids)[s - 1]) return VBA_ERR_INVALID_ARGUMENT;
+    }
+    return VBA_OK;
+}
+bool find_slot(const std::vector<uint64_t>& sorted, uint64_t id, uint32_t* slot) {
+    auto it = std::lower_bound(sorted.begin(), sorted.end(), id);
+    if (it == sorted.end() || *it != id) return false;
+    *slot = (uint32_t)(it - sorted.begin());
+    return true;
+}
+SE3 pose_from7(const double* p) { return {{p[0], p[1], p[2], p[3]}, {p[4], p[5], p[6]}}; }
+void pose_to7(const SE3& T, double* p) {
+    p[0] = T.q.w; p[1] = T.q.i; p[2] = T.q.j; p[3] = T.q.k;
+    p[4] = T.t.x; p[5] = T.t.y; p[6] = T.t.z;
+}
+
+// Validation in the order of optimize_weighted_backend (lm.rs:168-229, 234-252).
+int32_t load_problem(const vba_problem* pb, Problem* out, uint64_t* error_id) {
+    *error_id = 0;
+    if (!pb) return VBA_ERR_INVALID_ARGUMENT;
+    Problem& P = *out;
+    P.camera = to_cam(pb->camera);
+    Intr in;
+    if (!known_model(P.camera.model) || !intrinsics(P.camera, &in)) return VBA_ERR_UNSUPPORTED_CAMERA_MODEL;
+    if (pb->num_poses == 0) return VBA_ERR_NO_POSES;
+    bool has_visual = pb->num_observations > 0 || pb->num_rig_observations > 0;
+    if (!has_visual) return VBA_ERR_NO_OBSERVATIONS;
+    if (pb->num_landmarks == 0) return VBA_ERR_NO_LANDMARKS;
+    std::vector<uint64_t> pose_ids, lm_ids;
+    if (sorted_slots(pb->num_poses, pb->pose_ids, &P.pose_slot_of_input, &pose_ids) != VBA_OK)
+        return VBA_ERR_INVALID_ARGUMENT;
+    if (sorted_slots(pb->num_landmarks, pb->landmark_ids, &P.lm_slot_of_input, &lm_ids) != VBA_OK)
+        return VBA_ERR_INVALID_ARGUMENT;
+    P.poses.resize(pb->num_poses);
+    P.pose_fixed.assign(pb->num_poses, 0);
+    for (uint64_t i = 0; i < pb->num_poses; ++i) {
+        uint32_t s = P.pose_slot_of_input[i];
+        P.poses[s] = pose_from7(pb->poses + 7 * i);
+        P.pose_fixed[s] = pb->pose_fixed ? (pb->pose_fixed[i] != 0) : 0;
+    }
+    P.lms.resize(pb->num_landmarks);
+    P.lm_fixed.assign(pb->num_landmarks, 0);
+    for (uint64_t i = 0; i < pb->num_landmarks; ++i) {
+        uint32_t s = P.lm_slot_of_input[i];
+        const double* x = pb->landmarks + 3 * i;
+        P.lms[s] = {x[0], x[1], x[2]};
+        P.lm_fixed[s] = pb->landmark_fixed ? (pb->landmark_fixed[i] != 0) : 0;
+    }
+    P.obs.clear();
+    P.obs.reserve(pb->num_observations + pb->num_rig_observations);
+    for (uint64_t i = 0; i < pb->num_observations; ++i) {
+        Obs o;
+        if (!find_slot(pose_ids, pb->obs_pose_ids[i], &o.pose)) {
+            *error_id = pb->obs_pose_ids[i];
+            return VBA_ERR_MISSING_POSE;
+        }
+        if (!find_slot(lm_ids, pb->obs_landmark_ids[i], &o.lm)) {
+            *error_id = pb->obs_landmark_ids[i];
+            return VBA_ERR_MISSING_LANDMARK;
+        }
+        o.x = pb->obs_xy[2 * i];
+        o.y = pb->obs_xy[2 * i + 1];
+        o.sensor = -1;
+        P.obs.push_back(o);
+    }
+    P.num_mono = P.obs.size();
+    P.sensors.resize(pb->num_sensors);
+    for (uint64_t s = 0; s < pb->num_sensors; ++s) {
+        P.sensors[s].cam = to_cam(pb->sensors[s].camera);
+        P.sensors[s].sensor_from_rig = pose_from7(pb->sensors[s].sensor_from_rig);
+    }
+    for (uint64_t i = 0; i < pb->num_rig_observations; ++i) {
+        Obs o;
+        if (!find_slot(pose_ids, pb->rig_pose_ids[i], &o.pose)) {
+            *error_id = pb->rig_pose_ids[i];
+            return VBA_ERR_MISSING_POSE;
+        }
+        if (!find_slot(lm_ids, pb->rig_landmark_ids[i], &o.lm)) {
+            *error_id = pb->rig_landmark_ids[i];
+            return VBA_ERR_MISSING_LANDMARK;
+        }
+        uint32_t s = pb->rig_sensor[i];
+        if (s >= pb->num_sensors) return VBA_ERR_INVALID_ARGUMENT;
+        Intr si;
+        if (!known_model(P.sensors[s].cam.model) || !intrinsics(P.sensors[s].cam, &si))
+            return VBA_ERR_UNSUPPORTED_CAMERA_MODEL;
+        o.x = pb->rig_xy[2 * i];
+        o.y = pb->rig_xy[2 * i + 1];
+        o.sensor = (int32_t)s;
+        P.obs.push_back(o);
+    }
+    P.pose_var.assign(P.poses.size(), -1);
+    P.lm_var.assign(P.lms.size(), -1);
+    P.P = P.L = 0;
+    for (size_t s = 0; s < P.poses.size(); ++s)
+        if (!P.pose_fixed[s]) P.pose_var[s] = (int32_t)P.P++;
+    for (size_t s = 0; s < P.lms.size(); ++s)
+        if (!P.lm_fixed[s]) P.lm_var[s] = (int32_t)P.L++;
+    if (P.P == 0 && P.L == 0) return VBA_ERR_ALL_POSES_FIXED;
+    return VBA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Per-observation residual and Jacobians.
+// ---------------------------------------------------------------------------
+struct Lin {
+    double r[2];
+    M26 jp;
+    M23 jl;
+};
+// Monocular: normal_equations.rs:418-458.
+bool linearize_mono(const Problem& P, const Intr& in, const Proj& pr, const Obs& o, Lin* out) {
+    const SE3& T = P.poses[o.pose];
+    const V3& X = P.lms[o.lm];
+    M3 R = qmatrix(T.q);
+    V3 xc = se3_apply(T, X);
+    if (xc.z <= 0.0) return false;
+    double uv[2];
+    M23 jpi;
+    if (!mono_project_with_jacobian(pr, P.camera, in, xc, uv, &jpi)) return false;
+    out->r[0] = uv[0] - o.x;
+    out->r[1] = uv[1] - o.y;
+    M36 dx = M36::zeros();
+    M3 nRS = mul(neg(R), skew(X));
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            dx(r, c) = R(r, c);
+            dx(r, 3 + c) = nRS(r, c);
+        }
+    out->jp = mul(jpi, dx);
+    out->jl = mul(jpi, R);
+    return true;
+}
+// Rig: rig_residual_jacobians (sqrt_qr.rs:960-1001).
+bool linearize_rig(const Problem& P, const Obs& o, Lin* out) {
+    const Sensor& S = P.sensors[o.sensor];
+    Intr in;
+    if (!intrinsics(S.cam, &in)) return false;
+    const SE3& T = P.poses[o.pose];
+    const V3& X = P.lms[o.lm];
+    V3 prig = se3_apply(T, X);
+    V3 ps = se3_apply(S.sensor_from_rig, prig);
+    Proj pr = mono_projection_for(S.cam);
+    double uv[2];
+    M23 proj;
+    if (pr.kind == PROJ_PINHOLE) {
+        if (!project_pinhole(in, ps, uv)) return false;
+        if (!pinhole_projection_jacobian(in, ps, &proj)) return false;
+    } else {
+        if (!mono_project_with_jacobian(pr, S.cam, in, ps, uv, &proj)) return false;
+    }
+    out->r[0] = uv[0] - o.x;
+    out->r[1] = uv[1] - o.y;
+    M3 Rwr = qmatrix(T.q);
+    M3 Rrs = qmatrix(S.sensor_from_rig.q);
+    M36 d = M36::zeros();
+    M3 nRS = mul(neg(Rwr), skew(X));
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            d(r, c) = Rwr(r, c);
+            d(r, 3 + c) = nRS(r, c);
+        }
+    M23 pr_rs = mul(proj, Rrs);
+    out->jp = mul(pr_rs, d);
+    out->jl = mul(pr_rs, Rwr);
+    return true;
+}
+bool linearize_obs(const Problem& P, const Intr& in, const Proj& pr, const Obs& o, Lin* out) {
+    return o.sensor < 0 ? linearize_mono(P, in, pr, o, out) : linearize_rig(P, o, out);
+}
+
+// robust_cost_weighted (problem.rs:497-583) and
+// nonprojectable_observation_count (problem.rs:252-303).
+void cost_and_nonprojectable(const Problem& P, const Kernel& K, double* cost, uint64_t* nonproj) {
+    double total = 0.0;
+    uint64_t bad = 0;
+    for (const Obs& o : P.obs) {
+        if (o.sensor < 0) {
+            V3 xc = se3_apply(P.poses[o.pose], P.lms[o.lm]);
+            double uv[2];
+            if (xc.z <= 0.0 || !camera_project(P.camera, xc, uv)) {
+                ++bad;
+                continue;
+            }
+            double rx = uv[0] - o.x, ry = uv[1] - o.y;
+            double s = rx * rx + ry * ry;
+            total += 1.0 * K.cost(s);
+        } else {
+            Lin lin;
+            if (!linearize_rig(P, o, &lin)) {
+                ++bad;
+                continue;
+            }
+            double s = lin.r[0] * lin.r[0] + lin.r[1] * lin.r[1];
+            total += 1.0 * K.cost(s);
+        }
+    }
+    *cost = total;
+    *nonproj = bad;
+}
+
+// ---------------------------------------------------------------------------
+// Normal equations (NormalEquationsBa, normal_equations.rs:10-18, 78-86).
+// ---------------------------------------------------------------------------
+struct CrossEntry {
+    uint32_t pose;
+    M63 w;
+};
+struct LandmarkBlock {
+    M3 h_ll;
+    double b_l[3];
+    std::vector<CrossEntry> cross;
+};
+struct Normal {
+    uint32_t P = 0;
+    bool dense = true;
+    std::vector<double> H;        // dense (6P)^2 row-major, when dense
+    std::vector<M66> diag;        // pose blocks, when !dense
+    std::vector<double> b_p;      // 6P
+    std::vector<LandmarkBlock> lm;
+    double& h(uint32_t r, uint32_t c) {
+        return dense ? H[(size_t)r * 6 * P + c] : diag[r / 6](r % 6, c % 6);
+    }
+};
+
+struct Contribution {
+    bool valid = false;
+    int32_t p = -1, l = -1;
+    M66 hpp;
+    double bp[6];
+    M3 hll;
+    double bl[3];
+    M63 cross;
+};
+// compute_mono_contribution (normal_equations.rs:203-265) and its rig twin.
+void compute_contribution(const Problem& P, const Intr& in, const Proj& pr, const Kernel& K,
+                          const Obs& o, Contribution* c) {
+    Lin lin;
+    c->valid = linearize_obs(P, in, pr, o, &lin);
+    if (!c->valid) return;
+    double s = lin.r[0] * lin.r[0] + lin.r[1] * lin.r[1];
+    double w = K.weight(s) * 1.0;
+    c->p = P.pose_var[o.pose];
+    c->l = P.lm_var[o.lm];
+    if (c->p >= 0) {
+        M66 h = mul(tr(lin.jp), lin.jp);
+        double b[6];
+        Mat<6, 2> jpt = tr(lin.jp);
+        mulv(jpt, lin.r, b);
+        c->hpp = scale(w, h);
+        for (int k = 0; k < 6; ++k) c->bp[k] = w * b[k];
+    }
+    if (c->l >= 0) {
+        M3 h = mul(tr(lin.jl), lin.jl);
+        double b[3];
+        Mat<3, 2> jlt = tr(lin.jl);
+        mulv(jlt, lin.r, b);
+        c->hll = scale(w, h);
+        for (int k = 0; k < 3; ++k) c->bl[k] = w * b[k];
+    }
+    if (c->p >= 0 && c->l >= 0) c->cross = scale(w, mul(tr(lin.jp), lin.jl));
+}
+void apply_contribution(const Contribution& c, Normal* ne) {
+    if (!c.valid) return;
+    if (c.p >= 0) {
+        uint32_t p = (uint32_t)c.p;
+        for (int r = 0; r < 6; ++r) {
+            for (int cc = 0; cc < 6; ++cc) {
+                double& dst = ne->h(p * 6 + r, p * 6 + cc);
+                dst = dst + c.hpp(r, cc);
+            }
+            ne->b_p[p * 6 + r] = ne->b_p[p * 6 + r] + c.bp[r];
+        }
+    }
+    if (c.l >= 0) {
+        LandmarkBlock& L = ne->lm[c.l];
+        add_assign(L.h_ll, c.hll);
+        for (int k = 0; k < 3; ++k) L.b_l[k] = L.b_l[k] + c.bl[k];
+    }
+    if (c.p >= 0 && c.l >= 0) ne->lm[c.l].cross.push_back({(uint32_t)c.p, c.cross});
+}
+// build_normal_equations (normal_equations.rs:352-484, 617-651). threads > 1
+// computes contributions chunk-wise in parallel and scatters serially in
+// observation order (assemble_mono_observations_parallel, :267-350).
+void build_normal_equations(const Problem& P, const Kernel& K, bool pose_blocks, int threads,
+                            Normal* ne) {
+    ne->P = P.P;
+    ne->dense = !pose_blocks;
+    if (ne->dense) {
+        ne->H.assign((size_t)36 * P.P * P.P, 0.0);
+        ne->diag.clear();
+    } else {
+        ne->diag.assign(P.P, M66::zeros());
+        ne->H.clear();
+    }
+    ne->b_p.assign((size_t)6 * P.P, 0.0);
+    ne->lm.assign(P.L, LandmarkBlock{M3::zeros(), {0.0, 0.0, 0.0}, {}});
+    Intr in;
+    intrinsics(P.camera, &in);
+    Proj pr = mono_projection_for(P.camera);
+    const size_t n = P.obs.size();
+    if (threads <= 1 || n < 4096) {
+        Contribution c;
+        for (size_t i = 0; i < n; ++i) {
+            compute_contribution(P, in, pr, K, P.obs[i], &c);
+            apply_contribution(c, ne);
+        }
+        return;
+    }
+    const size_t chunk = 65536;
+    std::vector<Contribution> buf(chunk);
+    for (size_t start = 0; start < n; start += chunk) {
+        size_t end = std::min(n, start + chunk);
+        size_t m = end - start;
+        std::vector<std::thread> pool;
+        for (int t = 0; t < threads; ++t) {
+            pool.emplace_back([&, t]() {
+                for (size_t i = t; i < m; i += (size_t)threads)
+                    compute_contribution(P, in, pr, K, P.obs[start + i], &buf[i]);
+            });
+        }
+        for (auto& th : pool) th.join();
+        for (size_t i = 0; i < m; ++i) apply_contribution(buf[i], ne);
+    }
+}
+
+// nalgebra Matrix3::try_inverse (cofactor expansion; None only for det == 0).
+bool try_inverse3(const M3& m, M3* out) {
+    double m11 = m(0, 0), m12 = m(0, 1), m13 = m(0, 2);
+    double m21 = m(1, 0), m22 = m(1, 1), m23 = m(1, 2);
+    double m31 = m(2, 0), m32 = m(2, 1), m33 = m(2, 2);
+    double minor_m12_m23 = m22 * m33 - m32 * m23;
+    double minor_m11_m23 = m21 * m33 - m31 * m23;
+    double minor_m11_m22 = m21 * m32 - m31 * m22;
+    double det = m11 * minor_m12_m23 - m12 * minor_m11_m23 + m13 * minor_m11_m22;
+    if (det == 0.0) return false;
+    (*out)(0, 0) = minor_m12_m23 / det;
+    (*out)(0, 1) = (m13 * m32 - m33 * m12) / det;
+    (*out)(0, 2) = (m12 * m23 - m22 * m13) / det;
+    (*out)(1, 0) = -minor_m11_m23 / det;
+    (*out)(1, 1) = (m11 * m33 - m31 * m13) / det;
+    (*out)(1, 2) = (m13 * m21 - m23 * m11) / det;
+    (*out)(2, 0) = minor_m11_m22 / det;
+    (*out)(2, 1) = (m12 * m31 - m32 * m11) / det;
+    (*out)(2, 2) = (m11 * m22 - m21 * m12) / det;
+    return true;
+}
+M3 damped3(const M3& h, double lambda) {
+    M3 d = h;
+    if (lambda > 0.0) {
+        d(0, 0) += lambda;
+        d(1, 1) += lambda;
+        d(2, 2) += lambda;
+    }
+    return d;
+}
+
+// nalgebra Cholesky::new on a dense n x n row-major matrix (column-oriented,
+// left-looking). Returns false for a non-positive or non-finite pivot.
+bool cholesky_dense(std::vector<double>& m, size_t n) {
+    auto at = [&](size_t r, size_t c) -> double& { return m[r * n + c]; };
+    for (size_t j = 0; j < n; ++j) {
+        for (size_t k = 0; k < j; ++k) {
+            double factor = -at(j, k);
+            for (size_t i = j; i < n; ++i) at(i, j) = factor * at(i, k) + at(i, j);
+        }
+        double diag = at(j, j);
+        if (diag != 0.0 && diag >= 0.0) {
+            double denom = std::sqrt(diag);
+            at(j, j) = denom;
+            for (size_t i = j + 1; i < n; ++i) at(i, j) = at(i, j) / denom;
+            continue;
+        }
+        return false;
+    }
+    return true;
+}
+// Cholesky::solve: forward (column axpy) then the adjoint solve (dynamic dot).
+void cholesky_solve(const std::vector<double>& L, size_t n, std::vector<double>& b) {
+    auto at = [&](size_t r, size_t c) { return L[r * n + c]; };
+    for (size_t i = 0; i < n; ++i) {
+        double coeff = b[i] / at(i, i);
+        b[i] = coeff;
+        for (size_t r = i + 1; r < n; ++r) b[r] = -coeff * at(r, i) + b[r];
+    }
+    std::vector<double> col;
+    for (size_t ii = n; ii-- > 0;) {
+        size_t len = n - ii - 1;
+        col.resize(len);
+        for (size_t r = 0; r < len; ++r) col[r] = at(ii + 1 + r, ii);
+        double dot = dyn_dot(col.data(), b.data() + ii + 1, len);
+        b[ii] = (b[ii] - dot) / at(ii, ii);
+    }
+}
+// nalgebra LU::new (partial pivoting, gauss_step / gauss_step_swap) + solve.
+bool lu_solve(std::vector<double> m, size_t n, std::vector<double>& b) {
+    auto at = [&](size_t r, size_t c) -> double& { return m[r * n + c]; };
+    std::vector<std::pair<size_t, size_t>> perm;
+    for (size_t i = 0; i < n; ++i) {
+        size_t piv = i;
+        double best = std::fabs(at(i, i));
+        for (size_t r = i + 1; r < n; ++r)
+            if (std::fabs(at(r, i)) > best) {
+                best = std::fabs(at(r, i));
+                piv = r;
+            }
+        double diag = at(piv, i);
+        if (diag == 0.0) continue;
+        if (piv != i) {
+            perm.push_back({i, piv});
+            for (size_t c = 0; c < i; ++c) std::swap(at(i, c), at(piv, c));
+            // gauss_step_swap
+            double inv_diag = 1.0 / diag;
+            std::swap(at(i, i), at(piv, i));
+            for (size_t r = i + 1; r < n; ++r) at(r, i) *= inv_diag;
+            for (size_t c = i + 1; c < n; ++c) {
+                std::swap(at(i, c), at(piv, c));
+                double f = -at(i, c);
+                for (size_t r = i + 1; r < n; ++r) at(r, c) = f * at(r, i) + at(r, c);
+            }
+        } else {
+            double inv_diag = 1.0 / diag;
+            for (size_t r = i + 1; r < n; ++r) at(r, i) *= inv_diag;
+            for (size_t c = i + 1; c < n; ++c) {
+                double f = -at(i, c);
+                for (size_t r = i + 1; r < n; ++r) at(r, c) = f * at(r, i) + at(r, c);
+            }
+        }
+    }
+    for (auto& pr : perm) std::swap(b[pr.first], b[pr.second]);
+    for (size_t i = 0; i + 1 < n; ++i) {
+        double coeff = b[i] / 1.0;
+        for (size_t r = i + 1; r < n; ++r) b[r] = -coeff * at(r, i) + b[r];
+    }
+    for (size_t ii = n; ii-- > 0;) {
+        double diag = at(ii, ii);
+        if (diag == 0.0) return false;
+        double coeff = b[ii] / diag;
+        b[ii] = coeff;
+        for (size_t r = 0; r < ii; ++r) b[r] = -coeff * at(r, ii) + b[r];
+    }
+    return true;
+}
+// solve_normal_equations (pose_graph.rs:2259-2270).
+bool solve_normal_equations(const std::vector<double>& A, size_t n, const std::vector<double>& b,
+                            std::vector<double>* x) {
+    std::vector<double> L = A;
+    if (cholesky_dense(L, n)) {
+        *x = b;
+        cholesky_solve(L, n, *x);
+        return true;
+    }
+    *x = b;
+    return lu_solve(A, n, *x);
+}
+
+// 6x6 Cholesky::new + inverse (static size; same column algorithm).
+bool cholesky_inverse6(const M66& block, M66* inv) {
+    std::vector<double> m(block.a, block.a + 36);
+    if (!cholesky_dense(m, 6)) return false;
+    for (int c = 0; c < 6; ++c) {
+        std::vector<double> e(6, 0.0);
+        e[c] = 1.0;
+        cholesky_solve(m, 6, e);
+        for (int r = 0; r < 6; ++r) (*inv)(r, c) = e[r];
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Dense Schur step: solve_step_with_debug (normal_equations.rs:1314-1592).
+// ---------------------------------------------------------------------------
+int32_t solve_step_dense(Normal& ne, uint32_t L, double lambda, std::vector<double>* dp,
+                         std::vector<double>* dl) {
+    const uint32_t P = ne.P;
+    if (P == 0) {  // landmark-only branch (:1332-1371)
+        dl->assign((size_t)3 * L, 0.0);
+        for (uint32_t l = 0; l < L; ++l) {
+            M3 inv;
+            if (!try_inverse3(damped3(ne.lm[l].h_ll, lambda), &inv)) return VBA_ERR_SINGULAR_SYSTEM;
+            double v[3];
+            mulv(inv, ne.lm[l].b_l, v);
+            for (int k = 0; k < 3; ++k) (*dl)[l * 3 + k] = -v[k];
+        }
+        dp->clear();
+        return VBA_OK;
+    }
+    const size_t dim = (size_t)6 * P;
+    std::vector<double>& S = ne.H;
+    if (!ne.dense) {
+        S.assign(dim * dim, 0.0);
+        for (uint32_t p = 0; p < P; ++p)
+            for (int r = 0; r < 6; ++r)
+                for (int c = 0; c < 6; ++c) S[(p * 6 + r) * dim + p * 6 + c] = ne.diag[p](r, c);
+    }
+    if (lambda > 0.0)
+        for (size_t k = 0; k < dim; ++k) S[k * dim + k] += lambda;
+    std::vector<double> b(dim);
+    for (size_t k = 0; k < dim; ++k) b[k] = -ne.b_p[k];
+    std::vector<M3> inv(L);
+    std::vector<uint8_t> has_inv(L, 0);
+    for (uint32_t l = 0; l < L; ++l) {
+        const LandmarkBlock& lb = ne.lm[l];
+        has_inv[l] = try_inverse3(damped3(lb.h_ll, lambda), &inv[l]);
+        if (!has_inv[l]) continue;
+        for (const CrossEntry& a : lb.cross) {
+            M63 a_h = mul(a.w, inv[l]);
+            for (const CrossEntry& bq : lb.cross) {
+                M66 blk = mul(a_h, tr(bq.w));
+                for (int r = 0; r < 6; ++r)
+                    for (int c = 0; c < 6; ++c) {
+                        double& dst = S[(a.pose * 6 + r) * dim + bq.pose * 6 + c];
+                        dst -= blk(r, c);
+                    }
+            }
+            double upd[6];
+            mulv(a_h, lb.b_l, upd);
+            for (int k = 0; k < 6; ++k) b[a.pose * 6 + k] += upd[k];
+        }
+    }
+    if (!solve_normal_equations(S, dim, b, dp)) return VBA_ERR_SINGULAR_SYSTEM;
+    dl->assign((size_t)3 * L, 0.0);
+    for (uint32_t l = 0; l < L; ++l) {
+        if (!has_inv[l]) continue;
+        const LandmarkBlock& lb = ne.lm[l];
+        double acc[3] = {-lb.b_l[0], -lb.b_l[1], -lb.b_l[2]};
+        for (const CrossEntry& a : lb.cross) {
+            double sub[3];
+            mulv(tr(a.w), dp->data() + a.pose * 6, sub);
+            for (int k = 0; k < 3; ++k) acc[k] -= sub[k];
+        }
+        double d[3];
+        mulv(inv[l], acc, d);
+        for (int k = 0; k < 3; ++k) (*dl)[l * 3 + k] = d[k];
+    }
+    return VBA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Matrix-free implicit Schur operator + PCG (implicit_schur.rs).
+// ---------------------------------------------------------------------------
+enum ImplicitErr {
+    IE_OK = 0,
+    IE_NONFINITE = -1,
+    IE_INVALID_LAMBDA = -2,
+    IE_EMPTY = -3,
+    IE_NON_SPD_PRECONDITIONER = -4,
+    IE_NON_POSITIVE_CURVATURE = -5,
+    IE_RESIDUAL_CHECK_FAILED = -6,
+    IE_MAX_ITERATIONS = -7,
+    IE_INVALID_TOLERANCE = -8
+};
+struct Implicit {
+    const Normal* ne = nullptr;
+    std::vector<M66> diagonal;  // damped
+    std::vector<M3> inv;
+    std::vector<uint8_t> has_inv;
+    std::vector<M66> precond_inv;
+    std::vector<double> rhs;
+    size_t dim() const { return diagonal.size() * 6; }
+};
+// ImplicitSchurOperator::new (implicit_schur.rs:133-297).
+int build_implicit(const Normal& ne, double lambda, Implicit* op) {
+    if (!std::isfinite(lambda) || lambda < 0.0) return IE_INVALID_LAMBDA;
+    if (ne.diag.empty()) return IE_EMPTY;
+    op->ne = &ne;
+    const uint32_t P = (uint32_t)ne.diag.size();
+    if (!all_finite(ne.b_p)) return IE_NONFINITE;
+    for (const M66& b : ne.diag)
+        if (!all_finite(b)) return IE_NONFINITE;
+    op->diagonal = ne.diag;
+    if (lambda > 0.0)
+        for (M66& b : op->diagonal)
+            for (int k = 0; k < 6; ++k) b(k, k) += lambda;
+    for (const M66& b : op->diagonal)
+        if (!all_finite(b)) return IE_NONFINITE;
+    const size_t L = ne.lm.size();
+    op->inv.assign(L, M3::zeros());
+    op->has_inv.assign(L, 0);
+    op->rhs.resize((size_t)6 * P);
+    for (size_t k = 0; k < op->rhs.size(); ++k) op->rhs[k] = -ne.b_p[k];
+    for (size_t l = 0; l < L; ++l) {
+        const LandmarkBlock& lb = ne.lm[l];
+        if (!all_finite(lb.h_ll) || !std::isfinite(lb.b_l[0]) || !std::isfinite(lb.b_l[1]) ||
+            !std::isfinite(lb.b_l[2]))
+            return IE_NONFINITE;
+        for (const CrossEntry& c : lb.cross)
+            if (!all_finite(c.w)) return IE_NONFINITE;
+        M3 h = lb.h_ll;
+        if (lambda > 0.0)
+            for (int k = 0; k < 3; ++k) h(k, k) += lambda;
+        if (!all_finite(h)) return IE_NONFINITE;
+        M3 inv;
+        bool ok = try_inverse3(h, &inv);
+        if (ok && !all_finite(inv)) return IE_NONFINITE;
+        op->has_inv[l] = ok;
+        if (!ok) continue;
+        op->inv[l] = inv;
+        for (const CrossEntry& c : lb.cross) {
+            M63 ci = mul(c.w, inv);
+            double upd[6];
+            mulv(ci, lb.b_l, upd);
+            for (int k = 0; k < 6; ++k) op->rhs[c.pose * 6 + k] += upd[k];
+        }
+    }
+    if (!all_finite(op->rhs)) return IE_NONFINITE;
+    std::vector<M66> pre = op->diagonal;
+    std::vector<std::pair<uint32_t, M63>> grouped;
+    for (size_t l = 0; l < L; ++l) {
+        if (!op->has_inv[l]) continue;
+        grouped.clear();  // BTreeMap<pose, sum of cross>: ascending pose
+        for (const CrossEntry& c : ne.lm[l].cross) {
+            auto it = std::find_if(grouped.begin(), grouped.end(),
+                                   [&](const std::pair<uint32_t, M63>& g) { return g.first == c.pose; });
+            if (it == grouped.end())
+                grouped.push_back({c.pose, add(M63::zeros(), c.w)});
+            else
+                add_assign(it->second, c.w);
+        }
+        std::sort(grouped.begin(), grouped.end(),
+                  [](const std::pair<uint32_t, M63>& a, const std::pair<uint32_t, M63>& b) {
+                      return a.first < b.first;
+                  });
+        for (auto& g : grouped) sub_assign(pre[g.first], mul(mul(g.second, op->inv[l]), tr(g.second)));
+    }
+    for (const M66& b : pre)
+        if (!all_finite(b)) return IE_NONFINITE;
+    op->precond_inv.resize(P);
+    for (uint32_t p = 0; p < P; ++p) {
+        if (!cholesky_inverse6(pre[p], &op->precond_inv[p])) return IE_NON_SPD_PRECONDITIONER;
+        if (!all_finite(op->precond_inv[p])) return IE_NONFINITE;
+    }
+    return IE_OK;
+}
+// ImplicitSchurOperator::apply (implicit_schur.rs:390-431).
+int implicit_apply(const Implicit& op, const std::vector<double>& x, std::vector<double>* out) {
+    if (!all_finite(x)) return IE_NONFINITE;
+    out->assign(op.dim(), 0.0);
+    for (size_t p = 0; p < op.diagonal.size(); ++p) mulv(op.diagonal[p], x.data() + p * 6, out->data() + p * 6);
+    const Normal& ne = *op.ne;
+    for (size_t l = 0; l < ne.lm.size(); ++l) {
+        if (!op.has_inv[l]) continue;
+        double projected[3] = {0.0, 0.0, 0.0};
+        for (const CrossEntry& c : ne.lm[l].cross) {
+            double v[3];
+            mulv(tr(c.w), x.data() + c.pose * 6, v);
+            for (int k = 0; k < 3; ++k) projected[k] += v[k];
+        }
+        double reduced[3];
+        mulv(op.inv[l], projected, reduced);
+        for (const CrossEntry& c : ne.lm[l].cross) {
+            double v[6];
+            mulv(c.w, reduced, v);
+            for (int k = 0; k < 6; ++k) (*out)[c.pose * 6 + k] -= v[k];
+        }
+    }
+    if (!all_finite(*out)) return IE_NONFINITE;
+    return IE_OK;
+}
+int implicit_precondition(const Implicit& op, const std::vector<double>& r, std::vector<double>* out) {
+    if (!all_finite(r)) return IE_NONFINITE;
+    out->assign(op.dim(), 0.0);
+    for (size_t p = 0; p < op.precond_inv.size(); ++p)
+        mulv(op.precond_inv[p], r.data() + p * 6, out->data() + p * 6);
+    if (!all_finite(*out)) return IE_NONFINITE;
+    return IE_OK;
+}
+struct PcgOut {
+    std::vector<double> x;
+    uint32_t iterations = 0;
+    bool has_iterations = false;
+    double residual_norm = 0.0, target = 0.0;
+    bool has_diag = false;
+};
+int true_residual(const Implicit& op, const std::vector<double>& rhs, const std::vector<double>& x,
+                  std::vector<double>* tr_out, double* norm) {
+    std::vector<double> ax;
+    int e = implicit_apply(op, x, &ax);
+    if (e) return e;
+    tr_out->resize(rhs.size());
+    for (size_t i = 0; i < rhs.size(); ++i) (*tr_out)[i] = rhs[i] - ax[i];
+    *norm = dyn_norm(*tr_out);
+    if (!std::isfinite(*norm)) return IE_NONFINITE;
+    return IE_OK;
+}
+// solve_pcg_with_restart_hook with max_restarts = 0 (implicit_schur.rs:522-736).
+// accept_iterate: a solve that ends unconverged returns its iterate instead
+// of failing (the reference GPU backend's semantics).
+int solve_pcg(const Implicit& op, uint32_t max_it, double rtol, double atol, bool accept_iterate,
+              PcgOut* out) {
+    const std::vector<double>& rhs = op.rhs;
+    if (!all_finite(rhs)) return IE_NONFINITE;
+    if (!std::isfinite(rtol) || rtol < 0.0 || !std::isfinite(atol) || atol < 0.0) return IE_INVALID_TOLERANCE;
+    double rhs_norm = dyn_norm(rhs);
+    double target = std::max(atol, rtol * rhs_norm);
+    if (!std::isfinite(target)) return IE_NONFINITE;
+    out->target = target;
+    std::vector<double> x(op.dim(), 0.0), r = rhs, z, ad, trv;
+    double rn = dyn_norm(r);
+    if (!std::isfinite(rn)) return IE_NONFINITE;
+    out->has_iterations = true;
+    out->iterations = 0;
+    if (rn <= target) {
+        double tn;
+        int e = true_residual(op, rhs, x, &trv, &tn);
+        if (e) return e;
+        out->residual_norm = tn;
+        out->has_diag = true;
+        out->x = x;
+        if (tn <= target || accept_iterate) return IE_OK;
+        return IE_RESIDUAL_CHECK_FAILED;
+    }
+    if (max_it == 0) {
+        out->residual_norm = rn;
+        out->has_diag = true;
+        if (accept_iterate) {
+            out->x = x;
+            return IE_OK;
+        }
+        return IE_MAX_ITERATIONS;
+    }
+    int e = implicit_precondition(op, r, &z);
+    if (e) return e;
+    double rho = dyn_dot(r.data(), z.data(), r.size());
+    if (!std::isfinite(rho) || rho <= 0.0) return IE_NON_POSITIVE_CURVATURE;
+    std::vector<double> d = z;
+    for (uint32_t it = 1; it <= max_it; ++it) {
+        e = implicit_apply(op, d, &ad);
+        if (e) return e;
+        double curv = dyn_dot(d.data(), ad.data(), d.size());
+        if (!std::isfinite(curv) || curv <= 0.0) return IE_NON_POSITIVE_CURVATURE;
+        double alpha = rho / curv;
+        if (!std::isfinite(alpha)) return IE_NONFINITE;
+        for (size_t i = 0; i < x.size(); ++i) x[i] = x[i] + alpha * d[i];
+        for (size_t i = 0; i < r.size(); ++i) r[i] = r[i] - alpha * ad[i];
+        rn = dyn_norm(r);
+        out->iterations = it;
+        if (!all_finite(x) || !std::isfinite(rn)) return IE_NONFINITE;
+        if (rn <= target) {
+            double tn;
+            e = true_residual(op, rhs, x, &trv, &tn);
+            if (e) return e;
+            out->residual_norm = tn;
+            out->has_diag = true;
+            out->x = x;
+            if (tn <= target || accept_iterate) return IE_OK;
+            return IE_RESIDUAL_CHECK_FAILED;
+        }
+        if (it == max_it) {
+            double tn;
+            e = true_residual(op, rhs, x, &trv, &tn);
+            if (e) return e;
+            out->residual_norm = tn;
+            out->has_diag = true;
+            out->x = x;
+            if (tn <= target || accept_iterate) return IE_OK;
+            return IE_MAX_ITERATIONS;
+        }
+        e = implicit_precondition(op, r, &z);
+        if (e) return e;
+        double next_rho = dyn_dot(r.data(), z.data(), r.size());
+        if (!std::isfinite(next_rho) || next_rho <= 0.0) return IE_NON_POSITIVE_CURVATURE;
+        double beta = next_rho / rho;
+        if (!std::isfinite(beta)) return IE_NONFINITE;
+        for (size_t i = 0; i < d.size(); ++i) d[i] = z[i] + beta * d[i];
+        rho = next_rho;
+    }
+    return IE_MAX_ITERATIONS;
+}
+// complete_delta (implicit_schur.rs:789-824).
+int complete_delta(const Implicit& op, const std::vector<double>& dp, std::vector<double>* dl) {
+    if (!all_finite(dp)) return IE_NONFINITE;
+    const Normal& ne = *op.ne;
+    dl->assign(ne.lm.size() * 3, 0.0);
+    for (size_t l = 0; l < ne.lm.size(); ++l) {
+        if (!op.has_inv[l]) continue;
+        double acc[3] = {-ne.lm[l].b_l[0], -ne.lm[l].b_l[1], -ne.lm[l].b_l[2]};
+        for (const CrossEntry& c : ne.lm[l].cross) {
+            double v[3];
+            mulv(tr(c.w), dp.data() + c.pose * 6, v);
+            for (int k = 0; k < 3; ++k) acc[k] -= v[k];
+        }
+        double d[3];
+        mulv(op.inv[l], acc, d);
+        for (int k = 0; k < 3; ++k) (*dl)[l * 3 + k] = d[k];
+    }
+    if (!all_finite(*dl)) return IE_NONFINITE;
+    return IE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Matrix-free entry validation (optimize.rs:382-626), restricted to the
+// terms this path carries.
+// ---------------------------------------------------------------------------
+int32_t validate_matrix_free(const Problem& P, const vba_options& o, const Kernel& K) {
+    if (o.refine_intrinsics) return VBA_ERR_INELIGIBLE;
+    if (o.max_iterations == 0) return VBA_ERR_INVALID_CONFIGURATION;
+    if (!o.has_initial_lambda) return VBA_ERR_INVALID_CONFIGURATION;
+    double il = o.initial_lambda;
+    if (!(std::isfinite(il) && il > 0.0)) return VBA_ERR_INVALID_CONFIGURATION;
+    if (!std::isfinite(o.lambda_increase_factor) || o.lambda_increase_factor <= 1.0 ||
+        !std::isfinite(o.lambda_decrease_factor) || o.lambda_decrease_factor <= 0.0 ||
+        o.lambda_decrease_factor >= 1.0 || !std::isfinite(o.max_lambda) || o.max_lambda < il ||
+        !std::isfinite(o.min_lambda) || o.min_lambda <= 0.0 || o.min_lambda > il ||
+        !std::isfinite(o.step_tolerance) || o.step_tolerance < 0.0 || !std::isfinite(o.cost_tolerance) ||
+        o.cost_tolerance < 0.0 ||
+        (o.has_relative_cost_tolerance &&
+         (!std::isfinite(o.relative_cost_tolerance) || o.relative_cost_tolerance < 0.0)))
+        return VBA_ERR_INVALID_CONFIGURATION;
+    if (o.max_pcg_iterations == 0 || !std::isfinite(o.pcg_relative_tolerance) ||
+        o.pcg_relative_tolerance < 0.0 || !std::isfinite(o.pcg_absolute_tolerance) ||
+        o.pcg_absolute_tolerance <= 0.0)
+        return VBA_ERR_INVALID_CONFIGURATION;
+    if (K.kind != VBA_KERNEL_NONE && !(std::isfinite(K.scale) && K.scale > 0.0))
+        return VBA_ERR_INVALID_CONFIGURATION;
+    auto finite_cam = [](const Cam& c) {
+        Intr in;
+        if (!intrinsics(c, &in)) return false;
+        for (int i = 0; i < c.np; ++i)
+            if (!std::isfinite(c.p[i])) return false;
+        return std::isfinite(in.fx) && std::isfinite(in.fy) && std::isfinite(in.cx) &&
+               std::isfinite(in.cy) && in.fx > 0.0 && in.fy > 0.0;
+    };
+    if (!finite_cam(P.camera)) return VBA_ERR_INELIGIBLE;
+    bool any_fixed = false, any_var = false;
+    for (uint8_t f : P.pose_fixed) (f ? any_fixed : any_var) = true;
+    if (!any_fixed) return VBA_ERR_INELIGIBLE;
+    if (!any_var) return VBA_ERR_INELIGIBLE;
+    for (const SE3& T : P.poses) {
+        const double v[7] = {T.q.w, T.q.i, T.q.j, T.q.k, T.t.x, T.t.y, T.t.z};
+        for (double x : v)
+            if (!std::isfinite(x)) return VBA_ERR_INELIGIBLE;
+    }
+    for (const V3& X : P.lms)
+        if (!std::isfinite(X.x) || !std::isfinite(X.y) || !std::isfinite(X.z)) return VBA_ERR_INELIGIBLE;
+    double c;
+    uint64_t np;
+    cost_and_nonprojectable(P, K, &c, &np);
+    if (!std::isfinite(c)) return VBA_ERR_INELIGIBLE;
+    for (const Obs& ob : P.obs) {
+        if (!std::isfinite(ob.x) || !std::isfinite(ob.y)) return VBA_ERR_INELIGIBLE;
+        if (ob.sensor >= 0) {
+            const Sensor& s = P.sensors[ob.sensor];
+            if (!finite_cam(s.cam)) return VBA_ERR_INELIGIBLE;
+        }
+    }
+    return VBA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// LM loop: optimize_weighted_backend (lm.rs:162-1017).
+// ---------------------------------------------------------------------------
+double norm6_seq(const double* v) {  // Vector6::norm (generic dot, n < 8: sequential)
+    double res = 0.0;
+    for (int k = 0; k < 6; ++k) res += v[k] * v[k];
+    return std::sqrt(res);
+}
+
+int32_t run_lm(Problem& P, const vba_options& o, int threads, vba_iteration_stats* stats,
+               uint32_t cap, vba_result* res) {
+    const Kernel K{o.robust_kernel, o.robust_scale};
+    const bool matrix_free = o.linear_solver == VBA_SOLVER_MATRIX_FREE || o.linear_solver == VBA_SOLVER_PCG_EXPLICIT;
+    const bool accept_iterate = o.pcg_maxit_policy == VBA_PCG_MAXIT_ACCEPT_ITERATE;
+    double initial_cost;
+    uint64_t current_nonproj;
+    cost_and_nonprojectable(P, K, &initial_cost, &current_nonproj);
+    double current_cost = initial_cost;
+    double lambda = o.has_initial_lambda ? o.initial_lambda : 0.0;
+    bool converged = false;
+    uint32_t written = 0, total_pcg = 0, linearizations = 0;
+    auto push = [&](const vba_iteration_stats& s) {
+        if (written < cap && stats) stats[written] = s;
+        ++written;
+    };
+    Normal ne;
+    for (uint32_t iteration = 0; iteration < o.max_iterations; ++iteration) {
+        build_normal_equations(P, K, matrix_free, threads, &ne);
+        ++linearizations;
+        std::vector<SE3> saved_poses = P.poses;
+        std::vector<V3> saved_lms = P.lms;
+        const double cost_before = current_cost;
+        std::vector<double> dp, dl;
+        int32_t solve_status = VBA_OK;
+        vba_iteration_stats st;
+        std::memset(&st, 0, sizeof(st));
+        st.iteration = iteration;
+        st.pcg_iterations = -1;
+        if (!matrix_free) {
+            solve_status = solve_step_dense(ne, P.L, lambda, &dp, &dl);
+        } else {
+            Implicit op;
+            int e = build_implicit(ne, lambda, &op);
+            PcgOut pcg;
+            if (e == IE_OK) e = solve_pcg(op, o.max_pcg_iterations, o.pcg_relative_tolerance,
+                                          o.pcg_absolute_tolerance, accept_iterate, &pcg);
+            if (pcg.has_iterations && (e == IE_OK || e == IE_MAX_ITERATIONS || e == IE_RESIDUAL_CHECK_FAILED)) {
+                st.pcg_iterations = (int32_t)pcg.iterations;
+                st.pcg_residual_norm = pcg.residual_norm;
+                st.pcg_target = pcg.target;
+                total_pcg += pcg.iterations;
+            }
+            if (e == IE_OK) {
+                dp = pcg.x;
+                e = complete_delta(op, dp, &dl);
+            }
+            if (e != IE_OK) solve_status = VBA_ERR_SINGULAR_SYSTEM;
+        }
+        if (solve_status != VBA_OK) {
+            lambda = std::min(lambda * o.lambda_increase_factor, o.max_lambda);
+            st.cost_before = cost_before;
+            st.cost_after = cost_before;
+            st.lambda = lambda;
+            st.step_accepted = 0;
+            st.nonprojectable_after = (uint32_t)current_nonproj;
+            push(st);
+            if (lambda >= o.max_lambda) break;
+            continue;
+        }
+        double max_pose_step = 0.0;
+        for (size_t s = 0; s < P.poses.size(); ++s) {
+            int32_t i = P.pose_var[s];
+            if (i < 0) continue;
+            const double* xi = dp.data() + (size_t)i * 6;
+            double step = norm6_seq(xi);
+            if (step > max_pose_step) max_pose_step = step;
+            P.poses[s] = se3_compose(P.poses[s], se3_exp(xi));
+        }
+        double max_landmark_step = 0.0;
+        for (size_t s = 0; s < P.lms.size(); ++s) {
+            int32_t i = P.lm_var[s];
+            if (i < 0) continue;
+            const double* d = dl.data() + (size_t)i * 3;
+            V3 v{d[0], d[1], d[2]};
+            double step = std::sqrt(norm_sq3(v));
+            if (step > max_landmark_step) max_landmark_step = step;
+            P.lms[s] = {P.lms[s].x + v.x, P.lms[s].y + v.y, P.lms[s].z + v.z};
+        }
+        double cost_after;
+        uint64_t nonproj_after;
+        cost_and_nonprojectable(P, K, &cost_after, &nonproj_after);
+        bool cost_accepted = o.has_initial_lambda ? (cost_after < cost_before) : true;
+        bool feasible = nonproj_after <= current_nonproj;
+        bool accepted = cost_accepted && feasible;
+        st.cost_before = cost_before;
+        st.cost_after = cost_after;
+        st.max_pose_step = max_pose_step;
+        st.max_landmark_step = max_landmark_step;
+        st.nonprojectable_after = (uint32_t)nonproj_after;
+        if (!accepted) {
+            P.poses = saved_poses;
+            P.lms = saved_lms;
+            lambda = std::min(lambda * o.lambda_increase_factor, o.max_lambda);
+            st.lambda = lambda;
+            st.step_accepted = 0;
+            push(st);
+            if (!o.has_initial_lambda) break;
+            if (lambda >= o.max_lambda) break;
+            continue;
+        }
+        st.lambda = lambda;
+        st.step_accepted = 1;
+        push(st);
+        current_cost = cost_after;
+        current_nonproj = nonproj_after;
+        if (o.has_initial_lambda) lambda = std::max(lambda * o.lambda_decrease_factor, o.min_lambda);
+        if (max_pose_step < o.step_tolerance && max_landmark_step < o.step_tolerance) {
+            converged = true;
+            break;
+        }
+        if (std::fabs(cost_before - cost_after) < o.cost_tolerance) {
+            converged = true;
+            break;
+        }
+        if (o.has_relative_cost_tolerance && std::isfinite(o.relative_cost_tolerance) &&
+            o.relative_cost_tolerance >= 0.0 &&
+            (cost_before - cost_after) / std::max(std::fabs(cost_before), std::numeric_limits<double>::epsilon()) <
+                o.relative_cost_tolerance) {
+            converged = true;
+            break;
+        }
+    }
+    res->initial_cost = initial_cost;
+    res->final_cost = current_cost;
+    res->num_iterations = std::min(written, cap);
+    res->converged = converged;
+    res->num_observations_used = P.obs.size();
+    res->total_pcg_iterations = total_pcg;
+    res->linearizations = linearizations;
+    return VBA_OK;
+}
+
+// Build a Normal from the C normal-system view.
+void normal_from_view(const oracle_normal_system* v, Normal* ne) {
+    ne->P = v->num_poses;
+    ne->dense = false;
+    ne->diag.resize(v->num_poses);
+    for (uint32_t p = 0; p < v->num_poses; ++p)
+        for (int i = 0; i < 36; ++i) ne->diag[p].a[i] = v->diag[p * 36 + i];
+    ne->b_p.assign(v->b_p, v->b_p + 6 * (size_t)v->num_poses);
+    ne->lm.resize(v->num_landmarks);
+    for (uint32_t l = 0; l < v->num_landmarks; ++l) {
+        LandmarkBlock& lb = ne->lm[l];
+        for (int i = 0; i < 9; ++i) lb.h_ll.a[i] = v->h_ll[l * 9 + i];
+        for (int i = 0; i < 3; ++i) lb.b_l[i] = v->b_l[l * 3 + i];
+        lb.cross.clear();
+        for (uint32_t e = v->cross_ptr[l]; e < v->cross_ptr[l + 1]; ++e) {
+            CrossEntry c;
+            c.pose = v->cross_pose[e];
+            for (int i = 0; i < 18; ++i) c.w.a[i] = v->cross[e * 18 + i];
+            lb.cross.push_back(c);
+        }
+    }
+}
+
+}  // namespace
+
+// ===========================================================================
+// C API
+// ===========================================================================
+extern "C" {
+
+int32_t oracle_optimize(const vba_problem* problem, const vba_options* options, double* poses_out,
+                        double* landmarks_out, vba_iteration_stats* stats, uint32_t stats_capacity,
+                        vba_result* result, int32_t threads) {
+    if (!problem || !options || !result) return VBA_ERR_INVALID_ARGUMENT;
+    std::memset(result, 0, sizeof(*result));
+    if (problem->declined_features) return VBA_DECLINED;
+    Problem P;
+    uint64_t error_id = 0;
+    const bool matrix_free =
+        options->linear_solver == VBA_SOLVER_MATRIX_FREE || options->linear_solver == VBA_SOLVER_PCG_EXPLICIT;
+    int32_t st = load_problem(problem, &P, &error_id);
+    result->error_id = error_id;
+    if (st != VBA_OK) return st;
+    if (matrix_free) {
+        st = validate_matrix_free(P, *options, Kernel{options->robust_kernel, options->robust_scale});
+        if (st != VBA_OK) return st;
+    }
+    st = run_lm(P, *options, threads, stats, stats_capacity, result);
+    if (st != VBA_OK) return st;
+    for (uint64_t i = 0; i < problem->num_poses; ++i)
+        if (poses_out) pose_to7(P.poses[P.pose_slot_of_input[i]], poses_out + 7 * i);
+    for (uint64_t i = 0; i < problem->num_landmarks; ++i) {
+        if (!landmarks_out) break;
+        const V3& X = P.lms[P.lm_slot_of_input[i]];
+        landmarks_out[3 * i] = X.x;
+        landmarks_out[3 * i + 1] = X.y;
+        landmarks_out[3 * i + 2] = X.z;
+    }
+    return VBA_OK;
+}
+
+int32_t oracle_camera_project(const vba_camera* camera, const double xc[3], double uv[2]) {
+    return camera_project(to_cam(*camera), V3{xc[0], xc[1], xc[2]}, uv) ? 1 : 0;
+}
+
+int32_t oracle_mono_project(const vba_camera* camera, const double xc[3], double uv[2], double j[6]) {
+    Cam c = to_cam(*camera);
+    Intr in;
+    if (!intrinsics(c, &in)) return 0;
+    M23 J;
+    if (!mono_project_with_jacobian(mono_projection_for(c), c, in, V3{xc[0], xc[1], xc[2]}, uv, &J)) return 0;
+    for (int i = 0; i < 6; ++i) j[i] = J.a[i];
+    return 1;
+}
+
+void oracle_robust(uint32_t kernel, double scale, double s, double* cost, double* weight) {
+    Kernel K{kernel, scale};
+    if (cost) *cost = K.cost(s);
+    if (weight) *weight = K.weight(s);
+}
+
+int32_t oracle_linearize(const vba_problem* problem, uint32_t kernel, double scale, double* r,
+                         double* j_pose, double* j_landmark, double* w, uint8_t* valid) {
+    Problem P;
+    uint64_t eid;
+    int32_t st = load_problem(problem, &P, &eid);
+    if (st != VBA_OK) return st;
+    Kernel K{kernel, scale};
+    Intr in;
+    intrinsics(P.camera, &in);
+    Proj pr = mono_projection_for(P.camera);
+    for (size_t i = 0; i < P.obs.size(); ++i) {
+        Lin lin;
+        bool ok = linearize_obs(P, in, pr, P.obs[i], &lin);
+        valid[i] = ok;
+        if (!ok) {
+            std::memset(r + 2 * i, 0, 2 * sizeof(double));
+            std::memset(j_pose + 12 * i, 0, 12 * sizeof(double));
+            std::memset(j_landmark + 6 * i, 0, 6 * sizeof(double));
+            w[i] = 0.0;
+            continue;
+        }
+        r[2 * i] = lin.r[0];
+        r[2 * i + 1] = lin.r[1];
+        for (int k = 0; k < 12; ++k) j_pose[12 * i + k] = lin.jp.a[k];
+        for (int k = 0; k < 6; ++k) j_landmark[6 * i + k] = lin.jl.a[k];
+        w[i] = K.weight(lin.r[0] * lin.r[0] + lin.r[1] * lin.r[1]);
+    }
+    return VBA_OK;
+}
+
+int32_t oracle_cost(const vba_problem* problem, uint32_t kernel, double scale, double* cost,
+                    uint64_t* nonprojectable) {
+    Problem P;
+    uint64_t eid;
+    int32_t st = load_problem(problem, &P, &eid);
+    if (st != VBA_OK) return st;
+    cost_and_nonprojectable(P, Kernel{kernel, scale}, cost, nonprojectable);
+    return VBA_OK;
+}
+
+void oracle_se3_exp(const double xi[6], double pose[7]) { pose_to7(se3_exp(xi), pose); }
+void oracle_se3_log(const double pose[7], double xi[6]) { se3_log(pose_from7(pose), xi); }
+void oracle_se3_compose(const double a[7], const double b[7], double out[7]) {
+    pose_to7(se3_compose(pose_from7(a), pose_from7(b)), out);
+}
+void oracle_camera_center(const double pose[7], double center[3]) {
+    SE3 inv = se3_inverse(pose_from7(pose));
+    center[0] = inv.t.x;
+    center[1] = inv.t.y;
+    center[2] = inv.t.z;
+}
+
+void oracle_ns_reduce_dense(const oracle_normal_system* ns, double lambda, double* s, double* rhs) {
+    Normal ne;
+    normal_from_view(ns, &ne);
+    const uint32_t P = ne.P;
+    const size_t dim = (size_t)6 * P;
+    std::vector<double> S(dim * dim, 0.0);
+    for (uint32_t p = 0; p < P; ++p)
+        for (int r = 0; r < 6; ++r)
+            for (int c = 0; c < 6; ++c) S[(p * 6 + r) * dim + p * 6 + c] = ne.diag[p](r, c);
+    if (lambda > 0.0)
+        for (size_t k = 0; k < dim; ++k) S[k * dim + k] += lambda;
+    std::vector<double> b(dim);
+    for (size_t k = 0; k < dim; ++k) b[k] = -ne.b_p[k];
+    for (const LandmarkBlock& lb : ne.lm) {
+        M3 inv;
+        if (!try_inverse3(damped3(lb.h_ll, lambda), &inv)) continue;
+        for (const CrossEntry& a : lb.cross) {
+            M63 a_h = mul(a.w, inv);
+            for (const CrossEntry& bq : lb.cross) {
+                M66 blk = mul(a_h, tr(bq.w));
+                for (int r = 0; r < 6; ++r)
+                    for (int c = 0; c < 6; ++c) S[(a.pose * 6 + r) * dim + bq.pose * 6 + c] -= blk(r, c);
+            }
+            double upd[6];
+            mulv(a_h, lb.b_l, upd);
+            for (int k = 0; k < 6; ++k) b[a.pose * 6 + k] += upd[k];
+        }
+    }
+    std::copy(S.begin(), S.end(), s);
+    std::copy(b.begin(), b.end(), rhs);
+}
+
+int32_t oracle_ns_solve_dense(const oracle_normal_system* ns, double lambda, double* delta_p,
+                              double* delta_l) {
+    Normal ne;
+    normal_from_view(ns, &ne);
+    std::vector<double> dp, dl;
+    int32_t st = solve_step_dense(ne, ns->num_landmarks, lambda, &dp, &dl);
+    if (st != VBA_OK) return st;
+    std::copy(dp.begin(), dp.end(), delta_p);
+    std::copy(dl.begin(), dl.end(), delta_l);
+    return VBA_OK;
+}
+
+int32_t oracle_ns_operator(const oracle_normal_system* ns, double lambda, const double* x, double* y,
+                           const double* r, double* z, double* rhs) {
+    Normal ne;
+    normal_from_view(ns, &ne);
+    Implicit op;
+    int e = build_implicit(ne, lambda, &op);
+    if (e) return e;
+    const size_t dim = op.dim();
+    if (x && y) {
+        std::vector<double> xv(x, x + dim), yv;
+        e = implicit_apply(op, xv, &yv);
+        if (e) return e;
+        std::copy(yv.begin(), yv.end(), y);
+    }
+    if (r && z) {
+        std::vector<double> rv(r, r + dim), zv;
+        e = implicit_precondition(op, rv, &zv);
+        if (e) return e;
+        std::copy(zv.begin(), zv.end(), z);
+    }
+    if (rhs) std::copy(op.rhs.begin(), op.rhs.end(), rhs);
+    return 0;
+}
+
+int32_t oracle_ns_solve_pcg(const oracle_normal_system* ns, double lambda, uint32_t max_iterations,
+                            double rtol, double atol, double* delta_p, double* delta_l,
+                            uint32_t* iterations, double* residual_norm, double* target) {
+    Normal ne;
+    normal_from_view(ns, &ne);
+    Implicit op;
+    int e = build_implicit(ne, lambda, &op);
+    if (e) return e;
+    PcgOut out;
+    e = solve_pcg(op, max_iterations, rtol, atol, false, &out);
+    if (iterations) *iterations = out.iterations;
+    if (residual_norm) *residual_norm = out.residual_norm;
+    if (target) *target = out.target;
+    if (e) return e;
+    std::vector<double> dl;
+    e = complete_delta(op, out.x, &dl);
+    if (e) return e;
+    std::copy(out.x.begin(), out.x.end(), delta_p);
+    std::copy(dl.begin(), dl.end(), delta_l);
+    return 0;
+}
+
+int32_t oracle_solve_normal_equations(uint32_t n, const double* a, const double* b, double* x) {
+    std::vector<double> A(a, a + (size_t)n * n), B(b, b + n), X;
+    if (!solve_normal_equations(A, n, B, &X)) return 0;
+    std::copy(X.begin(), X.end(), x);
+    return 1;
+}
+
+}  // extern "C"

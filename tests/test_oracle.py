@@ -1,0 +1,447 @@
+"""Pins the CPU FP64 oracle (oracle/ba_oracle.cpp) before anything is checked
+against it: the reference's known-answer tests, closed forms, finite
+differences and a brute-force dense solve of the full normal system."""
+import numpy as np
+import pytest
+
+import oracle
+import scenes
+import vba
+from vba import abi
+
+
+# ---------------------------------------------------------------------------
+# implicit_schur.rs:893-912, 1117-1188 — hand_check_system fixture
+# ---------------------------------------------------------------------------
+def hand_check_system():
+    A = np.zeros((6, 3))
+    A[0, 0] = A[1, 1] = A[2, 2] = 1.0
+    return oracle.System(
+        diag=[np.eye(6) * 10.0, np.eye(6) * 10.0],
+        b_p=np.arange(1, 13, dtype=float),
+        h_ll=[np.eye(3) * 4.0],
+        b_l=[[1.0, 2.0, 3.0]],
+        cross=[[(0, A), (0, 2.0 * A), (1, -A)]],
+    )
+
+
+def synthetic_system():
+    """implicit_schur.rs:866-891."""
+    c0 = np.fromfunction(lambda r, c: 0.02 * (r + 1) * (c + 1), (6, 3))
+    c1 = np.fromfunction(lambda r, c: 0.015 * (r + 2) * (c + 1), (6, 3))
+    c2 = np.fromfunction(lambda r, c: 0.01 * (r + 3) * (c + 2), (6, 3))
+    return oracle.System(
+        diag=[np.eye(6) * 20.0, np.eye(6) * 22.0],
+        b_p=0.03 * np.arange(1, 13),
+        h_ll=[np.diag([7.0, 8.0, 9.0])],
+        b_l=[[0.2, -0.1, 0.3]],
+        cross=[[(0, c0), (0, c1), (1, c2)]],
+    )
+
+
+def full_normal(ns: oracle.System, lam: float):
+    """The undamped-then-damped full (6P + 3L) normal matrix and -gradient,
+    assembled directly (implicit_schur.rs:914-948)."""
+    P, L = ns.P, ns.L
+    n = 6 * P + 3 * L
+    H = np.zeros((n, n))
+    for p in range(P):
+        H[6 * p:6 * p + 6, 6 * p:6 * p + 6] = ns.diag[p].reshape(6, 6) + lam * np.eye(6)
+    for l in range(L):
+        o = 6 * P + 3 * l
+        H[o:o + 3, o:o + 3] = ns.h_ll[l].reshape(3, 3) + lam * np.eye(3)
+        for e in range(ns.cross_ptr[l], ns.cross_ptr[l + 1]):
+            p = ns.cross_pose[e]
+            W = ns.cross[e].reshape(6, 3)
+            H[6 * p:6 * p + 6, o:o + 3] += W
+            H[o:o + 3, 6 * p:6 * p + 6] += W.T
+    g = np.concatenate([ns.b_p, ns.b_l.reshape(-1)])
+    return H, -g
+
+
+@pytest.mark.parametrize("lam", [0.0, 0.5])
+def test_hand_check_fixture_matches_closed_form(lam):
+    ns = hand_check_system()
+    S, rhs = ns.reduce_dense(lam)
+    e_pose0 = 7.75 if lam == 0.0 else 8.5
+    e_pose1 = 9.75 if lam == 0.0 else 10.277777777777779
+    e_cross = 0.75 if lam == 0.0 else 0.6666666666666666
+    for c in range(3):
+        assert abs(S[c, c] - e_pose0) < 1e-12
+        assert abs(S[6 + c, 6 + c] - e_pose1) < 1e-12
+        assert abs(S[c, 6 + c] - e_cross) < 1e-12
+        assert abs(S[6 + c, c] - e_cross) < 1e-12
+    for c in range(3, 6):
+        assert abs(S[c, c] - (10.0 + lam)) < 1e-12
+        assert abs(S[6 + c, 6 + c] - (10.0 + lam)) < 1e-12
+        assert S[c, 6 + c] == 0.0
+    e_rhs0 = [-0.25, -0.5, -0.75] if lam == 0.0 else [-1 / 3, -2 / 3, -1.0]
+    e_rhs1 = [-7.25, -8.5, -9.75] if lam == 0.0 else [-65 / 9, -76 / 9, -29 / 3]
+    st, _, _, op_rhs = ns.operator(lam)
+    assert st == 0
+    for c in range(3):
+        assert abs(rhs[c] - e_rhs0[c]) < 1e-12 and abs(op_rhs[c] - e_rhs0[c]) < 1e-12
+        assert abs(rhs[6 + c] - e_rhs1[c]) < 1e-12 and abs(op_rhs[6 + c] - e_rhs1[c]) < 1e-12
+    # brute force: the full normal system solved directly
+    H, b = full_normal(ns, lam)
+    full = np.linalg.solve(H, b)
+    st, dp, dl = ns.solve_dense(lam)
+    assert st == 0
+    assert np.linalg.norm(dp - full[:12]) < 1e-10
+    assert np.linalg.norm(dl - full[12:]) < 1e-10
+    st, dp, dl, it, rn, tg = ns.solve_pcg(lam)
+    assert st == 0 and rn <= tg
+    assert np.linalg.norm(dp - full[:12]) < 1e-9
+    assert np.linalg.norm(dl - full[12:]) < 1e-9
+
+
+def test_implicit_operator_matches_explicit_schur_and_preconditioner():
+    """implicit_schur.rs:952-1015, with the explicit Schur complement formed
+    by numpy from the full normal matrix."""
+    lam = 0.25
+    ns = synthetic_system()
+    H, b = full_normal(ns, lam)
+    A, B, D = H[:12, :12], H[:12, 12:], H[12:, 12:]
+    S = A - B @ np.linalg.inv(D) @ B.T
+    x = 0.04 * np.arange(1, 13)
+    st, y, _, rhs = ns.operator(lam, x=x)
+    assert st == 0
+    assert np.linalg.norm(S @ x - y) < 1e-12
+    assert np.allclose(rhs, b[:12] - B @ np.linalg.solve(D, b[12:]), atol=1e-14)
+    # pose-0 preconditioner block groups the two pose-0 cross blocks
+    c01 = ns.cross[0].reshape(6, 3) + ns.cross[1].reshape(6, 3)
+    M0 = np.eye(6) * (20.0 + lam) - c01 @ np.linalg.inv(np.diag([7.0, 8.0, 9.0]) + lam * np.eye(3)) @ c01.T
+    probe = np.arange(1, 13, dtype=float)
+    st, _, z, _ = ns.operator(lam, r=probe)
+    assert np.linalg.norm(z[:6] - np.linalg.solve(M0, probe[:6])) < 1e-12
+    # PCG step equals the direct step
+    st, dp, dl, it, rn, tg = ns.solve_pcg(lam)
+    full = np.linalg.solve(H, b)
+    assert st == 0 and np.linalg.norm(dp - full[:12]) < 1e-9 and np.linalg.norm(dl - full[12:]) < 1e-9
+
+
+def test_pcg_limits():
+    ns = synthetic_system()
+    st, *_ = ns.solve_pcg(0.25, max_iterations=0)
+    assert st == -7  # MaxIterations: a zero budget cannot solve a nonzero rhs
+    st, dp, dl, it, rn, tg = ns.solve_pcg(0.25, max_iterations=1, rtol=1e-12, atol=1e-12)
+    assert st == -7 and it == 1
+
+
+def test_non_spd_preconditioner_is_rejected():
+    """A Schur block that is not positive definite fails the 6x6 Cholesky
+    (implicit_schur.rs:279-287, test at :1444)."""
+    cross = np.zeros((6, 3))
+    cross[0, 0] = cross[1, 1] = cross[2, 2] = 0.8
+    ns = oracle.System(diag=[np.eye(6), np.eye(6)], b_p=np.ones(12), h_ll=[0.5 * np.eye(3)],
+                       b_l=[[0.0, 0.0, 0.0]], cross=[[(0, cross), (1, cross)]])
+    st, *_ = ns.operator(0.0)
+    assert st == -4
+
+
+# ---------------------------------------------------------------------------
+# Robust kernels (pose_graph.rs:87-140; bundle_adjustment.rs:1445-1485)
+# ---------------------------------------------------------------------------
+def test_robust_kernel_values():
+    assert oracle.robust(abi.KERNEL_HUBER, 2.0, 0.0)[1] == 1.0
+    assert oracle.robust(abi.KERNEL_HUBER, 2.0, 4.0)[1] == 1.0
+    assert abs(oracle.robust(abi.KERNEL_HUBER, 2.0, 9.0)[1] - 2.0 / 3.0) < 1e-12
+    assert abs(oracle.robust(abi.KERNEL_HUBER, 10.0, 5000.0)[0] - (20.0 * np.sqrt(5000.0) - 100.0)) < 1e-9
+    assert abs(oracle.robust(abi.KERNEL_CAUCHY, 10.0, 5000.0)[0] - 100.0 * np.log(51.0)) < 1e-9
+    assert abs(oracle.robust(abi.KERNEL_CAUCHY, 1.0, 3.0)[1] - 0.25) < 1e-15
+    # Huber weight is the derivative of its cost (visual_jacobian_audit_tests.rs:141)
+    for s in [0.5, 3.9, 4.1, 25.0, 400.0]:
+        h = 1e-6 * s
+        c_plus = oracle.robust(abi.KERNEL_HUBER, 2.0, s + h)[0]
+        c_minus = oracle.robust(abi.KERNEL_HUBER, 2.0, s - h)[0]
+        assert abs((c_plus - c_minus) / (2 * h) - oracle.robust(abi.KERNEL_HUBER, 2.0, s)[1]) < 1e-6
+
+
+def test_robust_cost_clips_outlier_to_analytic_huber_value():
+    ba = scenes.over_determined_bundle(True)
+    squared, _ = oracle.cost(ba)
+    huber, _ = oracle.cost(ba, abi.KERNEL_HUBER, 10.0)
+    cauchy, _ = oracle.cost(ba, abi.KERNEL_CAUCHY, 10.0)
+    assert abs(squared - 5000.0) < 1e-6
+    assert huber < squared and cauchy < huber
+    assert abs(huber - 1314.0) < 1.0
+
+
+# ---------------------------------------------------------------------------
+# Projection (jacobians.rs:29-124; camera.rs:351-507, 852-941)
+# ---------------------------------------------------------------------------
+def test_projection_closed_forms():
+    pin = vba.camera(abi.CAMERA_PINHOLE, [500.0, 500.0, 320.0, 240.0])
+    uv, J = oracle.mono_project(pin, [1.0, -0.5, 5.0])
+    assert np.allclose(uv, [420.0, 190.0], atol=1e-12)
+    assert np.allclose(J, [[100.0, 0.0, -20.0], [0.0, 100.0, 10.0]], atol=1e-12)
+    # a point on the optical axis projects to the principal point for every lens
+    for cam in [pin, vba.camera(abi.CAMERA_SIMPLE_RADIAL, [500.0, 320.0, 240.0, 0.3]),
+                vba.camera(abi.CAMERA_OPENCV, [510.0, 490.0, 321.0, 239.0, -0.1, 0.05, 0.01, -0.02])]:
+        uv = oracle.camera_project(cam, [0.0, 0.0, 3.0])
+        assert np.allclose(uv, [cam.params[1] if cam.model == abi.CAMERA_SIMPLE_RADIAL else cam.params[2],
+                                cam.params[2] if cam.model == abi.CAMERA_SIMPLE_RADIAL else cam.params[3]],
+                           atol=1e-12)
+    # SIMPLE_RADIAL [f, cx, cy, k]: x = 0.5, r^2 = 0.25, d = 1 + 0.01 * 0.25
+    rad = vba.camera(abi.CAMERA_SIMPLE_RADIAL, [500.0, 320.0, 240.0, 0.01])
+    uv = oracle.camera_project(rad, [0.5, 0.0, 1.0])
+    assert abs(uv[0] - 570.625) < 1e-12 and abs(uv[1] - 240.0) < 1e-12
+    # behind the camera: None
+    for cam in [pin, rad]:
+        assert oracle.camera_project(cam, [0.1, 0.1, -1.0]) is None
+        assert oracle.mono_project(cam, [0.1, 0.1, -1.0]) is None
+
+
+LENSES = [
+    vba.camera(abi.CAMERA_PINHOLE, [512.5, 498.25, 321.0, 239.5]),
+    vba.camera(abi.CAMERA_SIMPLE_RADIAL, [500.0, 320.0, 240.0, 0.01]),
+    vba.camera(abi.CAMERA_PINHOLE, [512.5, 498.25, 321.0, 239.5, -0.17, 0.06]),
+    vba.camera(abi.CAMERA_OPENCV, [500.0, 500.0, 320.0, 240.0, -0.15, 0.05, 0.004, -0.003]),
+    vba.camera(abi.CAMERA_OPENCV, [500.0, 500.0, 320.0, 240.0, -0.15, 0.05, 0.0, 0.0]),
+]
+
+
+def sample_points():
+    """mono_projection_tests.rs:15-26."""
+    return [np.array([-1.3 + 0.41 * i, -0.9 + 0.43 * j, 1.5 + 0.37 * (i + j)]) for i in range(7) for j in range(5)]
+
+
+@pytest.mark.parametrize("cam", LENSES, ids=lambda c: f"model{c.model}")
+def test_projection_jacobian_matches_finite_differences(cam):
+    """mono_projection_tests.rs:126-191."""
+    for xc in sample_points():
+        uv, J = oracle.mono_project(cam, xc)
+        uv_cost = oracle.camera_project(cam, xc)
+        assert np.allclose(uv, uv_cost, rtol=0, atol=1e-10)
+        h = 1e-6
+        for axis in range(3):
+            p, m = xc.copy(), xc.copy()
+            p[axis] += h
+            m[axis] -= h
+            num = (oracle.camera_project(cam, p) - oracle.camera_project(cam, m)) / (2 * h)
+            for row in range(2):
+                assert abs(J[row, axis] - num[row]) < 1e-5 * (1 + abs(J[row, axis]))
+
+
+def test_radial_projection_equals_cost_projection_bitwise():
+    """The cost evaluates Camera::project: identical bits for the radial
+    closed form (mono_projection_tests.rs:118-120)."""
+    for cam in LENSES[1:3] + LENSES[4:5]:
+        for xc in sample_points():
+            uv, _ = oracle.mono_project(cam, xc)
+            assert np.array_equal(uv, oracle.camera_project(cam, xc))
+
+
+# ---------------------------------------------------------------------------
+# SE3 (se3.rs:57-134, tests :153-211)
+# ---------------------------------------------------------------------------
+def test_se3_exp_closed_forms_and_log_round_trip():
+    # pure translation
+    p = oracle.se3_exp([0.3, -0.2, 0.1, 0.0, 0.0, 0.0])
+    assert np.allclose(p, [1, 0, 0, 0, 0.3, -0.2, 0.1], atol=1e-15)
+    # rotation about z by theta: q = (cos th/2, 0, 0, sin th/2)
+    th = 0.7
+    p = oracle.se3_exp([0, 0, 0, 0, 0, th])
+    assert np.allclose(p[:4], [np.cos(th / 2), 0, 0, np.sin(th / 2)], atol=1e-15)
+    assert np.allclose(scenes.quat_to_matrix(p[:4]), scenes.rodrigues(np.array([0, 0, th])), atol=1e-15)
+    for xi in [[0.1, -0.2, 0.3, 0.01, 0.02, -0.03], [1.0, 2.0, -0.5, 0.5, -1.2, 0.4], [0, 0, 0, 1e-9, 0, 0],
+               [0.2, 0.0, 0.0, 2.5, 0.3, -0.1]]:
+        back = oracle.se3_log(oracle.se3_exp(xi))
+        assert np.allclose(back, xi, atol=1e-12)
+    # exp(xi) applied to a point equals the 4x4 matrix exponential
+    xi = np.array([0.4, -0.1, 0.2, 0.3, -0.2, 0.5])
+    Xi = np.zeros((4, 4))
+    Xi[:3, :3] = [[0, -xi[5], xi[4]], [xi[5], 0, -xi[3]], [-xi[4], xi[3], 0]]
+    Xi[:3, 3] = xi[:3]
+    from scipy.linalg import expm
+
+    T = expm(Xi)
+    p = oracle.se3_exp(xi)
+    assert np.allclose(scenes.quat_to_matrix(p[:4]), T[:3, :3], atol=1e-14)
+    assert np.allclose(p[4:], T[:3, 3], atol=1e-14)
+
+
+# ---------------------------------------------------------------------------
+# Per-observation Jacobians vs finite differences of the residual
+# (visual_jacobian_audit_tests.rs:49; right perturbation T <- T Exp(xi)).
+# ---------------------------------------------------------------------------
+def residual(cam, pose, X, uv):
+    return oracle.camera_project(cam, scenes.transform(pose, X)) - uv
+
+
+@pytest.mark.parametrize("cam", LENSES, ids=lambda c: f"model{c.model}")
+def test_linearization_matches_finite_differences(cam):
+    rng = np.random.default_rng(7)
+    poses = [(1, scenes.pose_from_rt(scenes.rodrigues(rng.normal(0, 0.2, 3)), rng.normal(0, 0.3, 3)))]
+    lms = [(i + 1, np.array([rng.uniform(-1, 1), rng.uniform(-1, 1), rng.uniform(4, 6)])) for i in range(8)]
+    obs = [(1, l, np.array([300.0, 250.0]) + rng.normal(0, 3, 2)) for l, _ in lms]
+    cam.width, cam.height = 640, 480
+    ba = scenes.make_problem(cam, poses, lms, obs)
+    r, jp, jl, w, valid = oracle.linearize(ba)
+    assert valid.all() and np.all(w == 1.0)
+    h = 1e-6
+    for k, (lid, X) in enumerate(lms):
+        pose = poses[0][1]
+        assert np.allclose(r[k], residual(cam, pose, X, obs[k][2]), atol=1e-9)
+        for a in range(6):
+            xi = np.zeros(6)
+            xi[a] = h
+            rp = residual(cam, oracle.se3_compose(pose, oracle.se3_exp(xi)), X, obs[k][2])
+            xi[a] = -h
+            rm = residual(cam, oracle.se3_compose(pose, oracle.se3_exp(xi)), X, obs[k][2])
+            assert np.allclose(jp[k][:, a], (rp - rm) / (2 * h), rtol=1e-5, atol=1e-4)
+        for a in range(3):
+            d = np.zeros(3)
+            d[a] = h
+            num = (residual(cam, pose, X + d, obs[k][2]) - residual(cam, pose, X - d, obs[k][2])) / (2 * h)
+            assert np.allclose(jl[k][:, a], num, rtol=1e-5, atol=1e-4)
+
+
+# ---------------------------------------------------------------------------
+# End-to-end LM (pipelines/slam/tests/bundle_adjustment.rs)
+# ---------------------------------------------------------------------------
+def test_cost_is_zero_at_truth():
+    c, nonproj = oracle.cost(scenes.truth_bundle())
+    assert c < 1e-20 and nonproj == 0
+
+
+def test_drifted_pose_recovered():
+    """bundle_adjustment.rs:367-402."""
+    ba = scenes.truth_bundle(drift30=[1.05, 0.02, 0.03])
+    assert oracle.cost(ba)[0] > 1e-3
+    out = oracle.optimize(ba, vba.default_options())
+    assert out.status == abi.OK
+    assert out.result.final_cost < 1e-12 and out.result.final_cost < out.result.initial_cost
+    assert out.result.converged
+    centre = vba.camera_centres(out.poses)[2]
+    assert np.linalg.norm(centre - [1.0, 0.0, 0.0]) < 1e-6
+
+
+def test_feasibility_gate_rejects_step_behind_camera():
+    """bundle_adjustment.rs:1004-1032."""
+    ba = scenes.make_problem(scenes.pinhole_camera(), [(10, scenes.pose_at([0, 0, 0]))],
+                             [(1, np.array([1.0, 0.0, 0.1]))], [(10, 1, np.array([105320.0, 240.0]))],
+                             fixed_poses=(10,))
+    out = oracle.optimize(ba, vba.default_options(max_iterations=1))
+    assert out.status == abi.OK
+    assert np.array_equal(out.landmarks[0], [1.0, 0.0, 0.1])
+    assert len(out.iterations) == 1 and not out.iterations[0]["step_accepted"]
+    assert out.result.final_cost == out.result.initial_cost
+
+
+def test_dense_and_matrix_free_agree_on_small_scene():
+    """bundle_adjustment.rs:1160-1200 (sparse == dense), here dense ==
+    matrix-free with a tight PCG."""
+    ba = scenes.truth_bundle(drift30=[1.04, 0.01, -0.02])
+    d = oracle.optimize(ba, vba.default_options())
+    m = oracle.optimize(ba, vba.default_options(linear_solver=abi.SOLVER_MATRIX_FREE))
+    assert d.status == m.status == abi.OK
+    assert d.result.converged == m.result.converged
+    assert abs(d.result.final_cost - m.result.final_cost) < 1e-12
+    assert np.linalg.norm(vba.camera_centres(d.poses)[2] - vba.camera_centres(m.poses)[2]) < 1e-9
+
+
+def test_first_lm_step_equals_brute_force_full_solve():
+    """Schur reduction + back-substitution + update against the full damped
+    normal system solved by numpy from the oracle's per-observation
+    linearization (itself pinned by finite differences above)."""
+    s = vba.synthesize(abi.SYNTH_CIRCLE, num_cameras=5, num_landmarks=40, seed=11)
+    ba = s.problem
+    lam = 1e-4
+    r, jp, jl, w, valid = oracle.linearize(ba)
+    P = len(ba.pose_ids)
+    var_pose = {i: k for k, i in enumerate([i for i in range(P) if not ba.pose_fixed[i]])}
+    Pv, L = len(var_pose), len(ba.landmark_ids)
+    pose_index = {int(pid): i for i, pid in enumerate(ba.pose_ids)}
+    lm_index = {int(lid): i for i, lid in enumerate(ba.landmark_ids)}
+    n = 6 * Pv + 3 * L
+    H = np.zeros((n, n))
+    g = np.zeros(n)
+    for k in range(ba.num_observations):
+        if not valid[k]:
+            continue
+        p = var_pose.get(pose_index[int(ba.obs_pose_ids[k])])
+        l = lm_index[int(ba.obs_landmark_ids[k])]
+        J = np.zeros((2, n))
+        if p is not None:
+            J[:, 6 * p:6 * p + 6] = jp[k]
+        J[:, 6 * Pv + 3 * l:6 * Pv + 3 * l + 3] = jl[k]
+        H += w[k] * J.T @ J
+        g += w[k] * J.T @ r[k]
+    delta = np.linalg.solve(H + lam * np.eye(n), -g)
+    out = oracle.optimize(ba, vba.default_options(max_iterations=1))
+    assert out.iterations[0]["step_accepted"]
+    for i, k in var_pose.items():
+        expect = oracle.se3_compose(ba.poses[i], oracle.se3_exp(delta[6 * k:6 * k + 6]))
+        assert np.allclose(out.poses[i], expect, atol=1e-9)
+    assert np.allclose(out.landmarks, ba.landmarks + delta[6 * Pv:].reshape(-1, 3), atol=1e-9)
+
+
+def test_parallel_assembly_is_bitwise_equal_to_serial():
+    """parallel_ba_tests.rs:113-160."""
+    s = vba.synthesize(abi.SYNTH_LOOP, num_cameras=40, num_landmarks=2500, seed=5, scene_scale=12.0)
+    o = vba.default_options(max_iterations=4)
+    a = oracle.optimize(s.problem, o, threads=1)
+    b = oracle.optimize(s.problem, o, threads=4)
+    assert a.status == b.status == abi.OK
+    assert a.result.final_cost == b.result.final_cost
+    assert np.array_equal(a.poses, b.poses) and np.array_equal(a.landmarks, b.landmarks)
+    assert a.iterations == b.iterations
+
+
+def test_kernel_none_matches_gauss_newton_weights():
+    """bundle_adjustment.rs:1371: RobustKernel::None gives unit weights."""
+    ba = scenes.over_determined_bundle(True)
+    _, _, _, w, valid = oracle.linearize(ba)
+    assert valid.all() and np.all(w == 1.0)
+    _, _, _, wh, _ = oracle.linearize(ba, abi.KERNEL_HUBER, 10.0)
+    assert np.sum(wh < 1.0) == 1
+
+
+def test_error_codes():
+    cam = scenes.pinhole_camera()
+    empty = scenes.make_problem(cam, [], [], [])
+    assert oracle.optimize(empty, vba.default_options()).status == abi.ERR_NO_POSES
+    one = scenes.make_problem(cam, [(1, scenes.pose_at([0, 0, 0]))], [], [])
+    assert oracle.optimize(one, vba.default_options()).status == abi.ERR_NO_OBSERVATIONS
+    nolm = scenes.make_problem(cam, [(1, scenes.pose_at([0, 0, 0]))], [], [(1, 99, np.array([320.0, 240.0]))])
+    assert oracle.optimize(nolm, vba.default_options()).status == abi.ERR_NO_LANDMARKS
+    ba = scenes.truth_bundle()
+    bad = ba.copy()
+    bad.obs_pose_ids[3] = 77
+    out = oracle.optimize(bad, vba.default_options())
+    assert out.status == abi.ERR_MISSING_POSE and out.result.error_id == 77
+    bad = ba.copy()
+    bad.obs_landmark_ids[5] = 1234
+    out = oracle.optimize(bad, vba.default_options())
+    assert out.status == abi.ERR_MISSING_LANDMARK and out.result.error_id == 1234
+    allfixed = ba.copy()
+    allfixed.pose_fixed[:] = 1
+    allfixed.landmark_fixed[:] = 1
+    assert oracle.optimize(allfixed, vba.default_options()).status == abi.ERR_ALL_POSES_FIXED
+
+
+def test_matrix_free_entry_validation():
+    """optimize.rs:382-626."""
+    ba = scenes.truth_bundle(fixed=(), drift30=[1.05, 0.0, 0.0])
+    mf = vba.default_options(linear_solver=abi.SOLVER_MATRIX_FREE)
+    assert oracle.optimize(ba, mf).status == abi.ERR_INELIGIBLE  # no gauge anchor
+    ba = scenes.truth_bundle(drift30=[1.05, 0.0, 0.0])
+    assert oracle.optimize(ba, vba.default_options(linear_solver=abi.SOLVER_MATRIX_FREE,
+                                                   has_initial_lambda=0)).status == abi.ERR_INVALID_CONFIGURATION
+    assert oracle.optimize(ba, vba.default_options(linear_solver=abi.SOLVER_MATRIX_FREE,
+                                                   max_pcg_iterations=0)).status == abi.ERR_INVALID_CONFIGURATION
+    assert oracle.optimize(ba, vba.default_options(linear_solver=abi.SOLVER_MATRIX_FREE,
+                                                   robust_kernel=abi.KERNEL_HUBER,
+                                                   robust_scale=0.0)).status == abi.ERR_INVALID_CONFIGURATION
+
+
+def test_circle_lm_converges_and_recovers_cameras():
+    """A reduced CIRCLE workload: LM converges and the camera centres move
+    towards the truth."""
+    s = vba.synthesize(abi.SYNTH_CIRCLE, num_cameras=10, num_landmarks=200, seed=1)
+    out = oracle.optimize(s.problem, vba.default_options())
+    assert out.status == abi.OK and out.result.final_cost < 0.05 * out.result.initial_cost
+    before = np.linalg.norm(vba.camera_centres(s.problem.poses) - vba.camera_centres(s.truth_poses), axis=1)
+    after = np.linalg.norm(vba.camera_centres(out.poses) - vba.camera_centres(s.truth_poses), axis=1)
+    assert after.max() < before.max()

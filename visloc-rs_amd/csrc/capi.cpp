@@ -1,0 +1,150 @@
+// capi.cpp — the extern "C" entry points of include/vba.h.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <new>
+
+#include "../../include/vba.h"
+#include "host_common.hpp"
+#include "problem.hpp"
+#include "solver.hpp"
+
+struct vba_context : vba::Context {};
+
+extern "C" {
+
+uint32_t vba_abi_version(void) { return VBA_ABI_VERSION; }
+
+vba_context* vba_create(int32_t device) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+        vba::set_thread_error("vba_create: no HIP device available");
+        return nullptr;
+    }
+    if (device < 0 || device >= count) {
+        vba::set_thread_error("vba_create: device %d out of range (%d devices)", device, count);
+        return nullptr;
+    }
+    if (hipSetDevice(device) != hipSuccess) {
+        vba::set_thread_error("vba_create: hipSetDevice(%d) failed", device);
+        return nullptr;
+    }
+    std::unique_ptr<vba_context> ctx(new (std::nothrow) vba_context());
+    if (!ctx) {
+        vba::set_thread_error("vba_create: out of host memory");
+        return nullptr;
+    }
+    ctx->device = device;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&ctx->ev_start) != hipSuccess || hipEventCreate(&ctx->ev_stop) != hipSuccess) {
+        vba::set_thread_error("vba_create: stream/event creation failed");
+        return nullptr;
+    }
+    return ctx.release();
+}
+
+void vba_destroy(vba_context* ctx) {
+    if (!ctx) return;
+    {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        vba::release_workspace(ctx->ws);
+        if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
+        if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
+        if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    }
+    delete ctx;
+}
+
+const char* vba_last_error(const vba_context* ctx) {
+    if (!ctx) return vba::thread_error().c_str();
+    return ctx->last_error.c_str();
+}
+
+void vba_default_options(vba_options* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    // BaConfig::default (pipelines/slam/src/bundle/mod.rs:729-752)
+    o->max_iterations = 20;
+    o->has_initial_lambda = 1;
+    o->initial_lambda = 1e-4;
+    o->lambda_increase_factor = 10.0;
+    o->lambda_decrease_factor = 0.1;
+    o->max_lambda = 1e12;
+    o->min_lambda = 1e-9;
+    o->step_tolerance = 1e-7;
+    o->cost_tolerance = 1e-9;
+    o->robust_kernel = VBA_KERNEL_NONE;
+    o->linear_solver = VBA_SOLVER_DENSE;
+    o->precision = VBA_PRECISION_FP64;
+    // MatrixFreeBaOptions::default (mod.rs:789-798)
+    o->max_pcg_iterations = 128;
+    o->pcg_relative_tolerance = 1e-12;
+    o->pcg_absolute_tolerance = 1e-12;
+    o->pcg_maxit_policy = VBA_PCG_MAXIT_REJECT;
+}
+
+int32_t vba_check(const vba_problem* problem, const vba_options* options) {
+    if (!problem || !options) {
+        vba::set_thread_error("vba_check: null argument");
+        return VBA_ERR_INVALID_ARGUMENT;
+    }
+    std::string reason;
+    int32_t st = vba::check_support(*problem, *options, &reason);
+    if (st != VBA_OK) vba::set_thread_error("declined: %s", reason.c_str());
+    return st;
+}
+
+int32_t vba_optimize(vba_context* ctx, const vba_problem* problem, const vba_options* options, double* poses_out,
+                     double* landmarks_out, vba_iteration_stats* stats, uint32_t stats_capacity,
+                     vba_result* result) {
+    if (!ctx || !problem || !options || !result) {
+        vba::set_thread_error("vba_optimize: null argument");
+        if (ctx) ctx->last_error = "vba_optimize: null argument";
+        return VBA_ERR_INVALID_ARGUMENT;
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    std::memset(result, 0, sizeof(*result));
+    ctx->last_error.clear();
+    std::string reason;
+    int32_t st = vba::check_support(*problem, *options, &reason);
+    if (st != VBA_OK) {
+        ctx->last_error = "declined: " + reason;
+        return st;
+    }
+    if (hipSetDevice(ctx->device) != hipSuccess) {
+        ctx->last_error = "hipSetDevice failed";
+        return VBA_ERR_DEVICE;
+    }
+    vba::Model model;
+    uint64_t error_id = 0;
+    std::string msg;
+    try {
+        st = vba::build_model(*problem, &model, &error_id, &msg);
+    } catch (const std::bad_alloc&) {
+        ctx->last_error = "out of host memory";
+        return VBA_ERR_OUT_OF_MEMORY;
+    }
+    result->error_id = error_id;
+    if (st != VBA_OK) {
+        ctx->last_error = msg.empty() ? "invalid problem" : msg;
+        return st;
+    }
+    if (options->linear_solver != VBA_SOLVER_DENSE) {
+        st = vba::validate_matrix_free(model, *options, &msg);
+        if (st != VBA_OK) {
+            ctx->last_error = msg;
+            return st;
+        }
+    }
+    try {
+        return vba::solve(*ctx, model, *options, poses_out, landmarks_out, stats, stats_capacity, result);
+    } catch (const std::bad_alloc&) {
+        ctx->last_error = "out of host memory";
+        return VBA_ERR_OUT_OF_MEMORY;
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,1176 @@
+// kernels.hip — CDNA4 (gfx950) kernels of the Levenberg–Marquardt Schur path.
+//
+// Work units: camera tiles (a slice of one pose's c-order edges) and landmark
+// tiles (a run of whole landmarks in l-order). Every reduction is a fixed
+// tree inside one work-group (wave shuffles, then LDS) followed by an
+// in-order sum over tiles, so results are deterministic and independent of
+// scheduling; no global atomics are used.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+
+#include "kernels.hpp"
+
+namespace vba {
+
+namespace {
+
+// Work-groups are dispatched round-robin over the 8 XCDs. Give XCD k a
+// contiguous range of tiles so that neighbouring cameras, which share
+// landmarks, share that XCD's L2.
+__device__ __forceinline__ int xcd_tile(int bid, int n) {
+    const int per = (n + 7) >> 3;
+    return (bid & 7) * per + (bid >> 3);
+}
+inline int xcd_grid(int n) { return ((n + 7) >> 3) << 3; }
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+    return v;
+}
+
+// Sum N values over an NT-thread work-group; the result is valid in thread 0
+// and the order of additions is fixed. lds must hold (NT/64)*N values.
+template <int NT, typename T, int N>
+__device__ __forceinline__ void block_sum(T (&v)[N], T* lds) {
+    constexpr int NW = NT / 64;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = wave_sum(v[i]);
+    if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) lds[wave * N + i] = v[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            T s = lds[i];
+            for (int w = 1; w < NW; ++w) s += lds[w * N + i];
+            v[i] = s;
+        }
+    }
+    __syncthreads();
+}
+// Sum broadcast to every thread.
+template <int NT, typename T, int N>
+__device__ __forceinline__ void block_sum_all(T (&v)[N], T* lds) {
+    block_sum<NT, T, N>(v, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) lds[i] = v[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = lds[i];
+    __syncthreads();
+}
+template <int NT>
+__device__ __forceinline__ double block_max(double v, double* lds) {
+    constexpr int NW = NT / 64;
+    v = wave_max(v);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = lds[0];
+        for (int w = 1; w < NW; ++w) m = fmax(m, lds[w]);
+        v = m;
+    }
+    __syncthreads();
+    return v;
+}
+
+// Neumaier compensated pair (sum, compensation) and its exact-order merge.
+struct CSum {
+    double s, c;
+};
+__device__ __forceinline__ void csum_add(CSum& a, double x) {
+    double t = a.s + x;
+    if (fabs(a.s) >= fabs(x)) a.c += (a.s - t) + x;
+    else a.c += (x - t) + a.s;
+    a.s = t;
+}
+__device__ __forceinline__ CSum csum_merge(CSum a, CSum b) {
+    CSum r;
+    r.s = a.s + b.s;
+    double bp = r.s - a.s;
+    double e = (a.s - (r.s - bp)) + (b.s - bp);
+    r.c = a.c + b.c + e;
+    return r;
+}
+
+// Packed upper-triangle index of a symmetric 6x6.
+__host__ __device__ constexpr int up6(int i, int j) { return i * 6 - i * (i - 1) / 2 + (j - i); }
+
+// Accumulate s * [[A, -A K], [K A, -K A K]] (upper triangle) with K = [X]x,
+// A symmetric (a00 a01 a02 a11 a12 a22).
+template <typename T>
+__device__ __forceinline__ void add_schur_block(const T A[6], const T X[3], T s, T out[21]) {
+    const T a[9] = {A[0], A[1], A[2], A[1], A[3], A[4], A[2], A[4], A[5]};
+    const T K[9] = {T(0), -X[2], X[1], X[2], T(0), -X[0], -X[1], X[0], T(0)};
+    T AK[9], KAK[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) AK[3 * i + j] = a[3 * i] * K[j] + a[3 * i + 1] * K[3 + j] + a[3 * i + 2] * K[6 + j];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) KAK[3 * i + j] = K[3 * i] * AK[j] + K[3 * i + 1] * AK[3 + j] + K[3 * i + 2] * AK[6 + j];
+    out[up6(0, 0)] += s * a[0];
+    out[up6(0, 1)] += s * a[1];
+    out[up6(0, 2)] += s * a[2];
+    out[up6(1, 1)] += s * a[4];
+    out[up6(1, 2)] += s * a[5];
+    out[up6(2, 2)] += s * a[8];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) out[up6(i, 3 + j)] -= s * AK[3 * i + j];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = i; j < 3; ++j) out[up6(3 + i, 3 + j)] -= s * KAK[3 * i + j];
+}
+
+// y = sym(A) x for packed symmetric 3x3.
+template <typename T>
+__device__ __forceinline__ void sym3_mul(const T A[6], const T x[3], T y[3]) {
+    y[0] = A[0] * x[0] + A[1] * x[1] + A[2] * x[2];
+    y[1] = A[1] * x[0] + A[3] * x[1] + A[4] * x[2];
+    y[2] = A[2] * x[0] + A[4] * x[1] + A[5] * x[2];
+}
+// y = sym(U) x for packed symmetric 6x6.
+__device__ __forceinline__ void sym6_mul(const double* U, const double* x, double* y) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) s += U[i <= j ? up6(i, j) : up6(j, i)] * x[j];
+        y[i] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Linearization: residuals, Jacobians and IRLS weights in FP64 at the current
+// state, G and h per edge into c-order and l-order records, and the tile's
+// share of U_p, g_p (normal_equations.rs:418-484, 617-651).
+// ---------------------------------------------------------------------------
+template <bool RIG>
+__global__ __launch_bounds__(kBlock) void k_linearize(DevProblem P, int store32) {
+    __shared__ double lds[(kBlock / 64) * 27];
+    const int tile = xcd_tile(blockIdx.x, P.num_cam_tiles);
+    if (tile >= P.num_cam_tiles) return;
+    const CamTile ct = P.cam_tiles[tile];
+    const double* pq = P.poses + 7 * (size_t)ct.pose;
+    double R[9];
+    qmatrix<double>(Quat{pq[0], pq[1], pq[2], pq[3]}, R);
+    const double t0 = pq[4], t1 = pq[5], t2 = pq[6];
+    double acc[27];
+#pragma unroll
+    for (int i = 0; i < 27; ++i) acc[i] = 0.0;
+    for (int e = ct.begin + (int)threadIdx.x; e < ct.end; e += kBlock) {
+        const int l = P.edge_lm[e];
+        const double X[3] = {P.lms[3 * (size_t)l], P.lms[3 * (size_t)l + 1], P.lms[3 * (size_t)l + 2]};
+        const double xr[3] = {R[0] * X[0] + R[1] * X[1] + R[2] * X[2] + t0,
+                              R[3] * X[0] + R[4] * X[1] + R[5] * X[2] + t1,
+                              R[6] * X[0] + R[7] * X[1] + R[8] * X[2] + t2};
+        double G[6] = {0, 0, 0, 0, 0, 0}, h[3] = {0, 0, 0};
+        const int o_end = P.edge_obs[e + 1];
+        for (int o = P.edge_obs[e]; o < o_end; ++o) {
+            const DevSensor& S = P.sensors[RIG ? P.obs_sensor[o] : 0];
+            double xs[3], M[9];
+            if (RIG && !S.identity) {
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    xs[i] = S.R[3 * i] * xr[0] + S.R[3 * i + 1] * xr[1] + S.R[3 * i + 2] * xr[2] + S.t[i];
+#pragma unroll
+                    for (int j = 0; j < 3; ++j)
+                        M[3 * i + j] = S.R[3 * i] * R[j] + S.R[3 * i + 1] * R[3 + j] + S.R[3 * i + 2] * R[6 + j];
+                }
+            } else {
+                xs[0] = xr[0]; xs[1] = xr[1]; xs[2] = xr[2];
+#pragma unroll
+                for (int i = 0; i < 9; ++i) M[i] = R[i];
+            }
+            double uv[2], Jp[6];
+            if (!project<double>(S.lens, xs[0], xs[1], xs[2], uv, Jp)) continue;
+            const double r0 = uv[0] - P.obs_uv[2 * (size_t)o], r1 = uv[1] - P.obs_uv[2 * (size_t)o + 1];
+            const double w = robust_weight<double>((uint32_t)P.kernel_kind, P.kernel_scale, r0 * r0 + r1 * r1);
+            double J[6];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                J[c] = Jp[0] * M[c] + Jp[1] * M[3 + c] + Jp[2] * M[6 + c];
+                J[3 + c] = Jp[3] * M[c] + Jp[4] * M[3 + c] + Jp[5] * M[6 + c];
+            }
+            G[0] += w * (J[0] * J[0] + J[3] * J[3]);
+            G[1] += w * (J[0] * J[1] + J[3] * J[4]);
+            G[2] += w * (J[0] * J[2] + J[3] * J[5]);
+            G[3] += w * (J[1] * J[1] + J[4] * J[4]);
+            G[4] += w * (J[1] * J[2] + J[4] * J[5]);
+            G[5] += w * (J[2] * J[2] + J[5] * J[5]);
+            h[0] += w * (J[0] * r0 + J[3] * r1);
+            h[1] += w * (J[1] * r0 + J[4] * r1);
+            h[2] += w * (J[2] * r0 + J[5] * r1);
+        }
+        const int pl = P.perm_cl[e];
+        EdgeRec64 rc, rl;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) rc.G[i] = rl.G[i] = G[i];
+        rc.idx = l;
+        rl.idx = ct.pose;
+        rc.pad[0] = rc.pad[1] = rc.pad[2] = rl.pad[0] = rl.pad[1] = rl.pad[2] = 0;
+        P.ec64[e] = rc;
+        P.el64[pl] = rl;
+        P.hl64[3 * (size_t)pl] = h[0];
+        P.hl64[3 * (size_t)pl + 1] = h[1];
+        P.hl64[3 * (size_t)pl + 2] = h[2];
+        if (store32) {
+            EdgeRec32 fc, fl;
+#pragma unroll
+            for (int i = 0; i < 6; ++i) fc.G[i] = fl.G[i] = (float)G[i];
+            fc.idx = l;
+            fl.idx = ct.pose;
+            fc.pad = fl.pad = 0;
+            P.ec32[e] = fc;
+            P.el32[pl] = fl;
+        }
+        add_schur_block<double>(G, X, 1.0, acc);
+        acc[21] += h[0];
+        acc[22] += h[1];
+        acc[23] += h[2];
+        acc[24] += X[1] * h[2] - X[2] * h[1];
+        acc[25] += X[2] * h[0] - X[0] * h[2];
+        acc[26] += X[0] * h[1] - X[1] * h[0];
+    }
+    block_sum<kBlock, double, 27>(acc, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < 27; ++i) P.tile27[27 * (size_t)tile + i] = acc[i];
+    }
+}
+
+// U_p, g_p: in-order sum of the pose's tile partials.
+__global__ void k_camera_finalize(DevProblem P) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P.num_pose_slots) return;
+    double acc[27];
+#pragma unroll
+    for (int i = 0; i < 27; ++i) acc[i] = 0.0;
+    for (int t = P.cam_tile_ptr[p]; t < P.cam_tile_ptr[p + 1]; ++t)
+#pragma unroll
+        for (int i = 0; i < 27; ++i) acc[i] += P.tile27[27 * (size_t)t + i];
+#pragma unroll
+    for (int i = 0; i < 21; ++i) P.U[21 * (size_t)p + i] = acc[i];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) P.g[6 * (size_t)p + i] = acc[21 + i];
+}
+
+// V_l, g_l: in-order sum over the landmark's l-order edges.
+__global__ void k_landmark_reduce(DevProblem P) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= P.num_lm_slots) return;
+    double V[6] = {0, 0, 0, 0, 0, 0}, g[3] = {0, 0, 0};
+    for (int e = P.lm_edge_ptr[l]; e < P.lm_edge_ptr[l + 1]; ++e) {
+        const EdgeRec64& r = P.el64[e];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) V[i] += r.G[i];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) g[i] += P.hl64[3 * (size_t)e + i];
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) P.V[6 * (size_t)l + i] = V[i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) P.gl[3 * (size_t)l + i] = g[i];
+}
+
+// Damped landmark inverse (V + lambda I)^-1 by cofactors, zero when the
+// determinant is exactly zero (the landmark then drops out, as try_inverse
+// returning None does in normal_equations.rs:1432-1445), and y = V^-1 g_l.
+__global__ void k_landmark_prepare(DevProblem P, double lambda) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= P.num_lm_slots) return;
+    double X[3] = {P.lms[3 * (size_t)l], P.lms[3 * (size_t)l + 1], P.lms[3 * (size_t)l + 2]};
+    double inv[6] = {0, 0, 0, 0, 0, 0}, y[3] = {0, 0, 0};
+    if (P.lm_var[l] >= 0) {
+        const double* V = P.V + 6 * (size_t)l;
+        double a = V[0], b = V[1], c = V[2], d = V[3], e = V[4], f = V[5];
+        if (lambda > 0.0) {
+            a += lambda;
+            d += lambda;
+            f += lambda;
+        }
+        // symmetric [[a b c][b d e][c e f]]
+        const double c00 = d * f - e * e, c01 = c * e - b * f, c02 = b * e - c * d;
+        const double det = a * c00 + b * c01 + c * c02;
+        if (det != 0.0 && isfinite(det)) {
+            inv[0] = c00 / det;
+            inv[1] = c01 / det;
+            inv[2] = c02 / det;
+            inv[3] = (a * f - c * c) / det;
+            inv[4] = (b * c - a * e) / det;
+            inv[5] = (a * d - b * b) / det;
+            sym3_mul(inv, P.gl + 3 * (size_t)l, y);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) P.Vinv64[6 * (size_t)l + i] = inv[i];
+    LmPack<double> p64;
+    p64.X[0] = X[0]; p64.X[1] = X[1]; p64.X[2] = X[2]; p64.pad0 = 0.0;
+    p64.u[0] = y[0]; p64.u[1] = y[1]; p64.u[2] = y[2]; p64.pad1 = 0.0;
+    P.pack64[l] = p64;
+    if (P.pack32) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) P.Vinv32[6 * (size_t)l + i] = (float)inv[i];
+        LmPack<float> p32;
+        p32.X[0] = (float)X[0]; p32.X[1] = (float)X[1]; p32.X[2] = (float)X[2]; p32.pad0 = 0.f;
+        p32.u[0] = (float)y[0]; p32.u[1] = (float)y[1]; p32.u[2] = (float)y[2]; p32.pad1 = 0.f;
+        P.pack32[l] = p32;
+    }
+}
+
+// Block-Jacobi blocks: per camera tile, the sum of W V^-1 W^T over its edges
+// (implicit_schur.rs:241-287); W V^-1 W^T = [[C, -C K], [K C, -K C K]] with
+// C = G V^-1 G.
+__global__ __launch_bounds__(kBlock) void k_precondition_tiles(DevProblem P) {
+    __shared__ double lds[(kBlock / 64) * 21];
+    const int tile = xcd_tile(blockIdx.x, P.num_cam_tiles);
+    if (tile >= P.num_cam_tiles) return;
+    const CamTile ct = P.cam_tiles[tile];
+    if (P.pose_var[ct.pose] < 0) return;
+    double acc[21];
+#pragma unroll
+    for (int i = 0; i < 21; ++i) acc[i] = 0.0;
+    for (int e = ct.begin + (int)threadIdx.x; e < ct.end; e += kBlock) {
+        const EdgeRec64 rc = P.ec64[e];
+        const int l = rc.idx;
+        const double* Vi = P.Vinv64 + 6 * (size_t)l;
+        const double vi[9] = {Vi[0], Vi[1], Vi[2], Vi[1], Vi[3], Vi[4], Vi[2], Vi[4], Vi[5]};
+        const double g[9] = {rc.G[0], rc.G[1], rc.G[2], rc.G[1], rc.G[3], rc.G[4], rc.G[2], rc.G[4], rc.G[5]};
+        double T[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) T[3 * i + j] = g[3 * i] * vi[j] + g[3 * i + 1] * vi[3 + j] + g[3 * i + 2] * vi[6 + j];
+        double C[6];
+        C[0] = T[0] * g[0] + T[1] * g[3] + T[2] * g[6];
+        C[1] = T[0] * g[1] + T[1] * g[4] + T[2] * g[7];
+        C[2] = T[0] * g[2] + T[1] * g[5] + T[2] * g[8];
+        C[3] = T[3] * g[1] + T[4] * g[4] + T[5] * g[7];
+        C[4] = T[3] * g[2] + T[4] * g[5] + T[5] * g[8];
+        C[5] = T[6] * g[2] + T[7] * g[5] + T[8] * g[8];
+        const LmPack<double> pk = P.pack64[l];
+        add_schur_block<double>(C, pk.X, 1.0, acc);
+    }
+    block_sum<kBlock, double, 21>(acc, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < 21; ++i) P.tile27[27 * (size_t)tile + i] = acc[i];
+    }
+}
+
+// 6x6 Cholesky inverse; false for a non-positive or non-finite pivot.
+__device__ bool cholesky_inverse6(double A[36], double out[36]) {
+    double L[36];
+#pragma unroll
+    for (int i = 0; i < 36; ++i) L[i] = 0.0;
+    for (int j = 0; j < 6; ++j) {
+        double d = A[6 * j + j];
+        for (int k = 0; k < j; ++k) d -= L[6 * j + k] * L[6 * j + k];
+        if (!(d > 0.0) || !isfinite(d)) return false;
+        const double ljj = sqrt(d);
+        L[6 * j + j] = ljj;
+        for (int i = j + 1; i < 6; ++i) {
+            double s = A[6 * i + j];
+            for (int k = 0; k < j; ++k) s -= L[6 * i + k] * L[6 * j + k];
+            L[6 * i + j] = s / ljj;
+        }
+    }
+    for (int c = 0; c < 6; ++c) {
+        double y[6];
+        for (int i = 0; i < 6; ++i) {
+            double s = (i == c) ? 1.0 : 0.0;
+            for (int k = 0; k < i; ++k) s -= L[6 * i + k] * y[k];
+            y[i] = s / L[6 * i + i];
+        }
+        for (int i = 5; i >= 0; --i) {
+            double s = y[i];
+            for (int k = i + 1; k < 6; ++k) s -= L[6 * k + i] * y[k];
+            y[i] = s / L[6 * i + i];
+        }
+        for (int i = 0; i < 6; ++i) out[6 * i + c] = y[i];
+    }
+    for (int i = 0; i < 36; ++i)
+        if (!isfinite(out[i])) return false;
+    return true;
+}
+
+__global__ void k_precondition_finalize(DevProblem P, double lambda) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P.num_pose_slots) return;
+    double* out = P.Minv + 36 * (size_t)p;
+    if (P.pose_var[p] < 0) {
+        for (int i = 0; i < 36; ++i) out[i] = 0.0;
+        return;
+    }
+    double acc[21];
+    for (int i = 0; i < 21; ++i) acc[i] = P.U[21 * (size_t)p + i];
+    for (int t = P.cam_tile_ptr[p]; t < P.cam_tile_ptr[p + 1]; ++t)
+        for (int i = 0; i < 21; ++i) acc[i] -= P.tile27[27 * (size_t)t + i];
+    double M[36];
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j) M[6 * i + j] = acc[i <= j ? up6(i, j) : up6(j, i)];
+    if (lambda > 0.0)
+        for (int k = 0; k < 6; ++k) M[7 * k] += lambda;
+    double inv[36];
+    if (!cholesky_inverse6(M, inv)) {
+        P.scalars->precond_failed = 1;
+        for (int i = 0; i < 36; ++i) out[i] = 0.0;
+        return;
+    }
+    for (int i = 0; i < 36; ++i) out[i] = inv[i];
+}
+
+// ---------------------------------------------------------------------------
+// Matrix-free Schur operator, S x = (U + lambda I) x - W V^-1 W^T x
+// (implicit_schur.rs:390-431), as a landmark pass and a camera pass.
+// ---------------------------------------------------------------------------
+template <typename T, typename Rec>
+__global__ __launch_bounds__(kBlock) void k_lm_pass(DevProblem P, const T* __restrict__ x,
+                                                    const Rec* __restrict__ el, const T* __restrict__ vinv,
+                                                    LmPack<T>* __restrict__ pack, const int32_t* run_flag) {
+    if (run_flag && *run_flag != 0) return;
+    const LmTile lt = P.lm_tiles[blockIdx.x];
+    const int l = lt.lm_begin + (int)threadIdx.x;
+    if (l >= lt.lm_end || P.lm_var[l] < 0) return;
+    const LmPack<T> pk = pack[l];
+    const T X0 = pk.X[0], X1 = pk.X[1], X2 = pk.X[2];
+    T t0 = T(0), t1 = T(0), t2 = T(0);
+    const int e1 = P.lm_edge_ptr[l + 1];
+    for (int e = P.lm_edge_ptr[l]; e < e1; ++e) {
+        const Rec r = el[e];
+        const T* xp = x + 6 * (size_t)r.idx;
+        const T v0 = xp[0] + xp[4] * X2 - xp[5] * X1;
+        const T v1 = xp[1] + xp[5] * X0 - xp[3] * X2;
+        const T v2 = xp[2] + xp[3] * X1 - xp[4] * X0;
+        t0 += (T)r.G[0] * v0 + (T)r.G[1] * v1 + (T)r.G[2] * v2;
+        t1 += (T)r.G[1] * v0 + (T)r.G[3] * v1 + (T)r.G[4] * v2;
+        t2 += (T)r.G[2] * v0 + (T)r.G[4] * v1 + (T)r.G[5] * v2;
+    }
+    const T* Vi = vinv + 6 * (size_t)l;
+    pack[l].u[0] = Vi[0] * t0 + Vi[1] * t1 + Vi[2] * t2;
+    pack[l].u[1] = Vi[1] * t0 + Vi[3] * t1 + Vi[4] * t2;
+    pack[l].u[2] = Vi[2] * t0 + Vi[4] * t1 + Vi[5] * t2;
+}
+
+template <typename T, typename Rec>
+__global__ __launch_bounds__(kBlock) void k_cam_pass(DevProblem P, const Rec* __restrict__ ec,
+                                                     const LmPack<T>* __restrict__ pack, T* __restrict__ part,
+                                                     const int32_t* run_flag) {
+    __shared__ T lds[(kBlock / 64) * 6];
+    if (run_flag && *run_flag != 0) return;
+    const int tile = xcd_tile(blockIdx.x, P.num_cam_tiles);
+    if (tile >= P.num_cam_tiles) return;
+    const CamTile ct = P.cam_tiles[tile];
+    if (P.pose_var[ct.pose] < 0) return;
+    T acc[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+    for (int e = ct.begin + (int)threadIdx.x; e < ct.end; e += kBlock) {
+        const Rec r = ec[e];
+        const LmPack<T> pk = pack[r.idx];
+        const T a0 = (T)r.G[0] * pk.u[0] + (T)r.G[1] * pk.u[1] + (T)r.G[2] * pk.u[2];
+        const T a1 = (T)r.G[1] * pk.u[0] + (T)r.G[3] * pk.u[1] + (T)r.G[4] * pk.u[2];
+        const T a2 = (T)r.G[2] * pk.u[0] + (T)r.G[4] * pk.u[1] + (T)r.G[5] * pk.u[2];
+        acc[0] += a0;
+        acc[1] += a1;
+        acc[2] += a2;
+        acc[3] += pk.X[1] * a2 - pk.X[2] * a1;
+        acc[4] += pk.X[2] * a0 - pk.X[0] * a2;
+        acc[5] += pk.X[0] * a1 - pk.X[1] * a0;
+    }
+    block_sum<kBlock, T, 6>(acc, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) part[6 * (size_t)tile + i] = acc[i];
+    }
+}
+
+// b = -g_p + sum_e W_e V^-1 g_l (implicit_schur.rs:181-231): the camera pass
+// was run with u = V^-1 g_l.
+__global__ void k_rhs_finalize(DevProblem P) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P.num_pose_slots) return;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    if (P.pose_var[p] >= 0) {
+        for (int t = P.cam_tile_ptr[p]; t < P.cam_tile_ptr[p + 1]; ++t)
+            for (int i = 0; i < 6; ++i) acc[i] += P.part64[6 * (size_t)t + i];
+        for (int i = 0; i < 6; ++i) acc[i] -= P.g[6 * (size_t)p + i];
+    }
+    for (int i = 0; i < 6; ++i) P.b[6 * (size_t)p + i] = acc[i];
+}
+
+// ---------------------------------------------------------------------------
+// PCG vector work on the reduced system (6 unknowns per pose), one
+// work-group. Control flow of solve_pcg_with_restart_hook
+// (implicit_schur.rs:522-736): the recursive residual decides convergence,
+// the host then checks the true residual.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void minv_apply(const DevProblem& P, int p, const double* r, double* z) {
+    const double* M = P.Minv + 36 * (size_t)p;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) s += M[6 * i + j] * r[j];
+        z[i] = s;
+    }
+}
+
+__global__ __launch_bounds__(kSmallBlock) void k_pcg_init(DevProblem P, double rtol, double atol, int max_iter,
+                                                          double lambda) {
+    __shared__ double lds[(kSmallBlock / 64) * 2];
+    double acc[2] = {0.0, 0.0};  // r.z, b.b
+    for (int p = threadIdx.x; p < P.num_pose_slots; p += kSmallBlock) {
+        double r[6], z[6];
+        const bool var = P.pose_var[p] >= 0;
+        for (int i = 0; i < 6; ++i) r[i] = var ? P.b[6 * (size_t)p + i] : 0.0;
+        if (var) minv_apply(P, p, r, z);
+        else for (int i = 0; i < 6; ++i) z[i] = 0.0;
+        for (int i = 0; i < 6; ++i) {
+            const size_t k = 6 * (size_t)p + i;
+            P.x[k] = 0.0;
+            P.r[k] = r[i];
+            P.z[k] = z[i];
+            P.d[k] = z[i];
+            P.d32[k] = (float)z[i];
+            P.q[k] = 0.0;
+            acc[0] += r[i] * z[i];
+            acc[1] += r[i] * r[i];
+        }
+    }
+    block_sum<kSmallBlock, double, 2>(acc, lds);
+    if (threadIdx.x == 0) {
+        PcgState s;
+        s.rho = acc[0];
+        s.rhs_norm = sqrt(acc[1]);
+        s.target = fmax(atol, rtol * s.rhs_norm);
+        s.rnorm = s.rhs_norm;
+        s.true_norm = 0.0;
+        s.lambda = lambda;
+        s.iter = 0;
+        s.max_iter = max_iter;
+        s.flags = 0;
+        if (P.scalars->precond_failed) s.status = PCG_NONPOSITIVE;  // NonSpdPreconditioner
+        else if (!isfinite(s.rhs_norm) || !isfinite(s.target)) s.status = PCG_NONFINITE;
+        else if (s.rhs_norm <= s.target) s.status = PCG_CONVERGED;
+        else if (max_iter == 0) s.status = PCG_MAXIT;
+        else if (!isfinite(s.rho) || !(s.rho > 0.0)) s.status = PCG_NONPOSITIVE;
+        else s.status = PCG_RUNNING;
+        *P.pcg = s;
+    }
+}
+
+// One PCG step after the passes left sum_e W V^-1 W^T d in the tile partials.
+template <typename T>
+__global__ __launch_bounds__(kSmallBlock) void k_pcg_update(DevProblem P, const T* __restrict__ part) {
+    __shared__ double lds[(kSmallBlock / 64) * 2];
+    PcgState* st = P.pcg;
+    if (st->status != PCG_RUNNING) return;
+    const double lambda = st->lambda;
+    double acc1[1] = {0.0};
+    for (int p = threadIdx.x; p < P.num_pose_slots; p += kSmallBlock) {
+        if (P.pose_var[p] < 0) continue;
+        double dp[6], q[6], s[6] = {0, 0, 0, 0, 0, 0};
+        for (int i = 0; i < 6; ++i) dp[i] = P.d[6 * (size_t)p + i];
+        for (int t = P.cam_tile_ptr[p]; t < P.cam_tile_ptr[p + 1]; ++t)
+            for (int i = 0; i < 6; ++i) s[i] += (double)part[6 * (size_t)t + i];
+        sym6_mul(P.U + 21 * (size_t)p, dp, q);
+        for (int i = 0; i < 6; ++i) {
+            q[i] = q[i] + lambda * dp[i] - s[i];
+            P.q[6 * (size_t)p + i] = q[i];
+            acc1[0] += dp[i] * q[i];
+        }
+    }
+    block_sum_all<kSmallBlock, double, 1>(acc1, lds);
+    const double curvature = acc1[0];
+    if (!isfinite(curvature) || !(curvature > 0.0)) {
+        if (threadIdx.x == 0) st->status = PCG_NONPOSITIVE;
+        return;
+    }
+    const double alpha = st->rho / curvature;
+    if (!isfinite(alpha)) {
+        if (threadIdx.x == 0) st->status = PCG_NONFINITE;
+        return;
+    }
+    double acc2[1] = {0.0};
+    for (int p = threadIdx.x; p < P.num_pose_slots; p += kSmallBlock) {
+        if (P.pose_var[p] < 0) continue;
+        for (int i = 0; i < 6; ++i) {
+            const size_t k = 6 * (size_t)p + i;
+            P.x[k] = P.x[k] + alpha * P.d[k];
+            const double rk = P.r[k] - alpha * P.q[k];
+            P.r[k] = rk;
+            acc2[0] += rk * rk;
+        }
+    }
+    block_sum_all<kSmallBlock, double, 1>(acc2, lds);
+    const double rn = sqrt(acc2[0]);
+    const int iter = st->iter + 1;
+    int status = PCG_RUNNING;
+    if (!isfinite(rn)) status = PCG_NONFINITE;
+    else if (rn <= st->target) status = PCG_CONVERGED;
+    else if (iter >= st->max_iter) status = PCG_MAXIT;
+    if (status != PCG_RUNNING) {
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            st->iter = iter;
+            st->rnorm = rn;
+            st->status = status;
+        }
+        return;
+    }
+    double acc3[1] = {0.0};
+    for (int p = threadIdx.x; p < P.num_pose_slots; p += kSmallBlock) {
+        if (P.pose_var[p] < 0) continue;
+        double r[6], z[6];
+        for (int i = 0; i < 6; ++i) r[i] = P.r[6 * (size_t)p + i];
+        minv_apply(P, p, r, z);
+        for (int i = 0; i < 6; ++i) {
+            P.z[6 * (size_t)p + i] = z[i];
+            acc3[0] += r[i] * z[i];
+        }
+    }
+    block_sum_all<kSmallBlock, double, 1>(acc3, lds);
+    const double rz = acc3[0];
+    if (!isfinite(rz) || !(rz > 0.0)) {
+        if (threadIdx.x == 0) {
+            st->iter = iter;
+            st->rnorm = rn;
+            st->status = PCG_NONPOSITIVE;
+        }
+        return;
+    }
+    const double beta = rz / st->rho;
+    for (int p = threadIdx.x; p < P.num_pose_slots; p += kSmallBlock) {
+        if (P.pose_var[p] < 0) continue;
+        for (int i = 0; i < 6; ++i) {
+            const size_t k = 6 * (size_t)p + i;
+            const double dk = P.z[k] + beta * P.d[k];
+            P.d[k] = dk;
+            P.d32[k] = (float)dk;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        st->iter = iter;
+        st->rnorm = rn;
+        st->rho = rz;
+    }
+}
+
+// True residual b - S x (part64 holds the passes run on x). With restart the
+// recursion continues from it: r = b - S x, d = z = M^-1 r.
+__global__ __launch_bounds__(kSmallBlock) void k_pcg_true_residual(DevProblem P, int restart) {
+    __shared__ double lds[(kSmallBlock / 64) * 2];
+    PcgState* st = P.pcg;
+    const double lambda = st->lambda;
+    double acc[2] = {0.0, 0.0};
+    for (int p = threadIdx.x; p < P.num_pose_slots; p += kSmallBlock) {
+        if (P.pose_var[p] < 0) continue;
+        double xp[6], sx[6], s[6] = {0, 0, 0, 0, 0, 0};
+        for (int i = 0; i < 6; ++i) xp[i] = P.x[6 * (size_t)p + i];
+        for (int t = P.cam_tile_ptr[p]; t < P.cam_tile_ptr[p + 1]; ++t)
+            for (int i = 0; i < 6; ++i) s[i] += P.part64[6 * (size_t)t + i];
+        sym6_mul(P.U + 21 * (size_t)p, xp, sx);
+        double rt[6];
+        for (int i = 0; i < 6; ++i) {
+            rt[i] = P.b[6 * (size_t)p + i] - (sx[i] + lambda * xp[i] - s[i]);
+            acc[0] += rt[i] * rt[i];
+        }
+        if (restart) {
+            double z[6];
+            minv_apply(P, p, rt, z);
+            for (int i = 0; i < 6; ++i) {
+                const size_t k = 6 * (size_t)p + i;
+                P.r[k] = rt[i];
+                P.z[k] = z[i];
+                P.d[k] = z[i];
+                P.d32[k] = (float)z[i];
+                acc[1] += rt[i] * z[i];
+            }
+        }
+    }
+    block_sum<kSmallBlock, double, 2>(acc, lds);
+    if (threadIdx.x == 0) {
+        st->true_norm = sqrt(acc[0]);
+        if (restart) {
+            st->rnorm = st->true_norm;
+            st->rho = acc[1];
+            if (!isfinite(acc[1]) || !(acc[1] > 0.0)) st->status = PCG_NONPOSITIVE;
+            else if (st->iter >= st->max_iter) st->status = PCG_MAXIT;
+            else st->status = PCG_RUNNING;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Back-substitution, trial state and cost.
+// ---------------------------------------------------------------------------
+// delta_l = -V^-1 (g_l + sum_e W_e^T x_p) (normal_equations.rs:1551-1592,
+// implicit_schur.rs:789-824); trial landmark X + delta_l.
+__global__ __launch_bounds__(kBlock) void k_backsub(DevProblem P) {
+    __shared__ double lds[kBlock / 64];
+    const LmTile lt = P.lm_tiles[blockIdx.x];
+    const int l = lt.lm_begin + (int)threadIdx.x;
+    double step = 0.0;
+    if (l < lt.lm_end) {
+        const double X[3] = {P.lms[3 * (size_t)l], P.lms[3 * (size_t)l + 1], P.lms[3 * (size_t)l + 2]};
+        double dl[3] = {0.0, 0.0, 0.0};
+        if (P.lm_var[l] >= 0) {
+            double t[3] = {P.gl[3 * (size_t)l], P.gl[3 * (size_t)l + 1], P.gl[3 * (size_t)l + 2]};
+            for (int e = P.lm_edge_ptr[l]; e < P.lm_edge_ptr[l + 1]; ++e) {
+                const EdgeRec64 r = P.el64[e];
+                const double* xp = P.x + 6 * (size_t)r.idx;
+                const double v[3] = {xp[0] + xp[4] * X[2] - xp[5] * X[1], xp[1] + xp[5] * X[0] - xp[3] * X[2],
+                                     xp[2] + xp[3] * X[1] - xp[4] * X[0]};
+                double gv[3];
+                sym3_mul(r.G, v, gv);
+                t[0] += gv[0];
+                t[1] += gv[1];
+                t[2] += gv[2];
+            }
+            sym3_mul(P.Vinv64 + 6 * (size_t)l, t, dl);
+            dl[0] = -dl[0];
+            dl[1] = -dl[1];
+            dl[2] = -dl[2];
+            step = sqrt(dl[0] * dl[0] + dl[1] * dl[1] + dl[2] * dl[2]);
+        }
+        for (int i = 0; i < 3; ++i) {
+            P.delta_l[3 * (size_t)l + i] = dl[i];
+            P.lms_trial[3 * (size_t)l + i] = X[i] + dl[i];
+        }
+    }
+    step = block_max<kBlock>(step, lds);
+    if (threadIdx.x == 0) P.tile_scalar[4 * (size_t)(P.num_cam_tiles + blockIdx.x) + 3] = step;
+}
+
+// Trial pose T * Exp(x_p) (lm.rs:672-690) and the pose step norm.
+__global__ __launch_bounds__(kBlock) void k_pose_update(DevProblem P) {
+    __shared__ double lds[kBlock / 64];
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    double step = 0.0;
+    if (p < P.num_pose_slots) {
+        const double* s = P.poses + 7 * (size_t)p;
+        double* o = P.poses_trial + 7 * (size_t)p;
+        if (P.pose_var[p] >= 0) {
+            const double* xi = P.x + 6 * (size_t)p;
+            double ss = 0.0;
+            for (int i = 0; i < 6; ++i) ss += xi[i] * xi[i];
+            step = sqrt(ss);
+            Pose T{{s[0], s[1], s[2], s[3]}, {s[4], s[5], s[6]}};
+            Pose n = pose_compose(T, se3_exp(xi));
+            o[0] = n.q.w; o[1] = n.q.x; o[2] = n.q.y; o[3] = n.q.z;
+            o[4] = n.t.x; o[5] = n.t.y; o[6] = n.t.z;
+        } else {
+            for (int i = 0; i < 7; ++i) o[i] = s[i];
+        }
+    }
+    step = block_max<kBlock>(step, lds);
+    if (threadIdx.x == 0)
+        P.tile_scalar[4 * (size_t)(P.num_cam_tiles + P.num_lm_tiles + blockIdx.x) + 3] = step;
+}
+
+// Robust cost and non-projectable count (problem.rs:252-303, 497-583) with a
+// compensated FP64 sum per tile.
+template <bool RIG>
+__global__ __launch_bounds__(kBlock) void k_cost(DevProblem P, int trial) {
+    __shared__ double lds[(kBlock / 64) * 3];
+    const int tile = xcd_tile(blockIdx.x, P.num_cam_tiles);
+    if (tile >= P.num_cam_tiles) return;
+    const CamTile ct = P.cam_tiles[tile];
+    const double* poses = trial ? P.poses_trial : P.poses;
+    const double* lms = trial ? P.lms_trial : P.lms;
+    const double* pq = poses + 7 * (size_t)ct.pose;
+    double R[9];
+    qmatrix<double>(Quat{pq[0], pq[1], pq[2], pq[3]}, R);
+    CSum cs{0.0, 0.0};
+    double bad = 0.0;
+    for (int e = ct.begin + (int)threadIdx.x; e < ct.end; e += kBlock) {
+        const int l = P.edge_lm[e];
+        const double X[3] = {lms[3 * (size_t)l], lms[3 * (size_t)l + 1], lms[3 * (size_t)l + 2]};
+        const double xr[3] = {R[0] * X[0] + R[1] * X[1] + R[2] * X[2] + pq[4],
+                              R[3] * X[0] + R[4] * X[1] + R[5] * X[2] + pq[5],
+                              R[6] * X[0] + R[7] * X[1] + R[8] * X[2] + pq[6]};
+        for (int o = P.edge_obs[e]; o < P.edge_obs[e + 1]; ++o) {
+            const DevSensor& S = P.sensors[RIG ? P.obs_sensor[o] : 0];
+            double xs[3] = {xr[0], xr[1], xr[2]};
+            if (RIG && !S.identity) {
+                for (int i = 0; i < 3; ++i)
+                    xs[i] = S.R[3 * i] * xr[0] + S.R[3 * i + 1] * xr[1] + S.R[3 * i + 2] * xr[2] + S.t[i];
+            }
+            double uv[2];
+            if (!project<double>(S.lens, xs[0], xs[1], xs[2], uv, (double*)nullptr)) {
+                bad += 1.0;
+                continue;
+            }
+            const double r0 = uv[0] - P.obs_uv[2 * (size_t)o], r1 = uv[1] - P.obs_uv[2 * (size_t)o + 1];
+            csum_add(cs, robust_cost<double>((uint32_t)P.kernel_kind, P.kernel_scale, r0 * r0 + r1 * r1));
+        }
+    }
+    // compensated tree: wave shuffles of (s, c) pairs, then LDS in fixed order
+    for (int off = 32; off > 0; off >>= 1) {
+        CSum o{__shfl_down(cs.s, off, 64), __shfl_down(cs.c, off, 64)};
+        bad += __shfl_down(bad, off, 64);
+        if ((threadIdx.x & 63) + off < 64) cs = csum_merge(cs, o);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        const int w = threadIdx.x >> 6;
+        lds[3 * w] = cs.s;
+        lds[3 * w + 1] = cs.c;
+        lds[3 * w + 2] = bad;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        CSum a{lds[0], lds[1]};
+        double b = lds[2];
+        for (int w = 1; w < kBlock / 64; ++w) {
+            a = csum_merge(a, CSum{lds[3 * w], lds[3 * w + 1]});
+            b += lds[3 * w + 2];
+        }
+        double* out = P.tile_scalar + 4 * (size_t)tile;
+        out[0] = a.s;
+        out[1] = a.c;
+        out[2] = b;
+    }
+}
+
+__global__ __launch_bounds__(kSmallBlock) void k_scalars_finalize(DevProblem P, int num_pose_blocks) {
+    __shared__ double lds[(kSmallBlock / 64) * 4];
+    // cost partials in tile order: each thread merges a contiguous run
+    const int n = P.num_cam_tiles;
+    const int per = (n + kSmallBlock - 1) / kSmallBlock;
+    CSum cs{0.0, 0.0};
+    double bad = 0.0, mp = 0.0, ml = 0.0;
+    for (int i = threadIdx.x * per; i < min(n, (int)(threadIdx.x + 1) * per); ++i) {
+        const double* t = P.tile_scalar + 4 * (size_t)i;
+        cs = csum_merge(cs, CSum{t[0], t[1]});
+        bad += t[2];
+    }
+    for (int i = threadIdx.x; i < P.num_lm_tiles; i += kSmallBlock)
+        ml = fmax(ml, P.tile_scalar[4 * (size_t)(n + i) + 3]);
+    for (int i = threadIdx.x; i < num_pose_blocks; i += kSmallBlock)
+        mp = fmax(mp, P.tile_scalar[4 * (size_t)(n + P.num_lm_tiles + i) + 3]);
+    // ordered merge across threads through LDS (thread order = tile order)
+    __shared__ double ss[kSmallBlock], sc[kSmallBlock];
+    ss[threadIdx.x] = cs.s;
+    sc[threadIdx.x] = cs.c;
+    double v[1] = {bad};
+    block_sum<kSmallBlock, double, 1>(v, lds);
+    const double m1 = block_max<kSmallBlock>(mp, lds + 16);
+    const double m2 = block_max<kSmallBlock>(ml, lds + 32);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        CSum a{ss[0], sc[0]};
+        for (int i = 1; i < kSmallBlock; ++i) a = csum_merge(a, CSum{ss[i], sc[i]});
+        StepScalars* o = P.scalars;
+        o->cost_sum = a.s;
+        o->cost_comp = a.c;
+        o->nonprojectable = (uint64_t)v[0];
+        o->max_pose_step = m1;
+        o->max_landmark_step = m2;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Dense reduced camera system (LinearSolver::Dense): explicit S and a
+// one-work-group Cholesky with an LU fallback (pose_graph.rs:2259-2270).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_dense_assemble(DevProblem P, DensePairs D, double* S, int n,
+                                                           double lambda) {
+    __shared__ double lds[(kBlock / 64) * 36];
+    const int b = blockIdx.x;
+    const int sp = D.block_pq[2 * b], sq = D.block_pq[2 * b + 1];
+    const int vp = P.pose_var[sp], vq = P.pose_var[sq];
+    double acc[36];
+#pragma unroll
+    for (int i = 0; i < 36; ++i) acc[i] = 0.0;
+    for (int k = D.block_ptr[b] + (int)threadIdx.x; k < D.block_ptr[b + 1]; k += kBlock) {
+        const int l = D.terms[3 * (size_t)k], ep = D.terms[3 * (size_t)k + 1], eq = D.terms[3 * (size_t)k + 2];
+        const double* Vi = P.Vinv64 + 6 * (size_t)l;
+        const double vi[9] = {Vi[0], Vi[1], Vi[2], Vi[1], Vi[3], Vi[4], Vi[2], Vi[4], Vi[5]};
+        const double* A = P.el64[ep].G;
+        const double* B = P.el64[eq].G;
+        const double a[9] = {A[0], A[1], A[2], A[1], A[3], A[4], A[2], A[4], A[5]};
+        const double bb[9] = {B[0], B[1], B[2], B[1], B[3], B[4], B[2], B[4], B[5]};
+        double T[9], C[9];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) T[3 * i + j] = a[3 * i] * vi[j] + a[3 * i + 1] * vi[3 + j] + a[3 * i + 2] * vi[6 + j];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) C[3 * i + j] = T[3 * i] * bb[j] + T[3 * i + 1] * bb[3 + j] + T[3 * i + 2] * bb[6 + j];
+        const double* X = P.pack64[l].X;
+        const double K[9] = {0.0, -X[2], X[1], X[2], 0.0, -X[0], -X[1], X[0], 0.0};
+        double CK[9], KC[9], KCK[9];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                CK[3 * i + j] = C[3 * i] * K[j] + C[3 * i + 1] * K[3 + j] + C[3 * i + 2] * K[6 + j];
+                KC[3 * i + j] = K[3 * i] * C[j] + K[3 * i + 1] * C[3 + j] + K[3 * i + 2] * C[6 + j];
+            }
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) KCK[3 * i + j] = KC[3 * i] * K[j] + KC[3 * i + 1] * K[3 + j] + KC[3 * i + 2] * K[6 + j];
+        // [[C, -C K], [K C, -K C K]]
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                acc[6 * i + j] += C[3 * i + j];
+                acc[6 * i + 3 + j] -= CK[3 * i + j];
+                acc[6 * (3 + i) + j] += KC[3 * i + j];
+                acc[6 * (3 + i) + 3 + j] -= KCK[3 * i + j];
+            }
+    }
+    block_sum<kBlock, double, 36>(acc, lds);
+    if (threadIdx.x == 0) {
+        if (sp == sq) {
+            const double* U = P.U + 21 * (size_t)sp;
+            for (int i = 0; i < 6; ++i)
+                for (int j = 0; j < 6; ++j) {
+                    double u = U[i <= j ? up6(i, j) : up6(j, i)] + (i == j && lambda > 0.0 ? lambda : 0.0);
+                    S[(size_t)(6 * vp + i) * n + 6 * vq + j] = u - acc[6 * i + j];
+                }
+        } else {
+            for (int i = 0; i < 6; ++i)
+                for (int j = 0; j < 6; ++j) {
+                    S[(size_t)(6 * vp + i) * n + 6 * vq + j] = -acc[6 * i + j];
+                    S[(size_t)(6 * vq + j) * n + 6 * vp + i] = -acc[6 * i + j];
+                }
+        }
+    }
+}
+
+// Cholesky (left-looking, lower triangle in place), then forward/back
+// substitution; on a non-positive pivot an LU with partial pivoting on the
+// saved copy. One work-group.
+__global__ __launch_bounds__(kSmallBlock) void k_dense_solve(DevProblem P, DensePairs D, double* S, double* work,
+                                                             int n, int* status) {
+    __shared__ int fail;
+    __shared__ double piv_val[kSmallBlock];
+    __shared__ int piv_idx[kSmallBlock];
+    double* A2 = work;                    // n*n copy for the LU fallback
+    double* y = work + (size_t)n * n;     // n
+    for (size_t k = threadIdx.x; k < (size_t)n * n; k += kSmallBlock) A2[k] = S[k];
+    for (int i = threadIdx.x; i < n; i += kSmallBlock) y[i] = P.b[6 * (size_t)D.var_slot[i / 6] + (i % 6)];
+    if (threadIdx.x == 0) fail = 0;
+    __syncthreads();
+    for (int j = 0; j < n; ++j) {
+        for (int i = j + threadIdx.x; i < n; i += kSmallBlock) {
+            double s = S[(size_t)i * n + j];
+            const double* Li = S + (size_t)i * n;
+            const double* Lj = S + (size_t)j * n;
+            for (int k = 0; k < j; ++k) s -= Li[k] * Lj[k];
+            S[(size_t)i * n + j] = s;
+        }
+        __syncthreads();
+        const double d = S[(size_t)j * n + j];
+        if (!(d > 0.0) || !isfinite(d)) {
+            if (threadIdx.x == 0) fail = 1;
+            break;
+        }
+        const double ljj = sqrt(d);
+        for (int i = j + 1 + threadIdx.x; i < n; i += kSmallBlock) S[(size_t)i * n + j] /= ljj;
+        __syncthreads();
+        if (threadIdx.x == 0) S[(size_t)j * n + j] = ljj;
+        __syncthreads();
+    }
+    __syncthreads();
+    if (!fail) {
+        for (int j = 0; j < n; ++j) {  // L y = b
+            if (threadIdx.x == 0) y[j] = y[j] / S[(size_t)j * n + j];
+            __syncthreads();
+            const double yj = y[j];
+            for (int i = j + 1 + threadIdx.x; i < n; i += kSmallBlock) y[i] -= S[(size_t)i * n + j] * yj;
+            __syncthreads();
+        }
+        for (int j = n - 1; j >= 0; --j) {  // L^T x = y
+            if (threadIdx.x == 0) y[j] = y[j] / S[(size_t)j * n + j];
+            __syncthreads();
+            const double xj = y[j];
+            for (int i = threadIdx.x; i < j; i += kSmallBlock) y[i] -= S[(size_t)j * n + i] * xj;
+            __syncthreads();
+        }
+    } else {
+        // LU with partial pivoting on A2; y still holds b.
+        for (int i = threadIdx.x; i < n; i += kSmallBlock) y[i] = P.b[6 * (size_t)D.var_slot[i / 6] + (i % 6)];
+        __syncthreads();
+        for (int k = 0; k < n; ++k) {
+            double best = -1.0;
+            int bi = k;
+            for (int i = k + threadIdx.x; i < n; i += kSmallBlock) {
+                const double v = fabs(A2[(size_t)i * n + k]);
+                if (v > best) {
+                    best = v;
+                    bi = i;
+                }
+            }
+            piv_val[threadIdx.x] = best;
+            piv_idx[threadIdx.x] = bi;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                double bv = piv_val[0];
+                int bidx = piv_idx[0];
+                for (int t = 1; t < kSmallBlock; ++t)
+                    if (piv_val[t] > bv || (piv_val[t] == bv && piv_idx[t] < bidx)) {
+                        bv = piv_val[t];
+                        bidx = piv_idx[t];
+                    }
+                piv_idx[0] = bidx;
+                piv_val[0] = bv;
+            }
+            __syncthreads();
+            const int pr = piv_idx[0];
+            const double pv = A2[(size_t)pr * n + k];
+            __syncthreads();
+            if (pv == 0.0) continue;
+            if (pr != k) {
+                for (int c = threadIdx.x; c < n; c += kSmallBlock) {
+                    const double t = A2[(size_t)k * n + c];
+                    A2[(size_t)k * n + c] = A2[(size_t)pr * n + c];
+                    A2[(size_t)pr * n + c] = t;
+                }
+                if (threadIdx.x == 0) {
+                    const double t = y[k];
+                    y[k] = y[pr];
+                    y[pr] = t;
+                }
+            }
+            __syncthreads();
+            const double inv = 1.0 / A2[(size_t)k * n + k];
+            for (int i = k + 1 + threadIdx.x; i < n; i += kSmallBlock) A2[(size_t)i * n + k] *= inv;
+            __syncthreads();
+            const int rows = n - k - 1;
+            for (int idx = threadIdx.x; idx < rows * rows; idx += kSmallBlock) {
+                const int i = k + 1 + idx / rows, c = k + 1 + idx % rows;
+                A2[(size_t)i * n + c] -= A2[(size_t)i * n + k] * A2[(size_t)k * n + c];
+            }
+            if (threadIdx.x == 0) {
+                const double yk = y[k];
+                for (int i = k + 1; i < n; ++i) y[i] -= A2[(size_t)i * n + k] * yk;
+            }
+            __syncthreads();
+        }
+        for (int j = n - 1; j >= 0; --j) {
+            const double dj = A2[(size_t)j * n + j];
+            if (dj == 0.0) {
+                if (threadIdx.x == 0) fail = 2;
+                break;
+            }
+            if (threadIdx.x == 0) y[j] = y[j] / dj;
+            __syncthreads();
+            const double xj = y[j];
+            for (int i = threadIdx.x; i < j; i += kSmallBlock) y[i] -= A2[(size_t)i * n + j] * xj;
+            __syncthreads();
+        }
+    }
+    __syncthreads();
+    const int f = fail;
+    for (int p = threadIdx.x; p < P.num_pose_slots; p += kSmallBlock)
+        for (int i = 0; i < 6; ++i) P.x[6 * (size_t)p + i] = 0.0;
+    __syncthreads();
+    if (f != 2)
+        for (int i = threadIdx.x; i < n; i += kSmallBlock) P.x[6 * (size_t)D.var_slot[i / 6] + (i % 6)] = y[i];
+    if (threadIdx.x == 0) *status = (f == 2) ? 1 : 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+static inline int blocks_for(int n, int per) { return (n + per - 1) / per; }
+
+void launch_linearize(const DevProblem& P, bool store32, hipStream_t s) {
+    if (P.num_cam_tiles == 0) return;
+    if (P.any_rig)
+        hipLaunchKernelGGL(k_linearize<true>, dim3(xcd_grid(P.num_cam_tiles)), dim3(kBlock), 0, s, P, (int)store32);
+    else
+        hipLaunchKernelGGL(k_linearize<false>, dim3(xcd_grid(P.num_cam_tiles)), dim3(kBlock), 0, s, P, (int)store32);
+}
+void launch_camera_finalize(const DevProblem& P, hipStream_t s) {
+    hipLaunchKernelGGL(k_camera_finalize, dim3(blocks_for(P.num_pose_slots, 256)), dim3(256), 0, s, P);
+}
+void launch_landmark_reduce(const DevProblem& P, hipStream_t s) {
+    hipLaunchKernelGGL(k_landmark_reduce, dim3(blocks_for(P.num_lm_slots, 256)), dim3(256), 0, s, P);
+}
+void launch_landmark_prepare(const DevProblem& P, double lambda, hipStream_t s) {
+    hipLaunchKernelGGL(k_landmark_prepare, dim3(blocks_for(P.num_lm_slots, 256)), dim3(256), 0, s, P, lambda);
+}
+void launch_precondition(const DevProblem& P, double lambda, hipStream_t s) {
+    if (P.num_cam_tiles)
+        hipLaunchKernelGGL(k_precondition_tiles, dim3(xcd_grid(P.num_cam_tiles)), dim3(kBlock), 0, s, P);
+    hipLaunchKernelGGL(k_precondition_finalize, dim3(blocks_for(P.num_pose_slots, 128)), dim3(128), 0, s, P, lambda);
+}
+void launch_lm_pass(const DevProblem& P, bool fp32, const void* x, const int32_t* run_flag, hipStream_t s) {
+    if (P.num_lm_tiles == 0) return;
+    if (fp32)
+        hipLaunchKernelGGL((k_lm_pass<float, EdgeRec32>), dim3(P.num_lm_tiles), dim3(kBlock), 0, s, P,
+                           (const float*)x, (const EdgeRec32*)P.el32, (const float*)P.Vinv32, P.pack32, run_flag);
+    else
+        hipLaunchKernelGGL((k_lm_pass<double, EdgeRec64>), dim3(P.num_lm_tiles), dim3(kBlock), 0, s, P,
+                           (const double*)x, (const EdgeRec64*)P.el64, (const double*)P.Vinv64, P.pack64, run_flag);
+}
+void launch_cam_pass(const DevProblem& P, bool fp32, const int32_t* run_flag, hipStream_t s) {
+    if (P.num_cam_tiles == 0) return;
+    if (fp32)
+        hipLaunchKernelGGL((k_cam_pass<float, EdgeRec32>), dim3(xcd_grid(P.num_cam_tiles)), dim3(kBlock), 0, s, P,
+                           (const EdgeRec32*)P.ec32, (const LmPack<float>*)P.pack32, P.part32, run_flag);
+    else
+        hipLaunchKernelGGL((k_cam_pass<double, EdgeRec64>), dim3(xcd_grid(P.num_cam_tiles)), dim3(kBlock), 0, s, P,
+                           (const EdgeRec64*)P.ec64, (const LmPack<double>*)P.pack64, P.part64, run_flag);
+}
+void launch_rhs_finalize(const DevProblem& P, hipStream_t s) {
+    hipLaunchKernelGGL(k_rhs_finalize, dim3(blocks_for(P.num_pose_slots, 256)), dim3(256), 0, s, P);
+}
+void launch_pcg_init(const DevProblem& P, double rtol, double atol, int32_t max_iter, double lambda,
+                     hipStream_t s) {
+    hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(kSmallBlock), 0, s, P, rtol, atol, max_iter, lambda);
+}
+void launch_pcg_update(const DevProblem& P, bool fp32, hipStream_t s) {
+    if (fp32)
+        hipLaunchKernelGGL(k_pcg_update<float>, dim3(1), dim3(kSmallBlock), 0, s, P, (const float*)P.part32);
+    else
+        hipLaunchKernelGGL(k_pcg_update<double>, dim3(1), dim3(kSmallBlock), 0, s, P, (const double*)P.part64);
+}
+void launch_pcg_true_residual(const DevProblem& P, bool restart, hipStream_t s) {
+    hipLaunchKernelGGL(k_pcg_true_residual, dim3(1), dim3(kSmallBlock), 0, s, P, (int)restart);
+}
+void launch_backsub(const DevProblem& P, hipStream_t s) {
+    if (P.num_lm_tiles) hipLaunchKernelGGL(k_backsub, dim3(P.num_lm_tiles), dim3(kBlock), 0, s, P);
+}
+void launch_pose_update(const DevProblem& P, hipStream_t s) {
+    hipLaunchKernelGGL(k_pose_update, dim3(blocks_for(P.num_pose_slots, kBlock)), dim3(kBlock), 0, s, P);
+}
+void launch_cost(const DevProblem& P, bool trial, hipStream_t s) {
+    if (P.num_cam_tiles == 0) return;
+    if (P.any_rig)
+        hipLaunchKernelGGL(k_cost<true>, dim3(xcd_grid(P.num_cam_tiles)), dim3(kBlock), 0, s, P, (int)trial);
+    else
+        hipLaunchKernelGGL(k_cost<false>, dim3(xcd_grid(P.num_cam_tiles)), dim3(kBlock), 0, s, P, (int)trial);
+}
+void launch_scalars_finalize(const DevProblem& P, hipStream_t s) {
+    hipLaunchKernelGGL(k_scalars_finalize, dim3(1), dim3(kSmallBlock), 0, s, P, blocks_for(P.num_pose_slots, kBlock));
+}
+void launch_dense_assemble(const DevProblem& P, const DensePairs& D, double* S, int32_t n, double lambda,
+                           hipStream_t s) {
+    if (D.num_blocks) hipLaunchKernelGGL(k_dense_assemble, dim3(D.num_blocks), dim3(kBlock), 0, s, P, D, S, n, lambda);
+}
+void launch_dense_solve(const DevProblem& P, const DensePairs& D, double* S, double* work, int32_t n,
+                        int32_t* status, hipStream_t s) {
+    hipLaunchKernelGGL(k_dense_solve, dim3(1), dim3(kSmallBlock), 0, s, P, D, S, work, n, status);
+}
+
+}  // namespace vba

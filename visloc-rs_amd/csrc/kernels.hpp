@@ -1,0 +1,165 @@
+// kernels.hpp — device data layout and kernel launchers (kernels.hip).
+//
+// Per-edge data. For an observation of landmark X from pose T with IRLS
+// weight w the reference builds J_pose = J_pi [R | -R [X]x] and
+// J_lm = J_pi R (pipelines/slam/src/bundle/normal_equations.rs:440-452; rig
+// form sqrt_qr.rs:989-999), so J_pose = J_lm [I | -[X]x]. Every block the
+// Schur path needs then follows from the symmetric 3x3 G = w J_lm^T J_lm,
+// the 3-vector h = w J_lm^T r and X:
+//   U_p  += [[G, -G[X]x], [[X]x G, -[X]x G [X]x]]     g_p += [h; X x h]
+//   V_l  += G                                          g_l += h
+//   W^T x_p = G (rho + omega x X)                      W u = [G u; X x (G u)]
+// An edge is a (pose, landmark) pair; observations sharing it (a rig's
+// sensors) are summed into one G, h, which is the grouping the reference's
+// block-Jacobi preconditioner applies (implicit_schur.rs:241-258).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "geom.hpp"
+#include "problem.hpp"
+
+namespace vba {
+
+constexpr int kBlock = 256;     // threads per tile work-group
+constexpr int kSmallBlock = 1024;  // single work-group vector kernels
+
+template <typename T>
+struct LmPack {  // per landmark slot: position and the current landmark vector
+    T X[3];
+    T pad0;
+    T u[3];
+    T pad1;
+};
+
+// One edge as streamed by the Schur kernels: the upper triangle of G
+// (g00 g01 g02 g11 g12 g22) and the index of the other end of the edge (the
+// landmark slot in c-order records, the pose slot in l-order records),
+// padded to one 32 B (FP32) or 64 B (FP64) sector.
+struct alignas(32) EdgeRec32 {
+    float G[6];
+    int32_t idx;
+    int32_t pad;
+};
+struct alignas(64) EdgeRec64 {
+    double G[6];
+    int32_t idx;
+    int32_t pad[3];
+};
+
+// Scalars exchanged with the host once per LM iteration.
+struct StepScalars {
+    double cost_sum, cost_comp;
+    double max_pose_step, max_landmark_step;
+    uint64_t nonprojectable;
+    int32_t precond_failed;
+    int32_t pad;
+};
+
+// PCG state kept on the device (implicit_schur.rs:522-736 control flow).
+struct PcgState {
+    double rho, rhs_norm, target, rnorm, true_norm, lambda;
+    int32_t iter, status, max_iter, flags;
+};
+enum PcgStatus : int32_t {
+    PCG_RUNNING = 0,
+    PCG_CONVERGED = 1,   // recursive residual <= target
+    PCG_MAXIT = 2,       // iteration budget reached above target
+    PCG_NONPOSITIVE = -1,
+    PCG_NONFINITE = -2
+};
+
+// All device pointers of one problem.
+struct DevProblem {
+    int32_t num_pose_slots, num_lm_slots, num_edges, num_obs;
+    int32_t num_cam_tiles, num_lm_tiles, any_rig, kernel_kind;
+    double kernel_scale;
+    // state
+    double* poses;        // 7 per slot
+    double* lms;          // 3 per slot
+    double* poses_trial;
+    double* lms_trial;
+    const DevSensor* sensors;
+    const int32_t* pose_var;
+    const int32_t* lm_var;
+    // structure
+    const double* obs_uv;
+    const uint16_t* obs_sensor;
+    const int32_t* edge_lm;
+    const int32_t* edge_obs;
+    const int32_t* perm_cl;
+    const int32_t* l_pose;
+    const int32_t* lm_edge_ptr;
+    const CamTile* cam_tiles;
+    const int32_t* cam_tile_ptr;
+    const LmTile* lm_tiles;
+    // linearization
+    EdgeRec64* ec64;  // c-order edges (idx = landmark slot)
+    EdgeRec64* el64;  // l-order edges (idx = pose slot)
+    double* hl64;     // 3 per edge, l-order
+    EdgeRec32* ec32;  // mixed precision copies
+    EdgeRec32* el32;
+    double* tile27;  // per camera tile partials (27)
+    double* U;       // 21 per pose slot (upper 6x6, row-major)
+    double* g;       // 6 per pose slot
+    double* V;       // 6 per landmark slot (upper 3x3)
+    double* gl;      // 3 per landmark slot
+    double* Vinv64;  // 6 per landmark slot
+    float* Vinv32;
+    LmPack<double>* pack64;
+    LmPack<float>* pack32;
+    double* Minv;    // 36 per pose slot (block-Jacobi inverse)
+    // PCG vectors, 6 per pose slot
+    double *b, *x, *r, *z, *d, *q;
+    float* d32;
+    double* part64;  // per camera tile partials (6)
+    float* part32;
+    PcgState* pcg;
+    // step outputs
+    double* delta_l;       // 3 per landmark slot
+    double* tile_scalar;   // per tile partial scalars (cost sum/comp, count, max)
+    StepScalars* scalars;
+};
+
+// ---- launchers (all asynchronous on `stream`) ----
+void launch_linearize(const DevProblem& P, bool store32, hipStream_t stream);
+void launch_camera_finalize(const DevProblem& P, hipStream_t stream);
+void launch_landmark_reduce(const DevProblem& P, hipStream_t stream);
+void launch_landmark_prepare(const DevProblem& P, double lambda, hipStream_t stream);
+void launch_precondition(const DevProblem& P, double lambda, hipStream_t stream);
+// Landmark pass u_l = V_l^-1 sum_e W_e^T x_p(e) into pack.u. x is 6 per pose slot.
+void launch_lm_pass(const DevProblem& P, bool fp32, const void* x, const int32_t* run_flag,
+                    hipStream_t stream);
+// Camera pass partial_tile = sum_e W_e u_l(e) over each camera tile.
+void launch_cam_pass(const DevProblem& P, bool fp32, const int32_t* run_flag, hipStream_t stream);
+void launch_rhs_finalize(const DevProblem& P, hipStream_t stream);
+void launch_pcg_init(const DevProblem& P, double rtol, double atol, int32_t max_iter, double lambda,
+                     hipStream_t stream);
+void launch_pcg_update(const DevProblem& P, bool fp32, hipStream_t stream);
+// True residual b - S x from part64 (after passes on x); restart = reset the
+// recursion from it.
+void launch_pcg_true_residual(const DevProblem& P, bool restart, hipStream_t stream);
+void launch_backsub(const DevProblem& P, hipStream_t stream);
+void launch_pose_update(const DevProblem& P, hipStream_t stream);
+// Cost and non-projectable count of (poses, lms) or of the trial state.
+void launch_cost(const DevProblem& P, bool trial, hipStream_t stream);
+void launch_scalars_finalize(const DevProblem& P, hipStream_t stream);
+
+// Dense reduced system (LinearSolver::Dense).
+struct DensePairs {
+    int32_t num_blocks;
+    const int32_t* block_pq;   // 2 per block: pose slots (var(p) <= var(q))
+    const int32_t* block_ptr;  // num_blocks + 1 term ranges
+    const int32_t* terms;      // 3 per term: landmark slot, l-order edge of p, of q
+    const int32_t* var_slot;   // pose slot of each variable pose
+};
+void launch_dense_assemble(const DevProblem& P, const DensePairs& D, double* S, int32_t n,
+                           double lambda, hipStream_t stream);
+// Solve S x = b (Cholesky, LU fallback) for the variable poses; writes x and
+// sets *status (0 ok, 1 singular). S is overwritten.
+void launch_dense_solve(const DevProblem& P, const DensePairs& D, double* S, double* work,
+                        int32_t n, int32_t* status, hipStream_t stream);
+
+}  // namespace vba

@@ -1,0 +1,357 @@
+// problem.cpp — validation and structure-of-arrays layout (problem.hpp).
+#include "problem.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+namespace vba {
+
+namespace {
+
+bool supported_model(int32_t m) { return m >= VBA_CAMERA_PINHOLE && m <= VBA_CAMERA_OPENCV; }
+
+// Camera::intrinsics (crates/core/src/types/camera.rs:241-271).
+bool camera_intrinsics(const vba_camera& c, double* fx, double* fy, double* cx, double* cy) {
+    switch (c.model) {
+        case VBA_CAMERA_PINHOLE:
+        case VBA_CAMERA_OPENCV:
+            if (c.num_params < 4) return false;
+            *fx = c.params[0]; *fy = c.params[1]; *cx = c.params[2]; *cy = c.params[3];
+            return true;
+        case VBA_CAMERA_SIMPLE_PINHOLE:
+        case VBA_CAMERA_SIMPLE_RADIAL:
+        case VBA_CAMERA_RADIAL:
+            if (c.num_params < 3) return false;
+            *fx = *fy = c.params[0]; *cx = c.params[1]; *cy = c.params[2];
+            return true;
+        default:
+            return false;
+    }
+}
+
+void quat_to_matrix(const double* q, double R[9]) {
+    Quat u{q[0], q[1], q[2], q[3]};
+    qmatrix<double>(u, R);
+}
+
+// Stable argsort of ids; false on a duplicate id.
+bool slot_order(uint64_t n, const uint64_t* ids, std::vector<uint32_t>* slot_of_input,
+                std::vector<uint64_t>* sorted) {
+    std::vector<uint32_t> order(n);
+    std::iota(order.begin(), order.end(), 0u);
+    std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return ids[a] < ids[b]; });
+    slot_of_input->assign(n, 0);
+    sorted->resize(n);
+    for (uint64_t s = 0; s < n; ++s) {
+        (*slot_of_input)[order[s]] = (uint32_t)s;
+        (*sorted)[s] = ids[order[s]];
+        if (s > 0 && (*sorted)[s] == (*sorted)[s - 1]) return false;
+    }
+    return true;
+}
+bool lookup(const std::vector<uint64_t>& sorted, uint64_t id, uint32_t* slot) {
+    auto it = std::lower_bound(sorted.begin(), sorted.end(), id);
+    if (it == sorted.end() || *it != id) return false;
+    *slot = (uint32_t)(it - sorted.begin());
+    return true;
+}
+
+uint32_t next_pow2(uint64_t v) {
+    uint32_t p = 1;
+    while (p < v && p < (1u << 30)) p <<= 1;
+    return p;
+}
+
+}  // namespace
+
+bool lens_from_camera(const vba_camera& c, Lens<double>* L) {
+    double fx, fy, cx, cy;
+    if (!supported_model(c.model) || !camera_intrinsics(c, &fx, &fy, &cx, &cy)) return false;
+    *L = Lens<double>{LENS_PINHOLE, 0, fx, fy, cx, cy, 0.0, 0.0, 0.0, 0.0};
+    // Camera::radial_distortion / tangential_distortion (camera.rs:213-242).
+    double k1 = 0.0, k2 = 0.0, p1 = 0.0, p2 = 0.0;
+    switch (c.model) {
+        case VBA_CAMERA_PINHOLE:
+        case VBA_CAMERA_OPENCV:
+            if (c.num_params > 4) k1 = c.params[4];
+            if (c.num_params > 5) k2 = c.params[5];
+            if (c.model == VBA_CAMERA_OPENCV) {
+                if (c.num_params > 6) p1 = c.params[6];
+                if (c.num_params > 7) p2 = c.params[7];
+            }
+            break;
+        case VBA_CAMERA_SIMPLE_RADIAL:
+            if (c.num_params > 3) k1 = c.params[3];
+            break;
+        case VBA_CAMERA_RADIAL:
+            if (c.num_params > 3) k1 = c.params[3];
+            if (c.num_params > 4) k2 = c.params[4];
+            break;
+        default:
+            break;
+    }
+    L->k1 = k1;
+    L->k2 = k2;
+    L->p1 = p1;
+    L->p2 = p2;
+    if (p1 != 0.0 || p2 != 0.0) L->kind = LENS_OPENCV;
+    else if (k1 != 0.0 || k2 != 0.0) L->kind = LENS_RADIAL;
+    return true;
+}
+
+int32_t check_support(const vba_problem& pb, const vba_options& o, std::string* reason) {
+    // GpuBundleAdjuster::check (crates/ba-gpu/src/solver.rs:212-251) widened to
+    // the lens models, rig observations and kernels implemented here.
+    if (o.refine_intrinsics) {
+        *reason = "intrinsics refinement";
+        return VBA_DECLINED;
+    }
+    if (o.robust_kernel > VBA_KERNEL_CAUCHY) {
+        *reason = "robust kernel";
+        return VBA_DECLINED;
+    }
+    if (o.linear_solver != VBA_SOLVER_DENSE && o.linear_solver != VBA_SOLVER_MATRIX_FREE &&
+        o.linear_solver != VBA_SOLVER_PCG_EXPLICIT) {
+        *reason = "linear solver";
+        return VBA_DECLINED;
+    }
+    if (o.precision > VBA_PRECISION_MIXED) {
+        *reason = "precision";
+        return VBA_DECLINED;
+    }
+    if (pb.declined_features & (VBA_FEATURE_STEREO_OBSERVATIONS | VBA_FEATURE_GENERAL_STEREO)) {
+        *reason = "stereo observations";
+        return VBA_DECLINED;
+    }
+    if (pb.declined_features & (VBA_FEATURE_PRIORS | VBA_FEATURE_INERTIAL)) {
+        *reason = "prior/inertial terms";
+        return VBA_DECLINED;
+    }
+    if (pb.declined_features & VBA_FEATURE_FIXED_ROTATIONS) {
+        *reason = "rotation-fixed poses";
+        return VBA_DECLINED;
+    }
+    if (pb.declined_features & VBA_FEATURE_OBSERVATION_WEIGHTS) {
+        *reason = "observation weights";
+        return VBA_DECLINED;
+    }
+    if (pb.declined_features) {
+        *reason = "unknown declined feature";
+        return VBA_DECLINED;
+    }
+    if (!supported_model(pb.camera.model)) {
+        *reason = "camera model";
+        return VBA_DECLINED;
+    }
+    for (uint64_t s = 0; s < pb.num_sensors; ++s)
+        if (!supported_model(pb.sensors[s].camera.model)) {
+            *reason = "rig camera model";
+            return VBA_DECLINED;
+        }
+    if (pb.num_sensors > 65535) {
+        *reason = "more than 65535 rig sensors";
+        return VBA_DECLINED;
+    }
+    if (pb.num_observations + pb.num_rig_observations > (uint64_t)INT32_MAX) {
+        *reason = "more than 2^31 observations";
+        return VBA_DECLINED;
+    }
+    return VBA_OK;
+}
+
+int32_t build_model(const vba_problem& pb, Model* mp, uint64_t* error_id, std::string* message) {
+    Model& m = *mp;
+    *error_id = 0;
+    double fx, fy, cx, cy;
+    // lm.rs:168-229: intrinsics, poses, observations, landmarks, ids.
+    if (!supported_model(pb.camera.model) || !camera_intrinsics(pb.camera, &fx, &fy, &cx, &cy)) {
+        *message = "camera model has no intrinsics";
+        return VBA_ERR_UNSUPPORTED_CAMERA_MODEL;
+    }
+    if (pb.num_poses == 0) return VBA_ERR_NO_POSES;
+    if (pb.num_observations == 0 && pb.num_rig_observations == 0) return VBA_ERR_NO_OBSERVATIONS;
+    if (pb.num_landmarks == 0) return VBA_ERR_NO_LANDMARKS;
+    if (!pb.pose_ids || !pb.poses || !pb.landmark_ids || !pb.landmarks ||
+        (pb.num_observations && (!pb.obs_pose_ids || !pb.obs_landmark_ids || !pb.obs_xy)) ||
+        (pb.num_rig_observations &&
+         (!pb.rig_pose_ids || !pb.rig_landmark_ids || !pb.rig_xy || !pb.rig_sensor || !pb.sensors))) {
+        *message = "null array in problem";
+        return VBA_ERR_INVALID_ARGUMENT;
+    }
+    std::vector<uint64_t> pose_ids, lm_ids;
+    if (!slot_order(pb.num_poses, pb.pose_ids, &m.pose_slot_of_input, &pose_ids)) {
+        *message = "duplicate pose id";
+        return VBA_ERR_INVALID_ARGUMENT;
+    }
+    if (!slot_order(pb.num_landmarks, pb.landmark_ids, &m.lm_slot_of_input, &lm_ids)) {
+        *message = "duplicate landmark id";
+        return VBA_ERR_INVALID_ARGUMENT;
+    }
+    m.num_pose_slots = (uint32_t)pb.num_poses;
+    m.num_lm_slots = (uint32_t)pb.num_landmarks;
+    m.poses.assign(7 * (size_t)pb.num_poses, 0.0);
+    std::vector<uint8_t> pose_fixed(pb.num_poses, 0), lm_fixed(pb.num_landmarks, 0);
+    for (uint64_t i = 0; i < pb.num_poses; ++i) {
+        uint32_t s = m.pose_slot_of_input[i];
+        std::memcpy(&m.poses[7 * (size_t)s], pb.poses + 7 * i, 7 * sizeof(double));
+        pose_fixed[s] = pb.pose_fixed ? (pb.pose_fixed[i] != 0) : 0;
+    }
+    m.lms.assign(3 * (size_t)pb.num_landmarks, 0.0);
+    for (uint64_t i = 0; i < pb.num_landmarks; ++i) {
+        uint32_t s = m.lm_slot_of_input[i];
+        std::memcpy(&m.lms[3 * (size_t)s], pb.landmarks + 3 * i, 3 * sizeof(double));
+        lm_fixed[s] = pb.landmark_fixed ? (pb.landmark_fixed[i] != 0) : 0;
+    }
+
+    // Sensors: 0 = shared camera, 1 + s = rig sensor s.
+    m.sensors.clear();
+    DevSensor shared{};
+    lens_from_camera(pb.camera, &shared.lens);
+    shared.R[0] = shared.R[4] = shared.R[8] = 1.0;
+    shared.identity = 1;
+    m.sensors.push_back(shared);
+    for (uint64_t s = 0; s < pb.num_sensors; ++s) {
+        DevSensor d{};
+        const vba_sensor& vs = pb.sensors[s];
+        if (!lens_from_camera(vs.camera, &d.lens)) d.lens.kind = -1;  // checked per observation
+        quat_to_matrix(vs.sensor_from_rig, d.R);
+        d.t[0] = vs.sensor_from_rig[4];
+        d.t[1] = vs.sensor_from_rig[5];
+        d.t[2] = vs.sensor_from_rig[6];
+        d.identity = 0;
+        m.sensors.push_back(d);
+    }
+
+    // Observations: mono then rig, ids resolved in input order (lm.rs:193-229).
+    const uint64_t nm = pb.num_observations, nr = pb.num_rig_observations, n = nm + nr;
+    std::vector<uint32_t> opose(n), olm(n);
+    std::vector<uint16_t> osensor(n, 0);
+    for (uint64_t i = 0; i < nm; ++i) {
+        if (!lookup(pose_ids, pb.obs_pose_ids[i], &opose[i])) {
+            *error_id = pb.obs_pose_ids[i];
+            return VBA_ERR_MISSING_POSE;
+        }
+        if (!lookup(lm_ids, pb.obs_landmark_ids[i], &olm[i])) {
+            *error_id = pb.obs_landmark_ids[i];
+            return VBA_ERR_MISSING_LANDMARK;
+        }
+    }
+    for (uint64_t i = 0; i < nr; ++i) {
+        uint64_t k = nm + i;
+        if (!lookup(pose_ids, pb.rig_pose_ids[i], &opose[k])) {
+            *error_id = pb.rig_pose_ids[i];
+            return VBA_ERR_MISSING_POSE;
+        }
+        if (!lookup(lm_ids, pb.rig_landmark_ids[i], &olm[k])) {
+            *error_id = pb.rig_landmark_ids[i];
+            return VBA_ERR_MISSING_LANDMARK;
+        }
+        uint32_t s = pb.rig_sensor[i];
+        if (s >= pb.num_sensors) {
+            *message = "rig sensor index out of range";
+            return VBA_ERR_INVALID_ARGUMENT;
+        }
+        if (m.sensors[s + 1].lens.kind < 0) {
+            *message = "rig camera model has no intrinsics";
+            return VBA_ERR_UNSUPPORTED_CAMERA_MODEL;
+        }
+        osensor[k] = (uint16_t)(s + 1);
+    }
+    m.any_rig = nr > 0;
+    m.pose_var.assign(m.num_pose_slots, -1);
+    m.lm_var.assign(m.num_lm_slots, -1);
+    m.P = m.L = 0;
+    m.any_fixed_pose = false;
+    for (uint32_t s = 0; s < m.num_pose_slots; ++s) {
+        if (pose_fixed[s]) m.any_fixed_pose = true;
+        else m.pose_var[s] = (int32_t)m.P++;
+    }
+    for (uint32_t s = 0; s < m.num_lm_slots; ++s)
+        if (!lm_fixed[s]) m.lm_var[s] = (int32_t)m.L++;
+    if (m.P == 0 && m.L == 0) return VBA_ERR_ALL_POSES_FIXED;
+
+    // c-order: counting sort by pose slot (stable), then (landmark, input) within.
+    m.num_obs = n;
+    std::vector<uint32_t> count(m.num_pose_slots + 1, 0);
+    for (uint64_t i = 0; i < n; ++i) ++count[opose[i] + 1];
+    for (uint32_t s = 0; s < m.num_pose_slots; ++s) count[s + 1] += count[s];
+    std::vector<uint32_t> cord(n);
+    {
+        std::vector<uint32_t> cursor(count.begin(), count.end() - 1);
+        for (uint64_t i = 0; i < n; ++i) cord[cursor[opose[i]]++] = (uint32_t)i;
+    }
+    for (uint32_t s = 0; s < m.num_pose_slots; ++s)
+        std::sort(cord.begin() + count[s], cord.begin() + count[s + 1],
+                  [&](uint32_t a, uint32_t b) { return olm[a] != olm[b] ? olm[a] < olm[b] : a < b; });
+    m.obs_uv.resize(2 * n);
+    m.obs_input = cord;
+    if (m.any_rig) m.obs_sensor.resize(n);
+    else m.obs_sensor.clear();
+    for (uint64_t k = 0; k < n; ++k) {
+        uint32_t i = cord[k];
+        const double* xy = i < nm ? pb.obs_xy + 2 * i : pb.rig_xy + 2 * (i - nm);
+        m.obs_uv[2 * k] = xy[0];
+        m.obs_uv[2 * k + 1] = xy[1];
+        if (m.any_rig) m.obs_sensor[k] = osensor[i];
+    }
+    // Edges: runs of equal (pose, landmark).
+    m.edge_pose.clear();
+    m.edge_lm.clear();
+    m.edge_obs.clear();
+    for (uint64_t k = 0; k < n; ++k) {
+        uint32_t i = cord[k];
+        if (k == 0 || opose[i] != opose[cord[k - 1]] || olm[i] != olm[cord[k - 1]]) {
+            m.edge_pose.push_back((int32_t)opose[i]);
+            m.edge_lm.push_back((int32_t)olm[i]);
+            m.edge_obs.push_back((int32_t)k);
+        }
+    }
+    m.num_edges = m.edge_pose.size();
+    m.edge_obs.push_back((int32_t)n);
+    m.cam_edge_ptr.assign(m.num_pose_slots + 1, 0);
+    for (uint64_t e = 0; e < m.num_edges; ++e) ++m.cam_edge_ptr[m.edge_pose[e] + 1];
+    for (uint32_t s = 0; s < m.num_pose_slots; ++s) m.cam_edge_ptr[s + 1] += m.cam_edge_ptr[s];
+
+    // l-order: counting sort of c-order edges by landmark (stable: pose-sorted).
+    m.lm_edge_ptr.assign(m.num_lm_slots + 1, 0);
+    for (uint64_t e = 0; e < m.num_edges; ++e) ++m.lm_edge_ptr[m.edge_lm[e] + 1];
+    for (uint32_t s = 0; s < m.num_lm_slots; ++s) m.lm_edge_ptr[s + 1] += m.lm_edge_ptr[s];
+    m.perm_cl.resize(m.num_edges);
+    m.l_pose.resize(m.num_edges);
+    {
+        std::vector<int32_t> cursor(m.lm_edge_ptr.begin(), m.lm_edge_ptr.end() - 1);
+        for (uint64_t e = 0; e < m.num_edges; ++e) {
+            int32_t pos = cursor[m.edge_lm[e]]++;
+            m.perm_cl[e] = pos;
+            m.l_pose[pos] = m.edge_pose[e];
+        }
+    }
+
+    // Camera tiles: about 1,500 work units over the whole problem, each a
+    // slice of one pose's edges (256..4096 edges).
+    const uint32_t tc = std::min<uint32_t>(4096, std::max<uint32_t>(256, next_pow2(m.num_edges / 1536 + 1)));
+    m.cam_tiles.clear();
+    m.cam_tile_ptr.assign(m.num_pose_slots + 1, 0);
+    for (uint32_t s = 0; s < m.num_pose_slots; ++s) {
+        m.cam_tile_ptr[s] = (int32_t)m.cam_tiles.size();
+        for (int32_t b = m.cam_edge_ptr[s]; b < m.cam_edge_ptr[s + 1]; b += (int32_t)tc)
+            m.cam_tiles.push_back({(int32_t)s, b, std::min<int32_t>(b + (int32_t)tc, m.cam_edge_ptr[s + 1]), 0});
+    }
+    m.cam_tile_ptr[m.num_pose_slots] = (int32_t)m.cam_tiles.size();
+    // Landmark tiles: at most 256 landmarks and 4096 edges (a single landmark
+    // with more edges forms its own tile).
+    m.lm_tiles.clear();
+    m.max_lm_tile_edges = 0;
+    for (uint32_t b = 0; b < m.num_lm_slots;) {
+        uint32_t e = b + 1;
+        while (e < m.num_lm_slots && e - b < 256 && m.lm_edge_ptr[e + 1] - m.lm_edge_ptr[b] <= 4096) ++e;
+        m.lm_tiles.push_back({(int32_t)b, (int32_t)e});
+        m.max_lm_tile_edges = std::max(m.max_lm_tile_edges, m.lm_edge_ptr[e] - m.lm_edge_ptr[b]);
+        b = e;
+    }
+    return VBA_OK;
+}
+
+}  // namespace vba

@@ -1,0 +1,95 @@
+// problem.hpp — host-side layout of one bundle-adjustment problem.
+//
+// The caller's id-keyed problem (vba_problem, the view of `BundleAdjustment`,
+// pipelines/slam/src/bundle/mod.rs:529-622) becomes dense slots in ascending
+// id order (the reference's BTreeMap order, lm.rs:234-249) and a
+// structure-of-arrays layout sized for streaming from HBM:
+//
+//   observations   sorted by (pose slot, landmark slot, input index): the
+//                  camera-sorted "c-order";
+//   edges          runs of observations with the same (pose, landmark) in
+//                  c-order (a stereo rig's two sensors seeing one landmark
+//                  from one rig pose form one edge);
+//   l-order        the same edges sorted by (landmark slot, pose slot), with
+//                  the landmark CSR lm_edge_ptr and perm_cl (c -> l position);
+//   camera tiles   slices of one pose's c-order edges, the work unit of every
+//                  camera-keyed kernel (segmented reduction, no atomics);
+//   landmark tiles runs of whole landmarks, the work unit of every
+//                  landmark-keyed kernel.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/vba.h"
+#include "geom.hpp"
+
+namespace vba {
+
+// A lens plus the fixed extrinsic T_sensor<-rig as a rotation matrix and a
+// translation. Sensor 0 is the shared camera of monocular observations with
+// the identity extrinsic.
+struct DevSensor {
+    Lens<double> lens;
+    double R[9];
+    double t[3];
+    int32_t identity;
+    int32_t pad;
+};
+
+struct CamTile {
+    int32_t pose;   // pose slot
+    int32_t begin;  // c-order edge range
+    int32_t end;
+    int32_t pad;
+};
+struct LmTile {
+    int32_t lm_begin;  // landmark slot range
+    int32_t lm_end;
+};
+
+struct Model {
+    uint32_t num_pose_slots = 0, num_lm_slots = 0;
+    uint32_t P = 0, L = 0;  // variable poses / landmarks
+    std::vector<uint32_t> pose_slot_of_input, lm_slot_of_input;
+    std::vector<int32_t> pose_var, lm_var;  // variable index per slot, -1 = fixed
+    std::vector<double> poses;              // 7 per slot (qw qx qy qz tx ty tz)
+    std::vector<double> lms;                // 3 per slot
+    std::vector<DevSensor> sensors;
+    bool any_rig = false;
+    bool any_fixed_pose = false;
+
+    uint64_t num_obs = 0;
+    std::vector<double> obs_uv;         // 2 per observation, c-order
+    std::vector<uint16_t> obs_sensor;   // c-order (empty when !any_rig)
+    std::vector<uint32_t> obs_input;    // c-order -> input index (mono, then rig)
+
+    uint64_t num_edges = 0;
+    std::vector<int32_t> edge_pose, edge_lm;  // c-order
+    std::vector<int32_t> edge_obs;            // num_edges + 1: first observation
+    std::vector<int32_t> cam_edge_ptr;        // num_pose_slots + 1
+    std::vector<int32_t> lm_edge_ptr;         // num_lm_slots + 1 (l-order)
+    std::vector<int32_t> l_pose;              // l-order pose slot
+    std::vector<int32_t> perm_cl;             // c-order edge -> l-order position
+
+    std::vector<CamTile> cam_tiles;
+    std::vector<int32_t> cam_tile_ptr;  // num_pose_slots + 1
+    std::vector<LmTile> lm_tiles;
+    int32_t max_lm_tile_edges = 0;
+};
+
+// Lens of a vba_camera, resolved as MonoProjection::for_camera does
+// (jacobians.rs:29-58). Returns false for an unsupported model.
+bool lens_from_camera(const vba_camera& camera, Lens<double>* lens);
+
+// Validate and lay out a problem. Status codes and their order follow
+// optimize_weighted_backend (lm.rs:168-229); *error_id receives the missing
+// id for VBA_ERR_MISSING_POSE / _LANDMARK. `message` explains a failure.
+int32_t build_model(const vba_problem& problem, Model* model, uint64_t* error_id,
+                    std::string* message);
+
+// vba_check: VBA_OK or VBA_DECLINED with a reason.
+int32_t check_support(const vba_problem& problem, const vba_options& options, std::string* reason);
+
+}  // namespace vba

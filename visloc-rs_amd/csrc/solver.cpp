@@ -1,0 +1,496 @@
+// solver.cpp — device-resident Levenberg–Marquardt loop.
+//
+// Control flow and acceptance rules follow optimize_weighted_backend
+// (pipelines/slam/src/bundle/lm.rs:162-1017): strict cost decrease plus a
+// non-increasing non-projectable count, lambda x10 (capped) on reject or on a
+// failed linear solve, x0.1 (floored) on accept, and the step / cost /
+// relative-cost convergence tests. The state stays on the device; the host
+// reads back a few scalars per iteration (cost, count, step norms, PCG
+// status).
+#include "solver.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+
+#include "host_common.hpp"
+
+namespace vba {
+
+namespace {
+
+enum BufId {
+    B_POSES, B_LMS, B_POSES_T, B_LMS_T, B_SENSORS, B_POSE_VAR, B_LM_VAR, B_OBS_UV, B_OBS_SENSOR,
+    B_EDGE_LM, B_EDGE_OBS, B_PERM_CL, B_L_POSE, B_LM_EDGE_PTR, B_CAM_TILES, B_CAM_TILE_PTR,
+    B_LM_TILES, B_EC64, B_EL64, B_HL64, B_EC32, B_EL32, B_TILE27, B_U, B_G, B_V, B_GL, B_VINV64,
+    B_VINV32, B_PACK64, B_PACK32, B_MINV, B_VB, B_VX, B_VR, B_VZ, B_VD, B_VQ, B_VD32, B_PART64,
+    B_PART32, B_PCG, B_DELTA_L, B_TILE_SCALAR, B_SCALARS, B_DENSE_BLOCK_PQ, B_DENSE_BLOCK_PTR,
+    B_DENSE_TERMS, B_DENSE_VAR_SLOT, B_DENSE_S, B_DENSE_WORK, B_DENSE_STATUS, B_COUNT
+};
+
+struct Fail {
+    int32_t code;
+    std::string msg;
+};
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw Fail{e == hipErrorOutOfMemory ? VBA_ERR_OUT_OF_MEMORY : VBA_ERR_DEVICE,
+                                    std::string(what) + ": " + hipGetErrorString(e)};
+}
+
+void* ensure(Workspace& ws, int id, size_t bytes) {
+    if (ws.bufs.size() < B_COUNT) ws.bufs.resize(B_COUNT);
+    DevBuffer& b = ws.bufs[id];
+    if (bytes == 0) bytes = 64;
+    if (b.bytes < bytes) {
+        if (b.ptr) hip_check(hipFree(b.ptr), "hipFree");
+        b.ptr = nullptr;
+        b.bytes = 0;
+        hip_check(hipMalloc(&b.ptr, bytes), "hipMalloc");
+        b.bytes = bytes;
+    }
+    return b.ptr;
+}
+template <typename T>
+T* upload(Workspace& ws, int id, const std::vector<T>& v, hipStream_t s) {
+    T* p = (T*)ensure(ws, id, v.size() * sizeof(T));
+    if (!v.empty()) hip_check(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+    return p;
+}
+
+// Dense reduced system structure: one block per (var p <= var q) sharing a
+// landmark plus every diagonal block; each term is (landmark, e_p, e_q).
+struct DenseHost {
+    std::vector<int32_t> block_pq, block_ptr, terms, var_slot;
+};
+void build_dense(const Model& m, DenseHost* D) {
+    const uint32_t P = m.P;
+    D->var_slot.assign(P, 0);
+    for (uint32_t s = 0; s < m.num_pose_slots; ++s)
+        if (m.pose_var[s] >= 0) D->var_slot[m.pose_var[s]] = (int32_t)s;
+    std::vector<int32_t> block_id((size_t)P * P, -1);
+    std::vector<int64_t> count;
+    auto block_of = [&](int32_t a, int32_t b) {
+        int32_t& id = block_id[(size_t)a * P + b];
+        if (id < 0) {
+            id = (int32_t)count.size();
+            count.push_back(0);
+            D->block_pq.push_back(D->var_slot[a]);
+            D->block_pq.push_back(D->var_slot[b]);
+        }
+        return id;
+    };
+    for (uint32_t v = 0; v < P; ++v) block_of((int32_t)v, (int32_t)v);
+    std::vector<std::pair<int32_t, int32_t>> ends;  // (var pose, l-order edge)
+    for (uint32_t l = 0; l < m.num_lm_slots; ++l) {
+        if (m.lm_var[l] < 0) continue;
+        ends.clear();
+        for (int32_t e = m.lm_edge_ptr[l]; e < m.lm_edge_ptr[l + 1]; ++e)
+            if (m.pose_var[m.l_pose[e]] >= 0) ends.push_back({m.pose_var[m.l_pose[e]], e});
+        for (size_t a = 0; a < ends.size(); ++a)
+            for (size_t b = a; b < ends.size(); ++b) ++count[block_of(ends[a].first, ends[b].first)];
+    }
+    const size_t nb = count.size();
+    D->block_ptr.assign(nb + 1, 0);
+    for (size_t k = 0; k < nb; ++k) D->block_ptr[k + 1] = D->block_ptr[k] + (int32_t)count[k];
+    D->terms.assign(3 * (size_t)D->block_ptr[nb], 0);
+    std::vector<int32_t> cursor(D->block_ptr.begin(), D->block_ptr.end() - 1);
+    for (uint32_t l = 0; l < m.num_lm_slots; ++l) {
+        if (m.lm_var[l] < 0) continue;
+        ends.clear();
+        for (int32_t e = m.lm_edge_ptr[l]; e < m.lm_edge_ptr[l + 1]; ++e)
+            if (m.pose_var[m.l_pose[e]] >= 0) ends.push_back({m.pose_var[m.l_pose[e]], e});
+        for (size_t a = 0; a < ends.size(); ++a)
+            for (size_t b = a; b < ends.size(); ++b) {
+                int32_t k = cursor[block_id[(size_t)ends[a].first * P + ends[b].first]]++;
+                D->terms[3 * (size_t)k] = (int32_t)l;
+                D->terms[3 * (size_t)k + 1] = ends[a].second;
+                D->terms[3 * (size_t)k + 2] = ends[b].second;
+            }
+    }
+}
+
+double norm6(const double* v) {
+    double s = 0.0;
+    for (int i = 0; i < 6; ++i) s += v[i] * v[i];
+    return std::sqrt(s);
+}
+
+}  // namespace
+
+int32_t validate_matrix_free(const Model& m, const vba_options& o, std::string* msg) {
+    // optimize.rs:382-460
+    if (o.refine_intrinsics) {
+        *msg = "intrinsics/distortion refinement is not supported";
+        return VBA_ERR_INELIGIBLE;
+    }
+    const double il = o.initial_lambda;
+    if (o.max_iterations == 0 || !o.has_initial_lambda || !(std::isfinite(il) && il > 0.0) ||
+        !std::isfinite(o.lambda_increase_factor) || o.lambda_increase_factor <= 1.0 ||
+        !std::isfinite(o.lambda_decrease_factor) || o.lambda_decrease_factor <= 0.0 ||
+        o.lambda_decrease_factor >= 1.0 || !std::isfinite(o.max_lambda) || o.max_lambda < il ||
+        !std::isfinite(o.min_lambda) || o.min_lambda <= 0.0 || o.min_lambda > il ||
+        !std::isfinite(o.step_tolerance) || o.step_tolerance < 0.0 || !std::isfinite(o.cost_tolerance) ||
+        o.cost_tolerance < 0.0 ||
+        (o.has_relative_cost_tolerance &&
+         (!std::isfinite(o.relative_cost_tolerance) || o.relative_cost_tolerance < 0.0))) {
+        *msg = "LM damping/tolerance values are not finite or ordered";
+        return VBA_ERR_INVALID_CONFIGURATION;
+    }
+    if (o.max_pcg_iterations == 0 || !std::isfinite(o.pcg_relative_tolerance) || o.pcg_relative_tolerance < 0.0 ||
+        !std::isfinite(o.pcg_absolute_tolerance) || o.pcg_absolute_tolerance <= 0.0) {
+        *msg = "PCG iteration and tolerance values are invalid";
+        return VBA_ERR_INVALID_CONFIGURATION;
+    }
+    if (o.robust_kernel != VBA_KERNEL_NONE && !(std::isfinite(o.robust_scale) && o.robust_scale > 0.0)) {
+        *msg = "robust-kernel scale must be finite and positive";
+        return VBA_ERR_INVALID_CONFIGURATION;
+    }
+    auto finite_lens = [](const Lens<double>& L) {
+        const double v[8] = {L.fx, L.fy, L.cx, L.cy, L.k1, L.k2, L.p1, L.p2};
+        for (double x : v)
+            if (!std::isfinite(x)) return false;
+        return L.fx > 0.0 && L.fy > 0.0;
+    };
+    for (const DevSensor& s : m.sensors)
+        if (!finite_lens(s.lens)) {
+            *msg = "camera calibration is unsupported or non-finite";
+            return VBA_ERR_INELIGIBLE;
+        }
+    if (!m.any_fixed_pose) {
+        *msg = "at least one existing pose must be fixed as a gauge anchor";
+        return VBA_ERR_INELIGIBLE;
+    }
+    if (m.P == 0) {
+        *msg = "matrix-free backend requires a variable pose";
+        return VBA_ERR_INELIGIBLE;
+    }
+    for (double v : m.poses)
+        if (!std::isfinite(v)) {
+            *msg = "pose or landmark state is non-finite";
+            return VBA_ERR_INELIGIBLE;
+        }
+    for (double v : m.lms)
+        if (!std::isfinite(v)) {
+            *msg = "pose or landmark state is non-finite";
+            return VBA_ERR_INELIGIBLE;
+        }
+    for (double v : m.obs_uv)
+        if (!std::isfinite(v)) {
+            *msg = "observation has an invalid pixel";
+            return VBA_ERR_INELIGIBLE;
+        }
+    return VBA_OK;
+}
+
+void release_workspace(Workspace& ws) {
+    for (DevBuffer& b : ws.bufs)
+        if (b.ptr) (void)hipFree(b.ptr);
+    ws.bufs.clear();
+    if (ws.host_scalars) (void)hipHostFree(ws.host_scalars);
+    if (ws.host_pcg) (void)hipHostFree(ws.host_pcg);
+    if (ws.host_status) (void)hipHostFree(ws.host_status);
+    ws.host_scalars = nullptr;
+    ws.host_pcg = nullptr;
+    ws.host_status = nullptr;
+}
+
+int32_t solve(Context& ctx, const Model& m, const vba_options& o, double* poses_out, double* landmarks_out,
+              vba_iteration_stats* stats, uint32_t cap, vba_result* res) {
+    Workspace& ws = ctx.ws;
+    hipStream_t s = ctx.stream;
+    const bool matrix_free = o.linear_solver != VBA_SOLVER_DENSE;
+    const bool mixed = matrix_free && o.precision == VBA_PRECISION_MIXED;
+    const bool accept_iterate = o.pcg_maxit_policy == VBA_PCG_MAXIT_ACCEPT_ITERATE;
+    try {
+        if (!ws.host_scalars) {
+            hip_check(hipHostMalloc((void**)&ws.host_scalars, sizeof(StepScalars), hipHostMallocDefault), "hipHostMalloc");
+            hip_check(hipHostMalloc((void**)&ws.host_pcg, sizeof(PcgState), hipHostMallocDefault), "hipHostMalloc");
+            hip_check(hipHostMalloc((void**)&ws.host_status, sizeof(int32_t), hipHostMallocDefault), "hipHostMalloc");
+        }
+        DenseHost dh;
+        if (!matrix_free) {
+            if (m.P > 1024) throw Fail{VBA_ERR_INVALID_ARGUMENT, "dense reduced solve limited to 1024 variable poses"};
+            build_dense(m, &dh);
+        }
+        // ---- upload ----
+        DevProblem P{};
+        P.num_pose_slots = (int32_t)m.num_pose_slots;
+        P.num_lm_slots = (int32_t)m.num_lm_slots;
+        P.num_edges = (int32_t)m.num_edges;
+        P.num_obs = (int32_t)m.num_obs;
+        P.num_cam_tiles = (int32_t)m.cam_tiles.size();
+        P.num_lm_tiles = (int32_t)m.lm_tiles.size();
+        P.any_rig = m.any_rig;
+        P.kernel_kind = (int32_t)o.robust_kernel;
+        P.kernel_scale = o.robust_scale;
+        const size_t NP = m.num_pose_slots, NL = m.num_lm_slots, NE = m.num_edges, NT = m.cam_tiles.size();
+        const size_t NLT = m.lm_tiles.size();
+        const size_t pose_blocks = (NP + kBlock - 1) / kBlock;
+        P.poses = upload(ws, B_POSES, m.poses, s);
+        P.lms = upload(ws, B_LMS, m.lms, s);
+        P.poses_trial = (double*)ensure(ws, B_POSES_T, m.poses.size() * sizeof(double));
+        P.lms_trial = (double*)ensure(ws, B_LMS_T, m.lms.size() * sizeof(double));
+        P.sensors = upload(ws, B_SENSORS, m.sensors, s);
+        P.pose_var = upload(ws, B_POSE_VAR, m.pose_var, s);
+        P.lm_var = upload(ws, B_LM_VAR, m.lm_var, s);
+        P.obs_uv = upload(ws, B_OBS_UV, m.obs_uv, s);
+        P.obs_sensor = m.any_rig ? upload(ws, B_OBS_SENSOR, m.obs_sensor, s) : nullptr;
+        P.edge_lm = upload(ws, B_EDGE_LM, m.edge_lm, s);
+        P.edge_obs = upload(ws, B_EDGE_OBS, m.edge_obs, s);
+        P.perm_cl = upload(ws, B_PERM_CL, m.perm_cl, s);
+        P.l_pose = upload(ws, B_L_POSE, m.l_pose, s);
+        P.lm_edge_ptr = upload(ws, B_LM_EDGE_PTR, m.lm_edge_ptr, s);
+        P.cam_tiles = upload(ws, B_CAM_TILES, m.cam_tiles, s);
+        P.cam_tile_ptr = upload(ws, B_CAM_TILE_PTR, m.cam_tile_ptr, s);
+        P.lm_tiles = upload(ws, B_LM_TILES, m.lm_tiles, s);
+        P.ec64 = (EdgeRec64*)ensure(ws, B_EC64, NE * sizeof(EdgeRec64));
+        P.el64 = (EdgeRec64*)ensure(ws, B_EL64, NE * sizeof(EdgeRec64));
+        P.hl64 = (double*)ensure(ws, B_HL64, 3 * NE * sizeof(double));
+        if (mixed) {
+            P.ec32 = (EdgeRec32*)ensure(ws, B_EC32, NE * sizeof(EdgeRec32));
+            P.el32 = (EdgeRec32*)ensure(ws, B_EL32, NE * sizeof(EdgeRec32));
+            P.Vinv32 = (float*)ensure(ws, B_VINV32, 6 * NL * sizeof(float));
+            P.pack32 = (LmPack<float>*)ensure(ws, B_PACK32, NL * sizeof(LmPack<float>));
+            P.part32 = (float*)ensure(ws, B_PART32, 6 * std::max<size_t>(NT, 1) * sizeof(float));
+        }
+        P.tile27 = (double*)ensure(ws, B_TILE27, 27 * std::max<size_t>(NT, 1) * sizeof(double));
+        P.U = (double*)ensure(ws, B_U, 21 * NP * sizeof(double));
+        P.g = (double*)ensure(ws, B_G, 6 * NP * sizeof(double));
+        P.V = (double*)ensure(ws, B_V, 6 * NL * sizeof(double));
+        P.gl = (double*)ensure(ws, B_GL, 3 * NL * sizeof(double));
+        P.Vinv64 = (double*)ensure(ws, B_VINV64, 6 * NL * sizeof(double));
+        P.pack64 = (LmPack<double>*)ensure(ws, B_PACK64, NL * sizeof(LmPack<double>));
+        P.Minv = (double*)ensure(ws, B_MINV, 36 * NP * sizeof(double));
+        P.b = (double*)ensure(ws, B_VB, 6 * NP * sizeof(double));
+        P.x = (double*)ensure(ws, B_VX, 6 * NP * sizeof(double));
+        P.r = (double*)ensure(ws, B_VR, 6 * NP * sizeof(double));
+        P.z = (double*)ensure(ws, B_VZ, 6 * NP * sizeof(double));
+        P.d = (double*)ensure(ws, B_VD, 6 * NP * sizeof(double));
+        P.q = (double*)ensure(ws, B_VQ, 6 * NP * sizeof(double));
+        P.d32 = (float*)ensure(ws, B_VD32, 6 * NP * sizeof(float));
+        P.part64 = (double*)ensure(ws, B_PART64, 6 * std::max<size_t>(NT, 1) * sizeof(double));
+        P.pcg = (PcgState*)ensure(ws, B_PCG, sizeof(PcgState));
+        P.delta_l = (double*)ensure(ws, B_DELTA_L, 3 * NL * sizeof(double));
+        P.tile_scalar = (double*)ensure(ws, B_TILE_SCALAR, 4 * (NT + NLT + pose_blocks + 1) * sizeof(double));
+        P.scalars = (StepScalars*)ensure(ws, B_SCALARS, sizeof(StepScalars));
+        hip_check(hipMemsetAsync(P.x, 0, 6 * NP * sizeof(double), s), "memset");
+        hip_check(hipMemsetAsync(P.tile_scalar, 0, 4 * (NT + NLT + pose_blocks + 1) * sizeof(double), s), "memset");
+        DensePairs D{};
+        double* dS = nullptr;
+        double* dwork = nullptr;
+        int32_t* dstatus = nullptr;
+        const int32_t ndense = (int32_t)(6 * m.P);
+        if (!matrix_free) {
+            D.num_blocks = (int32_t)(dh.block_ptr.size() - 1);
+            D.block_pq = upload(ws, B_DENSE_BLOCK_PQ, dh.block_pq, s);
+            D.block_ptr = upload(ws, B_DENSE_BLOCK_PTR, dh.block_ptr, s);
+            D.terms = upload(ws, B_DENSE_TERMS, dh.terms, s);
+            D.var_slot = upload(ws, B_DENSE_VAR_SLOT, dh.var_slot, s);
+            dS = (double*)ensure(ws, B_DENSE_S, (size_t)ndense * ndense * sizeof(double));
+            dwork = (double*)ensure(ws, B_DENSE_WORK, ((size_t)ndense * ndense + ndense) * sizeof(double));
+            dstatus = (int32_t*)ensure(ws, B_DENSE_STATUS, sizeof(int32_t));
+        }
+        hip_check(hipEventRecord(ctx.ev_start, s), "event");
+
+        auto read_scalars = [&]() {
+            launch_scalars_finalize(P, s);
+            hip_check(hipGetLastError(), "kernel launch");
+            hip_check(hipMemcpyAsync(ws.host_scalars, P.scalars, sizeof(StepScalars), hipMemcpyDeviceToHost, s), "D2H");
+            hip_check(hipStreamSynchronize(s), "sync");
+            return *ws.host_scalars;
+        };
+        auto read_pcg = [&]() {
+            hip_check(hipGetLastError(), "kernel launch");
+            hip_check(hipMemcpyAsync(ws.host_pcg, P.pcg, sizeof(PcgState), hipMemcpyDeviceToHost, s), "D2H");
+            hip_check(hipStreamSynchronize(s), "sync");
+            return *ws.host_pcg;
+        };
+
+        // ---- initial cost (lm.rs:323-327) ----
+        hip_check(hipMemsetAsync(P.scalars, 0, sizeof(StepScalars), s), "memset");
+        launch_cost(P, false, s);
+        StepScalars sc = read_scalars();
+        const double initial_cost = sc.cost_sum + sc.cost_comp;
+        uint64_t current_nonproj = sc.nonprojectable;
+        if (matrix_free && !std::isfinite(initial_cost))
+            throw Fail{VBA_ERR_INELIGIBLE, "initial visual cost is non-finite"};
+
+        double current_cost = initial_cost;
+        double lambda = o.has_initial_lambda ? o.initial_lambda : 0.0;
+        bool converged = false, need_lin = true;
+        uint32_t written = 0, total_pcg = 0, linearizations = 0;
+        const uint32_t chunk = 10;
+        auto push = [&](const vba_iteration_stats& st) {
+            if (stats && written < cap) stats[written] = st;
+            ++written;
+        };
+
+        for (uint32_t iteration = 0; iteration < o.max_iterations; ++iteration) {
+            if (need_lin) {
+                launch_linearize(P, mixed, s);
+                launch_camera_finalize(P, s);
+                launch_landmark_reduce(P, s);
+                need_lin = false;
+                ++linearizations;
+            }
+            hip_check(hipMemsetAsync(P.scalars, 0, sizeof(StepScalars), s), "memset");
+            launch_landmark_prepare(P, lambda, s);
+            launch_cam_pass(P, false, nullptr, s);  // u = V^-1 g_l
+            launch_rhs_finalize(P, s);
+
+            vba_iteration_stats st;
+            std::memset(&st, 0, sizeof(st));
+            st.iteration = iteration;
+            st.pcg_iterations = -1;
+            bool solved = true;
+            if (!matrix_free) {
+                hip_check(hipMemsetAsync(dS, 0, (size_t)ndense * ndense * sizeof(double), s), "memset");
+                launch_dense_assemble(P, D, dS, ndense, lambda, s);
+                launch_dense_solve(P, D, dS, dwork, ndense, dstatus, s);
+                hip_check(hipMemcpyAsync(ws.host_status, dstatus, sizeof(int32_t), hipMemcpyDeviceToHost, s), "D2H");
+                hip_check(hipStreamSynchronize(s), "sync");
+                solved = *ws.host_status == 0;
+            } else {
+                launch_precondition(P, lambda, s);
+                launch_pcg_init(P, o.pcg_relative_tolerance, o.pcg_absolute_tolerance, (int32_t)o.max_pcg_iterations,
+                                lambda, s);
+                PcgState ps = read_pcg();
+                const int32_t* flag = &P.pcg->status;
+                while (true) {
+                    while (ps.status == PCG_RUNNING) {
+                        const int before = ps.iter;
+                        const int n = (int)std::min<uint32_t>(chunk, (uint32_t)std::max(0, ps.max_iter - ps.iter));
+                        for (int k = 0; k < n; ++k) {
+                            launch_lm_pass(P, mixed, mixed ? (const void*)P.d32 : (const void*)P.d, flag, s);
+                            launch_cam_pass(P, mixed, flag, s);
+                            launch_pcg_update(P, mixed, s);
+                        }
+                        hip_check(hipGetLastError(), "PCG launch");
+                        ps = read_pcg();
+                        if (ps.status == PCG_RUNNING && ps.iter <= before)
+                            throw Fail{VBA_ERR_DEVICE, "PCG made no progress"};
+                    }
+                    if (ps.status != PCG_CONVERGED && ps.status != PCG_MAXIT) {
+                        solved = false;
+                        break;
+                    }
+                    // true residual re-check (implicit_schur.rs:613-690, 750-761)
+                    launch_lm_pass(P, false, P.x, nullptr, s);
+                    launch_cam_pass(P, false, nullptr, s);
+                    launch_pcg_true_residual(P, false, s);
+                    ps = read_pcg();
+                    if (std::isfinite(ps.true_norm) && ps.true_norm <= ps.target) break;
+                    if (!std::isfinite(ps.true_norm)) {
+                        solved = false;
+                        break;
+                    }
+                    if (mixed && ps.iter < ps.max_iter) {
+                        // FP64 refinement: continue from the true residual
+                        launch_pcg_true_residual(P, true, s);
+                        ps = read_pcg();
+                        continue;
+                    }
+                    solved = accept_iterate;
+                    break;
+                }
+                st.pcg_iterations = ps.iter;
+                st.pcg_residual_norm = ps.true_norm;
+                st.pcg_target = ps.target;
+                total_pcg += (uint32_t)ps.iter;
+            }
+            const double cost_before = current_cost;
+            if (!solved) {  // lm.rs:610-662: a failed solve is a rejected step
+                lambda = std::min(lambda * o.lambda_increase_factor, o.max_lambda);
+                st.cost_before = cost_before;
+                st.cost_after = cost_before;
+                st.lambda = lambda;
+                st.nonprojectable_after = (uint32_t)current_nonproj;
+                push(st);
+                if (lambda >= o.max_lambda) break;
+                continue;
+            }
+            launch_backsub(P, s);
+            launch_pose_update(P, s);
+            launch_cost(P, true, s);
+            sc = read_scalars();
+            const double cost_after = sc.cost_sum + sc.cost_comp;
+            const uint64_t nonproj_after = sc.nonprojectable;
+            const bool cost_accepted = o.has_initial_lambda ? (cost_after < cost_before) : true;
+            const bool accepted = cost_accepted && nonproj_after <= current_nonproj;
+            st.cost_before = cost_before;
+            st.cost_after = cost_after;
+            st.max_pose_step = sc.max_pose_step;
+            st.max_landmark_step = sc.max_landmark_step;
+            st.nonprojectable_after = (uint32_t)nonproj_after;
+            if (!accepted) {  // lm.rs:890-947
+                lambda = std::min(lambda * o.lambda_increase_factor, o.max_lambda);
+                st.lambda = lambda;
+                st.step_accepted = 0;
+                push(st);
+                if (!o.has_initial_lambda) break;
+                if (lambda >= o.max_lambda) break;
+                continue;
+            }
+            st.lambda = lambda;  // lm.rs:949-991
+            st.step_accepted = 1;
+            push(st);
+            std::swap(P.poses, P.poses_trial);
+            std::swap(P.lms, P.lms_trial);
+            need_lin = true;
+            current_cost = cost_after;
+            current_nonproj = nonproj_after;
+            if (o.has_initial_lambda) lambda = std::max(lambda * o.lambda_decrease_factor, o.min_lambda);
+            if (sc.max_pose_step < o.step_tolerance && sc.max_landmark_step < o.step_tolerance) {
+                converged = true;
+                break;
+            }
+            if (std::fabs(cost_before - cost_after) < o.cost_tolerance) {
+                converged = true;
+                break;
+            }
+            if (o.has_relative_cost_tolerance && std::isfinite(o.relative_cost_tolerance) &&
+                o.relative_cost_tolerance >= 0.0 &&
+                (cost_before - cost_after) / std::max(std::fabs(cost_before), std::numeric_limits<double>::epsilon()) <
+                    o.relative_cost_tolerance) {
+                converged = true;
+                break;
+            }
+        }
+        hip_check(hipEventRecord(ctx.ev_stop, s), "event");
+        std::vector<double> poses(m.poses.size()), lms(m.lms.size());
+        hip_check(hipMemcpyAsync(poses.data(), P.poses, poses.size() * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+        hip_check(hipMemcpyAsync(lms.data(), P.lms, lms.size() * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+        hip_check(hipStreamSynchronize(s), "sync");
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, ctx.ev_start, ctx.ev_stop), "event");
+        // keep the accepted state in buffer slot order for the next call
+        if (P.poses != (double*)ws.bufs[B_POSES].ptr) {
+            std::swap(ws.bufs[B_POSES], ws.bufs[B_POSES_T]);
+            std::swap(ws.bufs[B_LMS], ws.bufs[B_LMS_T]);
+        }
+        if (poses_out)
+            for (size_t i = 0; i < m.pose_slot_of_input.size(); ++i)
+                std::memcpy(poses_out + 7 * i, poses.data() + 7 * (size_t)m.pose_slot_of_input[i], 7 * sizeof(double));
+        if (landmarks_out)
+            for (size_t i = 0; i < m.lm_slot_of_input.size(); ++i)
+                std::memcpy(landmarks_out + 3 * i, lms.data() + 3 * (size_t)m.lm_slot_of_input[i], 3 * sizeof(double));
+        res->initial_cost = initial_cost;
+        res->final_cost = current_cost;
+        res->num_iterations = std::min(written, cap);
+        res->converged = converged;
+        res->num_observations_used = m.num_obs;
+        res->total_pcg_iterations = total_pcg;
+        res->linearizations = linearizations;
+        res->device_seconds = ms * 1e-3;
+        (void)norm6;
+        return VBA_OK;
+    } catch (const Fail& f) {
+        ctx.last_error = f.msg;
+        (void)hipStreamSynchronize(s);
+        return f.code;
+    }
+}
+
+}  // namespace vba

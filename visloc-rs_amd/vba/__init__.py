@@ -1,0 +1,14 @@
+"""Bundle adjustment on AMD MI355X: Python host side of libvba.so.
+
+Drop-in for the Levenberg–Marquardt Schur-complement path of visloc-rs
+(`BundleAdjustment::optimize`, `BaAccelerator`); see include/vba.h.
+"""
+from . import abi
+from .abi import library
+from .api import (BundleProblem, Device, SolveOutput, Synthetic, camera, camera_centres, check,
+                  default_options, run_solver, synth_params, synthesize)
+
+__all__ = [
+    "abi", "library", "BundleProblem", "Device", "SolveOutput", "Synthetic", "camera",
+    "camera_centres", "check", "default_options", "run_solver", "synth_params", "synthesize",
+]

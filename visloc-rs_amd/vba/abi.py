@@ -1,0 +1,237 @@
+"""ctypes mirror of include/vba.h and include/vba_synth.h.
+
+The structures below must match the C layouts field for field; the library
+is libvba.so built in-tree by `make -C visloc-rs_amd`. Loading fails loudly
+when it is missing: there is no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+ABI_VERSION = 1
+
+OK = 0
+DECLINED = 1
+ERR_NO_POSES = -1
+ERR_NO_LANDMARKS = -2
+ERR_NO_OBSERVATIONS = -3
+ERR_ALL_POSES_FIXED = -4
+ERR_MISSING_POSE = -5
+ERR_MISSING_LANDMARK = -6
+ERR_UNSUPPORTED_CAMERA_MODEL = -7
+ERR_MISSING_STEREO_BASELINE = -8
+ERR_SINGULAR_SYSTEM = -13
+ERR_INELIGIBLE = -20
+ERR_INVALID_CONFIGURATION = -21
+ERR_LINEAR_SOLVE = -22
+ERR_INVALID_ARGUMENT = -100
+ERR_DEVICE = -101
+ERR_OUT_OF_MEMORY = -102
+
+CAMERA_PINHOLE = 0
+CAMERA_SIMPLE_PINHOLE = 1
+CAMERA_SIMPLE_RADIAL = 2
+CAMERA_RADIAL = 3
+CAMERA_OPENCV = 4
+
+KERNEL_NONE = 0
+KERNEL_HUBER = 1
+KERNEL_CAUCHY = 2
+
+SOLVER_DENSE = 0
+SOLVER_MATRIX_FREE = 2
+SOLVER_PCG_EXPLICIT = 3
+
+PRECISION_FP64 = 0
+PRECISION_MIXED = 1
+
+PCG_MAXIT_REJECT = 0
+PCG_MAXIT_ACCEPT_ITERATE = 1
+
+FEATURE_STEREO_OBSERVATIONS = 1 << 0
+FEATURE_GENERAL_STEREO = 1 << 1
+FEATURE_PRIORS = 1 << 2
+FEATURE_INERTIAL = 1 << 3
+FEATURE_FIXED_ROTATIONS = 1 << 4
+FEATURE_OBSERVATION_WEIGHTS = 1 << 5
+
+SYNTH_CIRCLE = 1
+SYNTH_LOOP = 2
+SYNTH_STEREO_WINDOW = 3
+SYNTH_POSE_ONLY = 4
+SYNTH_CITY = 5
+
+P64 = C.POINTER(C.c_double)
+PU64 = C.POINTER(C.c_uint64)
+PU32 = C.POINTER(C.c_uint32)
+PU8 = C.POINTER(C.c_uint8)
+
+
+class Camera(C.Structure):
+    _fields_ = [
+        ("model", C.c_int32),
+        ("width", C.c_uint32),
+        ("height", C.c_uint32),
+        ("num_params", C.c_uint32),
+        ("params", C.c_double * 12),
+    ]
+
+
+class Sensor(C.Structure):
+    _fields_ = [("camera", Camera), ("sensor_from_rig", C.c_double * 7)]
+
+
+class Problem(C.Structure):
+    _fields_ = [
+        ("camera", Camera),
+        ("num_poses", C.c_uint64),
+        ("pose_ids", PU64),
+        ("poses", P64),
+        ("pose_fixed", PU8),
+        ("num_landmarks", C.c_uint64),
+        ("landmark_ids", PU64),
+        ("landmarks", P64),
+        ("landmark_fixed", PU8),
+        ("num_observations", C.c_uint64),
+        ("obs_pose_ids", PU64),
+        ("obs_landmark_ids", PU64),
+        ("obs_xy", P64),
+        ("num_rig_observations", C.c_uint64),
+        ("rig_pose_ids", PU64),
+        ("rig_landmark_ids", PU64),
+        ("rig_xy", P64),
+        ("rig_sensor", PU32),
+        ("num_sensors", C.c_uint64),
+        ("sensors", C.POINTER(Sensor)),
+        ("declined_features", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("max_iterations", C.c_uint32),
+        ("has_initial_lambda", C.c_uint32),
+        ("initial_lambda", C.c_double),
+        ("lambda_increase_factor", C.c_double),
+        ("lambda_decrease_factor", C.c_double),
+        ("max_lambda", C.c_double),
+        ("min_lambda", C.c_double),
+        ("step_tolerance", C.c_double),
+        ("cost_tolerance", C.c_double),
+        ("has_relative_cost_tolerance", C.c_uint32),
+        ("robust_kernel", C.c_uint32),
+        ("relative_cost_tolerance", C.c_double),
+        ("robust_scale", C.c_double),
+        ("linear_solver", C.c_uint32),
+        ("precision", C.c_uint32),
+        ("max_pcg_iterations", C.c_uint32),
+        ("pcg_maxit_policy", C.c_uint32),
+        ("pcg_relative_tolerance", C.c_double),
+        ("pcg_absolute_tolerance", C.c_double),
+        ("refine_intrinsics", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
+
+
+class IterationStats(C.Structure):
+    _fields_ = [
+        ("iteration", C.c_uint32),
+        ("step_accepted", C.c_uint32),
+        ("cost_before", C.c_double),
+        ("cost_after", C.c_double),
+        ("max_pose_step", C.c_double),
+        ("max_landmark_step", C.c_double),
+        ("lambda_", C.c_double),
+        ("pcg_iterations", C.c_int32),
+        ("nonprojectable_after", C.c_uint32),
+        ("pcg_residual_norm", C.c_double),
+        ("pcg_target", C.c_double),
+    ]
+
+
+class Result(C.Structure):
+    _fields_ = [
+        ("initial_cost", C.c_double),
+        ("final_cost", C.c_double),
+        ("num_iterations", C.c_uint32),
+        ("converged", C.c_uint32),
+        ("error_id", C.c_uint64),
+        ("num_observations_used", C.c_uint64),
+        ("total_pcg_iterations", C.c_uint32),
+        ("linearizations", C.c_uint32),
+        ("device_seconds", C.c_double),
+    ]
+
+
+class SynthParams(C.Structure):
+    _fields_ = [
+        ("kind", C.c_uint32),
+        ("num_cameras", C.c_uint32),
+        ("seed", C.c_uint64),
+        ("num_landmarks", C.c_uint64),
+        ("pixel_noise", C.c_double),
+        ("rotation_noise", C.c_double),
+        ("translation_noise", C.c_double),
+        ("landmark_noise", C.c_double),
+        ("outlier_fraction", C.c_double),
+        ("mean_track_length", C.c_double),
+        ("min_track_length", C.c_uint32),
+        ("max_track_length", C.c_uint32),
+        ("scene_scale", C.c_double),
+        ("focal", C.c_double),
+        ("distortion", C.c_double),
+        ("width", C.c_uint32),
+        ("height", C.c_uint32),
+        ("fixed_cameras", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
+
+
+# Every symbol include/vba.h and include/vba_synth.h declare, with its
+# signature (restype, argtypes).
+SIGNATURES = {
+    "vba_abi_version": (C.c_uint32, []),
+    "vba_create": (C.c_void_p, [C.c_int32]),
+    "vba_destroy": (None, [C.c_void_p]),
+    "vba_last_error": (C.c_char_p, [C.c_void_p]),
+    "vba_default_options": (None, [C.POINTER(Options)]),
+    "vba_check": (C.c_int32, [C.POINTER(Problem), C.POINTER(Options)]),
+    "vba_optimize": (
+        C.c_int32,
+        [C.c_void_p, C.POINTER(Problem), C.POINTER(Options), P64, P64, C.POINTER(IterationStats),
+         C.c_uint32, C.POINTER(Result)],
+    ),
+    "vba_synth_default": (None, [C.c_uint32, C.POINTER(SynthParams)]),
+    "vba_synth_generate": (C.c_void_p, [C.POINTER(SynthParams)]),
+    "vba_synth_free": (None, [C.c_void_p]),
+    "vba_synth_view": (None, [C.c_void_p, C.POINTER(Problem)]),
+    "vba_synth_truth": (None, [C.c_void_p, C.POINTER(P64), C.POINTER(P64)]),
+    "vba_synth_scene_extent": (C.c_double, [C.c_void_p]),
+}
+
+PACKAGE_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIBRARY_PATH = os.path.join(PACKAGE_ROOT, "libvba.so")
+
+_lib = None
+
+
+def library() -> C.CDLL:
+    """Load libvba.so (built in-tree); raises when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBRARY_PATH):
+            raise RuntimeError(
+                f"{LIBRARY_PATH} is missing: build it with `make -C visloc-rs_amd` "
+                "(or __graft_entry__.build()); there is no fallback implementation"
+            )
+        lib = C.CDLL(LIBRARY_PATH)
+        for name, (restype, argtypes) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = restype
+            fn.argtypes = argtypes
+        if lib.vba_abi_version() != ABI_VERSION:
+            raise RuntimeError("libvba.so ABI version mismatch")
+        _lib = lib
+    return _lib

@@ -1,0 +1,295 @@
+"""Python host side of the bundle-adjustment library.
+
+`BundleProblem` owns numpy arrays laid out as the C view `vba_problem`
+(the caller-side image of visloc-rs `BundleAdjustment`,
+pipelines/slam/src/bundle/mod.rs:529-622); `Device.optimize` is
+`BaAccelerator::optimize` (pipelines/slam/src/ba_accel.rs:22-32) through the
+C ABI and returns the reference's `BaResult` fields.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from . import abi
+
+
+def camera(model: int, params, width: int = 0, height: int = 0) -> abi.Camera:
+    cam = abi.Camera()
+    cam.model = model
+    cam.width = width
+    cam.height = height
+    params = list(params)
+    cam.num_params = len(params)
+    for i, v in enumerate(params):
+        cam.params[i] = float(v)
+    return cam
+
+
+def _ptr(a: Optional[np.ndarray], ctype):
+    if a is None or a.size == 0:
+        return C.cast(None, C.POINTER(ctype))
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+@dataclass
+class BundleProblem:
+    camera: abi.Camera
+    pose_ids: np.ndarray           # uint64 [P]
+    poses: np.ndarray              # float64 [P, 7] (qw qx qy qz tx ty tz), world->camera
+    pose_fixed: np.ndarray         # uint8 [P]
+    landmark_ids: np.ndarray       # uint64 [L]
+    landmarks: np.ndarray          # float64 [L, 3]
+    landmark_fixed: np.ndarray     # uint8 [L]
+    obs_pose_ids: np.ndarray       # uint64 [N]
+    obs_landmark_ids: np.ndarray   # uint64 [N]
+    obs_xy: np.ndarray             # float64 [N, 2]
+    rig_pose_ids: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint64))
+    rig_landmark_ids: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint64))
+    rig_xy: np.ndarray = field(default_factory=lambda: np.zeros((0, 2)))
+    rig_sensor: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint32))
+    sensors: List[abi.Sensor] = field(default_factory=list)
+    declined_features: int = 0
+
+    def __post_init__(self):
+        self.pose_ids = np.ascontiguousarray(self.pose_ids, np.uint64)
+        self.poses = np.ascontiguousarray(self.poses, np.float64).reshape(-1, 7)
+        self.pose_fixed = np.ascontiguousarray(self.pose_fixed, np.uint8)
+        self.landmark_ids = np.ascontiguousarray(self.landmark_ids, np.uint64)
+        self.landmarks = np.ascontiguousarray(self.landmarks, np.float64).reshape(-1, 3)
+        self.landmark_fixed = np.ascontiguousarray(self.landmark_fixed, np.uint8)
+        self.obs_pose_ids = np.ascontiguousarray(self.obs_pose_ids, np.uint64)
+        self.obs_landmark_ids = np.ascontiguousarray(self.obs_landmark_ids, np.uint64)
+        self.obs_xy = np.ascontiguousarray(self.obs_xy, np.float64).reshape(-1, 2)
+        self.rig_pose_ids = np.ascontiguousarray(self.rig_pose_ids, np.uint64)
+        self.rig_landmark_ids = np.ascontiguousarray(self.rig_landmark_ids, np.uint64)
+        self.rig_xy = np.ascontiguousarray(self.rig_xy, np.float64).reshape(-1, 2)
+        self.rig_sensor = np.ascontiguousarray(self.rig_sensor, np.uint32)
+        self._sensor_array = (abi.Sensor * max(1, len(self.sensors)))(*self.sensors)
+
+    @property
+    def num_observations(self) -> int:
+        return len(self.obs_pose_ids) + len(self.rig_pose_ids)
+
+    def view(self) -> abi.Problem:
+        """The C view; valid while this object is alive and unmodified."""
+        v = abi.Problem()
+        v.camera = self.camera
+        v.num_poses = len(self.pose_ids)
+        v.pose_ids = _ptr(self.pose_ids, C.c_uint64)
+        v.poses = _ptr(self.poses, C.c_double)
+        v.pose_fixed = _ptr(self.pose_fixed, C.c_uint8)
+        v.num_landmarks = len(self.landmark_ids)
+        v.landmark_ids = _ptr(self.landmark_ids, C.c_uint64)
+        v.landmarks = _ptr(self.landmarks, C.c_double)
+        v.landmark_fixed = _ptr(self.landmark_fixed, C.c_uint8)
+        v.num_observations = len(self.obs_pose_ids)
+        v.obs_pose_ids = _ptr(self.obs_pose_ids, C.c_uint64)
+        v.obs_landmark_ids = _ptr(self.obs_landmark_ids, C.c_uint64)
+        v.obs_xy = _ptr(self.obs_xy, C.c_double)
+        v.num_rig_observations = len(self.rig_pose_ids)
+        v.rig_pose_ids = _ptr(self.rig_pose_ids, C.c_uint64)
+        v.rig_landmark_ids = _ptr(self.rig_landmark_ids, C.c_uint64)
+        v.rig_xy = _ptr(self.rig_xy, C.c_double)
+        v.rig_sensor = _ptr(self.rig_sensor, C.c_uint32)
+        v.num_sensors = len(self.sensors)
+        v.sensors = C.cast(self._sensor_array, C.POINTER(abi.Sensor)) if self.sensors else None
+        v.declined_features = self.declined_features
+        return v
+
+    def copy(self) -> "BundleProblem":
+        return BundleProblem(
+            camera=self.camera, pose_ids=self.pose_ids.copy(), poses=self.poses.copy(),
+            pose_fixed=self.pose_fixed.copy(), landmark_ids=self.landmark_ids.copy(),
+            landmarks=self.landmarks.copy(), landmark_fixed=self.landmark_fixed.copy(),
+            obs_pose_ids=self.obs_pose_ids.copy(), obs_landmark_ids=self.obs_landmark_ids.copy(),
+            obs_xy=self.obs_xy.copy(), rig_pose_ids=self.rig_pose_ids.copy(),
+            rig_landmark_ids=self.rig_landmark_ids.copy(), rig_xy=self.rig_xy.copy(),
+            rig_sensor=self.rig_sensor.copy(), sensors=list(self.sensors),
+            declined_features=self.declined_features)
+
+    def with_state(self, poses: np.ndarray, landmarks: np.ndarray) -> "BundleProblem":
+        p = self.copy()
+        p.poses = np.ascontiguousarray(poses, np.float64).reshape(-1, 7)
+        p.landmarks = np.ascontiguousarray(landmarks, np.float64).reshape(-1, 3)
+        return p
+
+
+def _np_from(ptr, n, dtype, shape):
+    if n == 0 or not ptr:
+        return np.zeros(shape, dtype)
+    buf = np.ctypeslib.as_array(ptr, shape=(int(np.prod(shape)),))
+    return np.array(buf, dtype=dtype).reshape(shape)
+
+
+@dataclass
+class Synthetic:
+    problem: BundleProblem
+    truth_poses: np.ndarray
+    truth_landmarks: np.ndarray
+    scene_extent: float
+    params: abi.SynthParams
+
+
+def synth_params(kind: int, **overrides) -> abi.SynthParams:
+    lib = abi.library()
+    p = abi.SynthParams()
+    lib.vba_synth_default(kind, C.byref(p))
+    for k, v in overrides.items():
+        if not hasattr(p, k):
+            raise AttributeError(k)
+        setattr(p, k, v)
+    return p
+
+
+def synthesize(kind: int, **overrides) -> Synthetic:
+    """Generate a seeded synthetic problem (include/vba_synth.h) and copy it
+    into numpy arrays."""
+    lib = abi.library()
+    params = synth_params(kind, **overrides)
+    h = lib.vba_synth_generate(C.byref(params))
+    if not h:
+        raise ValueError(lib.vba_last_error(None).decode())
+    try:
+        v = abi.Problem()
+        lib.vba_synth_view(h, C.byref(v))
+        P, L = v.num_poses, v.num_landmarks
+        N, R = v.num_observations, v.num_rig_observations
+        sensors = [abi.Sensor.from_buffer_copy(v.sensors[i]) for i in range(v.num_sensors)]
+        prob = BundleProblem(
+            camera=abi.Camera.from_buffer_copy(v.camera),
+            pose_ids=_np_from(v.pose_ids, P, np.uint64, (P,)),
+            poses=_np_from(v.poses, P, np.float64, (P, 7)),
+            pose_fixed=_np_from(v.pose_fixed, P, np.uint8, (P,)),
+            landmark_ids=_np_from(v.landmark_ids, L, np.uint64, (L,)),
+            landmarks=_np_from(v.landmarks, L, np.float64, (L, 3)),
+            landmark_fixed=_np_from(v.landmark_fixed, L, np.uint8, (L,)),
+            obs_pose_ids=_np_from(v.obs_pose_ids, N, np.uint64, (N,)),
+            obs_landmark_ids=_np_from(v.obs_landmark_ids, N, np.uint64, (N,)),
+            obs_xy=_np_from(v.obs_xy, N, np.float64, (N, 2)),
+            rig_pose_ids=_np_from(v.rig_pose_ids, R, np.uint64, (R,)),
+            rig_landmark_ids=_np_from(v.rig_landmark_ids, R, np.uint64, (R,)),
+            rig_xy=_np_from(v.rig_xy, R, np.float64, (R, 2)),
+            rig_sensor=_np_from(v.rig_sensor, R, np.uint32, (R,)),
+            sensors=sensors)
+        tp, tl = abi.P64(), abi.P64()
+        lib.vba_synth_truth(h, C.byref(tp), C.byref(tl))
+        truth_poses = _np_from(tp, P, np.float64, (P, 7))
+        truth_lms = _np_from(tl, L, np.float64, (L, 3))
+        extent = lib.vba_synth_scene_extent(h)
+    finally:
+        lib.vba_synth_free(h)
+    return Synthetic(prob, truth_poses, truth_lms, extent, params)
+
+
+def default_options(**overrides) -> abi.Options:
+    """BaConfig::default() (+ MatrixFreeBaOptions::default()), with overrides."""
+    o = abi.Options()
+    abi.library().vba_default_options(C.byref(o))
+    for k, v in overrides.items():
+        if not hasattr(o, k):
+            raise AttributeError(k)
+        setattr(o, k, v)
+    return o
+
+
+@dataclass
+class SolveOutput:
+    status: int
+    result: abi.Result
+    iterations: List[dict]
+    poses: Optional[np.ndarray]
+    landmarks: Optional[np.ndarray]
+    message: str = ""
+
+    @property
+    def accepted(self) -> List[bool]:
+        return [bool(it["step_accepted"]) for it in self.iterations]
+
+
+def stats_list(stats, n) -> List[dict]:
+    out = []
+    for i in range(n):
+        s = stats[i]
+        out.append({f[0].rstrip("_"): getattr(s, f[0]) for f in abi.IterationStats._fields_})
+    return out
+
+
+def run_solver(fn, problem: BundleProblem, options: abi.Options, cap: int = 1024, **extra) -> SolveOutput:
+    poses = np.empty_like(problem.poses)
+    lms = np.empty_like(problem.landmarks)
+    stats = (abi.IterationStats * cap)()
+    res = abi.Result()
+    view = problem.view()
+    st = fn(view, options, poses, lms, stats, cap, res, **extra)
+    n = min(res.num_iterations, cap)
+    ok = st == abi.OK
+    return SolveOutput(st, res, stats_list(stats, n) if ok else [], poses if ok else None, lms if ok else None)
+
+
+class Device:
+    """A libvba context on one HIP device (vba_create / vba_destroy)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = abi.library()
+        self.ctx = self.lib.vba_create(device)
+        if not self.ctx:
+            raise RuntimeError("vba_create failed: " + self.lib.vba_last_error(None).decode())
+
+    def close(self):
+        if self.ctx:
+            self.lib.vba_destroy(self.ctx)
+            self.ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def last_error(self) -> str:
+        return self.lib.vba_last_error(self.ctx).decode()
+
+    def optimize(self, problem: BundleProblem, options: abi.Options, cap: int = 1024) -> SolveOutput:
+        def call(view, opts, poses, lms, stats, cap_, res):
+            return self.lib.vba_optimize(
+                self.ctx, C.byref(view), C.byref(opts), poses.ctypes.data_as(abi.P64),
+                lms.ctypes.data_as(abi.P64), stats, cap_, C.byref(res))
+
+        out = run_solver(call, problem, options, cap)
+        if out.status != abi.OK:
+            out.message = self.last_error()
+        return out
+
+
+def check(problem: BundleProblem, options: abi.Options) -> int:
+    lib = abi.library()
+    view = problem.view()
+    return lib.vba_check(C.byref(view), C.byref(options))
+
+
+def camera_centres(poses: np.ndarray) -> np.ndarray:
+    """Pose::camera_center_world (crates/core/src/geometry/pose.rs:31-33):
+    C = -R^T t for world->camera (q, t)."""
+    poses = np.asarray(poses, np.float64).reshape(-1, 7)
+    w, x, y, z = poses[:, 0], poses[:, 1], poses[:, 2], poses[:, 3]
+    R = np.empty((len(poses), 3, 3))
+    R[:, 0, 0] = w * w + x * x - y * y - z * z
+    R[:, 0, 1] = 2 * (x * y - w * z)
+    R[:, 0, 2] = 2 * (w * y + x * z)
+    R[:, 1, 0] = 2 * (x * y + w * z)
+    R[:, 1, 1] = w * w - x * x + y * y - z * z
+    R[:, 1, 2] = 2 * (y * z - w * x)
+    R[:, 2, 0] = 2 * (x * z - w * y)
+    R[:, 2, 1] = 2 * (w * x + y * z)
+    R[:, 2, 2] = w * w - x * x - y * y + z * z
+    return -np.einsum("nji,nj->ni", R, poses[:, 4:7])
