@@ -1,0 +1,234 @@
+#!/usr/bin/env python3
+"""Bundle-adjustment throughput on one MI355X.
+
+Workload: the loop problem — 1,200 forward-facing pinhole cameras (f=800,
+1024x768) on a closed 200 m loop, 400,000 landmarks in a 20 m corridor, each
+seen by a run of consecutive cameras (~3.5M observations), 0.7 px noise,
+Huber delta = 1 px, matrix-free block-Jacobi PCG (rtol 1e-4, max 100),
+30 LM iterations. A step is one full solve (30 LM iterations) from the same
+initial state with the problem resident in HBM (vba_prepare once, then
+vba_solve_prepared per step).
+
+value = observations x LM iterations executed (accepted + rejected), summed
+over the timed steps, / wall seconds of the timed steps. The mixed-precision
+solve is checked against the plain FP64 solve (final cost within 1e-6
+relative) in the same run.
+"""
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "visloc-rs_amd"))
+
+import numpy as np  # noqa: E402
+
+import vba  # noqa: E402
+from vba import abi  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "BA obs/s + LM iters/s at ≤1e-6 rel cost vs FP64; served p50/p99 solve latency"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def loop_options(precision, policy=abi.PCG_MAXIT_ACCEPT_ITERATE):
+    """PCG rtol 1e-4 / max 100 are the reference GPU backend's GpuBaSettings
+    defaults (crates/ba-gpu/src/solver.rs:36-48); a solve that reaches the cap
+    uses its last iterate, as that backend does (solver.rs:497-508)."""
+    return vba.default_options(linear_solver=abi.SOLVER_MATRIX_FREE, max_iterations=30, max_pcg_iterations=100,
+                               pcg_relative_tolerance=1e-4, robust_kernel=abi.KERNEL_HUBER, robust_scale=1.0,
+                               precision=precision, pcg_maxit_policy=policy)
+
+
+def cpu_model():
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return platform.processor()
+
+
+def cpu_baseline(problem, options):
+    """The CPU FP64 oracle (a restatement of the reference's Rust solver) on a
+    bounded sample of the same workload: its first LM iteration (assembly,
+    Schur operator, PCG up to 100 iterations, back-substitution, trial cost)
+    on one core."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    o = vba.default_options(**{f: getattr(options, f) for f, _ in abi.Options._fields_})
+    o.max_iterations = 1
+    o.precision = abi.PRECISION_FP64
+    t0 = time.perf_counter()
+    out = oracle.optimize(problem, o, threads=1)
+    dt = time.perf_counter() - t0
+    iters = max(1, len(out.iterations))
+    obs = problem.num_observations * iters
+    return {
+        "value": obs / dt,
+        "unit": "obs/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"1 LM iteration ({out.iterations[0]['pcg_iterations'] if out.iterations else 0} PCG iterations) "
+                  f"of the full {problem.num_observations}-observation loop workload, FP64, serial; "
+                  f"{dt:.1f} s on {cpu_model()} ({os.cpu_count()} host threads visible)",
+        "seconds": dt,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--landmarks", type=int, default=400_000, help="landmark count (default: full size)")
+    ap.add_argument("--pcg-policy", choices=["accept", "reject"], default="accept",
+                    help="at the PCG iteration cap: use the iterate (GPU backend) or reject the step (CPU "
+                         "matrix-free entry)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (run with --gpus 1).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    t0 = time.perf_counter()
+    s = vba.synthesize(abi.SYNTH_LOOP, num_landmarks=args.landmarks)
+    problem = s.problem
+    N = problem.num_observations
+    log(f"bench: loop workload {len(problem.pose_ids)} cameras, {len(problem.landmark_ids)} landmarks, "
+        f"{N} observations (generated in {time.perf_counter() - t0:.1f} s)")
+    precision = abi.PRECISION_MIXED if args.precision == "mixed" else abi.PRECISION_FP64
+    policy = abi.PCG_MAXIT_ACCEPT_ITERATE if args.pcg_policy == "accept" else abi.PCG_MAXIT_REJECT
+    opts = loop_options(precision, policy)
+    dev = vba.Device(0)
+    t0 = time.perf_counter()
+    prep = dev.prepare(problem, opts)
+    log(f"bench: prepared (layout + upload) in {time.perf_counter() - t0:.2f} s")
+
+    for w in range(args.warmup):
+        out = prep.solve(opts)
+        if out.status != abi.OK:
+            raise SystemExit(f"warmup solve failed: {out.status} {out.message}")
+        log(f"bench: warmup {w}: {out.result.num_iterations} LM iterations, {out.result.total_pcg_iterations} PCG, "
+            f"cost {out.result.initial_cost:.6e} -> {out.result.final_cost:.6e}, "
+            f"device {out.result.device_seconds:.3f} s")
+
+    dev.profile(True)
+    lm_iters = 0
+    pcg_iters = 0
+    final_cost = None
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        out = prep.solve(opts)
+        if out.status != abi.OK:
+            raise SystemExit(f"timed solve failed: {out.status} {out.message}")
+        lm_iters += out.result.num_iterations
+        pcg_iters += out.result.total_pcg_iterations
+        final_cost = out.result.final_cost
+    wall = time.perf_counter() - t0
+    dev.profile(False, reset=False)
+    accepted = sum(it["step_accepted"] for it in out.iterations)
+    log(f"bench: {args.steps} steps in {wall:.3f} s; last solve {out.result.num_iterations} LM iterations "
+        f"({accepted} accepted), {out.result.total_pcg_iterations} PCG iterations")
+
+    kernels = {}
+    for name in ("linearize", "lm_pass", "cam_pass", "pcg_update", "cost"):
+        q = dev.profile_query(name)
+        if q["launches"]:
+            q["avg_us"] = 1e6 * q["seconds"] / q["launches"]
+            q["achieved_gbs"] = q["bytes"] / q["seconds"] / 1e9 if q["seconds"] > 0 else 0.0
+            q["bytes_per_launch"] = q["bytes"] / q["launches"]
+        kernels[name] = q
+        log(f"bench: kernel {name}: {q}")
+    dominant = max((k for k in kernels if kernels[k]["bytes"] > 0), key=lambda k: kernels[k]["seconds"])
+    dk = kernels[dominant]
+    traffic = None
+    traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(traffic_file):
+        with open(traffic_file) as f:
+            tr = json.load(f)
+        entry = tr.get(f"{dominant}/{args.precision}")
+        if entry:
+            traffic = entry.get("hbm_bytes_per_launch")
+    roofline = {
+        "kernel": dominant,
+        "bound": "hbm",
+        "achieved": round(dk["achieved_gbs"], 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(dk["achieved_gbs"] / HBM_PEAK_GBS, 4),
+        "traffic": traffic,
+        "algorithmic_bytes_per_launch": dk["bytes_per_launch"],
+        "avg_launch_us": round(dk["avg_us"], 2),
+    }
+
+    # Parity of the measured solve against the plain FP64 solve (untimed).
+    ref_rel = None
+    fp64_obs_per_s = None
+    if precision == abi.PRECISION_MIXED:
+        f64 = loop_options(abi.PRECISION_FP64, policy)
+        t1 = time.perf_counter()
+        ref = prep.solve(f64)
+        t_ref = time.perf_counter() - t1
+        if ref.status == abi.OK:
+            ref_rel = abs(final_cost - ref.result.final_cost) / abs(ref.result.final_cost)
+            fp64_obs_per_s = N * ref.result.num_iterations / t_ref
+        log(f"bench: fp64 solve final cost {ref.result.final_cost:.9e}, mixed {final_cost:.9e}, rel {ref_rel:.3e}, "
+            f"fp64 {t_ref:.3f} s")
+
+    value = N * lm_iters / wall
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "obs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000.0 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if precision == abi.PRECISION_MIXED else "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "loop: 1200 cameras on a 200 m closed loop, 400k landmarks, "
+                        f"{N} observations, Huber 1 px, matrix-free block-Jacobi PCG rtol 1e-4 max 100, "
+                        "30 LM iterations",
+            "observations": N,
+            "precision": args.precision,
+            "pcg_at_cap": args.pcg_policy,
+        },
+        "lm_iters_per_s": lm_iters / wall,
+        "pcg_iters_per_s": pcg_iters / wall,
+        "final_cost": final_cost,
+        "rel_cost_vs_fp64": ref_rel,
+        "fp64_obs_per_s": fp64_obs_per_s,
+        "speedup_vs_fp64": (value / fp64_obs_per_s) if fp64_obs_per_s else None,
+        "roofline": roofline,
+        "kernels": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                    for k, v in kernels.items()},
+    }
+    prep.free()
+    dev.close()
+    if not args.no_cpu_baseline:
+        log("bench: CPU baseline (oracle, 1 LM iteration) ...")
+        result["cpu_baseline"] = cpu_baseline(problem, opts)
+    else:
+        result["cpu_baseline"] = None
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

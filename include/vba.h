@@ -226,6 +226,8 @@ typedef struct vba_result {
     double device_seconds;           /* device time of the solve (HIP events) */
 } vba_result;
 
+typedef struct vba_context vba_context;
+
 /* ---- lifecycle (the reference's *_abi_version/_create/_destroy/_last_error) */
 uint32_t vba_abi_version(void);
 /* Create a context on a HIP device ordinal. NULL on failure (see
@@ -234,8 +236,6 @@ struct vba_context* vba_create(int32_t device);
 void vba_destroy(struct vba_context* ctx);
 /* Last error message of ctx, or of the calling thread when ctx is NULL. */
 const char* vba_last_error(const struct vba_context* ctx);
-
-typedef struct vba_context vba_context;
 
 /* BaConfig::default() + MatrixFreeBaOptions::default()
  * (mod.rs:729-752, 789-798). */
@@ -256,6 +256,37 @@ int32_t vba_optimize(vba_context* ctx, const vba_problem* problem,
                      const vba_options* options, double* poses_out,
                      double* landmarks_out, vba_iteration_stats* stats,
                      uint32_t stats_capacity, vba_result* result);
+
+/* ---- device-resident problems ------------------------------------------
+ * No counterpart in the reference, whose GPU backend uploads on every call
+ * (solver.rs:253-385). vba_optimize is vba_prepare + vba_solve_prepared +
+ * vba_prepared_state + vba_prepared_free. Callers that solve one problem
+ * repeatedly (benchmarks, the serving loop) prepare it once: validation,
+ * layout and the host->device copy happen in vba_prepare; every
+ * vba_solve_prepared starts from the prepared initial state and leaves its
+ * final state on the device. Status codes are those of vba_optimize. */
+typedef struct vba_prepared vba_prepared;
+int32_t vba_prepare(vba_context* ctx, const vba_problem* problem, const vba_options* options,
+                    vba_prepared** out, vba_result* result);
+int32_t vba_solve_prepared(vba_context* ctx, vba_prepared* prepared, const vba_options* options,
+                           vba_iteration_stats* stats, uint32_t stats_capacity,
+                           vba_result* result);
+/* Final state of the last solve (the initial state before any), in the
+ * caller's input order. */
+int32_t vba_prepared_state(vba_context* ctx, const vba_prepared* prepared, double* poses_out,
+                           double* landmarks_out);
+void vba_prepared_free(vba_context* ctx, vba_prepared* prepared);
+
+/* ---- per-kernel device time ----------------------------------------------
+ * While enabled, HIP events are recorded around the launches of the kernel
+ * families "linearize", "lm_pass", "cam_pass", "pcg_update" and "cost" on
+ * the context's stream. vba_profile_query returns the launches that did
+ * work, their summed device seconds and the algorithmic bytes they moved
+ * (the minimum HBM traffic of the data layout, summed over launches). */
+void vba_profile_enable(vba_context* ctx, int32_t enable);
+void vba_profile_reset(vba_context* ctx);
+int32_t vba_profile_query(vba_context* ctx, const char* kernel, uint64_t* launches,
+                          double* seconds, double* bytes);
 
 #ifdef __cplusplus
 } /* extern "C" */

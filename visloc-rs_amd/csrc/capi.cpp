@@ -11,6 +11,49 @@
 #include "solver.hpp"
 
 struct vba_context : vba::Context {};
+struct vba_prepared : vba::Prepared {};
+
+namespace {
+
+// Validate and lay out a problem into `prep` (host side of vba_prepare).
+int32_t build_prepared(vba_context* ctx, const vba_problem& problem, const vba_options& options,
+                       vba_prepared* prep, vba_result* result) {
+    std::string reason;
+    int32_t st = vba::check_support(problem, options, &reason);
+    if (st != VBA_OK) {
+        ctx->last_error = "declined: " + reason;
+        return st;
+    }
+    if (hipSetDevice(ctx->device) != hipSuccess) {
+        ctx->last_error = "hipSetDevice failed";
+        return VBA_ERR_DEVICE;
+    }
+    uint64_t error_id = 0;
+    std::string msg;
+    try {
+        st = vba::build_model(problem, &prep->m, &error_id, &msg);
+    } catch (const std::bad_alloc&) {
+        ctx->last_error = "out of host memory";
+        return VBA_ERR_OUT_OF_MEMORY;
+    }
+    if (result) result->error_id = error_id;
+    if (st != VBA_OK) {
+        ctx->last_error = msg.empty() ? "invalid problem" : msg;
+        return st;
+    }
+    if (options.linear_solver != VBA_SOLVER_DENSE) {
+        st = vba::validate_matrix_free(prep->m, options, &msg);
+        if (st != VBA_OK) {
+            ctx->last_error = msg;
+            return st;
+        }
+    }
+    st = vba::prepare(*ctx, *prep, &msg);
+    if (st != VBA_OK) ctx->last_error = msg;
+    return st;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -50,7 +93,7 @@ void vba_destroy(vba_context* ctx) {
         std::lock_guard<std::mutex> lock(ctx->mu);
         (void)hipSetDevice(ctx->device);
         (void)hipStreamSynchronize(ctx->stream);
-        vba::release_workspace(ctx->ws);
+        vba::release_context(*ctx);
         if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
         if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
         if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -108,43 +151,120 @@ int32_t vba_optimize(vba_context* ctx, const vba_problem* problem, const vba_opt
     std::lock_guard<std::mutex> lock(ctx->mu);
     std::memset(result, 0, sizeof(*result));
     ctx->last_error.clear();
-    std::string reason;
-    int32_t st = vba::check_support(*problem, *options, &reason);
-    if (st != VBA_OK) {
-        ctx->last_error = "declined: " + reason;
-        return st;
+    vba_prepared prep;
+    int32_t st = build_prepared(ctx, *problem, *options, &prep, result);
+    if (st == VBA_OK) {
+        try {
+            st = vba::run(*ctx, prep, *options, stats, stats_capacity, result);
+        } catch (const std::bad_alloc&) {
+            ctx->last_error = "out of host memory";
+            st = VBA_ERR_OUT_OF_MEMORY;
+        }
     }
-    if (hipSetDevice(ctx->device) != hipSuccess) {
-        ctx->last_error = "hipSetDevice failed";
-        return VBA_ERR_DEVICE;
+    if (st == VBA_OK) st = vba::download_state(*ctx, prep, poses_out, landmarks_out);
+    vba::release(prep);
+    return st;
+}
+
+int32_t vba_prepare(vba_context* ctx, const vba_problem* problem, const vba_options* options, vba_prepared** out,
+                    vba_result* result) {
+    if (!ctx || !problem || !options || !out) {
+        vba::set_thread_error("vba_prepare: null argument");
+        return VBA_ERR_INVALID_ARGUMENT;
     }
-    vba::Model model;
-    uint64_t error_id = 0;
-    std::string msg;
-    try {
-        st = vba::build_model(*problem, &model, &error_id, &msg);
-    } catch (const std::bad_alloc&) {
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    *out = nullptr;
+    if (result) std::memset(result, 0, sizeof(*result));
+    ctx->last_error.clear();
+    std::unique_ptr<vba_prepared> prep(new (std::nothrow) vba_prepared());
+    if (!prep) {
         ctx->last_error = "out of host memory";
         return VBA_ERR_OUT_OF_MEMORY;
     }
-    result->error_id = error_id;
+    int32_t st = build_prepared(ctx, *problem, *options, prep.get(), result);
     if (st != VBA_OK) {
-        ctx->last_error = msg.empty() ? "invalid problem" : msg;
+        vba::release(*prep);
         return st;
     }
+    *out = prep.release();
+    return VBA_OK;
+}
+
+int32_t vba_solve_prepared(vba_context* ctx, vba_prepared* prepared, const vba_options* options,
+                           vba_iteration_stats* stats, uint32_t stats_capacity, vba_result* result) {
+    if (!ctx || !prepared || !options || !result) {
+        vba::set_thread_error("vba_solve_prepared: null argument");
+        return VBA_ERR_INVALID_ARGUMENT;
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    std::memset(result, 0, sizeof(*result));
+    ctx->last_error.clear();
+    std::string msg;
     if (options->linear_solver != VBA_SOLVER_DENSE) {
-        st = vba::validate_matrix_free(model, *options, &msg);
+        int32_t st = vba::validate_matrix_free(prepared->m, *options, &msg);
         if (st != VBA_OK) {
             ctx->last_error = msg;
             return st;
         }
     }
     try {
-        return vba::solve(*ctx, model, *options, poses_out, landmarks_out, stats, stats_capacity, result);
+        return vba::run(*ctx, *prepared, *options, stats, stats_capacity, result);
     } catch (const std::bad_alloc&) {
         ctx->last_error = "out of host memory";
         return VBA_ERR_OUT_OF_MEMORY;
     }
+}
+
+int32_t vba_prepared_state(vba_context* ctx, const vba_prepared* prepared, double* poses_out,
+                           double* landmarks_out) {
+    if (!ctx || !prepared) return VBA_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    return vba::download_state(*ctx, *prepared, poses_out, landmarks_out);
+}
+
+void vba_prepared_free(vba_context* ctx, vba_prepared* prepared) {
+    if (!prepared) return;
+    if (ctx) {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        (void)hipStreamSynchronize(ctx->stream);
+        vba::release(*prepared);
+    } else {
+        vba::release(*prepared);
+    }
+    delete prepared;
+}
+
+void vba_profile_enable(vba_context* ctx, int32_t enable) {
+    if (!ctx) return;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->profile = enable != 0;
+}
+
+void vba_profile_reset(vba_context* ctx) {
+    if (!ctx) return;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    for (vba::KernelProfile& k : ctx->kernels) {
+        k.launches = 0;
+        k.seconds = 0.0;
+        k.bytes = 0.0;
+    }
+}
+
+int32_t vba_profile_query(vba_context* ctx, const char* kernel, uint64_t* launches, double* seconds,
+                          double* bytes) {
+    if (!ctx || !kernel) return VBA_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    for (const vba::KernelProfile& k : ctx->kernels)
+        if (k.name == kernel) {
+            if (launches) *launches = k.launches;
+            if (seconds) *seconds = k.seconds;
+            if (bytes) *bytes = k.bytes;
+            return VBA_OK;
+        }
+    if (launches) *launches = 0;
+    if (seconds) *seconds = 0.0;
+    if (bytes) *bytes = 0.0;
+    return VBA_OK;
 }
 
 }  // extern "C"

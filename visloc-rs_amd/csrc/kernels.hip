@@ -228,7 +228,8 @@ __global__ __launch_bounds__(kBlock) void k_linearize(DevProblem P, int store32)
         for (int i = 0; i < 6; ++i) rc.G[i] = rl.G[i] = G[i];
         rc.idx = l;
         rl.idx = ct.pose;
-        rc.pad[0] = rc.pad[1] = rc.pad[2] = rl.pad[0] = rl.pad[1] = rl.pad[2] = 0;
+        rc.pad[0] = rc.pad[1] = rc.pad[2] = rl.pad[1] = rl.pad[2] = 0;
+        rl.pad[0] = l;  // l-order records carry their landmark for the staged passes
         P.ec64[e] = rc;
         P.el64[pl] = rl;
         P.hl64[3 * (size_t)pl] = h[0];
@@ -240,7 +241,8 @@ __global__ __launch_bounds__(kBlock) void k_linearize(DevProblem P, int store32)
             for (int i = 0; i < 6; ++i) fc.G[i] = fl.G[i] = (float)G[i];
             fc.idx = l;
             fl.idx = ct.pose;
-            fc.pad = fl.pad = 0;
+            fc.pad = 0;
+            fl.pad = l;
             P.ec32[e] = fc;
             P.el32[pl] = fl;
         }
@@ -444,27 +446,72 @@ __global__ void k_precondition_finalize(DevProblem P, double lambda) {
 // Matrix-free Schur operator, S x = (U + lambda I) x - W V^-1 W^T x
 // (implicit_schur.rs:390-431), as a landmark pass and a camera pass.
 // ---------------------------------------------------------------------------
+// Landmark slot of an l-order record (stored in its padding).
+__device__ __forceinline__ int rec_lm(const EdgeRec32& r) { return r.pad; }
+__device__ __forceinline__ int rec_lm(const EdgeRec64& r) { return r.pad[0]; }
+
+// Landmark pass, LDS-staged: each lane takes one l-order edge of the tile
+// (coalesced record loads), computes W_e^T x_p = G (rho + omega x X) into
+// LDS, then the thread owning each landmark sums its contiguous segment and
+// applies V^-1. A tile holding a single landmark with more than kTileEdges
+// edges is summed straight from global memory by its owner.
 template <typename T, typename Rec>
 __global__ __launch_bounds__(kBlock) void k_lm_pass(DevProblem P, const T* __restrict__ x,
                                                     const Rec* __restrict__ el, const T* __restrict__ vinv,
                                                     LmPack<T>* __restrict__ pack, const int32_t* run_flag) {
+    __shared__ T cbuf[kTileEdges * 3];
+    __shared__ T xs[kBlock * 3];
     if (run_flag && *run_flag != 0) return;
     const LmTile lt = P.lm_tiles[blockIdx.x];
+    const int e0 = P.lm_edge_ptr[lt.lm_begin], e1 = P.lm_edge_ptr[lt.lm_end];
     const int l = lt.lm_begin + (int)threadIdx.x;
-    if (l >= lt.lm_end || P.lm_var[l] < 0) return;
-    const LmPack<T> pk = pack[l];
-    const T X0 = pk.X[0], X1 = pk.X[1], X2 = pk.X[2];
+    const bool owner = l < lt.lm_end;
+    if (owner) {
+        const LmPack<T> pk = pack[l];
+        xs[3 * threadIdx.x] = pk.X[0];
+        xs[3 * threadIdx.x + 1] = pk.X[1];
+        xs[3 * threadIdx.x + 2] = pk.X[2];
+    }
+    const bool staged = e1 - e0 <= kTileEdges;
+    __syncthreads();
+    if (staged) {
+        for (int e = e0 + (int)threadIdx.x; e < e1; e += kBlock) {
+            const Rec r = el[e];
+            const int j = rec_lm(r) - lt.lm_begin;
+            const T X0 = xs[3 * j], X1 = xs[3 * j + 1], X2 = xs[3 * j + 2];
+            const T* xp = x + 6 * (size_t)r.idx;
+            const T v0 = xp[0] + xp[4] * X2 - xp[5] * X1;
+            const T v1 = xp[1] + xp[5] * X0 - xp[3] * X2;
+            const T v2 = xp[2] + xp[3] * X1 - xp[4] * X0;
+            T* c = cbuf + 3 * (e - e0);
+            c[0] = (T)r.G[0] * v0 + (T)r.G[1] * v1 + (T)r.G[2] * v2;
+            c[1] = (T)r.G[1] * v0 + (T)r.G[3] * v1 + (T)r.G[4] * v2;
+            c[2] = (T)r.G[2] * v0 + (T)r.G[4] * v1 + (T)r.G[5] * v2;
+        }
+        __syncthreads();
+    }
+    if (!owner || P.lm_var[l] < 0) return;
+    const T X0 = xs[3 * threadIdx.x], X1 = xs[3 * threadIdx.x + 1], X2 = xs[3 * threadIdx.x + 2];
     T t0 = T(0), t1 = T(0), t2 = T(0);
-    const int e1 = P.lm_edge_ptr[l + 1];
-    for (int e = P.lm_edge_ptr[l]; e < e1; ++e) {
-        const Rec r = el[e];
-        const T* xp = x + 6 * (size_t)r.idx;
-        const T v0 = xp[0] + xp[4] * X2 - xp[5] * X1;
-        const T v1 = xp[1] + xp[5] * X0 - xp[3] * X2;
-        const T v2 = xp[2] + xp[3] * X1 - xp[4] * X0;
-        t0 += (T)r.G[0] * v0 + (T)r.G[1] * v1 + (T)r.G[2] * v2;
-        t1 += (T)r.G[1] * v0 + (T)r.G[3] * v1 + (T)r.G[4] * v2;
-        t2 += (T)r.G[2] * v0 + (T)r.G[4] * v1 + (T)r.G[5] * v2;
+    const int b = P.lm_edge_ptr[l], end = P.lm_edge_ptr[l + 1];
+    if (staged) {
+        for (int e = b; e < end; ++e) {
+            const T* c = cbuf + 3 * (e - e0);
+            t0 += c[0];
+            t1 += c[1];
+            t2 += c[2];
+        }
+    } else {
+        for (int e = b; e < end; ++e) {
+            const Rec r = el[e];
+            const T* xp = x + 6 * (size_t)r.idx;
+            const T v0 = xp[0] + xp[4] * X2 - xp[5] * X1;
+            const T v1 = xp[1] + xp[5] * X0 - xp[3] * X2;
+            const T v2 = xp[2] + xp[3] * X1 - xp[4] * X0;
+            t0 += (T)r.G[0] * v0 + (T)r.G[1] * v1 + (T)r.G[2] * v2;
+            t1 += (T)r.G[1] * v0 + (T)r.G[3] * v1 + (T)r.G[4] * v2;
+            t2 += (T)r.G[2] * v0 + (T)r.G[4] * v1 + (T)r.G[5] * v2;
+        }
     }
     const T* Vi = vinv + 6 * (size_t)l;
     pack[l].u[0] = Vi[0] * t0 + Vi[1] * t1 + Vi[2] * t2;

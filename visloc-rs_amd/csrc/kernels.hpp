@@ -25,6 +25,7 @@ namespace vba {
 
 constexpr int kBlock = 256;     // threads per tile work-group
 constexpr int kSmallBlock = 1024;  // single work-group vector kernels
+constexpr int kTileEdges = 1024;   // edges of a landmark tile staged in LDS
 
 template <typename T>
 struct LmPack {  // per landmark slot: position and the current landmark vector

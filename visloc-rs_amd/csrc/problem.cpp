@@ -340,13 +340,14 @@ int32_t build_model(const vba_problem& pb, Model* mp, uint64_t* error_id, std::s
             m.cam_tiles.push_back({(int32_t)s, b, std::min<int32_t>(b + (int32_t)tc, m.cam_edge_ptr[s + 1]), 0});
     }
     m.cam_tile_ptr[m.num_pose_slots] = (int32_t)m.cam_tiles.size();
-    // Landmark tiles: at most 256 landmarks and 4096 edges (a single landmark
-    // with more edges forms its own tile).
+    // Landmark tiles: at most 256 landmarks and 1024 edges, the LDS staging
+    // capacity of the landmark kernels (a single landmark with more edges
+    // forms its own tile).
     m.lm_tiles.clear();
     m.max_lm_tile_edges = 0;
     for (uint32_t b = 0; b < m.num_lm_slots;) {
         uint32_t e = b + 1;
-        while (e < m.num_lm_slots && e - b < 256 && m.lm_edge_ptr[e + 1] - m.lm_edge_ptr[b] <= 4096) ++e;
+        while (e < m.num_lm_slots && e - b < 256 && m.lm_edge_ptr[e + 1] - m.lm_edge_ptr[b] <= 1024) ++e;
         m.lm_tiles.push_back({(int32_t)b, (int32_t)e});
         m.max_lm_tile_edges = std::max(m.max_lm_tile_edges, m.lm_edge_ptr[e] - m.lm_edge_ptr[b]);
         b = e;

@@ -21,13 +21,18 @@ namespace vba {
 namespace {
 
 enum BufId {
-    B_POSES, B_LMS, B_POSES_T, B_LMS_T, B_SENSORS, B_POSE_VAR, B_LM_VAR, B_OBS_UV, B_OBS_SENSOR,
-    B_EDGE_LM, B_EDGE_OBS, B_PERM_CL, B_L_POSE, B_LM_EDGE_PTR, B_CAM_TILES, B_CAM_TILE_PTR,
-    B_LM_TILES, B_EC64, B_EL64, B_HL64, B_EC32, B_EL32, B_TILE27, B_U, B_G, B_V, B_GL, B_VINV64,
-    B_VINV32, B_PACK64, B_PACK32, B_MINV, B_VB, B_VX, B_VR, B_VZ, B_VD, B_VQ, B_VD32, B_PART64,
-    B_PART32, B_PCG, B_DELTA_L, B_TILE_SCALAR, B_SCALARS, B_DENSE_BLOCK_PQ, B_DENSE_BLOCK_PTR,
-    B_DENSE_TERMS, B_DENSE_VAR_SLOT, B_DENSE_S, B_DENSE_WORK, B_DENSE_STATUS, B_COUNT
+    B_POSES0, B_LMS0, B_POSES, B_LMS, B_POSES_T, B_LMS_T, B_SENSORS, B_POSE_VAR, B_LM_VAR, B_OBS_UV,
+    B_OBS_SENSOR, B_EDGE_LM, B_EDGE_OBS, B_PERM_CL, B_L_POSE, B_LM_EDGE_PTR, B_CAM_TILES,
+    B_CAM_TILE_PTR, B_LM_TILES, B_EC64, B_EL64, B_HL64, B_EC32, B_EL32, B_TILE27, B_U, B_G, B_V, B_GL,
+    B_VINV64, B_VINV32, B_PACK64, B_PACK32, B_MINV, B_VB, B_VX, B_VR, B_VZ, B_VD, B_VQ, B_VD32,
+    B_PART64, B_PART32, B_PCG, B_DELTA_L, B_TILE_SCALAR, B_SCALARS, B_DENSE_BLOCK_PQ,
+    B_DENSE_BLOCK_PTR, B_DENSE_TERMS, B_DENSE_VAR_SLOT, B_DENSE_S, B_DENSE_WORK, B_DENSE_STATUS,
+    B_COUNT
 };
+
+// Profiled kernel families.
+enum ProfId { K_LINEARIZE, K_LM_PASS, K_CAM_PASS, K_PCG_UPDATE, K_COST, K_COUNT };
+const char* kProfNames[K_COUNT] = {"linearize", "lm_pass", "cam_pass", "pcg_update", "cost"};
 
 struct Fail {
     int32_t code;
@@ -35,13 +40,14 @@ struct Fail {
 };
 
 void hip_check(hipError_t e, const char* what) {
-    if (e != hipSuccess) throw Fail{e == hipErrorOutOfMemory ? VBA_ERR_OUT_OF_MEMORY : VBA_ERR_DEVICE,
-                                    std::string(what) + ": " + hipGetErrorString(e)};
+    if (e != hipSuccess)
+        throw Fail{e == hipErrorOutOfMemory ? VBA_ERR_OUT_OF_MEMORY : VBA_ERR_DEVICE,
+                   std::string(what) + ": " + hipGetErrorString(e)};
 }
 
-void* ensure(Workspace& ws, int id, size_t bytes) {
-    if (ws.bufs.size() < B_COUNT) ws.bufs.resize(B_COUNT);
-    DevBuffer& b = ws.bufs[id];
+void* ensure(Prepared& pr, int id, size_t bytes) {
+    if (pr.bufs.size() < B_COUNT) pr.bufs.resize(B_COUNT);
+    DevBuffer& b = pr.bufs[id];
     if (bytes == 0) bytes = 64;
     if (b.bytes < bytes) {
         if (b.ptr) hip_check(hipFree(b.ptr), "hipFree");
@@ -53,68 +59,141 @@ void* ensure(Workspace& ws, int id, size_t bytes) {
     return b.ptr;
 }
 template <typename T>
-T* upload(Workspace& ws, int id, const std::vector<T>& v, hipStream_t s) {
-    T* p = (T*)ensure(ws, id, v.size() * sizeof(T));
-    if (!v.empty()) hip_check(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+T* upload(Prepared& pr, int id, const std::vector<T>& v, hipStream_t s) {
+    T* p = (T*)ensure(pr, id, v.size() * sizeof(T));
+    if (!v.empty())
+        hip_check(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s), "upload");
     return p;
+}
+template <typename T>
+T* buf(Prepared& pr, int id) {
+    return (T*)pr.bufs[id].ptr;
 }
 
 // Dense reduced system structure: one block per (var p <= var q) sharing a
 // landmark plus every diagonal block; each term is (landmark, e_p, e_q).
-struct DenseHost {
-    std::vector<int32_t> block_pq, block_ptr, terms, var_slot;
-};
-void build_dense(const Model& m, DenseHost* D) {
+void build_dense(Prepared& pr) {
+    const Model& m = pr.m;
     const uint32_t P = m.P;
-    D->var_slot.assign(P, 0);
+    pr.dense_var_slot.assign(P, 0);
     for (uint32_t s = 0; s < m.num_pose_slots; ++s)
-        if (m.pose_var[s] >= 0) D->var_slot[m.pose_var[s]] = (int32_t)s;
+        if (m.pose_var[s] >= 0) pr.dense_var_slot[m.pose_var[s]] = (int32_t)s;
     std::vector<int32_t> block_id((size_t)P * P, -1);
     std::vector<int64_t> count;
+    pr.dense_block_pq.clear();
     auto block_of = [&](int32_t a, int32_t b) {
         int32_t& id = block_id[(size_t)a * P + b];
         if (id < 0) {
             id = (int32_t)count.size();
             count.push_back(0);
-            D->block_pq.push_back(D->var_slot[a]);
-            D->block_pq.push_back(D->var_slot[b]);
+            pr.dense_block_pq.push_back(pr.dense_var_slot[a]);
+            pr.dense_block_pq.push_back(pr.dense_var_slot[b]);
         }
         return id;
     };
     for (uint32_t v = 0; v < P; ++v) block_of((int32_t)v, (int32_t)v);
     std::vector<std::pair<int32_t, int32_t>> ends;  // (var pose, l-order edge)
-    for (uint32_t l = 0; l < m.num_lm_slots; ++l) {
-        if (m.lm_var[l] < 0) continue;
+    auto collect = [&](uint32_t l) {
         ends.clear();
         for (int32_t e = m.lm_edge_ptr[l]; e < m.lm_edge_ptr[l + 1]; ++e)
             if (m.pose_var[m.l_pose[e]] >= 0) ends.push_back({m.pose_var[m.l_pose[e]], e});
+    };
+    for (uint32_t l = 0; l < m.num_lm_slots; ++l) {
+        if (m.lm_var[l] < 0) continue;
+        collect(l);
         for (size_t a = 0; a < ends.size(); ++a)
             for (size_t b = a; b < ends.size(); ++b) ++count[block_of(ends[a].first, ends[b].first)];
     }
     const size_t nb = count.size();
-    D->block_ptr.assign(nb + 1, 0);
-    for (size_t k = 0; k < nb; ++k) D->block_ptr[k + 1] = D->block_ptr[k] + (int32_t)count[k];
-    D->terms.assign(3 * (size_t)D->block_ptr[nb], 0);
-    std::vector<int32_t> cursor(D->block_ptr.begin(), D->block_ptr.end() - 1);
+    int64_t total = 0;
+    for (int64_t c : count) total += c;
+    if (total > (int64_t)INT32_MAX / 3) throw Fail{VBA_ERR_INVALID_ARGUMENT, "dense reduced system too large"};
+    pr.dense_block_ptr.assign(nb + 1, 0);
+    for (size_t k = 0; k < nb; ++k) pr.dense_block_ptr[k + 1] = pr.dense_block_ptr[k] + (int32_t)count[k];
+    pr.dense_terms.assign(3 * (size_t)total, 0);
+    std::vector<int32_t> cursor(pr.dense_block_ptr.begin(), pr.dense_block_ptr.end() - 1);
     for (uint32_t l = 0; l < m.num_lm_slots; ++l) {
         if (m.lm_var[l] < 0) continue;
-        ends.clear();
-        for (int32_t e = m.lm_edge_ptr[l]; e < m.lm_edge_ptr[l + 1]; ++e)
-            if (m.pose_var[m.l_pose[e]] >= 0) ends.push_back({m.pose_var[m.l_pose[e]], e});
+        collect(l);
         for (size_t a = 0; a < ends.size(); ++a)
             for (size_t b = a; b < ends.size(); ++b) {
                 int32_t k = cursor[block_id[(size_t)ends[a].first * P + ends[b].first]]++;
-                D->terms[3 * (size_t)k] = (int32_t)l;
-                D->terms[3 * (size_t)k + 1] = ends[a].second;
-                D->terms[3 * (size_t)k + 2] = ends[b].second;
+                pr.dense_terms[3 * (size_t)k] = (int32_t)l;
+                pr.dense_terms[3 * (size_t)k + 1] = ends[a].second;
+                pr.dense_terms[3 * (size_t)k + 2] = ends[b].second;
             }
     }
+    pr.dense_built = true;
 }
 
-double norm6(const double* v) {
-    double s = 0.0;
-    for (int i = 0; i < 6; ++i) s += v[i] * v[i];
-    return std::sqrt(s);
+// ---- profiling with HIP events on the context stream ----
+hipEvent_t take_event(Context& ctx) {
+    if (!ctx.event_pool.empty()) {
+        hipEvent_t e = ctx.event_pool.back();
+        ctx.event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    hip_check(hipEventCreate(&e), "hipEventCreate");
+    return e;
+}
+struct ProfScope {
+    Context& ctx;
+    int kernel;
+    double bytes;
+    hipEvent_t a = nullptr;
+    ProfScope(Context& c, int k, double b) : ctx(c), kernel(k), bytes(b) {
+        if (ctx.profile) {
+            a = take_event(ctx);
+            hip_check(hipEventRecord(a, ctx.stream), "hipEventRecord");
+        }
+    }
+    ~ProfScope() {
+        if (!a) return;
+        hipEvent_t b = nullptr;
+        if (!ctx.event_pool.empty()) {
+            b = ctx.event_pool.back();
+            ctx.event_pool.pop_back();
+        } else if (hipEventCreate(&b) != hipSuccess) {
+            ctx.event_pool.push_back(a);
+            return;
+        }
+        if (hipEventRecord(b, ctx.stream) != hipSuccess) {
+            ctx.event_pool.push_back(a);
+            ctx.event_pool.push_back(b);
+            return;
+        }
+        ctx.pending.push_back({a, b, kernel, bytes});
+    }
+};
+// Fold finished event pairs into the per-kernel totals (after a sync).
+void harvest(Context& ctx) {
+    if (ctx.kernels.size() < K_COUNT) {
+        ctx.kernels.resize(K_COUNT);
+        for (int k = 0; k < K_COUNT; ++k) ctx.kernels[k].name = kProfNames[k];
+    }
+    for (const Context::Pending& p : ctx.pending) {
+        if (p.kernel < 0) {  // a launch that exited early (PCG already finished)
+            ctx.event_pool.push_back(p.a);
+            ctx.event_pool.push_back(p.b);
+            continue;
+        }
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, p.a, p.b), "hipEventElapsedTime");
+        KernelProfile& k = ctx.kernels[p.kernel];
+        ++k.launches;
+        k.seconds += ms * 1e-3;
+        k.bytes += p.bytes;
+        ctx.event_pool.push_back(p.a);
+        ctx.event_pool.push_back(p.b);
+    }
+    ctx.pending.clear();
+}
+
+// Keep only the first `keep` pending launches of `kernel` in the current batch.
+void keep_first(Context& ctx, size_t from, int kernel, int keep) {
+    for (size_t i = from; i < ctx.pending.size(); ++i)
+        if (ctx.pending[i].kernel == kernel && keep-- <= 0) ctx.pending[i].kernel = -1;
 }
 
 }  // namespace
@@ -184,37 +263,102 @@ int32_t validate_matrix_free(const Model& m, const vba_options& o, std::string* 
     return VBA_OK;
 }
 
-void release_workspace(Workspace& ws) {
-    for (DevBuffer& b : ws.bufs)
+void release(Prepared& pr) {
+    for (DevBuffer& b : pr.bufs)
         if (b.ptr) (void)hipFree(b.ptr);
-    ws.bufs.clear();
-    if (ws.host_scalars) (void)hipHostFree(ws.host_scalars);
-    if (ws.host_pcg) (void)hipHostFree(ws.host_pcg);
-    if (ws.host_status) (void)hipHostFree(ws.host_status);
-    ws.host_scalars = nullptr;
-    ws.host_pcg = nullptr;
-    ws.host_status = nullptr;
+    pr.bufs.clear();
 }
 
-int32_t solve(Context& ctx, const Model& m, const vba_options& o, double* poses_out, double* landmarks_out,
-              vba_iteration_stats* stats, uint32_t cap, vba_result* res) {
-    Workspace& ws = ctx.ws;
+void release_context(Context& ctx) {
+    if (ctx.host_scalars) (void)hipHostFree(ctx.host_scalars);
+    if (ctx.host_pcg) (void)hipHostFree(ctx.host_pcg);
+    if (ctx.host_status) (void)hipHostFree(ctx.host_status);
+    ctx.host_scalars = nullptr;
+    ctx.host_pcg = nullptr;
+    ctx.host_status = nullptr;
+    for (const Context::Pending& p : ctx.pending) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    ctx.pending.clear();
+    for (hipEvent_t e : ctx.event_pool) (void)hipEventDestroy(e);
+    ctx.event_pool.clear();
+}
+
+int32_t prepare(Context& ctx, Prepared& pr, std::string* msg) {
     hipStream_t s = ctx.stream;
+    const Model& m = pr.m;
+    try {
+        if (!ctx.host_scalars) {
+            hip_check(hipHostMalloc((void**)&ctx.host_scalars, sizeof(StepScalars), hipHostMallocDefault), "hipHostMalloc");
+            hip_check(hipHostMalloc((void**)&ctx.host_pcg, sizeof(PcgState), hipHostMallocDefault), "hipHostMalloc");
+            hip_check(hipHostMalloc((void**)&ctx.host_status, sizeof(int32_t), hipHostMallocDefault), "hipHostMalloc");
+        }
+        upload(pr, B_POSES0, m.poses, s);
+        upload(pr, B_LMS0, m.lms, s);
+        ensure(pr, B_POSES, m.poses.size() * sizeof(double));
+        ensure(pr, B_LMS, m.lms.size() * sizeof(double));
+        ensure(pr, B_POSES_T, m.poses.size() * sizeof(double));
+        ensure(pr, B_LMS_T, m.lms.size() * sizeof(double));
+        upload(pr, B_SENSORS, m.sensors, s);
+        upload(pr, B_POSE_VAR, m.pose_var, s);
+        upload(pr, B_LM_VAR, m.lm_var, s);
+        upload(pr, B_OBS_UV, m.obs_uv, s);
+        if (m.any_rig) upload(pr, B_OBS_SENSOR, m.obs_sensor, s);
+        upload(pr, B_EDGE_LM, m.edge_lm, s);
+        upload(pr, B_EDGE_OBS, m.edge_obs, s);
+        upload(pr, B_PERM_CL, m.perm_cl, s);
+        upload(pr, B_L_POSE, m.l_pose, s);
+        upload(pr, B_LM_EDGE_PTR, m.lm_edge_ptr, s);
+        upload(pr, B_CAM_TILES, m.cam_tiles, s);
+        upload(pr, B_CAM_TILE_PTR, m.cam_tile_ptr, s);
+        upload(pr, B_LM_TILES, m.lm_tiles, s);
+        hip_check(hipStreamSynchronize(s), "sync");
+        pr.final_slot = -1;
+        return VBA_OK;
+    } catch (const Fail& f) {
+        *msg = f.msg;
+        (void)hipStreamSynchronize(s);
+        return f.code;
+    }
+}
+
+int32_t download_state(Context& ctx, const Prepared& pr, double* poses_out, double* landmarks_out) {
+    hipStream_t s = ctx.stream;
+    const Model& m = pr.m;
+    try {
+        std::vector<double> poses(m.poses.size()), lms(m.lms.size());
+        const int ps = pr.final_slot >= 0 ? pr.final_slot : B_POSES0;
+        const int ls = pr.final_slot >= 0 ? pr.final_lm_slot : B_LMS0;
+        hip_check(hipMemcpyAsync(poses.data(), pr.bufs[ps].ptr, poses.size() * sizeof(double), hipMemcpyDeviceToHost, s),
+                  "D2H");
+        hip_check(hipMemcpyAsync(lms.data(), pr.bufs[ls].ptr, lms.size() * sizeof(double), hipMemcpyDeviceToHost, s),
+                  "D2H");
+        hip_check(hipStreamSynchronize(s), "sync");
+        if (poses_out)
+            for (size_t i = 0; i < m.pose_slot_of_input.size(); ++i)
+                std::memcpy(poses_out + 7 * i, poses.data() + 7 * (size_t)m.pose_slot_of_input[i], 7 * sizeof(double));
+        if (landmarks_out)
+            for (size_t i = 0; i < m.lm_slot_of_input.size(); ++i)
+                std::memcpy(landmarks_out + 3 * i, lms.data() + 3 * (size_t)m.lm_slot_of_input[i], 3 * sizeof(double));
+        return VBA_OK;
+    } catch (const Fail& f) {
+        ctx.last_error = f.msg;
+        return f.code;
+    }
+}
+
+int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stats* stats, uint32_t cap,
+            vba_result* res) {
+    hipStream_t s = ctx.stream;
+    const Model& m = pr.m;
     const bool matrix_free = o.linear_solver != VBA_SOLVER_DENSE;
     const bool mixed = matrix_free && o.precision == VBA_PRECISION_MIXED;
     const bool accept_iterate = o.pcg_maxit_policy == VBA_PCG_MAXIT_ACCEPT_ITERATE;
     try {
-        if (!ws.host_scalars) {
-            hip_check(hipHostMalloc((void**)&ws.host_scalars, sizeof(StepScalars), hipHostMallocDefault), "hipHostMalloc");
-            hip_check(hipHostMalloc((void**)&ws.host_pcg, sizeof(PcgState), hipHostMallocDefault), "hipHostMalloc");
-            hip_check(hipHostMalloc((void**)&ws.host_status, sizeof(int32_t), hipHostMallocDefault), "hipHostMalloc");
-        }
-        DenseHost dh;
-        if (!matrix_free) {
-            if (m.P > 1024) throw Fail{VBA_ERR_INVALID_ARGUMENT, "dense reduced solve limited to 1024 variable poses"};
-            build_dense(m, &dh);
-        }
-        // ---- upload ----
+        if (!matrix_free && m.P > 1024)
+            throw Fail{VBA_ERR_INVALID_ARGUMENT, "dense reduced solve limited to 1024 variable poses"};
+        // ---- bind device buffers (allocated on first use, kept for later solves) ----
         DevProblem P{};
         P.num_pose_slots = (int32_t)m.num_pose_slots;
         P.num_lm_slots = (int32_t)m.num_lm_slots;
@@ -228,89 +372,116 @@ int32_t solve(Context& ctx, const Model& m, const vba_options& o, double* poses_
         const size_t NP = m.num_pose_slots, NL = m.num_lm_slots, NE = m.num_edges, NT = m.cam_tiles.size();
         const size_t NLT = m.lm_tiles.size();
         const size_t pose_blocks = (NP + kBlock - 1) / kBlock;
-        P.poses = upload(ws, B_POSES, m.poses, s);
-        P.lms = upload(ws, B_LMS, m.lms, s);
-        P.poses_trial = (double*)ensure(ws, B_POSES_T, m.poses.size() * sizeof(double));
-        P.lms_trial = (double*)ensure(ws, B_LMS_T, m.lms.size() * sizeof(double));
-        P.sensors = upload(ws, B_SENSORS, m.sensors, s);
-        P.pose_var = upload(ws, B_POSE_VAR, m.pose_var, s);
-        P.lm_var = upload(ws, B_LM_VAR, m.lm_var, s);
-        P.obs_uv = upload(ws, B_OBS_UV, m.obs_uv, s);
-        P.obs_sensor = m.any_rig ? upload(ws, B_OBS_SENSOR, m.obs_sensor, s) : nullptr;
-        P.edge_lm = upload(ws, B_EDGE_LM, m.edge_lm, s);
-        P.edge_obs = upload(ws, B_EDGE_OBS, m.edge_obs, s);
-        P.perm_cl = upload(ws, B_PERM_CL, m.perm_cl, s);
-        P.l_pose = upload(ws, B_L_POSE, m.l_pose, s);
-        P.lm_edge_ptr = upload(ws, B_LM_EDGE_PTR, m.lm_edge_ptr, s);
-        P.cam_tiles = upload(ws, B_CAM_TILES, m.cam_tiles, s);
-        P.cam_tile_ptr = upload(ws, B_CAM_TILE_PTR, m.cam_tile_ptr, s);
-        P.lm_tiles = upload(ws, B_LM_TILES, m.lm_tiles, s);
-        P.ec64 = (EdgeRec64*)ensure(ws, B_EC64, NE * sizeof(EdgeRec64));
-        P.el64 = (EdgeRec64*)ensure(ws, B_EL64, NE * sizeof(EdgeRec64));
-        P.hl64 = (double*)ensure(ws, B_HL64, 3 * NE * sizeof(double));
+        P.poses = buf<double>(pr, B_POSES);
+        P.lms = buf<double>(pr, B_LMS);
+        P.poses_trial = buf<double>(pr, B_POSES_T);
+        P.lms_trial = buf<double>(pr, B_LMS_T);
+        hip_check(hipMemcpyAsync(P.poses, pr.bufs[B_POSES0].ptr, m.poses.size() * sizeof(double),
+                                 hipMemcpyDeviceToDevice, s), "D2D");
+        hip_check(hipMemcpyAsync(P.lms, pr.bufs[B_LMS0].ptr, m.lms.size() * sizeof(double), hipMemcpyDeviceToDevice, s),
+                  "D2D");
+        P.sensors = buf<DevSensor>(pr, B_SENSORS);
+        P.pose_var = buf<int32_t>(pr, B_POSE_VAR);
+        P.lm_var = buf<int32_t>(pr, B_LM_VAR);
+        P.obs_uv = buf<double>(pr, B_OBS_UV);
+        P.obs_sensor = m.any_rig ? buf<uint16_t>(pr, B_OBS_SENSOR) : nullptr;
+        P.edge_lm = buf<int32_t>(pr, B_EDGE_LM);
+        P.edge_obs = buf<int32_t>(pr, B_EDGE_OBS);
+        P.perm_cl = buf<int32_t>(pr, B_PERM_CL);
+        P.l_pose = buf<int32_t>(pr, B_L_POSE);
+        P.lm_edge_ptr = buf<int32_t>(pr, B_LM_EDGE_PTR);
+        P.cam_tiles = buf<CamTile>(pr, B_CAM_TILES);
+        P.cam_tile_ptr = buf<int32_t>(pr, B_CAM_TILE_PTR);
+        P.lm_tiles = buf<LmTile>(pr, B_LM_TILES);
+        P.ec64 = (EdgeRec64*)ensure(pr, B_EC64, NE * sizeof(EdgeRec64));
+        P.el64 = (EdgeRec64*)ensure(pr, B_EL64, NE * sizeof(EdgeRec64));
+        P.hl64 = (double*)ensure(pr, B_HL64, 3 * NE * sizeof(double));
         if (mixed) {
-            P.ec32 = (EdgeRec32*)ensure(ws, B_EC32, NE * sizeof(EdgeRec32));
-            P.el32 = (EdgeRec32*)ensure(ws, B_EL32, NE * sizeof(EdgeRec32));
-            P.Vinv32 = (float*)ensure(ws, B_VINV32, 6 * NL * sizeof(float));
-            P.pack32 = (LmPack<float>*)ensure(ws, B_PACK32, NL * sizeof(LmPack<float>));
-            P.part32 = (float*)ensure(ws, B_PART32, 6 * std::max<size_t>(NT, 1) * sizeof(float));
+            P.ec32 = (EdgeRec32*)ensure(pr, B_EC32, NE * sizeof(EdgeRec32));
+            P.el32 = (EdgeRec32*)ensure(pr, B_EL32, NE * sizeof(EdgeRec32));
+            P.Vinv32 = (float*)ensure(pr, B_VINV32, 6 * NL * sizeof(float));
+            P.pack32 = (LmPack<float>*)ensure(pr, B_PACK32, NL * sizeof(LmPack<float>));
+            P.part32 = (float*)ensure(pr, B_PART32, 6 * std::max<size_t>(NT, 1) * sizeof(float));
         }
-        P.tile27 = (double*)ensure(ws, B_TILE27, 27 * std::max<size_t>(NT, 1) * sizeof(double));
-        P.U = (double*)ensure(ws, B_U, 21 * NP * sizeof(double));
-        P.g = (double*)ensure(ws, B_G, 6 * NP * sizeof(double));
-        P.V = (double*)ensure(ws, B_V, 6 * NL * sizeof(double));
-        P.gl = (double*)ensure(ws, B_GL, 3 * NL * sizeof(double));
-        P.Vinv64 = (double*)ensure(ws, B_VINV64, 6 * NL * sizeof(double));
-        P.pack64 = (LmPack<double>*)ensure(ws, B_PACK64, NL * sizeof(LmPack<double>));
-        P.Minv = (double*)ensure(ws, B_MINV, 36 * NP * sizeof(double));
-        P.b = (double*)ensure(ws, B_VB, 6 * NP * sizeof(double));
-        P.x = (double*)ensure(ws, B_VX, 6 * NP * sizeof(double));
-        P.r = (double*)ensure(ws, B_VR, 6 * NP * sizeof(double));
-        P.z = (double*)ensure(ws, B_VZ, 6 * NP * sizeof(double));
-        P.d = (double*)ensure(ws, B_VD, 6 * NP * sizeof(double));
-        P.q = (double*)ensure(ws, B_VQ, 6 * NP * sizeof(double));
-        P.d32 = (float*)ensure(ws, B_VD32, 6 * NP * sizeof(float));
-        P.part64 = (double*)ensure(ws, B_PART64, 6 * std::max<size_t>(NT, 1) * sizeof(double));
-        P.pcg = (PcgState*)ensure(ws, B_PCG, sizeof(PcgState));
-        P.delta_l = (double*)ensure(ws, B_DELTA_L, 3 * NL * sizeof(double));
-        P.tile_scalar = (double*)ensure(ws, B_TILE_SCALAR, 4 * (NT + NLT + pose_blocks + 1) * sizeof(double));
-        P.scalars = (StepScalars*)ensure(ws, B_SCALARS, sizeof(StepScalars));
+        P.tile27 = (double*)ensure(pr, B_TILE27, 27 * std::max<size_t>(NT, 1) * sizeof(double));
+        P.U = (double*)ensure(pr, B_U, 21 * NP * sizeof(double));
+        P.g = (double*)ensure(pr, B_G, 6 * NP * sizeof(double));
+        P.V = (double*)ensure(pr, B_V, 6 * NL * sizeof(double));
+        P.gl = (double*)ensure(pr, B_GL, 3 * NL * sizeof(double));
+        P.Vinv64 = (double*)ensure(pr, B_VINV64, 6 * NL * sizeof(double));
+        P.pack64 = (LmPack<double>*)ensure(pr, B_PACK64, NL * sizeof(LmPack<double>));
+        P.Minv = (double*)ensure(pr, B_MINV, 36 * NP * sizeof(double));
+        P.b = (double*)ensure(pr, B_VB, 6 * NP * sizeof(double));
+        P.x = (double*)ensure(pr, B_VX, 6 * NP * sizeof(double));
+        P.r = (double*)ensure(pr, B_VR, 6 * NP * sizeof(double));
+        P.z = (double*)ensure(pr, B_VZ, 6 * NP * sizeof(double));
+        P.d = (double*)ensure(pr, B_VD, 6 * NP * sizeof(double));
+        P.q = (double*)ensure(pr, B_VQ, 6 * NP * sizeof(double));
+        P.d32 = (float*)ensure(pr, B_VD32, 6 * NP * sizeof(float));
+        P.part64 = (double*)ensure(pr, B_PART64, 6 * std::max<size_t>(NT, 1) * sizeof(double));
+        P.pcg = (PcgState*)ensure(pr, B_PCG, sizeof(PcgState));
+        P.delta_l = (double*)ensure(pr, B_DELTA_L, 3 * NL * sizeof(double));
+        const size_t nscalar = 4 * (NT + NLT + pose_blocks + 1);
+        P.tile_scalar = (double*)ensure(pr, B_TILE_SCALAR, nscalar * sizeof(double));
+        P.scalars = (StepScalars*)ensure(pr, B_SCALARS, sizeof(StepScalars));
         hip_check(hipMemsetAsync(P.x, 0, 6 * NP * sizeof(double), s), "memset");
-        hip_check(hipMemsetAsync(P.tile_scalar, 0, 4 * (NT + NLT + pose_blocks + 1) * sizeof(double), s), "memset");
+        hip_check(hipMemsetAsync(P.tile_scalar, 0, nscalar * sizeof(double), s), "memset");
         DensePairs D{};
         double* dS = nullptr;
         double* dwork = nullptr;
         int32_t* dstatus = nullptr;
         const int32_t ndense = (int32_t)(6 * m.P);
         if (!matrix_free) {
-            D.num_blocks = (int32_t)(dh.block_ptr.size() - 1);
-            D.block_pq = upload(ws, B_DENSE_BLOCK_PQ, dh.block_pq, s);
-            D.block_ptr = upload(ws, B_DENSE_BLOCK_PTR, dh.block_ptr, s);
-            D.terms = upload(ws, B_DENSE_TERMS, dh.terms, s);
-            D.var_slot = upload(ws, B_DENSE_VAR_SLOT, dh.var_slot, s);
-            dS = (double*)ensure(ws, B_DENSE_S, (size_t)ndense * ndense * sizeof(double));
-            dwork = (double*)ensure(ws, B_DENSE_WORK, ((size_t)ndense * ndense + ndense) * sizeof(double));
-            dstatus = (int32_t*)ensure(ws, B_DENSE_STATUS, sizeof(int32_t));
+            if (!pr.dense_built) {
+                build_dense(pr);
+                upload(pr, B_DENSE_BLOCK_PQ, pr.dense_block_pq, s);
+                upload(pr, B_DENSE_BLOCK_PTR, pr.dense_block_ptr, s);
+                upload(pr, B_DENSE_TERMS, pr.dense_terms, s);
+                upload(pr, B_DENSE_VAR_SLOT, pr.dense_var_slot, s);
+            }
+            D.num_blocks = (int32_t)(pr.dense_block_ptr.size() - 1);
+            D.block_pq = buf<int32_t>(pr, B_DENSE_BLOCK_PQ);
+            D.block_ptr = buf<int32_t>(pr, B_DENSE_BLOCK_PTR);
+            D.terms = buf<int32_t>(pr, B_DENSE_TERMS);
+            D.var_slot = buf<int32_t>(pr, B_DENSE_VAR_SLOT);
+            dS = (double*)ensure(pr, B_DENSE_S, (size_t)ndense * ndense * sizeof(double));
+            dwork = (double*)ensure(pr, B_DENSE_WORK, ((size_t)ndense * ndense + ndense) * sizeof(double));
+            dstatus = (int32_t*)ensure(pr, B_DENSE_STATUS, sizeof(int32_t));
         }
+        // Algorithmic bytes per launch of the profiled kernels (include/vba.h,
+        // vba_profile_query).
+        const double rec = mixed ? (double)sizeof(EdgeRec32) : (double)sizeof(EdgeRec64);
+        const double real = mixed ? 4.0 : 8.0;
+        const double bytes_lm_pass = NE * rec + NL * (8.0 + 12.0 * real) + NP * 6.0 * real;
+        const double bytes_cam_pass = NE * rec + NL * 6.0 * real + NT * (20.0 + 6.0 * real);
+        const double bytes_linearize = m.num_obs * 16.0 + NE * (4.0 + 4.0 + 4.0 + 2.0 * sizeof(EdgeRec64) + 24.0 +
+                                                                 (mixed ? 2.0 * sizeof(EdgeRec32) : 0.0)) +
+                                       NL * 24.0 + NT * (16.0 + 27.0 * 8.0) + NP * 56.0;
+        const double bytes_cost = m.num_obs * 16.0 + NE * 8.0 + NL * 24.0 + NT * (16.0 + 24.0) + NP * 56.0;
+
         hip_check(hipEventRecord(ctx.ev_start, s), "event");
 
         auto read_scalars = [&]() {
             launch_scalars_finalize(P, s);
             hip_check(hipGetLastError(), "kernel launch");
-            hip_check(hipMemcpyAsync(ws.host_scalars, P.scalars, sizeof(StepScalars), hipMemcpyDeviceToHost, s), "D2H");
+            hip_check(hipMemcpyAsync(ctx.host_scalars, P.scalars, sizeof(StepScalars), hipMemcpyDeviceToHost, s), "D2H");
             hip_check(hipStreamSynchronize(s), "sync");
-            return *ws.host_scalars;
+            harvest(ctx);
+            return *ctx.host_scalars;
         };
         auto read_pcg = [&]() {
             hip_check(hipGetLastError(), "kernel launch");
-            hip_check(hipMemcpyAsync(ws.host_pcg, P.pcg, sizeof(PcgState), hipMemcpyDeviceToHost, s), "D2H");
+            hip_check(hipMemcpyAsync(ctx.host_pcg, P.pcg, sizeof(PcgState), hipMemcpyDeviceToHost, s), "D2H");
             hip_check(hipStreamSynchronize(s), "sync");
-            return *ws.host_pcg;
+            return *ctx.host_pcg;
         };
 
         // ---- initial cost (lm.rs:323-327) ----
         hip_check(hipMemsetAsync(P.scalars, 0, sizeof(StepScalars), s), "memset");
-        launch_cost(P, false, s);
+        {
+            ProfScope ps(ctx, K_COST, bytes_cost);
+            launch_cost(P, false, s);
+        }
         StepScalars sc = read_scalars();
         const double initial_cost = sc.cost_sum + sc.cost_comp;
         uint64_t current_nonproj = sc.nonprojectable;
@@ -329,7 +500,10 @@ int32_t solve(Context& ctx, const Model& m, const vba_options& o, double* poses_
 
         for (uint32_t iteration = 0; iteration < o.max_iterations; ++iteration) {
             if (need_lin) {
-                launch_linearize(P, mixed, s);
+                {
+                    ProfScope ps(ctx, K_LINEARIZE, bytes_linearize);
+                    launch_linearize(P, mixed, s);
+                }
                 launch_camera_finalize(P, s);
                 launch_landmark_reduce(P, s);
                 need_lin = false;
@@ -349,9 +523,10 @@ int32_t solve(Context& ctx, const Model& m, const vba_options& o, double* poses_
                 hip_check(hipMemsetAsync(dS, 0, (size_t)ndense * ndense * sizeof(double), s), "memset");
                 launch_dense_assemble(P, D, dS, ndense, lambda, s);
                 launch_dense_solve(P, D, dS, dwork, ndense, dstatus, s);
-                hip_check(hipMemcpyAsync(ws.host_status, dstatus, sizeof(int32_t), hipMemcpyDeviceToHost, s), "D2H");
+                hip_check(hipGetLastError(), "kernel launch");
+                hip_check(hipMemcpyAsync(ctx.host_status, dstatus, sizeof(int32_t), hipMemcpyDeviceToHost, s), "D2H");
                 hip_check(hipStreamSynchronize(s), "sync");
-                solved = *ws.host_status == 0;
+                solved = *ctx.host_status == 0;
             } else {
                 launch_precondition(P, lambda, s);
                 launch_pcg_init(P, o.pcg_relative_tolerance, o.pcg_absolute_tolerance, (int32_t)o.max_pcg_iterations,
@@ -361,14 +536,26 @@ int32_t solve(Context& ctx, const Model& m, const vba_options& o, double* poses_
                 while (true) {
                     while (ps.status == PCG_RUNNING) {
                         const int before = ps.iter;
+                        const size_t batch = ctx.pending.size();
                         const int n = (int)std::min<uint32_t>(chunk, (uint32_t)std::max(0, ps.max_iter - ps.iter));
                         for (int k = 0; k < n; ++k) {
-                            launch_lm_pass(P, mixed, mixed ? (const void*)P.d32 : (const void*)P.d, flag, s);
-                            launch_cam_pass(P, mixed, flag, s);
-                            launch_pcg_update(P, mixed, s);
+                            {
+                                ProfScope a(ctx, K_LM_PASS, bytes_lm_pass);
+                                launch_lm_pass(P, mixed, mixed ? (const void*)P.d32 : (const void*)P.d, flag, s);
+                            }
+                            {
+                                ProfScope b(ctx, K_CAM_PASS, bytes_cam_pass);
+                                launch_cam_pass(P, mixed, flag, s);
+                            }
+                            {
+                                ProfScope c(ctx, K_PCG_UPDATE, 0.0);
+                                launch_pcg_update(P, mixed, s);
+                            }
                         }
-                        hip_check(hipGetLastError(), "PCG launch");
                         ps = read_pcg();
+                        const int done = ps.iter - before;
+                        for (int k : {K_LM_PASS, K_CAM_PASS, K_PCG_UPDATE}) keep_first(ctx, batch, k, done);
+                        harvest(ctx);
                         if (ps.status == PCG_RUNNING && ps.iter <= before)
                             throw Fail{VBA_ERR_DEVICE, "PCG made no progress"};
                     }
@@ -413,7 +600,10 @@ int32_t solve(Context& ctx, const Model& m, const vba_options& o, double* poses_
             }
             launch_backsub(P, s);
             launch_pose_update(P, s);
-            launch_cost(P, true, s);
+            {
+                ProfScope ps(ctx, K_COST, bytes_cost);
+                launch_cost(P, true, s);
+            }
             sc = read_scalars();
             const double cost_after = sc.cost_sum + sc.cost_comp;
             const uint64_t nonproj_after = sc.nonprojectable;
@@ -459,23 +649,12 @@ int32_t solve(Context& ctx, const Model& m, const vba_options& o, double* poses_
             }
         }
         hip_check(hipEventRecord(ctx.ev_stop, s), "event");
-        std::vector<double> poses(m.poses.size()), lms(m.lms.size());
-        hip_check(hipMemcpyAsync(poses.data(), P.poses, poses.size() * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
-        hip_check(hipMemcpyAsync(lms.data(), P.lms, lms.size() * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
         hip_check(hipStreamSynchronize(s), "sync");
+        harvest(ctx);
         float ms = 0.f;
         hip_check(hipEventElapsedTime(&ms, ctx.ev_start, ctx.ev_stop), "event");
-        // keep the accepted state in buffer slot order for the next call
-        if (P.poses != (double*)ws.bufs[B_POSES].ptr) {
-            std::swap(ws.bufs[B_POSES], ws.bufs[B_POSES_T]);
-            std::swap(ws.bufs[B_LMS], ws.bufs[B_LMS_T]);
-        }
-        if (poses_out)
-            for (size_t i = 0; i < m.pose_slot_of_input.size(); ++i)
-                std::memcpy(poses_out + 7 * i, poses.data() + 7 * (size_t)m.pose_slot_of_input[i], 7 * sizeof(double));
-        if (landmarks_out)
-            for (size_t i = 0; i < m.lm_slot_of_input.size(); ++i)
-                std::memcpy(landmarks_out + 3 * i, lms.data() + 3 * (size_t)m.lm_slot_of_input[i], 3 * sizeof(double));
+        pr.final_slot = P.poses == buf<double>(pr, B_POSES) ? B_POSES : B_POSES_T;
+        pr.final_lm_slot = P.lms == buf<double>(pr, B_LMS) ? B_LMS : B_LMS_T;
         res->initial_cost = initial_cost;
         res->final_cost = current_cost;
         res->num_iterations = std::min(written, cap);
@@ -484,11 +663,15 @@ int32_t solve(Context& ctx, const Model& m, const vba_options& o, double* poses_
         res->total_pcg_iterations = total_pcg;
         res->linearizations = linearizations;
         res->device_seconds = ms * 1e-3;
-        (void)norm6;
         return VBA_OK;
     } catch (const Fail& f) {
         ctx.last_error = f.msg;
         (void)hipStreamSynchronize(s);
+        for (const Context::Pending& p : ctx.pending) {
+            ctx.event_pool.push_back(p.a);
+            ctx.event_pool.push_back(p.b);
+        }
+        ctx.pending.clear();
         return f.code;
     }
 }

@@ -14,18 +14,19 @@
 
 namespace vba {
 
-// A device allocation that only grows; kept by the context between calls so
-// repeated solves of similar size allocate nothing.
+// A device allocation that only grows.
 struct DevBuffer {
     void* ptr = nullptr;
     size_t bytes = 0;
 };
 
-struct Workspace {
-    std::vector<DevBuffer> bufs;
-    StepScalars* host_scalars = nullptr;  // pinned
-    PcgState* host_pcg = nullptr;         // pinned
-    int32_t* host_status = nullptr;       // pinned
+// Device time of one kernel family, from HIP events recorded around its
+// launches on the context's stream while profiling is enabled.
+struct KernelProfile {
+    std::string name;
+    uint64_t launches = 0;
+    double seconds = 0.0;
+    double bytes = 0.0;  // algorithmic bytes moved, summed over launches
 };
 
 struct Context {
@@ -34,18 +35,49 @@ struct Context {
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     std::mutex mu;
     std::string last_error;
-    Workspace ws;
+    // pinned staging for the per-iteration scalars
+    StepScalars* host_scalars = nullptr;
+    PcgState* host_pcg = nullptr;
+    int32_t* host_status = nullptr;
+    // profiling
+    bool profile = false;
+    std::vector<KernelProfile> kernels;
+    struct Pending {
+        hipEvent_t a, b;
+        int kernel;
+        double bytes;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
+};
+
+// One problem laid out on the device: structure, initial state and working
+// buffers. Solves start from the initial state; the final state stays on the
+// device until read back.
+struct Prepared {
+    Model m;
+    std::vector<DevBuffer> bufs;
+    bool dense_built = false;
+    std::vector<int32_t> dense_block_pq, dense_block_ptr, dense_terms, dense_var_slot;
+    int final_slot = -1;  // buffer holding the final poses (-1: none yet)
+    int final_lm_slot = -1;
 };
 
 // Matrix-free entry validation (optimize.rs:382-626) on a built model.
 int32_t validate_matrix_free(const Model& m, const vba_options& o, std::string* message);
 
-// Run the LM loop of optimize_weighted_backend (lm.rs:162-1017) on the
-// device. Writes final poses / landmarks in the caller's input order.
-int32_t solve(Context& ctx, const Model& m, const vba_options& o, double* poses_out,
-              double* landmarks_out, vba_iteration_stats* stats, uint32_t stats_capacity,
-              vba_result* result);
+// Upload a built model (structure + initial state).
+int32_t prepare(Context& ctx, Prepared& prep, std::string* message);
 
-void release_workspace(Workspace& ws);
+// The LM loop of optimize_weighted_backend (lm.rs:162-1017) on the device,
+// from the prepared initial state.
+int32_t run(Context& ctx, Prepared& prep, const vba_options& o, vba_iteration_stats* stats,
+            uint32_t stats_capacity, vba_result* result);
+
+// Final (or, before any solve, initial) poses / landmarks in input order.
+int32_t download_state(Context& ctx, const Prepared& prep, double* poses_out, double* landmarks_out);
+
+void release(Prepared& prep);
+void release_context(Context& ctx);
 
 }  // namespace vba

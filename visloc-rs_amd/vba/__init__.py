@@ -5,10 +5,10 @@ Drop-in for the Levenberg–Marquardt Schur-complement path of visloc-rs
 """
 from . import abi
 from .abi import library
-from .api import (BundleProblem, Device, SolveOutput, Synthetic, camera, camera_centres, check,
-                  default_options, run_solver, synth_params, synthesize)
+from .api import (BundleProblem, Device, PreparedProblem, PrepareError, SolveOutput, Synthetic, camera,
+                  camera_centres, check, default_options, run_solver, synth_params, synthesize)
 
 __all__ = [
-    "abi", "library", "BundleProblem", "Device", "SolveOutput", "Synthetic", "camera",
+    "abi", "library", "BundleProblem", "Device", "PreparedProblem", "PrepareError", "SolveOutput", "Synthetic", "camera",
     "camera_centres", "check", "default_options", "run_solver", "synth_params", "synthesize",
 ]

@@ -203,6 +203,15 @@ SIGNATURES = {
         [C.c_void_p, C.POINTER(Problem), C.POINTER(Options), P64, P64, C.POINTER(IterationStats),
          C.c_uint32, C.POINTER(Result)],
     ),
+    "vba_prepare": (C.c_int32, [C.c_void_p, C.POINTER(Problem), C.POINTER(Options), C.POINTER(C.c_void_p),
+                                C.POINTER(Result)]),
+    "vba_solve_prepared": (C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(Options), C.POINTER(IterationStats),
+                                       C.c_uint32, C.POINTER(Result)]),
+    "vba_prepared_state": (C.c_int32, [C.c_void_p, C.c_void_p, P64, P64]),
+    "vba_prepared_free": (None, [C.c_void_p, C.c_void_p]),
+    "vba_profile_enable": (None, [C.c_void_p, C.c_int32]),
+    "vba_profile_reset": (None, [C.c_void_p]),
+    "vba_profile_query": (C.c_int32, [C.c_void_p, C.c_char_p, PU64, P64, P64]),
     "vba_synth_default": (None, [C.c_uint32, C.POINTER(SynthParams)]),
     "vba_synth_generate": (C.c_void_p, [C.POINTER(SynthParams)]),
     "vba_synth_free": (None, [C.c_void_p]),

@@ -259,6 +259,21 @@ class Device:
     def last_error(self) -> str:
         return self.lib.vba_last_error(self.ctx).decode()
 
+    def prepare(self, problem: BundleProblem, options: abi.Options) -> "PreparedProblem":
+        return PreparedProblem(self, problem, options)
+
+    def profile(self, enable: bool = True, reset: bool = True):
+        if reset:
+            self.lib.vba_profile_reset(self.ctx)
+        self.lib.vba_profile_enable(self.ctx, 1 if enable else 0)
+
+    def profile_query(self, kernel: str) -> dict:
+        launches = C.c_uint64()
+        secs = C.c_double()
+        nbytes = C.c_double()
+        self.lib.vba_profile_query(self.ctx, kernel.encode(), C.byref(launches), C.byref(secs), C.byref(nbytes))
+        return {"launches": int(launches.value), "seconds": float(secs.value), "bytes": float(nbytes.value)}
+
     def optimize(self, problem: BundleProblem, options: abi.Options, cap: int = 1024) -> SolveOutput:
         def call(view, opts, poses, lms, stats, cap_, res):
             return self.lib.vba_optimize(
@@ -269,6 +284,61 @@ class Device:
         if out.status != abi.OK:
             out.message = self.last_error()
         return out
+
+
+class PreparedProblem:
+    """A problem laid out and resident on the device (vba_prepare). solve()
+    starts from the prepared initial state each time; state() reads back the
+    final state of the last solve."""
+
+    def __init__(self, device: "Device", problem: BundleProblem, options: abi.Options):
+        self.device = device
+        self.problem = problem
+        self.handle = C.c_void_p()
+        res = abi.Result()
+        view = problem.view()
+        st = device.lib.vba_prepare(device.ctx, C.byref(view), C.byref(options), C.byref(self.handle), C.byref(res))
+        if st != abi.OK:
+            self.handle = None
+            raise PrepareError(st, device.last_error(), res.error_id)
+
+    def solve(self, options: abi.Options, cap: int = 1024) -> SolveOutput:
+        stats = (abi.IterationStats * cap)()
+        res = abi.Result()
+        st = self.device.lib.vba_solve_prepared(self.device.ctx, self.handle, C.byref(options), stats, cap,
+                                                C.byref(res))
+        n = min(res.num_iterations, cap)
+        out = SolveOutput(st, res, stats_list(stats, n) if st == abi.OK else [], None, None)
+        if st != abi.OK:
+            out.message = self.device.last_error()
+        return out
+
+    def state(self):
+        poses = np.empty_like(self.problem.poses)
+        lms = np.empty_like(self.problem.landmarks)
+        st = self.device.lib.vba_prepared_state(self.device.ctx, self.handle, poses.ctypes.data_as(abi.P64),
+                                                lms.ctypes.data_as(abi.P64))
+        if st != abi.OK:
+            raise RuntimeError(self.device.last_error())
+        return poses, lms
+
+    def free(self):
+        if self.handle:
+            self.device.lib.vba_prepared_free(self.device.ctx, self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class PrepareError(RuntimeError):
+    def __init__(self, status, message, error_id=0):
+        super().__init__(f"vba_prepare status {status}: {message}")
+        self.status = status
+        self.error_id = error_id
 
 
 def check(problem: BundleProblem, options: abi.Options) -> int:
