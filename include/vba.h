@@ -277,6 +277,27 @@ int32_t vba_prepared_state(vba_context* ctx, const vba_prepared* prepared, doubl
                            double* landmarks_out);
 void vba_prepared_free(vba_context* ctx, vba_prepared* prepared);
 
+/* ---- batched local windows ----------------------------------------------
+ * No counterpart in the reference, whose local BA calls
+ * BaAccelerator::optimize once per window (refinement.rs:1067-1072). Solves
+ * `count` independent problems together: one device-resident batch whose LM
+ * control (lambda, accept/reject, convergence) runs per window on the
+ * device, with no host round trip per iteration. Window i gets the results
+ * vba_optimize gives it alone with the dense solver: the same accept/reject
+ * sequence and the same arithmetic. `options` applies to every window and
+ * must select VBA_SOLVER_DENSE; a window has at most 256 variable poses.
+ * statuses[i] is window i's status (the code vba_optimize would return);
+ * windows with an error are skipped and their outputs left untouched.
+ * poses_out[i] / landmarks_out[i] (either array, or an entry, may be NULL)
+ * receive window i's final state in its input order; stats holds
+ * count x stats_capacity records, window i's at i * stats_capacity.
+ * Returns VBA_OK when the batch ran (see statuses), else the error that
+ * stopped the whole batch. */
+int32_t vba_optimize_batch(vba_context* ctx, uint32_t count, const vba_problem* problems,
+                           const vba_options* options, double* const* poses_out,
+                           double* const* landmarks_out, vba_iteration_stats* stats,
+                           uint32_t stats_capacity, vba_result* results, int32_t* statuses);
+
 /* ---- per-kernel device time ----------------------------------------------
  * While enabled, HIP events are recorded around the launches of the kernel
  * families "linearize", "lm_pass", "cam_pass", "pcg_update" and "cost" on

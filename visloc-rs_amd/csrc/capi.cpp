@@ -4,6 +4,8 @@
 #include <cstring>
 #include <memory>
 #include <new>
+#include <string>
+#include <vector>
 
 #include "../../include/vba.h"
 #include "host_common.hpp"
@@ -232,6 +234,53 @@ void vba_prepared_free(vba_context* ctx, vba_prepared* prepared) {
         vba::release(*prepared);
     }
     delete prepared;
+}
+
+int32_t vba_optimize_batch(vba_context* ctx, uint32_t count, const vba_problem* problems, const vba_options* options,
+                           double* const* poses_out, double* const* landmarks_out, vba_iteration_stats* stats,
+                           uint32_t stats_capacity, vba_result* results, int32_t* statuses) {
+    if (!ctx || !options || !results || !statuses || (count && !problems)) {
+        vba::set_thread_error("vba_optimize_batch: null argument");
+        if (ctx) ctx->last_error = "vba_optimize_batch: null argument";
+        return VBA_ERR_INVALID_ARGUMENT;
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->last_error.clear();
+    if (options->linear_solver != VBA_SOLVER_DENSE) {
+        ctx->last_error = "vba_optimize_batch: windows use the dense reduced solve (linear_solver = VBA_SOLVER_DENSE)";
+        return VBA_ERR_INVALID_CONFIGURATION;
+    }
+    if (hipSetDevice(ctx->device) != hipSuccess) {
+        ctx->last_error = "hipSetDevice failed";
+        return VBA_ERR_DEVICE;
+    }
+    std::vector<vba::Model> models;
+    std::vector<int32_t> valid(count, 0);
+    try {
+        models.resize(count);
+        for (uint32_t i = 0; i < count; ++i) {
+            std::memset(&results[i], 0, sizeof(vba_result));
+            std::string reason;
+            int32_t st = vba::check_support(problems[i], *options, &reason);
+            if (st == VBA_OK) {
+                uint64_t error_id = 0;
+                st = vba::build_model(problems[i], &models[i], &error_id, &reason);
+                results[i].error_id = error_id;
+            }
+            statuses[i] = st;
+            valid[i] = st == VBA_OK;
+        }
+    } catch (const std::bad_alloc&) {
+        ctx->last_error = "out of host memory";
+        return VBA_ERR_OUT_OF_MEMORY;
+    }
+    try {
+        return vba::run_batch(*ctx, models, valid, *options, poses_out, landmarks_out, stats, stats_capacity, results,
+                              statuses);
+    } catch (const std::bad_alloc&) {
+        ctx->last_error = "out of host memory";
+        return VBA_ERR_OUT_OF_MEMORY;
+    }
 }
 
 void vba_profile_enable(vba_context* ctx, int32_t enable) {

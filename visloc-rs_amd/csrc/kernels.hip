@@ -7,6 +7,7 @@
 // scheduling; no global atomics are used.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 
@@ -170,6 +171,7 @@ __global__ __launch_bounds__(kBlock) void k_linearize(DevProblem P, int store32)
     __shared__ double lds[(kBlock / 64) * 27];
     const int tile = xcd_tile(blockIdx.x, P.num_cam_tiles);
     if (tile >= P.num_cam_tiles) return;
+    if (P.wstate && !P.wstate[P.cam_tile_win[tile]].relin) return;
     const CamTile ct = P.cam_tiles[tile];
     const double* pq = P.poses + 7 * (size_t)ct.pose;
     double R[9];
@@ -265,6 +267,7 @@ __global__ __launch_bounds__(kBlock) void k_linearize(DevProblem P, int store32)
 __global__ void k_camera_finalize(DevProblem P) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P.num_pose_slots) return;
+    if (P.wstate && !P.wstate[P.pose_win[p]].relin) return;
     double acc[27];
 #pragma unroll
     for (int i = 0; i < 27; ++i) acc[i] = 0.0;
@@ -281,6 +284,7 @@ __global__ void k_camera_finalize(DevProblem P) {
 __global__ void k_landmark_reduce(DevProblem P) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= P.num_lm_slots) return;
+    if (P.wstate && !P.wstate[P.lm_win[l]].relin) return;
     double V[6] = {0, 0, 0, 0, 0, 0}, g[3] = {0, 0, 0};
     for (int e = P.lm_edge_ptr[l]; e < P.lm_edge_ptr[l + 1]; ++e) {
         const EdgeRec64& r = P.el64[e];
@@ -301,6 +305,11 @@ __global__ void k_landmark_reduce(DevProblem P) {
 __global__ void k_landmark_prepare(DevProblem P, double lambda) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= P.num_lm_slots) return;
+    if (P.wstate) {
+        const WindowState& w = P.wstate[P.lm_win[l]];
+        if (!w.active) return;
+        lambda = w.lambda;
+    }
     double X[3] = {P.lms[3 * (size_t)l], P.lms[3 * (size_t)l + 1], P.lms[3 * (size_t)l + 2]};
     double inv[6] = {0, 0, 0, 0, 0, 0}, y[3] = {0, 0, 0};
     if (P.lm_var[l] >= 0) {
@@ -527,6 +536,7 @@ __global__ __launch_bounds__(kBlock) void k_cam_pass(DevProblem P, const Rec* __
     if (run_flag && *run_flag != 0) return;
     const int tile = xcd_tile(blockIdx.x, P.num_cam_tiles);
     if (tile >= P.num_cam_tiles) return;
+    if (P.wstate && !P.wstate[P.cam_tile_win[tile]].active) return;
     const CamTile ct = P.cam_tiles[tile];
     if (P.pose_var[ct.pose] < 0) return;
     T acc[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
@@ -775,6 +785,7 @@ __global__ __launch_bounds__(kSmallBlock) void k_pcg_true_residual(DevProblem P,
 // implicit_schur.rs:789-824); trial landmark X + delta_l.
 __global__ __launch_bounds__(kBlock) void k_backsub(DevProblem P) {
     __shared__ double lds[kBlock / 64];
+    if (P.wstate && !P.wstate[P.lm_tile_win[blockIdx.x]].active) return;
     const LmTile lt = P.lm_tiles[blockIdx.x];
     const int l = lt.lm_begin + (int)threadIdx.x;
     double step = 0.0;
@@ -814,7 +825,7 @@ __global__ __launch_bounds__(kBlock) void k_pose_update(DevProblem P) {
     __shared__ double lds[kBlock / 64];
     const int p = blockIdx.x * kBlock + threadIdx.x;
     double step = 0.0;
-    if (p < P.num_pose_slots) {
+    if (p < P.num_pose_slots && !(P.wstate && !P.wstate[P.pose_win[p]].active)) {
         const double* s = P.poses + 7 * (size_t)p;
         double* o = P.poses_trial + 7 * (size_t)p;
         if (P.pose_var[p] >= 0) {
@@ -842,6 +853,7 @@ __global__ __launch_bounds__(kBlock) void k_cost(DevProblem P, int trial) {
     __shared__ double lds[(kBlock / 64) * 3];
     const int tile = xcd_tile(blockIdx.x, P.num_cam_tiles);
     if (tile >= P.num_cam_tiles) return;
+    if (P.wstate && !P.wstate[P.cam_tile_win[tile]].active) return;
     const CamTile ct = P.cam_tiles[tile];
     const double* poses = trial ? P.poses_trial : P.poses;
     const double* lms = trial ? P.lms_trial : P.lms;
@@ -940,16 +952,17 @@ __global__ __launch_bounds__(kSmallBlock) void k_scalars_finalize(DevProblem P, 
 // Dense reduced camera system (LinearSolver::Dense): explicit S and a
 // one-work-group Cholesky with an LU fallback (pose_graph.rs:2259-2270).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_dense_assemble(DevProblem P, DensePairs D, double* S, int n,
-                                                           double lambda) {
-    __shared__ double lds[(kBlock / 64) * 36];
-    const int b = blockIdx.x;
+// One 6x6 block (p, q) of S = U + lambda I - W V^-1 W^T, written at the
+// variable indices local to its system (and mirrored when p != q).
+template <int NT>
+__device__ void dense_assemble_block(const DevProblem& P, const DensePairs& D, int b, double* S, int n, int var_begin,
+                                     double lambda, double* lds) {
     const int sp = D.block_pq[2 * b], sq = D.block_pq[2 * b + 1];
-    const int vp = P.pose_var[sp], vq = P.pose_var[sq];
+    const int vp = P.pose_var[sp] - var_begin, vq = P.pose_var[sq] - var_begin;
     double acc[36];
 #pragma unroll
     for (int i = 0; i < 36; ++i) acc[i] = 0.0;
-    for (int k = D.block_ptr[b] + (int)threadIdx.x; k < D.block_ptr[b + 1]; k += kBlock) {
+    for (int k = D.block_ptr[b] + (int)threadIdx.x; k < D.block_ptr[b + 1]; k += NT) {
         const int l = D.terms[3 * (size_t)k], ep = D.terms[3 * (size_t)k + 1], eq = D.terms[3 * (size_t)k + 2];
         const double* Vi = P.Vinv64 + 6 * (size_t)l;
         const double vi[9] = {Vi[0], Vi[1], Vi[2], Vi[1], Vi[3], Vi[4], Vi[2], Vi[4], Vi[5]};
@@ -981,7 +994,7 @@ __global__ __launch_bounds__(kBlock) void k_dense_assemble(DevProblem P, DensePa
                 acc[6 * (3 + i) + 3 + j] -= KCK[3 * i + j];
             }
     }
-    block_sum<kBlock, double, 36>(acc, lds);
+    block_sum<NT, double, 36>(acc, lds);
     if (threadIdx.x == 0) {
         if (sp == sq) {
             const double* U = P.U + 21 * (size_t)sp;
@@ -1000,22 +1013,27 @@ __global__ __launch_bounds__(kBlock) void k_dense_assemble(DevProblem P, DensePa
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_dense_assemble(DevProblem P, DensePairs D, double* S, int n,
+                                                           double lambda) {
+    __shared__ double lds[(kBlock / 64) * 36];
+    dense_assemble_block<kBlock>(P, D, blockIdx.x, S, n, 0, lambda, lds);
+}
+
 // Cholesky (left-looking, lower triangle in place), then forward/back
 // substitution; on a non-positive pivot an LU with partial pivoting on the
-// saved copy. One work-group.
-__global__ __launch_bounds__(kSmallBlock) void k_dense_solve(DevProblem P, DensePairs D, double* S, double* work,
-                                                             int n, int* status) {
-    __shared__ int fail;
-    __shared__ double piv_val[kSmallBlock];
-    __shared__ int piv_idx[kSmallBlock];
-    double* A2 = work;                    // n*n copy for the LU fallback
-    double* y = work + (size_t)n * n;     // n
-    for (size_t k = threadIdx.x; k < (size_t)n * n; k += kSmallBlock) A2[k] = S[k];
-    for (int i = threadIdx.x; i < n; i += kSmallBlock) y[i] = P.b[6 * (size_t)D.var_slot[i / 6] + (i % 6)];
-    if (threadIdx.x == 0) fail = 0;
+// saved copy. Solves S x = b for the n = 6 * (variable poses) unknowns whose
+// pose slots are var_slot[0 .. n/6). One work-group of NT threads.
+template <int NT>
+__device__ void dense_solve_body(const DevProblem& P, const int32_t* var_slot, double* S, double* work, int n,
+                                 int* status, int* fail_sh, double* piv_val, int* piv_idx) {
+    double* A2 = work;                 // n*n copy for the LU fallback
+    double* y = work + (size_t)n * n;  // n
+    for (size_t k = threadIdx.x; k < (size_t)n * n; k += NT) A2[k] = S[k];
+    for (int i = threadIdx.x; i < n; i += NT) y[i] = P.b[6 * (size_t)var_slot[i / 6] + (i % 6)];
+    if (threadIdx.x == 0) *fail_sh = 0;
     __syncthreads();
     for (int j = 0; j < n; ++j) {
-        for (int i = j + threadIdx.x; i < n; i += kSmallBlock) {
+        for (int i = j + threadIdx.x; i < n; i += NT) {
             double s = S[(size_t)i * n + j];
             const double* Li = S + (size_t)i * n;
             const double* Lj = S + (size_t)j * n;
@@ -1025,39 +1043,39 @@ __global__ __launch_bounds__(kSmallBlock) void k_dense_solve(DevProblem P, Dense
         __syncthreads();
         const double d = S[(size_t)j * n + j];
         if (!(d > 0.0) || !isfinite(d)) {
-            if (threadIdx.x == 0) fail = 1;
+            if (threadIdx.x == 0) *fail_sh = 1;
             break;
         }
         const double ljj = sqrt(d);
-        for (int i = j + 1 + threadIdx.x; i < n; i += kSmallBlock) S[(size_t)i * n + j] /= ljj;
+        for (int i = j + 1 + threadIdx.x; i < n; i += NT) S[(size_t)i * n + j] /= ljj;
         __syncthreads();
         if (threadIdx.x == 0) S[(size_t)j * n + j] = ljj;
         __syncthreads();
     }
     __syncthreads();
-    if (!fail) {
+    if (!*fail_sh) {
         for (int j = 0; j < n; ++j) {  // L y = b
             if (threadIdx.x == 0) y[j] = y[j] / S[(size_t)j * n + j];
             __syncthreads();
             const double yj = y[j];
-            for (int i = j + 1 + threadIdx.x; i < n; i += kSmallBlock) y[i] -= S[(size_t)i * n + j] * yj;
+            for (int i = j + 1 + threadIdx.x; i < n; i += NT) y[i] -= S[(size_t)i * n + j] * yj;
             __syncthreads();
         }
         for (int j = n - 1; j >= 0; --j) {  // L^T x = y
             if (threadIdx.x == 0) y[j] = y[j] / S[(size_t)j * n + j];
             __syncthreads();
             const double xj = y[j];
-            for (int i = threadIdx.x; i < j; i += kSmallBlock) y[i] -= S[(size_t)j * n + i] * xj;
+            for (int i = threadIdx.x; i < j; i += NT) y[i] -= S[(size_t)j * n + i] * xj;
             __syncthreads();
         }
     } else {
-        // LU with partial pivoting on A2; y still holds b.
-        for (int i = threadIdx.x; i < n; i += kSmallBlock) y[i] = P.b[6 * (size_t)D.var_slot[i / 6] + (i % 6)];
+        // LU with partial pivoting on A2; y holds b again.
+        for (int i = threadIdx.x; i < n; i += NT) y[i] = P.b[6 * (size_t)var_slot[i / 6] + (i % 6)];
         __syncthreads();
         for (int k = 0; k < n; ++k) {
             double best = -1.0;
             int bi = k;
-            for (int i = k + threadIdx.x; i < n; i += kSmallBlock) {
+            for (int i = k + threadIdx.x; i < n; i += NT) {
                 const double v = fabs(A2[(size_t)i * n + k]);
                 if (v > best) {
                     best = v;
@@ -1070,7 +1088,7 @@ __global__ __launch_bounds__(kSmallBlock) void k_dense_solve(DevProblem P, Dense
             if (threadIdx.x == 0) {
                 double bv = piv_val[0];
                 int bidx = piv_idx[0];
-                for (int t = 1; t < kSmallBlock; ++t)
+                for (int t = 1; t < NT; ++t)
                     if (piv_val[t] > bv || (piv_val[t] == bv && piv_idx[t] < bidx)) {
                         bv = piv_val[t];
                         bidx = piv_idx[t];
@@ -1084,7 +1102,7 @@ __global__ __launch_bounds__(kSmallBlock) void k_dense_solve(DevProblem P, Dense
             __syncthreads();
             if (pv == 0.0) continue;
             if (pr != k) {
-                for (int c = threadIdx.x; c < n; c += kSmallBlock) {
+                for (int c = threadIdx.x; c < n; c += NT) {
                     const double t = A2[(size_t)k * n + c];
                     A2[(size_t)k * n + c] = A2[(size_t)pr * n + c];
                     A2[(size_t)pr * n + c] = t;
@@ -1097,10 +1115,10 @@ __global__ __launch_bounds__(kSmallBlock) void k_dense_solve(DevProblem P, Dense
             }
             __syncthreads();
             const double inv = 1.0 / A2[(size_t)k * n + k];
-            for (int i = k + 1 + threadIdx.x; i < n; i += kSmallBlock) A2[(size_t)i * n + k] *= inv;
+            for (int i = k + 1 + threadIdx.x; i < n; i += NT) A2[(size_t)i * n + k] *= inv;
             __syncthreads();
             const int rows = n - k - 1;
-            for (int idx = threadIdx.x; idx < rows * rows; idx += kSmallBlock) {
+            for (int idx = threadIdx.x; idx < rows * rows; idx += NT) {
                 const int i = k + 1 + idx / rows, c = k + 1 + idx % rows;
                 A2[(size_t)i * n + c] -= A2[(size_t)i * n + k] * A2[(size_t)k * n + c];
             }
@@ -1113,24 +1131,209 @@ __global__ __launch_bounds__(kSmallBlock) void k_dense_solve(DevProblem P, Dense
         for (int j = n - 1; j >= 0; --j) {
             const double dj = A2[(size_t)j * n + j];
             if (dj == 0.0) {
-                if (threadIdx.x == 0) fail = 2;
+                if (threadIdx.x == 0) *fail_sh = 2;
                 break;
             }
             if (threadIdx.x == 0) y[j] = y[j] / dj;
             __syncthreads();
             const double xj = y[j];
-            for (int i = threadIdx.x; i < j; i += kSmallBlock) y[i] -= A2[(size_t)i * n + j] * xj;
+            for (int i = threadIdx.x; i < j; i += NT) y[i] -= A2[(size_t)i * n + j] * xj;
             __syncthreads();
         }
     }
     __syncthreads();
-    const int f = fail;
-    for (int p = threadIdx.x; p < P.num_pose_slots; p += kSmallBlock)
-        for (int i = 0; i < 6; ++i) P.x[6 * (size_t)p + i] = 0.0;
-    __syncthreads();
-    if (f != 2)
-        for (int i = threadIdx.x; i < n; i += kSmallBlock) P.x[6 * (size_t)D.var_slot[i / 6] + (i % 6)] = y[i];
+    const int f = *fail_sh;
+    for (int i = threadIdx.x; i < n; i += NT) P.x[6 * (size_t)var_slot[i / 6] + (i % 6)] = (f != 2) ? y[i] : 0.0;
     if (threadIdx.x == 0) *status = (f == 2) ? 1 : 0;
+}
+
+__global__ __launch_bounds__(kSmallBlock) void k_dense_solve(DevProblem P, DensePairs D, double* S, double* work,
+                                                             int n, int* status) {
+    __shared__ int fail;
+    __shared__ double piv_val[kSmallBlock];
+    __shared__ int piv_idx[kSmallBlock];
+    dense_solve_body<kSmallBlock>(P, D.var_slot, S, work, n, status, &fail, piv_val, piv_idx);
+}
+
+// ---------------------------------------------------------------------------
+// Batches of independent windows: the LM loop of lm.rs:162-1017 run for
+// every window at once, with per-window lambda, cost and accept/reject kept
+// on the device (no host round trip per iteration).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_batch_dense_assemble(DevProblem P, DensePairs D, BatchDev B) {
+    __shared__ double lds[(kBlock / 64) * 36];
+    const int b = blockIdx.x;
+    const int w = P.pose_win[D.block_pq[2 * b]];
+    const WindowState& st = B.state[w];
+    if (!st.active) return;
+    const WindowLayout L = B.layout[w];
+    dense_assemble_block<kBlock>(P, D, b, B.S + L.s_offset, 6 * (L.var_end - L.var_begin), L.var_begin, st.lambda,
+                                 lds);
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_dense_zero(DevProblem P, BatchDev B) {
+    const int w = blockIdx.y;
+    if (!B.state[w].active) return;
+    const WindowLayout L = B.layout[w];
+    const size_t n = 6 * (size_t)(L.var_end - L.var_begin);
+    double* S = B.S + L.s_offset;
+    for (size_t k = blockIdx.x * (size_t)kBlock + threadIdx.x; k < n * n; k += (size_t)gridDim.x * kBlock) S[k] = 0.0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_dense_solve(DevProblem P, DensePairs D, BatchDev B) {
+    __shared__ int fail;
+    __shared__ double piv_val[kBlock];
+    __shared__ int piv_idx[kBlock];
+    const int w = blockIdx.x;
+    WindowState& st = B.state[w];
+    if (!st.active) return;
+    const WindowLayout L = B.layout[w];
+    const int n = 6 * (L.var_end - L.var_begin);
+    if (n == 0) {
+        if (threadIdx.x == 0) st.solve_status = 0;
+        return;
+    }
+    dense_solve_body<kBlock>(P, D.var_slot + L.var_begin, B.S + L.s_offset, B.work + B.work_offset[w], n,
+                             &st.solve_status, &fail, piv_val, piv_idx);
+}
+
+// Per window: compensated cost and non-projectable count over its camera
+// tiles (tile order), and the largest pose / landmark step.
+__global__ __launch_bounds__(kBlock) void k_batch_scalars(DevProblem P, BatchDev B) {
+    __shared__ double ss[kBlock], sc[kBlock];
+    __shared__ double lds[(kBlock / 64) * 3];
+    const int w = blockIdx.x;
+    WindowState& st = B.state[w];
+    if (!st.active) return;
+    const WindowLayout L = B.layout[w];
+    const int nt = L.tile_end - L.tile_begin;
+    const int per = (nt + kBlock - 1) / kBlock;
+    CSum cs{0.0, 0.0};
+    double v[3] = {0.0, 0.0, 0.0};  // non-projectable, max pose step, max landmark step
+    for (int i = L.tile_begin + threadIdx.x * per; i < min(L.tile_end, L.tile_begin + (int)(threadIdx.x + 1) * per); ++i) {
+        const double* t = P.tile_scalar + 4 * (size_t)i;
+        cs = csum_merge(cs, CSum{t[0], t[1]});
+        v[0] += t[2];
+    }
+    for (int p = L.pose_begin + threadIdx.x; p < L.pose_end; p += kBlock) {
+        if (P.pose_var[p] < 0) continue;
+        const double* xi = P.x + 6 * (size_t)p;
+        double s2 = 0.0;
+        for (int i = 0; i < 6; ++i) s2 += xi[i] * xi[i];
+        v[1] = fmax(v[1], sqrt(s2));
+    }
+    for (int l = L.lm_begin + threadIdx.x; l < L.lm_end; l += kBlock) {
+        const double* d = P.delta_l + 3 * (size_t)l;
+        v[2] = fmax(v[2], sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]));
+    }
+    ss[threadIdx.x] = cs.s;
+    sc[threadIdx.x] = cs.c;
+    double cnt[1] = {v[0]};
+    block_sum<kBlock, double, 1>(cnt, lds);
+    const double mp = block_max<kBlock>(v[1], lds);
+    const double ml = block_max<kBlock>(v[2], lds);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        CSum a{ss[0], sc[0]};
+        for (int i = 1; i < kBlock; ++i) a = csum_merge(a, CSum{ss[i], sc[i]});
+        st.cost_after = a.s + a.c;
+        st.nonproj_after = (uint64_t)cnt[0];
+        st.max_pose_step = mp;
+        st.max_landmark_step = ml;
+    }
+}
+
+// Initial LM state (lm.rs:323-333) from the cost of the current state.
+__global__ void k_batch_init(BatchDev B, double initial_lambda) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= B.num_windows) return;
+    WindowState& st = B.state[w];
+    if (!st.active) return;
+    st.current_cost = st.cost_after;
+    st.initial_cost = st.cost_after;
+    st.current_nonproj = st.nonproj_after;
+    st.lambda = initial_lambda;
+    st.relin = 1;
+    st.accept = 0;
+    st.converged = 0;
+    st.iterations = 0;
+}
+
+struct ControlOptions {
+    int32_t has_initial_lambda, has_relative_cost_tolerance;
+    double increase, decrease, max_lambda, min_lambda, step_tolerance, cost_tolerance, relative_cost_tolerance;
+};
+
+// Accept / reject and lambda update of one LM iteration per window
+// (lm.rs:610-662, 804-1008).
+__global__ void k_batch_control(BatchDev B, ControlOptions o, int iteration) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= B.num_windows) return;
+    WindowState& st = B.state[w];
+    st.accept = 0;
+    st.relin = 0;
+    if (!st.active) return;
+    vba_iteration_stats rec;
+    rec.iteration = (uint32_t)iteration;
+    rec.step_accepted = 0;
+    rec.cost_before = st.current_cost;
+    rec.pcg_iterations = -1;
+    rec.pcg_residual_norm = 0.0;
+    rec.pcg_target = 0.0;
+    bool stop = false;
+    if (st.solve_status != 0) {  // singular reduced system: a rejected step
+        st.lambda = fmin(st.lambda * o.increase, o.max_lambda);
+        rec.cost_after = st.current_cost;
+        rec.max_pose_step = 0.0;
+        rec.max_landmark_step = 0.0;
+        rec.lambda = st.lambda;
+        rec.nonprojectable_after = (uint32_t)st.current_nonproj;
+        stop = st.lambda >= o.max_lambda;
+    } else {
+        const double cost_before = st.current_cost;
+        const bool cost_ok = o.has_initial_lambda ? (st.cost_after < cost_before) : true;
+        const bool accepted = cost_ok && st.nonproj_after <= st.current_nonproj;
+        rec.cost_after = st.cost_after;
+        rec.max_pose_step = st.max_pose_step;
+        rec.max_landmark_step = st.max_landmark_step;
+        rec.nonprojectable_after = (uint32_t)st.nonproj_after;
+        if (!accepted) {
+            st.lambda = fmin(st.lambda * o.increase, o.max_lambda);
+            rec.lambda = st.lambda;
+            stop = !o.has_initial_lambda || st.lambda >= o.max_lambda;
+        } else {
+            rec.lambda = st.lambda;
+            rec.step_accepted = 1;
+            st.accept = 1;
+            st.relin = 1;
+            st.current_cost = st.cost_after;
+            st.current_nonproj = st.nonproj_after;
+            if (o.has_initial_lambda) st.lambda = fmax(st.lambda * o.decrease, o.min_lambda);
+            bool conv = (st.max_pose_step < o.step_tolerance && st.max_landmark_step < o.step_tolerance) ||
+                        fabs(cost_before - st.cost_after) < o.cost_tolerance;
+            if (!conv && o.has_relative_cost_tolerance && isfinite(o.relative_cost_tolerance) &&
+                o.relative_cost_tolerance >= 0.0 &&
+                (cost_before - st.cost_after) / fmax(fabs(cost_before), 2.220446049250313e-16) <
+                    o.relative_cost_tolerance)
+                conv = true;
+            if (conv) {
+                st.converged = 1;
+                stop = true;
+            }
+        }
+    }
+    if (st.iterations < B.stats_capacity) B.stats[(size_t)w * B.stats_capacity + st.iterations] = rec;
+    st.iterations += 1;
+    if (stop) st.active = 0;
+}
+
+// Copy the trial state of accepted windows into the current state.
+__global__ void k_batch_commit(DevProblem P, BatchDev B) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < P.num_pose_slots && B.state[P.pose_win[i]].accept)
+        for (int k = 0; k < 7; ++k) P.poses[7 * (size_t)i + k] = P.poses_trial[7 * (size_t)i + k];
+    if (i < P.num_lm_slots && B.state[P.lm_win[i]].accept)
+        for (int k = 0; k < 3; ++k) P.lms[3 * (size_t)i + k] = P.lms_trial[3 * (size_t)i + k];
 }
 
 }  // namespace
@@ -1218,6 +1421,44 @@ void launch_dense_assemble(const DevProblem& P, const DensePairs& D, double* S, 
 void launch_dense_solve(const DevProblem& P, const DensePairs& D, double* S, double* work, int32_t n,
                         int32_t* status, hipStream_t s) {
     hipLaunchKernelGGL(k_dense_solve, dim3(1), dim3(kSmallBlock), 0, s, P, D, S, work, n, status);
+}
+void launch_batch_dense_assemble(const DevProblem& P, const DensePairs& D, const BatchDev& B, hipStream_t s) {
+    if (D.num_blocks) hipLaunchKernelGGL(k_batch_dense_assemble, dim3(D.num_blocks), dim3(kBlock), 0, s, P, D, B);
+}
+void launch_batch_dense_zero(const DevProblem& P, const BatchDev& B, hipStream_t s) {
+    if (B.num_windows) hipLaunchKernelGGL(k_batch_dense_zero, dim3(16, B.num_windows), dim3(kBlock), 0, s, P, B);
+}
+void launch_batch_dense_solve(const DevProblem& P, const DensePairs& D, const BatchDev& B, hipStream_t s) {
+    if (B.num_windows) hipLaunchKernelGGL(k_batch_dense_solve, dim3(B.num_windows), dim3(kBlock), 0, s, P, D, B);
+}
+void launch_batch_scalars(const DevProblem& P, const BatchDev& B, hipStream_t s) {
+    if (B.num_windows) hipLaunchKernelGGL(k_batch_scalars, dim3(B.num_windows), dim3(kBlock), 0, s, P, B);
+}
+void launch_batch_init(const DevProblem& P, const BatchDev& B, const vba_options& o, hipStream_t s) {
+    (void)P;
+    if (B.num_windows)
+        hipLaunchKernelGGL(k_batch_init, dim3(blocks_for(B.num_windows, 256)), dim3(256), 0, s, B,
+                           o.has_initial_lambda ? o.initial_lambda : 0.0);
+}
+void launch_batch_control(const DevProblem& P, const BatchDev& B, const vba_options& o, int32_t iteration,
+                          hipStream_t s) {
+    (void)P;
+    ControlOptions c;
+    c.has_initial_lambda = (int32_t)o.has_initial_lambda;
+    c.has_relative_cost_tolerance = (int32_t)o.has_relative_cost_tolerance;
+    c.increase = o.lambda_increase_factor;
+    c.decrease = o.lambda_decrease_factor;
+    c.max_lambda = o.max_lambda;
+    c.min_lambda = o.min_lambda;
+    c.step_tolerance = o.step_tolerance;
+    c.cost_tolerance = o.cost_tolerance;
+    c.relative_cost_tolerance = o.relative_cost_tolerance;
+    if (B.num_windows)
+        hipLaunchKernelGGL(k_batch_control, dim3(blocks_for(B.num_windows, 256)), dim3(256), 0, s, B, c, iteration);
+}
+void launch_batch_commit(const DevProblem& P, const BatchDev& B, hipStream_t s) {
+    const int n = std::max(P.num_pose_slots, P.num_lm_slots);
+    hipLaunchKernelGGL(k_batch_commit, dim3(blocks_for(n, 256)), dim3(256), 0, s, P, B);
 }
 
 }  // namespace vba

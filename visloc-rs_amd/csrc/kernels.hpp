@@ -18,6 +18,7 @@
 
 #include <cstdint>
 
+#include "../../include/vba.h"
 #include "geom.hpp"
 #include "problem.hpp"
 
@@ -72,6 +73,14 @@ enum PcgStatus : int32_t {
     PCG_NONFINITE = -2
 };
 
+// Per-window LM state of a batch (device side of the LM control,
+// lm.rs:804-1008, one thread per window).
+struct WindowState {
+    double lambda, current_cost, cost_after, max_pose_step, max_landmark_step, initial_cost;
+    uint64_t current_nonproj, nonproj_after;
+    int32_t active, relin, accept, converged, iterations, solve_status;
+};
+
 // All device pointers of one problem.
 struct DevProblem {
     int32_t num_pose_slots, num_lm_slots, num_edges, num_obs;
@@ -122,6 +131,23 @@ struct DevProblem {
     double* delta_l;       // 3 per landmark slot
     double* tile_scalar;   // per tile partial scalars (cost sum/comp, count, max)
     StepScalars* scalars;
+    // Batches of independent windows (null for a single problem): the window
+    // of every slot and tile, each window's lambda and its control flags.
+    int32_t num_windows;
+    const int32_t* pose_win;
+    const int32_t* lm_win;
+    const int32_t* cam_tile_win;
+    const int32_t* lm_tile_win;
+    const WindowState* wstate;  // lambda, active (iterating), relin (state changed)
+};
+
+struct WindowLayout {
+    int32_t tile_begin, tile_end;  // camera tiles
+    int32_t pose_begin, pose_end;  // pose slots
+    int32_t lm_begin, lm_end;      // landmark slots
+    int32_t var_begin, var_end;    // variable pose indices
+    int64_t s_offset;              // dense S offset (doubles)
+    int64_t pad;
 };
 
 // ---- launchers (all asynchronous on `stream`) ----
@@ -162,5 +188,32 @@ void launch_dense_assemble(const DevProblem& P, const DensePairs& D, double* S, 
 // sets *status (0 ok, 1 singular). S is overwritten.
 void launch_dense_solve(const DevProblem& P, const DensePairs& D, double* S, double* work,
                         int32_t n, int32_t* status, hipStream_t stream);
+
+// ---- batches of windows (one work-group per window where noted) ----
+struct BatchDev {
+    int32_t num_windows;
+    const WindowLayout* layout;
+    WindowState* state;
+    vba_iteration_stats* stats;  // num_windows x stats_capacity
+    int32_t stats_capacity;
+    double* S;      // dense systems, WindowLayout::s_offset
+    double* work;   // per window: n*n + n doubles at s_offset + s_offset/... (see solver)
+    const int64_t* work_offset;
+};
+// Assemble every active window's dense S (lambda per window).
+void launch_batch_dense_assemble(const DevProblem& P, const DensePairs& D, const BatchDev& B, hipStream_t stream);
+// Zero the dense systems of active windows.
+void launch_batch_dense_zero(const DevProblem& P, const BatchDev& B, hipStream_t stream);
+// One work-group per window: Cholesky (LU fallback) solve into P.x.
+void launch_batch_dense_solve(const DevProblem& P, const DensePairs& D, const BatchDev& B, hipStream_t stream);
+// Per-window cost / count / step maxima of the trial (or current) state.
+void launch_batch_scalars(const DevProblem& P, const BatchDev& B, hipStream_t stream);
+// Initial LM state from the current-state cost.
+void launch_batch_init(const DevProblem& P, const BatchDev& B, const vba_options& o, hipStream_t stream);
+// Accept / reject, lambda update, convergence, stats (one thread per window).
+void launch_batch_control(const DevProblem& P, const BatchDev& B, const vba_options& o, int32_t iteration,
+                          hipStream_t stream);
+// Copy the trial state of accepted windows into the current state.
+void launch_batch_commit(const DevProblem& P, const BatchDev& B, hipStream_t stream);
 
 }  // namespace vba

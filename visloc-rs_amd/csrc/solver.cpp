@@ -72,22 +72,24 @@ T* buf(Prepared& pr, int id) {
 
 // Dense reduced system structure: one block per (var p <= var q) sharing a
 // landmark plus every diagonal block; each term is (landmark, e_p, e_q).
-void build_dense(Prepared& pr) {
-    const Model& m = pr.m;
+struct DenseLists {
+    std::vector<int32_t> block_pq, block_ptr, terms, var_slot;
+};
+void build_dense_lists(const Model& m, DenseLists* out) {
     const uint32_t P = m.P;
-    pr.dense_var_slot.assign(P, 0);
+    out->var_slot.assign(P, 0);
     for (uint32_t s = 0; s < m.num_pose_slots; ++s)
-        if (m.pose_var[s] >= 0) pr.dense_var_slot[m.pose_var[s]] = (int32_t)s;
+        if (m.pose_var[s] >= 0) out->var_slot[m.pose_var[s]] = (int32_t)s;
     std::vector<int32_t> block_id((size_t)P * P, -1);
     std::vector<int64_t> count;
-    pr.dense_block_pq.clear();
+    out->block_pq.clear();
     auto block_of = [&](int32_t a, int32_t b) {
         int32_t& id = block_id[(size_t)a * P + b];
         if (id < 0) {
             id = (int32_t)count.size();
             count.push_back(0);
-            pr.dense_block_pq.push_back(pr.dense_var_slot[a]);
-            pr.dense_block_pq.push_back(pr.dense_var_slot[b]);
+            out->block_pq.push_back(out->var_slot[a]);
+            out->block_pq.push_back(out->var_slot[b]);
         }
         return id;
     };
@@ -108,21 +110,29 @@ void build_dense(Prepared& pr) {
     int64_t total = 0;
     for (int64_t c : count) total += c;
     if (total > (int64_t)INT32_MAX / 3) throw Fail{VBA_ERR_INVALID_ARGUMENT, "dense reduced system too large"};
-    pr.dense_block_ptr.assign(nb + 1, 0);
-    for (size_t k = 0; k < nb; ++k) pr.dense_block_ptr[k + 1] = pr.dense_block_ptr[k] + (int32_t)count[k];
-    pr.dense_terms.assign(3 * (size_t)total, 0);
-    std::vector<int32_t> cursor(pr.dense_block_ptr.begin(), pr.dense_block_ptr.end() - 1);
+    out->block_ptr.assign(nb + 1, 0);
+    for (size_t k = 0; k < nb; ++k) out->block_ptr[k + 1] = out->block_ptr[k] + (int32_t)count[k];
+    out->terms.assign(3 * (size_t)total, 0);
+    std::vector<int32_t> cursor(out->block_ptr.begin(), out->block_ptr.end() - 1);
     for (uint32_t l = 0; l < m.num_lm_slots; ++l) {
         if (m.lm_var[l] < 0) continue;
         collect(l);
         for (size_t a = 0; a < ends.size(); ++a)
             for (size_t b = a; b < ends.size(); ++b) {
                 int32_t k = cursor[block_id[(size_t)ends[a].first * P + ends[b].first]]++;
-                pr.dense_terms[3 * (size_t)k] = (int32_t)l;
-                pr.dense_terms[3 * (size_t)k + 1] = ends[a].second;
-                pr.dense_terms[3 * (size_t)k + 2] = ends[b].second;
+                out->terms[3 * (size_t)k] = (int32_t)l;
+                out->terms[3 * (size_t)k + 1] = ends[a].second;
+                out->terms[3 * (size_t)k + 2] = ends[b].second;
             }
     }
+}
+void build_dense(Prepared& pr) {
+    DenseLists d;
+    build_dense_lists(pr.m, &d);
+    pr.dense_block_pq = std::move(d.block_pq);
+    pr.dense_block_ptr = std::move(d.block_ptr);
+    pr.dense_terms = std::move(d.terms);
+    pr.dense_var_slot = std::move(d.var_slot);
     pr.dense_built = true;
 }
 
@@ -672,6 +682,312 @@ int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stat
             ctx.event_pool.push_back(p.b);
         }
         ctx.pending.clear();
+        return f.code;
+    }
+}
+
+// ===========================================================================
+// Batches of independent windows
+// ===========================================================================
+namespace {
+
+enum BatchBufId {
+    BB_LAYOUT = B_COUNT, BB_STATE, BB_STATS, BB_POSE_WIN, BB_LM_WIN, BB_CAM_TILE_WIN, BB_LM_TILE_WIN,
+    BB_S, BB_WORK, BB_WORK_OFFSET, BB_COUNT
+};
+
+// Concatenate per-window models into one problem with disjoint slots.
+void merge_models(const std::vector<const Model*>& ms, Model* out, std::vector<WindowLayout>* layout,
+                  std::vector<int32_t>* pose_win, std::vector<int32_t>* lm_win, std::vector<int32_t>* cam_tile_win,
+                  std::vector<int32_t>* lm_tile_win, const std::vector<DenseLists>& dense, DenseLists* dense_out) {
+    Model& M = *out;
+    M = Model();
+    M.any_rig = true;
+    layout->clear();
+    pose_win->clear();
+    lm_win->clear();
+    cam_tile_win->clear();
+    lm_tile_win->clear();
+    *dense_out = DenseLists();
+    int64_t po = 0, lo = 0, eo = 0, oo = 0, to = 0, vo = 0, vlo = 0, so = 0, soff = 0, term_off = 0;
+    for (size_t w = 0; w < ms.size(); ++w) {
+        const Model& m = *ms[w];
+        WindowLayout L{};
+        L.tile_begin = (int32_t)to;
+        L.tile_end = (int32_t)(to + m.cam_tiles.size());
+        L.pose_begin = (int32_t)po;
+        L.pose_end = (int32_t)(po + m.num_pose_slots);
+        L.lm_begin = (int32_t)lo;
+        L.lm_end = (int32_t)(lo + m.num_lm_slots);
+        L.var_begin = (int32_t)vo;
+        L.var_end = (int32_t)(vo + m.P);
+        L.s_offset = soff;
+        layout->push_back(L);
+        soff += (int64_t)36 * m.P * m.P;
+        for (uint32_t s2 = 0; s2 < m.num_pose_slots; ++s2) {
+            M.pose_var.push_back(m.pose_var[s2] >= 0 ? (int32_t)(m.pose_var[s2] + vo) : -1);
+            pose_win->push_back((int32_t)w);
+        }
+        for (uint32_t l = 0; l < m.num_lm_slots; ++l) {
+            M.lm_var.push_back(m.lm_var[l] >= 0 ? (int32_t)(m.lm_var[l] + vlo) : -1);
+            lm_win->push_back((int32_t)w);
+        }
+        M.poses.insert(M.poses.end(), m.poses.begin(), m.poses.end());
+        M.lms.insert(M.lms.end(), m.lms.begin(), m.lms.end());
+        M.sensors.insert(M.sensors.end(), m.sensors.begin(), m.sensors.end());
+        M.obs_uv.insert(M.obs_uv.end(), m.obs_uv.begin(), m.obs_uv.end());
+        for (uint64_t o = 0; o < m.num_obs; ++o)
+            M.obs_sensor.push_back((uint16_t)(so + (m.any_rig ? m.obs_sensor[o] : 0)));
+        for (uint64_t e = 0; e < m.num_edges; ++e) {
+            M.edge_pose.push_back((int32_t)(m.edge_pose[e] + po));
+            M.edge_lm.push_back((int32_t)(m.edge_lm[e] + lo));
+            M.edge_obs.push_back((int32_t)(m.edge_obs[e] + oo));
+            M.l_pose.push_back((int32_t)(m.l_pose[e] + po));
+            M.perm_cl.push_back((int32_t)(m.perm_cl[e] + eo));
+        }
+        for (uint32_t s2 = 0; s2 < m.num_pose_slots; ++s2) {
+            M.cam_edge_ptr.push_back((int32_t)(m.cam_edge_ptr[s2] + eo));
+            M.cam_tile_ptr.push_back((int32_t)(m.cam_tile_ptr[s2] + to));
+        }
+        for (uint32_t l = 0; l < m.num_lm_slots; ++l) M.lm_edge_ptr.push_back((int32_t)(m.lm_edge_ptr[l] + eo));
+        for (const CamTile& t : m.cam_tiles) {
+            M.cam_tiles.push_back({(int32_t)(t.pose + po), (int32_t)(t.begin + eo), (int32_t)(t.end + eo), 0});
+            cam_tile_win->push_back((int32_t)w);
+        }
+        for (const LmTile& t : m.lm_tiles) {
+            M.lm_tiles.push_back({(int32_t)(t.lm_begin + lo), (int32_t)(t.lm_end + lo)});
+            lm_tile_win->push_back((int32_t)w);
+        }
+        M.max_lm_tile_edges = std::max(M.max_lm_tile_edges, m.max_lm_tile_edges);
+        if (m.any_fixed_pose) M.any_fixed_pose = true;
+        const DenseLists& d = dense[w];
+        for (size_t k = 0; k < d.block_pq.size(); ++k) dense_out->block_pq.push_back((int32_t)(d.block_pq[k] + po));
+        for (size_t k = 0; k + 1 < d.block_ptr.size(); ++k)
+            dense_out->block_ptr.push_back((int32_t)(d.block_ptr[k] + term_off));
+        for (size_t k = 0; k < d.terms.size(); k += 3) {
+            dense_out->terms.push_back((int32_t)(d.terms[k] + lo));
+            dense_out->terms.push_back((int32_t)(d.terms[k + 1] + eo));
+            dense_out->terms.push_back((int32_t)(d.terms[k + 2] + eo));
+        }
+        for (int32_t v : d.var_slot) dense_out->var_slot.push_back((int32_t)(v + po));
+        term_off += d.block_ptr.empty() ? 0 : d.block_ptr.back();
+        po += m.num_pose_slots;
+        lo += m.num_lm_slots;
+        eo += (int64_t)m.num_edges;
+        oo += (int64_t)m.num_obs;
+        to += (int64_t)m.cam_tiles.size();
+        vo += m.P;
+        vlo += m.L;
+        so += (int64_t)m.sensors.size();
+    }
+    M.num_pose_slots = (uint32_t)po;
+    M.num_lm_slots = (uint32_t)lo;
+    M.P = (uint32_t)vo;
+    M.L = (uint32_t)vlo;
+    M.num_obs = (uint64_t)oo;
+    M.num_edges = (uint64_t)eo;
+    M.edge_obs.push_back((int32_t)oo);
+    M.cam_edge_ptr.push_back((int32_t)eo);
+    M.cam_tile_ptr.push_back((int32_t)to);
+    M.lm_edge_ptr.push_back((int32_t)eo);
+    dense_out->block_ptr.push_back((int32_t)term_off);
+}
+
+}  // namespace
+
+int32_t run_batch(Context& ctx, std::vector<Model>& models, const std::vector<int32_t>& valid, const vba_options& o,
+                  double* const* poses_out, double* const* landmarks_out, vba_iteration_stats* stats, uint32_t cap,
+                  vba_result* results, int32_t* statuses) {
+    hipStream_t s = ctx.stream;
+    Prepared pr;
+    try {
+        if (!ctx.host_scalars) {
+            hip_check(hipHostMalloc((void**)&ctx.host_scalars, sizeof(StepScalars), hipHostMallocDefault), "hipHostMalloc");
+            hip_check(hipHostMalloc((void**)&ctx.host_pcg, sizeof(PcgState), hipHostMallocDefault), "hipHostMalloc");
+            hip_check(hipHostMalloc((void**)&ctx.host_status, sizeof(int32_t), hipHostMallocDefault), "hipHostMalloc");
+        }
+        std::vector<const Model*> ms;
+        std::vector<int32_t> index;  // batch window -> caller index
+        std::vector<DenseLists> dense;
+        for (size_t i = 0; i < models.size(); ++i) {
+            if (!valid[i]) continue;
+            if (models[i].P > 256) {
+                statuses[i] = VBA_ERR_INVALID_ARGUMENT;
+                continue;
+            }
+            ms.push_back(&models[i]);
+            index.push_back((int32_t)i);
+            dense.emplace_back();
+            build_dense_lists(models[i], &dense.back());
+        }
+        const int32_t W = (int32_t)ms.size();
+        if (W == 0) return VBA_OK;
+        size_t num_sensors = 0;
+        for (const Model* mw : ms) num_sensors += mw->sensors.size();
+        if (num_sensors > 65535) throw Fail{VBA_ERR_INVALID_ARGUMENT, "batch holds more than 65535 sensors"};
+        std::vector<WindowLayout> layout;
+        std::vector<int32_t> pose_win, lm_win, cam_tile_win, lm_tile_win;
+        DenseLists dl;
+        merge_models(ms, &pr.m, &layout, &pose_win, &lm_win, &cam_tile_win, &lm_tile_win, dense, &dl);
+        std::string msg;
+        int32_t st = prepare(ctx, pr, &msg);
+        if (st != VBA_OK) throw Fail{st, msg};
+        const Model& m = pr.m;
+        // ---- bind (single-problem buffers + window arrays) ----
+        DevProblem P{};
+        P.num_pose_slots = (int32_t)m.num_pose_slots;
+        P.num_lm_slots = (int32_t)m.num_lm_slots;
+        P.num_edges = (int32_t)m.num_edges;
+        P.num_obs = (int32_t)m.num_obs;
+        P.num_cam_tiles = (int32_t)m.cam_tiles.size();
+        P.num_lm_tiles = (int32_t)m.lm_tiles.size();
+        P.any_rig = 1;
+        P.kernel_kind = (int32_t)o.robust_kernel;
+        P.kernel_scale = o.robust_scale;
+        const size_t NP = m.num_pose_slots, NL = m.num_lm_slots, NE = m.num_edges, NT = m.cam_tiles.size();
+        const size_t NLT = m.lm_tiles.size();
+        const size_t pose_blocks = (NP + kBlock - 1) / kBlock;
+        P.poses = buf<double>(pr, B_POSES);
+        P.lms = buf<double>(pr, B_LMS);
+        P.poses_trial = buf<double>(pr, B_POSES_T);
+        P.lms_trial = buf<double>(pr, B_LMS_T);
+        hip_check(hipMemcpyAsync(P.poses, pr.bufs[B_POSES0].ptr, m.poses.size() * sizeof(double), hipMemcpyDeviceToDevice, s), "D2D");
+        hip_check(hipMemcpyAsync(P.lms, pr.bufs[B_LMS0].ptr, m.lms.size() * sizeof(double), hipMemcpyDeviceToDevice, s), "D2D");
+        P.sensors = buf<DevSensor>(pr, B_SENSORS);
+        P.pose_var = buf<int32_t>(pr, B_POSE_VAR);
+        P.lm_var = buf<int32_t>(pr, B_LM_VAR);
+        P.obs_uv = buf<double>(pr, B_OBS_UV);
+        P.obs_sensor = buf<uint16_t>(pr, B_OBS_SENSOR);
+        P.edge_lm = buf<int32_t>(pr, B_EDGE_LM);
+        P.edge_obs = buf<int32_t>(pr, B_EDGE_OBS);
+        P.perm_cl = buf<int32_t>(pr, B_PERM_CL);
+        P.l_pose = buf<int32_t>(pr, B_L_POSE);
+        P.lm_edge_ptr = buf<int32_t>(pr, B_LM_EDGE_PTR);
+        P.cam_tiles = buf<CamTile>(pr, B_CAM_TILES);
+        P.cam_tile_ptr = buf<int32_t>(pr, B_CAM_TILE_PTR);
+        P.lm_tiles = buf<LmTile>(pr, B_LM_TILES);
+        P.ec64 = (EdgeRec64*)ensure(pr, B_EC64, NE * sizeof(EdgeRec64));
+        P.el64 = (EdgeRec64*)ensure(pr, B_EL64, NE * sizeof(EdgeRec64));
+        P.hl64 = (double*)ensure(pr, B_HL64, 3 * NE * sizeof(double));
+        P.tile27 = (double*)ensure(pr, B_TILE27, 27 * std::max<size_t>(NT, 1) * sizeof(double));
+        P.U = (double*)ensure(pr, B_U, 21 * NP * sizeof(double));
+        P.g = (double*)ensure(pr, B_G, 6 * NP * sizeof(double));
+        P.V = (double*)ensure(pr, B_V, 6 * NL * sizeof(double));
+        P.gl = (double*)ensure(pr, B_GL, 3 * NL * sizeof(double));
+        P.Vinv64 = (double*)ensure(pr, B_VINV64, 6 * NL * sizeof(double));
+        P.pack64 = (LmPack<double>*)ensure(pr, B_PACK64, NL * sizeof(LmPack<double>));
+        P.b = (double*)ensure(pr, B_VB, 6 * NP * sizeof(double));
+        P.x = (double*)ensure(pr, B_VX, 6 * NP * sizeof(double));
+        P.part64 = (double*)ensure(pr, B_PART64, 6 * std::max<size_t>(NT, 1) * sizeof(double));
+        P.delta_l = (double*)ensure(pr, B_DELTA_L, 3 * NL * sizeof(double));
+        const size_t nscalar = 4 * (NT + NLT + pose_blocks + 1);
+        P.tile_scalar = (double*)ensure(pr, B_TILE_SCALAR, nscalar * sizeof(double));
+        P.scalars = (StepScalars*)ensure(pr, B_SCALARS, sizeof(StepScalars));
+        hip_check(hipMemsetAsync(P.x, 0, 6 * NP * sizeof(double), s), "memset");
+        hip_check(hipMemsetAsync(P.delta_l, 0, 3 * NL * sizeof(double), s), "memset");
+        hip_check(hipMemsetAsync(P.tile_scalar, 0, nscalar * sizeof(double), s), "memset");
+        pr.bufs.resize(BB_COUNT);
+        P.pose_win = upload(pr, BB_POSE_WIN, pose_win, s);
+        P.lm_win = upload(pr, BB_LM_WIN, lm_win, s);
+        P.cam_tile_win = upload(pr, BB_CAM_TILE_WIN, cam_tile_win, s);
+        P.lm_tile_win = upload(pr, BB_LM_TILE_WIN, lm_tile_win, s);
+        P.num_windows = W;
+        std::vector<WindowState> init(W);
+        std::memset(init.data(), 0, W * sizeof(WindowState));
+        for (int32_t w = 0; w < W; ++w) init[w].active = 1, init[w].relin = 1;
+        WindowState* dstate = upload(pr, BB_STATE, init, s);
+        P.wstate = dstate;
+        BatchDev B{};
+        B.num_windows = W;
+        B.layout = upload(pr, BB_LAYOUT, layout, s);
+        B.state = dstate;
+        B.stats_capacity = (int32_t)std::max<uint32_t>(cap, 1);
+        B.stats = (vba_iteration_stats*)ensure(pr, BB_STATS, (size_t)W * B.stats_capacity * sizeof(vba_iteration_stats));
+        std::vector<int64_t> work_off(W);
+        int64_t s_total = 0, work_total = 0;
+        for (int32_t w = 0; w < W; ++w) {
+            const int64_t n = 6 * (int64_t)(layout[w].var_end - layout[w].var_begin);
+            s_total += n * n;
+            work_off[w] = work_total;
+            work_total += n * n + n;
+        }
+        B.S = (double*)ensure(pr, BB_S, std::max<int64_t>(s_total, 1) * sizeof(double));
+        B.work = (double*)ensure(pr, BB_WORK, std::max<int64_t>(work_total, 1) * sizeof(double));
+        B.work_offset = upload(pr, BB_WORK_OFFSET, work_off, s);
+        DensePairs D{};
+        D.num_blocks = (int32_t)(dl.block_ptr.size() - 1);
+        D.block_pq = upload(pr, B_DENSE_BLOCK_PQ, dl.block_pq, s);
+        D.block_ptr = upload(pr, B_DENSE_BLOCK_PTR, dl.block_ptr, s);
+        D.terms = upload(pr, B_DENSE_TERMS, dl.terms, s);
+        D.var_slot = upload(pr, B_DENSE_VAR_SLOT, dl.var_slot, s);
+
+        hip_check(hipEventRecord(ctx.ev_start, s), "event");
+        // ---- initial cost and state, then max_iterations LM steps with no
+        // host synchronisation: every window decides on the device ----
+        launch_cost(P, false, s);
+        launch_batch_scalars(P, B, s);
+        launch_batch_init(P, B, o, s);
+        for (uint32_t it = 0; it < o.max_iterations; ++it) {
+            launch_linearize(P, false, s);
+            launch_camera_finalize(P, s);
+            launch_landmark_reduce(P, s);
+            launch_landmark_prepare(P, 0.0, s);
+            launch_cam_pass(P, false, nullptr, s);
+            launch_rhs_finalize(P, s);
+            launch_batch_dense_zero(P, B, s);
+            launch_batch_dense_assemble(P, D, B, s);
+            launch_batch_dense_solve(P, D, B, s);
+            launch_backsub(P, s);
+            launch_pose_update(P, s);
+            launch_cost(P, true, s);
+            launch_batch_scalars(P, B, s);
+            launch_batch_control(P, B, o, (int32_t)it, s);
+            launch_batch_commit(P, B, s);
+        }
+        hip_check(hipGetLastError(), "kernel launch");
+        hip_check(hipEventRecord(ctx.ev_stop, s), "event");
+        std::vector<WindowState> fin(W);
+        std::vector<vba_iteration_stats> st_host((size_t)W * B.stats_capacity);
+        std::vector<double> poses(m.poses.size()), lms(m.lms.size());
+        hip_check(hipMemcpyAsync(fin.data(), dstate, W * sizeof(WindowState), hipMemcpyDeviceToHost, s), "D2H");
+        hip_check(hipMemcpyAsync(st_host.data(), B.stats, st_host.size() * sizeof(vba_iteration_stats), hipMemcpyDeviceToHost, s), "D2H");
+        hip_check(hipMemcpyAsync(poses.data(), P.poses, poses.size() * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+        hip_check(hipMemcpyAsync(lms.data(), P.lms, lms.size() * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+        hip_check(hipStreamSynchronize(s), "sync");
+        float ms_elapsed = 0.f;
+        hip_check(hipEventElapsedTime(&ms_elapsed, ctx.ev_start, ctx.ev_stop), "event");
+        for (int32_t w = 0; w < W; ++w) {
+            const int32_t i = index[w];
+            const Model& mw = models[i];
+            const WindowLayout& L = layout[w];
+            if (poses_out && poses_out[i])
+                for (size_t k = 0; k < mw.pose_slot_of_input.size(); ++k)
+                    std::memcpy(poses_out[i] + 7 * k, poses.data() + 7 * ((size_t)L.pose_begin + mw.pose_slot_of_input[k]),
+                                7 * sizeof(double));
+            if (landmarks_out && landmarks_out[i])
+                for (size_t k = 0; k < mw.lm_slot_of_input.size(); ++k)
+                    std::memcpy(landmarks_out[i] + 3 * k, lms.data() + 3 * ((size_t)L.lm_begin + mw.lm_slot_of_input[k]),
+                                3 * sizeof(double));
+            const uint32_t n = (uint32_t)std::min<int32_t>(fin[w].iterations, B.stats_capacity);
+            if (stats)
+                std::memcpy(stats + (size_t)i * cap, st_host.data() + (size_t)w * B.stats_capacity,
+                            std::min(n, cap) * sizeof(vba_iteration_stats));
+            vba_result& r = results[i];
+            r.initial_cost = fin[w].initial_cost;
+            r.final_cost = fin[w].current_cost;
+            r.num_iterations = std::min(n, cap);
+            r.converged = fin[w].converged;
+            r.num_observations_used = mw.num_obs;
+            r.linearizations = 0;
+            r.device_seconds = ms_elapsed * 1e-3;
+            statuses[i] = VBA_OK;
+        }
+        release(pr);
+        return VBA_OK;
+    } catch (const Fail& f) {
+        ctx.last_error = f.msg;
+        (void)hipStreamSynchronize(s);
+        release(pr);
         return f.code;
     }
 }

@@ -77,6 +77,13 @@ int32_t run(Context& ctx, Prepared& prep, const vba_options& o, vba_iteration_st
 // Final (or, before any solve, initial) poses / landmarks in input order.
 int32_t download_state(Context& ctx, const Prepared& prep, double* poses_out, double* landmarks_out);
 
+// Solve independent windows together (one device-resident batch, LM control
+// on the device). models[i] is window i, valid[i] = 0 skips it; outputs are
+// indexed by window; statuses[i] receives each window's status.
+int32_t run_batch(Context& ctx, std::vector<Model>& models, const std::vector<int32_t>& valid,
+                  const vba_options& o, double* const* poses_out, double* const* landmarks_out,
+                  vba_iteration_stats* stats, uint32_t stats_capacity, vba_result* results, int32_t* statuses);
+
 void release(Prepared& prep);
 void release_context(Context& ctx);
 

@@ -209,6 +209,11 @@ SIGNATURES = {
                                        C.c_uint32, C.POINTER(Result)]),
     "vba_prepared_state": (C.c_int32, [C.c_void_p, C.c_void_p, P64, P64]),
     "vba_prepared_free": (None, [C.c_void_p, C.c_void_p]),
+    "vba_optimize_batch": (
+        C.c_int32,
+        [C.c_void_p, C.c_uint32, C.POINTER(Problem), C.POINTER(Options), C.POINTER(P64), C.POINTER(P64),
+         C.POINTER(IterationStats), C.c_uint32, C.POINTER(Result), C.POINTER(C.c_int32)],
+    ),
     "vba_profile_enable": (None, [C.c_void_p, C.c_int32]),
     "vba_profile_reset": (None, [C.c_void_p]),
     "vba_profile_query": (C.c_int32, [C.c_void_p, C.c_char_p, PU64, P64, P64]),

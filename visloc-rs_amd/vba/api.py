@@ -285,6 +285,40 @@ class Device:
             out.message = self.last_error()
         return out
 
+    def optimize_batch(self, problems: List[BundleProblem], options: abi.Options,
+                       cap: int = 64) -> "BatchOutput":
+        """vba_optimize_batch: independent windows solved together with the
+        dense reduced solve, LM control per window on the device."""
+        n = len(problems)
+        views = (abi.Problem * max(n, 1))()
+        for i, p in enumerate(problems):
+            views[i] = p.view()
+        poses = [np.empty_like(p.poses) for p in problems]
+        lms = [np.empty_like(p.landmarks) for p in problems]
+        pose_ptrs = (abi.P64 * max(n, 1))(*[a.ctypes.data_as(abi.P64) for a in poses])
+        lm_ptrs = (abi.P64 * max(n, 1))(*[a.ctypes.data_as(abi.P64) for a in lms])
+        stats = (abi.IterationStats * max(n * cap, 1))()
+        results = (abi.Result * max(n, 1))()
+        statuses = (C.c_int32 * max(n, 1))()
+        st = self.lib.vba_optimize_batch(self.ctx, n, views, C.byref(options), pose_ptrs, lm_ptrs, stats, cap,
+                                         results, statuses)
+        outs = []
+        for i in range(n):
+            ok = st == abi.OK and statuses[i] == abi.OK
+            k = min(results[i].num_iterations, cap)
+            its = [{f[0].rstrip("_"): getattr(stats[i * cap + j], f[0]) for f in abi.IterationStats._fields_}
+                   for j in range(k)] if ok else []
+            outs.append(SolveOutput(statuses[i] if st == abi.OK else st, abi.Result.from_buffer_copy(results[i]),
+                                    its, poses[i] if ok else None, lms[i] if ok else None))
+        return BatchOutput(st, outs, self.last_error() if st != abi.OK else "")
+
+
+@dataclass
+class BatchOutput:
+    status: int
+    windows: List[SolveOutput]
+    message: str = ""
+
 
 class PreparedProblem:
     """A problem laid out and resident on the device (vba_prepare). solve()
