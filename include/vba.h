@@ -43,6 +43,7 @@ extern "C" {
 enum vba_status {
     VBA_OK = 0,
     VBA_DECLINED = 1,
+    VBA_PENDING = 2,                       /* vba_server_wait: the request is still running */
     VBA_ERR_NO_POSES = -1,                 /* BaError::NoPoses */
     VBA_ERR_NO_LANDMARKS = -2,             /* BaError::NoLandmarks */
     VBA_ERR_NO_OBSERVATIONS = -3,          /* BaError::NoObservations */
@@ -298,9 +299,62 @@ int32_t vba_optimize_batch(vba_context* ctx, uint32_t count, const vba_problem* 
                            double* const* landmarks_out, vba_iteration_stats* stats,
                            uint32_t stats_capacity, vba_result* results, int32_t* statuses);
 
+/* The batch kept on the device, as vba_prepare does for one problem:
+ * vba_optimize_batch is vba_prepare_batch + vba_solve_prepared_batch +
+ * vba_prepared_batch_state per window + vba_prepared_batch_free. Every
+ * solve starts from the prepared initial state. results may be NULL in
+ * vba_prepare_batch (it then only reports statuses). */
+typedef struct vba_prepared_batch vba_prepared_batch;
+int32_t vba_prepare_batch(vba_context* ctx, uint32_t count, const vba_problem* problems,
+                          const vba_options* options, vba_prepared_batch** out,
+                          vba_result* results, int32_t* statuses);
+int32_t vba_solve_prepared_batch(vba_context* ctx, vba_prepared_batch* batch,
+                                 const vba_options* options, vba_iteration_stats* stats,
+                                 uint32_t stats_capacity, vba_result* results, int32_t* statuses);
+/* Final state of window `window` (the caller's index) after the last solve. */
+int32_t vba_prepared_batch_state(vba_context* ctx, const vba_prepared_batch* batch, uint32_t window,
+                                 double* poses_out, double* landmarks_out);
+void vba_prepared_batch_free(vba_context* ctx, vba_prepared_batch* batch);
+
+/* ---- serving: continuous batching of independent solves -----------------
+ * No counterpart in the reference. A server owns one device and a request
+ * loop: vba_server_submit copies a problem and returns at once; lane threads
+ * admit queued requests as streams and device memory free up and solve them
+ * while others run. Dense requests with at most 256 variable poses
+ * (pose-only refinements, local windows) are gathered, per equal options,
+ * into device batches of up to max_window_batch windows; other requests
+ * (matrix-free, large dense) are solved one per submap stream. Admission
+ * keeps the estimated device footprint of running work within
+ * hbm_budget_bytes. A result equals the one vba_optimize gives the request
+ * alone. */
+typedef struct vba_server_config {
+    int32_t device;
+    uint32_t max_window_batch;    /* windows per device batch */
+    uint32_t window_streams;      /* window batches in flight at once */
+    uint32_t submap_streams;      /* single-problem solves in flight at once */
+    uint64_t hbm_budget_bytes;    /* admission limit on resident work */
+    double batch_wait_seconds;    /* how long a window batch waits to fill */
+} vba_server_config;
+typedef struct vba_server vba_server;
+void vba_server_default_config(vba_server_config* config);
+/* NULL on failure (vba_last_error(NULL)). */
+vba_server* vba_server_create(const vba_server_config* config);
+/* Queue a solve (the problem is copied). Returns a ticket > 0, or 0 on
+ * failure (vba_last_error(NULL)). */
+uint64_t vba_server_submit(vba_server* server, const vba_problem* problem, const vba_options* options);
+/* Wait up to timeout_seconds for a ticket. VBA_PENDING while it runs;
+ * otherwise the solve's status (the ticket is then retired), with outputs as
+ * vba_optimize writes them and the latency from submission to completion. */
+int32_t vba_server_wait(vba_server* server, uint64_t ticket, double timeout_seconds, double* poses_out,
+                        double* landmarks_out, vba_result* result, double* latency_seconds);
+void vba_server_counters(vba_server* server, uint64_t* completed, uint64_t* batches,
+                         uint64_t* batched_windows, uint64_t* queued);
+/* Finishes the queued requests, then stops. */
+void vba_server_destroy(vba_server* server);
+
 /* ---- per-kernel device time ----------------------------------------------
  * While enabled, HIP events are recorded around the launches of the kernel
- * families "linearize", "lm_pass", "cam_pass", "pcg_update" and "cost" on
+ * families "linearize", "schur_apply", "pcg_pose", "pcg_step" and "cost" on
  * the context's stream. vba_profile_query returns the launches that did
  * work, their summed device seconds and the algorithmic bytes they moved
  * (the minimum HBM traffic of the data layout, summed over launches). */

@@ -93,3 +93,21 @@ def test_batch_repeatable(device):
     for x, y in zip(a.windows, reversed(b.windows)):
         assert x.result.final_cost == y.result.final_cost
         assert np.array_equal(x.poses, y.poses)
+
+
+def test_prepared_batch_resolves_from_initial_state(device):
+    wins = make_windows(3, seed0=500)
+    o = window_options()
+    one_shot = device.optimize_batch([w.problem for w in wins], o)
+    pb = device.prepare_batch([w.problem for w in wins], o)
+    try:
+        for _ in range(2):
+            out = pb.solve(o)
+            assert out.status == abi.OK, out.message
+            for i, (a, b) in enumerate(zip(out.windows, one_shot.windows)):
+                assert a.result.final_cost == b.result.final_cost
+                assert a.accepted == b.accepted
+                poses, lms = pb.state(i)
+                assert np.array_equal(poses, b.poses) and np.array_equal(lms, b.landmarks)
+    finally:
+        pb.free()

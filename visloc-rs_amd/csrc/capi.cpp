@@ -10,10 +10,10 @@
 #include "../../include/vba.h"
 #include "host_common.hpp"
 #include "problem.hpp"
+#include "capi_types.hpp"
 #include "solver.hpp"
 
-struct vba_context : vba::Context {};
-struct vba_prepared : vba::Prepared {};
+
 
 namespace {
 
@@ -236,18 +236,14 @@ void vba_prepared_free(vba_context* ctx, vba_prepared* prepared) {
     delete prepared;
 }
 
-int32_t vba_optimize_batch(vba_context* ctx, uint32_t count, const vba_problem* problems, const vba_options* options,
-                           double* const* poses_out, double* const* landmarks_out, vba_iteration_stats* stats,
-                           uint32_t stats_capacity, vba_result* results, int32_t* statuses) {
-    if (!ctx || !options || !results || !statuses || (count && !problems)) {
-        vba::set_thread_error("vba_optimize_batch: null argument");
-        if (ctx) ctx->last_error = "vba_optimize_batch: null argument";
-        return VBA_ERR_INVALID_ARGUMENT;
-    }
-    std::lock_guard<std::mutex> lock(ctx->mu);
-    ctx->last_error.clear();
-    if (options->linear_solver != VBA_SOLVER_DENSE) {
-        ctx->last_error = "vba_optimize_batch: windows use the dense reduced solve (linear_solver = VBA_SOLVER_DENSE)";
+namespace {
+
+// Validate every window (vba_check, then the model build) and lay out the
+// valid ones as one batch.
+int32_t build_batch(vba_context* ctx, uint32_t count, const vba_problem* problems, const vba_options& options,
+                    vba_prepared_batch* pb, vba_result* results, int32_t* statuses) {
+    if (options.linear_solver != VBA_SOLVER_DENSE) {
+        ctx->last_error = "batched windows use the dense reduced solve (linear_solver = VBA_SOLVER_DENSE)";
         return VBA_ERR_INVALID_CONFIGURATION;
     }
     if (hipSetDevice(ctx->device) != hipSuccess) {
@@ -259,28 +255,117 @@ int32_t vba_optimize_batch(vba_context* ctx, uint32_t count, const vba_problem* 
     try {
         models.resize(count);
         for (uint32_t i = 0; i < count; ++i) {
-            std::memset(&results[i], 0, sizeof(vba_result));
+            if (results) std::memset(&results[i], 0, sizeof(vba_result));
             std::string reason;
-            int32_t st = vba::check_support(problems[i], *options, &reason);
+            int32_t st = vba::check_support(problems[i], options, &reason);
             if (st == VBA_OK) {
                 uint64_t error_id = 0;
                 st = vba::build_model(problems[i], &models[i], &error_id, &reason);
-                results[i].error_id = error_id;
+                if (results) results[i].error_id = error_id;
             }
             statuses[i] = st;
             valid[i] = st == VBA_OK;
         }
+        std::string msg;
+        int32_t st = vba::prepare_batch(*ctx, models, valid, pb, statuses, &msg);
+        if (st != VBA_OK) ctx->last_error = msg;
+        return st;
     } catch (const std::bad_alloc&) {
         ctx->last_error = "out of host memory";
         return VBA_ERR_OUT_OF_MEMORY;
+    }
+}
+
+}  // namespace
+
+int32_t vba_optimize_batch(vba_context* ctx, uint32_t count, const vba_problem* problems, const vba_options* options,
+                           double* const* poses_out, double* const* landmarks_out, vba_iteration_stats* stats,
+                           uint32_t stats_capacity, vba_result* results, int32_t* statuses) {
+    if (!ctx || !options || !results || !statuses || (count && !problems)) {
+        vba::set_thread_error("vba_optimize_batch: null argument");
+        if (ctx) ctx->last_error = "vba_optimize_batch: null argument";
+        return VBA_ERR_INVALID_ARGUMENT;
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->last_error.clear();
+    vba_prepared_batch pb;
+    int32_t st = build_batch(ctx, count, problems, *options, &pb, results, statuses);
+    if (st == VBA_OK) {
+        try {
+            st = vba::run_batch(*ctx, pb, *options, stats, stats_capacity, results, statuses);
+        } catch (const std::bad_alloc&) {
+            ctx->last_error = "out of host memory";
+            st = VBA_ERR_OUT_OF_MEMORY;
+        }
+    }
+    if (st == VBA_OK)
+        for (uint32_t i = 0; i < count && st == VBA_OK; ++i)
+            if (pb.window_of[i] >= 0)
+                st = vba::download_batch_window(*ctx, pb, i, poses_out ? poses_out[i] : nullptr,
+                                                landmarks_out ? landmarks_out[i] : nullptr);
+    vba::release(pb);
+    return st;
+}
+
+int32_t vba_prepare_batch(vba_context* ctx, uint32_t count, const vba_problem* problems, const vba_options* options,
+                          vba_prepared_batch** out, vba_result* results, int32_t* statuses) {
+    if (!ctx || !options || !out || !statuses || (count && !problems)) {
+        vba::set_thread_error("vba_prepare_batch: null argument");
+        return VBA_ERR_INVALID_ARGUMENT;
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    *out = nullptr;
+    ctx->last_error.clear();
+    std::unique_ptr<vba_prepared_batch> pb(new (std::nothrow) vba_prepared_batch());
+    if (!pb) {
+        ctx->last_error = "out of host memory";
+        return VBA_ERR_OUT_OF_MEMORY;
+    }
+    int32_t st = build_batch(ctx, count, problems, *options, pb.get(), results, statuses);
+    if (st != VBA_OK) {
+        vba::release(*pb);
+        return st;
+    }
+    *out = pb.release();
+    return VBA_OK;
+}
+
+int32_t vba_solve_prepared_batch(vba_context* ctx, vba_prepared_batch* batch, const vba_options* options,
+                                 vba_iteration_stats* stats, uint32_t stats_capacity, vba_result* results,
+                                 int32_t* statuses) {
+    if (!ctx || !batch || !options || !results || !statuses) {
+        vba::set_thread_error("vba_solve_prepared_batch: null argument");
+        return VBA_ERR_INVALID_ARGUMENT;
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->last_error.clear();
+    if (options->linear_solver != VBA_SOLVER_DENSE) {
+        ctx->last_error = "batched windows use the dense reduced solve (linear_solver = VBA_SOLVER_DENSE)";
+        return VBA_ERR_INVALID_CONFIGURATION;
     }
     try {
-        return vba::run_batch(*ctx, models, valid, *options, poses_out, landmarks_out, stats, stats_capacity, results,
-                              statuses);
+        return vba::run_batch(*ctx, *batch, *options, stats, stats_capacity, results, statuses);
     } catch (const std::bad_alloc&) {
         ctx->last_error = "out of host memory";
         return VBA_ERR_OUT_OF_MEMORY;
     }
+}
+
+int32_t vba_prepared_batch_state(vba_context* ctx, const vba_prepared_batch* batch, uint32_t window,
+                                 double* poses_out, double* landmarks_out) {
+    if (!ctx || !batch) return VBA_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    return vba::download_batch_window(*ctx, *batch, window, poses_out, landmarks_out);
+}
+
+void vba_prepared_batch_free(vba_context* ctx, vba_prepared_batch* batch) {
+    if (!batch) return;
+    if (ctx) {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        (void)hipStreamSynchronize(ctx->stream);
+    }
+    vba::release(*batch);
+    delete batch;
 }
 
 void vba_profile_enable(vba_context* ctx, int32_t enable) {

@@ -13,153 +13,11 @@
 
 #include "kernels.hpp"
 
+#include "device_common.hpp"
+
 namespace vba {
 
 namespace {
-
-// Work-groups are dispatched round-robin over the 8 XCDs. Give XCD k a
-// contiguous range of tiles so that neighbouring cameras, which share
-// landmarks, share that XCD's L2.
-__device__ __forceinline__ int xcd_tile(int bid, int n) {
-    const int per = (n + 7) >> 3;
-    return (bid & 7) * per + (bid >> 3);
-}
-inline int xcd_grid(int n) { return ((n + 7) >> 3) << 3; }
-
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    return v;
-}
-template <typename T>
-__device__ __forceinline__ T wave_max(T v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
-    return v;
-}
-
-// Sum N values over an NT-thread work-group; the result is valid in thread 0
-// and the order of additions is fixed. lds must hold (NT/64)*N values.
-template <int NT, typename T, int N>
-__device__ __forceinline__ void block_sum(T (&v)[N], T* lds) {
-    constexpr int NW = NT / 64;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int i = 0; i < N; ++i) v[i] = wave_sum(v[i]);
-    if (lane == 0) {
-#pragma unroll
-        for (int i = 0; i < N; ++i) lds[wave * N + i] = v[i];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-            T s = lds[i];
-            for (int w = 1; w < NW; ++w) s += lds[w * N + i];
-            v[i] = s;
-        }
-    }
-    __syncthreads();
-}
-// Sum broadcast to every thread.
-template <int NT, typename T, int N>
-__device__ __forceinline__ void block_sum_all(T (&v)[N], T* lds) {
-    block_sum<NT, T, N>(v, lds);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int i = 0; i < N; ++i) lds[i] = v[i];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < N; ++i) v[i] = lds[i];
-    __syncthreads();
-}
-template <int NT>
-__device__ __forceinline__ double block_max(double v, double* lds) {
-    constexpr int NW = NT / 64;
-    v = wave_max(v);
-    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double m = lds[0];
-        for (int w = 1; w < NW; ++w) m = fmax(m, lds[w]);
-        v = m;
-    }
-    __syncthreads();
-    return v;
-}
-
-// Neumaier compensated pair (sum, compensation) and its exact-order merge.
-struct CSum {
-    double s, c;
-};
-__device__ __forceinline__ void csum_add(CSum& a, double x) {
-    double t = a.s + x;
-    if (fabs(a.s) >= fabs(x)) a.c += (a.s - t) + x;
-    else a.c += (x - t) + a.s;
-    a.s = t;
-}
-__device__ __forceinline__ CSum csum_merge(CSum a, CSum b) {
-    CSum r;
-    r.s = a.s + b.s;
-    double bp = r.s - a.s;
-    double e = (a.s - (r.s - bp)) + (b.s - bp);
-    r.c = a.c + b.c + e;
-    return r;
-}
-
-// Packed upper-triangle index of a symmetric 6x6.
-__host__ __device__ constexpr int up6(int i, int j) { return i * 6 - i * (i - 1) / 2 + (j - i); }
-
-// Accumulate s * [[A, -A K], [K A, -K A K]] (upper triangle) with K = [X]x,
-// A symmetric (a00 a01 a02 a11 a12 a22).
-template <typename T>
-__device__ __forceinline__ void add_schur_block(const T A[6], const T X[3], T s, T out[21]) {
-    const T a[9] = {A[0], A[1], A[2], A[1], A[3], A[4], A[2], A[4], A[5]};
-    const T K[9] = {T(0), -X[2], X[1], X[2], T(0), -X[0], -X[1], X[0], T(0)};
-    T AK[9], KAK[9];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) AK[3 * i + j] = a[3 * i] * K[j] + a[3 * i + 1] * K[3 + j] + a[3 * i + 2] * K[6 + j];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) KAK[3 * i + j] = K[3 * i] * AK[j] + K[3 * i + 1] * AK[3 + j] + K[3 * i + 2] * AK[6 + j];
-    out[up6(0, 0)] += s * a[0];
-    out[up6(0, 1)] += s * a[1];
-    out[up6(0, 2)] += s * a[2];
-    out[up6(1, 1)] += s * a[4];
-    out[up6(1, 2)] += s * a[5];
-    out[up6(2, 2)] += s * a[8];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) out[up6(i, 3 + j)] -= s * AK[3 * i + j];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = i; j < 3; ++j) out[up6(3 + i, 3 + j)] -= s * KAK[3 * i + j];
-}
-
-// y = sym(A) x for packed symmetric 3x3.
-template <typename T>
-__device__ __forceinline__ void sym3_mul(const T A[6], const T x[3], T y[3]) {
-    y[0] = A[0] * x[0] + A[1] * x[1] + A[2] * x[2];
-    y[1] = A[1] * x[0] + A[3] * x[1] + A[4] * x[2];
-    y[2] = A[2] * x[0] + A[4] * x[1] + A[5] * x[2];
-}
-// y = sym(U) x for packed symmetric 6x6.
-__device__ __forceinline__ void sym6_mul(const double* U, const double* x, double* y) {
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-        double s = 0.0;
-#pragma unroll
-        for (int j = 0; j < 6; ++j) s += U[i <= j ? up6(i, j) : up6(j, i)] * x[j];
-        y[i] = s;
-    }
-}
 
 // ---------------------------------------------------------------------------
 // Linearization: residuals, Jacobians and IRLS weights in FP64 at the current
@@ -224,29 +82,20 @@ __global__ __launch_bounds__(kBlock) void k_linearize(DevProblem P, int store32)
             h[1] += w * (J[1] * r0 + J[4] * r1);
             h[2] += w * (J[2] * r0 + J[5] * r1);
         }
-        const int pl = P.perm_cl[e];
-        EdgeRec64 rc, rl;
-#pragma unroll
-        for (int i = 0; i < 6; ++i) rc.G[i] = rl.G[i] = G[i];
-        rc.idx = l;
-        rl.idx = ct.pose;
-        rc.pad[0] = rc.pad[1] = rc.pad[2] = rl.pad[1] = rl.pad[2] = 0;
-        rl.pad[0] = l;  // l-order records carry their landmark for the staged passes
-        P.ec64[e] = rc;
-        P.el64[pl] = rl;
-        P.hl64[3 * (size_t)pl] = h[0];
-        P.hl64[3 * (size_t)pl + 1] = h[1];
-        P.hl64[3 * (size_t)pl + 2] = h[2];
+        // l-order G (FP64, and FP32 for the mixed-precision operator) and h
+        const size_t pl = (size_t)P.perm_cl[e];
+        double2* g64 = (double2*)(P.Gl64 + 6 * pl);
+        g64[0] = make_double2(G[0], G[1]);
+        g64[1] = make_double2(G[2], G[3]);
+        g64[2] = make_double2(G[4], G[5]);
+        P.hl64[3 * pl] = h[0];
+        P.hl64[3 * pl + 1] = h[1];
+        P.hl64[3 * pl + 2] = h[2];
         if (store32) {
-            EdgeRec32 fc, fl;
-#pragma unroll
-            for (int i = 0; i < 6; ++i) fc.G[i] = fl.G[i] = (float)G[i];
-            fc.idx = l;
-            fl.idx = ct.pose;
-            fc.pad = 0;
-            fl.pad = l;
-            P.ec32[e] = fc;
-            P.el32[pl] = fl;
+            float2* g32 = (float2*)(P.Gl32 + 6 * pl);
+            g32[0] = make_float2((float)G[0], (float)G[1]);
+            g32[1] = make_float2((float)G[2], (float)G[3]);
+            g32[2] = make_float2((float)G[4], (float)G[5]);
         }
         add_schur_block<double>(G, X, 1.0, acc);
         acc[21] += h[0];
@@ -287,9 +136,9 @@ __global__ void k_landmark_reduce(DevProblem P) {
     if (P.wstate && !P.wstate[P.lm_win[l]].relin) return;
     double V[6] = {0, 0, 0, 0, 0, 0}, g[3] = {0, 0, 0};
     for (int e = P.lm_edge_ptr[l]; e < P.lm_edge_ptr[l + 1]; ++e) {
-        const EdgeRec64& r = P.el64[e];
+        const double* G = P.Gl64 + 6 * (size_t)e;
 #pragma unroll
-        for (int i = 0; i < 6; ++i) V[i] += r.G[i];
+        for (int i = 0; i < 6; ++i) V[i] += G[i];
 #pragma unroll
         for (int i = 0; i < 3; ++i) g[i] += P.hl64[3 * (size_t)e + i];
     }
@@ -300,8 +149,10 @@ __global__ void k_landmark_reduce(DevProblem P) {
 }
 
 // Damped landmark inverse (V + lambda I)^-1 by cofactors, zero when the
-// determinant is exactly zero (the landmark then drops out, as try_inverse
-// returning None does in normal_equations.rs:1432-1445), and y = V^-1 g_l.
+// determinant is exactly zero or the landmark is fixed (the landmark then
+// drops out of the Schur complement and back-substitution, as try_inverse
+// returning None does in normal_equations.rs:1432-1445); FP32 copies of the
+// inverse and the position for the mixed-precision operator.
 __global__ void k_landmark_prepare(DevProblem P, double lambda) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= P.num_lm_slots) return;
@@ -310,8 +161,7 @@ __global__ void k_landmark_prepare(DevProblem P, double lambda) {
         if (!w.active) return;
         lambda = w.lambda;
     }
-    double X[3] = {P.lms[3 * (size_t)l], P.lms[3 * (size_t)l + 1], P.lms[3 * (size_t)l + 2]};
-    double inv[6] = {0, 0, 0, 0, 0, 0}, y[3] = {0, 0, 0};
+    double inv[6] = {0, 0, 0, 0, 0, 0};
     if (P.lm_var[l] >= 0) {
         const double* V = P.V + 6 * (size_t)l;
         double a = V[0], b = V[1], c = V[2], d = V[3], e = V[4], f = V[5];
@@ -330,497 +180,18 @@ __global__ void k_landmark_prepare(DevProblem P, double lambda) {
             inv[3] = (a * f - c * c) / det;
             inv[4] = (b * c - a * e) / det;
             inv[5] = (a * d - b * b) / det;
-            sym3_mul(inv, P.gl + 3 * (size_t)l, y);
         }
     }
 #pragma unroll
     for (int i = 0; i < 6; ++i) P.Vinv64[6 * (size_t)l + i] = inv[i];
-    LmPack<double> p64;
-    p64.X[0] = X[0]; p64.X[1] = X[1]; p64.X[2] = X[2]; p64.pad0 = 0.0;
-    p64.u[0] = y[0]; p64.u[1] = y[1]; p64.u[2] = y[2]; p64.pad1 = 0.0;
-    P.pack64[l] = p64;
-    if (P.pack32) {
+    if (P.Vinv32) {
 #pragma unroll
         for (int i = 0; i < 6; ++i) P.Vinv32[6 * (size_t)l + i] = (float)inv[i];
-        LmPack<float> p32;
-        p32.X[0] = (float)X[0]; p32.X[1] = (float)X[1]; p32.X[2] = (float)X[2]; p32.pad0 = 0.f;
-        p32.u[0] = (float)y[0]; p32.u[1] = (float)y[1]; p32.u[2] = (float)y[2]; p32.pad1 = 0.f;
-        P.pack32[l] = p32;
+        P.X32[l] = make_float4((float)P.lms[3 * (size_t)l], (float)P.lms[3 * (size_t)l + 1],
+                               (float)P.lms[3 * (size_t)l + 2], 0.f);
     }
 }
 
-// Block-Jacobi blocks: per camera tile, the sum of W V^-1 W^T over its edges
-// (implicit_schur.rs:241-287); W V^-1 W^T = [[C, -C K], [K C, -K C K]] with
-// C = G V^-1 G.
-__global__ __launch_bounds__(kBlock) void k_precondition_tiles(DevProblem P) {
-    __shared__ double lds[(kBlock / 64) * 21];
-    const int tile = xcd_tile(blockIdx.x, P.num_cam_tiles);
-    if (tile >= P.num_cam_tiles) return;
-    const CamTile ct = P.cam_tiles[tile];
-    if (P.pose_var[ct.pose] < 0) return;
-    double acc[21];
-#pragma unroll
-    for (int i = 0; i < 21; ++i) acc[i] = 0.0;
-    for (int e = ct.begin + (int)threadIdx.x; e < ct.end; e += kBlock) {
-        const EdgeRec64 rc = P.ec64[e];
-        const int l = rc.idx;
-        const double* Vi = P.Vinv64 + 6 * (size_t)l;
-        const double vi[9] = {Vi[0], Vi[1], Vi[2], Vi[1], Vi[3], Vi[4], Vi[2], Vi[4], Vi[5]};
-        const double g[9] = {rc.G[0], rc.G[1], rc.G[2], rc.G[1], rc.G[3], rc.G[4], rc.G[2], rc.G[4], rc.G[5]};
-        double T[9];
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) T[3 * i + j] = g[3 * i] * vi[j] + g[3 * i + 1] * vi[3 + j] + g[3 * i + 2] * vi[6 + j];
-        double C[6];
-        C[0] = T[0] * g[0] + T[1] * g[3] + T[2] * g[6];
-        C[1] = T[0] * g[1] + T[1] * g[4] + T[2] * g[7];
-        C[2] = T[0] * g[2] + T[1] * g[5] + T[2] * g[8];
-        C[3] = T[3] * g[1] + T[4] * g[4] + T[5] * g[7];
-        C[4] = T[3] * g[2] + T[4] * g[5] + T[5] * g[8];
-        C[5] = T[6] * g[2] + T[7] * g[5] + T[8] * g[8];
-        const LmPack<double> pk = P.pack64[l];
-        add_schur_block<double>(C, pk.X, 1.0, acc);
-    }
-    block_sum<kBlock, double, 21>(acc, lds);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int i = 0; i < 21; ++i) P.tile27[27 * (size_t)tile + i] = acc[i];
-    }
-}
-
-// 6x6 Cholesky inverse; false for a non-positive or non-finite pivot.
-__device__ bool cholesky_inverse6(double A[36], double out[36]) {
-    double L[36];
-#pragma unroll
-    for (int i = 0; i < 36; ++i) L[i] = 0.0;
-    for (int j = 0; j < 6; ++j) {
-        double d = A[6 * j + j];
-        for (int k = 0; k < j; ++k) d -= L[6 * j + k] * L[6 * j + k];
-        if (!(d > 0.0) || !isfinite(d)) return false;
-        const double ljj = sqrt(d);
-        L[6 * j + j] = ljj;
-        for (int i = j + 1; i < 6; ++i) {
-            double s = A[6 * i + j];
-            for (int k = 0; k < j; ++k) s -= L[6 * i + k] * L[6 * j + k];
-            L[6 * i + j] = s / ljj;
-        }
-    }
-    for (int c = 0; c < 6; ++c) {
-        double y[6];
-        for (int i = 0; i < 6; ++i) {
-            double s = (i == c) ? 1.0 : 0.0;
-            for (int k = 0; k < i; ++k) s -= L[6 * i + k] * y[k];
-            y[i] = s / L[6 * i + i];
-        }
-        for (int i = 5; i >= 0; --i) {
-            double s = y[i];
-            for (int k = i + 1; k < 6; ++k) s -= L[6 * k + i] * y[k];
-            y[i] = s / L[6 * i + i];
-        }
-        for (int i = 0; i < 6; ++i) out[6 * i + c] = y[i];
-    }
-    for (int i = 0; i < 36; ++i)
-        if (!isfinite(out[i])) return false;
-    return true;
-}
-
-__global__ void k_precondition_finalize(DevProblem P, double lambda) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P.num_pose_slots) return;
-    double* out = P.Minv + 36 * (size_t)p;
-    if (P.pose_var[p] < 0) {
-        for (int i = 0; i < 36; ++i) out[i] = 0.0;
-        return;
-    }
-    double acc[21];
-    for (int i = 0; i < 21; ++i) acc[i] = P.U[21 * (size_t)p + i];
-    for (int t = P.cam_tile_ptr[p]; t < P.cam_tile_ptr[p + 1]; ++t)
-        for (int i = 0; i < 21; ++i) acc[i] -= P.tile27[27 * (size_t)t + i];
-    double M[36];
-    for (int i = 0; i < 6; ++i)
-        for (int j = 0; j < 6; ++j) M[6 * i + j] = acc[i <= j ? up6(i, j) : up6(j, i)];
-    if (lambda > 0.0)
-        for (int k = 0; k < 6; ++k) M[7 * k] += lambda;
-    double inv[36];
-    if (!cholesky_inverse6(M, inv)) {
-        P.scalars->precond_failed = 1;
-        for (int i = 0; i < 36; ++i) out[i] = 0.0;
-        return;
-    }
-    for (int i = 0; i < 36; ++i) out[i] = inv[i];
-}
-
-// ---------------------------------------------------------------------------
-// Matrix-free Schur operator, S x = (U + lambda I) x - W V^-1 W^T x
-// (implicit_schur.rs:390-431), as a landmark pass and a camera pass.
-// ---------------------------------------------------------------------------
-// Landmark slot of an l-order record (stored in its padding).
-__device__ __forceinline__ int rec_lm(const EdgeRec32& r) { return r.pad; }
-__device__ __forceinline__ int rec_lm(const EdgeRec64& r) { return r.pad[0]; }
-
-// Landmark pass, LDS-staged: each lane takes one l-order edge of the tile
-// (coalesced record loads), computes W_e^T x_p = G (rho + omega x X) into
-// LDS, then the thread owning each landmark sums its contiguous segment and
-// applies V^-1. A tile holding a single landmark with more than kTileEdges
-// edges is summed straight from global memory by its owner.
-template <typename T, typename Rec>
-__global__ __launch_bounds__(kBlock) void k_lm_pass(DevProblem P, const T* __restrict__ x,
-                                                    const Rec* __restrict__ el, const T* __restrict__ vinv,
-                                                    LmPack<T>* __restrict__ pack, const int32_t* run_flag) {
-    __shared__ T cbuf[kTileEdges * 3];
-    __shared__ T xs[kBlock * 3];
-    if (run_flag && *run_flag != 0) return;
-    const LmTile lt = P.lm_tiles[blockIdx.x];
-    const int e0 = P.lm_edge_ptr[lt.lm_begin], e1 = P.lm_edge_ptr[lt.lm_end];
-    const int l = lt.lm_begin + (int)threadIdx.x;
-    const bool owner = l < lt.lm_end;
-    if (owner) {
-        const LmPack<T> pk = pack[l];
-        xs[3 * threadIdx.x] = pk.X[0];
-        xs[3 * threadIdx.x + 1] = pk.X[1];
-        xs[3 * threadIdx.x + 2] = pk.X[2];
-    }
-    const bool staged = e1 - e0 <= kTileEdges;
-    __syncthreads();
-    if (staged) {
-        for (int e = e0 + (int)threadIdx.x; e < e1; e += kBlock) {
-            const Rec r = el[e];
-            const int j = rec_lm(r) - lt.lm_begin;
-            const T X0 = xs[3 * j], X1 = xs[3 * j + 1], X2 = xs[3 * j + 2];
-            const T* xp = x + 6 * (size_t)r.idx;
-            const T v0 = xp[0] + xp[4] * X2 - xp[5] * X1;
-            const T v1 = xp[1] + xp[5] * X0 - xp[3] * X2;
-            const T v2 = xp[2] + xp[3] * X1 - xp[4] * X0;
-            T* c = cbuf + 3 * (e - e0);
-            c[0] = (T)r.G[0] * v0 + (T)r.G[1] * v1 + (T)r.G[2] * v2;
-            c[1] = (T)r.G[1] * v0 + (T)r.G[3] * v1 + (T)r.G[4] * v2;
-            c[2] = (T)r.G[2] * v0 + (T)r.G[4] * v1 + (T)r.G[5] * v2;
-        }
-        __syncthreads();
-    }
-    if (!owner || P.lm_var[l] < 0) return;
-    const T X0 = xs[3 * threadIdx.x], X1 = xs[3 * threadIdx.x + 1], X2 = xs[3 * threadIdx.x + 2];
-    T t0 = T(0), t1 = T(0), t2 = T(0);
-    const int b = P.lm_edge_ptr[l], end = P.lm_edge_ptr[l + 1];
-    if (staged) {
-        for (int e = b; e < end; ++e) {
-            const T* c = cbuf + 3 * (e - e0);
-            t0 += c[0];
-            t1 += c[1];
-            t2 += c[2];
-        }
-    } else {
-        for (int e = b; e < end; ++e) {
-            const Rec r = el[e];
-            const T* xp = x + 6 * (size_t)r.idx;
-            const T v0 = xp[0] + xp[4] * X2 - xp[5] * X1;
-            const T v1 = xp[1] + xp[5] * X0 - xp[3] * X2;
-            const T v2 = xp[2] + xp[3] * X1 - xp[4] * X0;
-            t0 += (T)r.G[0] * v0 + (T)r.G[1] * v1 + (T)r.G[2] * v2;
-            t1 += (T)r.G[1] * v0 + (T)r.G[3] * v1 + (T)r.G[4] * v2;
-            t2 += (T)r.G[2] * v0 + (T)r.G[4] * v1 + (T)r.G[5] * v2;
-        }
-    }
-    const T* Vi = vinv + 6 * (size_t)l;
-    pack[l].u[0] = Vi[0] * t0 + Vi[1] * t1 + Vi[2] * t2;
-    pack[l].u[1] = Vi[1] * t0 + Vi[3] * t1 + Vi[4] * t2;
-    pack[l].u[2] = Vi[2] * t0 + Vi[4] * t1 + Vi[5] * t2;
-}
-
-template <typename T, typename Rec>
-__global__ __launch_bounds__(kBlock) void k_cam_pass(DevProblem P, const Rec* __restrict__ ec,
-                                                     const LmPack<T>* __restrict__ pack, T* __restrict__ part,
-                                                     const int32_t* run_flag) {
-    __shared__ T lds[(kBlock / 64) * 6];
-    if (run_flag && *run_flag != 0) return;
-    const int tile = xcd_tile(blockIdx.x, P.num_cam_tiles);
-    if (tile >= P.num_cam_tiles) return;
-    if (P.wstate && !P.wstate[P.cam_tile_win[tile]].active) return;
-    const CamTile ct = P.cam_tiles[tile];
-    if (P.pose_var[ct.pose] < 0) return;
-    T acc[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
-    for (int e = ct.begin + (int)threadIdx.x; e < ct.end; e += kBlock) {
-        const Rec r = ec[e];
-        const LmPack<T> pk = pack[r.idx];
-        const T a0 = (T)r.G[0] * pk.u[0] + (T)r.G[1] * pk.u[1] + (T)r.G[2] * pk.u[2];
-        const T a1 = (T)r.G[1] * pk.u[0] + (T)r.G[3] * pk.u[1] + (T)r.G[4] * pk.u[2];
-        const T a2 = (T)r.G[2] * pk.u[0] + (T)r.G[4] * pk.u[1] + (T)r.G[5] * pk.u[2];
-        acc[0] += a0;
-        acc[1] += a1;
-        acc[2] += a2;
-        acc[3] += pk.X[1] * a2 - pk.X[2] * a1;
-        acc[4] += pk.X[2] * a0 - pk.X[0] * a2;
-        acc[5] += pk.X[0] * a1 - pk.X[1] * a0;
-    }
-    block_sum<kBlock, T, 6>(acc, lds);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int i = 0; i < 6; ++i) part[6 * (size_t)tile + i] = acc[i];
-    }
-}
-
-// b = -g_p + sum_e W_e V^-1 g_l (implicit_schur.rs:181-231): the camera pass
-// was run with u = V^-1 g_l.
-__global__ void k_rhs_finalize(DevProblem P) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P.num_pose_slots) return;
-    double acc[6] = {0, 0, 0, 0, 0, 0};
-    if (P.pose_var[p] >= 0) {
-        for (int t = P.cam_tile_ptr[p]; t < P.cam_tile_ptr[p + 1]; ++t)
-            for (int i = 0; i < 6; ++i) acc[i] += P.part64[6 * (size_t)t + i];
-        for (int i = 0; i < 6; ++i) acc[i] -= P.g[6 * (size_t)p + i];
-    }
-    for (int i = 0; i < 6; ++i) P.b[6 * (size_t)p + i] = acc[i];
-}
-
-// ---------------------------------------------------------------------------
-// PCG vector work on the reduced system (6 unknowns per pose), one
-// work-group. Control flow of solve_pcg_with_restart_hook
-// (implicit_schur.rs:522-736): the recursive residual decides convergence,
-// the host then checks the true residual.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void minv_apply(const DevProblem& P, int p, const double* r, double* z) {
-    const double* M = P.Minv + 36 * (size_t)p;
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-        double s = 0.0;
-#pragma unroll
-        for (int j = 0; j < 6; ++j) s += M[6 * i + j] * r[j];
-        z[i] = s;
-    }
-}
-
-__global__ __launch_bounds__(kSmallBlock) void k_pcg_init(DevProblem P, double rtol, double atol, int max_iter,
-                                                          double lambda) {
-    __shared__ double lds[(kSmallBlock / 64) * 2];
-    double acc[2] = {0.0, 0.0};  // r.z, b.b
-    for (int p = threadIdx.x; p < P.num_pose_slots; p += kSmallBlock) {
-        double r[6], z[6];
-        const bool var = P.pose_var[p] >= 0;
-        for (int i = 0; i < 6; ++i) r[i] = var ? P.b[6 * (size_t)p + i] : 0.0;
-        if (var) minv_apply(P, p, r, z);
-        else for (int i = 0; i < 6; ++i) z[i] = 0.0;
-        for (int i = 0; i < 6; ++i) {
-            const size_t k = 6 * (size_t)p + i;
-            P.x[k] = 0.0;
-            P.r[k] = r[i];
-            P.z[k] = z[i];
-            P.d[k] = z[i];
-            P.d32[k] = (float)z[i];
-            P.q[k] = 0.0;
-            acc[0] += r[i] * z[i];
-            acc[1] += r[i] * r[i];
-        }
-    }
-    block_sum<kSmallBlock, double, 2>(acc, lds);
-    if (threadIdx.x == 0) {
-        PcgState s;
-        s.rho = acc[0];
-        s.rhs_norm = sqrt(acc[1]);
-        s.target = fmax(atol, rtol * s.rhs_norm);
-        s.rnorm = s.rhs_norm;
-        s.true_norm = 0.0;
-        s.lambda = lambda;
-        s.iter = 0;
-        s.max_iter = max_iter;
-        s.flags = 0;
-        if (P.scalars->precond_failed) s.status = PCG_NONPOSITIVE;  // NonSpdPreconditioner
-        else if (!isfinite(s.rhs_norm) || !isfinite(s.target)) s.status = PCG_NONFINITE;
-        else if (s.rhs_norm <= s.target) s.status = PCG_CONVERGED;
-        else if (max_iter == 0) s.status = PCG_MAXIT;
-        else if (!isfinite(s.rho) || !(s.rho > 0.0)) s.status = PCG_NONPOSITIVE;
-        else s.status = PCG_RUNNING;
-        *P.pcg = s;
-    }
-}
-
-// One PCG step after the passes left sum_e W V^-1 W^T d in the tile partials.
-template <typename T>
-__global__ __launch_bounds__(kSmallBlock) void k_pcg_update(DevProblem P, const T* __restrict__ part) {
-    __shared__ double lds[(kSmallBlock / 64) * 2];
-    PcgState* st = P.pcg;
-    if (st->status != PCG_RUNNING) return;
-    const double lambda = st->lambda;
-    double acc1[1] = {0.0};
-    for (int p = threadIdx.x; p < P.num_pose_slots; p += kSmallBlock) {
-        if (P.pose_var[p] < 0) continue;
-        double dp[6], q[6], s[6] = {0, 0, 0, 0, 0, 0};
-        for (int i = 0; i < 6; ++i) dp[i] = P.d[6 * (size_t)p + i];
-        for (int t = P.cam_tile_ptr[p]; t < P.cam_tile_ptr[p + 1]; ++t)
-            for (int i = 0; i < 6; ++i) s[i] += (double)part[6 * (size_t)t + i];
-        sym6_mul(P.U + 21 * (size_t)p, dp, q);
-        for (int i = 0; i < 6; ++i) {
-            q[i] = q[i] + lambda * dp[i] - s[i];
-            P.q[6 * (size_t)p + i] = q[i];
-            acc1[0] += dp[i] * q[i];
-        }
-    }
-    block_sum_all<kSmallBlock, double, 1>(acc1, lds);
-    const double curvature = acc1[0];
-    if (!isfinite(curvature) || !(curvature > 0.0)) {
-        if (threadIdx.x == 0) st->status = PCG_NONPOSITIVE;
-        return;
-    }
-    const double alpha = st->rho / curvature;
-    if (!isfinite(alpha)) {
-        if (threadIdx.x == 0) st->status = PCG_NONFINITE;
-        return;
-    }
-    double acc2[1] = {0.0};
-    for (int p = threadIdx.x; p < P.num_pose_slots; p += kSmallBlock) {
-        if (P.pose_var[p] < 0) continue;
-        for (int i = 0; i < 6; ++i) {
-            const size_t k = 6 * (size_t)p + i;
-            P.x[k] = P.x[k] + alpha * P.d[k];
-            const double rk = P.r[k] - alpha * P.q[k];
-            P.r[k] = rk;
-            acc2[0] += rk * rk;
-        }
-    }
-    block_sum_all<kSmallBlock, double, 1>(acc2, lds);
-    const double rn = sqrt(acc2[0]);
-    const int iter = st->iter + 1;
-    int status = PCG_RUNNING;
-    if (!isfinite(rn)) status = PCG_NONFINITE;
-    else if (rn <= st->target) status = PCG_CONVERGED;
-    else if (iter >= st->max_iter) status = PCG_MAXIT;
-    if (status != PCG_RUNNING) {
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            st->iter = iter;
-            st->rnorm = rn;
-            st->status = status;
-        }
-        return;
-    }
-    double acc3[1] = {0.0};
-    for (int p = threadIdx.x; p < P.num_pose_slots; p += kSmallBlock) {
-        if (P.pose_var[p] < 0) continue;
-        double r[6], z[6];
-        for (int i = 0; i < 6; ++i) r[i] = P.r[6 * (size_t)p + i];
-        minv_apply(P, p, r, z);
-        for (int i = 0; i < 6; ++i) {
-            P.z[6 * (size_t)p + i] = z[i];
-            acc3[0] += r[i] * z[i];
-        }
-    }
-    block_sum_all<kSmallBlock, double, 1>(acc3, lds);
-    const double rz = acc3[0];
-    if (!isfinite(rz) || !(rz > 0.0)) {
-        if (threadIdx.x == 0) {
-            st->iter = iter;
-            st->rnorm = rn;
-            st->status = PCG_NONPOSITIVE;
-        }
-        return;
-    }
-    const double beta = rz / st->rho;
-    for (int p = threadIdx.x; p < P.num_pose_slots; p += kSmallBlock) {
-        if (P.pose_var[p] < 0) continue;
-        for (int i = 0; i < 6; ++i) {
-            const size_t k = 6 * (size_t)p + i;
-            const double dk = P.z[k] + beta * P.d[k];
-            P.d[k] = dk;
-            P.d32[k] = (float)dk;
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        st->iter = iter;
-        st->rnorm = rn;
-        st->rho = rz;
-    }
-}
-
-// True residual b - S x (part64 holds the passes run on x). With restart the
-// recursion continues from it: r = b - S x, d = z = M^-1 r.
-__global__ __launch_bounds__(kSmallBlock) void k_pcg_true_residual(DevProblem P, int restart) {
-    __shared__ double lds[(kSmallBlock / 64) * 2];
-    PcgState* st = P.pcg;
-    const double lambda = st->lambda;
-    double acc[2] = {0.0, 0.0};
-    for (int p = threadIdx.x; p < P.num_pose_slots; p += kSmallBlock) {
-        if (P.pose_var[p] < 0) continue;
-        double xp[6], sx[6], s[6] = {0, 0, 0, 0, 0, 0};
-        for (int i = 0; i < 6; ++i) xp[i] = P.x[6 * (size_t)p + i];
-        for (int t = P.cam_tile_ptr[p]; t < P.cam_tile_ptr[p + 1]; ++t)
-            for (int i = 0; i < 6; ++i) s[i] += P.part64[6 * (size_t)t + i];
-        sym6_mul(P.U + 21 * (size_t)p, xp, sx);
-        double rt[6];
-        for (int i = 0; i < 6; ++i) {
-            rt[i] = P.b[6 * (size_t)p + i] - (sx[i] + lambda * xp[i] - s[i]);
-            acc[0] += rt[i] * rt[i];
-        }
-        if (restart) {
-            double z[6];
-            minv_apply(P, p, rt, z);
-            for (int i = 0; i < 6; ++i) {
-                const size_t k = 6 * (size_t)p + i;
-                P.r[k] = rt[i];
-                P.z[k] = z[i];
-                P.d[k] = z[i];
-                P.d32[k] = (float)z[i];
-                acc[1] += rt[i] * z[i];
-            }
-        }
-    }
-    block_sum<kSmallBlock, double, 2>(acc, lds);
-    if (threadIdx.x == 0) {
-        st->true_norm = sqrt(acc[0]);
-        if (restart) {
-            st->rnorm = st->true_norm;
-            st->rho = acc[1];
-            if (!isfinite(acc[1]) || !(acc[1] > 0.0)) st->status = PCG_NONPOSITIVE;
-            else if (st->iter >= st->max_iter) st->status = PCG_MAXIT;
-            else st->status = PCG_RUNNING;
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Back-substitution, trial state and cost.
-// ---------------------------------------------------------------------------
-// delta_l = -V^-1 (g_l + sum_e W_e^T x_p) (normal_equations.rs:1551-1592,
-// implicit_schur.rs:789-824); trial landmark X + delta_l.
-__global__ __launch_bounds__(kBlock) void k_backsub(DevProblem P) {
-    __shared__ double lds[kBlock / 64];
-    if (P.wstate && !P.wstate[P.lm_tile_win[blockIdx.x]].active) return;
-    const LmTile lt = P.lm_tiles[blockIdx.x];
-    const int l = lt.lm_begin + (int)threadIdx.x;
-    double step = 0.0;
-    if (l < lt.lm_end) {
-        const double X[3] = {P.lms[3 * (size_t)l], P.lms[3 * (size_t)l + 1], P.lms[3 * (size_t)l + 2]};
-        double dl[3] = {0.0, 0.0, 0.0};
-        if (P.lm_var[l] >= 0) {
-            double t[3] = {P.gl[3 * (size_t)l], P.gl[3 * (size_t)l + 1], P.gl[3 * (size_t)l + 2]};
-            for (int e = P.lm_edge_ptr[l]; e < P.lm_edge_ptr[l + 1]; ++e) {
-                const EdgeRec64 r = P.el64[e];
-                const double* xp = P.x + 6 * (size_t)r.idx;
-                const double v[3] = {xp[0] + xp[4] * X[2] - xp[5] * X[1], xp[1] + xp[5] * X[0] - xp[3] * X[2],
-                                     xp[2] + xp[3] * X[1] - xp[4] * X[0]};
-                double gv[3];
-                sym3_mul(r.G, v, gv);
-                t[0] += gv[0];
-                t[1] += gv[1];
-                t[2] += gv[2];
-            }
-            sym3_mul(P.Vinv64 + 6 * (size_t)l, t, dl);
-            dl[0] = -dl[0];
-            dl[1] = -dl[1];
-            dl[2] = -dl[2];
-            step = sqrt(dl[0] * dl[0] + dl[1] * dl[1] + dl[2] * dl[2]);
-        }
-        for (int i = 0; i < 3; ++i) {
-            P.delta_l[3 * (size_t)l + i] = dl[i];
-            P.lms_trial[3 * (size_t)l + i] = X[i] + dl[i];
-        }
-    }
-    step = block_max<kBlock>(step, lds);
-    if (threadIdx.x == 0) P.tile_scalar[4 * (size_t)(P.num_cam_tiles + blockIdx.x) + 3] = step;
-}
-
-// Trial pose T * Exp(x_p) (lm.rs:672-690) and the pose step norm.
 __global__ __launch_bounds__(kBlock) void k_pose_update(DevProblem P) {
     __shared__ double lds[kBlock / 64];
     const int p = blockIdx.x * kBlock + threadIdx.x;
@@ -966,8 +337,8 @@ __device__ void dense_assemble_block(const DevProblem& P, const DensePairs& D, i
         const int l = D.terms[3 * (size_t)k], ep = D.terms[3 * (size_t)k + 1], eq = D.terms[3 * (size_t)k + 2];
         const double* Vi = P.Vinv64 + 6 * (size_t)l;
         const double vi[9] = {Vi[0], Vi[1], Vi[2], Vi[1], Vi[3], Vi[4], Vi[2], Vi[4], Vi[5]};
-        const double* A = P.el64[ep].G;
-        const double* B = P.el64[eq].G;
+        const double* A = P.Gl64 + 6 * (size_t)ep;
+        const double* B = P.Gl64 + 6 * (size_t)eq;
         const double a[9] = {A[0], A[1], A[2], A[1], A[3], A[4], A[2], A[4], A[5]};
         const double bb[9] = {B[0], B[1], B[2], B[1], B[3], B[4], B[2], B[4], B[5]};
         double T[9], C[9];
@@ -975,7 +346,7 @@ __device__ void dense_assemble_block(const DevProblem& P, const DensePairs& D, i
             for (int j = 0; j < 3; ++j) T[3 * i + j] = a[3 * i] * vi[j] + a[3 * i + 1] * vi[3 + j] + a[3 * i + 2] * vi[6 + j];
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j) C[3 * i + j] = T[3 * i] * bb[j] + T[3 * i + 1] * bb[3 + j] + T[3 * i + 2] * bb[6 + j];
-        const double* X = P.pack64[l].X;
+        const double* X = P.lms + 3 * (size_t)l;
         const double K[9] = {0.0, -X[2], X[1], X[2], 0.0, -X[0], -X[1], X[0], 0.0};
         double CK[9], KC[9], KCK[9];
         for (int i = 0; i < 3; ++i)
@@ -1341,7 +712,6 @@ __global__ void k_batch_commit(DevProblem P, BatchDev B) {
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-static inline int blocks_for(int n, int per) { return (n + per - 1) / per; }
 
 void launch_linearize(const DevProblem& P, bool store32, hipStream_t s) {
     if (P.num_cam_tiles == 0) return;
@@ -1358,48 +728,6 @@ void launch_landmark_reduce(const DevProblem& P, hipStream_t s) {
 }
 void launch_landmark_prepare(const DevProblem& P, double lambda, hipStream_t s) {
     hipLaunchKernelGGL(k_landmark_prepare, dim3(blocks_for(P.num_lm_slots, 256)), dim3(256), 0, s, P, lambda);
-}
-void launch_precondition(const DevProblem& P, double lambda, hipStream_t s) {
-    if (P.num_cam_tiles)
-        hipLaunchKernelGGL(k_precondition_tiles, dim3(xcd_grid(P.num_cam_tiles)), dim3(kBlock), 0, s, P);
-    hipLaunchKernelGGL(k_precondition_finalize, dim3(blocks_for(P.num_pose_slots, 128)), dim3(128), 0, s, P, lambda);
-}
-void launch_lm_pass(const DevProblem& P, bool fp32, const void* x, const int32_t* run_flag, hipStream_t s) {
-    if (P.num_lm_tiles == 0) return;
-    if (fp32)
-        hipLaunchKernelGGL((k_lm_pass<float, EdgeRec32>), dim3(P.num_lm_tiles), dim3(kBlock), 0, s, P,
-                           (const float*)x, (const EdgeRec32*)P.el32, (const float*)P.Vinv32, P.pack32, run_flag);
-    else
-        hipLaunchKernelGGL((k_lm_pass<double, EdgeRec64>), dim3(P.num_lm_tiles), dim3(kBlock), 0, s, P,
-                           (const double*)x, (const EdgeRec64*)P.el64, (const double*)P.Vinv64, P.pack64, run_flag);
-}
-void launch_cam_pass(const DevProblem& P, bool fp32, const int32_t* run_flag, hipStream_t s) {
-    if (P.num_cam_tiles == 0) return;
-    if (fp32)
-        hipLaunchKernelGGL((k_cam_pass<float, EdgeRec32>), dim3(xcd_grid(P.num_cam_tiles)), dim3(kBlock), 0, s, P,
-                           (const EdgeRec32*)P.ec32, (const LmPack<float>*)P.pack32, P.part32, run_flag);
-    else
-        hipLaunchKernelGGL((k_cam_pass<double, EdgeRec64>), dim3(xcd_grid(P.num_cam_tiles)), dim3(kBlock), 0, s, P,
-                           (const EdgeRec64*)P.ec64, (const LmPack<double>*)P.pack64, P.part64, run_flag);
-}
-void launch_rhs_finalize(const DevProblem& P, hipStream_t s) {
-    hipLaunchKernelGGL(k_rhs_finalize, dim3(blocks_for(P.num_pose_slots, 256)), dim3(256), 0, s, P);
-}
-void launch_pcg_init(const DevProblem& P, double rtol, double atol, int32_t max_iter, double lambda,
-                     hipStream_t s) {
-    hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(kSmallBlock), 0, s, P, rtol, atol, max_iter, lambda);
-}
-void launch_pcg_update(const DevProblem& P, bool fp32, hipStream_t s) {
-    if (fp32)
-        hipLaunchKernelGGL(k_pcg_update<float>, dim3(1), dim3(kSmallBlock), 0, s, P, (const float*)P.part32);
-    else
-        hipLaunchKernelGGL(k_pcg_update<double>, dim3(1), dim3(kSmallBlock), 0, s, P, (const double*)P.part64);
-}
-void launch_pcg_true_residual(const DevProblem& P, bool restart, hipStream_t s) {
-    hipLaunchKernelGGL(k_pcg_true_residual, dim3(1), dim3(kSmallBlock), 0, s, P, (int)restart);
-}
-void launch_backsub(const DevProblem& P, hipStream_t s) {
-    if (P.num_lm_tiles) hipLaunchKernelGGL(k_backsub, dim3(P.num_lm_tiles), dim3(kBlock), 0, s, P);
 }
 void launch_pose_update(const DevProblem& P, hipStream_t s) {
     hipLaunchKernelGGL(k_pose_update, dim3(blocks_for(P.num_pose_slots, kBlock)), dim3(kBlock), 0, s, P);

@@ -24,32 +24,9 @@
 
 namespace vba {
 
-constexpr int kBlock = 256;     // threads per tile work-group
-constexpr int kSmallBlock = 1024;  // single work-group vector kernels
-constexpr int kTileEdges = 1024;   // edges of a landmark tile staged in LDS
-
-template <typename T>
-struct LmPack {  // per landmark slot: position and the current landmark vector
-    T X[3];
-    T pad0;
-    T u[3];
-    T pad1;
-};
-
-// One edge as streamed by the Schur kernels: the upper triangle of G
-// (g00 g01 g02 g11 g12 g22) and the index of the other end of the edge (the
-// landmark slot in c-order records, the pose slot in l-order records),
-// padded to one 32 B (FP32) or 64 B (FP64) sector.
-struct alignas(32) EdgeRec32 {
-    float G[6];
-    int32_t idx;
-    int32_t pad;
-};
-struct alignas(64) EdgeRec64 {
-    double G[6];
-    int32_t idx;
-    int32_t pad[3];
-};
+constexpr int kBlock = 256;        // threads per tile work-group
+constexpr int kSmallBlock = 1024;  // single work-group kernels
+constexpr int kPosesPerWave = 10;  // PCG step kernel: 6 lanes per pose
 
 // Scalars exchanged with the host once per LM iteration.
 struct StepScalars {
@@ -61,9 +38,13 @@ struct StepScalars {
 };
 
 // PCG state kept on the device (implicit_schur.rs:522-736 control flow).
+// rho holds r.z of the last two iterations by iteration parity, so the
+// kernels of one iteration read the previous value while the first
+// work-group of the operator kernel records the new one.
 struct PcgState {
-    double rho, rhs_norm, target, rnorm, true_norm, lambda;
-    int32_t iter, status, max_iter, flags;
+    double rhs_norm, target, rnorm, true_norm, lambda, beta;
+    double rho[2];
+    int32_t iter, status, max_iter, stop;
 };
 enum PcgStatus : int32_t {
     PCG_RUNNING = 0,
@@ -105,12 +86,10 @@ struct DevProblem {
     const CamTile* cam_tiles;
     const int32_t* cam_tile_ptr;
     const LmTile* lm_tiles;
-    // linearization
-    EdgeRec64* ec64;  // c-order edges (idx = landmark slot)
-    EdgeRec64* el64;  // l-order edges (idx = pose slot)
-    double* hl64;     // 3 per edge, l-order
-    EdgeRec32* ec32;  // mixed precision copies
-    EdgeRec32* el32;
+    // linearization: per l-order edge the upper triangle of G and h
+    double* Gl64;    // 6 per edge
+    float* Gl32;     // FP32 copy for the mixed-precision operator
+    double* hl64;    // 3 per edge
     double* tile27;  // per camera tile partials (27)
     double* U;       // 21 per pose slot (upper 6x6, row-major)
     double* g;       // 6 per pose slot
@@ -118,14 +97,26 @@ struct DevProblem {
     double* gl;      // 3 per landmark slot
     double* Vinv64;  // 6 per landmark slot
     float* Vinv32;
-    LmPack<double>* pack64;
-    LmPack<float>* pack32;
+    float4* X32;     // landmark positions, FP32
     double* Minv;    // 36 per pose slot (block-Jacobi inverse)
+    // Schur operator structure. The edges of a landmark tile, sorted by pose
+    // (tile_perm: tile-local l-order index at each sorted position), form
+    // camera runs; each run leaves one partial, numbered from
+    // tile_run_base[tile] in sorted order, and a pose sums its runs
+    // (pose_runs[pose_run_ptr[p] ..]) in ascending order.
+    const uint16_t* tile_perm;
+    const int32_t* tile_run_base;  // num_lm_tiles + 1
+    const int32_t* pose_run_ptr;   // num_pose_slots + 1
+    const int32_t* pose_runs;
+    double* part64;  // 6 per run
+    float* part32;
+    double* part21;  // 21 per run (block-Jacobi blocks)
     // PCG vectors, 6 per pose slot
     double *b, *x, *r, *z, *d, *q;
-    float* d32;
-    double* part64;  // per camera tile partials (6)
-    float* part32;
+    float *z32, *d32;
+    double* bpart;  // per work-group partials of the pose kernels (2 each)
+    double* cpart;  // per work-group (r.z, r.r) of the step kernel
+    int32_t num_bpart, num_cpart;
     PcgState* pcg;
     // step outputs
     double* delta_l;       // 3 per landmark slot
@@ -151,28 +142,43 @@ struct WindowLayout {
 };
 
 // ---- launchers (all asynchronous on `stream`) ----
+// kernels.hip
 void launch_linearize(const DevProblem& P, bool store32, hipStream_t stream);
 void launch_camera_finalize(const DevProblem& P, hipStream_t stream);
 void launch_landmark_reduce(const DevProblem& P, hipStream_t stream);
 void launch_landmark_prepare(const DevProblem& P, double lambda, hipStream_t stream);
-void launch_precondition(const DevProblem& P, double lambda, hipStream_t stream);
-// Landmark pass u_l = V_l^-1 sum_e W_e^T x_p(e) into pack.u. x is 6 per pose slot.
-void launch_lm_pass(const DevProblem& P, bool fp32, const void* x, const int32_t* run_flag,
-                    hipStream_t stream);
-// Camera pass partial_tile = sum_e W_e u_l(e) over each camera tile.
-void launch_cam_pass(const DevProblem& P, bool fp32, const int32_t* run_flag, hipStream_t stream);
-void launch_rhs_finalize(const DevProblem& P, hipStream_t stream);
-void launch_pcg_init(const DevProblem& P, double rtol, double atol, int32_t max_iter, double lambda,
-                     hipStream_t stream);
-void launch_pcg_update(const DevProblem& P, bool fp32, hipStream_t stream);
-// True residual b - S x from part64 (after passes on x); restart = reset the
-// recursion from it.
-void launch_pcg_true_residual(const DevProblem& P, bool restart, hipStream_t stream);
-void launch_backsub(const DevProblem& P, hipStream_t stream);
 void launch_pose_update(const DevProblem& P, hipStream_t stream);
 // Cost and non-projectable count of (poses, lms) or of the trial state.
 void launch_cost(const DevProblem& P, bool trial, hipStream_t stream);
 void launch_scalars_finalize(const DevProblem& P, hipStream_t stream);
+
+// schur.hip: the Schur operator over landmark tiles and the PCG vector work.
+// Block-Jacobi inverse blocks of U + lambda I - W V^-1 W^T into Minv.
+void launch_precondition(const DevProblem& P, double lambda, hipStream_t stream);
+// b = -g_p + sum_e W_e V^-1 g_l.
+void launch_rhs(const DevProblem& P, hipStream_t stream);
+// Operator partials of d = v1 + beta v2 (v2 may be null) into the run
+// partials: part32 (fp32) or part64. With pcg_iter >= 0 the launch is PCG
+// iteration pcg_iter: each work-group first derives the iteration's
+// scalars (beta, stop) from the step kernel's partials.
+void launch_schur_apply(const DevProblem& P, bool fp32, const void* v1, const void* v2, int32_t pcg_iter,
+                        int32_t fresh, hipStream_t stream);
+// PCG: r = b, z = M^-1 r, x = d = 0 and the state (implicit_schur.rs:522-560).
+void launch_pcg_init(const DevProblem& P, double rtol, double atol, int32_t max_iter, double lambda,
+                     hipStream_t stream);
+// d = z + beta d, q = S d from the run partials, and d.q partials.
+void launch_pcg_pose(const DevProblem& P, bool fp32, hipStream_t stream);
+// alpha, x += alpha d, r -= alpha q, z = M^-1 r and (r.z, r.r) partials.
+void launch_pcg_step(const DevProblem& P, bool fp32, int32_t pcg_iter, hipStream_t stream);
+// The scalars of iteration pcg_iter (status, residual norm) without its work.
+void launch_pcg_status(const DevProblem& P, int32_t pcg_iter, int32_t fresh, hipStream_t stream);
+// True residual b - S x in FP64 (state: true_norm).
+void launch_pcg_true_residual(const DevProblem& P, hipStream_t stream);
+// Continue the recursion from the residual of the last true-residual check:
+// r = b - S x, z = M^-1 r, d = 0 (FP64 refinement of the mixed solve).
+void launch_pcg_restart(const DevProblem& P, hipStream_t stream);
+// delta_l = -V^-1 (g_l + sum_e W_e^T x_p), trial landmarks, step maxima.
+void launch_backsub(const DevProblem& P, hipStream_t stream);
 
 // Dense reduced system (LinearSolver::Dense).
 struct DensePairs {

@@ -51,11 +51,77 @@ bool slot_order(uint64_t n, const uint64_t* ids, std::vector<uint32_t>* slot_of_
     }
     return true;
 }
-bool lookup(const std::vector<uint64_t>& sorted, uint64_t id, uint32_t* slot) {
-    auto it = std::lower_bound(sorted.begin(), sorted.end(), id);
-    if (it == sorted.end() || *it != id) return false;
-    *slot = (uint32_t)(it - sorted.begin());
-    return true;
+// Id -> slot over the ascending ids: arithmetic when the ids are
+// consecutive (the common case), binary search otherwise.
+struct IdIndex {
+    const std::vector<uint64_t>& sorted;
+    bool dense;
+    explicit IdIndex(const std::vector<uint64_t>& s)
+        : sorted(s), dense(!s.empty() && s.back() - s.front() == s.size() - 1) {}
+    bool operator()(uint64_t id, uint32_t* slot) const {
+        if (dense) {
+            if (id < sorted.front() || id > sorted.back()) return false;
+            *slot = (uint32_t)(id - sorted.front());
+            return true;
+        }
+        auto it = std::lower_bound(sorted.begin(), sorted.end(), id);
+        if (it == sorted.end() || *it != id) return false;
+        *slot = (uint32_t)(it - sorted.begin());
+        return true;
+    }
+};
+
+// Camera runs of every landmark tile (Model::tile_perm and the run lists).
+// Within a landmark, l-order edges are already in pose order and each pose
+// appears once, so a tile of one landmark needs no permutation.
+void build_runs(Model* mp) {
+    Model& m = *mp;
+    m.tile_perm.assign(m.num_edges, 0);
+    m.tile_run_base.assign(m.lm_tiles.size() + 1, 0);
+    std::vector<int32_t> run_pose, count;
+    std::vector<uint64_t> keys;
+    for (size_t t = 0; t < m.lm_tiles.size(); ++t) {
+        m.tile_run_base[t] = (int32_t)run_pose.size();
+        const int32_t e0 = m.lm_edge_ptr[m.lm_tiles[t].lm_begin], e1 = m.lm_edge_ptr[m.lm_tiles[t].lm_end];
+        const int32_t n = e1 - e0;
+        if (n > kTileEdges) {
+            for (int32_t s = 0; s < n; ++s) run_pose.push_back(m.l_pose[e0 + s]);
+            continue;
+        }
+        // stable sort by pose: counting sort over the tile's pose range when it
+        // is narrow (landmarks ordered by first camera), else a key sort
+        int32_t lo = m.l_pose[e0], hi = lo;
+        for (int32_t k = 1; k < n; ++k) {
+            lo = std::min(lo, m.l_pose[e0 + k]);
+            hi = std::max(hi, m.l_pose[e0 + k]);
+        }
+        const int32_t range = hi - lo + 1;
+        if (range <= 8 * n + 64) {
+            count.assign(range + 1, 0);
+            for (int32_t k = 0; k < n; ++k) ++count[m.l_pose[e0 + k] - lo + 1];
+            for (int32_t r = 0; r < range; ++r) {
+                if (count[r + 1]) run_pose.push_back(lo + r);
+                count[r + 1] += count[r];
+            }
+            for (int32_t k = 0; k < n; ++k) m.tile_perm[e0 + count[m.l_pose[e0 + k] - lo]++] = (uint16_t)k;
+        } else {
+            keys.resize(n);
+            for (int32_t k = 0; k < n; ++k) keys[k] = ((uint64_t)(uint32_t)m.l_pose[e0 + k] << 16) | (uint64_t)k;
+            std::sort(keys.begin(), keys.end());  // by pose, then l-order index
+            for (int32_t s = 0; s < n; ++s) {
+                m.tile_perm[e0 + s] = (uint16_t)(keys[s] & 0xffff);
+                if (s == 0 || (keys[s] >> 16) != (keys[s - 1] >> 16)) run_pose.push_back((int32_t)(keys[s] >> 16));
+            }
+        }
+    }
+    m.tile_run_base[m.lm_tiles.size()] = (int32_t)run_pose.size();
+    m.num_runs = run_pose.size();
+    m.pose_run_ptr.assign(m.num_pose_slots + 1, 0);
+    for (int32_t p : run_pose) ++m.pose_run_ptr[p + 1];
+    for (uint32_t s = 0; s < m.num_pose_slots; ++s) m.pose_run_ptr[s + 1] += m.pose_run_ptr[s];
+    m.pose_runs.assign(run_pose.size(), 0);
+    std::vector<int32_t> cursor(m.pose_run_ptr.begin(), m.pose_run_ptr.end() - 1);
+    for (size_t r = 0; r < run_pose.size(); ++r) m.pose_runs[cursor[run_pose[r]]++] = (int32_t)r;
 }
 
 uint32_t next_pow2(uint64_t v) {
@@ -227,24 +293,25 @@ int32_t build_model(const vba_problem& pb, Model* mp, uint64_t* error_id, std::s
     // Observations: mono then rig, ids resolved in input order (lm.rs:193-229).
     const uint64_t nm = pb.num_observations, nr = pb.num_rig_observations, n = nm + nr;
     std::vector<uint32_t> opose(n), olm(n);
+    const IdIndex pose_slot(pose_ids), lm_slot(lm_ids);
     std::vector<uint16_t> osensor(n, 0);
     for (uint64_t i = 0; i < nm; ++i) {
-        if (!lookup(pose_ids, pb.obs_pose_ids[i], &opose[i])) {
+        if (!pose_slot(pb.obs_pose_ids[i], &opose[i])) {
             *error_id = pb.obs_pose_ids[i];
             return VBA_ERR_MISSING_POSE;
         }
-        if (!lookup(lm_ids, pb.obs_landmark_ids[i], &olm[i])) {
+        if (!lm_slot(pb.obs_landmark_ids[i], &olm[i])) {
             *error_id = pb.obs_landmark_ids[i];
             return VBA_ERR_MISSING_LANDMARK;
         }
     }
     for (uint64_t i = 0; i < nr; ++i) {
         uint64_t k = nm + i;
-        if (!lookup(pose_ids, pb.rig_pose_ids[i], &opose[k])) {
+        if (!pose_slot(pb.rig_pose_ids[i], &opose[k])) {
             *error_id = pb.rig_pose_ids[i];
             return VBA_ERR_MISSING_POSE;
         }
-        if (!lookup(lm_ids, pb.rig_landmark_ids[i], &olm[k])) {
+        if (!lm_slot(pb.rig_landmark_ids[i], &olm[k])) {
             *error_id = pb.rig_landmark_ids[i];
             return VBA_ERR_MISSING_LANDMARK;
         }
@@ -260,6 +327,28 @@ int32_t build_model(const vba_problem& pb, Model* mp, uint64_t* error_id, std::s
         osensor[k] = (uint16_t)(s + 1);
     }
     m.any_rig = nr > 0;
+    // Landmark slots in order of their first observing pose slot (ties in id
+    // order): the landmarks of a tile then share cameras, so its edges form
+    // few, long camera runs. Outputs map back through lm_slot_of_input.
+    {
+        std::vector<uint32_t> first(m.num_lm_slots, m.num_pose_slots);
+        for (uint64_t i = 0; i < n; ++i) first[olm[i]] = std::min(first[olm[i]], opose[i]);
+        std::vector<uint32_t> start(m.num_pose_slots + 2, 0);
+        for (uint32_t l = 0; l < m.num_lm_slots; ++l) ++start[first[l] + 1];
+        for (uint32_t p = 0; p <= m.num_pose_slots; ++p) start[p + 1] += start[p];
+        std::vector<uint32_t> new_slot(m.num_lm_slots);
+        for (uint32_t l = 0; l < m.num_lm_slots; ++l) new_slot[l] = start[first[l]]++;
+        for (uint64_t i = 0; i < n; ++i) olm[i] = new_slot[olm[i]];
+        for (uint32_t& s : m.lm_slot_of_input) s = new_slot[s];
+        std::vector<double> lms(m.lms.size());
+        std::vector<uint8_t> fixed(lm_fixed.size());
+        for (uint32_t s = 0; s < m.num_lm_slots; ++s) {
+            std::memcpy(&lms[3 * (size_t)new_slot[s]], &m.lms[3 * (size_t)s], 3 * sizeof(double));
+            fixed[new_slot[s]] = lm_fixed[s];
+        }
+        m.lms.swap(lms);
+        lm_fixed.swap(fixed);
+    }
     m.pose_var.assign(m.num_pose_slots, -1);
     m.lm_var.assign(m.num_lm_slots, -1);
     m.P = m.L = 0;
@@ -272,19 +361,23 @@ int32_t build_model(const vba_problem& pb, Model* mp, uint64_t* error_id, std::s
         if (!lm_fixed[s]) m.lm_var[s] = (int32_t)m.L++;
     if (m.P == 0 && m.L == 0) return VBA_ERR_ALL_POSES_FIXED;
 
-    // c-order: counting sort by pose slot (stable), then (landmark, input) within.
+    // c-order: (pose, landmark, input index) by two stable counting sorts,
+    // by landmark and then by pose.
     m.num_obs = n;
-    std::vector<uint32_t> count(m.num_pose_slots + 1, 0);
-    for (uint64_t i = 0; i < n; ++i) ++count[opose[i] + 1];
-    for (uint32_t s = 0; s < m.num_pose_slots; ++s) count[s + 1] += count[s];
     std::vector<uint32_t> cord(n);
     {
-        std::vector<uint32_t> cursor(count.begin(), count.end() - 1);
-        for (uint64_t i = 0; i < n; ++i) cord[cursor[opose[i]]++] = (uint32_t)i;
+        std::vector<uint32_t> by_lm(n), cursor(std::max(m.num_lm_slots, m.num_pose_slots) + 1, 0);
+        for (uint64_t i = 0; i < n; ++i) ++cursor[olm[i] + 1];
+        for (uint32_t l = 0; l < m.num_lm_slots; ++l) cursor[l + 1] += cursor[l];
+        for (uint64_t i = 0; i < n; ++i) by_lm[cursor[olm[i]]++] = (uint32_t)i;
+        std::fill(cursor.begin(), cursor.end(), 0u);
+        for (uint64_t i = 0; i < n; ++i) ++cursor[opose[i] + 1];
+        for (uint32_t p = 0; p < m.num_pose_slots; ++p) cursor[p + 1] += cursor[p];
+        for (uint64_t k = 0; k < n; ++k) {
+            const uint32_t i = by_lm[k];
+            cord[cursor[opose[i]]++] = i;
+        }
     }
-    for (uint32_t s = 0; s < m.num_pose_slots; ++s)
-        std::sort(cord.begin() + count[s], cord.begin() + count[s + 1],
-                  [&](uint32_t a, uint32_t b) { return olm[a] != olm[b] ? olm[a] < olm[b] : a < b; });
     m.obs_uv.resize(2 * n);
     m.obs_input = cord;
     if (m.any_rig) m.obs_sensor.resize(n);
@@ -347,11 +440,14 @@ int32_t build_model(const vba_problem& pb, Model* mp, uint64_t* error_id, std::s
     m.max_lm_tile_edges = 0;
     for (uint32_t b = 0; b < m.num_lm_slots;) {
         uint32_t e = b + 1;
-        while (e < m.num_lm_slots && e - b < 256 && m.lm_edge_ptr[e + 1] - m.lm_edge_ptr[b] <= 1024) ++e;
+        while (e < m.num_lm_slots && e - b < (uint32_t)kTileLandmarks &&
+               m.lm_edge_ptr[e + 1] - m.lm_edge_ptr[b] <= kTileEdges)
+            ++e;
         m.lm_tiles.push_back({(int32_t)b, (int32_t)e});
         m.max_lm_tile_edges = std::max(m.max_lm_tile_edges, m.lm_edge_ptr[e] - m.lm_edge_ptr[b]);
         b = e;
     }
+    build_runs(&m);
     return VBA_OK;
 }
 

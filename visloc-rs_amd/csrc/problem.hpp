@@ -49,6 +49,12 @@ struct LmTile {
     int32_t lm_end;
 };
 
+// Landmark tiles: at most this many whole landmarks and l-order edges, the
+// LDS staging capacity of the landmark-tile kernels (a single landmark with
+// more edges forms a tile of its own).
+constexpr int kTileEdges = 1024;
+constexpr int kTileLandmarks = 256;
+
 struct Model {
     uint32_t num_pose_slots = 0, num_lm_slots = 0;
     uint32_t P = 0, L = 0;  // variable poses / landmarks
@@ -77,6 +83,15 @@ struct Model {
     std::vector<int32_t> cam_tile_ptr;  // num_pose_slots + 1
     std::vector<LmTile> lm_tiles;
     int32_t max_lm_tile_edges = 0;
+    // Camera runs of the landmark tiles: tile_perm[e] = tile-local l-order
+    // index at sorted position e - (tile's first edge), sorting the tile's
+    // edges by pose; a run is a maximal group of equal poses in that order.
+    // Runs are numbered tile by tile in sorted order (tile_run_base); each
+    // pose lists its runs in ascending order (pose_run_ptr / pose_runs).
+    std::vector<uint16_t> tile_perm;
+    std::vector<int32_t> tile_run_base;
+    std::vector<int32_t> pose_run_ptr, pose_runs;
+    uint64_t num_runs = 0;
 };
 
 // Lens of a vba_camera, resolved as MonoProjection::for_camera does

@@ -1,0 +1,848 @@
+// schur.hip — the reduced camera system on CDNA4 (gfx950): the Schur
+// operator S d = (U + lambda I) d - W V^-1 W^T d as one pass over landmark
+// tiles, its right-hand side, block-Jacobi blocks and back-substitution,
+// and the PCG vector work over poses (implicit_schur.rs:116-824).
+//
+// Landmark-tile pass: one work-group per tile (<= 256 whole landmarks,
+// <= 1024 l-order edges; a landmark with more edges is a tile of its own),
+// each edge record read once, coalesced:
+//   1. lane per edge: t_e = W_e^T d_p = G (rho + omega x X) into LDS;
+//   2. each landmark's owner lane sums its edges in order: u_l = V_l^-1 t;
+//   3. the tile's edges in pose-sorted order (tile_perm) give
+//      y_e = W_e u_l = [G u; X x (G u)]; a wave-level segmented scan keyed
+//      by pose sums each camera run, runs crossing 64-edge chunks are joined
+//      through LDS in chunk order, and every run writes one partial.
+// A pose then sums its runs' partials in ascending order. There are no
+// atomics: every sum has a fixed order, so results do not depend on
+// scheduling, and every edge's G is read from HBM once per operator
+// application instead of once per landmark pass and once per camera pass.
+//
+// PCG (implicit_schur.rs:522-736) runs as three kernels per iteration:
+// the tile pass (whose work-groups first derive beta and the stop test
+// from the step kernel's partials), a pose kernel (d = z + beta d,
+// q = S d, d.q partials) and a step kernel (alpha, x, r, z = M^-1 r, r.z and
+// r.r partials). Scalars never leave the device between iterations.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "device_common.hpp"
+
+namespace vba {
+
+namespace {
+
+enum SchurOp { OP_APPLY = 0, OP_RHS = 1, OP_PRECOND = 2, OP_BACKSUB = 3 };
+
+template <typename T>
+struct Vec2;
+template <>
+struct Vec2<float> {
+    using type = float2;
+};
+template <>
+struct Vec2<double> {
+    using type = double2;
+};
+
+// Six consecutive values at an address aligned to two of them.
+template <typename T>
+__device__ __forceinline__ void load6(const T* __restrict__ p, T v[6]) {
+    using V2 = typename Vec2<T>::type;
+    const V2* q = reinterpret_cast<const V2*>(p);
+    const V2 a = q[0], b = q[1], c = q[2];
+    v[0] = a.x;
+    v[1] = a.y;
+    v[2] = b.x;
+    v[3] = b.y;
+    v[4] = c.x;
+    v[5] = c.y;
+}
+
+__device__ __forceinline__ void load_X(const DevProblem& P, int l, float X[3]) {
+    const float4 x = P.X32[l];
+    X[0] = x.x;
+    X[1] = x.y;
+    X[2] = x.z;
+}
+__device__ __forceinline__ void load_X(const DevProblem& P, int l, double X[3]) {
+    X[0] = P.lms[3 * (size_t)l];
+    X[1] = P.lms[3 * (size_t)l + 1];
+    X[2] = P.lms[3 * (size_t)l + 2];
+}
+
+// A flag another work-group may set during this kernel, read once per
+// work-group so that all its threads take the same branch.
+__device__ __forceinline__ int block_flag(int32_t* p, int* sh) {
+    if (threadIdx.x == 0) *sh = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    return *sh;
+}
+__device__ __forceinline__ void set_stop(PcgState* st, int status) {
+    st->status = status;
+    __hip_atomic_store(&st->stop, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Scalars of PCG iteration k, the same in every work-group: r.z and r.r of
+// the current residual from the step kernel's partials (or the totals the
+// init / restart kernels left in entry 0), the stop test on the recursive
+// residual (implicit_schur.rs:640-700) and beta = rho_k / rho_{k-1}.
+// Work-group 0 records them. Returns false when the solve has stopped.
+__device__ bool pcg_prologue(const DevProblem& P, int k, int fresh, double* beta_out, double* lds, int* flag) {
+    PcgState* st = P.pcg;
+    if (block_flag(&st->stop, flag)) return false;
+    double v[2] = {0.0, 0.0};
+    for (int i = threadIdx.x; i < P.num_cpart; i += kBlock) {
+        v[0] += P.cpart[2 * (size_t)i];
+        v[1] += P.cpart[2 * (size_t)i + 1];
+    }
+    block_sum_all<kBlock, double, 2>(v, lds);
+    const double rz = v[0];
+    double rn, beta = 0.0;
+    int status = PCG_RUNNING;
+    if (fresh) {
+        rn = st->rnorm;
+    } else {
+        rn = sqrt(v[1]);
+        if (!isfinite(rn)) status = PCG_NONFINITE;
+        else if (rn <= st->target) status = PCG_CONVERGED;
+        else if (k >= st->max_iter) status = PCG_MAXIT;
+        else if (!isfinite(rz) || !(rz > 0.0)) status = PCG_NONPOSITIVE;
+        beta = rz / st->rho[(k - 1) & 1];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->rho[k & 1] = rz;
+        st->beta = beta;
+        st->iter = k;
+        st->rnorm = rn;
+        if (status != PCG_RUNNING) set_stop(st, status);
+    }
+    *beta_out = beta;
+    return status == PCG_RUNNING;
+}
+
+// y = [G u; X x (G u)]: the camera-side block W_e u of one edge.
+template <typename T>
+__device__ __forceinline__ void edge_wu(const T G[6], const T u[3], const T X[3], T y[6]) {
+    T gu[3];
+    sym3_mul(G, u, gu);
+    y[0] = gu[0];
+    y[1] = gu[1];
+    y[2] = gu[2];
+    y[3] = X[1] * gu[2] - X[2] * gu[1];
+    y[4] = X[2] * gu[0] - X[0] * gu[2];
+    y[5] = X[0] * gu[1] - X[1] * gu[0];
+}
+
+// Upper triangle of W_e V^-1 W_e^T = [[C, -C K], [K C, -K C K]], C = G V^-1 G,
+// K = [X]x (implicit_schur.rs:241-287).
+__device__ __forceinline__ void edge_wvw(const double G[6], const double Vi[6], const double X[3], double y[21]) {
+    const double vi[9] = {Vi[0], Vi[1], Vi[2], Vi[1], Vi[3], Vi[4], Vi[2], Vi[4], Vi[5]};
+    const double g[9] = {G[0], G[1], G[2], G[1], G[3], G[4], G[2], G[4], G[5]};
+    double T[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) T[3 * i + j] = g[3 * i] * vi[j] + g[3 * i + 1] * vi[3 + j] + g[3 * i + 2] * vi[6 + j];
+    double C[6];
+    C[0] = T[0] * g[0] + T[1] * g[3] + T[2] * g[6];
+    C[1] = T[0] * g[1] + T[1] * g[4] + T[2] * g[7];
+    C[2] = T[0] * g[2] + T[1] * g[5] + T[2] * g[8];
+    C[3] = T[3] * g[1] + T[4] * g[4] + T[5] * g[7];
+    C[4] = T[3] * g[2] + T[4] * g[5] + T[5] * g[8];
+    C[5] = T[6] * g[2] + T[7] * g[5] + T[8] * g[8];
+#pragma unroll
+    for (int i = 0; i < 21; ++i) y[i] = 0.0;
+    add_schur_block<double>(C, X, 1.0, y);
+}
+
+// t = G (rho + omega x X) with d = v1 + beta v2 at pose p.
+template <typename T>
+__device__ __forceinline__ void edge_wtd(const T G[6], const T* __restrict__ v1, const T* __restrict__ v2, T beta,
+                                         int p, const T X[3], T t[3]) {
+    T d[6];
+    load6(v1 + 6 * (size_t)p, d);
+    if (v2) {
+        T w[6];
+        load6(v2 + 6 * (size_t)p, w);
+#pragma unroll
+        for (int i = 0; i < 6; ++i) d[i] = d[i] + beta * w[i];
+    }
+    const T v[3] = {d[0] + d[4] * X[2] - d[5] * X[1], d[1] + d[5] * X[0] - d[3] * X[2],
+                    d[2] + d[3] * X[1] - d[4] * X[0]};
+    sym3_mul(G, v, t);
+}
+
+template <typename T, int OP>
+__global__ __launch_bounds__(kBlock) void k_schur_tile(DevProblem P, const T* __restrict__ v1,
+                                                       const T* __restrict__ v2, const T* __restrict__ Gl,
+                                                       const T* __restrict__ Vinv, T* __restrict__ part,
+                                                       int pcg_iter, int fresh) {
+    constexpr int NV = OP == OP_PRECOND ? 21 : 6;
+    constexpr bool kStageG = sizeof(T) == 4;  // FP64 G is re-read from L2 instead
+    constexpr int kChunks = kTileEdges / 64;
+    constexpr int kWaves = kBlock / 64;
+    __shared__ T sX[kTileLandmarks * 3];
+    __shared__ T sU[kTileLandmarks * 3];
+    __shared__ T sT[kTileEdges * 3];  // stage-1 products (OP_PRECOND: V^-1 per landmark)
+    __shared__ T sG[kStageG ? kTileEdges * 6 : 1];
+    __shared__ int32_t sKey[kTileEdges];  // pose at each sorted position
+    __shared__ uint16_t sPerm[kTileEdges];
+    __shared__ uint8_t sLm[kTileEdges];  // tile-local landmark of each l-order edge
+    __shared__ T sHead[kChunks * NV];
+    __shared__ T sEnd[kChunks * NV];
+    __shared__ int32_t sStarts[kChunks], sEndRun[kChunks];
+    __shared__ double sRed[kWaves * 3];
+    __shared__ T sBig[3];
+    __shared__ int sFlag;
+
+    const int tile = xcd_tile(blockIdx.x, P.num_lm_tiles);
+    if (tile >= P.num_lm_tiles) return;
+    if (P.wstate && !P.wstate[P.lm_tile_win[tile]].active) return;
+    double beta64 = 0.0;
+    if (OP == OP_APPLY && pcg_iter >= 0) {
+        if (!pcg_prologue(P, pcg_iter, fresh, &beta64, sRed, &sFlag)) return;
+    }
+    const T beta = (T)beta64;
+    const int tid = threadIdx.x;
+    const LmTile lt = P.lm_tiles[tile];
+    const int e0 = P.lm_edge_ptr[lt.lm_begin], e1 = P.lm_edge_ptr[lt.lm_end];
+    const int n = e1 - e0, nl = lt.lm_end - lt.lm_begin;
+    const int run0 = P.tile_run_base[tile];
+
+    if (n > kTileEdges) {
+        // One landmark with more edges than LDS holds. Its edges are in pose
+        // order and every pose appears once, so each edge is a run of its own.
+        const int l = lt.lm_begin;
+        T X[3];
+        load_X(P, l, X);
+        if (OP == OP_APPLY || OP == OP_BACKSUB) {
+            T t[3] = {T(0), T(0), T(0)};
+            for (int e = e0 + tid; e < e1; e += kBlock) {
+                T G[6], te[3];
+                load6(Gl + 6 * (size_t)e, G);
+                edge_wtd(G, v1, v2, beta, P.l_pose[e], X, te);
+                t[0] += te[0];
+                t[1] += te[1];
+                t[2] += te[2];
+            }
+            block_sum<kBlock, T, 3>(t, reinterpret_cast<T*>(sRed));
+            if (tid == 0) {
+                if (OP == OP_APPLY) {
+                    T vi[6];
+                    load6(Vinv + 6 * (size_t)l, vi);
+                    sym3_mul(vi, t, sBig);
+                } else {
+                    double dl[3] = {0.0, 0.0, 0.0};
+                    if (P.lm_var[l] >= 0) {
+                        const double tt[3] = {P.gl[3 * (size_t)l] + (double)t[0], P.gl[3 * (size_t)l + 1] + (double)t[1],
+                                              P.gl[3 * (size_t)l + 2] + (double)t[2]};
+                        sym3_mul(P.Vinv64 + 6 * (size_t)l, tt, dl);
+                        dl[0] = -dl[0];
+                        dl[1] = -dl[1];
+                        dl[2] = -dl[2];
+                    }
+                    for (int i = 0; i < 3; ++i) {
+                        P.delta_l[3 * (size_t)l + i] = dl[i];
+                        P.lms_trial[3 * (size_t)l + i] = P.lms[3 * (size_t)l + i] + dl[i];
+                    }
+                    P.tile_scalar[4 * (size_t)(P.num_cam_tiles + tile) + 3] =
+                        sqrt(dl[0] * dl[0] + dl[1] * dl[1] + dl[2] * dl[2]);
+                }
+            }
+            if (OP == OP_BACKSUB) return;
+        } else if (OP == OP_RHS && tid == 0) {
+            sym3_mul(P.Vinv64 + 6 * (size_t)l, P.gl + 3 * (size_t)l, reinterpret_cast<double*>(sBig));
+        }
+        __syncthreads();
+        T u[3] = {sBig[0], sBig[1], sBig[2]};
+        double vi[6];
+        if (OP == OP_PRECOND)
+            for (int i = 0; i < 6; ++i) vi[i] = P.Vinv64[6 * (size_t)l + i];
+        for (int s = tid; s < n; s += kBlock) {
+            T G[6], y[NV];
+            load6(Gl + 6 * (size_t)(e0 + s), G);
+            if constexpr (OP == OP_PRECOND) edge_wvw(G, vi, X, y);
+            else edge_wu(G, u, X, y);
+#pragma unroll
+            for (int i = 0; i < NV; ++i) part[(size_t)(run0 + s) * NV + i] = y[i];
+        }
+        return;
+    }
+
+    // ---- stage 0: landmark positions, edge owners, the sorted order ----
+    if (tid < nl) {
+        const int l = lt.lm_begin + tid;
+        T X[3];
+        load_X(P, l, X);
+        sX[3 * tid] = X[0];
+        sX[3 * tid + 1] = X[1];
+        sX[3 * tid + 2] = X[2];
+        const int b = P.lm_edge_ptr[l] - e0, e = P.lm_edge_ptr[l + 1] - e0;
+        for (int k = b; k < e; ++k) sLm[k] = (uint8_t)tid;
+        if (OP == OP_PRECOND)
+            for (int i = 0; i < 6; ++i) sT[6 * tid + i] = P.Vinv64[6 * (size_t)l + i];
+        if (OP == OP_RHS) {
+            double u[3];
+            sym3_mul(P.Vinv64 + 6 * (size_t)l, P.gl + 3 * (size_t)l, u);
+            sU[3 * tid] = u[0];
+            sU[3 * tid + 1] = u[1];
+            sU[3 * tid + 2] = u[2];
+        }
+    }
+    if (OP != OP_BACKSUB) {
+        for (int s = tid; s < n; s += kBlock) {
+            const int k = P.tile_perm[e0 + s];
+            sPerm[s] = (uint16_t)k;
+            sKey[s] = P.l_pose[e0 + k];
+        }
+    }
+    __syncthreads();
+
+    // ---- stage 1: t_e = W_e^T d_p per edge (and the FP32 G staged) ----
+    if (OP == OP_APPLY || OP == OP_BACKSUB || kStageG) {
+        for (int k = tid; k < n; k += kBlock) {
+            const int e = e0 + k;
+            T G[6];
+            load6(Gl + 6 * (size_t)e, G);
+            if (kStageG)
+#pragma unroll
+                for (int i = 0; i < 6; ++i) sG[6 * k + i] = G[i];
+            if (OP == OP_APPLY || OP == OP_BACKSUB) {
+                const int j = sLm[k];
+                const T X[3] = {sX[3 * j], sX[3 * j + 1], sX[3 * j + 2]};
+                T t[3];
+                edge_wtd(G, v1, v2, beta, P.l_pose[e], X, t);
+                sT[3 * k] = t[0];
+                sT[3 * k + 1] = t[1];
+                sT[3 * k + 2] = t[2];
+            }
+        }
+        __syncthreads();
+    }
+
+    // ---- stage 2: per landmark, in edge order ----
+    if (OP == OP_APPLY || OP == OP_BACKSUB) {
+        double step = 0.0;
+        if (tid < nl) {
+            const int l = lt.lm_begin + tid;
+            const int b = P.lm_edge_ptr[l] - e0, e = P.lm_edge_ptr[l + 1] - e0;
+            T t[3] = {T(0), T(0), T(0)};
+            for (int k = b; k < e; ++k) {
+                t[0] += sT[3 * k];
+                t[1] += sT[3 * k + 1];
+                t[2] += sT[3 * k + 2];
+            }
+            if (OP == OP_APPLY) {
+                T vi[6], u[3];
+                load6(Vinv + 6 * (size_t)l, vi);
+                sym3_mul(vi, t, u);
+                sU[3 * tid] = u[0];
+                sU[3 * tid + 1] = u[1];
+                sU[3 * tid + 2] = u[2];
+            } else {
+                // delta_l = -V^-1 (g_l + sum_e W_e^T x_p) (normal_equations.rs:1551-1592)
+                double dl[3] = {0.0, 0.0, 0.0};
+                if (P.lm_var[l] >= 0) {
+                    const double tt[3] = {P.gl[3 * (size_t)l] + (double)t[0], P.gl[3 * (size_t)l + 1] + (double)t[1],
+                                          P.gl[3 * (size_t)l + 2] + (double)t[2]};
+                    sym3_mul(P.Vinv64 + 6 * (size_t)l, tt, dl);
+                    dl[0] = -dl[0];
+                    dl[1] = -dl[1];
+                    dl[2] = -dl[2];
+                    step = sqrt(dl[0] * dl[0] + dl[1] * dl[1] + dl[2] * dl[2]);
+                }
+                for (int i = 0; i < 3; ++i) {
+                    P.delta_l[3 * (size_t)l + i] = dl[i];
+                    P.lms_trial[3 * (size_t)l + i] = P.lms[3 * (size_t)l + i] + dl[i];
+                }
+            }
+        }
+        if (OP == OP_BACKSUB) {
+            step = block_max<kBlock>(step, sRed);
+            if (tid == 0) P.tile_scalar[4 * (size_t)(P.num_cam_tiles + tile) + 3] = step;
+            return;
+        }
+        __syncthreads();
+    }
+
+    // ---- stage 3: y_e per sorted position, summed per camera run ----
+    const int lane = tid & 63, wave = tid >> 6;
+    const int nch = (n + 63) >> 6;
+    for (int c = wave; c < nch; c += kWaves) {
+        const int s = 64 * c + lane;
+        const bool start = s < n && (s == 0 || sKey[s - 1] != sKey[s]);
+        const uint64_t m = __ballot(start);
+        if (lane == 0) {
+            sStarts[c] = __popcll(m);
+            sEndRun[c] = -1;
+        }
+    }
+    __syncthreads();
+    for (int c = wave; c < nch; c += kWaves) {
+        const int s = 64 * c + lane;
+        const bool valid = s < n;
+        const int key = valid ? sKey[s] : -1 - lane;
+        const bool start = valid && (s == 0 || sKey[s - 1] != key);
+        const uint64_t m = __ballot(start);
+        int before = 0;
+        for (int c2 = 0; c2 < c; ++c2) before += sStarts[c2];
+        const uint64_t le = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+        const int run = run0 + before + __popcll(m & le) - 1;
+        T y[NV];
+        if (valid) {
+            const int k = sPerm[s];
+            const int j = sLm[k];
+            T G[6];
+            if (kStageG) {
+#pragma unroll
+                for (int i = 0; i < 6; ++i) G[i] = sG[6 * k + i];
+            } else {
+                load6(Gl + 6 * (size_t)(e0 + k), G);
+            }
+            const T X[3] = {sX[3 * j], sX[3 * j + 1], sX[3 * j + 2]};
+            if constexpr (OP == OP_PRECOND) {
+                edge_wvw(G, reinterpret_cast<const double*>(sT) + 6 * j, X, y);
+            } else {
+                const T u[3] = {sU[3 * j], sU[3 * j + 1], sU[3 * j + 2]};
+                edge_wu(G, u, X, y);
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < NV; ++i) y[i] = T(0);
+        }
+        // inclusive segmented scan over the wave (equal keys are contiguous)
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int ko = __shfl_up(key, off, 64);
+            const bool take = lane >= off && ko == key;
+#pragma unroll
+            for (int i = 0; i < NV; ++i) {
+                const T yo = __shfl_up(y[i], off, 64);
+                if (take) y[i] += yo;
+            }
+        }
+        int nk = __shfl_down(key, 1, 64);
+        if (lane == 63) nk = s + 1 < n ? sKey[s + 1] : -1000;
+        const bool is_end = valid && nk != key;
+        const bool cont = c > 0 && sKey[64 * c - 1] == sKey[64 * c];
+        const bool from_before = cont && key == sKey[64 * c];
+        if (is_end) {
+            if (!from_before) {
+#pragma unroll
+                for (int i = 0; i < NV; ++i) part[(size_t)run * NV + i] = y[i];
+            } else {
+#pragma unroll
+                for (int i = 0; i < NV; ++i) sEnd[c * NV + i] = y[i];
+                sEndRun[c] = run;
+            }
+        }
+        if (lane == 63 && valid && !is_end) {
+#pragma unroll
+            for (int i = 0; i < NV; ++i) sHead[c * NV + i] = y[i];
+        }
+    }
+    __syncthreads();
+    // runs that began in an earlier chunk: chunk tails in chunk order
+    if (tid < nch && sEndRun[tid] >= 0) {
+        const int c = tid;
+        int cs = c - 1;
+        while (cs > 0 && sStarts[cs] == 0) --cs;
+        T acc[NV];
+#pragma unroll
+        for (int i = 0; i < NV; ++i) acc[i] = sHead[cs * NV + i];
+        for (int c2 = cs + 1; c2 < c; ++c2)
+#pragma unroll
+            for (int i = 0; i < NV; ++i) acc[i] += sHead[c2 * NV + i];
+#pragma unroll
+        for (int i = 0; i < NV; ++i) part[(size_t)sEndRun[c] * NV + i] = acc[i] + sEnd[c * NV + i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Pose side: one wave per pose sums its runs' partials (lanes stride the
+// runs, then a butterfly that leaves the same sum in every lane).
+// ---------------------------------------------------------------------------
+template <typename T, int NV>
+__device__ __forceinline__ void pose_run_sum(const DevProblem& P, const T* __restrict__ part, int p, int lane,
+                                             double s[NV]) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) s[i] = 0.0;
+    for (int k = P.pose_run_ptr[p] + lane; k < P.pose_run_ptr[p + 1]; k += 64) {
+        const T* q = part + (size_t)P.pose_runs[k] * NV;
+#pragma unroll
+        for (int i = 0; i < NV; ++i) s[i] += (double)q[i];
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int i = 0; i < NV; ++i) s[i] += __shfl_xor(s[i], off, 64);
+}
+
+// Sum of one value per wave into a per-work-group partial, in wave order.
+__device__ __forceinline__ void wave_values_to_partial(const double v[2], double* lds, double* out) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        lds[2 * wave] = v[0];
+        lds[2 * wave + 1] = v[1];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = lds[0], b = lds[1];
+        for (int w = 1; w < kBlock / 64; ++w) {
+            a += lds[2 * w];
+            b += lds[2 * w + 1];
+        }
+        out[0] = a;
+        out[1] = b;
+    }
+}
+
+// q_c = (sum_j U_cj d_j) + lambda d_c - s_c on lane c < 6, d_j shuffled in.
+__device__ __forceinline__ double pose_apply_u(const DevProblem& P, int p, int c, double dc, double lambda,
+                                               double sc) {
+    double dv[6];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) dv[j] = __shfl(dc, j, 64);
+    const double* U = P.U + 21 * (size_t)p;
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) acc += U[c <= j ? up6(c, j) : up6(j, c)] * dv[j];
+    return acc + lambda * dc - sc;
+}
+
+// z_c = sum_j Minv_cj r_j on lane c < 6 (r_j shuffled from lanes base + j).
+__device__ __forceinline__ double pose_minv(const DevProblem& P, int p, int c, int base, double rc, bool valid) {
+    double z = 0.0;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        const double rj = __shfl(rc, min(base + j, 63), 64);
+        if (valid) z += P.Minv[36 * (size_t)p + 6 * c + j] * rj;
+    }
+    return z;
+}
+
+// b = -g_p + sum_e W_e V^-1 g_l (implicit_schur.rs:181-231).
+__global__ __launch_bounds__(kBlock) void k_rhs_pose(DevProblem P) {
+    const int lane = threadIdx.x & 63;
+    const int p = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    if (p >= P.num_pose_slots) return;
+    if (P.wstate && !P.wstate[P.pose_win[p]].active) return;
+    if (P.pose_var[p] < 0) {
+        if (lane < 6) P.b[6 * (size_t)p + lane] = 0.0;
+        return;
+    }
+    double s[6];
+    pose_run_sum<double, 6>(P, P.part64, p, lane, s);
+    if (lane < 6) P.b[6 * (size_t)p + lane] = s[lane] - P.g[6 * (size_t)p + lane];
+}
+
+// Block-Jacobi inverse of U + lambda I - sum W V^-1 W^T (implicit_schur.rs:241-287).
+__global__ __launch_bounds__(kBlock) void k_precond_pose(DevProblem P, double lambda) {
+    const int lane = threadIdx.x & 63;
+    const int p = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    if (p >= P.num_pose_slots) return;
+    double* out = P.Minv + 36 * (size_t)p;
+    if (P.pose_var[p] < 0) {
+        if (lane < 36) out[lane] = 0.0;
+        return;
+    }
+    double s[21];
+    pose_run_sum<double, 21>(P, P.part21, p, lane, s);
+    if (lane != 0) return;
+    double M[36];
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j) {
+            const int u = i <= j ? up6(i, j) : up6(j, i);
+            M[6 * i + j] = P.U[21 * (size_t)p + u] - s[u];
+        }
+    if (lambda > 0.0)
+        for (int k = 0; k < 6; ++k) M[7 * k] += lambda;
+    double inv[36];
+    if (!cholesky_inverse6(M, inv)) {
+        P.scalars->precond_failed = 1;
+        for (int i = 0; i < 36; ++i) out[i] = 0.0;
+        return;
+    }
+    for (int i = 0; i < 36; ++i) out[i] = inv[i];
+}
+
+// PCG start: r = b, z = M^-1 b, x = d = 0, tolerance and the first scalars.
+__global__ __launch_bounds__(kSmallBlock) void k_pcg_init(DevProblem P, double rtol, double atol, int max_iter,
+                                                          double lambda) {
+    __shared__ double lds[(kSmallBlock / 64) * 2];
+    double acc[2] = {0.0, 0.0};  // r.z, b.b
+    for (int p = threadIdx.x; p < P.num_pose_slots; p += kSmallBlock) {
+        double r[6], z[6];
+        const bool var = P.pose_var[p] >= 0;
+        for (int i = 0; i < 6; ++i) r[i] = var ? P.b[6 * (size_t)p + i] : 0.0;
+        for (int i = 0; i < 6; ++i) {
+            double sz = 0.0;
+            if (var)
+                for (int j = 0; j < 6; ++j) sz += P.Minv[36 * (size_t)p + 6 * i + j] * r[j];
+            z[i] = sz;
+        }
+        for (int i = 0; i < 6; ++i) {
+            const size_t k = 6 * (size_t)p + i;
+            P.x[k] = 0.0;
+            P.r[k] = r[i];
+            P.z[k] = z[i];
+            P.d[k] = 0.0;
+            P.q[k] = 0.0;
+            if (P.z32) {
+                P.z32[k] = (float)z[i];
+                P.d32[k] = 0.f;
+            }
+            acc[0] += r[i] * z[i];
+            acc[1] += r[i] * r[i];
+        }
+    }
+    block_sum<kSmallBlock, double, 2>(acc, lds);
+    for (int i = 1 + threadIdx.x; i < P.num_cpart; i += kSmallBlock) {
+        P.cpart[2 * (size_t)i] = 0.0;
+        P.cpart[2 * (size_t)i + 1] = 0.0;
+    }
+    if (threadIdx.x == 0) {
+        PcgState s;
+        s.rhs_norm = sqrt(acc[1]);
+        s.target = fmax(atol, rtol * s.rhs_norm);
+        s.rnorm = s.rhs_norm;
+        s.true_norm = 0.0;
+        s.lambda = lambda;
+        s.beta = 0.0;
+        s.rho[0] = s.rho[1] = acc[0];
+        s.iter = 0;
+        s.max_iter = max_iter;
+        if (P.scalars->precond_failed) s.status = PCG_NONPOSITIVE;  // NonSpdPreconditioner
+        else if (!isfinite(s.rhs_norm) || !isfinite(s.target)) s.status = PCG_NONFINITE;
+        else if (s.rhs_norm <= s.target) s.status = PCG_CONVERGED;
+        else if (max_iter == 0) s.status = PCG_MAXIT;
+        else if (!isfinite(acc[0]) || !(acc[0] > 0.0)) s.status = PCG_NONPOSITIVE;
+        else s.status = PCG_RUNNING;
+        s.stop = s.status != PCG_RUNNING;
+        *P.pcg = s;
+        P.cpart[0] = acc[0];
+        P.cpart[1] = acc[1];
+    }
+}
+
+// d = z + beta d and q = S d = (U + lambda I) d - sum of the pose's runs;
+// per work-group d.q partials.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pcg_pose(DevProblem P, const T* __restrict__ part) {
+    __shared__ int sFlag;
+    __shared__ double lds[(kBlock / 64) * 2];
+    PcgState* st = P.pcg;
+    if (block_flag(&st->stop, &sFlag)) return;
+    const double beta = st->beta, lambda = st->lambda;
+    const int lane = threadIdx.x & 63;
+    const int p = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    double v[2] = {0.0, 0.0};
+    if (p < P.num_pose_slots && P.pose_var[p] >= 0) {
+        double s[6];
+        pose_run_sum<T, 6>(P, part, p, lane, s);
+        const int c = lane < 6 ? lane : 0;
+        const size_t k = 6 * (size_t)p + c;
+        const double dc = P.z[k] + beta * P.d[k];
+        const double q = pose_apply_u(P, p, c, dc, lambda, s[c]);
+        if (lane < 6) {
+            P.d[k] = dc;
+            if (P.d32) P.d32[k] = (float)dc;
+            P.q[k] = q;
+            v[0] = dc * q;
+        }
+        v[0] = wave_sum(v[0]);
+    }
+    wave_values_to_partial(v, lds, P.bpart + 2 * (size_t)blockIdx.x);
+}
+
+// alpha = rho / d.q, x += alpha d, r -= alpha q, z = M^-1 r; per
+// work-group (r.z, r.r) partials. Lanes 6i..6i+5 of a wave hold pose i.
+__global__ __launch_bounds__(kBlock) void k_pcg_step(DevProblem P, int k) {
+    __shared__ int sFlag;
+    __shared__ double lds[(kBlock / 64) * 2];
+    PcgState* st = P.pcg;
+    if (block_flag(&st->stop, &sFlag)) return;
+    double cv[1] = {0.0};
+    for (int i = threadIdx.x; i < P.num_bpart; i += kBlock) cv[0] += P.bpart[2 * (size_t)i];
+    block_sum_all<kBlock, double, 1>(cv, lds);
+    const double curvature = cv[0];
+    if (!isfinite(curvature) || !(curvature > 0.0)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) set_stop(st, PCG_NONPOSITIVE);
+        return;
+    }
+    const double alpha = st->rho[k & 1] / curvature;
+    if (!isfinite(alpha)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) set_stop(st, PCG_NONFINITE);
+        return;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int pp = lane / 6, c = lane % 6;
+    const int p = (blockIdx.x * (kBlock / 64) + wave) * kPosesPerWave + pp;
+    const bool valid = lane < 6 * kPosesPerWave && p < P.num_pose_slots && P.pose_var[p] >= 0;
+    const size_t i = 6 * (size_t)p + c;
+    double rc = 0.0;
+    if (valid) {
+        P.x[i] = P.x[i] + alpha * P.d[i];
+        rc = P.r[i] - alpha * P.q[i];
+        P.r[i] = rc;
+    }
+    const double zc = pose_minv(P, p, c, 6 * pp, rc, valid);
+    double v[2] = {0.0, 0.0};
+    if (valid) {
+        P.z[i] = zc;
+        if (P.z32) P.z32[i] = (float)zc;
+        v[0] = rc * zc;
+        v[1] = rc * rc;
+    }
+    v[0] = wave_sum(v[0]);
+    v[1] = wave_sum(v[1]);
+    wave_values_to_partial(v, lds, P.cpart + 2 * (size_t)blockIdx.x);
+}
+
+__global__ __launch_bounds__(kBlock) void k_pcg_status(DevProblem P, int k, int fresh) {
+    __shared__ double lds[(kBlock / 64) * 2];
+    __shared__ int sFlag;
+    double beta;
+    (void)pcg_prologue(P, k, fresh, &beta, lds, &sFlag);
+}
+
+// True residual r_t = b - S x (part64 holds the tile pass run on x); with
+// restart the recursion continues from it: r = r_t, z = M^-1 r_t, d = 0
+// (implicit_schur.rs:613-690, 750-761). Partials (r_t.r_t, r_t.z).
+__global__ __launch_bounds__(kBlock) void k_pcg_true_pose(DevProblem P, int restart) {
+    __shared__ double lds[(kBlock / 64) * 2];
+    const double lambda = P.pcg->lambda;
+    const int lane = threadIdx.x & 63;
+    const int p = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    double v[2] = {0.0, 0.0};
+    if (p < P.num_pose_slots && P.pose_var[p] >= 0) {
+        double s[6];
+        pose_run_sum<double, 6>(P, P.part64, p, lane, s);
+        const int c = lane < 6 ? lane : 0;
+        const size_t k = 6 * (size_t)p + c;
+        const double xc = P.x[k];
+        const double rt = P.b[k] - pose_apply_u(P, p, c, xc, lambda, s[c]);
+        double zc = 0.0;
+        if (restart) zc = pose_minv(P, p, c, 0, rt, lane < 6);
+        if (lane < 6) {
+            v[0] = rt * rt;
+            if (restart) {
+                P.r[k] = rt;
+                P.z[k] = zc;
+                P.d[k] = 0.0;
+                if (P.z32) {
+                    P.z32[k] = (float)zc;
+                    P.d32[k] = 0.f;
+                }
+                v[1] = rt * zc;
+            }
+        }
+        v[0] = wave_sum(v[0]);
+        v[1] = wave_sum(v[1]);
+    }
+    wave_values_to_partial(v, lds, P.bpart + 2 * (size_t)blockIdx.x);
+}
+
+__global__ __launch_bounds__(kBlock) void k_pcg_true_finish(DevProblem P, int restart) {
+    __shared__ double lds[(kBlock / 64) * 2];
+    double v[2] = {0.0, 0.0};
+    for (int i = threadIdx.x; i < P.num_bpart; i += kBlock) {
+        v[0] += P.bpart[2 * (size_t)i];
+        v[1] += P.bpart[2 * (size_t)i + 1];
+    }
+    block_sum<kBlock, double, 2>(v, lds);
+    for (int i = 1 + threadIdx.x; i < P.num_cpart && restart; i += kBlock) {
+        P.cpart[2 * (size_t)i] = 0.0;
+        P.cpart[2 * (size_t)i + 1] = 0.0;
+    }
+    if (threadIdx.x == 0) {
+        PcgState* st = P.pcg;
+        st->true_norm = sqrt(v[0]);
+        if (restart) {
+            st->rnorm = st->true_norm;
+            P.cpart[0] = v[1];
+            P.cpart[1] = v[0];
+            int status = PCG_RUNNING;
+            if (!isfinite(v[1]) || !(v[1] > 0.0)) status = PCG_NONPOSITIVE;
+            else if (st->iter >= st->max_iter) status = PCG_MAXIT;
+            st->status = status;
+            st->stop = status != PCG_RUNNING;
+        }
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+static inline int gather_blocks(const DevProblem& P) { return blocks_for(P.num_pose_slots, kBlock / 64); }
+
+void launch_precondition(const DevProblem& P, double lambda, hipStream_t s) {
+    hipLaunchKernelGGL((k_schur_tile<double, OP_PRECOND>), dim3(xcd_grid(P.num_lm_tiles)), dim3(kBlock), 0, s, P,
+                       (const double*)nullptr, (const double*)nullptr, (const double*)P.Gl64,
+                       (const double*)P.Vinv64, P.part21, -1, 0);
+    hipLaunchKernelGGL(k_precond_pose, dim3(gather_blocks(P)), dim3(kBlock), 0, s, P, lambda);
+}
+
+void launch_rhs(const DevProblem& P, hipStream_t s) {
+    hipLaunchKernelGGL((k_schur_tile<double, OP_RHS>), dim3(xcd_grid(P.num_lm_tiles)), dim3(kBlock), 0, s, P,
+                       (const double*)nullptr, (const double*)nullptr, (const double*)P.Gl64,
+                       (const double*)P.Vinv64, P.part64, -1, 0);
+    hipLaunchKernelGGL(k_rhs_pose, dim3(gather_blocks(P)), dim3(kBlock), 0, s, P);
+}
+
+void launch_schur_apply(const DevProblem& P, bool fp32, const void* v1, const void* v2, int32_t pcg_iter,
+                        int32_t fresh, hipStream_t s) {
+    if (fp32)
+        hipLaunchKernelGGL((k_schur_tile<float, OP_APPLY>), dim3(xcd_grid(P.num_lm_tiles)), dim3(kBlock), 0, s, P,
+                           (const float*)v1, (const float*)v2, (const float*)P.Gl32, (const float*)P.Vinv32,
+                           P.part32, pcg_iter, fresh);
+    else
+        hipLaunchKernelGGL((k_schur_tile<double, OP_APPLY>), dim3(xcd_grid(P.num_lm_tiles)), dim3(kBlock), 0, s, P,
+                           (const double*)v1, (const double*)v2, (const double*)P.Gl64, (const double*)P.Vinv64,
+                           P.part64, pcg_iter, fresh);
+}
+
+void launch_pcg_init(const DevProblem& P, double rtol, double atol, int32_t max_iter, double lambda,
+                     hipStream_t s) {
+    hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(kSmallBlock), 0, s, P, rtol, atol, max_iter, lambda);
+}
+
+void launch_pcg_pose(const DevProblem& P, bool fp32, hipStream_t s) {
+    if (fp32)
+        hipLaunchKernelGGL(k_pcg_pose<float>, dim3(gather_blocks(P)), dim3(kBlock), 0, s, P, (const float*)P.part32);
+    else
+        hipLaunchKernelGGL(k_pcg_pose<double>, dim3(gather_blocks(P)), dim3(kBlock), 0, s, P,
+                           (const double*)P.part64);
+}
+
+void launch_pcg_step(const DevProblem& P, bool fp32, int32_t pcg_iter, hipStream_t s) {
+    (void)fp32;
+    hipLaunchKernelGGL(k_pcg_step, dim3(blocks_for(P.num_pose_slots, (kBlock / 64) * kPosesPerWave)), dim3(kBlock),
+                       0, s, P, pcg_iter);
+}
+
+void launch_pcg_status(const DevProblem& P, int32_t pcg_iter, int32_t fresh, hipStream_t s) {
+    hipLaunchKernelGGL(k_pcg_status, dim3(1), dim3(kBlock), 0, s, P, pcg_iter, fresh);
+}
+
+void launch_pcg_true_residual(const DevProblem& P, hipStream_t s) {
+    launch_schur_apply(P, false, P.x, nullptr, -1, 0, s);
+    hipLaunchKernelGGL(k_pcg_true_pose, dim3(gather_blocks(P)), dim3(kBlock), 0, s, P, 0);
+    hipLaunchKernelGGL(k_pcg_true_finish, dim3(1), dim3(kBlock), 0, s, P, 0);
+}
+
+void launch_pcg_restart(const DevProblem& P, hipStream_t s) {
+    hipLaunchKernelGGL(k_pcg_true_pose, dim3(gather_blocks(P)), dim3(kBlock), 0, s, P, 1);
+    hipLaunchKernelGGL(k_pcg_true_finish, dim3(1), dim3(kBlock), 0, s, P, 1);
+}
+
+void launch_backsub(const DevProblem& P, hipStream_t s) {
+    hipLaunchKernelGGL((k_schur_tile<double, OP_BACKSUB>), dim3(xcd_grid(P.num_lm_tiles)), dim3(kBlock), 0, s, P,
+                       (const double*)P.x, (const double*)nullptr, (const double*)P.Gl64, (const double*)P.Vinv64,
+                       (double*)nullptr, -1, 0);
+}
+
+}  // namespace vba

@@ -23,16 +23,16 @@ namespace {
 enum BufId {
     B_POSES0, B_LMS0, B_POSES, B_LMS, B_POSES_T, B_LMS_T, B_SENSORS, B_POSE_VAR, B_LM_VAR, B_OBS_UV,
     B_OBS_SENSOR, B_EDGE_LM, B_EDGE_OBS, B_PERM_CL, B_L_POSE, B_LM_EDGE_PTR, B_CAM_TILES,
-    B_CAM_TILE_PTR, B_LM_TILES, B_EC64, B_EL64, B_HL64, B_EC32, B_EL32, B_TILE27, B_U, B_G, B_V, B_GL,
-    B_VINV64, B_VINV32, B_PACK64, B_PACK32, B_MINV, B_VB, B_VX, B_VR, B_VZ, B_VD, B_VQ, B_VD32,
-    B_PART64, B_PART32, B_PCG, B_DELTA_L, B_TILE_SCALAR, B_SCALARS, B_DENSE_BLOCK_PQ,
-    B_DENSE_BLOCK_PTR, B_DENSE_TERMS, B_DENSE_VAR_SLOT, B_DENSE_S, B_DENSE_WORK, B_DENSE_STATUS,
-    B_COUNT
+    B_CAM_TILE_PTR, B_LM_TILES, B_TILE_PERM, B_TILE_RUN_BASE, B_POSE_RUN_PTR, B_POSE_RUNS, B_GL64, B_GL32,
+    B_HL64, B_TILE27, B_U, B_G, B_V, B_GL, B_VINV64, B_VINV32, B_X32, B_MINV, B_VB, B_VX, B_VR, B_VZ, B_VD,
+    B_VQ, B_VZ32, B_VD32, B_PART64, B_PART32, B_PART21, B_BPART, B_CPART, B_PCG, B_DELTA_L, B_TILE_SCALAR,
+    B_SCALARS, B_DENSE_BLOCK_PQ, B_DENSE_BLOCK_PTR, B_DENSE_TERMS, B_DENSE_VAR_SLOT, B_DENSE_S,
+    B_DENSE_WORK, B_DENSE_STATUS, B_COUNT
 };
 
 // Profiled kernel families.
-enum ProfId { K_LINEARIZE, K_LM_PASS, K_CAM_PASS, K_PCG_UPDATE, K_COST, K_COUNT };
-const char* kProfNames[K_COUNT] = {"linearize", "lm_pass", "cam_pass", "pcg_update", "cost"};
+enum ProfId { K_LINEARIZE, K_SCHUR_APPLY, K_PCG_POSE, K_PCG_STEP, K_COST, K_COUNT };
+const char* kProfNames[K_COUNT] = {"linearize", "schur_apply", "pcg_pose", "pcg_step", "cost"};
 
 struct Fail {
     int32_t code;
@@ -323,6 +323,10 @@ int32_t prepare(Context& ctx, Prepared& pr, std::string* msg) {
         upload(pr, B_CAM_TILES, m.cam_tiles, s);
         upload(pr, B_CAM_TILE_PTR, m.cam_tile_ptr, s);
         upload(pr, B_LM_TILES, m.lm_tiles, s);
+        upload(pr, B_TILE_PERM, m.tile_perm, s);
+        upload(pr, B_TILE_RUN_BASE, m.tile_run_base, s);
+        upload(pr, B_POSE_RUN_PTR, m.pose_run_ptr, s);
+        upload(pr, B_POSE_RUNS, m.pose_runs, s);
         hip_check(hipStreamSynchronize(s), "sync");
         pr.final_slot = -1;
         return VBA_OK;
@@ -358,6 +362,95 @@ int32_t download_state(Context& ctx, const Prepared& pr, double* poses_out, doub
     }
 }
 
+namespace {
+
+// Device pointers of a prepared problem for one solve: structure, the state
+// reset to the prepared initial state, and working buffers (allocated on
+// first use, kept for later solves).
+DevProblem bind(Prepared& pr, const vba_options& o, bool mixed, bool matrix_free, hipStream_t s) {
+    const Model& m = pr.m;
+    DevProblem P{};
+    P.num_pose_slots = (int32_t)m.num_pose_slots;
+    P.num_lm_slots = (int32_t)m.num_lm_slots;
+    P.num_edges = (int32_t)m.num_edges;
+    P.num_obs = (int32_t)m.num_obs;
+    P.num_cam_tiles = (int32_t)m.cam_tiles.size();
+    P.num_lm_tiles = (int32_t)m.lm_tiles.size();
+    P.any_rig = m.any_rig;
+    P.kernel_kind = (int32_t)o.robust_kernel;
+    P.kernel_scale = o.robust_scale;
+    const size_t NP = m.num_pose_slots, NL = m.num_lm_slots, NE = m.num_edges, NT = m.cam_tiles.size();
+    const size_t NLT = m.lm_tiles.size(), NR = std::max<uint64_t>(m.num_runs, 1);
+    const size_t pose_blocks = (NP + kBlock - 1) / kBlock;
+    P.poses = buf<double>(pr, B_POSES);
+    P.lms = buf<double>(pr, B_LMS);
+    P.poses_trial = buf<double>(pr, B_POSES_T);
+    P.lms_trial = buf<double>(pr, B_LMS_T);
+    hip_check(hipMemcpyAsync(P.poses, pr.bufs[B_POSES0].ptr, m.poses.size() * sizeof(double), hipMemcpyDeviceToDevice,
+                             s), "D2D");
+    hip_check(hipMemcpyAsync(P.lms, pr.bufs[B_LMS0].ptr, m.lms.size() * sizeof(double), hipMemcpyDeviceToDevice, s),
+              "D2D");
+    P.sensors = buf<DevSensor>(pr, B_SENSORS);
+    P.pose_var = buf<int32_t>(pr, B_POSE_VAR);
+    P.lm_var = buf<int32_t>(pr, B_LM_VAR);
+    P.obs_uv = buf<double>(pr, B_OBS_UV);
+    P.obs_sensor = m.any_rig ? buf<uint16_t>(pr, B_OBS_SENSOR) : nullptr;
+    P.edge_lm = buf<int32_t>(pr, B_EDGE_LM);
+    P.edge_obs = buf<int32_t>(pr, B_EDGE_OBS);
+    P.perm_cl = buf<int32_t>(pr, B_PERM_CL);
+    P.l_pose = buf<int32_t>(pr, B_L_POSE);
+    P.lm_edge_ptr = buf<int32_t>(pr, B_LM_EDGE_PTR);
+    P.cam_tiles = buf<CamTile>(pr, B_CAM_TILES);
+    P.cam_tile_ptr = buf<int32_t>(pr, B_CAM_TILE_PTR);
+    P.lm_tiles = buf<LmTile>(pr, B_LM_TILES);
+    P.tile_perm = buf<uint16_t>(pr, B_TILE_PERM);
+    P.tile_run_base = buf<int32_t>(pr, B_TILE_RUN_BASE);
+    P.pose_run_ptr = buf<int32_t>(pr, B_POSE_RUN_PTR);
+    P.pose_runs = buf<int32_t>(pr, B_POSE_RUNS);
+    P.Gl64 = (double*)ensure(pr, B_GL64, 6 * NE * sizeof(double));
+    P.hl64 = (double*)ensure(pr, B_HL64, 3 * NE * sizeof(double));
+    if (mixed) {
+        P.Gl32 = (float*)ensure(pr, B_GL32, 6 * NE * sizeof(float));
+        P.Vinv32 = (float*)ensure(pr, B_VINV32, 6 * NL * sizeof(float));
+        P.X32 = (float4*)ensure(pr, B_X32, NL * sizeof(float4));
+        P.part32 = (float*)ensure(pr, B_PART32, 6 * NR * sizeof(float));
+        P.z32 = (float*)ensure(pr, B_VZ32, 6 * NP * sizeof(float));
+        P.d32 = (float*)ensure(pr, B_VD32, 6 * NP * sizeof(float));
+    }
+    P.tile27 = (double*)ensure(pr, B_TILE27, 27 * std::max<size_t>(NT, 1) * sizeof(double));
+    P.U = (double*)ensure(pr, B_U, 21 * NP * sizeof(double));
+    P.g = (double*)ensure(pr, B_G, 6 * NP * sizeof(double));
+    P.V = (double*)ensure(pr, B_V, 6 * NL * sizeof(double));
+    P.gl = (double*)ensure(pr, B_GL, 3 * NL * sizeof(double));
+    P.Vinv64 = (double*)ensure(pr, B_VINV64, 6 * NL * sizeof(double));
+    P.part64 = (double*)ensure(pr, B_PART64, 6 * NR * sizeof(double));
+    P.b = (double*)ensure(pr, B_VB, 6 * NP * sizeof(double));
+    P.x = (double*)ensure(pr, B_VX, 6 * NP * sizeof(double));
+    if (matrix_free) {
+        P.Minv = (double*)ensure(pr, B_MINV, 36 * NP * sizeof(double));
+        P.part21 = (double*)ensure(pr, B_PART21, 21 * NR * sizeof(double));
+        P.r = (double*)ensure(pr, B_VR, 6 * NP * sizeof(double));
+        P.z = (double*)ensure(pr, B_VZ, 6 * NP * sizeof(double));
+        P.d = (double*)ensure(pr, B_VD, 6 * NP * sizeof(double));
+        P.q = (double*)ensure(pr, B_VQ, 6 * NP * sizeof(double));
+        P.num_bpart = (int32_t)((NP + (kBlock / 64) - 1) / (kBlock / 64));
+        P.num_cpart = (int32_t)((NP + (kBlock / 64) * kPosesPerWave - 1) / ((kBlock / 64) * kPosesPerWave));
+        P.bpart = (double*)ensure(pr, B_BPART, 2 * (size_t)std::max(P.num_bpart, 1) * sizeof(double));
+        P.cpart = (double*)ensure(pr, B_CPART, 2 * (size_t)std::max(P.num_cpart, 1) * sizeof(double));
+        P.pcg = (PcgState*)ensure(pr, B_PCG, sizeof(PcgState));
+    }
+    P.delta_l = (double*)ensure(pr, B_DELTA_L, 3 * NL * sizeof(double));
+    const size_t nscalar = 4 * (NT + NLT + pose_blocks + 1);
+    P.tile_scalar = (double*)ensure(pr, B_TILE_SCALAR, nscalar * sizeof(double));
+    P.scalars = (StepScalars*)ensure(pr, B_SCALARS, sizeof(StepScalars));
+    hip_check(hipMemsetAsync(P.x, 0, 6 * NP * sizeof(double), s), "memset");
+    hip_check(hipMemsetAsync(P.delta_l, 0, 3 * NL * sizeof(double), s), "memset");
+    hip_check(hipMemsetAsync(P.tile_scalar, 0, nscalar * sizeof(double), s), "memset");
+    return P;
+}
+
+}  // namespace
+
 int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stats* stats, uint32_t cap,
             vba_result* res) {
     hipStream_t s = ctx.stream;
@@ -368,74 +461,9 @@ int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stat
     try {
         if (!matrix_free && m.P > 1024)
             throw Fail{VBA_ERR_INVALID_ARGUMENT, "dense reduced solve limited to 1024 variable poses"};
-        // ---- bind device buffers (allocated on first use, kept for later solves) ----
-        DevProblem P{};
-        P.num_pose_slots = (int32_t)m.num_pose_slots;
-        P.num_lm_slots = (int32_t)m.num_lm_slots;
-        P.num_edges = (int32_t)m.num_edges;
-        P.num_obs = (int32_t)m.num_obs;
-        P.num_cam_tiles = (int32_t)m.cam_tiles.size();
-        P.num_lm_tiles = (int32_t)m.lm_tiles.size();
-        P.any_rig = m.any_rig;
-        P.kernel_kind = (int32_t)o.robust_kernel;
-        P.kernel_scale = o.robust_scale;
+        DevProblem P = bind(pr, o, mixed, matrix_free, s);
         const size_t NP = m.num_pose_slots, NL = m.num_lm_slots, NE = m.num_edges, NT = m.cam_tiles.size();
-        const size_t NLT = m.lm_tiles.size();
-        const size_t pose_blocks = (NP + kBlock - 1) / kBlock;
-        P.poses = buf<double>(pr, B_POSES);
-        P.lms = buf<double>(pr, B_LMS);
-        P.poses_trial = buf<double>(pr, B_POSES_T);
-        P.lms_trial = buf<double>(pr, B_LMS_T);
-        hip_check(hipMemcpyAsync(P.poses, pr.bufs[B_POSES0].ptr, m.poses.size() * sizeof(double),
-                                 hipMemcpyDeviceToDevice, s), "D2D");
-        hip_check(hipMemcpyAsync(P.lms, pr.bufs[B_LMS0].ptr, m.lms.size() * sizeof(double), hipMemcpyDeviceToDevice, s),
-                  "D2D");
-        P.sensors = buf<DevSensor>(pr, B_SENSORS);
-        P.pose_var = buf<int32_t>(pr, B_POSE_VAR);
-        P.lm_var = buf<int32_t>(pr, B_LM_VAR);
-        P.obs_uv = buf<double>(pr, B_OBS_UV);
-        P.obs_sensor = m.any_rig ? buf<uint16_t>(pr, B_OBS_SENSOR) : nullptr;
-        P.edge_lm = buf<int32_t>(pr, B_EDGE_LM);
-        P.edge_obs = buf<int32_t>(pr, B_EDGE_OBS);
-        P.perm_cl = buf<int32_t>(pr, B_PERM_CL);
-        P.l_pose = buf<int32_t>(pr, B_L_POSE);
-        P.lm_edge_ptr = buf<int32_t>(pr, B_LM_EDGE_PTR);
-        P.cam_tiles = buf<CamTile>(pr, B_CAM_TILES);
-        P.cam_tile_ptr = buf<int32_t>(pr, B_CAM_TILE_PTR);
-        P.lm_tiles = buf<LmTile>(pr, B_LM_TILES);
-        P.ec64 = (EdgeRec64*)ensure(pr, B_EC64, NE * sizeof(EdgeRec64));
-        P.el64 = (EdgeRec64*)ensure(pr, B_EL64, NE * sizeof(EdgeRec64));
-        P.hl64 = (double*)ensure(pr, B_HL64, 3 * NE * sizeof(double));
-        if (mixed) {
-            P.ec32 = (EdgeRec32*)ensure(pr, B_EC32, NE * sizeof(EdgeRec32));
-            P.el32 = (EdgeRec32*)ensure(pr, B_EL32, NE * sizeof(EdgeRec32));
-            P.Vinv32 = (float*)ensure(pr, B_VINV32, 6 * NL * sizeof(float));
-            P.pack32 = (LmPack<float>*)ensure(pr, B_PACK32, NL * sizeof(LmPack<float>));
-            P.part32 = (float*)ensure(pr, B_PART32, 6 * std::max<size_t>(NT, 1) * sizeof(float));
-        }
-        P.tile27 = (double*)ensure(pr, B_TILE27, 27 * std::max<size_t>(NT, 1) * sizeof(double));
-        P.U = (double*)ensure(pr, B_U, 21 * NP * sizeof(double));
-        P.g = (double*)ensure(pr, B_G, 6 * NP * sizeof(double));
-        P.V = (double*)ensure(pr, B_V, 6 * NL * sizeof(double));
-        P.gl = (double*)ensure(pr, B_GL, 3 * NL * sizeof(double));
-        P.Vinv64 = (double*)ensure(pr, B_VINV64, 6 * NL * sizeof(double));
-        P.pack64 = (LmPack<double>*)ensure(pr, B_PACK64, NL * sizeof(LmPack<double>));
-        P.Minv = (double*)ensure(pr, B_MINV, 36 * NP * sizeof(double));
-        P.b = (double*)ensure(pr, B_VB, 6 * NP * sizeof(double));
-        P.x = (double*)ensure(pr, B_VX, 6 * NP * sizeof(double));
-        P.r = (double*)ensure(pr, B_VR, 6 * NP * sizeof(double));
-        P.z = (double*)ensure(pr, B_VZ, 6 * NP * sizeof(double));
-        P.d = (double*)ensure(pr, B_VD, 6 * NP * sizeof(double));
-        P.q = (double*)ensure(pr, B_VQ, 6 * NP * sizeof(double));
-        P.d32 = (float*)ensure(pr, B_VD32, 6 * NP * sizeof(float));
-        P.part64 = (double*)ensure(pr, B_PART64, 6 * std::max<size_t>(NT, 1) * sizeof(double));
-        P.pcg = (PcgState*)ensure(pr, B_PCG, sizeof(PcgState));
-        P.delta_l = (double*)ensure(pr, B_DELTA_L, 3 * NL * sizeof(double));
-        const size_t nscalar = 4 * (NT + NLT + pose_blocks + 1);
-        P.tile_scalar = (double*)ensure(pr, B_TILE_SCALAR, nscalar * sizeof(double));
-        P.scalars = (StepScalars*)ensure(pr, B_SCALARS, sizeof(StepScalars));
-        hip_check(hipMemsetAsync(P.x, 0, 6 * NP * sizeof(double), s), "memset");
-        hip_check(hipMemsetAsync(P.tile_scalar, 0, nscalar * sizeof(double), s), "memset");
+        const size_t NR = m.num_runs;
         DensePairs D{};
         double* dS = nullptr;
         double* dwork = nullptr;
@@ -458,14 +486,15 @@ int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stat
             dwork = (double*)ensure(pr, B_DENSE_WORK, ((size_t)ndense * ndense + ndense) * sizeof(double));
             dstatus = (int32_t*)ensure(pr, B_DENSE_STATUS, sizeof(int32_t));
         }
-        // Algorithmic bytes per launch of the profiled kernels (include/vba.h,
-        // vba_profile_query).
-        const double rec = mixed ? (double)sizeof(EdgeRec32) : (double)sizeof(EdgeRec64);
+        // Algorithmic bytes per launch of the profiled kernels: what the data
+        // layout must move at least once (include/vba.h, vba_profile_query).
+        // Pose vectors gathered per edge are counted once per pose.
         const double real = mixed ? 4.0 : 8.0;
-        const double bytes_lm_pass = NE * rec + NL * (8.0 + 12.0 * real) + NP * 6.0 * real;
-        const double bytes_cam_pass = NE * rec + NL * 6.0 * real + NT * (20.0 + 6.0 * real);
-        const double bytes_linearize = m.num_obs * 16.0 + NE * (4.0 + 4.0 + 4.0 + 2.0 * sizeof(EdgeRec64) + 24.0 +
-                                                                 (mixed ? 2.0 * sizeof(EdgeRec32) : 0.0)) +
+        const double bytes_apply = NE * (6.0 * real + 4.0 + 2.0) + NL * ((mixed ? 16.0 : 24.0) + 6.0 * real + 4.0) +
+                                   NR * 6.0 * real + NP * 12.0 * real + m.lm_tiles.size() * 12.0;
+        const double bytes_pcg_pose = NR * (4.0 + 6.0 * real) + NP * (8.0 + 96.0 + 168.0 + 96.0 + (mixed ? 24.0 : 0.0));
+        const double bytes_pcg_step = NP * (4.0 * 48.0 + 288.0 + 3.0 * 48.0 + (mixed ? 24.0 : 0.0));
+        const double bytes_linearize = m.num_obs * 16.0 + NE * (4.0 + 4.0 + 4.0 + 48.0 + 24.0 + (mixed ? 24.0 : 0.0)) +
                                        NL * 24.0 + NT * (16.0 + 27.0 * 8.0) + NP * 56.0;
         const double bytes_cost = m.num_obs * 16.0 + NE * 8.0 + NL * 24.0 + NT * (16.0 + 24.0) + NP * 56.0;
 
@@ -502,7 +531,8 @@ int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stat
         double lambda = o.has_initial_lambda ? o.initial_lambda : 0.0;
         bool converged = false, need_lin = true;
         uint32_t written = 0, total_pcg = 0, linearizations = 0;
-        const uint32_t chunk = 10;
+        const int32_t chunk = 10;
+        const int32_t max_pcg = (int32_t)o.max_pcg_iterations;
         auto push = [&](const vba_iteration_stats& st) {
             if (stats && written < cap) stats[written] = st;
             ++written;
@@ -521,8 +551,7 @@ int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stat
             }
             hip_check(hipMemsetAsync(P.scalars, 0, sizeof(StepScalars), s), "memset");
             launch_landmark_prepare(P, lambda, s);
-            launch_cam_pass(P, false, nullptr, s);  // u = V^-1 g_l
-            launch_rhs_finalize(P, s);
+            launch_rhs(P, s);
 
             vba_iteration_stats st;
             std::memset(&st, 0, sizeof(st));
@@ -539,44 +568,48 @@ int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stat
                 solved = *ctx.host_status == 0;
             } else {
                 launch_precondition(P, lambda, s);
-                launch_pcg_init(P, o.pcg_relative_tolerance, o.pcg_absolute_tolerance, (int32_t)o.max_pcg_iterations,
-                                lambda, s);
-                PcgState ps = read_pcg();
-                const int32_t* flag = &P.pcg->status;
+                launch_pcg_init(P, o.pcg_relative_tolerance, o.pcg_absolute_tolerance, max_pcg, lambda, s);
+                // PCG iterations k .. in chunks; the device stops itself, the
+                // host reads the state once per chunk.
+                int32_t k = 0;
+                int32_t fresh = 1;
+                PcgState ps{};
                 while (true) {
-                    while (ps.status == PCG_RUNNING) {
-                        const int before = ps.iter;
+                    while (true) {
                         const size_t batch = ctx.pending.size();
-                        const int n = (int)std::min<uint32_t>(chunk, (uint32_t)std::max(0, ps.max_iter - ps.iter));
-                        for (int k = 0; k < n; ++k) {
+                        const int32_t k_end = std::min(k + chunk, max_pcg);
+                        for (int32_t kk = k; kk < k_end; ++kk) {
                             {
-                                ProfScope a(ctx, K_LM_PASS, bytes_lm_pass);
-                                launch_lm_pass(P, mixed, mixed ? (const void*)P.d32 : (const void*)P.d, flag, s);
+                                ProfScope a(ctx, K_SCHUR_APPLY, bytes_apply);
+                                launch_schur_apply(P, mixed, mixed ? (const void*)P.z32 : (const void*)P.z,
+                                                   mixed ? (const void*)P.d32 : (const void*)P.d, kk,
+                                                   fresh && kk == k, s);
                             }
                             {
-                                ProfScope b(ctx, K_CAM_PASS, bytes_cam_pass);
-                                launch_cam_pass(P, mixed, flag, s);
+                                ProfScope b(ctx, K_PCG_POSE, bytes_pcg_pose);
+                                launch_pcg_pose(P, mixed, s);
                             }
                             {
-                                ProfScope c(ctx, K_PCG_UPDATE, 0.0);
-                                launch_pcg_update(P, mixed, s);
+                                ProfScope c(ctx, K_PCG_STEP, bytes_pcg_step);
+                                launch_pcg_step(P, mixed, kk, s);
                             }
                         }
+                        launch_pcg_status(P, k_end, fresh && k_end == k, s);
                         ps = read_pcg();
-                        const int done = ps.iter - before;
-                        for (int k : {K_LM_PASS, K_CAM_PASS, K_PCG_UPDATE}) keep_first(ctx, batch, k, done);
+                        const int done = std::max(0, ps.iter - k);
+                        for (int kind : {K_SCHUR_APPLY, K_PCG_POSE, K_PCG_STEP}) keep_first(ctx, batch, kind, done);
                         harvest(ctx);
-                        if (ps.status == PCG_RUNNING && ps.iter <= before)
-                            throw Fail{VBA_ERR_DEVICE, "PCG made no progress"};
+                        if (ps.stop) break;
+                        if (ps.iter <= k) throw Fail{VBA_ERR_DEVICE, "PCG made no progress"};
+                        k = ps.iter;
+                        fresh = 0;
                     }
                     if (ps.status != PCG_CONVERGED && ps.status != PCG_MAXIT) {
                         solved = false;
                         break;
                     }
                     // true residual re-check (implicit_schur.rs:613-690, 750-761)
-                    launch_lm_pass(P, false, P.x, nullptr, s);
-                    launch_cam_pass(P, false, nullptr, s);
-                    launch_pcg_true_residual(P, false, s);
+                    launch_pcg_true_residual(P, s);
                     ps = read_pcg();
                     if (std::isfinite(ps.true_norm) && ps.true_norm <= ps.target) break;
                     if (!std::isfinite(ps.true_norm)) {
@@ -585,8 +618,14 @@ int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stat
                     }
                     if (mixed && ps.iter < ps.max_iter) {
                         // FP64 refinement: continue from the true residual
-                        launch_pcg_true_residual(P, true, s);
+                        launch_pcg_restart(P, s);
                         ps = read_pcg();
+                        if (ps.stop) {
+                            solved = ps.status == PCG_MAXIT && accept_iterate;
+                            break;
+                        }
+                        k = ps.iter;
+                        fresh = 1;
                         continue;
                     }
                     solved = accept_iterate;
@@ -709,7 +748,7 @@ void merge_models(const std::vector<const Model*>& ms, Model* out, std::vector<W
     cam_tile_win->clear();
     lm_tile_win->clear();
     *dense_out = DenseLists();
-    int64_t po = 0, lo = 0, eo = 0, oo = 0, to = 0, vo = 0, vlo = 0, so = 0, soff = 0, term_off = 0;
+    int64_t po = 0, lo = 0, eo = 0, oo = 0, to = 0, vo = 0, vlo = 0, so = 0, soff = 0, term_off = 0, ro = 0, rlo = 0;
     for (size_t w = 0; w < ms.size(); ++w) {
         const Model& m = *ms[w];
         WindowLayout L{};
@@ -758,6 +797,14 @@ void merge_models(const std::vector<const Model*>& ms, Model* out, std::vector<W
             M.lm_tiles.push_back({(int32_t)(t.lm_begin + lo), (int32_t)(t.lm_end + lo)});
             lm_tile_win->push_back((int32_t)w);
         }
+        // camera runs: tile-local permutations are unchanged; run ids shift
+        M.tile_perm.insert(M.tile_perm.end(), m.tile_perm.begin(), m.tile_perm.end());
+        for (size_t t = 0; t < m.lm_tiles.size(); ++t) M.tile_run_base.push_back((int32_t)(m.tile_run_base[t] + ro));
+        for (uint32_t s2 = 0; s2 < m.num_pose_slots; ++s2)
+            M.pose_run_ptr.push_back((int32_t)(m.pose_run_ptr[s2] + rlo));
+        for (int32_t r : m.pose_runs) M.pose_runs.push_back((int32_t)(r + ro));
+        ro += (int64_t)m.num_runs;
+        rlo += (int64_t)m.pose_runs.size();
         M.max_lm_tile_edges = std::max(M.max_lm_tile_edges, m.max_lm_tile_edges);
         if (m.any_fixed_pose) M.any_fixed_pose = true;
         const DenseLists& d = dense[w];
@@ -790,33 +837,33 @@ void merge_models(const std::vector<const Model*>& ms, Model* out, std::vector<W
     M.cam_edge_ptr.push_back((int32_t)eo);
     M.cam_tile_ptr.push_back((int32_t)to);
     M.lm_edge_ptr.push_back((int32_t)eo);
+    M.tile_run_base.push_back((int32_t)ro);
+    M.pose_run_ptr.push_back((int32_t)rlo);
+    M.num_runs = (uint64_t)ro;
     dense_out->block_ptr.push_back((int32_t)term_off);
 }
 
 }  // namespace
 
-int32_t run_batch(Context& ctx, std::vector<Model>& models, const std::vector<int32_t>& valid, const vba_options& o,
-                  double* const* poses_out, double* const* landmarks_out, vba_iteration_stats* stats, uint32_t cap,
-                  vba_result* results, int32_t* statuses) {
+int32_t prepare_batch(Context& ctx, std::vector<Model>& models, const std::vector<int32_t>& valid,
+                      PreparedBatch* pbp, int32_t* statuses, std::string* message) {
+    PreparedBatch& pb = *pbp;
     hipStream_t s = ctx.stream;
-    Prepared pr;
     try {
-        if (!ctx.host_scalars) {
-            hip_check(hipHostMalloc((void**)&ctx.host_scalars, sizeof(StepScalars), hipHostMallocDefault), "hipHostMalloc");
-            hip_check(hipHostMalloc((void**)&ctx.host_pcg, sizeof(PcgState), hipHostMallocDefault), "hipHostMalloc");
-            hip_check(hipHostMalloc((void**)&ctx.host_status, sizeof(int32_t), hipHostMallocDefault), "hipHostMalloc");
-        }
         std::vector<const Model*> ms;
-        std::vector<int32_t> index;  // batch window -> caller index
         std::vector<DenseLists> dense;
+        pb.count = (uint32_t)models.size();
+        pb.index.clear();
+        pb.window_of.assign(models.size(), -1);
         for (size_t i = 0; i < models.size(); ++i) {
             if (!valid[i]) continue;
             if (models[i].P > 256) {
                 statuses[i] = VBA_ERR_INVALID_ARGUMENT;
                 continue;
             }
+            pb.window_of[i] = (int32_t)ms.size();
             ms.push_back(&models[i]);
-            index.push_back((int32_t)i);
+            pb.index.push_back((int32_t)i);
             dense.emplace_back();
             build_dense_lists(models[i], &dense.back());
         }
@@ -825,101 +872,96 @@ int32_t run_batch(Context& ctx, std::vector<Model>& models, const std::vector<in
         size_t num_sensors = 0;
         for (const Model* mw : ms) num_sensors += mw->sensors.size();
         if (num_sensors > 65535) throw Fail{VBA_ERR_INVALID_ARGUMENT, "batch holds more than 65535 sensors"};
-        std::vector<WindowLayout> layout;
         std::vector<int32_t> pose_win, lm_win, cam_tile_win, lm_tile_win;
         DenseLists dl;
-        merge_models(ms, &pr.m, &layout, &pose_win, &lm_win, &cam_tile_win, &lm_tile_win, dense, &dl);
+        merge_models(ms, &pb.m, &pb.layout, &pose_win, &lm_win, &cam_tile_win, &lm_tile_win, dense, &dl);
+        pb.pose_slot_of_input.clear();
+        pb.lm_slot_of_input.clear();
+        pb.num_obs.clear();
+        for (const Model* mw : ms) {
+            pb.pose_slot_of_input.push_back(mw->pose_slot_of_input);
+            pb.lm_slot_of_input.push_back(mw->lm_slot_of_input);
+            pb.num_obs.push_back(mw->num_obs);
+        }
         std::string msg;
-        int32_t st = prepare(ctx, pr, &msg);
+        int32_t st = prepare(ctx, pb, &msg);
         if (st != VBA_OK) throw Fail{st, msg};
-        const Model& m = pr.m;
-        // ---- bind (single-problem buffers + window arrays) ----
-        DevProblem P{};
-        P.num_pose_slots = (int32_t)m.num_pose_slots;
-        P.num_lm_slots = (int32_t)m.num_lm_slots;
-        P.num_edges = (int32_t)m.num_edges;
-        P.num_obs = (int32_t)m.num_obs;
-        P.num_cam_tiles = (int32_t)m.cam_tiles.size();
-        P.num_lm_tiles = (int32_t)m.lm_tiles.size();
+        pb.bufs.resize(BB_COUNT);
+        upload(pb, BB_POSE_WIN, pose_win, s);
+        upload(pb, BB_LM_WIN, lm_win, s);
+        upload(pb, BB_CAM_TILE_WIN, cam_tile_win, s);
+        upload(pb, BB_LM_TILE_WIN, lm_tile_win, s);
+        upload(pb, BB_LAYOUT, pb.layout, s);
+        pb.work_offset.assign(W, 0);
+        pb.s_total = pb.work_total = 0;
+        for (int32_t w = 0; w < W; ++w) {
+            const int64_t n = 6 * (int64_t)(pb.layout[w].var_end - pb.layout[w].var_begin);
+            pb.s_total += n * n;
+            pb.work_offset[w] = pb.work_total;
+            pb.work_total += n * n + n;
+        }
+        upload(pb, BB_WORK_OFFSET, pb.work_offset, s);
+        pb.num_dense_blocks = (int32_t)(dl.block_ptr.size() - 1);
+        upload(pb, B_DENSE_BLOCK_PQ, dl.block_pq, s);
+        upload(pb, B_DENSE_BLOCK_PTR, dl.block_ptr, s);
+        upload(pb, B_DENSE_TERMS, dl.terms, s);
+        upload(pb, B_DENSE_VAR_SLOT, dl.var_slot, s);
+        ensure(pb, BB_S, std::max<int64_t>(pb.s_total, 1) * sizeof(double));
+        ensure(pb, BB_WORK, std::max<int64_t>(pb.work_total, 1) * sizeof(double));
+        ensure(pb, BB_STATE, W * sizeof(WindowState));
+        hip_check(hipStreamSynchronize(s), "sync");
+        // the structure now lives on the device; keep only what outputs need
+        pb.m.obs_uv = std::vector<double>();
+        pb.m.obs_sensor = std::vector<uint16_t>();
+        pb.m.obs_input = std::vector<uint32_t>();
+        pb.solved = false;
+        return VBA_OK;
+    } catch (const Fail& f) {
+        *message = f.msg;
+        (void)hipStreamSynchronize(s);
+        return f.code;
+    }
+}
+
+int32_t run_batch(Context& ctx, PreparedBatch& pb, const vba_options& o, vba_iteration_stats* stats, uint32_t cap,
+                  vba_result* results, int32_t* statuses) {
+    hipStream_t s = ctx.stream;
+    const int32_t W = (int32_t)pb.index.size();
+    if (W == 0) return VBA_OK;
+    try {
+        if (!ctx.host_scalars) {
+            hip_check(hipHostMalloc((void**)&ctx.host_scalars, sizeof(StepScalars), hipHostMallocDefault), "hipHostMalloc");
+            hip_check(hipHostMalloc((void**)&ctx.host_pcg, sizeof(PcgState), hipHostMallocDefault), "hipHostMalloc");
+            hip_check(hipHostMalloc((void**)&ctx.host_status, sizeof(int32_t), hipHostMallocDefault), "hipHostMalloc");
+        }
+        DevProblem P = bind(pb, o, false, false, s);
         P.any_rig = 1;
-        P.kernel_kind = (int32_t)o.robust_kernel;
-        P.kernel_scale = o.robust_scale;
-        const size_t NP = m.num_pose_slots, NL = m.num_lm_slots, NE = m.num_edges, NT = m.cam_tiles.size();
-        const size_t NLT = m.lm_tiles.size();
-        const size_t pose_blocks = (NP + kBlock - 1) / kBlock;
-        P.poses = buf<double>(pr, B_POSES);
-        P.lms = buf<double>(pr, B_LMS);
-        P.poses_trial = buf<double>(pr, B_POSES_T);
-        P.lms_trial = buf<double>(pr, B_LMS_T);
-        hip_check(hipMemcpyAsync(P.poses, pr.bufs[B_POSES0].ptr, m.poses.size() * sizeof(double), hipMemcpyDeviceToDevice, s), "D2D");
-        hip_check(hipMemcpyAsync(P.lms, pr.bufs[B_LMS0].ptr, m.lms.size() * sizeof(double), hipMemcpyDeviceToDevice, s), "D2D");
-        P.sensors = buf<DevSensor>(pr, B_SENSORS);
-        P.pose_var = buf<int32_t>(pr, B_POSE_VAR);
-        P.lm_var = buf<int32_t>(pr, B_LM_VAR);
-        P.obs_uv = buf<double>(pr, B_OBS_UV);
-        P.obs_sensor = buf<uint16_t>(pr, B_OBS_SENSOR);
-        P.edge_lm = buf<int32_t>(pr, B_EDGE_LM);
-        P.edge_obs = buf<int32_t>(pr, B_EDGE_OBS);
-        P.perm_cl = buf<int32_t>(pr, B_PERM_CL);
-        P.l_pose = buf<int32_t>(pr, B_L_POSE);
-        P.lm_edge_ptr = buf<int32_t>(pr, B_LM_EDGE_PTR);
-        P.cam_tiles = buf<CamTile>(pr, B_CAM_TILES);
-        P.cam_tile_ptr = buf<int32_t>(pr, B_CAM_TILE_PTR);
-        P.lm_tiles = buf<LmTile>(pr, B_LM_TILES);
-        P.ec64 = (EdgeRec64*)ensure(pr, B_EC64, NE * sizeof(EdgeRec64));
-        P.el64 = (EdgeRec64*)ensure(pr, B_EL64, NE * sizeof(EdgeRec64));
-        P.hl64 = (double*)ensure(pr, B_HL64, 3 * NE * sizeof(double));
-        P.tile27 = (double*)ensure(pr, B_TILE27, 27 * std::max<size_t>(NT, 1) * sizeof(double));
-        P.U = (double*)ensure(pr, B_U, 21 * NP * sizeof(double));
-        P.g = (double*)ensure(pr, B_G, 6 * NP * sizeof(double));
-        P.V = (double*)ensure(pr, B_V, 6 * NL * sizeof(double));
-        P.gl = (double*)ensure(pr, B_GL, 3 * NL * sizeof(double));
-        P.Vinv64 = (double*)ensure(pr, B_VINV64, 6 * NL * sizeof(double));
-        P.pack64 = (LmPack<double>*)ensure(pr, B_PACK64, NL * sizeof(LmPack<double>));
-        P.b = (double*)ensure(pr, B_VB, 6 * NP * sizeof(double));
-        P.x = (double*)ensure(pr, B_VX, 6 * NP * sizeof(double));
-        P.part64 = (double*)ensure(pr, B_PART64, 6 * std::max<size_t>(NT, 1) * sizeof(double));
-        P.delta_l = (double*)ensure(pr, B_DELTA_L, 3 * NL * sizeof(double));
-        const size_t nscalar = 4 * (NT + NLT + pose_blocks + 1);
-        P.tile_scalar = (double*)ensure(pr, B_TILE_SCALAR, nscalar * sizeof(double));
-        P.scalars = (StepScalars*)ensure(pr, B_SCALARS, sizeof(StepScalars));
-        hip_check(hipMemsetAsync(P.x, 0, 6 * NP * sizeof(double), s), "memset");
-        hip_check(hipMemsetAsync(P.delta_l, 0, 3 * NL * sizeof(double), s), "memset");
-        hip_check(hipMemsetAsync(P.tile_scalar, 0, nscalar * sizeof(double), s), "memset");
-        pr.bufs.resize(BB_COUNT);
-        P.pose_win = upload(pr, BB_POSE_WIN, pose_win, s);
-        P.lm_win = upload(pr, BB_LM_WIN, lm_win, s);
-        P.cam_tile_win = upload(pr, BB_CAM_TILE_WIN, cam_tile_win, s);
-        P.lm_tile_win = upload(pr, BB_LM_TILE_WIN, lm_tile_win, s);
+        P.obs_sensor = buf<uint16_t>(pb, B_OBS_SENSOR);
+        P.pose_win = buf<int32_t>(pb, BB_POSE_WIN);
+        P.lm_win = buf<int32_t>(pb, BB_LM_WIN);
+        P.cam_tile_win = buf<int32_t>(pb, BB_CAM_TILE_WIN);
+        P.lm_tile_win = buf<int32_t>(pb, BB_LM_TILE_WIN);
         P.num_windows = W;
         std::vector<WindowState> init(W);
         std::memset(init.data(), 0, W * sizeof(WindowState));
         for (int32_t w = 0; w < W; ++w) init[w].active = 1, init[w].relin = 1;
-        WindowState* dstate = upload(pr, BB_STATE, init, s);
+        WindowState* dstate = upload(pb, BB_STATE, init, s);
         P.wstate = dstate;
         BatchDev B{};
         B.num_windows = W;
-        B.layout = upload(pr, BB_LAYOUT, layout, s);
+        B.layout = buf<WindowLayout>(pb, BB_LAYOUT);
         B.state = dstate;
         B.stats_capacity = (int32_t)std::max<uint32_t>(cap, 1);
-        B.stats = (vba_iteration_stats*)ensure(pr, BB_STATS, (size_t)W * B.stats_capacity * sizeof(vba_iteration_stats));
-        std::vector<int64_t> work_off(W);
-        int64_t s_total = 0, work_total = 0;
-        for (int32_t w = 0; w < W; ++w) {
-            const int64_t n = 6 * (int64_t)(layout[w].var_end - layout[w].var_begin);
-            s_total += n * n;
-            work_off[w] = work_total;
-            work_total += n * n + n;
-        }
-        B.S = (double*)ensure(pr, BB_S, std::max<int64_t>(s_total, 1) * sizeof(double));
-        B.work = (double*)ensure(pr, BB_WORK, std::max<int64_t>(work_total, 1) * sizeof(double));
-        B.work_offset = upload(pr, BB_WORK_OFFSET, work_off, s);
+        B.stats = (vba_iteration_stats*)ensure(pb, BB_STATS, (size_t)W * B.stats_capacity * sizeof(vba_iteration_stats));
+        B.S = buf<double>(pb, BB_S);
+        B.work = buf<double>(pb, BB_WORK);
+        B.work_offset = buf<int64_t>(pb, BB_WORK_OFFSET);
         DensePairs D{};
-        D.num_blocks = (int32_t)(dl.block_ptr.size() - 1);
-        D.block_pq = upload(pr, B_DENSE_BLOCK_PQ, dl.block_pq, s);
-        D.block_ptr = upload(pr, B_DENSE_BLOCK_PTR, dl.block_ptr, s);
-        D.terms = upload(pr, B_DENSE_TERMS, dl.terms, s);
-        D.var_slot = upload(pr, B_DENSE_VAR_SLOT, dl.var_slot, s);
+        D.num_blocks = pb.num_dense_blocks;
+        D.block_pq = buf<int32_t>(pb, B_DENSE_BLOCK_PQ);
+        D.block_ptr = buf<int32_t>(pb, B_DENSE_BLOCK_PTR);
+        D.terms = buf<int32_t>(pb, B_DENSE_TERMS);
+        D.var_slot = buf<int32_t>(pb, B_DENSE_VAR_SLOT);
 
         hip_check(hipEventRecord(ctx.ev_start, s), "event");
         // ---- initial cost and state, then max_iterations LM steps with no
@@ -932,8 +974,7 @@ int32_t run_batch(Context& ctx, std::vector<Model>& models, const std::vector<in
             launch_camera_finalize(P, s);
             launch_landmark_reduce(P, s);
             launch_landmark_prepare(P, 0.0, s);
-            launch_cam_pass(P, false, nullptr, s);
-            launch_rhs_finalize(P, s);
+            launch_rhs(P, s);
             launch_batch_dense_zero(P, B, s);
             launch_batch_dense_assemble(P, D, B, s);
             launch_batch_dense_solve(P, D, B, s);
@@ -948,46 +989,68 @@ int32_t run_batch(Context& ctx, std::vector<Model>& models, const std::vector<in
         hip_check(hipEventRecord(ctx.ev_stop, s), "event");
         std::vector<WindowState> fin(W);
         std::vector<vba_iteration_stats> st_host((size_t)W * B.stats_capacity);
-        std::vector<double> poses(m.poses.size()), lms(m.lms.size());
         hip_check(hipMemcpyAsync(fin.data(), dstate, W * sizeof(WindowState), hipMemcpyDeviceToHost, s), "D2H");
-        hip_check(hipMemcpyAsync(st_host.data(), B.stats, st_host.size() * sizeof(vba_iteration_stats), hipMemcpyDeviceToHost, s), "D2H");
-        hip_check(hipMemcpyAsync(poses.data(), P.poses, poses.size() * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
-        hip_check(hipMemcpyAsync(lms.data(), P.lms, lms.size() * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+        hip_check(hipMemcpyAsync(st_host.data(), B.stats, st_host.size() * sizeof(vba_iteration_stats),
+                                 hipMemcpyDeviceToHost, s), "D2H");
         hip_check(hipStreamSynchronize(s), "sync");
         float ms_elapsed = 0.f;
         hip_check(hipEventElapsedTime(&ms_elapsed, ctx.ev_start, ctx.ev_stop), "event");
+        pb.final_slot = B_POSES;
+        pb.final_lm_slot = B_LMS;
+        pb.solved = true;
         for (int32_t w = 0; w < W; ++w) {
-            const int32_t i = index[w];
-            const Model& mw = models[i];
-            const WindowLayout& L = layout[w];
-            if (poses_out && poses_out[i])
-                for (size_t k = 0; k < mw.pose_slot_of_input.size(); ++k)
-                    std::memcpy(poses_out[i] + 7 * k, poses.data() + 7 * ((size_t)L.pose_begin + mw.pose_slot_of_input[k]),
-                                7 * sizeof(double));
-            if (landmarks_out && landmarks_out[i])
-                for (size_t k = 0; k < mw.lm_slot_of_input.size(); ++k)
-                    std::memcpy(landmarks_out[i] + 3 * k, lms.data() + 3 * ((size_t)L.lm_begin + mw.lm_slot_of_input[k]),
-                                3 * sizeof(double));
+            const int32_t i = pb.index[w];
             const uint32_t n = (uint32_t)std::min<int32_t>(fin[w].iterations, B.stats_capacity);
             if (stats)
                 std::memcpy(stats + (size_t)i * cap, st_host.data() + (size_t)w * B.stats_capacity,
                             std::min(n, cap) * sizeof(vba_iteration_stats));
             vba_result& r = results[i];
+            std::memset(&r, 0, sizeof(r));
             r.initial_cost = fin[w].initial_cost;
             r.final_cost = fin[w].current_cost;
             r.num_iterations = std::min(n, cap);
             r.converged = fin[w].converged;
-            r.num_observations_used = mw.num_obs;
-            r.linearizations = 0;
+            r.num_observations_used = pb.num_obs[w];
             r.device_seconds = ms_elapsed * 1e-3;
             statuses[i] = VBA_OK;
         }
-        release(pr);
         return VBA_OK;
     } catch (const Fail& f) {
         ctx.last_error = f.msg;
         (void)hipStreamSynchronize(s);
-        release(pr);
+        return f.code;
+    }
+}
+
+int32_t download_batch_window(Context& ctx, const PreparedBatch& pb, uint32_t i, double* poses_out,
+                              double* landmarks_out) {
+    if (i >= pb.window_of.size() || pb.window_of[i] < 0) {
+        ctx.last_error = "window is not in the batch";
+        return VBA_ERR_INVALID_ARGUMENT;
+    }
+    const int32_t w = pb.window_of[i];
+    const WindowLayout& L = pb.layout[w];
+    hipStream_t s = ctx.stream;
+    try {
+        const int ps = pb.final_slot >= 0 ? pb.final_slot : B_POSES0;
+        const int ls = pb.final_slot >= 0 ? pb.final_lm_slot : B_LMS0;
+        std::vector<double> poses(7 * (size_t)(L.pose_end - L.pose_begin)), lms(3 * (size_t)(L.lm_end - L.lm_begin));
+        hip_check(hipMemcpyAsync(poses.data(), (const double*)pb.bufs[ps].ptr + 7 * (size_t)L.pose_begin,
+                                 poses.size() * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+        hip_check(hipMemcpyAsync(lms.data(), (const double*)pb.bufs[ls].ptr + 3 * (size_t)L.lm_begin,
+                                 lms.size() * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+        hip_check(hipStreamSynchronize(s), "sync");
+        const auto& pso = pb.pose_slot_of_input[w];
+        const auto& lso = pb.lm_slot_of_input[w];
+        if (poses_out)
+            for (size_t k = 0; k < pso.size(); ++k)
+                std::memcpy(poses_out + 7 * k, poses.data() + 7 * (size_t)pso[k], 7 * sizeof(double));
+        if (landmarks_out)
+            for (size_t k = 0; k < lso.size(); ++k)
+                std::memcpy(landmarks_out + 3 * k, lms.data() + 3 * (size_t)lso[k], 3 * sizeof(double));
+        return VBA_OK;
+    } catch (const Fail& f) {
+        ctx.last_error = f.msg;
         return f.code;
     }
 }

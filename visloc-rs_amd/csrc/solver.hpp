@@ -77,12 +77,33 @@ int32_t run(Context& ctx, Prepared& prep, const vba_options& o, vba_iteration_st
 // Final (or, before any solve, initial) poses / landmarks in input order.
 int32_t download_state(Context& ctx, const Prepared& prep, double* poses_out, double* landmarks_out);
 
-// Solve independent windows together (one device-resident batch, LM control
-// on the device). models[i] is window i, valid[i] = 0 skips it; outputs are
-// indexed by window; statuses[i] receives each window's status.
-int32_t run_batch(Context& ctx, std::vector<Model>& models, const std::vector<int32_t>& valid,
-                  const vba_options& o, double* const* poses_out, double* const* landmarks_out,
-                  vba_iteration_stats* stats, uint32_t stats_capacity, vba_result* results, int32_t* statuses);
+// Independent windows laid out as one device-resident problem with disjoint
+// slots; the LM control runs per window on the device.
+struct PreparedBatch : Prepared {
+    uint32_t count = 0;                   // the caller's windows
+    std::vector<int32_t> index;           // batch window -> caller index
+    std::vector<int32_t> window_of;       // caller index -> batch window, -1 if not in the batch
+    std::vector<WindowLayout> layout;
+    std::vector<int64_t> work_offset;     // per window: dense solve workspace
+    int64_t s_total = 0, work_total = 0;  // doubles
+    int32_t num_dense_blocks = 0;
+    std::vector<std::vector<uint32_t>> pose_slot_of_input, lm_slot_of_input;  // per batch window
+    std::vector<uint64_t> num_obs;                                           // per batch window
+    bool solved = false;
+};
+
+// Lay out the valid windows (valid[i] = 1) of `models` and upload them.
+// A window with more than 256 variable poses gets VBA_ERR_INVALID_ARGUMENT
+// in statuses[i] and stays out of the batch.
+int32_t prepare_batch(Context& ctx, std::vector<Model>& models, const std::vector<int32_t>& valid,
+                      PreparedBatch* batch, int32_t* statuses, std::string* message);
+// Solve every window of the batch from its prepared initial state; results,
+// stats (stats_capacity per caller index) and statuses by caller index.
+int32_t run_batch(Context& ctx, PreparedBatch& batch, const vba_options& o, vba_iteration_stats* stats,
+                  uint32_t stats_capacity, vba_result* results, int32_t* statuses);
+// Final (before any solve: initial) state of the caller's window i.
+int32_t download_batch_window(Context& ctx, const PreparedBatch& batch, uint32_t i, double* poses_out,
+                              double* landmarks_out);
 
 void release(Prepared& prep);
 void release_context(Context& ctx);

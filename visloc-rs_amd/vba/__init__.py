@@ -5,10 +5,12 @@ Drop-in for the Levenberg–Marquardt Schur-complement path of visloc-rs
 """
 from . import abi
 from .abi import library
-from .api import (BundleProblem, Device, PreparedProblem, PrepareError, SolveOutput, Synthetic, camera,
-                  camera_centres, check, default_options, run_solver, synth_params, synthesize)
+from .api import (BatchOutput, BundleProblem, Device, PreparedBatch, PreparedProblem, PrepareError, ServedResult,
+                  Server, SolveOutput, Synthetic, camera, camera_centres, check, default_options, run_solver,
+                  synth_params, synthesize)
 
 __all__ = [
-    "abi", "library", "BundleProblem", "Device", "PreparedProblem", "PrepareError", "SolveOutput", "Synthetic", "camera",
-    "camera_centres", "check", "default_options", "run_solver", "synth_params", "synthesize",
+    "abi", "library", "BatchOutput", "BundleProblem", "Device", "PreparedBatch", "PreparedProblem", "PrepareError",
+    "ServedResult", "Server", "SolveOutput", "Synthetic", "camera", "camera_centres", "check", "default_options",
+    "run_solver", "synth_params", "synthesize",
 ]

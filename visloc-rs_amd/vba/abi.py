@@ -13,6 +13,7 @@ ABI_VERSION = 1
 
 OK = 0
 DECLINED = 1
+PENDING = 2
 ERR_NO_POSES = -1
 ERR_NO_LANDMARKS = -2
 ERR_NO_OBSERVATIONS = -3
@@ -191,6 +192,17 @@ class SynthParams(C.Structure):
 
 # Every symbol include/vba.h and include/vba_synth.h declare, with its
 # signature (restype, argtypes).
+class ServerConfig(C.Structure):
+    _fields_ = [
+        ("device", C.c_int32),
+        ("max_window_batch", C.c_uint32),
+        ("window_streams", C.c_uint32),
+        ("submap_streams", C.c_uint32),
+        ("hbm_budget_bytes", C.c_uint64),
+        ("batch_wait_seconds", C.c_double),
+    ]
+
+
 SIGNATURES = {
     "vba_abi_version": (C.c_uint32, []),
     "vba_create": (C.c_void_p, [C.c_int32]),
@@ -214,6 +226,25 @@ SIGNATURES = {
         [C.c_void_p, C.c_uint32, C.POINTER(Problem), C.POINTER(Options), C.POINTER(P64), C.POINTER(P64),
          C.POINTER(IterationStats), C.c_uint32, C.POINTER(Result), C.POINTER(C.c_int32)],
     ),
+    "vba_prepare_batch": (
+        C.c_int32,
+        [C.c_void_p, C.c_uint32, C.POINTER(Problem), C.POINTER(Options), C.POINTER(C.c_void_p), C.POINTER(Result),
+         C.POINTER(C.c_int32)],
+    ),
+    "vba_solve_prepared_batch": (
+        C.c_int32,
+        [C.c_void_p, C.c_void_p, C.POINTER(Options), C.POINTER(IterationStats), C.c_uint32, C.POINTER(Result),
+         C.POINTER(C.c_int32)],
+    ),
+    "vba_prepared_batch_state": (C.c_int32, [C.c_void_p, C.c_void_p, C.c_uint32, P64, P64]),
+    "vba_prepared_batch_free": (None, [C.c_void_p, C.c_void_p]),
+    "vba_server_default_config": (None, [C.POINTER(ServerConfig)]),
+    "vba_server_create": (C.c_void_p, [C.POINTER(ServerConfig)]),
+    "vba_server_submit": (C.c_uint64, [C.c_void_p, C.POINTER(Problem), C.POINTER(Options)]),
+    "vba_server_wait": (C.c_int32, [C.c_void_p, C.c_uint64, C.c_double, P64, P64, C.POINTER(Result),
+                                    C.POINTER(C.c_double)]),
+    "vba_server_counters": (None, [C.c_void_p, PU64, PU64, PU64, PU64]),
+    "vba_server_destroy": (None, [C.c_void_p]),
     "vba_profile_enable": (None, [C.c_void_p, C.c_int32]),
     "vba_profile_reset": (None, [C.c_void_p]),
     "vba_profile_query": (C.c_int32, [C.c_void_p, C.c_char_p, PU64, P64, P64]),

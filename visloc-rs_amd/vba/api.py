@@ -312,6 +312,9 @@ class Device:
                                     its, poses[i] if ok else None, lms[i] if ok else None))
         return BatchOutput(st, outs, self.last_error() if st != abi.OK else "")
 
+    def prepare_batch(self, problems: List[BundleProblem], options: abi.Options) -> "PreparedBatch":
+        return PreparedBatch(self, problems, options)
+
 
 @dataclass
 class BatchOutput:
@@ -364,6 +367,137 @@ class PreparedProblem:
     def __del__(self):
         try:
             self.free()
+        except Exception:
+            pass
+
+
+class PreparedBatch:
+    """Windows resident on the device as one batch (vba_prepare_batch)."""
+
+    def __init__(self, device: "Device", problems: List[BundleProblem], options: abi.Options):
+        self.device = device
+        self.problems = list(problems)
+        n = len(self.problems)
+        views = (abi.Problem * max(n, 1))()
+        for i, p in enumerate(self.problems):
+            views[i] = p.view()
+        self.handle = C.c_void_p()
+        results = (abi.Result * max(n, 1))()
+        statuses = (C.c_int32 * max(n, 1))()
+        st = device.lib.vba_prepare_batch(device.ctx, n, views, C.byref(options), C.byref(self.handle), results,
+                                          statuses)
+        if st != abi.OK:
+            self.handle = None
+            raise PrepareError(st, device.last_error())
+        self.statuses = [int(statuses[i]) for i in range(n)]
+        self.error_ids = [int(results[i].error_id) for i in range(n)]
+
+    def solve(self, options: abi.Options, cap: int = 64) -> BatchOutput:
+        n = len(self.problems)
+        stats = (abi.IterationStats * max(n * cap, 1))()
+        results = (abi.Result * max(n, 1))()
+        statuses = (C.c_int32 * max(n, 1))(*self.statuses)
+        st = self.device.lib.vba_solve_prepared_batch(self.device.ctx, self.handle, C.byref(options), stats, cap,
+                                                      results, statuses)
+        outs = []
+        for i in range(n):
+            ok = st == abi.OK and statuses[i] == abi.OK
+            k = min(results[i].num_iterations, cap)
+            its = [{f[0].rstrip("_"): getattr(stats[i * cap + j], f[0]) for f in abi.IterationStats._fields_}
+                   for j in range(k)] if ok else []
+            outs.append(SolveOutput(statuses[i] if st == abi.OK else st, abi.Result.from_buffer_copy(results[i]),
+                                    its, None, None))
+        return BatchOutput(st, outs, self.device.last_error() if st != abi.OK else "")
+
+    def state(self, i: int):
+        p = self.problems[i]
+        poses = np.empty_like(p.poses)
+        lms = np.empty_like(p.landmarks)
+        st = self.device.lib.vba_prepared_batch_state(self.device.ctx, self.handle, i, poses.ctypes.data_as(abi.P64),
+                                                      lms.ctypes.data_as(abi.P64))
+        if st != abi.OK:
+            raise RuntimeError(self.device.last_error())
+        return poses, lms
+
+    def free(self):
+        if self.handle:
+            self.device.lib.vba_prepared_batch_free(self.device.ctx, self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+@dataclass
+class ServedResult:
+    status: int
+    result: abi.Result
+    poses: Optional[np.ndarray]
+    landmarks: Optional[np.ndarray]
+    latency: float
+
+
+class Server:
+    """The request loop (vba_server_*): submit() returns a ticket at once,
+    wait() returns the finished solve with its latency."""
+
+    def __init__(self, **config):
+        self.lib = abi.library()
+        cfg = abi.ServerConfig()
+        self.lib.vba_server_default_config(C.byref(cfg))
+        for k, v in config.items():
+            if not hasattr(cfg, k):
+                raise AttributeError(k)
+            setattr(cfg, k, v)
+        self.handle = self.lib.vba_server_create(C.byref(cfg))
+        if not self.handle:
+            raise RuntimeError("vba_server_create failed: " + self.lib.vba_last_error(None).decode())
+        self.shapes = {}
+
+    def submit(self, problem: BundleProblem, options: abi.Options) -> int:
+        view = problem.view()
+        t = self.lib.vba_server_submit(self.handle, C.byref(view), C.byref(options))
+        if t == 0:
+            raise RuntimeError(self.lib.vba_last_error(None).decode())
+        self.shapes[t] = (problem.poses.shape, problem.landmarks.shape)
+        return t
+
+    def wait(self, ticket: int, timeout: float = 600.0) -> ServedResult:
+        ps, ls = self.shapes[ticket]
+        poses = np.empty(ps)
+        lms = np.empty(ls)
+        res = abi.Result()
+        lat = C.c_double()
+        st = self.lib.vba_server_wait(self.handle, ticket, timeout, poses.ctypes.data_as(abi.P64),
+                                      lms.ctypes.data_as(abi.P64), C.byref(res), C.byref(lat))
+        if st == abi.PENDING:
+            return ServedResult(st, res, None, None, float("nan"))
+        del self.shapes[ticket]
+        ok = st == abi.OK
+        return ServedResult(st, res, poses if ok else None, lms if ok else None, lat.value)
+
+    def counters(self) -> dict:
+        v = [C.c_uint64() for _ in range(4)]
+        self.lib.vba_server_counters(self.handle, *[C.byref(x) for x in v])
+        return dict(zip(("completed", "batches", "batched_windows", "queued"), (int(x.value) for x in v)))
+
+    def close(self):
+        if self.handle:
+            self.lib.vba_server_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
         except Exception:
             pass
 
