@@ -82,7 +82,8 @@ __global__ __launch_bounds__(kBlock) void k_linearize(DevProblem P, int store32)
             h[1] += w * (J[1] * r0 + J[4] * r1);
             h[2] += w * (J[2] * r0 + J[5] * r1);
         }
-        // l-order G (FP64, and FP32 for the mixed-precision operator) and h
+        // G (FP64, and FP32 for the mixed-precision operator) and h at the
+        // edge's storage position
         const size_t pl = (size_t)P.perm_cl[e];
         double2* g64 = (double2*)(P.Gl64 + 6 * pl);
         g64[0] = make_double2(G[0], G[1]);
@@ -136,11 +137,12 @@ __global__ void k_landmark_reduce(DevProblem P) {
     if (P.wstate && !P.wstate[P.lm_win[l]].relin) return;
     double V[6] = {0, 0, 0, 0, 0, 0}, g[3] = {0, 0, 0};
     for (int e = P.lm_edge_ptr[l]; e < P.lm_edge_ptr[l + 1]; ++e) {
-        const double* G = P.Gl64 + 6 * (size_t)e;
+        const size_t st = (size_t)P.lm_store[e];
+        const double* G = P.Gl64 + 6 * st;
 #pragma unroll
         for (int i = 0; i < 6; ++i) V[i] += G[i];
 #pragma unroll
-        for (int i = 0; i < 3; ++i) g[i] += P.hl64[3 * (size_t)e + i];
+        for (int i = 0; i < 3; ++i) g[i] += P.hl64[3 * st + i];
     }
 #pragma unroll
     for (int i = 0; i < 6; ++i) P.V[6 * (size_t)l + i] = V[i];
@@ -151,8 +153,8 @@ __global__ void k_landmark_reduce(DevProblem P) {
 // Damped landmark inverse (V + lambda I)^-1 by cofactors, zero when the
 // determinant is exactly zero or the landmark is fixed (the landmark then
 // drops out of the Schur complement and back-substitution, as try_inverse
-// returning None does in normal_equations.rs:1432-1445); FP32 copies of the
-// inverse and the position for the mixed-precision operator.
+// returning None does in normal_equations.rs:1432-1445); an FP32 copy of
+// the inverse for the mixed-precision operator.
 __global__ void k_landmark_prepare(DevProblem P, double lambda) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= P.num_lm_slots) return;
@@ -187,8 +189,6 @@ __global__ void k_landmark_prepare(DevProblem P, double lambda) {
     if (P.Vinv32) {
 #pragma unroll
         for (int i = 0; i < 6; ++i) P.Vinv32[6 * (size_t)l + i] = (float)inv[i];
-        P.X32[l] = make_float4((float)P.lms[3 * (size_t)l], (float)P.lms[3 * (size_t)l + 1],
-                               (float)P.lms[3 * (size_t)l + 2], 0.f);
     }
 }
 

@@ -80,13 +80,14 @@ struct DevProblem {
     const uint16_t* obs_sensor;
     const int32_t* edge_lm;
     const int32_t* edge_obs;
-    const int32_t* perm_cl;
-    const int32_t* l_pose;
+    const int32_t* perm_cl;   // c-order edge -> storage position
+    const int32_t* s_pose;    // pose at each storage position
+    const int32_t* lm_store;  // storage position of each l-order edge
     const int32_t* lm_edge_ptr;
     const CamTile* cam_tiles;
     const int32_t* cam_tile_ptr;
     const LmTile* lm_tiles;
-    // linearization: per l-order edge the upper triangle of G and h
+    // linearization: per edge (storage order) the upper triangle of G and h
     double* Gl64;    // 6 per edge
     float* Gl32;     // FP32 copy for the mixed-precision operator
     double* hl64;    // 3 per edge
@@ -97,23 +98,19 @@ struct DevProblem {
     double* gl;      // 3 per landmark slot
     double* Vinv64;  // 6 per landmark slot
     float* Vinv32;
-    float4* X32;     // landmark positions, FP32
     double* Minv;    // 36 per pose slot (block-Jacobi inverse)
     // Schur operator structure. The edges of a landmark tile, sorted by pose
     // (tile_perm: tile-local l-order index at each sorted position), form
-    // camera runs; each run leaves one partial, numbered from
-    // tile_run_base[tile] in sorted order, and a pose sums its runs
-    // (pose_runs[pose_run_ptr[p] ..]) in ascending order.
+    // camera runs; a run leaves one partial at the edge index of its last
+    // sorted position, and a pose sums its runs (pose_runs[pose_run_ptr[p]
+    // ..], ascending) in that order.
     const uint16_t* tile_perm;
-    const int32_t* tile_run_base;  // num_lm_tiles + 1
-    const int32_t* pose_run_ptr;   // num_pose_slots + 1
-    const int32_t* pose_runs;
-    double* part64;  // 6 per run
-    float* part32;
-    double* part21;  // 21 per run (block-Jacobi blocks)
+    const int32_t* pose_run_ptr;  // num_pose_slots + 1
+    const int32_t* pose_runs;     // run-end edge indices
+    double* part64;  // 6 per edge index (run ends only)
+    double* part21;  // 21 per edge index (block-Jacobi blocks)
     // PCG vectors, 6 per pose slot
     double *b, *x, *r, *z, *d, *q;
-    float *z32, *d32;
     double* bpart;  // per work-group partials of the pose kernels (2 each)
     double* cpart;  // per work-group (r.z, r.r) of the step kernel
     int32_t num_bpart, num_cpart;
@@ -158,18 +155,19 @@ void launch_precondition(const DevProblem& P, double lambda, hipStream_t stream)
 // b = -g_p + sum_e W_e V^-1 g_l.
 void launch_rhs(const DevProblem& P, hipStream_t stream);
 // Operator partials of d = v1 + beta v2 (v2 may be null) into the run
-// partials: part32 (fp32) or part64. With pcg_iter >= 0 the launch is PCG
+// partials part64; fp32 reads the FP32 copies of G and V^-1 (arithmetic
+// stays FP64). With pcg_iter >= 0 the launch is PCG
 // iteration pcg_iter: each work-group first derives the iteration's
 // scalars (beta, stop) from the step kernel's partials.
-void launch_schur_apply(const DevProblem& P, bool fp32, const void* v1, const void* v2, int32_t pcg_iter,
+void launch_schur_apply(const DevProblem& P, bool fp32, const double* v1, const double* v2, int32_t pcg_iter,
                         int32_t fresh, hipStream_t stream);
 // PCG: r = b, z = M^-1 r, x = d = 0 and the state (implicit_schur.rs:522-560).
 void launch_pcg_init(const DevProblem& P, double rtol, double atol, int32_t max_iter, double lambda,
                      hipStream_t stream);
 // d = z + beta d, q = S d from the run partials, and d.q partials.
-void launch_pcg_pose(const DevProblem& P, bool fp32, hipStream_t stream);
+void launch_pcg_pose(const DevProblem& P, hipStream_t stream);
 // alpha, x += alpha d, r -= alpha q, z = M^-1 r and (r.z, r.r) partials.
-void launch_pcg_step(const DevProblem& P, bool fp32, int32_t pcg_iter, hipStream_t stream);
+void launch_pcg_step(const DevProblem& P, int32_t pcg_iter, hipStream_t stream);
 // The scalars of iteration pcg_iter (status, residual norm) without its work.
 void launch_pcg_status(const DevProblem& P, int32_t pcg_iter, int32_t fresh, hipStream_t stream);
 // True residual b - S x in FP64 (state: true_norm).

@@ -77,51 +77,55 @@ struct IdIndex {
 void build_runs(Model* mp) {
     Model& m = *mp;
     m.tile_perm.assign(m.num_edges, 0);
-    m.tile_run_base.assign(m.lm_tiles.size() + 1, 0);
-    std::vector<int32_t> run_pose, count;
+    m.s_pose.assign(m.num_edges, 0);
+    m.lm_store.assign(m.num_edges, 0);
+    std::vector<int32_t> run_pose, run_end, count;
     std::vector<uint64_t> keys;
     for (size_t t = 0; t < m.lm_tiles.size(); ++t) {
-        m.tile_run_base[t] = (int32_t)run_pose.size();
         const int32_t e0 = m.lm_edge_ptr[m.lm_tiles[t].lm_begin], e1 = m.lm_edge_ptr[m.lm_tiles[t].lm_end];
         const int32_t n = e1 - e0;
         if (n > kTileEdges) {
-            for (int32_t s = 0; s < n; ++s) run_pose.push_back(m.l_pose[e0 + s]);
-            continue;
-        }
-        // stable sort by pose: counting sort over the tile's pose range when it
-        // is narrow (landmarks ordered by first camera), else a key sort
-        int32_t lo = m.l_pose[e0], hi = lo;
-        for (int32_t k = 1; k < n; ++k) {
-            lo = std::min(lo, m.l_pose[e0 + k]);
-            hi = std::max(hi, m.l_pose[e0 + k]);
-        }
-        const int32_t range = hi - lo + 1;
-        if (range <= 8 * n + 64) {
-            count.assign(range + 1, 0);
-            for (int32_t k = 0; k < n; ++k) ++count[m.l_pose[e0 + k] - lo + 1];
-            for (int32_t r = 0; r < range; ++r) {
-                if (count[r + 1]) run_pose.push_back(lo + r);
-                count[r + 1] += count[r];
-            }
-            for (int32_t k = 0; k < n; ++k) m.tile_perm[e0 + count[m.l_pose[e0 + k] - lo]++] = (uint16_t)k;
+            for (int32_t s = 0; s < n; ++s) m.tile_perm[e0 + s] = 0;  // identity, not read
         } else {
-            keys.resize(n);
-            for (int32_t k = 0; k < n; ++k) keys[k] = ((uint64_t)(uint32_t)m.l_pose[e0 + k] << 16) | (uint64_t)k;
-            std::sort(keys.begin(), keys.end());  // by pose, then l-order index
-            for (int32_t s = 0; s < n; ++s) {
-                m.tile_perm[e0 + s] = (uint16_t)(keys[s] & 0xffff);
-                if (s == 0 || (keys[s] >> 16) != (keys[s - 1] >> 16)) run_pose.push_back((int32_t)(keys[s] >> 16));
+            // stable sort by pose: counting sort over the tile's pose range when
+            // it is narrow (landmarks ordered by first camera), else a key sort
+            int32_t lo = m.l_pose[e0], hi = lo;
+            for (int32_t k = 1; k < n; ++k) {
+                lo = std::min(lo, m.l_pose[e0 + k]);
+                hi = std::max(hi, m.l_pose[e0 + k]);
+            }
+            const int32_t range = hi - lo + 1;
+            if (range <= 8 * n + 64) {
+                count.assign(range + 1, 0);
+                for (int32_t k = 0; k < n; ++k) ++count[m.l_pose[e0 + k] - lo + 1];
+                for (int32_t r = 0; r < range; ++r) count[r + 1] += count[r];
+                for (int32_t k = 0; k < n; ++k) m.tile_perm[e0 + count[m.l_pose[e0 + k] - lo]++] = (uint16_t)k;
+            } else {
+                keys.resize(n);
+                for (int32_t k = 0; k < n; ++k) keys[k] = ((uint64_t)(uint32_t)m.l_pose[e0 + k] << 16) | (uint64_t)k;
+                std::sort(keys.begin(), keys.end());  // by pose, then l-order index
+                for (int32_t s = 0; s < n; ++s) m.tile_perm[e0 + s] = (uint16_t)(keys[s] & 0xffff);
             }
         }
+        for (int32_t s = 0; s < n; ++s) {
+            const int32_t k = n > kTileEdges ? s : m.tile_perm[e0 + s];
+            m.s_pose[e0 + s] = m.l_pose[e0 + k];
+            m.lm_store[e0 + k] = e0 + s;
+        }
+        for (int32_t s = 0; s < n; ++s)
+            if (s == n - 1 || m.s_pose[e0 + s + 1] != m.s_pose[e0 + s]) {
+                run_pose.push_back(m.s_pose[e0 + s]);
+                run_end.push_back(e0 + s);
+            }
     }
-    m.tile_run_base[m.lm_tiles.size()] = (int32_t)run_pose.size();
+    for (int32_t& pos : m.perm_cl) pos = m.lm_store[pos];
     m.num_runs = run_pose.size();
     m.pose_run_ptr.assign(m.num_pose_slots + 1, 0);
     for (int32_t p : run_pose) ++m.pose_run_ptr[p + 1];
     for (uint32_t s = 0; s < m.num_pose_slots; ++s) m.pose_run_ptr[s + 1] += m.pose_run_ptr[s];
     m.pose_runs.assign(run_pose.size(), 0);
     std::vector<int32_t> cursor(m.pose_run_ptr.begin(), m.pose_run_ptr.end() - 1);
-    for (size_t r = 0; r < run_pose.size(); ++r) m.pose_runs[cursor[run_pose[r]]++] = (int32_t)r;
+    for (size_t r = 0; r < run_pose.size(); ++r) m.pose_runs[cursor[run_pose[r]]++] = run_end[r];
 }
 
 uint32_t next_pow2(uint64_t v) {

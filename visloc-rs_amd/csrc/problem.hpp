@@ -77,7 +77,7 @@ struct Model {
     std::vector<int32_t> cam_edge_ptr;        // num_pose_slots + 1
     std::vector<int32_t> lm_edge_ptr;         // num_lm_slots + 1 (l-order)
     std::vector<int32_t> l_pose;              // l-order pose slot
-    std::vector<int32_t> perm_cl;             // c-order edge -> l-order position
+    std::vector<int32_t> perm_cl;             // c-order edge -> storage position (below)
 
     std::vector<CamTile> cam_tiles;
     std::vector<int32_t> cam_tile_ptr;  // num_pose_slots + 1
@@ -85,11 +85,14 @@ struct Model {
     int32_t max_lm_tile_edges = 0;
     // Camera runs of the landmark tiles: tile_perm[e] = tile-local l-order
     // index at sorted position e - (tile's first edge), sorting the tile's
-    // edges by pose; a run is a maximal group of equal poses in that order.
-    // Runs are numbered tile by tile in sorted order (tile_run_base); each
-    // pose lists its runs in ascending order (pose_run_ptr / pose_runs).
+    // edges by pose; a run is a maximal group of equal poses in that order
+    // and is named by the edge index of its last sorted position. Each pose
+    // lists its runs in ascending order (pose_run_ptr / pose_runs).
+    // Per-edge device data (G, h) is stored in this sorted order: s_pose is
+    // the pose at each storage position, lm_store the storage position of
+    // each l-order edge.
     std::vector<uint16_t> tile_perm;
-    std::vector<int32_t> tile_run_base;
+    std::vector<int32_t> s_pose, lm_store;
     std::vector<int32_t> pose_run_ptr, pose_runs;
     uint64_t num_runs = 0;
 };

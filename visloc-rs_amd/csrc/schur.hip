@@ -24,6 +24,7 @@
 // r.r partials). Scalars never leave the device between iterations.
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cmath>
 
 #include "device_common.hpp"
@@ -59,18 +60,6 @@ __device__ __forceinline__ void load6(const T* __restrict__ p, T v[6]) {
     v[5] = c.y;
 }
 
-__device__ __forceinline__ void load_X(const DevProblem& P, int l, float X[3]) {
-    const float4 x = P.X32[l];
-    X[0] = x.x;
-    X[1] = x.y;
-    X[2] = x.z;
-}
-__device__ __forceinline__ void load_X(const DevProblem& P, int l, double X[3]) {
-    X[0] = P.lms[3 * (size_t)l];
-    X[1] = P.lms[3 * (size_t)l + 1];
-    X[2] = P.lms[3 * (size_t)l + 2];
-}
-
 // A flag another work-group may set during this kernel, read once per
 // work-group so that all its threads take the same branch.
 __device__ __forceinline__ int block_flag(int32_t* p, int* sh) {
@@ -88,15 +77,16 @@ __device__ __forceinline__ void set_stop(PcgState* st, int status) {
 // init / restart kernels left in entry 0), the stop test on the recursive
 // residual (implicit_schur.rs:640-700) and beta = rho_k / rho_{k-1}.
 // Work-group 0 records them. Returns false when the solve has stopped.
+template <int NT>
 __device__ bool pcg_prologue(const DevProblem& P, int k, int fresh, double* beta_out, double* lds, int* flag) {
     PcgState* st = P.pcg;
     if (block_flag(&st->stop, flag)) return false;
     double v[2] = {0.0, 0.0};
-    for (int i = threadIdx.x; i < P.num_cpart; i += kBlock) {
+    for (int i = threadIdx.x; i < P.num_cpart; i += NT) {
         v[0] += P.cpart[2 * (size_t)i];
         v[1] += P.cpart[2 * (size_t)i + 1];
     }
-    block_sum_all<kBlock, double, 2>(v, lds);
+    block_sum_all<NT, double, 2>(v, lds);
     const double rz = v[0];
     double rn, beta = 0.0;
     int status = PCG_RUNNING;
@@ -156,87 +146,111 @@ __device__ __forceinline__ void edge_wvw(const double G[6], const double Vi[6], 
     add_schur_block<double>(C, X, 1.0, y);
 }
 
-// t = G (rho + omega x X) with d = v1 + beta v2 at pose p.
-template <typename T>
-__device__ __forceinline__ void edge_wtd(const T G[6], const T* __restrict__ v1, const T* __restrict__ v2, T beta,
-                                         int p, const T X[3], T t[3]) {
-    T d[6];
+// Operator input at pose p: d = v1 + beta v2 (v2 may be null).
+__device__ __forceinline__ void load_d(const double* __restrict__ v1, const double* __restrict__ v2, double beta,
+                                       int p, double d[6]) {
     load6(v1 + 6 * (size_t)p, d);
     if (v2) {
-        T w[6];
+        double w[6];
         load6(v2 + 6 * (size_t)p, w);
 #pragma unroll
         for (int i = 0; i < 6; ++i) d[i] = d[i] + beta * w[i];
     }
-    const T v[3] = {d[0] + d[4] * X[2] - d[5] * X[1], d[1] + d[5] * X[0] - d[3] * X[2],
-                    d[2] + d[3] * X[1] - d[4] * X[0]};
+}
+
+// t = W_e^T d = G (rho + omega x X).
+template <typename TS>
+__device__ __forceinline__ void edge_wtd(const TS Gs[6], const double d[6], const double X[3], double t[3]) {
+    const double G[6] = {(double)Gs[0], (double)Gs[1], (double)Gs[2], (double)Gs[3], (double)Gs[4], (double)Gs[5]};
+    const double v[3] = {d[0] + d[4] * X[2] - d[5] * X[1], d[1] + d[5] * X[0] - d[3] * X[2],
+                         d[2] + d[3] * X[1] - d[4] * X[0]};
     sym3_mul(G, v, t);
 }
 
-template <typename T, int OP>
-__global__ __launch_bounds__(kBlock) void k_schur_tile(DevProblem P, const T* __restrict__ v1,
-                                                       const T* __restrict__ v2, const T* __restrict__ Gl,
-                                                       const T* __restrict__ Vinv, T* __restrict__ part,
+// Inclusive segmented scan over the 64 lanes of a wave: a lane with
+// `head` set starts a new segment. Fixed combination order.
+template <int NV>
+__device__ __forceinline__ void wave_segscan(double v[NV], int head, int lane) {
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int hp = __shfl_up(head, off, 64);
+        const bool take = lane >= off && !head;
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            const double vp = __shfl_up(v[i], off, 64);
+            if (take) v[i] = vp + v[i];
+        }
+        if (lane >= off) head = head | hp;
+    }
+}
+
+// Landmark-tile pass. Storage type TS of G and V^-1 (float for the mixed
+// operator, double otherwise); arithmetic, partials and vectors in FP64.
+// Lane t takes the tile's sorted positions 4t .. 4t+3.
+constexpr int kTileThreads = 512;
+
+template <typename TS, int OP>
+__global__ __launch_bounds__(kTileThreads) void k_schur_tile(DevProblem P, const double* __restrict__ v1,
+                                                       const double* __restrict__ v2, const TS* __restrict__ Gl,
+                                                       const TS* __restrict__ Vinv, double* __restrict__ part,
                                                        int pcg_iter, int fresh) {
     constexpr int NV = OP == OP_PRECOND ? 21 : 6;
-    constexpr bool kStageG = sizeof(T) == 4;  // FP64 G is re-read from L2 instead
-    constexpr int kChunks = kTileEdges / 64;
-    constexpr int kWaves = kBlock / 64;
-    __shared__ T sX[kTileLandmarks * 3];
-    __shared__ T sU[kTileLandmarks * 3];
-    __shared__ T sT[kTileEdges * 3];  // stage-1 products (OP_PRECOND: V^-1 per landmark)
-    __shared__ T sG[kStageG ? kTileEdges * 6 : 1];
-    __shared__ int32_t sKey[kTileEdges];  // pose at each sorted position
-    __shared__ uint16_t sPerm[kTileEdges];
-    __shared__ uint8_t sLm[kTileEdges];  // tile-local landmark of each l-order edge
-    __shared__ T sHead[kChunks * NV];
-    __shared__ T sEnd[kChunks * NV];
-    __shared__ int32_t sStarts[kChunks], sEndRun[kChunks];
+    constexpr int kWaves = kTileThreads / 64;
+    constexpr int kPer = kTileEdges / kTileThreads;  // sorted positions per lane
+    constexpr int kWaveSpan = 64 * kPer;       // sorted positions per wave
+    __shared__ double sX[kTileLandmarks * 3];
+    __shared__ double sU[kTileLandmarks * 3];
+    __shared__ double sT[kTileEdges * 3];  // t_e by l-order index (OP_PRECOND: V^-1 per landmark)
+    __shared__ int32_t sKey[kTileEdges];   // pose at each sorted position
+    __shared__ uint8_t sLm[kTileEdges];    // tile-local landmark of each l-order edge
+    __shared__ double sHead[kWaves * NV], sEnd[kWaves * NV];
+    __shared__ int32_t sEndPos[kWaves];
     __shared__ double sRed[kWaves * 3];
-    __shared__ T sBig[3];
+    __shared__ double sBig[3];
     __shared__ int sFlag;
 
     const int tile = xcd_tile(blockIdx.x, P.num_lm_tiles);
     if (tile >= P.num_lm_tiles) return;
     if (P.wstate && !P.wstate[P.lm_tile_win[tile]].active) return;
-    double beta64 = 0.0;
+    double beta = 0.0;
     if (OP == OP_APPLY && pcg_iter >= 0) {
-        if (!pcg_prologue(P, pcg_iter, fresh, &beta64, sRed, &sFlag)) return;
+        if (!pcg_prologue<kTileThreads>(P, pcg_iter, fresh, &beta, sRed, &sFlag)) return;
     }
-    const T beta = (T)beta64;
     const int tid = threadIdx.x;
     const LmTile lt = P.lm_tiles[tile];
     const int e0 = P.lm_edge_ptr[lt.lm_begin], e1 = P.lm_edge_ptr[lt.lm_end];
     const int n = e1 - e0, nl = lt.lm_end - lt.lm_begin;
-    const int run0 = P.tile_run_base[tile];
 
     if (n > kTileEdges) {
         // One landmark with more edges than LDS holds. Its edges are in pose
-        // order and every pose appears once, so each edge is a run of its own.
+        // order and every pose appears once, so each edge ends a run.
         const int l = lt.lm_begin;
-        T X[3];
-        load_X(P, l, X);
+        const double X[3] = {P.lms[3 * (size_t)l], P.lms[3 * (size_t)l + 1], P.lms[3 * (size_t)l + 2]};
         if (OP == OP_APPLY || OP == OP_BACKSUB) {
-            T t[3] = {T(0), T(0), T(0)};
-            for (int e = e0 + tid; e < e1; e += kBlock) {
-                T G[6], te[3];
+            double t[3] = {0.0, 0.0, 0.0};
+            for (int e = e0 + tid; e < e1; e += kTileThreads) {
+                TS G[6];
+                double d[6], te[3];
                 load6(Gl + 6 * (size_t)e, G);
-                edge_wtd(G, v1, v2, beta, P.l_pose[e], X, te);
+                load_d(v1, v2, beta, P.s_pose[e], d);
+                edge_wtd(G, d, X, te);
                 t[0] += te[0];
                 t[1] += te[1];
                 t[2] += te[2];
             }
-            block_sum<kBlock, T, 3>(t, reinterpret_cast<T*>(sRed));
+            block_sum<kTileThreads, double, 3>(t, sRed);
             if (tid == 0) {
                 if (OP == OP_APPLY) {
-                    T vi[6];
-                    load6(Vinv + 6 * (size_t)l, vi);
+                    TS vs[6];
+                    load6(Vinv + 6 * (size_t)l, vs);
+                    const double vi[6] = {(double)vs[0], (double)vs[1], (double)vs[2],
+                                          (double)vs[3], (double)vs[4], (double)vs[5]};
                     sym3_mul(vi, t, sBig);
                 } else {
                     double dl[3] = {0.0, 0.0, 0.0};
                     if (P.lm_var[l] >= 0) {
-                        const double tt[3] = {P.gl[3 * (size_t)l] + (double)t[0], P.gl[3 * (size_t)l + 1] + (double)t[1],
-                                              P.gl[3 * (size_t)l + 2] + (double)t[2]};
+                        const double tt[3] = {P.gl[3 * (size_t)l] + t[0], P.gl[3 * (size_t)l + 1] + t[1],
+                                              P.gl[3 * (size_t)l + 2] + t[2]};
                         sym3_mul(P.Vinv64 + 6 * (size_t)l, tt, dl);
                         dl[0] = -dl[0];
                         dl[1] = -dl[1];
@@ -244,7 +258,7 @@ __global__ __launch_bounds__(kBlock) void k_schur_tile(DevProblem P, const T* __
                     }
                     for (int i = 0; i < 3; ++i) {
                         P.delta_l[3 * (size_t)l + i] = dl[i];
-                        P.lms_trial[3 * (size_t)l + i] = P.lms[3 * (size_t)l + i] + dl[i];
+                        P.lms_trial[3 * (size_t)l + i] = X[i] + dl[i];
                     }
                     P.tile_scalar[4 * (size_t)(P.num_cam_tiles + tile) + 3] =
                         sqrt(dl[0] * dl[0] + dl[1] * dl[1] + dl[2] * dl[2]);
@@ -252,100 +266,101 @@ __global__ __launch_bounds__(kBlock) void k_schur_tile(DevProblem P, const T* __
             }
             if (OP == OP_BACKSUB) return;
         } else if (OP == OP_RHS && tid == 0) {
-            sym3_mul(P.Vinv64 + 6 * (size_t)l, P.gl + 3 * (size_t)l, reinterpret_cast<double*>(sBig));
+            sym3_mul(P.Vinv64 + 6 * (size_t)l, P.gl + 3 * (size_t)l, sBig);
         }
         __syncthreads();
-        T u[3] = {sBig[0], sBig[1], sBig[2]};
+        const double u[3] = {sBig[0], sBig[1], sBig[2]};
         double vi[6];
         if (OP == OP_PRECOND)
             for (int i = 0; i < 6; ++i) vi[i] = P.Vinv64[6 * (size_t)l + i];
-        for (int s = tid; s < n; s += kBlock) {
-            T G[6], y[NV];
-            load6(Gl + 6 * (size_t)(e0 + s), G);
+        for (int s = tid; s < n; s += kTileThreads) {
+            TS Gs[6];
+            load6(Gl + 6 * (size_t)(e0 + s), Gs);
+            const double G[6] = {(double)Gs[0], (double)Gs[1], (double)Gs[2], (double)Gs[3], (double)Gs[4], (double)Gs[5]};
+            double y[NV];
             if constexpr (OP == OP_PRECOND) edge_wvw(G, vi, X, y);
             else edge_wu(G, u, X, y);
 #pragma unroll
-            for (int i = 0; i < NV; ++i) part[(size_t)(run0 + s) * NV + i] = y[i];
+            for (int i = 0; i < NV; ++i) part[(size_t)(e0 + s) * NV + i] = y[i];
         }
         return;
     }
 
-    // ---- stage 0: landmark positions, edge owners, the sorted order ----
+    // ---- stage 0: landmark positions, edge owners, this lane's positions ----
+    TS vpre[6];  // this owner's V^-1, loaded early
     if (tid < nl) {
         const int l = lt.lm_begin + tid;
-        T X[3];
-        load_X(P, l, X);
-        sX[3 * tid] = X[0];
-        sX[3 * tid + 1] = X[1];
-        sX[3 * tid + 2] = X[2];
+        if (OP == OP_APPLY) load6(Vinv + 6 * (size_t)l, vpre);
+        sX[3 * tid] = P.lms[3 * (size_t)l];
+        sX[3 * tid + 1] = P.lms[3 * (size_t)l + 1];
+        sX[3 * tid + 2] = P.lms[3 * (size_t)l + 2];
         const int b = P.lm_edge_ptr[l] - e0, e = P.lm_edge_ptr[l + 1] - e0;
         for (int k = b; k < e; ++k) sLm[k] = (uint8_t)tid;
         if (OP == OP_PRECOND)
             for (int i = 0; i < 6; ++i) sT[6 * tid + i] = P.Vinv64[6 * (size_t)l + i];
-        if (OP == OP_RHS) {
-            double u[3];
-            sym3_mul(P.Vinv64 + 6 * (size_t)l, P.gl + 3 * (size_t)l, u);
-            sU[3 * tid] = u[0];
-            sU[3 * tid + 1] = u[1];
-            sU[3 * tid + 2] = u[2];
-        }
+        if (OP == OP_RHS) sym3_mul(P.Vinv64 + 6 * (size_t)l, P.gl + 3 * (size_t)l, sU + 3 * tid);
     }
-    if (OP != OP_BACKSUB) {
-        for (int s = tid; s < n; s += kBlock) {
-            const int k = P.tile_perm[e0 + s];
-            sPerm[s] = (uint16_t)k;
-            sKey[s] = P.l_pose[e0 + k];
+    if (tid < kWaves) sEndPos[tid] = -1;
+    const int s0 = kPer * tid;
+    const int nv = max(0, min(kPer, n - s0));
+    int kk[kPer], key[kPer];
+    TS G[kPer][6];
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+        kk[i] = 0;
+        key[i] = -1 - (s0 + i);  // unique for positions past the end
+        if (i < nv) {
+            kk[i] = P.tile_perm[e0 + s0 + i];
+            key[i] = P.s_pose[e0 + s0 + i];
+            sKey[s0 + i] = key[i];
+            load6(Gl + 6 * (size_t)(e0 + s0 + i), G[i]);
         }
     }
     __syncthreads();
 
-    // ---- stage 1: t_e = W_e^T d_p per edge (and the FP32 G staged) ----
-    if (OP == OP_APPLY || OP == OP_BACKSUB || kStageG) {
-        for (int k = tid; k < n; k += kBlock) {
-            const int e = e0 + k;
-            T G[6];
-            load6(Gl + 6 * (size_t)e, G);
-            if (kStageG)
+    // ---- stage 1: t_e = W_e^T d_p; one d per pose run of this lane ----
+    if (OP == OP_APPLY || OP == OP_BACKSUB) {
+        int pcur = -1;
+        double d[6];
 #pragma unroll
-                for (int i = 0; i < 6; ++i) sG[6 * k + i] = G[i];
-            if (OP == OP_APPLY || OP == OP_BACKSUB) {
-                const int j = sLm[k];
-                const T X[3] = {sX[3 * j], sX[3 * j + 1], sX[3 * j + 2]};
-                T t[3];
-                edge_wtd(G, v1, v2, beta, P.l_pose[e], X, t);
-                sT[3 * k] = t[0];
-                sT[3 * k + 1] = t[1];
-                sT[3 * k + 2] = t[2];
+        for (int i = 0; i < kPer; ++i) {
+            if (i < nv) {
+                if (key[i] != pcur) {
+                    load_d(v1, v2, beta, key[i], d);
+                    pcur = key[i];
+                }
+                const int j = sLm[kk[i]];
+                const double X[3] = {sX[3 * j], sX[3 * j + 1], sX[3 * j + 2]};
+                double t[3];
+                edge_wtd(G[i], d, X, t);
+                sT[3 * kk[i]] = t[0];
+                sT[3 * kk[i] + 1] = t[1];
+                sT[3 * kk[i] + 2] = t[2];
             }
         }
         __syncthreads();
-    }
 
-    // ---- stage 2: per landmark, in edge order ----
-    if (OP == OP_APPLY || OP == OP_BACKSUB) {
+        // ---- stage 2: per landmark, in edge order ----
         double step = 0.0;
         if (tid < nl) {
             const int l = lt.lm_begin + tid;
             const int b = P.lm_edge_ptr[l] - e0, e = P.lm_edge_ptr[l + 1] - e0;
-            T t[3] = {T(0), T(0), T(0)};
+            double t[3] = {0.0, 0.0, 0.0};
             for (int k = b; k < e; ++k) {
                 t[0] += sT[3 * k];
                 t[1] += sT[3 * k + 1];
                 t[2] += sT[3 * k + 2];
             }
             if (OP == OP_APPLY) {
-                T vi[6], u[3];
-                load6(Vinv + 6 * (size_t)l, vi);
-                sym3_mul(vi, t, u);
-                sU[3 * tid] = u[0];
-                sU[3 * tid + 1] = u[1];
-                sU[3 * tid + 2] = u[2];
+                const double vi[6] = {(double)vpre[0], (double)vpre[1], (double)vpre[2],
+                                      (double)vpre[3], (double)vpre[4], (double)vpre[5]};
+                sym3_mul(vi, t, sU + 3 * tid);
             } else {
                 // delta_l = -V^-1 (g_l + sum_e W_e^T x_p) (normal_equations.rs:1551-1592)
                 double dl[3] = {0.0, 0.0, 0.0};
                 if (P.lm_var[l] >= 0) {
-                    const double tt[3] = {P.gl[3 * (size_t)l] + (double)t[0], P.gl[3 * (size_t)l + 1] + (double)t[1],
-                                          P.gl[3 * (size_t)l + 2] + (double)t[2]};
+                    const double tt[3] = {P.gl[3 * (size_t)l] + t[0], P.gl[3 * (size_t)l + 1] + t[1],
+                                          P.gl[3 * (size_t)l + 2] + t[2]};
                     sym3_mul(P.Vinv64 + 6 * (size_t)l, tt, dl);
                     dl[0] = -dl[0];
                     dl[1] = -dl[1];
@@ -354,108 +369,121 @@ __global__ __launch_bounds__(kBlock) void k_schur_tile(DevProblem P, const T* __
                 }
                 for (int i = 0; i < 3; ++i) {
                     P.delta_l[3 * (size_t)l + i] = dl[i];
-                    P.lms_trial[3 * (size_t)l + i] = P.lms[3 * (size_t)l + i] + dl[i];
+                    P.lms_trial[3 * (size_t)l + i] = sX[3 * tid + i] + dl[i];
                 }
             }
         }
         if (OP == OP_BACKSUB) {
-            step = block_max<kBlock>(step, sRed);
+            step = block_max<kTileThreads>(step, sRed);
             if (tid == 0) P.tile_scalar[4 * (size_t)(P.num_cam_tiles + tile) + 3] = step;
             return;
         }
         __syncthreads();
     }
 
-    // ---- stage 3: y_e per sorted position, summed per camera run ----
+    // ---- stage 3: y_e = W_e u_l per position, summed per camera run ----
     const int lane = tid & 63, wave = tid >> 6;
-    const int nch = (n + 63) >> 6;
-    for (int c = wave; c < nch; c += kWaves) {
-        const int s = 64 * c + lane;
-        const bool start = s < n && (s == 0 || sKey[s - 1] != sKey[s]);
-        const uint64_t m = __ballot(start);
-        if (lane == 0) {
-            sStarts[c] = __popcll(m);
-            sEndRun[c] = -1;
+    double first[NV], last[NV];
+#pragma unroll
+    for (int c = 0; c < NV; ++c) first[c] = last[c] = 0.0;
+    bool single = true;
+    int first_end = -1;
+    {
+        int ck = key[0];
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            if (i < nv) {
+                if (key[i] != ck) {  // the run of ck ends at s0 + i - 1
+                    if (single) {
+#pragma unroll
+                        for (int c = 0; c < NV; ++c) first[c] = last[c];
+                        first_end = s0 + i - 1;
+                        single = false;
+                    } else {
+#pragma unroll
+                        for (int c = 0; c < NV; ++c) part[(size_t)(e0 + s0 + i - 1) * NV + c] = last[c];
+                    }
+#pragma unroll
+                    for (int c = 0; c < NV; ++c) last[c] = 0.0;
+                    ck = key[i];
+                }
+                const int j = sLm[kk[i]];
+                const double X[3] = {sX[3 * j], sX[3 * j + 1], sX[3 * j + 2]};
+                const double Gd[6] = {(double)G[i][0], (double)G[i][1], (double)G[i][2],
+                                      (double)G[i][3], (double)G[i][4], (double)G[i][5]};
+                double y[NV];
+                if constexpr (OP == OP_PRECOND) {
+                    edge_wvw(Gd, sT + 6 * j, X, y);
+                } else {
+                    const double u[3] = {sU[3 * j], sU[3 * j + 1], sU[3 * j + 2]};
+                    edge_wu(Gd, u, X, y);
+                }
+#pragma unroll
+                for (int c = 0; c < NV; ++c) last[c] += y[c];
+            }
+        }
+    }
+    const int fkey = key[0];
+    const int lkey = nv > 0 ? key[nv - 1] : key[0];
+    const int pk = (s0 > 0 && s0 - 1 < n) ? sKey[s0 - 1] : INT_MIN;  // key before this lane
+    const int nk = (nv == kPer && s0 + kPer < n) ? sKey[s0 + kPer] : INT_MIN;  // key after
+    // segmented scan of the lanes' last-run sums (a segment starts where a run starts)
+    double S[NV];
+#pragma unroll
+    for (int c = 0; c < NV; ++c) S[c] = last[c];
+    const int head = (!single || pk != fkey) ? 1 : 0;
+    wave_segscan<NV>(S, head, lane);
+    double carry[NV];  // this wave's part of the first run before this lane
+#pragma unroll
+    for (int c = 0; c < NV; ++c) {
+        const double sp = __shfl_up(S[c], 1, 64);
+        carry[c] = (lane > 0 && pk == fkey) ? sp : 0.0;
+    }
+    // runs continuing from the previous wave end in this wave at most once
+    const int wbase = wave * kWaveSpan;
+    const bool wave_cont = wave > 0 && wbase < n && sKey[wbase - 1] == sKey[wbase];
+    const int cont_key = wave_cont ? sKey[wbase] : INT_MIN;
+    if (nv > 0) {
+        if (!single) {  // the first run ends inside this lane
+            if (wave_cont && fkey == cont_key) {
+#pragma unroll
+                for (int c = 0; c < NV; ++c) sEnd[wave * NV + c] = carry[c] + first[c];
+                sEndPos[wave] = first_end;
+            } else {
+#pragma unroll
+                for (int c = 0; c < NV; ++c) part[(size_t)(e0 + first_end) * NV + c] = carry[c] + first[c];
+            }
+        }
+        const int last_pos = s0 + nv - 1;
+        if (nk != lkey) {  // the last run ends at this lane's last position
+            if (wave_cont && lkey == cont_key) {
+#pragma unroll
+                for (int c = 0; c < NV; ++c) sEnd[wave * NV + c] = S[c];
+                sEndPos[wave] = last_pos;
+            } else {
+#pragma unroll
+                for (int c = 0; c < NV; ++c) part[(size_t)(e0 + last_pos) * NV + c] = S[c];
+            }
+        } else if (lane == 63) {  // continues into the next wave
+#pragma unroll
+            for (int c = 0; c < NV; ++c) sHead[wave * NV + c] = S[c];
         }
     }
     __syncthreads();
-    for (int c = wave; c < nch; c += kWaves) {
-        const int s = 64 * c + lane;
-        const bool valid = s < n;
-        const int key = valid ? sKey[s] : -1 - lane;
-        const bool start = valid && (s == 0 || sKey[s - 1] != key);
-        const uint64_t m = __ballot(start);
-        int before = 0;
-        for (int c2 = 0; c2 < c; ++c2) before += sStarts[c2];
-        const uint64_t le = lane == 63 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
-        const int run = run0 + before + __popcll(m & le) - 1;
-        T y[NV];
-        if (valid) {
-            const int k = sPerm[s];
-            const int j = sLm[k];
-            T G[6];
-            if (kStageG) {
+    // runs that crossed wave boundaries: earlier waves' tails in order
+    if (tid > 0 && tid < kWaves && sEndPos[tid] >= 0) {
+        const int w = tid;
+        const int rk = sKey[w * kWaveSpan];
+        int cs = w - 1;
+        while (cs > 0 && sKey[cs * kWaveSpan] == rk && sKey[cs * kWaveSpan - 1] == rk) --cs;
+        double acc[NV];
 #pragma unroll
-                for (int i = 0; i < 6; ++i) G[i] = sG[6 * k + i];
-            } else {
-                load6(Gl + 6 * (size_t)(e0 + k), G);
-            }
-            const T X[3] = {sX[3 * j], sX[3 * j + 1], sX[3 * j + 2]};
-            if constexpr (OP == OP_PRECOND) {
-                edge_wvw(G, reinterpret_cast<const double*>(sT) + 6 * j, X, y);
-            } else {
-                const T u[3] = {sU[3 * j], sU[3 * j + 1], sU[3 * j + 2]};
-                edge_wu(G, u, X, y);
-            }
-        } else {
+        for (int c = 0; c < NV; ++c) acc[c] = sHead[cs * NV + c];
+        for (int c2 = cs + 1; c2 < w; ++c2)
 #pragma unroll
-            for (int i = 0; i < NV; ++i) y[i] = T(0);
-        }
-        // inclusive segmented scan over the wave (equal keys are contiguous)
+            for (int c = 0; c < NV; ++c) acc[c] += sHead[c2 * NV + c];
 #pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const int ko = __shfl_up(key, off, 64);
-            const bool take = lane >= off && ko == key;
-#pragma unroll
-            for (int i = 0; i < NV; ++i) {
-                const T yo = __shfl_up(y[i], off, 64);
-                if (take) y[i] += yo;
-            }
-        }
-        int nk = __shfl_down(key, 1, 64);
-        if (lane == 63) nk = s + 1 < n ? sKey[s + 1] : -1000;
-        const bool is_end = valid && nk != key;
-        const bool cont = c > 0 && sKey[64 * c - 1] == sKey[64 * c];
-        const bool from_before = cont && key == sKey[64 * c];
-        if (is_end) {
-            if (!from_before) {
-#pragma unroll
-                for (int i = 0; i < NV; ++i) part[(size_t)run * NV + i] = y[i];
-            } else {
-#pragma unroll
-                for (int i = 0; i < NV; ++i) sEnd[c * NV + i] = y[i];
-                sEndRun[c] = run;
-            }
-        }
-        if (lane == 63 && valid && !is_end) {
-#pragma unroll
-            for (int i = 0; i < NV; ++i) sHead[c * NV + i] = y[i];
-        }
-    }
-    __syncthreads();
-    // runs that began in an earlier chunk: chunk tails in chunk order
-    if (tid < nch && sEndRun[tid] >= 0) {
-        const int c = tid;
-        int cs = c - 1;
-        while (cs > 0 && sStarts[cs] == 0) --cs;
-        T acc[NV];
-#pragma unroll
-        for (int i = 0; i < NV; ++i) acc[i] = sHead[cs * NV + i];
-        for (int c2 = cs + 1; c2 < c; ++c2)
-#pragma unroll
-            for (int i = 0; i < NV; ++i) acc[i] += sHead[c2 * NV + i];
-#pragma unroll
-        for (int i = 0; i < NV; ++i) part[(size_t)sEndRun[c] * NV + i] = acc[i] + sEnd[c * NV + i];
+        for (int c = 0; c < NV; ++c) part[(size_t)(e0 + sEndPos[w]) * NV + c] = acc[c] + sEnd[w * NV + c];
     }
 }
 
@@ -463,15 +491,15 @@ __global__ __launch_bounds__(kBlock) void k_schur_tile(DevProblem P, const T* __
 // Pose side: one wave per pose sums its runs' partials (lanes stride the
 // runs, then a butterfly that leaves the same sum in every lane).
 // ---------------------------------------------------------------------------
-template <typename T, int NV>
-__device__ __forceinline__ void pose_run_sum(const DevProblem& P, const T* __restrict__ part, int p, int lane,
+template <int NV>
+__device__ __forceinline__ void pose_run_sum(const DevProblem& P, const double* __restrict__ part, int p, int lane,
                                              double s[NV]) {
 #pragma unroll
     for (int i = 0; i < NV; ++i) s[i] = 0.0;
     for (int k = P.pose_run_ptr[p] + lane; k < P.pose_run_ptr[p + 1]; k += 64) {
-        const T* q = part + (size_t)P.pose_runs[k] * NV;
+        const double* q = part + (size_t)P.pose_runs[k] * NV;
 #pragma unroll
-        for (int i = 0; i < NV; ++i) s[i] += (double)q[i];
+        for (int i = 0; i < NV; ++i) s[i] += q[i];
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1)
@@ -533,7 +561,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_pose(DevProblem P) {
         return;
     }
     double s[6];
-    pose_run_sum<double, 6>(P, P.part64, p, lane, s);
+    pose_run_sum<6>(P, P.part64, p, lane, s);
     if (lane < 6) P.b[6 * (size_t)p + lane] = s[lane] - P.g[6 * (size_t)p + lane];
 }
 
@@ -548,7 +576,7 @@ __global__ __launch_bounds__(kBlock) void k_precond_pose(DevProblem P, double la
         return;
     }
     double s[21];
-    pose_run_sum<double, 21>(P, P.part21, p, lane, s);
+    pose_run_sum<21>(P, P.part21, p, lane, s);
     if (lane != 0) return;
     double M[36];
     for (int i = 0; i < 6; ++i)
@@ -589,10 +617,6 @@ __global__ __launch_bounds__(kSmallBlock) void k_pcg_init(DevProblem P, double r
             P.z[k] = z[i];
             P.d[k] = 0.0;
             P.q[k] = 0.0;
-            if (P.z32) {
-                P.z32[k] = (float)z[i];
-                P.d32[k] = 0.f;
-            }
             acc[0] += r[i] * z[i];
             acc[1] += r[i] * r[i];
         }
@@ -628,8 +652,7 @@ __global__ __launch_bounds__(kSmallBlock) void k_pcg_init(DevProblem P, double r
 
 // d = z + beta d and q = S d = (U + lambda I) d - sum of the pose's runs;
 // per work-group d.q partials.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_pcg_pose(DevProblem P, const T* __restrict__ part) {
+__global__ __launch_bounds__(kBlock) void k_pcg_pose(DevProblem P, const double* __restrict__ part) {
     __shared__ int sFlag;
     __shared__ double lds[(kBlock / 64) * 2];
     PcgState* st = P.pcg;
@@ -640,14 +663,13 @@ __global__ __launch_bounds__(kBlock) void k_pcg_pose(DevProblem P, const T* __re
     double v[2] = {0.0, 0.0};
     if (p < P.num_pose_slots && P.pose_var[p] >= 0) {
         double s[6];
-        pose_run_sum<T, 6>(P, part, p, lane, s);
+        pose_run_sum<6>(P, part, p, lane, s);
         const int c = lane < 6 ? lane : 0;
         const size_t k = 6 * (size_t)p + c;
         const double dc = P.z[k] + beta * P.d[k];
         const double q = pose_apply_u(P, p, c, dc, lambda, s[c]);
         if (lane < 6) {
             P.d[k] = dc;
-            if (P.d32) P.d32[k] = (float)dc;
             P.q[k] = q;
             v[0] = dc * q;
         }
@@ -691,7 +713,6 @@ __global__ __launch_bounds__(kBlock) void k_pcg_step(DevProblem P, int k) {
     double v[2] = {0.0, 0.0};
     if (valid) {
         P.z[i] = zc;
-        if (P.z32) P.z32[i] = (float)zc;
         v[0] = rc * zc;
         v[1] = rc * rc;
     }
@@ -704,7 +725,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_status(DevProblem P, int k, int 
     __shared__ double lds[(kBlock / 64) * 2];
     __shared__ int sFlag;
     double beta;
-    (void)pcg_prologue(P, k, fresh, &beta, lds, &sFlag);
+    (void)pcg_prologue<kBlock>(P, k, fresh, &beta, lds, &sFlag);
 }
 
 // True residual r_t = b - S x (part64 holds the tile pass run on x); with
@@ -718,7 +739,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_true_pose(DevProblem P, int rest
     double v[2] = {0.0, 0.0};
     if (p < P.num_pose_slots && P.pose_var[p] >= 0) {
         double s[6];
-        pose_run_sum<double, 6>(P, P.part64, p, lane, s);
+        pose_run_sum<6>(P, P.part64, p, lane, s);
         const int c = lane < 6 ? lane : 0;
         const size_t k = 6 * (size_t)p + c;
         const double xc = P.x[k];
@@ -731,10 +752,6 @@ __global__ __launch_bounds__(kBlock) void k_pcg_true_pose(DevProblem P, int rest
                 P.r[k] = rt;
                 P.z[k] = zc;
                 P.d[k] = 0.0;
-                if (P.z32) {
-                    P.z32[k] = (float)zc;
-                    P.d32[k] = 0.f;
-                }
                 v[1] = rt * zc;
             }
         }
@@ -780,29 +797,27 @@ __global__ __launch_bounds__(kBlock) void k_pcg_true_finish(DevProblem P, int re
 static inline int gather_blocks(const DevProblem& P) { return blocks_for(P.num_pose_slots, kBlock / 64); }
 
 void launch_precondition(const DevProblem& P, double lambda, hipStream_t s) {
-    hipLaunchKernelGGL((k_schur_tile<double, OP_PRECOND>), dim3(xcd_grid(P.num_lm_tiles)), dim3(kBlock), 0, s, P,
+    hipLaunchKernelGGL((k_schur_tile<double, OP_PRECOND>), dim3(xcd_grid(P.num_lm_tiles)), dim3(kTileThreads), 0, s, P,
                        (const double*)nullptr, (const double*)nullptr, (const double*)P.Gl64,
                        (const double*)P.Vinv64, P.part21, -1, 0);
     hipLaunchKernelGGL(k_precond_pose, dim3(gather_blocks(P)), dim3(kBlock), 0, s, P, lambda);
 }
 
 void launch_rhs(const DevProblem& P, hipStream_t s) {
-    hipLaunchKernelGGL((k_schur_tile<double, OP_RHS>), dim3(xcd_grid(P.num_lm_tiles)), dim3(kBlock), 0, s, P,
+    hipLaunchKernelGGL((k_schur_tile<double, OP_RHS>), dim3(xcd_grid(P.num_lm_tiles)), dim3(kTileThreads), 0, s, P,
                        (const double*)nullptr, (const double*)nullptr, (const double*)P.Gl64,
                        (const double*)P.Vinv64, P.part64, -1, 0);
     hipLaunchKernelGGL(k_rhs_pose, dim3(gather_blocks(P)), dim3(kBlock), 0, s, P);
 }
 
-void launch_schur_apply(const DevProblem& P, bool fp32, const void* v1, const void* v2, int32_t pcg_iter,
+void launch_schur_apply(const DevProblem& P, bool fp32, const double* v1, const double* v2, int32_t pcg_iter,
                         int32_t fresh, hipStream_t s) {
     if (fp32)
-        hipLaunchKernelGGL((k_schur_tile<float, OP_APPLY>), dim3(xcd_grid(P.num_lm_tiles)), dim3(kBlock), 0, s, P,
-                           (const float*)v1, (const float*)v2, (const float*)P.Gl32, (const float*)P.Vinv32,
-                           P.part32, pcg_iter, fresh);
+        hipLaunchKernelGGL((k_schur_tile<float, OP_APPLY>), dim3(xcd_grid(P.num_lm_tiles)), dim3(kTileThreads), 0, s, P,
+                           v1, v2, (const float*)P.Gl32, (const float*)P.Vinv32, P.part64, pcg_iter, fresh);
     else
-        hipLaunchKernelGGL((k_schur_tile<double, OP_APPLY>), dim3(xcd_grid(P.num_lm_tiles)), dim3(kBlock), 0, s, P,
-                           (const double*)v1, (const double*)v2, (const double*)P.Gl64, (const double*)P.Vinv64,
-                           P.part64, pcg_iter, fresh);
+        hipLaunchKernelGGL((k_schur_tile<double, OP_APPLY>), dim3(xcd_grid(P.num_lm_tiles)), dim3(kTileThreads), 0, s, P,
+                           v1, v2, (const double*)P.Gl64, (const double*)P.Vinv64, P.part64, pcg_iter, fresh);
 }
 
 void launch_pcg_init(const DevProblem& P, double rtol, double atol, int32_t max_iter, double lambda,
@@ -810,16 +825,11 @@ void launch_pcg_init(const DevProblem& P, double rtol, double atol, int32_t max_
     hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(kSmallBlock), 0, s, P, rtol, atol, max_iter, lambda);
 }
 
-void launch_pcg_pose(const DevProblem& P, bool fp32, hipStream_t s) {
-    if (fp32)
-        hipLaunchKernelGGL(k_pcg_pose<float>, dim3(gather_blocks(P)), dim3(kBlock), 0, s, P, (const float*)P.part32);
-    else
-        hipLaunchKernelGGL(k_pcg_pose<double>, dim3(gather_blocks(P)), dim3(kBlock), 0, s, P,
-                           (const double*)P.part64);
+void launch_pcg_pose(const DevProblem& P, hipStream_t s) {
+    hipLaunchKernelGGL(k_pcg_pose, dim3(gather_blocks(P)), dim3(kBlock), 0, s, P, (const double*)P.part64);
 }
 
-void launch_pcg_step(const DevProblem& P, bool fp32, int32_t pcg_iter, hipStream_t s) {
-    (void)fp32;
+void launch_pcg_step(const DevProblem& P, int32_t pcg_iter, hipStream_t s) {
     hipLaunchKernelGGL(k_pcg_step, dim3(blocks_for(P.num_pose_slots, (kBlock / 64) * kPosesPerWave)), dim3(kBlock),
                        0, s, P, pcg_iter);
 }
@@ -840,7 +850,7 @@ void launch_pcg_restart(const DevProblem& P, hipStream_t s) {
 }
 
 void launch_backsub(const DevProblem& P, hipStream_t s) {
-    hipLaunchKernelGGL((k_schur_tile<double, OP_BACKSUB>), dim3(xcd_grid(P.num_lm_tiles)), dim3(kBlock), 0, s, P,
+    hipLaunchKernelGGL((k_schur_tile<double, OP_BACKSUB>), dim3(xcd_grid(P.num_lm_tiles)), dim3(kTileThreads), 0, s, P,
                        (const double*)P.x, (const double*)nullptr, (const double*)P.Gl64, (const double*)P.Vinv64,
                        (double*)nullptr, -1, 0);
 }

@@ -207,7 +207,10 @@ struct vba_server {
             std::string msg;
             int32_t st = vba::prepare_batch(*ctx, models, valid, &pb, statuses.data(), &msg);
             models.clear();
-            if (st == VBA_OK) st = vba::run_batch(*ctx, pb, batch[0]->opts, nullptr, 0, results.data(), statuses.data());
+            const uint32_t cap = std::max<uint32_t>(batch[0]->opts.max_iterations, 1);
+            stats.resize((size_t)n * cap);
+            if (st == VBA_OK)
+                st = vba::run_batch(*ctx, pb, batch[0]->opts, stats.data(), cap, results.data(), statuses.data());
             for (uint32_t i = 0; i < n; ++i) {
                 Request& r = *batch[i];
                 if (st != VBA_OK) {
@@ -243,8 +246,9 @@ struct vba_server {
             }
             r->poses_out.resize(7 * r->prob.view.num_poses);
             r->lms_out.resize(3 * r->prob.view.num_landmarks);
-            r->status = vba_optimize(ctx, &r->prob.view, &r->opts, r->poses_out.data(), r->lms_out.data(), nullptr,
-                                     0, &r->result);
+            std::vector<vba_iteration_stats> stats(std::max<uint32_t>(r->opts.max_iterations, 1));
+            r->status = vba_optimize(ctx, &r->prob.view, &r->opts, r->poses_out.data(), r->lms_out.data(),
+                                     stats.data(), (uint32_t)stats.size(), &r->result);
             {
                 std::lock_guard<std::mutex> lock(mu);
                 hbm_in_use -= r->est_bytes;

@@ -22,10 +22,10 @@ namespace {
 
 enum BufId {
     B_POSES0, B_LMS0, B_POSES, B_LMS, B_POSES_T, B_LMS_T, B_SENSORS, B_POSE_VAR, B_LM_VAR, B_OBS_UV,
-    B_OBS_SENSOR, B_EDGE_LM, B_EDGE_OBS, B_PERM_CL, B_L_POSE, B_LM_EDGE_PTR, B_CAM_TILES,
-    B_CAM_TILE_PTR, B_LM_TILES, B_TILE_PERM, B_TILE_RUN_BASE, B_POSE_RUN_PTR, B_POSE_RUNS, B_GL64, B_GL32,
-    B_HL64, B_TILE27, B_U, B_G, B_V, B_GL, B_VINV64, B_VINV32, B_X32, B_MINV, B_VB, B_VX, B_VR, B_VZ, B_VD,
-    B_VQ, B_VZ32, B_VD32, B_PART64, B_PART32, B_PART21, B_BPART, B_CPART, B_PCG, B_DELTA_L, B_TILE_SCALAR,
+    B_OBS_SENSOR, B_EDGE_LM, B_EDGE_OBS, B_PERM_CL, B_L_POSE, B_LM_STORE, B_LM_EDGE_PTR, B_CAM_TILES,
+    B_CAM_TILE_PTR, B_LM_TILES, B_TILE_PERM, B_POSE_RUN_PTR, B_POSE_RUNS, B_GL64, B_GL32,
+    B_HL64, B_TILE27, B_U, B_G, B_V, B_GL, B_VINV64, B_VINV32, B_MINV, B_VB, B_VX, B_VR, B_VZ, B_VD,
+    B_VQ, B_PART64, B_PART21, B_BPART, B_CPART, B_PCG, B_DELTA_L, B_TILE_SCALAR,
     B_SCALARS, B_DENSE_BLOCK_PQ, B_DENSE_BLOCK_PTR, B_DENSE_TERMS, B_DENSE_VAR_SLOT, B_DENSE_S,
     B_DENSE_WORK, B_DENSE_STATUS, B_COUNT
 };
@@ -98,7 +98,7 @@ void build_dense_lists(const Model& m, DenseLists* out) {
     auto collect = [&](uint32_t l) {
         ends.clear();
         for (int32_t e = m.lm_edge_ptr[l]; e < m.lm_edge_ptr[l + 1]; ++e)
-            if (m.pose_var[m.l_pose[e]] >= 0) ends.push_back({m.pose_var[m.l_pose[e]], e});
+            if (m.pose_var[m.l_pose[e]] >= 0) ends.push_back({m.pose_var[m.l_pose[e]], m.lm_store[e]});
     };
     for (uint32_t l = 0; l < m.num_lm_slots; ++l) {
         if (m.lm_var[l] < 0) continue;
@@ -318,13 +318,13 @@ int32_t prepare(Context& ctx, Prepared& pr, std::string* msg) {
         upload(pr, B_EDGE_LM, m.edge_lm, s);
         upload(pr, B_EDGE_OBS, m.edge_obs, s);
         upload(pr, B_PERM_CL, m.perm_cl, s);
-        upload(pr, B_L_POSE, m.l_pose, s);
+        upload(pr, B_L_POSE, m.s_pose, s);
+        upload(pr, B_LM_STORE, m.lm_store, s);
         upload(pr, B_LM_EDGE_PTR, m.lm_edge_ptr, s);
         upload(pr, B_CAM_TILES, m.cam_tiles, s);
         upload(pr, B_CAM_TILE_PTR, m.cam_tile_ptr, s);
         upload(pr, B_LM_TILES, m.lm_tiles, s);
         upload(pr, B_TILE_PERM, m.tile_perm, s);
-        upload(pr, B_TILE_RUN_BASE, m.tile_run_base, s);
         upload(pr, B_POSE_RUN_PTR, m.pose_run_ptr, s);
         upload(pr, B_POSE_RUNS, m.pose_runs, s);
         hip_check(hipStreamSynchronize(s), "sync");
@@ -380,7 +380,7 @@ DevProblem bind(Prepared& pr, const vba_options& o, bool mixed, bool matrix_free
     P.kernel_kind = (int32_t)o.robust_kernel;
     P.kernel_scale = o.robust_scale;
     const size_t NP = m.num_pose_slots, NL = m.num_lm_slots, NE = m.num_edges, NT = m.cam_tiles.size();
-    const size_t NLT = m.lm_tiles.size(), NR = std::max<uint64_t>(m.num_runs, 1);
+    const size_t NLT = m.lm_tiles.size(), NR = std::max<uint64_t>(NE, 1);  // partials by edge index
     const size_t pose_blocks = (NP + kBlock - 1) / kBlock;
     P.poses = buf<double>(pr, B_POSES);
     P.lms = buf<double>(pr, B_LMS);
@@ -398,13 +398,13 @@ DevProblem bind(Prepared& pr, const vba_options& o, bool mixed, bool matrix_free
     P.edge_lm = buf<int32_t>(pr, B_EDGE_LM);
     P.edge_obs = buf<int32_t>(pr, B_EDGE_OBS);
     P.perm_cl = buf<int32_t>(pr, B_PERM_CL);
-    P.l_pose = buf<int32_t>(pr, B_L_POSE);
+    P.s_pose = buf<int32_t>(pr, B_L_POSE);
+    P.lm_store = buf<int32_t>(pr, B_LM_STORE);
     P.lm_edge_ptr = buf<int32_t>(pr, B_LM_EDGE_PTR);
     P.cam_tiles = buf<CamTile>(pr, B_CAM_TILES);
     P.cam_tile_ptr = buf<int32_t>(pr, B_CAM_TILE_PTR);
     P.lm_tiles = buf<LmTile>(pr, B_LM_TILES);
     P.tile_perm = buf<uint16_t>(pr, B_TILE_PERM);
-    P.tile_run_base = buf<int32_t>(pr, B_TILE_RUN_BASE);
     P.pose_run_ptr = buf<int32_t>(pr, B_POSE_RUN_PTR);
     P.pose_runs = buf<int32_t>(pr, B_POSE_RUNS);
     P.Gl64 = (double*)ensure(pr, B_GL64, 6 * NE * sizeof(double));
@@ -412,10 +412,6 @@ DevProblem bind(Prepared& pr, const vba_options& o, bool mixed, bool matrix_free
     if (mixed) {
         P.Gl32 = (float*)ensure(pr, B_GL32, 6 * NE * sizeof(float));
         P.Vinv32 = (float*)ensure(pr, B_VINV32, 6 * NL * sizeof(float));
-        P.X32 = (float4*)ensure(pr, B_X32, NL * sizeof(float4));
-        P.part32 = (float*)ensure(pr, B_PART32, 6 * NR * sizeof(float));
-        P.z32 = (float*)ensure(pr, B_VZ32, 6 * NP * sizeof(float));
-        P.d32 = (float*)ensure(pr, B_VD32, 6 * NP * sizeof(float));
     }
     P.tile27 = (double*)ensure(pr, B_TILE27, 27 * std::max<size_t>(NT, 1) * sizeof(double));
     P.U = (double*)ensure(pr, B_U, 21 * NP * sizeof(double));
@@ -490,10 +486,13 @@ int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stat
         // layout must move at least once (include/vba.h, vba_profile_query).
         // Pose vectors gathered per edge are counted once per pose.
         const double real = mixed ? 4.0 : 8.0;
-        const double bytes_apply = NE * (6.0 * real + 4.0 + 2.0) + NL * ((mixed ? 16.0 : 24.0) + 6.0 * real + 4.0) +
-                                   NR * 6.0 * real + NP * 12.0 * real + m.lm_tiles.size() * 12.0;
-        const double bytes_pcg_pose = NR * (4.0 + 6.0 * real) + NP * (8.0 + 96.0 + 168.0 + 96.0 + (mixed ? 24.0 : 0.0));
-        const double bytes_pcg_step = NP * (4.0 * 48.0 + 288.0 + 3.0 * 48.0 + (mixed ? 24.0 : 0.0));
+        // tile pass: per edge G (6 reals), its pose (4) and sorted position (2);
+        // per landmark X (24), V^-1 (6 reals), edge pointer (4); per run the
+        // FP64 partial (48); per pose z and d (96); per tile its bounds (8)
+        const double bytes_apply = NE * (6.0 * real + 4.0 + 2.0) + NL * (24.0 + 6.0 * real + 4.0) + NR * 48.0 +
+                                   NP * 96.0 + m.lm_tiles.size() * 8.0;
+        const double bytes_pcg_pose = NR * (4.0 + 48.0) + NP * (8.0 + 96.0 + 168.0 + 96.0);
+        const double bytes_pcg_step = NP * (4.0 * 48.0 + 288.0 + 3.0 * 48.0);
         const double bytes_linearize = m.num_obs * 16.0 + NE * (4.0 + 4.0 + 4.0 + 48.0 + 24.0 + (mixed ? 24.0 : 0.0)) +
                                        NL * 24.0 + NT * (16.0 + 27.0 * 8.0) + NP * 56.0;
         const double bytes_cost = m.num_obs * 16.0 + NE * 8.0 + NL * 24.0 + NT * (16.0 + 24.0) + NP * 56.0;
@@ -581,17 +580,16 @@ int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stat
                         for (int32_t kk = k; kk < k_end; ++kk) {
                             {
                                 ProfScope a(ctx, K_SCHUR_APPLY, bytes_apply);
-                                launch_schur_apply(P, mixed, mixed ? (const void*)P.z32 : (const void*)P.z,
-                                                   mixed ? (const void*)P.d32 : (const void*)P.d, kk,
+                                launch_schur_apply(P, mixed, P.z, P.d, kk,
                                                    fresh && kk == k, s);
                             }
                             {
                                 ProfScope b(ctx, K_PCG_POSE, bytes_pcg_pose);
-                                launch_pcg_pose(P, mixed, s);
+                                launch_pcg_pose(P, s);
                             }
                             {
                                 ProfScope c(ctx, K_PCG_STEP, bytes_pcg_step);
-                                launch_pcg_step(P, mixed, kk, s);
+                                launch_pcg_step(P, kk, s);
                             }
                         }
                         launch_pcg_status(P, k_end, fresh && k_end == k, s);
@@ -782,6 +780,8 @@ void merge_models(const std::vector<const Model*>& ms, Model* out, std::vector<W
             M.edge_lm.push_back((int32_t)(m.edge_lm[e] + lo));
             M.edge_obs.push_back((int32_t)(m.edge_obs[e] + oo));
             M.l_pose.push_back((int32_t)(m.l_pose[e] + po));
+            M.s_pose.push_back((int32_t)(m.s_pose[e] + po));
+            M.lm_store.push_back((int32_t)(m.lm_store[e] + eo));
             M.perm_cl.push_back((int32_t)(m.perm_cl[e] + eo));
         }
         for (uint32_t s2 = 0; s2 < m.num_pose_slots; ++s2) {
@@ -797,12 +797,12 @@ void merge_models(const std::vector<const Model*>& ms, Model* out, std::vector<W
             M.lm_tiles.push_back({(int32_t)(t.lm_begin + lo), (int32_t)(t.lm_end + lo)});
             lm_tile_win->push_back((int32_t)w);
         }
-        // camera runs: tile-local permutations are unchanged; run ids shift
+        // camera runs: tile-local permutations are unchanged; run ends shift
+        // with the edges
         M.tile_perm.insert(M.tile_perm.end(), m.tile_perm.begin(), m.tile_perm.end());
-        for (size_t t = 0; t < m.lm_tiles.size(); ++t) M.tile_run_base.push_back((int32_t)(m.tile_run_base[t] + ro));
         for (uint32_t s2 = 0; s2 < m.num_pose_slots; ++s2)
             M.pose_run_ptr.push_back((int32_t)(m.pose_run_ptr[s2] + rlo));
-        for (int32_t r : m.pose_runs) M.pose_runs.push_back((int32_t)(r + ro));
+        for (int32_t r : m.pose_runs) M.pose_runs.push_back((int32_t)(r + eo));
         ro += (int64_t)m.num_runs;
         rlo += (int64_t)m.pose_runs.size();
         M.max_lm_tile_edges = std::max(M.max_lm_tile_edges, m.max_lm_tile_edges);
@@ -837,7 +837,6 @@ void merge_models(const std::vector<const Model*>& ms, Model* out, std::vector<W
     M.cam_edge_ptr.push_back((int32_t)eo);
     M.cam_tile_ptr.push_back((int32_t)to);
     M.lm_edge_ptr.push_back((int32_t)eo);
-    M.tile_run_base.push_back((int32_t)ro);
     M.pose_run_ptr.push_back((int32_t)rlo);
     M.num_runs = (uint64_t)ro;
     dense_out->block_ptr.push_back((int32_t)term_off);
