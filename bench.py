@@ -171,6 +171,136 @@ def bench_windows(args):
     print(json.dumps(result), flush=True)
 
 
+def serve_pools(seed=11, n_pose=48, n_win=32, n_sub=6):
+    """Request pools of the served mixed stream: pose-only refinements (1
+    camera, 200-2000 fixed points, 1 px noise, 5% outliers), local windows
+    (as the windows workload) and submaps (100-300 cameras, 20k-120k
+    landmarks on a loop trajectory)."""
+    rng = np.random.default_rng(seed)
+    dense = vba.default_options(linear_solver=abi.SOLVER_DENSE, max_iterations=10, robust_kernel=abi.KERNEL_CAUCHY,
+                                robust_scale=1.0)
+    sub_opts = loop_options(abi.PRECISION_MIXED)
+    pose = [vba.synthesize(abi.SYNTH_POSE_ONLY, num_landmarks=int(rng.integers(200, 2001)), seed=5000 + i).problem
+            for i in range(n_pose)]
+    wins = window_problems(n_win, seed=seed)
+    subs = []
+    for i in range(n_sub):
+        cams = int(rng.integers(100, 301))
+        subs.append(vba.synthesize(abi.SYNTH_LOOP, num_cameras=cams, num_landmarks=int(rng.integers(20000, 120001)),
+                                   seed=7000 + i, scene_scale=200.0 * cams / 1200.0).problem)
+    return {"pose": (pose, dense), "window": (wins, dense), "submap": (subs, sub_opts)}
+
+
+def bench_serve(args):
+    import threading
+    t0 = time.perf_counter()
+    pools = serve_pools()
+    log(f"bench: serve pools generated in {time.perf_counter() - t0:.1f} s")
+    rng = np.random.default_rng(3)
+    n_req = rng.poisson(args.rate * args.serve_seconds)
+    arrivals = np.sort(rng.uniform(0.0, args.serve_seconds, n_req))
+    kinds = rng.choice(["pose", "window", "submap"], size=n_req, p=[0.6, 0.3, 0.1])
+    picks = [int(rng.integers(0, len(pools[k][0]))) for k in kinds]
+    srv = vba.Server(window_streams=2, submap_streams=2, batch_wait_seconds=0.002)
+    # warm the device paths once
+    for k in ("pose", "window", "submap"):
+        srv.wait(srv.submit(pools[k][0][0], pools[k][1], borrow=True))
+    tickets = []
+    lock = threading.Lock()
+    done = []
+
+    def waiter():
+        i = 0
+        while True:
+            with lock:
+                if i >= len(tickets):
+                    if submitting[0]:
+                        pass
+                    else:
+                        return
+                    t = None
+                else:
+                    t, kind, p = tickets[i]
+            if t is None:
+                time.sleep(0.0005)
+                continue
+            r = srv.wait(t)
+            done.append((kind, r.status, r.latency, p.num_observations * r.result.num_iterations))
+            i += 1
+
+    submitting = [True]
+    th = threading.Thread(target=waiter)
+    th.start()
+    start = time.perf_counter()
+    for at, kind, pick in zip(arrivals, kinds, picks):
+        now = time.perf_counter() - start
+        if at > now:
+            time.sleep(at - now)
+        prob, opts = pools[kind][0][pick], pools[kind][1]
+        t = srv.submit(prob, opts, borrow=True)
+        with lock:
+            tickets.append((t, kind, prob))
+    submitted_at = time.perf_counter() - start
+    with lock:
+        submitting[0] = False
+    th.join()
+    wall = time.perf_counter() - start
+    counters = srv.counters()
+    srv.close()
+    lat = np.array([d[2] for d in done])
+    ok = all(d[1] == abi.OK for d in done)
+    per_kind = {}
+    for k in ("pose", "window", "submap"):
+        lk = np.array([d[2] for d in done if d[0] == k])
+        if len(lk):
+            per_kind[k] = {"count": int(len(lk)), "p50_ms": round(1e3 * float(np.percentile(lk, 50)), 2),
+                           "p99_ms": round(1e3 * float(np.percentile(lk, 99)), 2)}
+    obs_iters = sum(d[3] for d in done)
+    log(f"bench: {len(done)} requests submitted over {submitted_at:.2f} s, all done at {wall:.2f} s; "
+        f"batches {counters['batches']} ({counters['batched_windows']} windows); per kind {per_kind}")
+    # one-at-a-time plain FP64 solving of a sample, for comparison
+    dev = vba.Device(0)
+    solo = {}
+    for k, cnt in (("pose", 12), ("window", 6), ("submap", 2)):
+        probs, opts = pools[k]
+        o = vba.default_options(**{f: getattr(opts, f) for f, _ in abi.Options._fields_})
+        o.precision = abi.PRECISION_FP64
+        ts = []
+        for p in probs[:cnt]:
+            t1 = time.perf_counter()
+            dev.optimize(p, o)
+            ts.append(time.perf_counter() - t1)
+        solo[k] = float(np.mean(ts))
+    dev.close()
+    mix_mean = 0.6 * solo["pose"] + 0.3 * solo["window"] + 0.1 * solo["submap"]
+    result = {
+        "metric": METRIC,
+        "value": float(np.percentile(lat, 99)),
+        "unit": "s (p99 latency)",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 1,
+        "ms_per_step": 1000.0 * wall,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (windows, pose-only) / f32+f64 (submaps)",
+        "data": "synthetic",
+        "config": {"workload": f"served mixed stream: Poisson {args.rate:g} req/s for {args.serve_seconds:g} s, "
+                               "60/30/10 pose-only / local windows / submaps, request pools of 48/32/6 problems",
+                   "requests": len(done)},
+        "all_ok": ok,
+        "p50_s": float(np.percentile(lat, 50)),
+        "p99_s": float(np.percentile(lat, 99)),
+        "per_kind": per_kind,
+        "solves_per_s": len(done) / wall,
+        "offered_req_per_s": args.rate,
+        "served_obs_iters_per_s": obs_iters / wall,
+        "one_at_a_time_fp64": {"mean_s": solo, "solves_per_s": 1.0 / mix_mean},
+    }
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -182,8 +312,11 @@ def main():
     ap.add_argument("--pcg-policy", choices=["accept", "reject"], default="accept",
                     help="at the PCG iteration cap: use the iterate (GPU backend) or reject the step (CPU "
                          "matrix-free entry)")
-    ap.add_argument("--workload", choices=["loop", "windows"], default="loop",
-                    help="loop: the 3.5M-observation single problem (default); windows: a batch of local windows")
+    ap.add_argument("--workload", choices=["loop", "windows", "serve"], default="loop",
+                    help="loop: the 3.5M-observation single problem (default); windows: a batch of local windows; "
+                         "serve: the request loop under a Poisson stream")
+    ap.add_argument("--rate", type=float, default=2000.0, help="requests per second (--workload serve)")
+    ap.add_argument("--serve-seconds", type=float, default=1.0, help="arrival window (--workload serve)")
     ap.add_argument("--windows", type=int, default=256, help="windows in the batch (--workload windows)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -192,6 +325,9 @@ def main():
         sys.exit(2)
     if args.workload == "windows":
         bench_windows(args)
+        return
+    if args.workload == "serve":
+        bench_serve(args)
         return
 
     t0 = time.perf_counter()

@@ -334,6 +334,8 @@ typedef struct vba_server_config {
     uint32_t submap_streams;      /* single-problem solves in flight at once */
     uint64_t hbm_budget_bytes;    /* admission limit on resident work */
     double batch_wait_seconds;    /* how long a window batch waits to fill */
+    uint32_t build_threads;       /* host threads laying out a window batch */
+    uint32_t reserved;
 } vba_server_config;
 typedef struct vba_server vba_server;
 void vba_server_default_config(vba_server_config* config);
@@ -342,6 +344,10 @@ vba_server* vba_server_create(const vba_server_config* config);
 /* Queue a solve (the problem is copied). Returns a ticket > 0, or 0 on
  * failure (vba_last_error(NULL)). */
 uint64_t vba_server_submit(vba_server* server, const vba_problem* problem, const vba_options* options);
+/* As vba_server_submit without the copy: the caller keeps every array of
+ * `problem` valid and unchanged until the ticket is retired. */
+uint64_t vba_server_submit_borrowed(vba_server* server, const vba_problem* problem,
+                                    const vba_options* options);
 /* Wait up to timeout_seconds for a ticket. VBA_PENDING while it runs;
  * otherwise the solve's status (the ticket is then retired), with outputs as
  * vba_optimize writes them and the latency from submission to completion. */

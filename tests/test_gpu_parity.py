@@ -60,21 +60,25 @@ def test_loop_matrix_free_parity(device, policy):
 
 
 def test_mixed_precision_matches_fp64(device):
-    """At the PCG iteration cap the step is the last iterate, as the
-    reference GPU backend takes it (crates/ba-gpu/src/solver.rs:497-508):
-    under the reject-at-cap policy of the CPU matrix-free entry the two
-    precisions reach the cap in different iterations, so their LM paths can
-    part after a rejected step."""
+    """Mixed precision stores G and V^-1 in FP32 (the operator's arithmetic
+    stays FP64) and refines against the FP64 true residual. With PCG run to
+    its tolerance, both precisions take the same LM path and agree to the
+    bar. (At a PCG iteration cap both solves stop on truncated Krylov
+    iterates, whose rounding sensitivity the bench reports separately as
+    rel_cost_vs_fp64.)"""
     s = vba.synthesize(abi.SYNTH_LOOP, num_cameras=80, num_landmarks=6000, seed=3, scene_scale=16.0)
-    o = vba.default_options(linear_solver=abi.SOLVER_MATRIX_FREE, max_iterations=15, max_pcg_iterations=100,
-                            pcg_relative_tolerance=1e-4, robust_kernel=abi.KERNEL_HUBER, robust_scale=1.0,
+    o = vba.default_options(linear_solver=abi.SOLVER_MATRIX_FREE, max_iterations=15, max_pcg_iterations=500,
+                            pcg_relative_tolerance=1e-6, robust_kernel=abi.KERNEL_HUBER, robust_scale=1.0,
                             pcg_maxit_policy=abi.PCG_MAXIT_ACCEPT_ITERATE)
     fp64 = device.optimize(s.problem, o)
     o.precision = abi.PRECISION_MIXED
     mixed = device.optimize(s.problem, o)
     assert fp64.status == mixed.status == abi.OK, mixed.message
+    assert mixed.accepted == fp64.accepted
     rel = abs(mixed.result.final_cost - fp64.result.final_cost) / fp64.result.final_cost
     assert rel <= MIXED_COST_RTOL, rel
+    d = np.linalg.norm(vba.camera_centres(mixed.poses) - vba.camera_centres(fp64.poses), axis=1).max()
+    assert d <= CENTRE_TOL * s.scene_extent, (d, s.scene_extent)
 
 
 def test_stereo_rig_window_parity(device):

@@ -36,7 +36,7 @@ def mix():
 def test_served_results_equal_solo(device):
     reqs = mix()
     with vba.Server(window_streams=2, submap_streams=2, batch_wait_seconds=0.005) as srv:
-        tickets = [srv.submit(p, o) for p, o in reqs]
+        tickets = [srv.submit(p, o, borrow=(i % 2 == 1)) for i, (p, o) in enumerate(reqs)]
         served = [srv.wait(t) for t in tickets]
         counters = srv.counters()
     assert counters["completed"] == len(reqs)

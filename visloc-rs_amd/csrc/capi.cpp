@@ -81,6 +81,8 @@ vba_context* vba_create(int32_t device) {
         return nullptr;
     }
     ctx->device = device;
+    if (hipDeviceGetAttribute(&ctx->num_cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        ctx->num_cus = 0;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&ctx->ev_start) != hipSuccess || hipEventCreate(&ctx->ev_stop) != hipSuccess) {
         vba::set_thread_error("vba_create: stream/event creation failed");

@@ -148,13 +148,50 @@ __global__ void k_landmark_reduce(DevProblem P) {
     for (int i = 0; i < 6; ++i) P.V[6 * (size_t)l + i] = V[i];
 #pragma unroll
     for (int i = 0; i < 3; ++i) P.gl[3 * (size_t)l + i] = g[i];
+    if (P.Vm) {  // V of the FP32-stored G, the one the mixed operator pairs with them
+        double Vm[6] = {0, 0, 0, 0, 0, 0};
+        for (int e = P.lm_edge_ptr[l]; e < P.lm_edge_ptr[l + 1]; ++e) {
+            const float* G = P.Gl32 + 6 * (size_t)P.lm_store[e];
+#pragma unroll
+            for (int i = 0; i < 6; ++i) Vm[i] += (double)G[i];
+        }
+#pragma unroll
+        for (int i = 0; i < 6; ++i) P.Vm[6 * (size_t)l + i] = Vm[i];
+    }
+}
+
+// (V + lambda I)^-1 of a symmetric 3x3 by cofactors; zero when the
+// determinant is exactly zero or not finite.
+__device__ __forceinline__ void damped_inverse3(const double* V, double lambda, double inv[6]) {
+    double a = V[0], b = V[1], c = V[2], d = V[3], e = V[4], f = V[5];
+    if (lambda > 0.0) {
+        a += lambda;
+        d += lambda;
+        f += lambda;
+    }
+    // symmetric [[a b c][b d e][c e f]]
+    const double c00 = d * f - e * e, c01 = c * e - b * f, c02 = b * e - c * d;
+    const double det = a * c00 + b * c01 + c * c02;
+    for (int i = 0; i < 6; ++i) inv[i] = 0.0;
+    if (det != 0.0 && isfinite(det)) {
+        inv[0] = c00 / det;
+        inv[1] = c01 / det;
+        inv[2] = c02 / det;
+        inv[3] = (a * f - c * c) / det;
+        inv[4] = (b * c - a * e) / det;
+        inv[5] = (a * d - b * b) / det;
+    }
 }
 
 // Damped landmark inverse (V + lambda I)^-1 by cofactors, zero when the
 // determinant is exactly zero or the landmark is fixed (the landmark then
 // drops out of the Schur complement and back-substitution, as try_inverse
-// returning None does in normal_equations.rs:1432-1445); an FP32 copy of
-// the inverse for the mixed-precision operator.
+// returning None does in normal_equations.rs:1432-1445). The mixed operator
+// gets the inverse of the V summed from its FP32-stored G, so that its
+// W V^-1 W^T is the exact Schur term of one consistent set of blocks:
+// pairing FP32 G with the FP64 inverse would leave rounding errors of G
+// along a poorly observed landmark's depth direction amplified by V's
+// condition number.
 __global__ void k_landmark_prepare(DevProblem P, double lambda) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= P.num_lm_slots) return;
@@ -163,32 +200,15 @@ __global__ void k_landmark_prepare(DevProblem P, double lambda) {
         if (!w.active) return;
         lambda = w.lambda;
     }
+    const bool var = P.lm_var[l] >= 0;
     double inv[6] = {0, 0, 0, 0, 0, 0};
-    if (P.lm_var[l] >= 0) {
-        const double* V = P.V + 6 * (size_t)l;
-        double a = V[0], b = V[1], c = V[2], d = V[3], e = V[4], f = V[5];
-        if (lambda > 0.0) {
-            a += lambda;
-            d += lambda;
-            f += lambda;
-        }
-        // symmetric [[a b c][b d e][c e f]]
-        const double c00 = d * f - e * e, c01 = c * e - b * f, c02 = b * e - c * d;
-        const double det = a * c00 + b * c01 + c * c02;
-        if (det != 0.0 && isfinite(det)) {
-            inv[0] = c00 / det;
-            inv[1] = c01 / det;
-            inv[2] = c02 / det;
-            inv[3] = (a * f - c * c) / det;
-            inv[4] = (b * c - a * e) / det;
-            inv[5] = (a * d - b * b) / det;
-        }
-    }
+    if (var) damped_inverse3(P.V + 6 * (size_t)l, lambda, inv);
 #pragma unroll
     for (int i = 0; i < 6; ++i) P.Vinv64[6 * (size_t)l + i] = inv[i];
-    if (P.Vinv32) {
+    if (P.VinvM) {
+        if (var) damped_inverse3(P.Vm + 6 * (size_t)l, lambda, inv);
 #pragma unroll
-        for (int i = 0; i < 6; ++i) P.Vinv32[6 * (size_t)l + i] = (float)inv[i];
+        for (int i = 0; i < 6; ++i) P.VinvM[6 * (size_t)l + i] = inv[i];
     }
 }
 
@@ -325,15 +345,21 @@ __global__ __launch_bounds__(kSmallBlock) void k_scalars_finalize(DevProblem P, 
 // ---------------------------------------------------------------------------
 // One 6x6 block (p, q) of S = U + lambda I - W V^-1 W^T, written at the
 // variable indices local to its system (and mirrored when p != q).
-template <int NT>
-__device__ void dense_assemble_block(const DevProblem& P, const DensePairs& D, int b, double* S, int n, int var_begin,
-                                     double lambda, double* lds) {
+// Lanes that assemble one 6x6 block of S together.
+constexpr int kBlockLanes = 8;
+
+// One 6x6 block (p, q) of S = U + lambda I - W V^-1 W^T, written at the
+// variable indices local to its system (and mirrored when p != q), by a
+// group of kBlockLanes lanes: each sums a strided share of the block's terms,
+// a butterfly within the group combines them in a fixed order.
+__device__ void dense_assemble_group(const DevProblem& P, const DensePairs& D, int b, double* S, int n, int var_begin,
+                                     double lambda, int g) {
     const int sp = D.block_pq[2 * b], sq = D.block_pq[2 * b + 1];
     const int vp = P.pose_var[sp] - var_begin, vq = P.pose_var[sq] - var_begin;
     double acc[36];
 #pragma unroll
     for (int i = 0; i < 36; ++i) acc[i] = 0.0;
-    for (int k = D.block_ptr[b] + (int)threadIdx.x; k < D.block_ptr[b + 1]; k += NT) {
+    for (int k = D.block_ptr[b] + g; k < D.block_ptr[b + 1]; k += kBlockLanes) {
         const int l = D.terms[3 * (size_t)k], ep = D.terms[3 * (size_t)k + 1], eq = D.terms[3 * (size_t)k + 2];
         const double* Vi = P.Vinv64 + 6 * (size_t)l;
         const double vi[9] = {Vi[0], Vi[1], Vi[2], Vi[1], Vi[3], Vi[4], Vi[2], Vi[4], Vi[5]};
@@ -342,22 +368,32 @@ __device__ void dense_assemble_block(const DevProblem& P, const DensePairs& D, i
         const double a[9] = {A[0], A[1], A[2], A[1], A[3], A[4], A[2], A[4], A[5]};
         const double bb[9] = {B[0], B[1], B[2], B[1], B[3], B[4], B[2], B[4], B[5]};
         double T[9], C[9];
+#pragma unroll
         for (int i = 0; i < 3; ++i)
+#pragma unroll
             for (int j = 0; j < 3; ++j) T[3 * i + j] = a[3 * i] * vi[j] + a[3 * i + 1] * vi[3 + j] + a[3 * i + 2] * vi[6 + j];
+#pragma unroll
         for (int i = 0; i < 3; ++i)
+#pragma unroll
             for (int j = 0; j < 3; ++j) C[3 * i + j] = T[3 * i] * bb[j] + T[3 * i + 1] * bb[3 + j] + T[3 * i + 2] * bb[6 + j];
         const double* X = P.lms + 3 * (size_t)l;
         const double K[9] = {0.0, -X[2], X[1], X[2], 0.0, -X[0], -X[1], X[0], 0.0};
         double CK[9], KC[9], KCK[9];
+#pragma unroll
         for (int i = 0; i < 3; ++i)
+#pragma unroll
             for (int j = 0; j < 3; ++j) {
                 CK[3 * i + j] = C[3 * i] * K[j] + C[3 * i + 1] * K[3 + j] + C[3 * i + 2] * K[6 + j];
                 KC[3 * i + j] = K[3 * i] * C[j] + K[3 * i + 1] * C[3 + j] + K[3 * i + 2] * C[6 + j];
             }
+#pragma unroll
         for (int i = 0; i < 3; ++i)
+#pragma unroll
             for (int j = 0; j < 3; ++j) KCK[3 * i + j] = KC[3 * i] * K[j] + KC[3 * i + 1] * K[3 + j] + KC[3 * i + 2] * K[6 + j];
         // [[C, -C K], [K C, -K C K]]
+#pragma unroll
         for (int i = 0; i < 3; ++i)
+#pragma unroll
             for (int j = 0; j < 3; ++j) {
                 acc[6 * i + j] += C[3 * i + j];
                 acc[6 * i + 3 + j] -= CK[3 * i + j];
@@ -365,80 +401,194 @@ __device__ void dense_assemble_block(const DevProblem& P, const DensePairs& D, i
                 acc[6 * (3 + i) + 3 + j] -= KCK[3 * i + j];
             }
     }
-    block_sum<NT, double, 36>(acc, lds);
-    if (threadIdx.x == 0) {
-        if (sp == sq) {
-            const double* U = P.U + 21 * (size_t)sp;
-            for (int i = 0; i < 6; ++i)
-                for (int j = 0; j < 6; ++j) {
-                    double u = U[i <= j ? up6(i, j) : up6(j, i)] + (i == j && lambda > 0.0 ? lambda : 0.0);
-                    S[(size_t)(6 * vp + i) * n + 6 * vq + j] = u - acc[6 * i + j];
-                }
-        } else {
-            for (int i = 0; i < 6; ++i)
-                for (int j = 0; j < 6; ++j) {
-                    S[(size_t)(6 * vp + i) * n + 6 * vq + j] = -acc[6 * i + j];
-                    S[(size_t)(6 * vq + j) * n + 6 * vp + i] = -acc[6 * i + j];
-                }
+#pragma unroll
+    for (int off = 1; off < kBlockLanes; off <<= 1)
+#pragma unroll
+        for (int i = 0; i < 36; ++i) acc[i] += __shfl_xor(acc[i], off, kBlockLanes);
+    // every lane holds the sum; lane g writes entries g, g + 8, ...
+    const double* U = P.U + 21 * (size_t)sp;
+#pragma unroll
+    for (int idx0 = 0; idx0 < 36; idx0 += kBlockLanes) {
+        const int idx = idx0 + g;
+        if (idx < 36) {
+            const int i = idx / 6, j = idx % 6;
+            double v = 0.0;
+#pragma unroll
+            for (int t = 0; t < 36; ++t)
+                if (t == idx) v = acc[t];
+            if (sp == sq) {
+                const double u = U[i <= j ? up6(i, j) : up6(j, i)] + (i == j && lambda > 0.0 ? lambda : 0.0);
+                S[(size_t)(6 * vp + i) * n + 6 * vq + j] = u - v;
+            } else {
+                S[(size_t)(6 * vp + i) * n + 6 * vq + j] = -v;
+                S[(size_t)(6 * vq + j) * n + 6 * vp + i] = -v;
+            }
         }
     }
 }
 
 __global__ __launch_bounds__(kBlock) void k_dense_assemble(DevProblem P, DensePairs D, double* S, int n,
                                                            double lambda) {
-    __shared__ double lds[(kBlock / 64) * 36];
-    dense_assemble_block<kBlock>(P, D, blockIdx.x, S, n, 0, lambda, lds);
+    const int b = blockIdx.x * (kBlock / kBlockLanes) + (int)threadIdx.x / kBlockLanes;
+    if (b >= D.num_blocks) return;
+    dense_assemble_group(P, D, b, S, n, 0, lambda, threadIdx.x % kBlockLanes);
 }
 
-// Cholesky (left-looking, lower triangle in place), then forward/back
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// S(r, c) of an n x n row-major matrix, extended by the identity to whole
+// 16 x 16 blocks.
+__device__ __forceinline__ double padded_at(const double* S, int n, int r, int c) {
+    return (r < n && c < n) ? S[(size_t)r * n + c] : (r == c ? 1.0 : 0.0);
+}
+
+// Blocked right-looking Cholesky S = L L^T in place (lower triangle,
+// row-major), 16 x 16 blocks: the diagonal block is factored in LDS, the
+// panel below it by one forward substitution per row, and the trailing
+// blocks are updated as C -= L_i L_j^T on FP64 MFMA (v_mfma_f64_16x16x4:
+// operand lane l holds A[l%16][l/16] and B[l/16][l%16]; result register q of
+// lane l is C[4q + l/16][l%16]). One work-group of NT >= 256 threads.
+// *fail = 1 on a pivot that is not positive and finite.
+template <int NT>
+__device__ void chol_blocked(double* S, int n, double* dblk, int* fail) {
+    constexpr int NW = NT / 64;
+    const int nb = (n + 15) >> 4;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ti = threadIdx.x >> 4, tc = threadIdx.x & 15;  // element of the diagonal block (threads < 256)
+    for (int k = 0; k < nb; ++k) {
+        const int k0 = 16 * k;
+        if (threadIdx.x < 256) dblk[ti * 17 + tc] = padded_at(S, n, k0 + ti, k0 + tc);
+        __syncthreads();
+        for (int j = 0; j < 16; ++j) {
+            const double d = dblk[j * 17 + j];
+            if (!(d > 0.0) || !isfinite(d)) {
+                if (threadIdx.x == 0) *fail = 1;
+                break;
+            }
+            const double ljj = sqrt(d);
+            double v = 0.0;
+            const bool col = threadIdx.x < 256 && tc == j && ti >= j;
+            if (col) v = ti == j ? ljj : dblk[ti * 17 + j] / ljj;
+            __syncthreads();
+            if (col) dblk[ti * 17 + j] = v;
+            __syncthreads();
+            if (threadIdx.x < 256 && tc > j && tc <= ti) dblk[ti * 17 + tc] -= dblk[ti * 17 + j] * dblk[tc * 17 + j];
+            __syncthreads();
+        }
+        __syncthreads();
+        if (*fail) return;
+        if (threadIdx.x < 256 && tc <= ti && k0 + ti < n) S[(size_t)(k0 + ti) * n + k0 + tc] = dblk[ti * 17 + tc];
+        // panel: L[R][k0..k0+15] = S[R][k0..] L_kk^-T
+        for (int R = k0 + 16 + (int)threadIdx.x; R < n; R += NT) {
+            double x[16];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) x[c] = k0 + c < n ? S[(size_t)R * n + k0 + c] : 0.0;
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                double t = x[c];
+#pragma unroll
+                for (int jj = 0; jj < c; ++jj) t -= x[jj] * dblk[c * 17 + jj];
+                x[c] = t / dblk[c * 17 + c];
+            }
+#pragma unroll
+            for (int c = 0; c < 16; ++c)
+                if (k0 + c < n) S[(size_t)R * n + k0 + c] = x[c];
+        }
+        __syncthreads();
+        // trailing blocks (i, j), k < j <= i < nb
+        const int m = nb - k - 1;
+        for (int bb = wave; bb < m * (m + 1) / 2; bb += NW) {
+            int i = (int)((sqrt(8.0 * bb + 1.0) - 1.0) * 0.5);
+            while ((i + 1) * (i + 2) / 2 <= bb) ++i;
+            while (i * (i + 1) / 2 > bb) --i;
+            const int j = bb - i * (i + 1) / 2;
+            const int bi = 16 * (k + 1 + i), bj = 16 * (k + 1 + j);
+            v4d acc;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[q] = padded_at(S, n, bi + 4 * q + (lane >> 4), bj + (lane & 15));
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+                const double a = -padded_at(S, n, bi + (lane & 15), k0 + 4 * kk + (lane >> 4));
+                const double bv = padded_at(S, n, bj + (lane & 15), k0 + 4 * kk + (lane >> 4));
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int r = bi + 4 * q + (lane >> 4), c = bj + (lane & 15);
+                if (r < n && c < n) S[(size_t)r * n + c] = acc[q];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// y = L^-1 y then y = L^-T y for the factor of chol_blocked, by 16-row
+// blocks: the diagonal block by wave 0 (one row per lane, shuffles), the
+// rest as one short dot product per row.
+template <int NT>
+__device__ void chol_blocked_solve(const double* S, int n, double* y) {
+    const int nb = (n + 15) >> 4;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int k = 0; k < nb; ++k) {  // forward: L y = b
+        const int k0 = 16 * k;
+        if (wave == 0) {
+            const int r = k0 + lane;
+            double v = lane < 16 && r < n ? y[r] : 0.0;
+            for (int j = 0; j < 16 && k0 + j < n; ++j) {
+                if (lane == j) v = v / S[(size_t)r * n + r];
+                const double yj = __shfl(v, j, 64);
+                if (lane > j && lane < 16 && r < n) v -= S[(size_t)r * n + k0 + j] * yj;
+            }
+            if (lane < 16 && r < n) y[r] = v;
+        }
+        __syncthreads();
+        for (int R = k0 + 16 + (int)threadIdx.x; R < n; R += NT) {
+            double t = y[R];
+            for (int c = 0; c < 16; ++c) t -= S[(size_t)R * n + k0 + c] * y[k0 + c];
+            y[R] = t;
+        }
+        __syncthreads();
+    }
+    for (int k = nb - 1; k >= 0; --k) {  // backward: L^T x = y
+        const int k0 = 16 * k;
+        if (wave == 0) {
+            const int r = k0 + lane;
+            double v = lane < 16 && r < n ? y[r] : 0.0;
+            for (int j = 15; j >= 0; --j) {
+                if (k0 + j >= n) continue;
+                if (lane == j) v = v / S[(size_t)(k0 + j) * n + k0 + j];
+                const double xj = __shfl(v, j, 64);
+                if (lane < j && r < n) v -= S[(size_t)(k0 + j) * n + r] * xj;
+            }
+            if (lane < 16 && r < n) y[r] = v;
+        }
+        __syncthreads();
+        for (int R = (int)threadIdx.x; R < k0; R += NT) {
+            double t = y[R];
+            for (int c = 0; c < 16 && k0 + c < n; ++c) t -= S[(size_t)(k0 + c) * n + R] * y[k0 + c];
+            y[R] = t;
+        }
+        __syncthreads();
+    }
+}
+
+// Cholesky (blocked, MFMA trailing updates), then forward/back
 // substitution; on a non-positive pivot an LU with partial pivoting on the
 // saved copy. Solves S x = b for the n = 6 * (variable poses) unknowns whose
 // pose slots are var_slot[0 .. n/6). One work-group of NT threads.
 template <int NT>
 __device__ void dense_solve_body(const DevProblem& P, const int32_t* var_slot, double* S, double* work, int n,
-                                 int* status, int* fail_sh, double* piv_val, int* piv_idx) {
+                                 int* status, int* fail_sh, double* piv_val, int* piv_idx, double* dblk) {
     double* A2 = work;                 // n*n copy for the LU fallback
     double* y = work + (size_t)n * n;  // n
     for (size_t k = threadIdx.x; k < (size_t)n * n; k += NT) A2[k] = S[k];
     for (int i = threadIdx.x; i < n; i += NT) y[i] = P.b[6 * (size_t)var_slot[i / 6] + (i % 6)];
     if (threadIdx.x == 0) *fail_sh = 0;
     __syncthreads();
-    for (int j = 0; j < n; ++j) {
-        for (int i = j + threadIdx.x; i < n; i += NT) {
-            double s = S[(size_t)i * n + j];
-            const double* Li = S + (size_t)i * n;
-            const double* Lj = S + (size_t)j * n;
-            for (int k = 0; k < j; ++k) s -= Li[k] * Lj[k];
-            S[(size_t)i * n + j] = s;
-        }
-        __syncthreads();
-        const double d = S[(size_t)j * n + j];
-        if (!(d > 0.0) || !isfinite(d)) {
-            if (threadIdx.x == 0) *fail_sh = 1;
-            break;
-        }
-        const double ljj = sqrt(d);
-        for (int i = j + 1 + threadIdx.x; i < n; i += NT) S[(size_t)i * n + j] /= ljj;
-        __syncthreads();
-        if (threadIdx.x == 0) S[(size_t)j * n + j] = ljj;
-        __syncthreads();
-    }
+    chol_blocked<NT>(S, n, dblk, fail_sh);
     __syncthreads();
     if (!*fail_sh) {
-        for (int j = 0; j < n; ++j) {  // L y = b
-            if (threadIdx.x == 0) y[j] = y[j] / S[(size_t)j * n + j];
-            __syncthreads();
-            const double yj = y[j];
-            for (int i = j + 1 + threadIdx.x; i < n; i += NT) y[i] -= S[(size_t)i * n + j] * yj;
-            __syncthreads();
-        }
-        for (int j = n - 1; j >= 0; --j) {  // L^T x = y
-            if (threadIdx.x == 0) y[j] = y[j] / S[(size_t)j * n + j];
-            __syncthreads();
-            const double xj = y[j];
-            for (int i = threadIdx.x; i < j; i += NT) y[i] -= S[(size_t)j * n + i] * xj;
-            __syncthreads();
-        }
+        chol_blocked_solve<NT>(S, n, y);
     } else {
         // LU with partial pivoting on A2; y holds b again.
         for (int i = threadIdx.x; i < n; i += NT) y[i] = P.b[6 * (size_t)var_slot[i / 6] + (i % 6)];
@@ -523,7 +673,8 @@ __global__ __launch_bounds__(kSmallBlock) void k_dense_solve(DevProblem P, Dense
     __shared__ int fail;
     __shared__ double piv_val[kSmallBlock];
     __shared__ int piv_idx[kSmallBlock];
-    dense_solve_body<kSmallBlock>(P, D.var_slot, S, work, n, status, &fail, piv_val, piv_idx);
+    __shared__ double dblk[16 * 17];
+    dense_solve_body<kSmallBlock>(P, D.var_slot, S, work, n, status, &fail, piv_val, piv_idx, dblk);
 }
 
 // ---------------------------------------------------------------------------
@@ -532,14 +683,14 @@ __global__ __launch_bounds__(kSmallBlock) void k_dense_solve(DevProblem P, Dense
 // on the device (no host round trip per iteration).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_batch_dense_assemble(DevProblem P, DensePairs D, BatchDev B) {
-    __shared__ double lds[(kBlock / 64) * 36];
-    const int b = blockIdx.x;
+    const int b = blockIdx.x * (kBlock / kBlockLanes) + (int)threadIdx.x / kBlockLanes;
+    if (b >= D.num_blocks) return;
     const int w = P.pose_win[D.block_pq[2 * b]];
     const WindowState& st = B.state[w];
     if (!st.active) return;
     const WindowLayout L = B.layout[w];
-    dense_assemble_block<kBlock>(P, D, b, B.S + L.s_offset, 6 * (L.var_end - L.var_begin), L.var_begin, st.lambda,
-                                 lds);
+    dense_assemble_group(P, D, b, B.S + L.s_offset, 6 * (L.var_end - L.var_begin), L.var_begin, st.lambda,
+                         threadIdx.x % kBlockLanes);
 }
 
 __global__ __launch_bounds__(kBlock) void k_batch_dense_zero(DevProblem P, BatchDev B) {
@@ -564,8 +715,9 @@ __global__ __launch_bounds__(kBlock) void k_batch_dense_solve(DevProblem P, Dens
         if (threadIdx.x == 0) st.solve_status = 0;
         return;
     }
+    __shared__ double dblk[16 * 17];
     dense_solve_body<kBlock>(P, D.var_slot + L.var_begin, B.S + L.s_offset, B.work + B.work_offset[w], n,
-                             &st.solve_status, &fail, piv_val, piv_idx);
+                             &st.solve_status, &fail, piv_val, piv_idx, dblk);
 }
 
 // Per window: compensated cost and non-projectable count over its camera
@@ -744,14 +896,18 @@ void launch_scalars_finalize(const DevProblem& P, hipStream_t s) {
 }
 void launch_dense_assemble(const DevProblem& P, const DensePairs& D, double* S, int32_t n, double lambda,
                            hipStream_t s) {
-    if (D.num_blocks) hipLaunchKernelGGL(k_dense_assemble, dim3(D.num_blocks), dim3(kBlock), 0, s, P, D, S, n, lambda);
+    if (D.num_blocks)
+        hipLaunchKernelGGL(k_dense_assemble, dim3(blocks_for(D.num_blocks, kBlock / kBlockLanes)), dim3(kBlock), 0, s, P,
+                           D, S, n, lambda);
 }
 void launch_dense_solve(const DevProblem& P, const DensePairs& D, double* S, double* work, int32_t n,
                         int32_t* status, hipStream_t s) {
     hipLaunchKernelGGL(k_dense_solve, dim3(1), dim3(kSmallBlock), 0, s, P, D, S, work, n, status);
 }
 void launch_batch_dense_assemble(const DevProblem& P, const DensePairs& D, const BatchDev& B, hipStream_t s) {
-    if (D.num_blocks) hipLaunchKernelGGL(k_batch_dense_assemble, dim3(D.num_blocks), dim3(kBlock), 0, s, P, D, B);
+    if (D.num_blocks)
+        hipLaunchKernelGGL(k_batch_dense_assemble, dim3(blocks_for(D.num_blocks, kBlock / kBlockLanes)), dim3(kBlock),
+                           0, s, P, D, B);
 }
 void launch_batch_dense_zero(const DevProblem& P, const BatchDev& B, hipStream_t s) {
     if (B.num_windows) hipLaunchKernelGGL(k_batch_dense_zero, dim3(16, B.num_windows), dim3(kBlock), 0, s, P, B);

@@ -66,6 +66,7 @@ struct WindowState {
 struct DevProblem {
     int32_t num_pose_slots, num_lm_slots, num_edges, num_obs;
     int32_t num_cam_tiles, num_lm_tiles, any_rig, kernel_kind;
+    int32_t num_cus, pad0;  // compute units of the device (persistent grids)
     double kernel_scale;
     // state
     double* poses;        // 7 per slot
@@ -97,7 +98,8 @@ struct DevProblem {
     double* V;       // 6 per landmark slot (upper 3x3)
     double* gl;      // 3 per landmark slot
     double* Vinv64;  // 6 per landmark slot
-    float* Vinv32;
+    double* Vm;      // 6 per landmark: V summed from the FP32 G (mixed precision)
+    double* VinvM;   // its damped inverse, the one the mixed operator uses
     double* Minv;    // 36 per pose slot (block-Jacobi inverse)
     // Schur operator structure. The edges of a landmark tile, sorted by pose
     // (tile_perm: tile-local l-order index at each sorted position), form
@@ -105,6 +107,8 @@ struct DevProblem {
     // sorted position, and a pose sums its runs (pose_runs[pose_run_ptr[p]
     // ..], ascending) in that order.
     const uint16_t* tile_perm;
+    const int32_t* big_tiles;  // tiles of one landmark with more than kTileEdges edges
+    int32_t num_big_tiles, pad1;
     const int32_t* pose_run_ptr;  // num_pose_slots + 1
     const int32_t* pose_runs;     // run-end edge indices
     double* part64;  // 6 per edge index (run ends only)

@@ -79,13 +79,14 @@ void build_runs(Model* mp) {
     m.tile_perm.assign(m.num_edges, 0);
     m.s_pose.assign(m.num_edges, 0);
     m.lm_store.assign(m.num_edges, 0);
+    m.big_tiles.clear();
     std::vector<int32_t> run_pose, run_end, count;
     std::vector<uint64_t> keys;
     for (size_t t = 0; t < m.lm_tiles.size(); ++t) {
         const int32_t e0 = m.lm_edge_ptr[m.lm_tiles[t].lm_begin], e1 = m.lm_edge_ptr[m.lm_tiles[t].lm_end];
         const int32_t n = e1 - e0;
         if (n > kTileEdges) {
-            for (int32_t s = 0; s < n; ++s) m.tile_perm[e0 + s] = 0;  // identity, not read
+            m.big_tiles.push_back((int32_t)t);  // identity order; tile_perm is not read
         } else {
             // stable sort by pose: counting sort over the tile's pose range when
             // it is narrow (landmarks ordered by first camera), else a key sort

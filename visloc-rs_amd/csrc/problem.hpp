@@ -93,6 +93,7 @@ struct Model {
     // each l-order edge.
     std::vector<uint16_t> tile_perm;
     std::vector<int32_t> s_pose, lm_store;
+    std::vector<int32_t> big_tiles;  // tiles of one landmark with more than kTileEdges edges
     std::vector<int32_t> pose_run_ptr, pose_runs;
     uint64_t num_runs = 0;
 };

@@ -192,7 +192,7 @@ constexpr int kTileThreads = 512;
 template <typename TS, int OP>
 __global__ __launch_bounds__(kTileThreads) void k_schur_tile(DevProblem P, const double* __restrict__ v1,
                                                        const double* __restrict__ v2, const TS* __restrict__ Gl,
-                                                       const TS* __restrict__ Vinv, double* __restrict__ part,
+                                                       const double* __restrict__ Vinv, double* __restrict__ part,
                                                        int pcg_iter, int fresh) {
     constexpr int NV = OP == OP_PRECOND ? 21 : 6;
     constexpr int kWaves = kTileThreads / 64;
@@ -241,11 +241,7 @@ __global__ __launch_bounds__(kTileThreads) void k_schur_tile(DevProblem P, const
             block_sum<kTileThreads, double, 3>(t, sRed);
             if (tid == 0) {
                 if (OP == OP_APPLY) {
-                    TS vs[6];
-                    load6(Vinv + 6 * (size_t)l, vs);
-                    const double vi[6] = {(double)vs[0], (double)vs[1], (double)vs[2],
-                                          (double)vs[3], (double)vs[4], (double)vs[5]};
-                    sym3_mul(vi, t, sBig);
+                    sym3_mul(Vinv + 6 * (size_t)l, t, sBig);
                 } else {
                     double dl[3] = {0.0, 0.0, 0.0};
                     if (P.lm_var[l] >= 0) {
@@ -287,7 +283,7 @@ __global__ __launch_bounds__(kTileThreads) void k_schur_tile(DevProblem P, const
     }
 
     // ---- stage 0: landmark positions, edge owners, this lane's positions ----
-    TS vpre[6];  // this owner's V^-1, loaded early
+    double vpre[6];  // this owner's V^-1, loaded early
     if (tid < nl) {
         const int l = lt.lm_begin + tid;
         if (OP == OP_APPLY) load6(Vinv + 6 * (size_t)l, vpre);
@@ -352,9 +348,7 @@ __global__ __launch_bounds__(kTileThreads) void k_schur_tile(DevProblem P, const
                 t[2] += sT[3 * k + 2];
             }
             if (OP == OP_APPLY) {
-                const double vi[6] = {(double)vpre[0], (double)vpre[1], (double)vpre[2],
-                                      (double)vpre[3], (double)vpre[4], (double)vpre[5]};
-                sym3_mul(vi, t, sU + 3 * tid);
+                sym3_mul(vpre, t, sU + 3 * tid);
             } else {
                 // delta_l = -V^-1 (g_l + sum_e W_e^T x_p) (normal_equations.rs:1551-1592)
                 double dl[3] = {0.0, 0.0, 0.0};
@@ -814,7 +808,7 @@ void launch_schur_apply(const DevProblem& P, bool fp32, const double* v1, const 
                         int32_t fresh, hipStream_t s) {
     if (fp32)
         hipLaunchKernelGGL((k_schur_tile<float, OP_APPLY>), dim3(xcd_grid(P.num_lm_tiles)), dim3(kTileThreads), 0, s, P,
-                           v1, v2, (const float*)P.Gl32, (const float*)P.Vinv32, P.part64, pcg_iter, fresh);
+                           v1, v2, (const float*)P.Gl32, (const double*)P.VinvM, P.part64, pcg_iter, fresh);
     else
         hipLaunchKernelGGL((k_schur_tile<double, OP_APPLY>), dim3(xcd_grid(P.num_lm_tiles)), dim3(kTileThreads), 0, s, P,
                            v1, v2, (const double*)P.Gl64, (const double*)P.Vinv64, P.part64, pcg_iter, fresh);

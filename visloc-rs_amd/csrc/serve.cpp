@@ -41,7 +41,7 @@ using Clock = std::chrono::steady_clock;
 
 // A deep copy of a vba_problem and a view over it.
 struct OwnedProblem {
-    vba_problem view{};
+    vba_problem view{};  // over the copies below, or over the caller's arrays when borrowed
     std::vector<uint64_t> pose_ids, lm_ids, obs_pose, obs_lm, rig_pose, rig_lm;
     std::vector<double> poses, lms, obs_xy, rig_xy;
     std::vector<uint8_t> pose_fixed, lm_fixed;
@@ -190,18 +190,30 @@ struct vba_server {
         std::vector<vba_iteration_stats> stats;
         vba::PreparedBatch pb;
         try {
-            for (uint32_t i = 0; i < n; ++i) {
-                std::memset(&results[i], 0, sizeof(vba_result));
-                std::string reason;
-                int32_t st = vba::check_support(batch[i]->prob.view, batch[i]->opts, &reason);
-                if (st == VBA_OK) {
-                    uint64_t error_id = 0;
-                    st = vba::build_model(batch[i]->prob.view, &models[i], &error_id, &reason);
-                    results[i].error_id = error_id;
+            // host layout of the windows, on up to cfg.build_threads threads
+            auto build = [&](uint32_t begin, uint32_t step) {
+                for (uint32_t i = begin; i < n; i += step) {
+                    std::memset(&results[i], 0, sizeof(vba_result));
+                    std::string reason;
+                    int32_t st = vba::check_support(batch[i]->prob.view, batch[i]->opts, &reason);
+                    if (st == VBA_OK) {
+                        uint64_t error_id = 0;
+                        try {
+                            st = vba::build_model(batch[i]->prob.view, &models[i], &error_id, &reason);
+                        } catch (const std::bad_alloc&) {
+                            st = VBA_ERR_OUT_OF_MEMORY;
+                        }
+                        results[i].error_id = error_id;
+                    }
+                    statuses[i] = st;
+                    valid[i] = st == VBA_OK;
                 }
-                statuses[i] = st;
-                valid[i] = st == VBA_OK;
-            }
+            };
+            const uint32_t nt = std::max<uint32_t>(1, std::min<uint32_t>(cfg.build_threads, n));
+            std::vector<std::thread> pool;
+            for (uint32_t t = 1; t < nt; ++t) pool.emplace_back(build, t, nt);
+            build(0, nt);
+            for (std::thread& th : pool) th.join();
             std::lock_guard<std::mutex> lock(ctx->mu);
             (void)hipSetDevice(ctx->device);
             std::string msg;
@@ -271,6 +283,7 @@ void vba_server_default_config(vba_server_config* c) {
     c->submap_streams = 2;
     c->hbm_budget_bytes = 200ull << 30;
     c->batch_wait_seconds = 0.002;
+    c->build_threads = 8;
 }
 
 vba_server* vba_server_create(const vba_server_config* config) {
@@ -278,7 +291,7 @@ vba_server* vba_server_create(const vba_server_config* config) {
     vba_server_default_config(&c);
     if (config) c = *config;
     if (c.max_window_batch == 0 || (c.window_streams == 0 && c.submap_streams == 0) ||
-        c.window_streams + c.submap_streams > 8 || !(c.batch_wait_seconds >= 0.0)) {
+        c.window_streams + c.submap_streams > 8 || !(c.batch_wait_seconds >= 0.0) || c.build_threads > 64) {
         vba::set_thread_error("vba_server_create: invalid configuration");
         return nullptr;
     }
@@ -304,14 +317,17 @@ vba_server* vba_server_create(const vba_server_config* config) {
     return s.release();
 }
 
-uint64_t vba_server_submit(vba_server* s, const vba_problem* problem, const vba_options* options) {
+namespace {
+
+uint64_t submit(vba_server* s, const vba_problem* problem, const vba_options* options, bool borrow) {
     if (!s || !problem || !options) {
         vba::set_thread_error("vba_server_submit: null argument");
         return 0;
     }
     auto r = std::make_shared<Request>();
     try {
-        r->prob.copy_from(*problem);
+        if (borrow) r->prob.view = *problem;
+        else r->prob.copy_from(*problem);
     } catch (const std::bad_alloc&) {
         vba::set_thread_error("vba_server_submit: out of host memory");
         return 0;
@@ -338,6 +354,16 @@ uint64_t vba_server_submit(vba_server* s, const vba_problem* problem, const vba_
     }
     s->cv_work.notify_all();
     return r->ticket;
+}
+
+}  // namespace
+
+uint64_t vba_server_submit(vba_server* s, const vba_problem* problem, const vba_options* options) {
+    return submit(s, problem, options, false);
+}
+
+uint64_t vba_server_submit_borrowed(vba_server* s, const vba_problem* problem, const vba_options* options) {
+    return submit(s, problem, options, true);
 }
 
 int32_t vba_server_wait(vba_server* s, uint64_t ticket, double timeout_seconds, double* poses_out,

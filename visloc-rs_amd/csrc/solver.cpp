@@ -23,8 +23,8 @@ namespace {
 enum BufId {
     B_POSES0, B_LMS0, B_POSES, B_LMS, B_POSES_T, B_LMS_T, B_SENSORS, B_POSE_VAR, B_LM_VAR, B_OBS_UV,
     B_OBS_SENSOR, B_EDGE_LM, B_EDGE_OBS, B_PERM_CL, B_L_POSE, B_LM_STORE, B_LM_EDGE_PTR, B_CAM_TILES,
-    B_CAM_TILE_PTR, B_LM_TILES, B_TILE_PERM, B_POSE_RUN_PTR, B_POSE_RUNS, B_GL64, B_GL32,
-    B_HL64, B_TILE27, B_U, B_G, B_V, B_GL, B_VINV64, B_VINV32, B_MINV, B_VB, B_VX, B_VR, B_VZ, B_VD,
+    B_CAM_TILE_PTR, B_LM_TILES, B_TILE_PERM, B_BIG_TILES, B_POSE_RUN_PTR, B_POSE_RUNS, B_GL64, B_GL32,
+    B_HL64, B_TILE27, B_U, B_G, B_V, B_GL, B_VINV64, B_VINV32, B_VM, B_MINV, B_VB, B_VX, B_VR, B_VZ, B_VD,
     B_VQ, B_PART64, B_PART21, B_BPART, B_CPART, B_PCG, B_DELTA_L, B_TILE_SCALAR,
     B_SCALARS, B_DENSE_BLOCK_PQ, B_DENSE_BLOCK_PTR, B_DENSE_TERMS, B_DENSE_VAR_SLOT, B_DENSE_S,
     B_DENSE_WORK, B_DENSE_STATUS, B_COUNT
@@ -325,6 +325,7 @@ int32_t prepare(Context& ctx, Prepared& pr, std::string* msg) {
         upload(pr, B_CAM_TILE_PTR, m.cam_tile_ptr, s);
         upload(pr, B_LM_TILES, m.lm_tiles, s);
         upload(pr, B_TILE_PERM, m.tile_perm, s);
+        upload(pr, B_BIG_TILES, m.big_tiles, s);
         upload(pr, B_POSE_RUN_PTR, m.pose_run_ptr, s);
         upload(pr, B_POSE_RUNS, m.pose_runs, s);
         hip_check(hipStreamSynchronize(s), "sync");
@@ -367,9 +368,11 @@ namespace {
 // Device pointers of a prepared problem for one solve: structure, the state
 // reset to the prepared initial state, and working buffers (allocated on
 // first use, kept for later solves).
-DevProblem bind(Prepared& pr, const vba_options& o, bool mixed, bool matrix_free, hipStream_t s) {
+DevProblem bind(const Context& ctx, Prepared& pr, const vba_options& o, bool mixed, bool matrix_free,
+                hipStream_t s) {
     const Model& m = pr.m;
     DevProblem P{};
+    P.num_cus = ctx.num_cus;
     P.num_pose_slots = (int32_t)m.num_pose_slots;
     P.num_lm_slots = (int32_t)m.num_lm_slots;
     P.num_edges = (int32_t)m.num_edges;
@@ -405,13 +408,16 @@ DevProblem bind(Prepared& pr, const vba_options& o, bool mixed, bool matrix_free
     P.cam_tile_ptr = buf<int32_t>(pr, B_CAM_TILE_PTR);
     P.lm_tiles = buf<LmTile>(pr, B_LM_TILES);
     P.tile_perm = buf<uint16_t>(pr, B_TILE_PERM);
+    P.big_tiles = buf<int32_t>(pr, B_BIG_TILES);
+    P.num_big_tiles = (int32_t)m.big_tiles.size();
     P.pose_run_ptr = buf<int32_t>(pr, B_POSE_RUN_PTR);
     P.pose_runs = buf<int32_t>(pr, B_POSE_RUNS);
     P.Gl64 = (double*)ensure(pr, B_GL64, 6 * NE * sizeof(double));
     P.hl64 = (double*)ensure(pr, B_HL64, 3 * NE * sizeof(double));
     if (mixed) {
         P.Gl32 = (float*)ensure(pr, B_GL32, 6 * NE * sizeof(float));
-        P.Vinv32 = (float*)ensure(pr, B_VINV32, 6 * NL * sizeof(float));
+        P.Vm = (double*)ensure(pr, B_VM, 6 * NL * sizeof(double));
+        P.VinvM = (double*)ensure(pr, B_VINV32, 6 * NL * sizeof(double));
     }
     P.tile27 = (double*)ensure(pr, B_TILE27, 27 * std::max<size_t>(NT, 1) * sizeof(double));
     P.U = (double*)ensure(pr, B_U, 21 * NP * sizeof(double));
@@ -457,7 +463,7 @@ int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stat
     try {
         if (!matrix_free && m.P > 1024)
             throw Fail{VBA_ERR_INVALID_ARGUMENT, "dense reduced solve limited to 1024 variable poses"};
-        DevProblem P = bind(pr, o, mixed, matrix_free, s);
+        DevProblem P = bind(ctx, pr, o, mixed, matrix_free, s);
         const size_t NP = m.num_pose_slots, NL = m.num_lm_slots, NE = m.num_edges, NT = m.cam_tiles.size();
         const size_t NR = m.num_runs;
         DensePairs D{};
@@ -487,9 +493,9 @@ int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stat
         // Pose vectors gathered per edge are counted once per pose.
         const double real = mixed ? 4.0 : 8.0;
         // tile pass: per edge G (6 reals), its pose (4) and sorted position (2);
-        // per landmark X (24), V^-1 (6 reals), edge pointer (4); per run the
+        // per landmark X (24), V^-1 (48), edge pointer (4); per run the
         // FP64 partial (48); per pose z and d (96); per tile its bounds (8)
-        const double bytes_apply = NE * (6.0 * real + 4.0 + 2.0) + NL * (24.0 + 6.0 * real + 4.0) + NR * 48.0 +
+        const double bytes_apply = NE * (6.0 * real + 4.0 + 2.0) + NL * (24.0 + 48.0 + 4.0) + NR * 48.0 +
                                    NP * 96.0 + m.lm_tiles.size() * 8.0;
         const double bytes_pcg_pose = NR * (4.0 + 48.0) + NP * (8.0 + 96.0 + 168.0 + 96.0);
         const double bytes_pcg_step = NP * (4.0 * 48.0 + 288.0 + 3.0 * 48.0);
@@ -793,6 +799,7 @@ void merge_models(const std::vector<const Model*>& ms, Model* out, std::vector<W
             M.cam_tiles.push_back({(int32_t)(t.pose + po), (int32_t)(t.begin + eo), (int32_t)(t.end + eo), 0});
             cam_tile_win->push_back((int32_t)w);
         }
+        for (int32_t t : m.big_tiles) M.big_tiles.push_back((int32_t)(t + M.lm_tiles.size()));
         for (const LmTile& t : m.lm_tiles) {
             M.lm_tiles.push_back({(int32_t)(t.lm_begin + lo), (int32_t)(t.lm_end + lo)});
             lm_tile_win->push_back((int32_t)w);
@@ -933,7 +940,7 @@ int32_t run_batch(Context& ctx, PreparedBatch& pb, const vba_options& o, vba_ite
             hip_check(hipHostMalloc((void**)&ctx.host_pcg, sizeof(PcgState), hipHostMallocDefault), "hipHostMalloc");
             hip_check(hipHostMalloc((void**)&ctx.host_status, sizeof(int32_t), hipHostMallocDefault), "hipHostMalloc");
         }
-        DevProblem P = bind(pb, o, false, false, s);
+        DevProblem P = bind(ctx, pb, o, false, false, s);
         P.any_rig = 1;
         P.obs_sensor = buf<uint16_t>(pb, B_OBS_SENSOR);
         P.pose_win = buf<int32_t>(pb, BB_POSE_WIN);

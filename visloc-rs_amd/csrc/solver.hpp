@@ -31,6 +31,7 @@ struct KernelProfile {
 
 struct Context {
     int device = 0;
+    int num_cus = 0;  // compute units of the device
     hipStream_t stream = nullptr;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     std::mutex mu;

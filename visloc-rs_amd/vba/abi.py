@@ -200,6 +200,8 @@ class ServerConfig(C.Structure):
         ("submap_streams", C.c_uint32),
         ("hbm_budget_bytes", C.c_uint64),
         ("batch_wait_seconds", C.c_double),
+        ("build_threads", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
 
@@ -241,6 +243,7 @@ SIGNATURES = {
     "vba_server_default_config": (None, [C.POINTER(ServerConfig)]),
     "vba_server_create": (C.c_void_p, [C.POINTER(ServerConfig)]),
     "vba_server_submit": (C.c_uint64, [C.c_void_p, C.POINTER(Problem), C.POINTER(Options)]),
+    "vba_server_submit_borrowed": (C.c_uint64, [C.c_void_p, C.POINTER(Problem), C.POINTER(Options)]),
     "vba_server_wait": (C.c_int32, [C.c_void_p, C.c_uint64, C.c_double, P64, P64, C.POINTER(Result),
                                     C.POINTER(C.c_double)]),
     "vba_server_counters": (None, [C.c_void_p, PU64, PU64, PU64, PU64]),

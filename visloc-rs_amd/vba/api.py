@@ -456,13 +456,20 @@ class Server:
         if not self.handle:
             raise RuntimeError("vba_server_create failed: " + self.lib.vba_last_error(None).decode())
         self.shapes = {}
+        self.borrowed = {}
 
-    def submit(self, problem: BundleProblem, options: abi.Options) -> int:
+    def submit(self, problem: BundleProblem, options: abi.Options, borrow: bool = False) -> int:
+        """Queue a solve. With borrow=True the server reads the problem's
+        arrays in place: keep `problem` alive and unchanged until wait()
+        has returned for the ticket."""
         view = problem.view()
-        t = self.lib.vba_server_submit(self.handle, C.byref(view), C.byref(options))
+        fn = self.lib.vba_server_submit_borrowed if borrow else self.lib.vba_server_submit
+        t = fn(self.handle, C.byref(view), C.byref(options))
         if t == 0:
             raise RuntimeError(self.lib.vba_last_error(None).decode())
         self.shapes[t] = (problem.poses.shape, problem.landmarks.shape)
+        if borrow:
+            self.borrowed[t] = (problem, view)
         return t
 
     def wait(self, ticket: int, timeout: float = 600.0) -> ServedResult:
@@ -476,6 +483,7 @@ class Server:
         if st == abi.PENDING:
             return ServedResult(st, res, None, None, float("nan"))
         del self.shapes[ticket]
+        self.borrowed.pop(ticket, None)
         ok = st == abi.OK
         return ServedResult(st, res, poses if ok else None, lms if ok else None, lat.value)
 
