@@ -60,15 +60,14 @@ def test_loop_matrix_free_parity(device, policy):
 
 
 def test_mixed_precision_matches_fp64(device):
-    """Mixed precision stores G and V^-1 in FP32 (the operator's arithmetic
-    stays FP64) and refines against the FP64 true residual. With PCG run to
-    its tolerance, both precisions take the same LM path and agree to the
-    bar. (At a PCG iteration cap both solves stop on truncated Krylov
-    iterates, whose rounding sensitivity the bench reports separately as
-    rel_cost_vs_fp64.)"""
+    """Mixed precision stores G in FP32 and pairs it with the inverse of the
+    V summed from those same FP32 blocks; the operator's arithmetic stays
+    FP64 and PCG is checked against the FP64 true residual. On the reference
+    GPU backend's PCG settings (rtol 1e-4, at most 100 iterations, the last
+    iterate used at the cap) both precisions take the same LM path."""
     s = vba.synthesize(abi.SYNTH_LOOP, num_cameras=80, num_landmarks=6000, seed=3, scene_scale=16.0)
-    o = vba.default_options(linear_solver=abi.SOLVER_MATRIX_FREE, max_iterations=15, max_pcg_iterations=500,
-                            pcg_relative_tolerance=1e-6, robust_kernel=abi.KERNEL_HUBER, robust_scale=1.0,
+    o = vba.default_options(linear_solver=abi.SOLVER_MATRIX_FREE, max_iterations=15, max_pcg_iterations=100,
+                            pcg_relative_tolerance=1e-4, robust_kernel=abi.KERNEL_HUBER, robust_scale=1.0,
                             pcg_maxit_policy=abi.PCG_MAXIT_ACCEPT_ITERATE)
     fp64 = device.optimize(s.problem, o)
     o.precision = abi.PRECISION_MIXED
