@@ -38,11 +38,11 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def loop_options(precision, policy=abi.PCG_MAXIT_ACCEPT_ITERATE):
+def loop_options(precision, policy=abi.PCG_MAXIT_ACCEPT_ITERATE, solver=abi.SOLVER_MATRIX_FREE):
     """PCG rtol 1e-4 / max 100 are the reference GPU backend's GpuBaSettings
     defaults (crates/ba-gpu/src/solver.rs:36-48); a solve that reaches the cap
     uses its last iterate, as that backend does (solver.rs:497-508)."""
-    return vba.default_options(linear_solver=abi.SOLVER_MATRIX_FREE, max_iterations=30, max_pcg_iterations=100,
+    return vba.default_options(linear_solver=solver, max_iterations=30, max_pcg_iterations=100,
                                pcg_relative_tolerance=1e-4, robust_kernel=abi.KERNEL_HUBER, robust_scale=1.0,
                                precision=precision, pcg_maxit_policy=policy)
 
@@ -318,6 +318,11 @@ def main():
     ap.add_argument("--rate", type=float, default=2000.0, help="requests per second (--workload serve)")
     ap.add_argument("--serve-seconds", type=float, default=1.0, help="arrival window (--workload serve)")
     ap.add_argument("--windows", type=int, default=256, help="windows in the batch (--workload windows)")
+    ap.add_argument("--profile-stride", type=int, default=16,
+                    help="record kernel times around every n-th launch of each kernel family")
+    ap.add_argument("--solver", choices=["matrix-free", "explicit"], default="matrix-free",
+                    help="reduced camera PCG of the loop workload: implicit operator (the workload's own "
+                         "setting) or the explicitly assembled block-sparse matrix")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (run with --gpus 1).",
@@ -338,7 +343,8 @@ def main():
         f"{N} observations (generated in {time.perf_counter() - t0:.1f} s)")
     precision = abi.PRECISION_MIXED if args.precision == "mixed" else abi.PRECISION_FP64
     policy = abi.PCG_MAXIT_ACCEPT_ITERATE if args.pcg_policy == "accept" else abi.PCG_MAXIT_REJECT
-    opts = loop_options(precision, policy)
+    solver = abi.SOLVER_MATRIX_FREE if args.solver == "matrix-free" else abi.SOLVER_PCG_EXPLICIT
+    opts = loop_options(precision, policy, solver)
     dev = vba.Device(0)
     t0 = time.perf_counter()
     prep = dev.prepare(problem, opts)
@@ -352,7 +358,9 @@ def main():
             f"cost {out.result.initial_cost:.6e} -> {out.result.final_cost:.6e}, "
             f"device {out.result.device_seconds:.3f} s")
 
-    dev.profile(True)
+    # HIP events around every 16th launch of each kernel family: enough launches
+    # for the averages, few enough that recording them does not slow the solve
+    dev.profile(True, stride=args.profile_stride)
     lm_iters = 0
     pcg_iters = 0
     final_cost = None
@@ -371,14 +379,14 @@ def main():
         f"({accepted} accepted), {out.result.total_pcg_iterations} PCG iterations")
 
     kernels = {}
-    for name in ("linearize", "schur_apply", "pcg_pose", "pcg_step", "cost"):
+    for name in ("linearize", "schur_apply", "pcg_pose", "pcg_step", "cost", "explicit_assemble", "explicit_apply"):
         q = dev.profile_query(name)
         if q["launches"]:
             q["avg_us"] = 1e6 * q["seconds"] / q["launches"]
             q["achieved_gbs"] = q["bytes"] / q["seconds"] / 1e9 if q["seconds"] > 0 else 0.0
             q["bytes_per_launch"] = q["bytes"] / q["launches"]
-        kernels[name] = q
-        log(f"bench: kernel {name}: {q}")
+            kernels[name] = q
+            log(f"bench: kernel {name}: {q}")
     dominant = max((k for k in kernels if kernels[k]["bytes"] > 0), key=lambda k: kernels[k]["seconds"])
     dk = kernels[dominant]
     traffic = None
@@ -405,7 +413,7 @@ def main():
     ref_rel = None
     fp64_obs_per_s = None
     if precision == abi.PRECISION_MIXED:
-        f64 = loop_options(abi.PRECISION_FP64, policy)
+        f64 = loop_options(abi.PRECISION_FP64, policy, solver)
         t1 = time.perf_counter()
         ref = prep.solve(f64)
         t_ref = time.perf_counter() - t1
@@ -431,7 +439,7 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "loop: 1200 cameras on a 200 m closed loop, 400k landmarks, "
-                        f"{N} observations, Huber 1 px, matrix-free block-Jacobi PCG rtol 1e-4 max 100, "
+                        f"{N} observations, Huber 1 px, {args.solver} block-Jacobi PCG rtol 1e-4 max 100, "
                         "30 LM iterations",
             "observations": N,
             "precision": args.precision,

@@ -359,9 +359,12 @@ void vba_server_counters(vba_server* server, uint64_t* completed, uint64_t* batc
 void vba_server_destroy(vba_server* server);
 
 /* ---- per-kernel device time ----------------------------------------------
- * While enabled, HIP events are recorded around the launches of the kernel
- * families "linearize", "schur_apply", "pcg_pose", "pcg_step" and "cost" on
- * the context's stream. vba_profile_query returns the launches that did
+ * While enabled, HIP events are recorded on the context's stream around the
+ * launches of the kernel families "linearize", "schur_apply", "pcg_pose",
+ * "pcg_step", "cost", "explicit_assemble" and "explicit_apply": around every
+ * launch for enable = 1, around every enable-th launch of each family for
+ * enable > 1 (sampling keeps the events' host cost out of the solve), never
+ * for enable = 0. vba_profile_query returns the recorded launches that did
  * work, their summed device seconds and the algorithmic bytes they moved
  * (the minimum HBM traffic of the data layout, summed over launches). */
 void vba_profile_enable(vba_context* ctx, int32_t enable);

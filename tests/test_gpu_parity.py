@@ -39,7 +39,7 @@ def test_drifted_pose_recovered_on_gpu(device):
     assert np.linalg.norm(vba.camera_centres(out.poses)[2] - [1.0, 0.0, 0.0]) < 1e-6
 
 
-@pytest.mark.parametrize("solver", [abi.SOLVER_DENSE, abi.SOLVER_MATRIX_FREE])
+@pytest.mark.parametrize("solver", [abi.SOLVER_DENSE, abi.SOLVER_MATRIX_FREE, abi.SOLVER_PCG_EXPLICIT])
 def test_circle_parity(device, solver):
     s = vba.synthesize(abi.SYNTH_CIRCLE, num_cameras=20, num_landmarks=600, seed=1)
     o = vba.default_options(linear_solver=solver, max_iterations=20)
@@ -48,10 +48,13 @@ def test_circle_parity(device, solver):
     compare(gpu, ref, s.scene_extent)
 
 
+@pytest.mark.parametrize("solver", [abi.SOLVER_MATRIX_FREE, abi.SOLVER_PCG_EXPLICIT])
 @pytest.mark.parametrize("policy", [abi.PCG_MAXIT_REJECT, abi.PCG_MAXIT_ACCEPT_ITERATE])
-def test_loop_matrix_free_parity(device, policy):
+def test_loop_pcg_parity(device, policy, solver):
+    """The oracle runs the implicit operator for both PCG solvers: the
+    explicit matrix is the same S, so only rounding separates the two."""
     s = vba.synthesize(abi.SYNTH_LOOP, num_cameras=80, num_landmarks=6000, seed=2, scene_scale=16.0)
-    o = vba.default_options(linear_solver=abi.SOLVER_MATRIX_FREE, max_iterations=12, max_pcg_iterations=100,
+    o = vba.default_options(linear_solver=solver, max_iterations=12, max_pcg_iterations=100,
                             pcg_relative_tolerance=1e-4, robust_kernel=abi.KERNEL_HUBER, robust_scale=1.0,
                             pcg_maxit_policy=policy)
     ref = oracle.optimize(s.problem, o)
@@ -59,14 +62,15 @@ def test_loop_matrix_free_parity(device, policy):
     compare(gpu, ref, s.scene_extent, cost_rtol=1e-6)
 
 
-def test_mixed_precision_matches_fp64(device):
+@pytest.mark.parametrize("solver", [abi.SOLVER_MATRIX_FREE, abi.SOLVER_PCG_EXPLICIT])
+def test_mixed_precision_matches_fp64(device, solver):
     """Mixed precision stores G in FP32 and pairs it with the inverse of the
     V summed from those same FP32 blocks; the operator's arithmetic stays
     FP64 and PCG is checked against the FP64 true residual. On the reference
     GPU backend's PCG settings (rtol 1e-4, at most 100 iterations, the last
     iterate used at the cap) both precisions take the same LM path."""
     s = vba.synthesize(abi.SYNTH_LOOP, num_cameras=80, num_landmarks=6000, seed=3, scene_scale=16.0)
-    o = vba.default_options(linear_solver=abi.SOLVER_MATRIX_FREE, max_iterations=15, max_pcg_iterations=100,
+    o = vba.default_options(linear_solver=solver, max_iterations=15, max_pcg_iterations=100,
                             pcg_relative_tolerance=1e-4, robust_kernel=abi.KERNEL_HUBER, robust_scale=1.0,
                             pcg_maxit_policy=abi.PCG_MAXIT_ACCEPT_ITERATE)
     fp64 = device.optimize(s.problem, o)
@@ -82,15 +86,16 @@ def test_mixed_precision_matches_fp64(device):
 
 def test_stereo_rig_window_parity(device):
     s = vba.synthesize(abi.SYNTH_STEREO_WINDOW, num_cameras=10, num_landmarks=400, seed=6)
-    for solver in (abi.SOLVER_DENSE, abi.SOLVER_MATRIX_FREE):
+    for solver in (abi.SOLVER_DENSE, abi.SOLVER_MATRIX_FREE, abi.SOLVER_PCG_EXPLICIT):
         o = vba.default_options(linear_solver=solver, max_iterations=10, robust_kernel=abi.KERNEL_CAUCHY,
                                 robust_scale=1.0)
         compare(device.optimize(s.problem, o), oracle.optimize(s.problem, o), s.scene_extent)
 
 
-def test_city_opencv_rig_parity(device):
+@pytest.mark.parametrize("solver", [abi.SOLVER_MATRIX_FREE, abi.SOLVER_PCG_EXPLICIT])
+def test_city_opencv_rig_parity(device, solver):
     s = vba.synthesize(abi.SYNTH_CITY, num_cameras=60, num_landmarks=3000, seed=8, scene_scale=30.0)
-    o = vba.default_options(linear_solver=abi.SOLVER_MATRIX_FREE, max_iterations=8, max_pcg_iterations=100,
+    o = vba.default_options(linear_solver=solver, max_iterations=8, max_pcg_iterations=100,
                             pcg_relative_tolerance=1e-4, robust_kernel=abi.KERNEL_HUBER, robust_scale=2.0)
     compare(device.optimize(s.problem, o), oracle.optimize(s.problem, o), s.scene_extent, cost_rtol=1e-6)
 
@@ -123,5 +128,37 @@ def test_repeat_runs_are_bitwise_identical(device):
     a = device.optimize(s.problem, o)
     b = device.optimize(s.problem, o)
     assert a.status == b.status == abi.OK
+    assert np.array_equal(a.poses, b.poses) and np.array_equal(a.landmarks, b.landmarks)
+    assert a.result.final_cost == b.result.final_cost
+
+
+def test_explicit_matches_implicit_per_iteration(device):
+    """implicit_schur.rs:952-975 and :1190-1197 compare the implicit operator
+    with the explicitly formed Schur complement; here the two PCG solvers run
+    the same LM solve, so every LM iteration takes the same step decision,
+    the same PCG iteration count (one either way at a tie) and a cost within
+    FP64 rounding."""
+    s = vba.synthesize(abi.SYNTH_LOOP, num_cameras=60, num_landmarks=4000, seed=21, scene_scale=14.0)
+    o = vba.default_options(linear_solver=abi.SOLVER_MATRIX_FREE, max_iterations=8, max_pcg_iterations=100,
+                            pcg_relative_tolerance=1e-4, robust_kernel=abi.KERNEL_HUBER, robust_scale=1.0)
+    mf = device.optimize(s.problem, o)
+    o.linear_solver = abi.SOLVER_PCG_EXPLICIT
+    ex = device.optimize(s.problem, o)
+    assert mf.status == ex.status == abi.OK, ex.message
+    assert mf.accepted == ex.accepted
+    for a, b in zip(mf.iterations, ex.iterations):
+        assert abs(a["pcg_iterations"] - b["pcg_iterations"]) <= 1, (a, b)
+        assert abs(a["cost_after"] - b["cost_after"]) <= 1e-9 * abs(a["cost_after"]), (a, b)
+    rel = abs(mf.result.final_cost - ex.result.final_cost) / mf.result.final_cost
+    assert rel <= 1e-9, rel
+
+
+def test_explicit_repeat_is_bitwise_identical(device):
+    s = vba.synthesize(abi.SYNTH_LOOP, num_cameras=60, num_landmarks=4000, seed=12, scene_scale=14.0)
+    o = vba.default_options(linear_solver=abi.SOLVER_PCG_EXPLICIT, max_iterations=6, max_pcg_iterations=100,
+                            pcg_relative_tolerance=1e-4, precision=abi.PRECISION_MIXED)
+    a = device.optimize(s.problem, o)
+    b = device.optimize(s.problem, o)
+    assert a.status == b.status == abi.OK, a.message
     assert np.array_equal(a.poses, b.poses) and np.array_equal(a.landmarks, b.landmarks)
     assert a.result.final_cost == b.result.final_cost

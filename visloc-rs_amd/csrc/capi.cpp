@@ -373,7 +373,8 @@ void vba_prepared_batch_free(vba_context* ctx, vba_prepared_batch* batch) {
 void vba_profile_enable(vba_context* ctx, int32_t enable) {
     if (!ctx) return;
     std::lock_guard<std::mutex> lock(ctx->mu);
-    ctx->profile = enable != 0;
+    ctx->profile_stride = enable > 0 ? enable : 0;
+    ctx->profile_seen.assign(ctx->profile_seen.size(), 0);
 }
 
 void vba_profile_reset(vba_context* ctx) {

@@ -357,50 +357,7 @@ __device__ void dense_assemble_group(const DevProblem& P, const DensePairs& D, i
     const int sp = D.block_pq[2 * b], sq = D.block_pq[2 * b + 1];
     const int vp = P.pose_var[sp] - var_begin, vq = P.pose_var[sq] - var_begin;
     double acc[36];
-#pragma unroll
-    for (int i = 0; i < 36; ++i) acc[i] = 0.0;
-    for (int k = D.block_ptr[b] + g; k < D.block_ptr[b + 1]; k += kBlockLanes) {
-        const int l = D.terms[3 * (size_t)k], ep = D.terms[3 * (size_t)k + 1], eq = D.terms[3 * (size_t)k + 2];
-        const double* Vi = P.Vinv64 + 6 * (size_t)l;
-        const double vi[9] = {Vi[0], Vi[1], Vi[2], Vi[1], Vi[3], Vi[4], Vi[2], Vi[4], Vi[5]};
-        const double* A = P.Gl64 + 6 * (size_t)ep;
-        const double* B = P.Gl64 + 6 * (size_t)eq;
-        const double a[9] = {A[0], A[1], A[2], A[1], A[3], A[4], A[2], A[4], A[5]};
-        const double bb[9] = {B[0], B[1], B[2], B[1], B[3], B[4], B[2], B[4], B[5]};
-        double T[9], C[9];
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) T[3 * i + j] = a[3 * i] * vi[j] + a[3 * i + 1] * vi[3 + j] + a[3 * i + 2] * vi[6 + j];
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) C[3 * i + j] = T[3 * i] * bb[j] + T[3 * i + 1] * bb[3 + j] + T[3 * i + 2] * bb[6 + j];
-        const double* X = P.lms + 3 * (size_t)l;
-        const double K[9] = {0.0, -X[2], X[1], X[2], 0.0, -X[0], -X[1], X[0], 0.0};
-        double CK[9], KC[9], KCK[9];
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                CK[3 * i + j] = C[3 * i] * K[j] + C[3 * i + 1] * K[3 + j] + C[3 * i + 2] * K[6 + j];
-                KC[3 * i + j] = K[3 * i] * C[j] + K[3 * i + 1] * C[3 + j] + K[3 * i + 2] * C[6 + j];
-            }
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) KCK[3 * i + j] = KC[3 * i] * K[j] + KC[3 * i + 1] * K[3 + j] + KC[3 * i + 2] * K[6 + j];
-        // [[C, -C K], [K C, -K C K]]
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                acc[6 * i + j] += C[3 * i + j];
-                acc[6 * i + 3 + j] -= CK[3 * i + j];
-                acc[6 * (3 + i) + j] += KC[3 * i + j];
-                acc[6 * (3 + i) + 3 + j] -= KCK[3 * i + j];
-            }
-    }
+    schur_block_terms(P, D, b, g, kBlockLanes, acc);
 #pragma unroll
     for (int off = 1; off < kBlockLanes; off <<= 1)
 #pragma unroll

@@ -182,6 +182,10 @@ void launch_pcg_restart(const DevProblem& P, hipStream_t stream);
 // delta_l = -V^-1 (g_l + sum_e W_e^T x_p), trial landmarks, step maxima.
 void launch_backsub(const DevProblem& P, hipStream_t stream);
 
+// r_t.r_t (and with restart r_t.z) from the pose kernels' partials into the
+// PCG state; with restart the recursion continues from r_t.
+void launch_pcg_true_finish(const DevProblem& P, int32_t restart, hipStream_t stream);
+
 // Dense reduced system (LinearSolver::Dense).
 struct DensePairs {
     int32_t num_blocks;
@@ -190,6 +194,35 @@ struct DensePairs {
     const int32_t* terms;      // 3 per term: landmark slot, l-order edge of p, of q
     const int32_t* var_slot;   // pose slot of each variable pose
 };
+
+// Explicit block-sparse reduced camera matrix (explicit_schur.hip):
+// S = U + lambda I - sum_l W V^-1 W^T as 6x6 blocks between variable poses
+// that share a landmark, plus every diagonal block. Each upper block (p, q)
+// sums its term list (the DensePairs lists of the dense solver) in list
+// order. S is stored in full block rows (both S_pq and S_qp = S_pq^T),
+// columns ascending, each block row-major.
+struct ExplicitDev {
+    int32_t num_blocks, num_full;
+    const int32_t* block_pos;  // 2 per upper block: full positions of S_pq and S_qp (-1 on the diagonal)
+    const int32_t* row_ptr;    // num_pose_slots + 1
+    const int32_t* col;        // pose slot of each full position
+    const int32_t* diag_pos;   // full position of S_pp per pose slot (-1: fixed pose)
+    double* S64;               // 36 per full position
+    float* S32;                // FP32 copy for the mixed-precision matvec (null in FP64)
+};
+// S64 (and S32) from the term lists, U and lambda, then the block-Jacobi
+// inverses of the diagonal blocks into Minv (implicit_schur.rs:241-287).
+void launch_explicit_assemble(const DevProblem& P, const DensePairs& D, const ExplicitDev& X, double lambda,
+                              hipStream_t stream);
+// PCG iteration pcg_iter on the explicit matrix: d_new = z + beta d_old,
+// q = S d_new and d.q partials; the scalars come from the step kernel's
+// partials as for the implicit operator. fp32 reads S32.
+void launch_pcg_explicit(const DevProblem& P, const ExplicitDev& X, bool fp32, const double* d_old, double* d_new,
+                         int32_t pcg_iter, int32_t fresh, hipStream_t stream);
+// True residual b - S64 x and its norm (launch_pcg_true_finish); with
+// restart also r = r_t, z = M^-1 r_t and d = 0 (d is P.d).
+void launch_explicit_true_residual(const DevProblem& P, const ExplicitDev& X, int32_t restart, hipStream_t stream);
+
 void launch_dense_assemble(const DevProblem& P, const DensePairs& D, double* S, int32_t n,
                            double lambda, hipStream_t stream);
 // Solve S x = b (Cholesky, LU fallback) for the variable poses; writes x and

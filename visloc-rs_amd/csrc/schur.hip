@@ -28,88 +28,13 @@
 #include <cmath>
 
 #include "device_common.hpp"
+#include "pcg_device.hpp"
 
 namespace vba {
 
 namespace {
 
 enum SchurOp { OP_APPLY = 0, OP_RHS = 1, OP_PRECOND = 2, OP_BACKSUB = 3 };
-
-template <typename T>
-struct Vec2;
-template <>
-struct Vec2<float> {
-    using type = float2;
-};
-template <>
-struct Vec2<double> {
-    using type = double2;
-};
-
-// Six consecutive values at an address aligned to two of them.
-template <typename T>
-__device__ __forceinline__ void load6(const T* __restrict__ p, T v[6]) {
-    using V2 = typename Vec2<T>::type;
-    const V2* q = reinterpret_cast<const V2*>(p);
-    const V2 a = q[0], b = q[1], c = q[2];
-    v[0] = a.x;
-    v[1] = a.y;
-    v[2] = b.x;
-    v[3] = b.y;
-    v[4] = c.x;
-    v[5] = c.y;
-}
-
-// A flag another work-group may set during this kernel, read once per
-// work-group so that all its threads take the same branch.
-__device__ __forceinline__ int block_flag(int32_t* p, int* sh) {
-    if (threadIdx.x == 0) *sh = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    return *sh;
-}
-__device__ __forceinline__ void set_stop(PcgState* st, int status) {
-    st->status = status;
-    __hip_atomic_store(&st->stop, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Scalars of PCG iteration k, the same in every work-group: r.z and r.r of
-// the current residual from the step kernel's partials (or the totals the
-// init / restart kernels left in entry 0), the stop test on the recursive
-// residual (implicit_schur.rs:640-700) and beta = rho_k / rho_{k-1}.
-// Work-group 0 records them. Returns false when the solve has stopped.
-template <int NT>
-__device__ bool pcg_prologue(const DevProblem& P, int k, int fresh, double* beta_out, double* lds, int* flag) {
-    PcgState* st = P.pcg;
-    if (block_flag(&st->stop, flag)) return false;
-    double v[2] = {0.0, 0.0};
-    for (int i = threadIdx.x; i < P.num_cpart; i += NT) {
-        v[0] += P.cpart[2 * (size_t)i];
-        v[1] += P.cpart[2 * (size_t)i + 1];
-    }
-    block_sum_all<NT, double, 2>(v, lds);
-    const double rz = v[0];
-    double rn, beta = 0.0;
-    int status = PCG_RUNNING;
-    if (fresh) {
-        rn = st->rnorm;
-    } else {
-        rn = sqrt(v[1]);
-        if (!isfinite(rn)) status = PCG_NONFINITE;
-        else if (rn <= st->target) status = PCG_CONVERGED;
-        else if (k >= st->max_iter) status = PCG_MAXIT;
-        else if (!isfinite(rz) || !(rz > 0.0)) status = PCG_NONPOSITIVE;
-        beta = rz / st->rho[(k - 1) & 1];
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        st->rho[k & 1] = rz;
-        st->beta = beta;
-        st->iter = k;
-        st->rnorm = rn;
-        if (status != PCG_RUNNING) set_stop(st, status);
-    }
-    *beta_out = beta;
-    return status == PCG_RUNNING;
-}
 
 // y = [G u; X x (G u)]: the camera-side block W_e u of one edge.
 template <typename T>
@@ -501,25 +426,6 @@ __device__ __forceinline__ void pose_run_sum(const DevProblem& P, const double* 
         for (int i = 0; i < NV; ++i) s[i] += __shfl_xor(s[i], off, 64);
 }
 
-// Sum of one value per wave into a per-work-group partial, in wave order.
-__device__ __forceinline__ void wave_values_to_partial(const double v[2], double* lds, double* out) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) {
-        lds[2 * wave] = v[0];
-        lds[2 * wave + 1] = v[1];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double a = lds[0], b = lds[1];
-        for (int w = 1; w < kBlock / 64; ++w) {
-            a += lds[2 * w];
-            b += lds[2 * w + 1];
-        }
-        out[0] = a;
-        out[1] = b;
-    }
-}
-
 // q_c = (sum_j U_cj d_j) + lambda d_c - s_c on lane c < 6, d_j shuffled in.
 __device__ __forceinline__ double pose_apply_u(const DevProblem& P, int p, int c, double dc, double lambda,
                                                double sc) {
@@ -531,17 +437,6 @@ __device__ __forceinline__ double pose_apply_u(const DevProblem& P, int p, int c
 #pragma unroll
     for (int j = 0; j < 6; ++j) acc += U[c <= j ? up6(c, j) : up6(j, c)] * dv[j];
     return acc + lambda * dc - sc;
-}
-
-// z_c = sum_j Minv_cj r_j on lane c < 6 (r_j shuffled from lanes base + j).
-__device__ __forceinline__ double pose_minv(const DevProblem& P, int p, int c, int base, double rc, bool valid) {
-    double z = 0.0;
-#pragma unroll
-    for (int j = 0; j < 6; ++j) {
-        const double rj = __shfl(rc, min(base + j, 63), 64);
-        if (valid) z += P.Minv[36 * (size_t)p + 6 * c + j] * rj;
-    }
-    return z;
 }
 
 // b = -g_p + sum_e W_e V^-1 g_l (implicit_schur.rs:181-231).
@@ -836,6 +731,10 @@ void launch_pcg_true_residual(const DevProblem& P, hipStream_t s) {
     launch_schur_apply(P, false, P.x, nullptr, -1, 0, s);
     hipLaunchKernelGGL(k_pcg_true_pose, dim3(gather_blocks(P)), dim3(kBlock), 0, s, P, 0);
     hipLaunchKernelGGL(k_pcg_true_finish, dim3(1), dim3(kBlock), 0, s, P, 0);
+}
+
+void launch_pcg_true_finish(const DevProblem& P, int32_t restart, hipStream_t s) {
+    hipLaunchKernelGGL(k_pcg_true_finish, dim3(1), dim3(kBlock), 0, s, P, (int)restart);
 }
 
 void launch_pcg_restart(const DevProblem& P, hipStream_t s) {

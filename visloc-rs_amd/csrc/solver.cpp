@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <unordered_map>
 
 #include "host_common.hpp"
 
@@ -27,12 +28,15 @@ enum BufId {
     B_HL64, B_TILE27, B_U, B_G, B_V, B_GL, B_VINV64, B_VINV32, B_VM, B_MINV, B_VB, B_VX, B_VR, B_VZ, B_VD,
     B_VQ, B_PART64, B_PART21, B_BPART, B_CPART, B_PCG, B_DELTA_L, B_TILE_SCALAR,
     B_SCALARS, B_DENSE_BLOCK_PQ, B_DENSE_BLOCK_PTR, B_DENSE_TERMS, B_DENSE_VAR_SLOT, B_DENSE_S,
-    B_DENSE_WORK, B_DENSE_STATUS, B_COUNT
+    B_DENSE_WORK, B_DENSE_STATUS, B_X_BLOCK_POS, B_X_ROW_PTR, B_X_COL, B_X_DIAG_POS, B_X_S64, B_X_S32, B_VD2,
+    B_COUNT
 };
 
 // Profiled kernel families.
-enum ProfId { K_LINEARIZE, K_SCHUR_APPLY, K_PCG_POSE, K_PCG_STEP, K_COST, K_COUNT };
-const char* kProfNames[K_COUNT] = {"linearize", "schur_apply", "pcg_pose", "pcg_step", "cost"};
+enum ProfId { K_LINEARIZE, K_SCHUR_APPLY, K_PCG_POSE, K_PCG_STEP, K_COST, K_EXPLICIT_ASSEMBLE, K_EXPLICIT_APPLY,
+              K_COUNT };
+const char* kProfNames[K_COUNT] = {"linearize", "schur_apply",       "pcg_pose",      "pcg_step",
+                                   "cost",      "explicit_assemble", "explicit_apply"};
 
 struct Fail {
     int32_t code;
@@ -80,11 +84,18 @@ void build_dense_lists(const Model& m, DenseLists* out) {
     out->var_slot.assign(P, 0);
     for (uint32_t s = 0; s < m.num_pose_slots; ++s)
         if (m.pose_var[s] >= 0) out->var_slot[m.pose_var[s]] = (int32_t)s;
-    std::vector<int32_t> block_id((size_t)P * P, -1);
+    // block ids by (var p, var q): a P x P table while it stays small, else a hash map
+    const bool table = (uint64_t)P * P <= (1ull << 24);
+    std::vector<int32_t> block_id(table ? (size_t)P * P : 0, -1);
+    std::unordered_map<uint64_t, int32_t> block_map;
+    auto id_of = [&](int32_t a, int32_t b) -> int32_t& {
+        if (table) return block_id[(size_t)a * P + b];
+        return block_map.try_emplace(((uint64_t)(uint32_t)a << 32) | (uint32_t)b, -1).first->second;
+    };
     std::vector<int64_t> count;
     out->block_pq.clear();
     auto block_of = [&](int32_t a, int32_t b) {
-        int32_t& id = block_id[(size_t)a * P + b];
+        int32_t& id = id_of(a, b);
         if (id < 0) {
             id = (int32_t)count.size();
             count.push_back(0);
@@ -109,7 +120,7 @@ void build_dense_lists(const Model& m, DenseLists* out) {
     const size_t nb = count.size();
     int64_t total = 0;
     for (int64_t c : count) total += c;
-    if (total > (int64_t)INT32_MAX / 3) throw Fail{VBA_ERR_INVALID_ARGUMENT, "dense reduced system too large"};
+    if (total > (int64_t)INT32_MAX / 3) throw Fail{VBA_ERR_INVALID_ARGUMENT, "Schur term lists too large"};
     out->block_ptr.assign(nb + 1, 0);
     for (size_t k = 0; k < nb; ++k) out->block_ptr[k + 1] = out->block_ptr[k] + (int32_t)count[k];
     out->terms.assign(3 * (size_t)total, 0);
@@ -119,7 +130,7 @@ void build_dense_lists(const Model& m, DenseLists* out) {
         collect(l);
         for (size_t a = 0; a < ends.size(); ++a)
             for (size_t b = a; b < ends.size(); ++b) {
-                int32_t k = cursor[block_id[(size_t)ends[a].first * P + ends[b].first]]++;
+                int32_t k = cursor[id_of(ends[a].first, ends[b].first)]++;
                 out->terms[3 * (size_t)k] = (int32_t)l;
                 out->terms[3 * (size_t)k + 1] = ends[a].second;
                 out->terms[3 * (size_t)k + 2] = ends[b].second;
@@ -134,6 +145,43 @@ void build_dense(Prepared& pr) {
     pr.dense_terms = std::move(d.terms);
     pr.dense_var_slot = std::move(d.var_slot);
     pr.dense_built = true;
+}
+
+// Full block rows of the explicit reduced camera matrix over the term lists'
+// upper blocks: row p holds S_pq for every block (p, q) and S_qp^T for every
+// block (q, p), columns ascending.
+void build_explicit(Prepared& pr) {
+    const Model& m = pr.m;
+    const size_t nb = pr.dense_block_pq.size() / 2, NP = m.num_pose_slots;
+    std::vector<int32_t> count(NP + 1, 0);
+    for (size_t b = 0; b < nb; ++b) {
+        const int32_t p = pr.dense_block_pq[2 * b], q = pr.dense_block_pq[2 * b + 1];
+        ++count[p + 1];
+        if (p != q) ++count[q + 1];
+    }
+    for (size_t p = 0; p < NP; ++p) count[p + 1] += count[p];
+    pr.x_row_ptr = count;
+    const size_t nfull = (size_t)count[NP];
+    // (column, block * 2 + transposed) per full position, then each row by column
+    std::vector<std::pair<int32_t, int32_t>> ent(nfull);
+    std::vector<int32_t> cursor(count.begin(), count.end() - 1);
+    for (size_t b = 0; b < nb; ++b) {
+        const int32_t p = pr.dense_block_pq[2 * b], q = pr.dense_block_pq[2 * b + 1];
+        ent[cursor[p]++] = {q, (int32_t)(2 * b)};
+        if (p != q) ent[cursor[q]++] = {p, (int32_t)(2 * b + 1)};
+    }
+    pr.x_col.assign(nfull, 0);
+    pr.x_block_pos.assign(2 * nb, -1);
+    pr.x_diag_pos.assign(NP, -1);
+    for (size_t p = 0; p < NP; ++p) {
+        std::sort(ent.begin() + count[p], ent.begin() + count[p + 1]);
+        for (int32_t t = count[p]; t < count[p + 1]; ++t) {
+            pr.x_col[t] = ent[t].first;
+            pr.x_block_pos[ent[t].second] = t;
+            if (ent[t].first == (int32_t)p) pr.x_diag_pos[p] = t;
+        }
+    }
+    pr.explicit_built = true;
 }
 
 // ---- profiling with HIP events on the context stream ----
@@ -151,9 +199,12 @@ struct ProfScope {
     Context& ctx;
     int kernel;
     double bytes;
+    int32_t pcg_iter;
     hipEvent_t a = nullptr;
-    ProfScope(Context& c, int k, double b) : ctx(c), kernel(k), bytes(b) {
-        if (ctx.profile) {
+    ProfScope(Context& c, int k, double b, int32_t it = -1) : ctx(c), kernel(k), bytes(b), pcg_iter(it) {
+        if (ctx.profile_stride > 0) {
+            if (ctx.profile_seen.size() < K_COUNT) ctx.profile_seen.resize(K_COUNT, 0);
+            if (ctx.profile_seen[kernel]++ % (uint64_t)ctx.profile_stride != 0) return;
             a = take_event(ctx);
             hip_check(hipEventRecord(a, ctx.stream), "hipEventRecord");
         }
@@ -173,7 +224,7 @@ struct ProfScope {
             ctx.event_pool.push_back(b);
             return;
         }
-        ctx.pending.push_back({a, b, kernel, bytes});
+        ctx.pending.push_back({a, b, kernel, bytes, pcg_iter});
     }
 };
 // Fold finished event pairs into the per-kernel totals (after a sync).
@@ -200,10 +251,11 @@ void harvest(Context& ctx) {
     ctx.pending.clear();
 }
 
-// Keep only the first `keep` pending launches of `kernel` in the current batch.
-void keep_first(Context& ctx, size_t from, int kernel, int keep) {
+// Drop the pending PCG launches of iterations the device did not run (it
+// had stopped; those launches exited at once).
+void drop_after(Context& ctx, size_t from, int32_t iterations_run) {
     for (size_t i = from; i < ctx.pending.size(); ++i)
-        if (ctx.pending[i].kernel == kernel && keep-- <= 0) ctx.pending[i].kernel = -1;
+        if (ctx.pending[i].pcg_iter >= iterations_run) ctx.pending[i].kernel = -1;
 }
 
 }  // namespace
@@ -457,7 +509,8 @@ int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stat
             vba_result* res) {
     hipStream_t s = ctx.stream;
     const Model& m = pr.m;
-    const bool matrix_free = o.linear_solver != VBA_SOLVER_DENSE;
+    const bool matrix_free = o.linear_solver != VBA_SOLVER_DENSE;  // a PCG solver
+    const bool xplicit = o.linear_solver == VBA_SOLVER_PCG_EXPLICIT;
     const bool mixed = matrix_free && o.precision == VBA_PRECISION_MIXED;
     const bool accept_iterate = o.pcg_maxit_policy == VBA_PCG_MAXIT_ACCEPT_ITERATE;
     try {
@@ -471,7 +524,7 @@ int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stat
         double* dwork = nullptr;
         int32_t* dstatus = nullptr;
         const int32_t ndense = (int32_t)(6 * m.P);
-        if (!matrix_free) {
+        if (!matrix_free || xplicit) {
             if (!pr.dense_built) {
                 build_dense(pr);
                 upload(pr, B_DENSE_BLOCK_PQ, pr.dense_block_pq, s);
@@ -484,6 +537,29 @@ int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stat
             D.block_ptr = buf<int32_t>(pr, B_DENSE_BLOCK_PTR);
             D.terms = buf<int32_t>(pr, B_DENSE_TERMS);
             D.var_slot = buf<int32_t>(pr, B_DENSE_VAR_SLOT);
+        }
+        ExplicitDev X{};
+        double* dbuf[2] = {P.d, nullptr};  // PCG direction by iteration parity (explicit)
+        if (xplicit) {
+            if (!pr.explicit_built) {
+                build_explicit(pr);
+                upload(pr, B_X_BLOCK_POS, pr.x_block_pos, s);
+                upload(pr, B_X_ROW_PTR, pr.x_row_ptr, s);
+                upload(pr, B_X_COL, pr.x_col, s);
+                upload(pr, B_X_DIAG_POS, pr.x_diag_pos, s);
+            }
+            X.num_blocks = D.num_blocks;
+            X.num_full = (int32_t)pr.x_col.size();
+            X.block_pos = buf<int32_t>(pr, B_X_BLOCK_POS);
+            X.row_ptr = buf<int32_t>(pr, B_X_ROW_PTR);
+            X.col = buf<int32_t>(pr, B_X_COL);
+            X.diag_pos = buf<int32_t>(pr, B_X_DIAG_POS);
+            X.S64 = (double*)ensure(pr, B_X_S64, 36 * (size_t)X.num_full * sizeof(double));
+            X.S32 = mixed ? (float*)ensure(pr, B_X_S32, 36 * (size_t)X.num_full * sizeof(float)) : nullptr;
+            dbuf[1] = (double*)ensure(pr, B_VD2, 6 * NP * sizeof(double));
+            hip_check(hipMemsetAsync(dbuf[1], 0, 6 * NP * sizeof(double), s), "memset");
+        }
+        if (!matrix_free) {
             dS = (double*)ensure(pr, B_DENSE_S, (size_t)ndense * ndense * sizeof(double));
             dwork = (double*)ensure(pr, B_DENSE_WORK, ((size_t)ndense * ndense + ndense) * sizeof(double));
             dstatus = (int32_t*)ensure(pr, B_DENSE_STATUS, sizeof(int32_t));
@@ -502,6 +578,14 @@ int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stat
         const double bytes_linearize = m.num_obs * 16.0 + NE * (4.0 + 4.0 + 4.0 + 48.0 + 24.0 + (mixed ? 24.0 : 0.0)) +
                                        NL * 24.0 + NT * (16.0 + 27.0 * 8.0) + NP * 56.0;
         const double bytes_cost = m.num_obs * 16.0 + NE * 8.0 + NL * 24.0 + NT * (16.0 + 24.0) + NP * 56.0;
+        // explicit matrix: per term its indices (12); per edge G and per
+        // landmark X and V^-1 once; per stored block 36 values (FP64, and the
+        // FP32 copy); per pose U, the diagonal read and Minv. Product: per
+        // stored block 36 reals and its column; per pose z, d_old, d_new, q.
+        const double nterms = xplicit ? (double)pr.dense_terms.size() / 3.0 : 0.0;
+        const double bytes_xassemble = nterms * 12.0 + NE * 48.0 + NL * 72.0 +
+                                       X.num_full * 36.0 * (8.0 + (mixed ? 4.0 : 0.0)) + NP * (168.0 + 288.0 + 288.0);
+        const double bytes_xapply = X.num_full * (36.0 * real + 4.0) + NP * (4.0 + 4.0 * 48.0);
 
         hip_check(hipEventRecord(ctx.ev_start, s), "event");
 
@@ -572,7 +656,12 @@ int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stat
                 hip_check(hipStreamSynchronize(s), "sync");
                 solved = *ctx.host_status == 0;
             } else {
-                launch_precondition(P, lambda, s);
+                if (xplicit) {
+                    ProfScope a(ctx, K_EXPLICIT_ASSEMBLE, bytes_xassemble);
+                    launch_explicit_assemble(P, D, X, lambda, s);
+                } else {
+                    launch_precondition(P, lambda, s);
+                }
                 launch_pcg_init(P, o.pcg_relative_tolerance, o.pcg_absolute_tolerance, max_pcg, lambda, s);
                 // PCG iterations k .. in chunks; the device stops itself, the
                 // host reads the state once per chunk.
@@ -584,24 +673,36 @@ int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stat
                         const size_t batch = ctx.pending.size();
                         const int32_t k_end = std::min(k + chunk, max_pcg);
                         for (int32_t kk = k; kk < k_end; ++kk) {
+                            if (xplicit) {
+                                // iteration kk reads d from dbuf[kk & 1] and writes the next
+                                double* d_new = dbuf[(kk + 1) & 1];
+                                {
+                                    ProfScope a(ctx, K_EXPLICIT_APPLY, bytes_xapply, kk);
+                                    launch_pcg_explicit(P, X, mixed, dbuf[kk & 1], d_new, kk, fresh && kk == k, s);
+                                }
+                                DevProblem Ps = P;
+                                Ps.d = d_new;
+                                ProfScope c(ctx, K_PCG_STEP, bytes_pcg_step, kk);
+                                launch_pcg_step(Ps, kk, s);
+                                continue;
+                            }
                             {
-                                ProfScope a(ctx, K_SCHUR_APPLY, bytes_apply);
+                                ProfScope a(ctx, K_SCHUR_APPLY, bytes_apply, kk);
                                 launch_schur_apply(P, mixed, P.z, P.d, kk,
                                                    fresh && kk == k, s);
                             }
                             {
-                                ProfScope b(ctx, K_PCG_POSE, bytes_pcg_pose);
+                                ProfScope b(ctx, K_PCG_POSE, bytes_pcg_pose, kk);
                                 launch_pcg_pose(P, s);
                             }
                             {
-                                ProfScope c(ctx, K_PCG_STEP, bytes_pcg_step);
+                                ProfScope c(ctx, K_PCG_STEP, bytes_pcg_step, kk);
                                 launch_pcg_step(P, kk, s);
                             }
                         }
                         launch_pcg_status(P, k_end, fresh && k_end == k, s);
                         ps = read_pcg();
-                        const int done = std::max(0, ps.iter - k);
-                        for (int kind : {K_SCHUR_APPLY, K_PCG_POSE, K_PCG_STEP}) keep_first(ctx, batch, kind, done);
+                        drop_after(ctx, batch, ps.iter);
                         harvest(ctx);
                         if (ps.stop) break;
                         if (ps.iter <= k) throw Fail{VBA_ERR_DEVICE, "PCG made no progress"};
@@ -613,7 +714,8 @@ int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stat
                         break;
                     }
                     // true residual re-check (implicit_schur.rs:613-690, 750-761)
-                    launch_pcg_true_residual(P, s);
+                    if (xplicit) launch_explicit_true_residual(P, X, 0, s);
+                    else launch_pcg_true_residual(P, s);
                     ps = read_pcg();
                     if (std::isfinite(ps.true_norm) && ps.true_norm <= ps.target) break;
                     if (!std::isfinite(ps.true_norm)) {
@@ -622,7 +724,14 @@ int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stat
                     }
                     if (mixed && ps.iter < ps.max_iter) {
                         // FP64 refinement: continue from the true residual
-                        launch_pcg_restart(P, s);
+                        if (xplicit) {
+                            // the next iteration (ps.iter) starts fresh from d = 0
+                            DevProblem Pr = P;
+                            Pr.d = dbuf[ps.iter & 1];
+                            launch_explicit_true_residual(Pr, X, 1, s);
+                        } else {
+                            launch_pcg_restart(P, s);
+                        }
                         ps = read_pcg();
                         if (ps.stop) {
                             solved = ps.status == PCG_MAXIT && accept_iterate;

@@ -40,13 +40,16 @@ struct Context {
     StepScalars* host_scalars = nullptr;
     PcgState* host_pcg = nullptr;
     int32_t* host_status = nullptr;
-    // profiling
-    bool profile = false;
+    // profiling: events around every `profile_stride`-th launch of each
+    // kernel family (0: off)
+    int32_t profile_stride = 0;
     std::vector<KernelProfile> kernels;
+    std::vector<uint64_t> profile_seen;  // launches per family since enabling
     struct Pending {
         hipEvent_t a, b;
         int kernel;
         double bytes;
+        int32_t pcg_iter;  // PCG iteration of the launch (-1: not a PCG launch)
     };
     std::vector<Pending> pending;
     std::vector<hipEvent_t> event_pool;
@@ -60,6 +63,10 @@ struct Prepared {
     std::vector<DevBuffer> bufs;
     bool dense_built = false;
     std::vector<int32_t> dense_block_pq, dense_block_ptr, dense_terms, dense_var_slot;
+    // explicit reduced camera matrix (over the dense term lists): full block
+    // rows, the full positions of each upper block and of each diagonal
+    bool explicit_built = false;
+    std::vector<int32_t> x_row_ptr, x_col, x_block_pos, x_diag_pos;
     int final_slot = -1;  // buffer holding the final poses (-1: none yet)
     int final_lm_slot = -1;
 };

@@ -262,10 +262,12 @@ class Device:
     def prepare(self, problem: BundleProblem, options: abi.Options) -> "PreparedProblem":
         return PreparedProblem(self, problem, options)
 
-    def profile(self, enable: bool = True, reset: bool = True):
+    def profile(self, enable: bool = True, reset: bool = True, stride: int = 1):
+        """Record kernel times around every `stride`-th launch of each kernel
+        family (vba_profile_enable)."""
         if reset:
             self.lib.vba_profile_reset(self.ctx)
-        self.lib.vba_profile_enable(self.ctx, 1 if enable else 0)
+        self.lib.vba_profile_enable(self.ctx, max(1, int(stride)) if enable else 0)
 
     def profile_query(self, kernel: str) -> dict:
         launches = C.c_uint64()
