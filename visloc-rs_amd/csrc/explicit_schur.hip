@@ -129,9 +129,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_explicit(DevProblem P, ExplicitD
                                                          const double* __restrict__ d_old, double* __restrict__ d_new,
                                                          int k, int fresh) {
     __shared__ double lds[(kBlock / 64) * 2];
-    __shared__ int sFlag;
     double beta;
-    if (!pcg_prologue<kBlock>(P, k, fresh, &beta, lds, &sFlag)) return;
+    if (!pcg_prologue(P, k, fresh, &beta)) return;
     const int lane = threadIdx.x & 63;
     const int p = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
     double v[2] = {0.0, 0.0};

@@ -102,17 +102,19 @@ struct DevProblem {
     double* VinvM;   // its damped inverse, the one the mixed operator uses
     double* Minv;    // 36 per pose slot (block-Jacobi inverse)
     // Schur operator structure. The edges of a landmark tile, sorted by pose
-    // (tile_perm: tile-local l-order index at each sorted position), form
-    // camera runs; a run leaves one partial at the edge index of its last
-    // sorted position, and a pose sums its runs (pose_runs[pose_run_ptr[p]
-    // ..], ascending) in that order.
+    // (tile_perm: tile-local l-order index at each sorted position; s_lm:
+    // tile-local landmark), form camera runs; a run leaves one partial in
+    // its slot (run_slot at the run's last sorted position), the slots of
+    // pose p being pose_run_ptr[p] .. pose_run_ptr[p + 1] - 1 in ascending
+    // storage order, and a pose sums its slots in that order.
     const uint16_t* tile_perm;
+    const uint8_t* s_lm;
     const int32_t* big_tiles;  // tiles of one landmark with more than kTileEdges edges
     int32_t num_big_tiles, pad1;
     const int32_t* pose_run_ptr;  // num_pose_slots + 1
-    const int32_t* pose_runs;     // run-end edge indices
-    double* part64;  // 6 per edge index (run ends only)
-    double* part21;  // 21 per edge index (block-Jacobi blocks)
+    const int32_t* run_slot;      // per storage position: slot of the run ending there, else -1
+    double* part64;  // 6 per run slot
+    double* part21;  // 21 per run slot (block-Jacobi blocks)
     // PCG vectors, 6 per pose slot
     double *b, *x, *r, *z, *d, *q;
     double* bpart;  // per work-group partials of the pose kernels (2 each)

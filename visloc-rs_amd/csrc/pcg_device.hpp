@@ -36,40 +36,58 @@ __device__ __forceinline__ void load6(const T* __restrict__ p, T v[6]) {
     v[5] = c.y;
 }
 
-// A flag another work-group may set during this kernel, read once per
-// work-group so that all its threads take the same branch.
-__device__ __forceinline__ int block_flag(int32_t* p, int* sh) {
-    if (threadIdx.x == 0) *sh = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    return *sh;
-}
 __device__ __forceinline__ void set_stop(PcgState* st, int status) {
     st->status = status;
     __hip_atomic_store(&st->stop, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Scalars of PCG iteration k, the same in every work-group: r.z and r.r of
-// the current residual from the step kernel's partials (or the totals the
-// init / restart kernels left in entry 0), the stop test on the recursive
-// residual (implicit_schur.rs:640-700) and beta = rho_k / rho_{k-1}.
-// Work-group 0 records them. Returns false when the solve has stopped.
-template <int NT>
-__device__ bool pcg_prologue(const DevProblem& P, int k, int fresh, double* beta_out, double* lds, int* flag) {
-    PcgState* st = P.pcg;
-    if (block_flag(&st->stop, flag)) return false;
-    double v[2] = {0.0, 0.0};
-    for (int i = threadIdx.x; i < P.num_cpart; i += NT) {
-        v[0] += P.cpart[2 * (size_t)i];
-        v[1] += P.cpart[2 * (size_t)i + 1];
+// The PCG stop flag, read by each wave on its own. During a launch it can
+// only be set by work-group 0 for a status that every wave derives itself
+// from the same inputs (pcg_prologue), so a wave that reads it late still
+// takes the same branch as the others.
+__device__ __forceinline__ int wave_stop(PcgState* st) {
+    return __builtin_amdgcn_readfirstlane(__hip_atomic_load(&st->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
+// Sum over a wave with the same result in every lane (xor butterfly:
+// each step adds the same two values in every lane).
+__device__ __forceinline__ double wave_allsum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Sum of pairs (a[2i], a[2i+1]) over i < n, the same in every lane of every
+// wave: lanes stride the pairs, then a butterfly.
+__device__ __forceinline__ void wave_pair_total(const double* __restrict__ a, int n, double* s0, double* s1) {
+    const int lane = threadIdx.x & 63;
+    double v0 = 0.0, v1 = 0.0;
+    for (int i = lane; i < n; i += 64) {
+        v0 += a[2 * (size_t)i];
+        v1 += a[2 * (size_t)i + 1];
     }
-    block_sum_all<NT, double, 2>(v, lds);
-    const double rz = v[0];
+    *s0 = wave_allsum(v0);
+    *s1 = wave_allsum(v1);
+}
+
+// Scalars of PCG iteration k, derived by every wave on its own without
+// synchronisation: r.z and r.r of the current residual from the step
+// kernel's partials (or the totals the init / restart kernels left in entry
+// 0), the stop test on the recursive residual (implicit_schur.rs:640-700)
+// and beta = rho_k / rho_{k-1}. Thread 0 of work-group 0 records them.
+// Returns false when the solve has stopped; the result is the same in every
+// wave of the launch.
+__device__ bool pcg_prologue(const DevProblem& P, int k, int fresh, double* beta_out) {
+    PcgState* st = P.pcg;
+    if (wave_stop(st)) return false;
+    double rz, rr;
+    wave_pair_total(P.cpart, P.num_cpart, &rz, &rr);
     double rn, beta = 0.0;
     int status = PCG_RUNNING;
     if (fresh) {
         rn = st->rnorm;
     } else {
-        rn = sqrt(v[1]);
+        rn = sqrt(rr);
         if (!isfinite(rn)) status = PCG_NONFINITE;
         else if (rn <= st->target) status = PCG_CONVERGED;
         else if (k >= st->max_iter) status = PCG_MAXIT;

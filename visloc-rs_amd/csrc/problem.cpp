@@ -77,11 +77,13 @@ struct IdIndex {
 void build_runs(Model* mp) {
     Model& m = *mp;
     m.tile_perm.assign(m.num_edges, 0);
+    m.s_lm.assign(m.num_edges, 0);
     m.s_pose.assign(m.num_edges, 0);
     m.lm_store.assign(m.num_edges, 0);
     m.big_tiles.clear();
     std::vector<int32_t> run_pose, run_end, count;
     std::vector<uint64_t> keys;
+    std::vector<uint8_t> lm_of_k;
     for (size_t t = 0; t < m.lm_tiles.size(); ++t) {
         const int32_t e0 = m.lm_edge_ptr[m.lm_tiles[t].lm_begin], e1 = m.lm_edge_ptr[m.lm_tiles[t].lm_end];
         const int32_t n = e1 - e0;
@@ -108,9 +110,16 @@ void build_runs(Model* mp) {
                 for (int32_t s = 0; s < n; ++s) m.tile_perm[e0 + s] = (uint16_t)(keys[s] & 0xffff);
             }
         }
+        // tile-local landmark of each l-order index (a tile holds at most
+        // kTileLandmarks landmarks; a tile of one landmark has local index 0)
+        lm_of_k.assign(n, 0);
+        for (int32_t l = m.lm_tiles[t].lm_begin; l < m.lm_tiles[t].lm_end; ++l)
+            for (int32_t k = m.lm_edge_ptr[l] - e0; k < m.lm_edge_ptr[l + 1] - e0; ++k)
+                lm_of_k[k] = (uint8_t)(l - m.lm_tiles[t].lm_begin);
         for (int32_t s = 0; s < n; ++s) {
             const int32_t k = n > kTileEdges ? s : m.tile_perm[e0 + s];
             m.s_pose[e0 + s] = m.l_pose[e0 + k];
+            m.s_lm[e0 + s] = lm_of_k[k];
             m.lm_store[e0 + k] = e0 + s;
         }
         for (int32_t s = 0; s < n; ++s)
@@ -124,9 +133,10 @@ void build_runs(Model* mp) {
     m.pose_run_ptr.assign(m.num_pose_slots + 1, 0);
     for (int32_t p : run_pose) ++m.pose_run_ptr[p + 1];
     for (uint32_t s = 0; s < m.num_pose_slots; ++s) m.pose_run_ptr[s + 1] += m.pose_run_ptr[s];
-    m.pose_runs.assign(run_pose.size(), 0);
+    // slots by pose, each pose's runs in ascending storage order
+    m.run_slot.assign(m.num_edges, -1);
     std::vector<int32_t> cursor(m.pose_run_ptr.begin(), m.pose_run_ptr.end() - 1);
-    for (size_t r = 0; r < run_pose.size(); ++r) m.pose_runs[cursor[run_pose[r]]++] = run_end[r];
+    for (size_t r = 0; r < run_pose.size(); ++r) m.run_slot[run_end[r]] = cursor[run_pose[r]]++;
 }
 
 uint32_t next_pow2(uint64_t v) {
@@ -332,17 +342,28 @@ int32_t build_model(const vba_problem& pb, Model* mp, uint64_t* error_id, std::s
         osensor[k] = (uint16_t)(s + 1);
     }
     m.any_rig = nr > 0;
-    // Landmark slots in order of their first observing pose slot (ties in id
-    // order): the landmarks of a tile then share cameras, so its edges form
-    // few, long camera runs. Outputs map back through lm_slot_of_input.
+    // Landmark slots in order of their first, then last, observing pose slot
+    // (ties in id order): the landmarks of a tile then span few cameras, so
+    // its edges form few, long camera runs. Two stable counting sorts, by the
+    // last pose and then by the first. Outputs map back through
+    // lm_slot_of_input.
     {
-        std::vector<uint32_t> first(m.num_lm_slots, m.num_pose_slots);
-        for (uint64_t i = 0; i < n; ++i) first[olm[i]] = std::min(first[olm[i]], opose[i]);
-        std::vector<uint32_t> start(m.num_pose_slots + 2, 0);
+        std::vector<uint32_t> first(m.num_lm_slots, m.num_pose_slots), last(m.num_lm_slots, 0);
+        for (uint64_t i = 0; i < n; ++i) {
+            first[olm[i]] = std::min(first[olm[i]], opose[i]);
+            last[olm[i]] = std::max(last[olm[i]], opose[i]);
+        }
+        for (uint32_t l = 0; l < m.num_lm_slots; ++l)
+            if (first[l] == m.num_pose_slots) last[l] = m.num_pose_slots;  // unobserved: at the end
+        std::vector<uint32_t> start(m.num_pose_slots + 2, 0), by_last(m.num_lm_slots);
+        for (uint32_t l = 0; l < m.num_lm_slots; ++l) ++start[last[l] + 1];
+        for (uint32_t p = 0; p <= m.num_pose_slots; ++p) start[p + 1] += start[p];
+        for (uint32_t l = 0; l < m.num_lm_slots; ++l) by_last[start[last[l]]++] = l;
+        start.assign(m.num_pose_slots + 2, 0);
         for (uint32_t l = 0; l < m.num_lm_slots; ++l) ++start[first[l] + 1];
         for (uint32_t p = 0; p <= m.num_pose_slots; ++p) start[p + 1] += start[p];
         std::vector<uint32_t> new_slot(m.num_lm_slots);
-        for (uint32_t l = 0; l < m.num_lm_slots; ++l) new_slot[l] = start[first[l]]++;
+        for (uint32_t l : by_last) new_slot[l] = start[first[l]]++;
         for (uint64_t i = 0; i < n; ++i) olm[i] = new_slot[olm[i]];
         for (uint32_t& s : m.lm_slot_of_input) s = new_slot[s];
         std::vector<double> lms(m.lms.size());
@@ -438,19 +459,39 @@ int32_t build_model(const vba_problem& pb, Model* mp, uint64_t* error_id, std::s
             m.cam_tiles.push_back({(int32_t)s, b, std::min<int32_t>(b + (int32_t)tc, m.cam_edge_ptr[s + 1]), 0});
     }
     m.cam_tile_ptr[m.num_pose_slots] = (int32_t)m.cam_tiles.size();
-    // Landmark tiles: at most 256 landmarks and 1024 edges, the LDS staging
-    // capacity of the landmark kernels (a single landmark with more edges
-    // forms its own tile).
+    // Landmark tiles: runs of whole landmarks with at most kTileEdges edges,
+    // kTileLandmarks landmarks and kTileRuns distinct poses; a landmark with
+    // more than kTileRuns edges (each a distinct pose) forms its own tile.
     m.lm_tiles.clear();
     m.max_lm_tile_edges = 0;
-    for (uint32_t b = 0; b < m.num_lm_slots;) {
-        uint32_t e = b + 1;
-        while (e < m.num_lm_slots && e - b < (uint32_t)kTileLandmarks &&
-               m.lm_edge_ptr[e + 1] - m.lm_edge_ptr[b] <= kTileEdges)
-            ++e;
-        m.lm_tiles.push_back({(int32_t)b, (int32_t)e});
-        m.max_lm_tile_edges = std::max(m.max_lm_tile_edges, m.lm_edge_ptr[e] - m.lm_edge_ptr[b]);
-        b = e;
+    {
+        std::vector<int32_t> mark(m.num_pose_slots, -1);  // tile that last counted the pose
+        auto edges_of = [&](uint32_t l) { return m.lm_edge_ptr[l + 1] - m.lm_edge_ptr[l]; };
+        for (uint32_t b = 0; b < m.num_lm_slots;) {
+            const int32_t tile = (int32_t)m.lm_tiles.size();
+            uint32_t e = b + 1;
+            if (edges_of(b) <= kTileRuns) {
+                int32_t poses = 0;
+                for (int32_t k = m.lm_edge_ptr[b]; k < m.lm_edge_ptr[b + 1]; ++k)
+                    if (mark[m.l_pose[k]] != tile) {
+                        mark[m.l_pose[k]] = tile;
+                        ++poses;
+                    }
+                while (e < m.num_lm_slots && e - b < (uint32_t)kTileLandmarks && edges_of(e) <= kTileRuns &&
+                       m.lm_edge_ptr[e + 1] - m.lm_edge_ptr[b] <= kTileEdges) {
+                    int32_t fresh = 0;
+                    for (int32_t k = m.lm_edge_ptr[e]; k < m.lm_edge_ptr[e + 1]; ++k)
+                        fresh += mark[m.l_pose[k]] != tile;
+                    if (poses + fresh > kTileRuns) break;
+                    for (int32_t k = m.lm_edge_ptr[e]; k < m.lm_edge_ptr[e + 1]; ++k) mark[m.l_pose[k]] = tile;
+                    poses += fresh;
+                    ++e;
+                }
+            }
+            m.lm_tiles.push_back({(int32_t)b, (int32_t)e});
+            m.max_lm_tile_edges = std::max(m.max_lm_tile_edges, m.lm_edge_ptr[e] - m.lm_edge_ptr[b]);
+            b = e;
+        }
     }
     build_runs(&m);
     return VBA_OK;

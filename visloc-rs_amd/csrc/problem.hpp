@@ -49,11 +49,14 @@ struct LmTile {
     int32_t lm_end;
 };
 
-// Landmark tiles: at most this many whole landmarks and l-order edges, the
-// LDS staging capacity of the landmark-tile kernels (a single landmark with
-// more edges forms a tile of its own).
-constexpr int kTileEdges = 1024;
-constexpr int kTileLandmarks = 256;
+// Landmark tiles: runs of whole landmarks, each the work of one wave in the
+// operator kernel: at most kTileEdges edges (four per lane), kTileLandmarks
+// landmarks (one per lane) and kTileRuns camera runs (distinct poses, one
+// LDS accumulator each). A landmark with more than kTileRuns edges forms a
+// tile of its own, in which every edge is a run.
+constexpr int kTileEdges = 256;
+constexpr int kTileLandmarks = 64;
+constexpr int kTileRuns = 64;
 
 struct Model {
     uint32_t num_pose_slots = 0, num_lm_slots = 0;
@@ -85,16 +88,19 @@ struct Model {
     int32_t max_lm_tile_edges = 0;
     // Camera runs of the landmark tiles: tile_perm[e] = tile-local l-order
     // index at sorted position e - (tile's first edge), sorting the tile's
-    // edges by pose; a run is a maximal group of equal poses in that order
-    // and is named by the edge index of its last sorted position. Each pose
-    // lists its runs in ascending order (pose_run_ptr / pose_runs).
+    // edges by pose; a run is a maximal group of equal poses in that order.
     // Per-edge device data (G, h) is stored in this sorted order: s_pose is
-    // the pose at each storage position, lm_store the storage position of
-    // each l-order edge.
+    // the pose and s_lm the tile-local landmark at each storage position,
+    // lm_store the storage position of each l-order edge. A run leaves its
+    // partial in a slot: the slots of pose p are pose_run_ptr[p] ..
+    // pose_run_ptr[p + 1] - 1, its runs in ascending storage order, and
+    // run_slot gives the slot at the storage position of each run's last edge
+    // (-1 at other positions).
     std::vector<uint16_t> tile_perm;
+    std::vector<uint8_t> s_lm;
     std::vector<int32_t> s_pose, lm_store;
     std::vector<int32_t> big_tiles;  // tiles of one landmark with more than kTileEdges edges
-    std::vector<int32_t> pose_run_ptr, pose_runs;
+    std::vector<int32_t> pose_run_ptr, run_slot;
     uint64_t num_runs = 0;
 };
 

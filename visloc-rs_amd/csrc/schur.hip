@@ -93,13 +93,16 @@ __device__ __forceinline__ void edge_wtd(const TS Gs[6], const double d[6], cons
 }
 
 // Inclusive segmented scan over the 64 lanes of a wave: a lane with
-// `head` set starts a new segment. Fixed combination order.
+// `head` set starts a new segment. Fixed combination order. The scan stops
+// once no lane's segment reaches further back: `head` then covers every
+// lane's window, and no later step would add anything.
 template <int NV>
 __device__ __forceinline__ void wave_segscan(double v[NV], int head, int lane) {
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
-        const int hp = __shfl_up(head, off, 64);
         const bool take = lane >= off && !head;
+        if (!__any(take)) break;
+        const int hp = __shfl_up(head, off, 64);
 #pragma unroll
         for (int i = 0; i < NV; ++i) {
             const double vp = __shfl_up(v[i], off, 64);
@@ -112,7 +115,7 @@ __device__ __forceinline__ void wave_segscan(double v[NV], int head, int lane) {
 // Landmark-tile pass. Storage type TS of G and V^-1 (float for the mixed
 // operator, double otherwise); arithmetic, partials and vectors in FP64.
 // Lane t takes the tile's sorted positions 4t .. 4t+3.
-constexpr int kTileThreads = 512;
+constexpr int kTileThreads = 64;
 
 template <typename TS, int OP>
 __global__ __launch_bounds__(kTileThreads) void k_schur_tile(DevProblem P, const double* __restrict__ v1,
@@ -132,14 +135,13 @@ __global__ __launch_bounds__(kTileThreads) void k_schur_tile(DevProblem P, const
     __shared__ int32_t sEndPos[kWaves];
     __shared__ double sRed[kWaves * 3];
     __shared__ double sBig[3];
-    __shared__ int sFlag;
 
     const int tile = xcd_tile(blockIdx.x, P.num_lm_tiles);
     if (tile >= P.num_lm_tiles) return;
     if (P.wstate && !P.wstate[P.lm_tile_win[tile]].active) return;
     double beta = 0.0;
     if (OP == OP_APPLY && pcg_iter >= 0) {
-        if (!pcg_prologue<kTileThreads>(P, pcg_iter, fresh, &beta, sRed, &sFlag)) return;
+        if (!pcg_prologue(P, pcg_iter, fresh, &beta)) return;
     }
     const int tid = threadIdx.x;
     const LmTile lt = P.lm_tiles[tile];
@@ -202,7 +204,7 @@ __global__ __launch_bounds__(kTileThreads) void k_schur_tile(DevProblem P, const
             if constexpr (OP == OP_PRECOND) edge_wvw(G, vi, X, y);
             else edge_wu(G, u, X, y);
 #pragma unroll
-            for (int i = 0; i < NV; ++i) part[(size_t)(e0 + s) * NV + i] = y[i];
+            for (int i = 0; i < NV; ++i) part[(size_t)P.run_slot[e0 + s] * NV + i] = y[i];
         }
         return;
     }
@@ -320,7 +322,7 @@ __global__ __launch_bounds__(kTileThreads) void k_schur_tile(DevProblem P, const
                         single = false;
                     } else {
 #pragma unroll
-                        for (int c = 0; c < NV; ++c) part[(size_t)(e0 + s0 + i - 1) * NV + c] = last[c];
+                        for (int c = 0; c < NV; ++c) part[(size_t)P.run_slot[e0 + s0 + i - 1] * NV + c] = last[c];
                     }
 #pragma unroll
                     for (int c = 0; c < NV; ++c) last[c] = 0.0;
@@ -370,7 +372,7 @@ __global__ __launch_bounds__(kTileThreads) void k_schur_tile(DevProblem P, const
                 sEndPos[wave] = first_end;
             } else {
 #pragma unroll
-                for (int c = 0; c < NV; ++c) part[(size_t)(e0 + first_end) * NV + c] = carry[c] + first[c];
+                for (int c = 0; c < NV; ++c) part[(size_t)P.run_slot[e0 + first_end] * NV + c] = carry[c] + first[c];
             }
         }
         const int last_pos = s0 + nv - 1;
@@ -381,7 +383,7 @@ __global__ __launch_bounds__(kTileThreads) void k_schur_tile(DevProblem P, const
                 sEndPos[wave] = last_pos;
             } else {
 #pragma unroll
-                for (int c = 0; c < NV; ++c) part[(size_t)(e0 + last_pos) * NV + c] = S[c];
+                for (int c = 0; c < NV; ++c) part[(size_t)P.run_slot[e0 + last_pos] * NV + c] = S[c];
             }
         } else if (lane == 63) {  // continues into the next wave
 #pragma unroll
@@ -402,7 +404,216 @@ __global__ __launch_bounds__(kTileThreads) void k_schur_tile(DevProblem P, const
 #pragma unroll
             for (int c = 0; c < NV; ++c) acc[c] += sHead[c2 * NV + c];
 #pragma unroll
-        for (int c = 0; c < NV; ++c) part[(size_t)(e0 + sEndPos[w]) * NV + c] = acc[c] + sEnd[w * NV + c];
+        for (int c = 0; c < NV; ++c) part[(size_t)P.run_slot[e0 + sEndPos[w]] * NV + c] = acc[c] + sEnd[w * NV + c];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The operator S d over landmark tiles with one wave per tile: the PCG
+// products and the true-residual check. Every sum inside a tile is
+// accumulated in LDS by the tile's single wave with ds_add_f64, applied in a
+// fixed order (program order, then lane order), so results do not depend on
+// scheduling.
+//   0. owner lane j < nl: X_j into LDS, V^-1_j into registers, t_j = 0;
+//      lane i: the sorted positions 4i .. 4i+3 (G, pose, local landmark);
+//   1. t_j += W_e^T d_p = G (rho + omega x X) for every position;
+//   2. owner lanes: u_j = V_j^-1 t_j;
+//   3. y_e = W_e u_j = [G u; X x (G u)] summed over each lane's positions
+//      run by run; a run that starts and ends inside one lane writes its
+//      partial at once, the others add into their run's LDS accumulator,
+//      which the lane holding the run's last position writes out.
+// A tile of one landmark with more than kTileRuns edges (each its own run)
+// reduces t over the wave and writes one partial per edge.
+// ---------------------------------------------------------------------------
+constexpr int kApplyPer = kTileEdges / 64;  // sorted positions per lane
+
+template <typename TS>
+__device__ void apply_one_landmark(const DevProblem& P, const double* __restrict__ v1,
+                                   const double* __restrict__ v2, double beta, const TS* __restrict__ Gl,
+                                   const double* __restrict__ Vinv, double* __restrict__ part, int l, int e0,
+                                   int n, int lane) {
+    const double X[3] = {P.lms[3 * (size_t)l], P.lms[3 * (size_t)l + 1], P.lms[3 * (size_t)l + 2]};
+    double t[3] = {0.0, 0.0, 0.0};
+    for (int s = lane; s < n; s += 64) {
+        TS G[6];
+        double d[6], te[3];
+        load6(Gl + 6 * (size_t)(e0 + s), G);
+        load_d(v1, v2, beta, P.s_pose[e0 + s], d);
+        edge_wtd(G, d, X, te);
+        t[0] += te[0];
+        t[1] += te[1];
+        t[2] += te[2];
+    }
+    t[0] = wave_allsum(t[0]);
+    t[1] = wave_allsum(t[1]);
+    t[2] = wave_allsum(t[2]);
+    double vi[6], u[3];
+    load6(Vinv + 6 * (size_t)l, vi);
+    sym3_mul(vi, t, u);
+    for (int s = lane; s < n; s += 64) {
+        TS Gs[6];
+        load6(Gl + 6 * (size_t)(e0 + s), Gs);
+        const double G[6] = {(double)Gs[0], (double)Gs[1], (double)Gs[2], (double)Gs[3], (double)Gs[4], (double)Gs[5]};
+        double y[6];
+        edge_wu(G, u, X, y);
+        double* o = part + 6 * (size_t)P.run_slot[e0 + s];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) o[c] = y[c];
+    }
+}
+
+template <typename TS>
+__global__ __launch_bounds__(64) void k_schur_apply(DevProblem P, const double* __restrict__ v1,
+                                                    const double* __restrict__ v2, const TS* __restrict__ Gl,
+                                                    const double* __restrict__ Vinv, double* __restrict__ part,
+                                                    int pcg_iter, int fresh) {
+    __shared__ double sX[kTileLandmarks * 3];
+    __shared__ double sT[kTileLandmarks * 3];  // t, then u, per landmark
+    __shared__ double sR[kTileRuns * 6];       // sums of runs that cross lanes
+    const int tile = xcd_tile(blockIdx.x, P.num_lm_tiles);
+    if (tile >= P.num_lm_tiles) return;
+    if (P.wstate && !P.wstate[P.lm_tile_win[tile]].active) return;
+    double beta = 0.0;
+    if (pcg_iter >= 0 && !pcg_prologue(P, pcg_iter, fresh, &beta)) return;
+    const int lane = threadIdx.x;
+    const LmTile lt = P.lm_tiles[tile];
+    const int e0 = P.lm_edge_ptr[lt.lm_begin], e1 = P.lm_edge_ptr[lt.lm_end];
+    const int n = e1 - e0, nl = lt.lm_end - lt.lm_begin;
+    if (nl == 1 && n > kTileRuns) {
+        apply_one_landmark<TS>(P, v1, v2, beta, Gl, Vinv, part, lt.lm_begin, e0, n, lane);
+        return;
+    }
+
+    // ---- stage 0 ----
+    double vi[6];
+    if (lane < nl) {
+        const int l = lt.lm_begin + lane;
+        load6(Vinv + 6 * (size_t)l, vi);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            sX[3 * lane + c] = P.lms[3 * (size_t)l + c];
+            sT[3 * lane + c] = 0.0;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) sR[6 * lane + c] = 0.0;  // kTileRuns == 64 accumulators
+    const int s0 = kApplyPer * lane;
+    const int nv = max(0, min(kApplyPer, n - s0));
+    int key[kApplyPer], jl[kApplyPer];
+    TS G[kApplyPer][6];
+#pragma unroll
+    for (int i = 0; i < kApplyPer; ++i) {
+        key[i] = -1 - (s0 + i);  // unique for positions past the end
+        jl[i] = 0;
+        if (i < nv) {
+            key[i] = P.s_pose[e0 + s0 + i];
+            jl[i] = P.s_lm[e0 + s0 + i];
+            load6(Gl + 6 * (size_t)(e0 + s0 + i), G[i]);
+        }
+    }
+    __syncthreads();
+
+    // ---- stage 1: t_j += W_e^T d_p, one d per pose run of this lane ----
+    {
+        int pcur = INT_MIN;
+        double d[6];
+#pragma unroll
+        for (int i = 0; i < kApplyPer; ++i) {
+            if (i < nv) {
+                if (key[i] != pcur) {
+                    load_d(v1, v2, beta, key[i], d);
+                    pcur = key[i];
+                }
+                const int j = jl[i];
+                const double X[3] = {sX[3 * j], sX[3 * j + 1], sX[3 * j + 2]};
+                double t[3];
+                edge_wtd(G[i], d, X, t);
+                atomicAdd(&sT[3 * j], t[0]);
+                atomicAdd(&sT[3 * j + 1], t[1]);
+                atomicAdd(&sT[3 * j + 2], t[2]);
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- stage 2: u_j = V_j^-1 t_j ----
+    if (lane < nl) {
+        const double t[3] = {sT[3 * lane], sT[3 * lane + 1], sT[3 * lane + 2]};
+        double u[3];
+        sym3_mul(vi, t, u);
+        sT[3 * lane] = u[0];
+        sT[3 * lane + 1] = u[1];
+        sT[3 * lane + 2] = u[2];
+    }
+    __syncthreads();
+
+    // ---- stage 3: y_e = W_e u_j summed per camera run ----
+    const int prev_key = __shfl_up(key[kApplyPer - 1], 1, 64);  // last key of the lane before
+    const int next_key = __shfl_down(key[0], 1, 64);             // first key of the lane after
+    bool head[kApplyPer], fin[kApplyPer];
+#pragma unroll
+    for (int i = 0; i < kApplyPer; ++i) {
+        const int pk = i == 0 ? (lane == 0 ? INT_MIN : prev_key) : key[i - 1];
+        const int nk = i + 1 < kApplyPer ? key[i + 1] : (lane == 63 ? INT_MIN : next_key);
+        head[i] = i < nv && key[i] != pk;
+        fin[i] = i < nv && (s0 + i == n - 1 || key[i] != nk);
+    }
+    // run index of each position within the tile: run starts before it
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    int rid[kApplyPer];
+    {
+        int cnt = 0;
+#pragma unroll
+        for (int i = 0; i < kApplyPer; ++i) cnt += __popcll(__ballot(head[i]) & below);
+#pragma unroll
+        for (int i = 0; i < kApplyPer; ++i) {
+            cnt += head[i] ? 1 : 0;
+            rid[i] = cnt - 1;
+        }
+    }
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    bool own = false;   // the current segment's run starts in this lane
+    int carried = -1;   // position whose run started in an earlier lane and ends here
+#pragma unroll
+    for (int i = 0; i < kApplyPer; ++i) {
+        if (i < nv) {
+            if (i == 0 || fin[i - 1]) own = head[i];
+            const int j = jl[i];
+            const double X[3] = {sX[3 * j], sX[3 * j + 1], sX[3 * j + 2]};
+            const double u[3] = {sT[3 * j], sT[3 * j + 1], sT[3 * j + 2]};
+            const double Gd[6] = {(double)G[i][0], (double)G[i][1], (double)G[i][2],
+                                  (double)G[i][3], (double)G[i][4], (double)G[i][5]};
+            double y[6];
+            edge_wu(Gd, u, X, y);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) acc[c] += y[c];
+            if (fin[i]) {
+                if (own) {
+                    double* o = part + 6 * (size_t)P.run_slot[e0 + s0 + i];
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) o[c] = acc[c];
+                } else if (rid[i] < kTileRuns) {
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) atomicAdd(&sR[6 * rid[i] + c], acc[c]);
+                    carried = i;
+                }
+#pragma unroll
+                for (int c = 0; c < 6; ++c) acc[c] = 0.0;
+            }
+        }
+    }
+    // a segment open at the lane's end continues in the next lane
+    if (nv > 0 && !fin[nv - 1] && rid[nv - 1] < kTileRuns) {
+        const int r = rid[nv - 1];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) atomicAdd(&sR[6 * r + c], acc[c]);
+    }
+    __syncthreads();
+    if (carried >= 0) {
+        const int r = rid[carried];
+        double* o = part + 6 * (size_t)P.run_slot[e0 + s0 + carried];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) o[c] = sR[6 * r + c];
     }
 }
 
@@ -416,7 +627,7 @@ __device__ __forceinline__ void pose_run_sum(const DevProblem& P, const double* 
 #pragma unroll
     for (int i = 0; i < NV; ++i) s[i] = 0.0;
     for (int k = P.pose_run_ptr[p] + lane; k < P.pose_run_ptr[p + 1]; k += 64) {
-        const double* q = part + (size_t)P.pose_runs[k] * NV;
+        const double* q = part + (size_t)k * NV;
 #pragma unroll
         for (int i = 0; i < NV; ++i) s[i] += q[i];
     }
@@ -542,10 +753,9 @@ __global__ __launch_bounds__(kSmallBlock) void k_pcg_init(DevProblem P, double r
 // d = z + beta d and q = S d = (U + lambda I) d - sum of the pose's runs;
 // per work-group d.q partials.
 __global__ __launch_bounds__(kBlock) void k_pcg_pose(DevProblem P, const double* __restrict__ part) {
-    __shared__ int sFlag;
     __shared__ double lds[(kBlock / 64) * 2];
     PcgState* st = P.pcg;
-    if (block_flag(&st->stop, &sFlag)) return;
+    if (wave_stop(st)) return;
     const double beta = st->beta, lambda = st->lambda;
     const int lane = threadIdx.x & 63;
     const int p = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
@@ -570,14 +780,11 @@ __global__ __launch_bounds__(kBlock) void k_pcg_pose(DevProblem P, const double*
 // alpha = rho / d.q, x += alpha d, r -= alpha q, z = M^-1 r; per
 // work-group (r.z, r.r) partials. Lanes 6i..6i+5 of a wave hold pose i.
 __global__ __launch_bounds__(kBlock) void k_pcg_step(DevProblem P, int k) {
-    __shared__ int sFlag;
     __shared__ double lds[(kBlock / 64) * 2];
     PcgState* st = P.pcg;
-    if (block_flag(&st->stop, &sFlag)) return;
-    double cv[1] = {0.0};
-    for (int i = threadIdx.x; i < P.num_bpart; i += kBlock) cv[0] += P.bpart[2 * (size_t)i];
-    block_sum_all<kBlock, double, 1>(cv, lds);
-    const double curvature = cv[0];
+    if (wave_stop(st)) return;
+    double curvature, unused;
+    wave_pair_total(P.bpart, P.num_bpart, &curvature, &unused);
     if (!isfinite(curvature) || !(curvature > 0.0)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) set_stop(st, PCG_NONPOSITIVE);
         return;
@@ -610,11 +817,9 @@ __global__ __launch_bounds__(kBlock) void k_pcg_step(DevProblem P, int k) {
     wave_values_to_partial(v, lds, P.cpart + 2 * (size_t)blockIdx.x);
 }
 
-__global__ __launch_bounds__(kBlock) void k_pcg_status(DevProblem P, int k, int fresh) {
-    __shared__ double lds[(kBlock / 64) * 2];
-    __shared__ int sFlag;
+__global__ __launch_bounds__(64) void k_pcg_status(DevProblem P, int k, int fresh) {
     double beta;
-    (void)pcg_prologue<kBlock>(P, k, fresh, &beta, lds, &sFlag);
+    (void)pcg_prologue(P, k, fresh, &beta);
 }
 
 // True residual r_t = b - S x (part64 holds the tile pass run on x); with
@@ -702,11 +907,11 @@ void launch_rhs(const DevProblem& P, hipStream_t s) {
 void launch_schur_apply(const DevProblem& P, bool fp32, const double* v1, const double* v2, int32_t pcg_iter,
                         int32_t fresh, hipStream_t s) {
     if (fp32)
-        hipLaunchKernelGGL((k_schur_tile<float, OP_APPLY>), dim3(xcd_grid(P.num_lm_tiles)), dim3(kTileThreads), 0, s, P,
-                           v1, v2, (const float*)P.Gl32, (const double*)P.VinvM, P.part64, pcg_iter, fresh);
+        hipLaunchKernelGGL(k_schur_apply<float>, dim3(xcd_grid(P.num_lm_tiles)), dim3(64), 0, s, P, v1, v2,
+                           (const float*)P.Gl32, (const double*)P.VinvM, P.part64, pcg_iter, fresh);
     else
-        hipLaunchKernelGGL((k_schur_tile<double, OP_APPLY>), dim3(xcd_grid(P.num_lm_tiles)), dim3(kTileThreads), 0, s, P,
-                           v1, v2, (const double*)P.Gl64, (const double*)P.Vinv64, P.part64, pcg_iter, fresh);
+        hipLaunchKernelGGL(k_schur_apply<double>, dim3(xcd_grid(P.num_lm_tiles)), dim3(64), 0, s, P, v1, v2,
+                           (const double*)P.Gl64, (const double*)P.Vinv64, P.part64, pcg_iter, fresh);
 }
 
 void launch_pcg_init(const DevProblem& P, double rtol, double atol, int32_t max_iter, double lambda,
@@ -724,7 +929,7 @@ void launch_pcg_step(const DevProblem& P, int32_t pcg_iter, hipStream_t s) {
 }
 
 void launch_pcg_status(const DevProblem& P, int32_t pcg_iter, int32_t fresh, hipStream_t s) {
-    hipLaunchKernelGGL(k_pcg_status, dim3(1), dim3(kBlock), 0, s, P, pcg_iter, fresh);
+    hipLaunchKernelGGL(k_pcg_status, dim3(1), dim3(64), 0, s, P, pcg_iter, fresh);
 }
 
 void launch_pcg_true_residual(const DevProblem& P, hipStream_t s) {

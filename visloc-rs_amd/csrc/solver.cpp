@@ -24,7 +24,7 @@ namespace {
 enum BufId {
     B_POSES0, B_LMS0, B_POSES, B_LMS, B_POSES_T, B_LMS_T, B_SENSORS, B_POSE_VAR, B_LM_VAR, B_OBS_UV,
     B_OBS_SENSOR, B_EDGE_LM, B_EDGE_OBS, B_PERM_CL, B_L_POSE, B_LM_STORE, B_LM_EDGE_PTR, B_CAM_TILES,
-    B_CAM_TILE_PTR, B_LM_TILES, B_TILE_PERM, B_BIG_TILES, B_POSE_RUN_PTR, B_POSE_RUNS, B_GL64, B_GL32,
+    B_CAM_TILE_PTR, B_LM_TILES, B_TILE_PERM, B_BIG_TILES, B_POSE_RUN_PTR, B_RUN_SLOT, B_S_LM, B_GL64, B_GL32,
     B_HL64, B_TILE27, B_U, B_G, B_V, B_GL, B_VINV64, B_VINV32, B_VM, B_MINV, B_VB, B_VX, B_VR, B_VZ, B_VD,
     B_VQ, B_PART64, B_PART21, B_BPART, B_CPART, B_PCG, B_DELTA_L, B_TILE_SCALAR,
     B_SCALARS, B_DENSE_BLOCK_PQ, B_DENSE_BLOCK_PTR, B_DENSE_TERMS, B_DENSE_VAR_SLOT, B_DENSE_S,
@@ -379,7 +379,8 @@ int32_t prepare(Context& ctx, Prepared& pr, std::string* msg) {
         upload(pr, B_TILE_PERM, m.tile_perm, s);
         upload(pr, B_BIG_TILES, m.big_tiles, s);
         upload(pr, B_POSE_RUN_PTR, m.pose_run_ptr, s);
-        upload(pr, B_POSE_RUNS, m.pose_runs, s);
+        upload(pr, B_RUN_SLOT, m.run_slot, s);
+        upload(pr, B_S_LM, m.s_lm, s);
         hip_check(hipStreamSynchronize(s), "sync");
         pr.final_slot = -1;
         return VBA_OK;
@@ -435,7 +436,7 @@ DevProblem bind(const Context& ctx, Prepared& pr, const vba_options& o, bool mix
     P.kernel_kind = (int32_t)o.robust_kernel;
     P.kernel_scale = o.robust_scale;
     const size_t NP = m.num_pose_slots, NL = m.num_lm_slots, NE = m.num_edges, NT = m.cam_tiles.size();
-    const size_t NLT = m.lm_tiles.size(), NR = std::max<uint64_t>(NE, 1);  // partials by edge index
+    const size_t NLT = m.lm_tiles.size(), NR = std::max<uint64_t>(m.num_runs, 1);  // partials by run slot
     const size_t pose_blocks = (NP + kBlock - 1) / kBlock;
     P.poses = buf<double>(pr, B_POSES);
     P.lms = buf<double>(pr, B_LMS);
@@ -463,7 +464,8 @@ DevProblem bind(const Context& ctx, Prepared& pr, const vba_options& o, bool mix
     P.big_tiles = buf<int32_t>(pr, B_BIG_TILES);
     P.num_big_tiles = (int32_t)m.big_tiles.size();
     P.pose_run_ptr = buf<int32_t>(pr, B_POSE_RUN_PTR);
-    P.pose_runs = buf<int32_t>(pr, B_POSE_RUNS);
+    P.run_slot = buf<int32_t>(pr, B_RUN_SLOT);
+    P.s_lm = buf<uint8_t>(pr, B_S_LM);
     P.Gl64 = (double*)ensure(pr, B_GL64, 6 * NE * sizeof(double));
     P.hl64 = (double*)ensure(pr, B_HL64, 3 * NE * sizeof(double));
     if (mixed) {
@@ -571,9 +573,9 @@ int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stat
         // tile pass: per edge G (6 reals), its pose (4) and sorted position (2);
         // per landmark X (24), V^-1 (48), edge pointer (4); per run the
         // FP64 partial (48); per pose z and d (96); per tile its bounds (8)
-        const double bytes_apply = NE * (6.0 * real + 4.0 + 2.0) + NL * (24.0 + 48.0 + 4.0) + NR * 48.0 +
+        const double bytes_apply = NE * (6.0 * real + 4.0 + 1.0) + NL * (24.0 + 48.0 + 4.0) + NR * (48.0 + 4.0) +
                                    NP * 96.0 + m.lm_tiles.size() * 8.0;
-        const double bytes_pcg_pose = NR * (4.0 + 48.0) + NP * (8.0 + 96.0 + 168.0 + 96.0);
+        const double bytes_pcg_pose = NR * 48.0 + NP * (8.0 + 96.0 + 168.0 + 96.0);
         const double bytes_pcg_step = NP * (4.0 * 48.0 + 288.0 + 3.0 * 48.0);
         const double bytes_linearize = m.num_obs * 16.0 + NE * (4.0 + 4.0 + 4.0 + 48.0 + 24.0 + (mixed ? 24.0 : 0.0)) +
                                        NL * 24.0 + NT * (16.0 + 27.0 * 8.0) + NP * 56.0;
@@ -861,7 +863,7 @@ void merge_models(const std::vector<const Model*>& ms, Model* out, std::vector<W
     cam_tile_win->clear();
     lm_tile_win->clear();
     *dense_out = DenseLists();
-    int64_t po = 0, lo = 0, eo = 0, oo = 0, to = 0, vo = 0, vlo = 0, so = 0, soff = 0, term_off = 0, ro = 0, rlo = 0;
+    int64_t po = 0, lo = 0, eo = 0, oo = 0, to = 0, vo = 0, vlo = 0, so = 0, soff = 0, term_off = 0, ro = 0;
     for (size_t w = 0; w < ms.size(); ++w) {
         const Model& m = *ms[w];
         WindowLayout L{};
@@ -916,11 +918,11 @@ void merge_models(const std::vector<const Model*>& ms, Model* out, std::vector<W
         // camera runs: tile-local permutations are unchanged; run ends shift
         // with the edges
         M.tile_perm.insert(M.tile_perm.end(), m.tile_perm.begin(), m.tile_perm.end());
+        M.s_lm.insert(M.s_lm.end(), m.s_lm.begin(), m.s_lm.end());
         for (uint32_t s2 = 0; s2 < m.num_pose_slots; ++s2)
-            M.pose_run_ptr.push_back((int32_t)(m.pose_run_ptr[s2] + rlo));
-        for (int32_t r : m.pose_runs) M.pose_runs.push_back((int32_t)(r + eo));
+            M.pose_run_ptr.push_back((int32_t)(m.pose_run_ptr[s2] + ro));
+        for (int32_t r : m.run_slot) M.run_slot.push_back(r < 0 ? -1 : (int32_t)(r + ro));
         ro += (int64_t)m.num_runs;
-        rlo += (int64_t)m.pose_runs.size();
         M.max_lm_tile_edges = std::max(M.max_lm_tile_edges, m.max_lm_tile_edges);
         if (m.any_fixed_pose) M.any_fixed_pose = true;
         const DenseLists& d = dense[w];
@@ -953,7 +955,7 @@ void merge_models(const std::vector<const Model*>& ms, Model* out, std::vector<W
     M.cam_edge_ptr.push_back((int32_t)eo);
     M.cam_tile_ptr.push_back((int32_t)to);
     M.lm_edge_ptr.push_back((int32_t)eo);
-    M.pose_run_ptr.push_back((int32_t)rlo);
+    M.pose_run_ptr.push_back((int32_t)ro);
     M.num_runs = (uint64_t)ro;
     dense_out->block_ptr.push_back((int32_t)term_off);
 }
