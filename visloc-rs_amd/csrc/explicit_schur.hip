@@ -100,20 +100,33 @@ __device__ __forceinline__ double row_product(const ExplicitDev& X, const TS* __
     const int i = lane >> 3, g = lane & 7;
     double acc = 0.0;
     if (lane < 48) {
-        for (int t = X.row_ptr[p] + g; t < X.row_ptr[p + 1]; t += 8) {
-            const int q = X.col[t];
-            TS s[6];
-            load6(S + 36 * (size_t)t + 6 * i, s);
-            double v[6];
-            load6(v1 + 6 * (size_t)q, v);
-            if (v2) {
-                double w[6];
-                load6(v2 + 6 * (size_t)q, w);
+        // blocks g, g + 8, ... in that order, four at a time: their loads are
+        // issued together before the products
+        constexpr int KB = 4;
+        const int end = X.row_ptr[p + 1];
+        for (int t0 = X.row_ptr[p] + g; t0 < end; t0 += 8 * KB) {
+            int q[KB];
 #pragma unroll
-                for (int j = 0; j < 6; ++j) v[j] = v[j] + beta * w[j];
+            for (int u = 0; u < KB; ++u) q[u] = t0 + 8 * u < end ? X.col[t0 + 8 * u] : 0;
+            TS s[KB][6];
+            double v[KB][6];
+#pragma unroll
+            for (int u = 0; u < KB; ++u) {
+                const int t = t0 + 8 * u < end ? t0 + 8 * u : t0;
+                load6(S + 36 * (size_t)t + 6 * i, s[u]);
+                load6(v1 + 6 * (size_t)q[u], v[u]);
+                if (v2) {
+                    double w[6];
+                    load6(v2 + 6 * (size_t)q[u], w);
+#pragma unroll
+                    for (int j = 0; j < 6; ++j) v[u][j] = v[u][j] + beta * w[j];
+                }
             }
 #pragma unroll
-            for (int j = 0; j < 6; ++j) acc += (double)s[j] * v[j];
+            for (int u = 0; u < KB; ++u)
+                if (t0 + 8 * u < end)
+#pragma unroll
+                    for (int j = 0; j < 6; ++j) acc += (double)s[u][j] * v[u][j];
         }
     }
     acc += __shfl_xor(acc, 1, 64);

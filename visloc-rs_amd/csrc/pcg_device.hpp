@@ -62,9 +62,21 @@ __device__ __forceinline__ double wave_allsum(double v) {
 __device__ __forceinline__ void wave_pair_total(const double* __restrict__ a, int n, double* s0, double* s1) {
     const int lane = threadIdx.x & 63;
     double v0 = 0.0, v1 = 0.0;
-    for (int i = lane; i < n; i += 64) {
-        v0 += a[2 * (size_t)i];
-        v1 += a[2 * (size_t)i + 1];
+    // lane k adds pairs k, k + 64, ... in that order, four loads at a time
+    for (int i0 = lane; i0 < n; i0 += 256) {
+        double a0[4], a1[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = i0 + 64 * u;
+            a0[u] = i < n ? a[2 * (size_t)i] : 0.0;
+            a1[u] = i < n ? a[2 * (size_t)i + 1] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (i0 + 64 * u < n) {
+                v0 += a0[u];
+                v1 += a1[u];
+            }
     }
     *s0 = wave_allsum(v0);
     *s1 = wave_allsum(v1);
@@ -79,9 +91,11 @@ __device__ __forceinline__ void wave_pair_total(const double* __restrict__ a, in
 // wave of the launch.
 __device__ bool pcg_prologue(const DevProblem& P, int k, int fresh, double* beta_out) {
     PcgState* st = P.pcg;
-    if (wave_stop(st)) return false;
+    // the flag and the partials are loaded together; the flag decides after
+    const int stop = __hip_atomic_load(&st->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     double rz, rr;
     wave_pair_total(P.cpart, P.num_cpart, &rz, &rr);
+    if (__builtin_amdgcn_readfirstlane(stop)) return false;
     double rn, beta = 0.0;
     int status = PCG_RUNNING;
     if (fresh) {

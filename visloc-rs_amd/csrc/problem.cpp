@@ -488,7 +488,7 @@ int32_t build_model(const vba_problem& pb, Model* mp, uint64_t* error_id, std::s
                     ++e;
                 }
             }
-            m.lm_tiles.push_back({(int32_t)b, (int32_t)e});
+            m.lm_tiles.push_back({(int32_t)b, (int32_t)e, m.lm_edge_ptr[b], m.lm_edge_ptr[e]});
             m.max_lm_tile_edges = std::max(m.max_lm_tile_edges, m.lm_edge_ptr[e] - m.lm_edge_ptr[b]);
             b = e;
         }

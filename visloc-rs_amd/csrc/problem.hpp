@@ -47,6 +47,8 @@ struct CamTile {
 struct LmTile {
     int32_t lm_begin;  // landmark slot range
     int32_t lm_end;
+    int32_t e_begin;   // its l-order edge range (lm_edge_ptr at lm_begin, lm_end)
+    int32_t e_end;
 };
 
 // Landmark tiles: runs of whole landmarks, each the work of one wave in the

@@ -467,38 +467,27 @@ __global__ __launch_bounds__(64) void k_schur_apply(DevProblem P, const double* 
                                                     const double* __restrict__ v2, const TS* __restrict__ Gl,
                                                     const double* __restrict__ Vinv, double* __restrict__ part,
                                                     int pcg_iter, int fresh) {
+    static_assert(kTileRuns == 64 && kTileLandmarks == 64, "one landmark and one run accumulator per lane");
     __shared__ double sX[kTileLandmarks * 3];
     __shared__ double sT[kTileLandmarks * 3];  // t, then u, per landmark
-    __shared__ double sR[kTileRuns * 6];       // sums of runs that cross lanes
+    // V^-1 per landmark (stages 0-2), then the sums of runs that cross lanes
+    // (stage 3)
+    __shared__ double sVR[kTileLandmarks * 6];
+    double* const sV = sVR;
+    double* const sR = sVR;
     const int tile = xcd_tile(blockIdx.x, P.num_lm_tiles);
     if (tile >= P.num_lm_tiles) return;
-    if (P.wstate && !P.wstate[P.lm_tile_win[tile]].active) return;
-    double beta = 0.0;
-    if (pcg_iter >= 0 && !pcg_prologue(P, pcg_iter, fresh, &beta)) return;
-    const int lane = threadIdx.x;
     const LmTile lt = P.lm_tiles[tile];
-    const int e0 = P.lm_edge_ptr[lt.lm_begin], e1 = P.lm_edge_ptr[lt.lm_end];
-    const int n = e1 - e0, nl = lt.lm_end - lt.lm_begin;
-    if (nl == 1 && n > kTileRuns) {
-        apply_one_landmark<TS>(P, v1, v2, beta, Gl, Vinv, part, lt.lm_begin, e0, n, lane);
-        return;
-    }
+    if (P.wstate && !P.wstate[P.lm_tile_win[tile]].active) return;
+    const int lane = threadIdx.x;
+    const int e0 = lt.e_begin;
+    const int n = lt.e_end - lt.e_begin, nl = lt.lm_end - lt.lm_begin;
+    const bool one = nl == 1 && n > kTileRuns;
 
-    // ---- stage 0 ----
-    double vi[6];
-    if (lane < nl) {
-        const int l = lt.lm_begin + lane;
-        load6(Vinv + 6 * (size_t)l, vi);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            sX[3 * lane + c] = P.lms[3 * (size_t)l + c];
-            sT[3 * lane + c] = 0.0;
-        }
-    }
-#pragma unroll
-    for (int c = 0; c < 6; ++c) sR[6 * lane + c] = 0.0;  // kTileRuns == 64 accumulators
+    // ---- stage 0: the tile's data, loaded before the PCG scalars so that
+    // both are in flight together (a stopped launch drops them) ----
     const int s0 = kApplyPer * lane;
-    const int nv = max(0, min(kApplyPer, n - s0));
+    const int nv = one ? 0 : max(0, min(kApplyPer, n - s0));
     int key[kApplyPer], jl[kApplyPer];
     TS G[kApplyPer][6];
 #pragma unroll
@@ -509,6 +498,24 @@ __global__ __launch_bounds__(64) void k_schur_apply(DevProblem P, const double* 
             key[i] = P.s_pose[e0 + s0 + i];
             jl[i] = P.s_lm[e0 + s0 + i];
             load6(Gl + 6 * (size_t)(e0 + s0 + i), G[i]);
+        }
+    }
+    double beta = 0.0;
+    if (pcg_iter >= 0 && !pcg_prologue(P, pcg_iter, fresh, &beta)) return;
+    if (one) {
+        apply_one_landmark<TS>(P, v1, v2, beta, Gl, Vinv, part, lt.lm_begin, e0, n, lane);
+        return;
+    }
+    if (lane < nl) {
+        const int l = lt.lm_begin + lane;
+        double vi[6];
+        load6(Vinv + 6 * (size_t)l, vi);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) sV[6 * lane + c] = vi[c];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            sX[3 * lane + c] = P.lms[3 * (size_t)l + c];
+            sT[3 * lane + c] = 0.0;
         }
     }
     __syncthreads();
@@ -539,12 +546,18 @@ __global__ __launch_bounds__(64) void k_schur_apply(DevProblem P, const double* 
     // ---- stage 2: u_j = V_j^-1 t_j ----
     if (lane < nl) {
         const double t[3] = {sT[3 * lane], sT[3 * lane + 1], sT[3 * lane + 2]};
-        double u[3];
+        double vi[6], u[3];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) vi[c] = sV[6 * lane + c];
         sym3_mul(vi, t, u);
         sT[3 * lane] = u[0];
         sT[3 * lane + 1] = u[1];
         sT[3 * lane + 2] = u[2];
     }
+    // the run accumulators replace V^-1: this wave's reads of it above come
+    // first in its LDS order
+#pragma unroll
+    for (int c = 0; c < 6; ++c) sR[6 * lane + c] = 0.0;
     __syncthreads();
 
     // ---- stage 3: y_e = W_e u_j summed per camera run ----
@@ -626,10 +639,23 @@ __device__ __forceinline__ void pose_run_sum(const DevProblem& P, const double* 
                                              double s[NV]) {
 #pragma unroll
     for (int i = 0; i < NV; ++i) s[i] = 0.0;
-    for (int k = P.pose_run_ptr[p] + lane; k < P.pose_run_ptr[p + 1]; k += 64) {
-        const double* q = part + (size_t)k * NV;
+    // lane k sums slots k, k + 64, ... in that order; KB slots' loads are
+    // issued together before their additions
+    constexpr int KB = NV <= 6 ? 4 : 1;
+    const int end = P.pose_run_ptr[p + 1];
+    for (int k0 = P.pose_run_ptr[p] + lane; k0 < end; k0 += 64 * KB) {
+        double q[KB][NV];
 #pragma unroll
-        for (int i = 0; i < NV; ++i) s[i] += q[i];
+        for (int u = 0; u < KB; ++u) {
+            const int k = k0 + 64 * u;
+#pragma unroll
+            for (int i = 0; i < NV; ++i) q[u][i] = k < end ? part[(size_t)k * NV + i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < KB; ++u)
+            if (k0 + 64 * u < end)
+#pragma unroll
+                for (int i = 0; i < NV; ++i) s[i] += q[u][i];
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1)

@@ -912,7 +912,8 @@ void merge_models(const std::vector<const Model*>& ms, Model* out, std::vector<W
         }
         for (int32_t t : m.big_tiles) M.big_tiles.push_back((int32_t)(t + M.lm_tiles.size()));
         for (const LmTile& t : m.lm_tiles) {
-            M.lm_tiles.push_back({(int32_t)(t.lm_begin + lo), (int32_t)(t.lm_end + lo)});
+            M.lm_tiles.push_back({(int32_t)(t.lm_begin + lo), (int32_t)(t.lm_end + lo), (int32_t)(t.e_begin + eo),
+                                  (int32_t)(t.e_end + eo)});
             lm_tile_win->push_back((int32_t)w);
         }
         // camera runs: tile-local permutations are unchanged; run ends shift
