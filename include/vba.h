@@ -372,6 +372,58 @@ void vba_profile_reset(vba_context* ctx);
 int32_t vba_profile_query(vba_context* ctx, const char* kernel, uint64_t* launches,
                           double* seconds, double* bytes);
 
+/* ---- reduced camera solve ------------------------------------------------
+ * The iterative reduced-camera solve of the COLMAP-port rig solver,
+ * `ReducedSolveAccelerator::solve` (pipelines/slam/src/ba_accel.rs:46-157;
+ * caller colmap_incremental/rig_ba_solver.rs:3072-3101): the same
+ * preconditioned conjugate gradients as its CPU pcg_core (:1406-1490), with
+ * the same preconditioner and stopping rule, in FP64 arithmetic on the FP32
+ * blocks the caller stores. Fields mirror `ReducedCameraSystem`:
+ * - diag: the damped 6x6 diagonal block of each free node, column-major
+ *   (nalgebra's Matrix6 layout);
+ * - edges: edge e couples nodes (edges[2e], edges[2e+1]) = (a, b), a < b,
+ *   sorted; offdiag[36e ..] is its (row b, column a) block, column-major;
+ * - rhs: 6 per node; tol: stop once |r| <= tol |rhs|; max_iterations: the
+ *   iteration cap (the reference passes 500);
+ * - cluster_size > 1 with cluster_inverses: `clusters`, the dense inverses
+ *   of the system restricted to runs of cluster_size consecutive nodes
+ *   (run c starts at node c * cluster_size; the last may be shorter), each
+ *   column-major (6m)^2, concatenated; 0 or 1: block-Jacobi;
+ * - coarse_group > 0 with coarse_nodes and coarse_inverse: `coarse`, the
+ *   additive coarse level over the leading coarse_nodes nodes in aggregates
+ *   of coarse_group, its inverse column-major (6M)^2 with
+ *   M = ceil(coarse_nodes / coarse_group);
+ * - structure_id: identifies `edges` across calls (0: unknown); while it
+ *   stays the same the library reuses the edge index it derived.
+ * The point elimination hook (`ReducedSolveAccelerator::eliminate`) is not
+ * offered: blocks always come in offdiag. */
+typedef struct vba_reduced_system {
+    uint64_t structure_id;
+    uint32_t num_nodes;
+    uint32_t num_edges;
+    const double* diag;
+    const uint32_t* edges;
+    const float* offdiag;
+    const double* rhs;
+    uint32_t cluster_size;
+    uint32_t coarse_group;
+    const double* cluster_inverses;
+    uint32_t coarse_nodes;
+    uint32_t max_iterations;
+    const double* coarse_inverse;
+    double tol;
+} vba_reduced_system;
+
+/* `ReducedSolveAccelerator::solve`: VBA_OK with step[6 * num_nodes] and the
+ * iteration count, or VBA_DECLINED (`None`, the caller solves on the CPU)
+ * when the system is outside what the device solve supports (unsorted edges,
+ * clusters longer than 256 nodes, more than 512 coarse aggregates, non-finite
+ * input) or when the iteration breaks down as pcg_core returns None (a
+ * diagonal block not positive definite, p^T A p not positive and finite).
+ * vba_last_error(ctx) says which. */
+int32_t vba_reduced_solve(vba_context* ctx, const vba_reduced_system* system, double* step,
+                          uint32_t* iterations);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

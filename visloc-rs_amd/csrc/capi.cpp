@@ -11,6 +11,7 @@
 #include "host_common.hpp"
 #include "problem.hpp"
 #include "capi_types.hpp"
+#include "reduced.hpp"
 #include "solver.hpp"
 
 
@@ -384,6 +385,23 @@ void vba_profile_reset(vba_context* ctx) {
         k.launches = 0;
         k.seconds = 0.0;
         k.bytes = 0.0;
+    }
+}
+
+int32_t vba_reduced_solve(vba_context* ctx, const vba_reduced_system* system, double* step, uint32_t* iterations) {
+    if (!ctx || !system || !step || !iterations) {
+        vba::set_thread_error("vba_reduced_solve: null argument");
+        if (ctx) ctx->last_error = "vba_reduced_solve: null argument";
+        return VBA_ERR_INVALID_ARGUMENT;
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->last_error.clear();
+    *iterations = 0;
+    try {
+        return vba::reduced_solve(*ctx, *system, step, iterations, &ctx->last_error);
+    } catch (const std::bad_alloc&) {
+        ctx->last_error = "out of host memory";
+        return VBA_ERR_OUT_OF_MEMORY;
     }
 }
 

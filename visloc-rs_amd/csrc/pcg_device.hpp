@@ -45,9 +45,10 @@ __device__ __forceinline__ void set_stop(PcgState* st, int status) {
 // only be set by work-group 0 for a status that every wave derives itself
 // from the same inputs (pcg_prologue), so a wave that reads it late still
 // takes the same branch as the others.
-__device__ __forceinline__ int wave_stop(PcgState* st) {
-    return __builtin_amdgcn_readfirstlane(__hip_atomic_load(&st->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+__device__ __forceinline__ int wave_flag(int32_t* flag) {
+    return __builtin_amdgcn_readfirstlane(__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
 }
+__device__ __forceinline__ int wave_stop(PcgState* st) { return wave_flag(&st->stop); }
 
 // Sum over a wave with the same result in every lane (xor butterfly:
 // each step adds the same two values in every lane).

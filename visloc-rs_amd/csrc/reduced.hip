@@ -1,0 +1,537 @@
+// reduced.hip — the COLMAP-port rig solver's iterative reduced-camera solve
+// on the device: the `ReducedSolveAccelerator::solve` hook
+// (pipelines/slam/src/ba_accel.rs:46-157) running the CPU solver's
+// preconditioned conjugate gradients (pcg_core, colmap_incremental/
+// rig_ba_solver.rs:1406-1490) with the same preconditioner and stopping
+// rule. The system comes as damped 6x6 diagonal blocks, an edge list
+// (a, b), a < b, whose FP32 blocks B are the (row b, column a) blocks, and
+// the right-hand side; optionally dense inverses over runs of k consecutive
+// nodes (cluster_inverses, :1112-1166) and the inverse of an aggregated
+// coarse matrix (coarse_correction, :1235-1275).
+//
+// Arithmetic is FP64 throughout; the blocks are read as the FP32 values the
+// caller stores, as the CPU solver's default product does (:1069-1072).
+// Every reduction has a fixed order, so a solve is deterministic. One PCG
+// iteration is a fixed sequence of kernels that decide the stop themselves
+// (each work-group derives the same scalars from the same partials); the
+// host reads the state once per ten iterations.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "device_common.hpp"
+#include "pcg_device.hpp"
+#include "reduced.hpp"
+#include "solver.hpp"
+
+namespace vba {
+
+namespace {
+
+enum RsStatus : int32_t { RS_RUNNING = 0, RS_CONVERGED = 1, RS_MAXIT = 2, RS_ZERO = 3, RS_FAILED = -1 };
+
+struct RsState {
+    double b_norm, tol;
+    double rz[2];  // r.z by iteration parity
+    int32_t iter, status, stop, max_iter;
+};
+
+struct RsDev {
+    int32_t n, num_edges, cluster_k, num_clusters;
+    int32_t coarse_group, coarse_nodes, coarse_m, pad;
+    const double* diag;        // 36 per node, column-major
+    const uint32_t* edges;     // 2 per edge
+    const float* off;          // 36 per edge, (row b, column a) block, column-major
+    const double* rhs;         // 6 per node
+    const int32_t* in_ptr;     // node i: edges with b == i are in_edge[in_ptr[i] .. in_ptr[i+1]]
+    const int32_t* in_edge;
+    const int32_t* out_ptr;    // node i: edges with a == i are out_ptr[i] .. out_ptr[i+1]
+    const double* cl_inv;      // cluster inverses, column-major, concatenated
+    const int64_t* cl_off;     // offset of cluster c's inverse (doubles)
+    const double* cinv;        // coarse inverse, column-major (6M)^2
+    double *x, *r, *z, *p, *ap, *minv, *rc, *yc;
+    double* part_a;            // per-work-group partials of p.ap (matvec) and r.z
+    double* part_b;            // per-work-group partials of r.r (update)
+    RsState* st;
+};
+
+constexpr int kRsThreads = 256;
+constexpr int kRsMaxCluster = 256;      // nodes per cluster run
+constexpr int kRsMaxAggregates = 512;   // coarse aggregates
+
+// Block-Jacobi inverses (the CPU computes them for every solve, clusters or
+// not, and gives up on a block that is not positive definite), x = 0,
+// r = b and per-work-group b.b partials.
+__global__ __launch_bounds__(kRsThreads) void k_rs_prepare(RsDev D) {
+    __shared__ double lds[(kRsThreads / 64) * 2];
+    const int t = blockIdx.x * kRsThreads + threadIdx.x;
+    double v[2] = {0.0, 0.0};
+    if (t < 6 * D.n) {
+        const double b = D.rhs[t];
+        D.x[t] = 0.0;
+        D.r[t] = b;
+        v[0] = b * b;
+    }
+    if (t < D.n) {
+        double M[36], inv[36];
+        for (int r = 0; r < 6; ++r)
+            for (int c = 0; c < 6; ++c) M[6 * r + c] = D.diag[36 * (size_t)t + 6 * c + r];
+        if (!cholesky_inverse6(M, inv)) {
+            D.st->status = RS_FAILED;
+            for (int i = 0; i < 36; ++i) inv[i] = 0.0;
+        }
+        for (int i = 0; i < 36; ++i) D.minv[36 * (size_t)t + i] = inv[i];
+    }
+    v[0] = wave_sum(v[0]);
+    wave_values_to_partial(v, lds, D.part_b + 2 * (size_t)blockIdx.x);
+}
+
+__global__ __launch_bounds__(64) void k_rs_start(RsDev D, int num_part, double tol, int max_iter) {
+    double bb, unused;
+    wave_pair_total(D.part_b, num_part, &bb, &unused);
+    if (threadIdx.x == 0) {
+        RsState* st = D.st;
+        st->b_norm = sqrt(bb);
+        st->tol = tol;
+        st->rz[0] = st->rz[1] = 0.0;
+        st->iter = 0;
+        st->max_iter = max_iter;
+        if (st->status == RS_FAILED) st->stop = 1;
+        else if (st->b_norm == 0.0) st->status = RS_ZERO, st->stop = 1;
+        else if (max_iter <= 0) st->status = RS_MAXIT, st->stop = 1;
+        else st->status = RS_RUNNING, st->stop = 0;
+    }
+}
+
+// z = M^-1 r for the nodes' local preconditioner: block-Jacobi (thread per
+// component) or one cluster inverse per work-group. At iteration k >= 0 the
+// work-groups first apply the stop test of pcg_core to the r.r partials.
+// Without a coarse level the work-groups also leave r.z partials.
+__device__ bool rs_check(const RsDev& D, int k, int num_part) {
+    if (wave_flag(&D.st->stop)) return false;
+    if (k < 0) return true;
+    double rr, unused;
+    wave_pair_total(D.part_b, num_part, &rr, &unused);
+    int status = RS_RUNNING;
+    if (sqrt(rr) <= D.st->tol * D.st->b_norm) status = RS_CONVERGED;
+    else if (k + 1 >= D.st->max_iter) status = RS_MAXIT;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        D.st->iter = k + 1;
+        if (status != RS_RUNNING) {
+            D.st->status = status;
+            __hip_atomic_store(&D.st->stop, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    return status == RS_RUNNING;
+}
+
+__global__ __launch_bounds__(kRsThreads) void k_rs_jacobi(RsDev D, int k, int num_part, int dot) {
+    __shared__ double lds[(kRsThreads / 64) * 2];
+    if (!rs_check(D, k, num_part)) return;
+    const int t = blockIdx.x * kRsThreads + threadIdx.x;
+    double v[2] = {0.0, 0.0};
+    if (t < 6 * D.n) {
+        const int i = t / 6, c = t % 6;
+        double z = 0.0;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) z += D.minv[36 * (size_t)i + 6 * c + j] * D.r[6 * (size_t)i + j];
+        D.z[t] = z;
+        v[0] = D.r[t] * z;
+    }
+    if (dot) {
+        v[0] = wave_sum(v[0]);
+        wave_values_to_partial(v, lds, D.part_a + 2 * (size_t)blockIdx.x);
+    }
+}
+
+__global__ __launch_bounds__(kRsThreads) void k_rs_cluster(RsDev D, int k, int num_part, int dot) {
+    __shared__ double sr[6 * kRsMaxCluster];
+    __shared__ double lds[(kRsThreads / 64) * 2];
+    if (!rs_check(D, k, num_part)) return;
+    const int c = blockIdx.x;
+    const int s = c * D.cluster_k;
+    const int m = min(D.cluster_k, D.n - s), rows = 6 * m;
+    for (int t = threadIdx.x; t < rows; t += kRsThreads) sr[t] = D.r[6 * (size_t)s + t];
+    __syncthreads();
+    const double* inv = D.cl_inv + D.cl_off[c];
+    double v[2] = {0.0, 0.0};
+    for (int t = threadIdx.x; t < rows; t += kRsThreads) {
+        double z = 0.0;
+        for (int col = 0; col < rows; ++col) z += inv[(size_t)col * rows + t] * sr[col];
+        D.z[6 * (size_t)s + t] = z;
+        v[0] += sr[t] * z;
+    }
+    if (dot) {
+        v[0] = wave_sum(v[0]);
+        wave_values_to_partial(v, lds, D.part_a + 2 * (size_t)blockIdx.x);
+    }
+}
+
+// Coarse level (CoarseCorrection::add_to, rig_ba_solver.rs:1196-1214):
+// rc = P^T r over the leading nodes, yc = C^-1 rc, z += P yc.
+__global__ __launch_bounds__(kRsThreads) void k_rs_restrict(RsDev D) {
+    if (wave_flag(&D.st->stop)) return;
+    const int t = blockIdx.x * kRsThreads + threadIdx.x;
+    if (t >= 6 * D.coarse_m) return;
+    const int g = t / 6, c = t % 6;
+    const int i1 = min(D.coarse_nodes, (g + 1) * D.coarse_group);
+    double s = 0.0;
+    for (int i = g * D.coarse_group; i < i1; ++i) s += D.r[6 * (size_t)i + c];
+    D.rc[t] = s;
+}
+
+__global__ __launch_bounds__(kRsThreads) void k_rs_coarse(RsDev D) {
+    __shared__ double src[6 * kRsMaxAggregates];
+    if (wave_flag(&D.st->stop)) return;
+    const int rows = 6 * D.coarse_m;
+    for (int t = threadIdx.x; t < rows; t += kRsThreads) src[t] = D.rc[t];
+    __syncthreads();
+    const int t = blockIdx.x * kRsThreads + threadIdx.x;
+    if (t >= rows) return;
+    double y = 0.0;
+    for (int col = 0; col < rows; ++col) y += D.cinv[(size_t)col * rows + t] * src[col];
+    D.yc[t] = y;
+}
+
+__global__ __launch_bounds__(kRsThreads) void k_rs_prolong(RsDev D) {
+    __shared__ double lds[(kRsThreads / 64) * 2];
+    if (wave_flag(&D.st->stop)) return;
+    const int t = blockIdx.x * kRsThreads + threadIdx.x;
+    double v[2] = {0.0, 0.0};
+    if (t < 6 * D.n) {
+        const int i = t / 6, c = t % 6;
+        double z = D.z[t];
+        if (i < D.coarse_nodes) z += D.yc[6 * (i / D.coarse_group) + c];
+        D.z[t] = z;
+        v[0] = D.r[t] * z;
+    }
+    v[0] = wave_sum(v[0]);
+    wave_values_to_partial(v, lds, D.part_a + 2 * (size_t)blockIdx.x);
+}
+
+// p = z + beta p with beta = r.z / (previous r.z); at the start (k < 0)
+// p = z. Records r.z for iteration k + 1.
+__global__ __launch_bounds__(kRsThreads) void k_rs_direction(RsDev D, int k, int num_part) {
+    if (wave_flag(&D.st->stop)) return;
+    double rz, unused;
+    wave_pair_total(D.part_a, num_part, &rz, &unused);
+    const int t = blockIdx.x * kRsThreads + threadIdx.x;
+    const int next = (k + 1) & 1;
+    if (t < 6 * D.n) {
+        if (k < 0) {
+            D.p[t] = D.z[t];
+        } else {
+            const double beta = rz / D.st->rz[k & 1];
+            D.p[t] = D.z[t] + beta * D.p[t];
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) D.st->rz[next] = rz;
+}
+
+// ap = A p, one wave per node: 48 lanes as 6 rows x 8 edge groups, the
+// diagonal block on group 0; in-edges add B p_a, out-edges B^T p_b; a
+// butterfly over the groups. Per-work-group p.ap partials.
+__global__ __launch_bounds__(kRsThreads) void k_rs_matvec(RsDev D) {
+    __shared__ double lds[(kRsThreads / 64) * 2];
+    if (wave_flag(&D.st->stop)) return;
+    const int lane = threadIdx.x & 63;
+    const int node = blockIdx.x * (kRsThreads / 64) + (threadIdx.x >> 6);
+    double v[2] = {0.0, 0.0};
+    if (node < D.n) {
+        const int i = lane >> 3, g = lane & 7;
+        double acc = 0.0;
+        if (lane < 48) {
+            if (g == 0) {
+#pragma unroll
+                for (int c = 0; c < 6; ++c)
+                    acc += D.diag[36 * (size_t)node + 6 * c + i] * D.p[6 * (size_t)node + c];
+            }
+            for (int q = D.in_ptr[node] + g; q < D.in_ptr[node + 1]; q += 8) {
+                const int e = D.in_edge[q];
+                const int a = (int)D.edges[2 * (size_t)e];
+                const float* B = D.off + 36 * (size_t)e;
+#pragma unroll
+                for (int c = 0; c < 6; ++c) acc += (double)B[6 * c + i] * D.p[6 * (size_t)a + c];
+            }
+            for (int e = D.out_ptr[node] + g; e < D.out_ptr[node + 1]; e += 8) {
+                const int b = (int)D.edges[2 * (size_t)e + 1];
+                const float* B = D.off + 36 * (size_t)e + 6 * i;
+#pragma unroll
+                for (int r = 0; r < 6; ++r) acc += (double)B[r] * D.p[6 * (size_t)b + r];
+            }
+        }
+        acc += __shfl_xor(acc, 1, 64);
+        acc += __shfl_xor(acc, 2, 64);
+        acc += __shfl_xor(acc, 4, 64);
+        const int c = lane < 6 ? lane : 0;
+        const double y = __shfl(acc, 8 * c, 64);
+        if (lane < 6) {
+            D.ap[6 * (size_t)node + c] = y;
+            v[0] = D.p[6 * (size_t)node + c] * y;
+        }
+        v[0] = wave_sum(v[0]);
+    }
+    wave_values_to_partial(v, lds, D.part_a + 2 * (size_t)blockIdx.x);
+}
+
+// alpha = r.z / p.ap (the solve gives up when p.ap is not positive and
+// finite), x += alpha p, r -= alpha ap; per-work-group r.r partials.
+__global__ __launch_bounds__(kRsThreads) void k_rs_update(RsDev D, int k, int num_part) {
+    __shared__ double lds[(kRsThreads / 64) * 2];
+    if (wave_flag(&D.st->stop)) return;
+    double pap, unused;
+    wave_pair_total(D.part_a, num_part, &pap, &unused);
+    if (!isfinite(pap) || !(pap > 0.0)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            D.st->status = RS_FAILED;
+            D.st->iter = k + 1;
+            __hip_atomic_store(&D.st->stop, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        return;
+    }
+    const double alpha = D.st->rz[k & 1] / pap;
+    const int t = blockIdx.x * kRsThreads + threadIdx.x;
+    double v[2] = {0.0, 0.0};
+    if (t < 6 * D.n) {
+        D.x[t] = D.x[t] + alpha * D.p[t];
+        const double r = D.r[t] - alpha * D.ap[t];
+        D.r[t] = r;
+        v[0] = r * r;
+    }
+    v[0] = wave_sum(v[0]);
+    wave_values_to_partial(v, lds, D.part_b + 2 * (size_t)blockIdx.x);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+struct ReducedState {
+    uint64_t structure_id = 0;
+    int32_t n = -1, num_edges = -1;
+    std::vector<void*> bufs;
+    std::vector<size_t> sizes;
+};
+
+namespace {
+
+enum RsBuf {
+    R_EDGES, R_IN_PTR, R_IN_EDGE, R_OUT_PTR, R_DIAG, R_OFF, R_RHS, R_CL_INV, R_CL_OFF, R_CINV, R_X, R_R, R_Z, R_P,
+    R_AP, R_MINV, R_RC, R_YC, R_PART_A, R_PART_B, R_STATE, R_COUNT
+};
+
+struct RsFail {
+    int32_t code;
+    std::string msg;
+};
+
+void rs_check_hip(hipError_t e, const char* what) {
+    if (e != hipSuccess)
+        throw RsFail{e == hipErrorOutOfMemory ? VBA_ERR_OUT_OF_MEMORY : VBA_ERR_DEVICE,
+                     std::string(what) + ": " + hipGetErrorString(e)};
+}
+
+void* rs_buf(ReducedState& R, int id, size_t bytes) {
+    if (R.bufs.size() < R_COUNT) {
+        R.bufs.resize(R_COUNT, nullptr);
+        R.sizes.resize(R_COUNT, 0);
+    }
+    if (bytes == 0) bytes = 64;
+    if (R.sizes[id] < bytes) {
+        if (R.bufs[id]) rs_check_hip(hipFree(R.bufs[id]), "hipFree");
+        R.bufs[id] = nullptr;
+        R.sizes[id] = 0;
+        rs_check_hip(hipMalloc(&R.bufs[id], bytes), "hipMalloc");
+        R.sizes[id] = bytes;
+    }
+    return R.bufs[id];
+}
+
+template <typename T>
+T* rs_upload(ReducedState& R, int id, const T* src, size_t count, hipStream_t s) {
+    T* p = (T*)rs_buf(R, id, count * sizeof(T));
+    if (count) rs_check_hip(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+    return p;
+}
+
+int32_t decline(std::string* msg, const char* why) {
+    *msg = why;
+    return VBA_DECLINED;
+}
+
+}  // namespace
+
+void release_reduced(Context& ctx) {
+    if (!ctx.reduced) return;
+    for (void* p : ctx.reduced->bufs)
+        if (p) (void)hipFree(p);
+    delete ctx.reduced;
+    ctx.reduced = nullptr;
+}
+
+int32_t reduced_solve(Context& ctx, const vba_reduced_system& sys, double* step, uint32_t* iterations,
+                      std::string* msg) {
+    const uint32_t n = sys.num_nodes, E = sys.num_edges;
+    // Shapes the solve supports (as the reference GPU backend's checks,
+    // crates/ba-gpu/src/reduced.rs:654-760, decline the rest).
+    if (n == 0 || !sys.diag || !sys.rhs || (E && (!sys.edges || !sys.offdiag)) || E >= (1u << 31) ||
+        n > (1u << 24))
+        return decline(msg, "reduced system: empty, missing arrays or too large");
+    for (uint32_t e = 0; e < E; ++e) {
+        const uint32_t a = sys.edges[2 * (size_t)e], b = sys.edges[2 * (size_t)e + 1];
+        if (!(a < b && b < n)) return decline(msg, "reduced system: an edge is not (a, b) with a < b < n");
+        if (e && !(sys.edges[2 * (size_t)e - 2] < a || (sys.edges[2 * (size_t)e - 2] == a && sys.edges[2 * (size_t)e - 1] < b)))
+            return decline(msg, "reduced system: edges are not sorted");
+    }
+    for (size_t i = 0; i < 6 * (size_t)n; ++i)
+        if (!std::isfinite(sys.rhs[i])) return decline(msg, "reduced system: non-finite right-hand side");
+    if (!(std::isfinite(sys.tol) && sys.tol >= 0.0)) return decline(msg, "reduced system: invalid tolerance");
+    const bool clusters = sys.cluster_size > 1;
+    const int32_t k = clusters ? (int32_t)sys.cluster_size : 1;
+    if (clusters && (!sys.cluster_inverses || sys.cluster_size > (uint32_t)kRsMaxCluster))
+        return decline(msg, "reduced system: cluster runs longer than 256 nodes or missing inverses");
+    const bool coarse = sys.coarse_group > 0;
+    int32_t cm = 0;
+    if (coarse) {
+        if (!sys.coarse_inverse || sys.coarse_nodes > n || sys.coarse_nodes == 0)
+            return decline(msg, "reduced system: invalid coarse level");
+        cm = (int32_t)((sys.coarse_nodes + sys.coarse_group - 1) / sys.coarse_group);
+        if (cm > kRsMaxAggregates) return decline(msg, "reduced system: more than 512 coarse aggregates");
+    }
+
+    if (!ctx.reduced) ctx.reduced = new ReducedState();
+    ReducedState& R = *ctx.reduced;
+    hipStream_t s = ctx.stream;
+    try {
+        RsDev D{};
+        D.n = (int32_t)n;
+        D.num_edges = (int32_t)E;
+        D.cluster_k = k;
+        D.num_clusters = clusters ? (int32_t)((n + k - 1) / k) : 0;
+        D.coarse_group = coarse ? (int32_t)sys.coarse_group : 1;
+        D.coarse_nodes = coarse ? (int32_t)sys.coarse_nodes : 0;
+        D.coarse_m = cm;
+        // edge structure: reused while the caller's structure id (nonzero) stays the same
+        if (sys.structure_id == 0 || sys.structure_id != R.structure_id || R.n != (int32_t)n ||
+            R.num_edges != (int32_t)E) {
+            std::vector<int32_t> in_ptr(n + 1, 0), in_edge(E), out_ptr(n + 1, 0);
+            for (uint32_t e = 0; e < E; ++e) {
+                ++out_ptr[sys.edges[2 * (size_t)e] + 1];
+                ++in_ptr[sys.edges[2 * (size_t)e + 1] + 1];
+            }
+            for (uint32_t i = 0; i < n; ++i) {
+                out_ptr[i + 1] += out_ptr[i];
+                in_ptr[i + 1] += in_ptr[i];
+            }
+            std::vector<int32_t> cur(in_ptr.begin(), in_ptr.end() - 1);
+            for (uint32_t e = 0; e < E; ++e) in_edge[cur[sys.edges[2 * (size_t)e + 1]]++] = (int32_t)e;
+            rs_upload(R, R_EDGES, sys.edges, 2 * (size_t)E, s);
+            rs_upload(R, R_IN_PTR, in_ptr.data(), in_ptr.size(), s);
+            rs_upload(R, R_IN_EDGE, in_edge.data(), in_edge.size(), s);
+            rs_upload(R, R_OUT_PTR, out_ptr.data(), out_ptr.size(), s);
+            rs_check_hip(hipStreamSynchronize(s), "sync");  // the host vectors go out of scope
+            R.structure_id = sys.structure_id;
+            R.n = (int32_t)n;
+            R.num_edges = (int32_t)E;
+        }
+        D.edges = (const uint32_t*)R.bufs[R_EDGES];
+        D.in_ptr = (const int32_t*)R.bufs[R_IN_PTR];
+        D.in_edge = (const int32_t*)R.bufs[R_IN_EDGE];
+        D.out_ptr = (const int32_t*)R.bufs[R_OUT_PTR];
+        D.diag = rs_upload(R, R_DIAG, sys.diag, 36 * (size_t)n, s);
+        D.off = rs_upload(R, R_OFF, sys.offdiag, 36 * (size_t)E, s);
+        D.rhs = rs_upload(R, R_RHS, sys.rhs, 6 * (size_t)n, s);
+        std::vector<int64_t> cl_off;
+        if (clusters) {
+            cl_off.resize(D.num_clusters + 1);
+            int64_t total = 0;
+            for (int32_t c = 0; c < D.num_clusters; ++c) {
+                const int64_t m = std::min<int64_t>(k, (int64_t)n - (int64_t)c * k);
+                cl_off[c] = total;
+                total += 36 * m * m;
+            }
+            cl_off[D.num_clusters] = total;
+            D.cl_inv = rs_upload(R, R_CL_INV, sys.cluster_inverses, (size_t)total, s);
+            D.cl_off = rs_upload(R, R_CL_OFF, cl_off.data(), cl_off.size(), s);
+        }
+        if (coarse) D.cinv = rs_upload(R, R_CINV, sys.coarse_inverse, 36 * (size_t)cm * cm, s);
+        const size_t n6 = 6 * (size_t)n;
+        D.x = (double*)rs_buf(R, R_X, n6 * sizeof(double));
+        D.r = (double*)rs_buf(R, R_R, n6 * sizeof(double));
+        D.z = (double*)rs_buf(R, R_Z, n6 * sizeof(double));
+        D.p = (double*)rs_buf(R, R_P, n6 * sizeof(double));
+        D.ap = (double*)rs_buf(R, R_AP, n6 * sizeof(double));
+        D.minv = (double*)rs_buf(R, R_MINV, 36 * (size_t)n * sizeof(double));
+        D.rc = (double*)rs_buf(R, R_RC, 6 * (size_t)std::max(cm, 1) * sizeof(double));
+        D.yc = (double*)rs_buf(R, R_YC, 6 * (size_t)std::max(cm, 1) * sizeof(double));
+        const int wg_vec = (int)((n6 + kRsThreads - 1) / kRsThreads);          // thread per component
+        const int wg_node = (int)((n + kRsThreads / 64 - 1) / (kRsThreads / 64));  // wave per node
+        const int wg_cl = D.num_clusters;
+        const int wg_coarse = (int)((6 * (size_t)cm + kRsThreads - 1) / kRsThreads);
+        const int num_part = std::max({wg_vec, wg_node, wg_cl, 1});
+        D.part_a = (double*)rs_buf(R, R_PART_A, 2 * (size_t)num_part * sizeof(double));
+        D.part_b = (double*)rs_buf(R, R_PART_B, 2 * (size_t)num_part * sizeof(double));
+        D.st = (RsState*)rs_buf(R, R_STATE, sizeof(RsState));
+        rs_check_hip(hipMemsetAsync(D.st, 0, sizeof(RsState), s), "memset");
+        // partials of the local preconditioner's r.z: one per work-group of
+        // the kernel that finishes z
+        const int zpart = coarse ? wg_vec : (clusters ? wg_cl : wg_vec);
+
+        auto precond = [&](int it, int check_parts) {
+            if (clusters)
+                hipLaunchKernelGGL(k_rs_cluster, dim3(wg_cl), dim3(kRsThreads), 0, s, D, it, check_parts, (int)!coarse);
+            else
+                hipLaunchKernelGGL(k_rs_jacobi, dim3(wg_vec), dim3(kRsThreads), 0, s, D, it, check_parts, (int)!coarse);
+            if (coarse) {
+                hipLaunchKernelGGL(k_rs_restrict, dim3(wg_coarse), dim3(kRsThreads), 0, s, D);
+                hipLaunchKernelGGL(k_rs_coarse, dim3(wg_coarse), dim3(kRsThreads), 0, s, D);
+                hipLaunchKernelGGL(k_rs_prolong, dim3(wg_vec), dim3(kRsThreads), 0, s, D);
+            }
+            hipLaunchKernelGGL(k_rs_direction, dim3(wg_vec), dim3(kRsThreads), 0, s, D, it, zpart);
+        };
+        hipLaunchKernelGGL(k_rs_prepare, dim3(wg_vec), dim3(kRsThreads), 0, s, D);
+        hipLaunchKernelGGL(k_rs_start, dim3(1), dim3(64), 0, s, D, wg_vec, sys.tol, (int)sys.max_iterations);
+        precond(-1, 0);  // z = M^-1 b, p = z, r.z
+
+        if (!ctx.host_pcg)  // pinned staging shared with the LM driver's PCG reads
+            rs_check_hip(hipHostMalloc((void**)&ctx.host_pcg, sizeof(PcgState), hipHostMallocDefault), "hipHostMalloc");
+        static_assert(sizeof(RsState) <= sizeof(PcgState), "state staging");
+        RsState* hs = (RsState*)ctx.host_pcg;
+        const int chunk = 10;
+        int it = 0;
+        while (true) {
+            rs_check_hip(hipMemcpyAsync(hs, D.st, sizeof(RsState), hipMemcpyDeviceToHost, s), "D2H");
+            rs_check_hip(hipStreamSynchronize(s), "sync");
+            if (hs->stop) break;
+            if (it >= (int)sys.max_iterations) throw RsFail{VBA_ERR_DEVICE, "reduced PCG did not stop"};
+            const int end = std::min<int>(it + chunk, (int)sys.max_iterations);
+            for (; it < end; ++it) {
+                hipLaunchKernelGGL(k_rs_matvec, dim3(wg_node), dim3(kRsThreads), 0, s, D);
+                hipLaunchKernelGGL(k_rs_update, dim3(wg_vec), dim3(kRsThreads), 0, s, D, it, wg_node);
+                precond(it, wg_vec);
+            }
+            rs_check_hip(hipGetLastError(), "kernel launch");
+        }
+        if (hs->status == RS_FAILED)
+            return decline(msg, "reduced PCG: a diagonal block is not positive definite or p.Ap is not positive");
+        if (hs->status == RS_ZERO) {
+            std::memset(step, 0, n6 * sizeof(double));
+            *iterations = 0;
+            return VBA_OK;
+        }
+        rs_check_hip(hipMemcpyAsync(step, D.x, n6 * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+        rs_check_hip(hipStreamSynchronize(s), "sync");
+        *iterations = (uint32_t)hs->iter;
+        return VBA_OK;
+    } catch (const RsFail& f) {
+        *msg = f.msg;
+        return f.code;
+    }
+}
+
+}  // namespace vba

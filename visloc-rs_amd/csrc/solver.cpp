@@ -16,6 +16,7 @@
 #include <unordered_map>
 
 #include "host_common.hpp"
+#include "reduced.hpp"
 
 namespace vba {
 
@@ -345,6 +346,7 @@ void release_context(Context& ctx) {
     ctx.pending.clear();
     for (hipEvent_t e : ctx.event_pool) (void)hipEventDestroy(e);
     ctx.event_pool.clear();
+    release_reduced(ctx);
 }
 
 int32_t prepare(Context& ctx, Prepared& pr, std::string* msg) {

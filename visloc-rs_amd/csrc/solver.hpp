@@ -29,6 +29,8 @@ struct KernelProfile {
     double bytes = 0.0;  // algorithmic bytes moved, summed over launches
 };
 
+struct ReducedState;  // reduced.hip
+
 struct Context {
     int device = 0;
     int num_cus = 0;  // compute units of the device
@@ -53,6 +55,8 @@ struct Context {
     };
     std::vector<Pending> pending;
     std::vector<hipEvent_t> event_pool;
+    // device buffers of the reduced-camera solve (reduced.hip), kept between calls
+    ReducedState* reduced = nullptr;
 };
 
 // One problem laid out on the device: structure, initial state and working

@@ -205,6 +205,25 @@ class ServerConfig(C.Structure):
     ]
 
 
+class ReducedSystem(C.Structure):
+    _fields_ = [
+        ("structure_id", C.c_uint64),
+        ("num_nodes", C.c_uint32),
+        ("num_edges", C.c_uint32),
+        ("diag", P64),
+        ("edges", C.POINTER(C.c_uint32)),
+        ("offdiag", C.POINTER(C.c_float)),
+        ("rhs", P64),
+        ("cluster_size", C.c_uint32),
+        ("coarse_group", C.c_uint32),
+        ("cluster_inverses", P64),
+        ("coarse_nodes", C.c_uint32),
+        ("max_iterations", C.c_uint32),
+        ("coarse_inverse", P64),
+        ("tol", C.c_double),
+    ]
+
+
 SIGNATURES = {
     "vba_abi_version": (C.c_uint32, []),
     "vba_create": (C.c_void_p, [C.c_int32]),
@@ -248,6 +267,7 @@ SIGNATURES = {
                                     C.POINTER(C.c_double)]),
     "vba_server_counters": (None, [C.c_void_p, PU64, PU64, PU64, PU64]),
     "vba_server_destroy": (None, [C.c_void_p]),
+    "vba_reduced_solve": (C.c_int32, [C.c_void_p, C.POINTER(ReducedSystem), P64, C.POINTER(C.c_uint32)]),
     "vba_profile_enable": (None, [C.c_void_p, C.c_int32]),
     "vba_profile_reset": (None, [C.c_void_p]),
     "vba_profile_query": (C.c_int32, [C.c_void_p, C.c_char_p, PU64, P64, P64]),

@@ -287,6 +287,66 @@ class Device:
             out.message = self.last_error()
         return out
 
+    def reduced_solve(self, diag, edges, offdiag, rhs, tol: float, max_iterations: int = 500, clusters=None,
+                      coarse=None, structure_id: int = 0):
+        """vba_reduced_solve, `ReducedSolveAccelerator::solve`
+        (pipelines/slam/src/ba_accel.rs:138-145). Blocks are numpy 6x6
+        (row-major here, passed column-major); `offdiag[e]` is edge e's
+        (row b, column a) block, stored as FP32; `clusters` a list of
+        (first node, inverse) over runs of equal length, `coarse` a tuple
+        (group, nodes covered, inverse). Returns (step (n, 6), iterations),
+        or None where the trait returns None (vba_last_error says why)."""
+        diag = np.asarray(diag, dtype=np.float64)
+        n = len(diag)
+        edges = np.ascontiguousarray(np.asarray(edges, dtype=np.uint32).reshape(-1, 2))
+        col = lambda m: np.ascontiguousarray(np.swapaxes(np.asarray(m), -1, -2))
+        d = col(diag).astype(np.float64)
+        off = col(np.asarray(offdiag, dtype=np.float32).reshape(-1, 6, 6)).astype(np.float32)
+        b = np.ascontiguousarray(np.asarray(rhs, dtype=np.float64).reshape(n, 6))
+        sysv = abi.ReducedSystem()
+        sysv.structure_id = structure_id
+        sysv.num_nodes = n
+        sysv.num_edges = len(edges)
+        sysv.diag = d.ctypes.data_as(abi.P64)
+        sysv.edges = edges.ctypes.data_as(C.POINTER(C.c_uint32))
+        sysv.offdiag = off.ctypes.data_as(C.POINTER(C.c_float))
+        sysv.rhs = b.ctypes.data_as(abi.P64)
+        sysv.tol = tol
+        sysv.max_iterations = max_iterations
+        keep = [d, edges, off, b]
+        if clusters is not None:
+            # runs of k nodes from node 0 (the last may be shorter), as the device solve takes them
+            k = clusters[0][1].shape[0] // 6
+            if len(clusters) != -(-n // k):
+                return None
+            for c, (start, inv) in enumerate(clusters):
+                m = min(k, n - c * k)
+                if start != c * k or np.shape(inv) != (6 * m, 6 * m):
+                    return None
+            cl = np.ascontiguousarray(np.concatenate([np.asarray(inv, dtype=np.float64).T.reshape(-1)
+                                                      for _, inv in clusters]))
+            sysv.cluster_size = k
+            sysv.cluster_inverses = cl.ctypes.data_as(abi.P64)
+            keep.append(cl)
+        if coarse is not None:
+            group, nodes, inv = coarse
+            m = -(-nodes // group)
+            if np.shape(inv) != (6 * m, 6 * m):
+                return None
+            ci = np.ascontiguousarray(np.asarray(inv, dtype=np.float64).T)
+            sysv.coarse_group = group
+            sysv.coarse_nodes = nodes
+            sysv.coarse_inverse = ci.ctypes.data_as(abi.P64)
+            keep.append(ci)
+        step = np.zeros((n, 6))
+        its = C.c_uint32(0)
+        st = self.lib.vba_reduced_solve(self.ctx, C.byref(sysv), step.ctypes.data_as(abi.P64), C.byref(its))
+        if st == abi.DECLINED:
+            return None
+        if st != abi.OK:
+            raise RuntimeError(f"vba_reduced_solve: {st} {self.last_error()}")
+        return step, int(its.value)
+
     def optimize_batch(self, problems: List[BundleProblem], options: abi.Options,
                        cap: int = 64) -> "BatchOutput":
         """vba_optimize_batch: independent windows solved together with the
