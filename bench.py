@@ -171,6 +171,118 @@ def bench_windows(args):
     print(json.dumps(result), flush=True)
 
 
+REDUCED_PUBLISHED_MS = 3.4  # GPU ms per PCG iteration at 3.4M camera-pair edges (docs/gpu_bundle_adjustment.md:97-107)
+
+
+def reduced_system(n=5822, reach=620, seed=13):
+    """A reduced camera system of SmallCity's size for the rig solver's
+    iterative solve: n nodes in capture order, each coupled to its +-reach
+    neighbours (3.42M edges at the defaults), FP32 (row b, column a) blocks
+    whose size falls with capture distance, and diagonal blocks that make the
+    matrix diagonally dominant, hence SPD."""
+    rng = np.random.default_rng(seed)
+    a = np.concatenate([np.arange(0, n - d, dtype=np.uint32) for d in range(1, reach + 1)])
+    d = np.concatenate([np.full(n - d, d, dtype=np.uint32) for d in range(1, reach + 1)])
+    order = np.lexsort((d, a))
+    a, d = a[order], d[order]
+    edges = np.stack([a, a + d], axis=1).astype(np.uint32)
+    off = rng.standard_normal((len(edges), 6, 6), dtype=np.float32)
+    off *= (0.05 / (1.0 + d / 32.0)).astype(np.float32)[:, None, None]
+    row = np.abs(off).sum(axis=(1, 2)).astype(np.float64)
+    weight = np.bincount(edges[:, 0], weights=row, minlength=n) + np.bincount(edges[:, 1], weights=row, minlength=n)
+    g = rng.standard_normal((n, 6, 6)) * 0.1
+    diag = np.einsum("nij,nkj->nik", g, g) + np.eye(6)[None] * (weight + 1.0)[:, None, None]
+    rhs = rng.standard_normal(6 * n)
+    return diag, edges, off, rhs
+
+
+def cluster_runs(diag, edges, off, k):
+    """Dense inverses of the matrix restricted to runs of k consecutive nodes
+    (the rig solver's cluster_preconditioner), from the edges inside runs."""
+    n = len(diag)
+    inside = (edges[:, 0] // k) == (edges[:, 1] // k)
+    mats = []
+    for s in range(0, n, k):
+        m = min(k, n - s)
+        M = np.zeros((6 * m, 6 * m))
+        for i in range(m):
+            M[6 * i:6 * i + 6, 6 * i:6 * i + 6] = diag[s + i]
+        mats.append((s, M))
+    for (ea, eb), blk in zip(edges[inside], off[inside].astype(np.float64)):
+        M = mats[ea // k][1]
+        ia, ib = ea % k, eb % k
+        M[6 * ib:6 * ib + 6, 6 * ia:6 * ia + 6] = blk
+        M[6 * ia:6 * ia + 6, 6 * ib:6 * ib + 6] = blk.T
+    return [(s, np.linalg.inv(M)) for s, M in mats]
+
+
+def bench_reduced(args):
+    """The ReducedSolveAccelerator path at the reference's published size:
+    ms per PCG iteration, from two solves with different fixed iteration
+    counts (tolerance 0), so the per-solve upload and setup cancel."""
+    t0 = time.perf_counter()
+    diag, edges, off, rhs = reduced_system()
+    clusters = cluster_runs(diag, edges, off, 32)
+    n, E = len(diag), len(edges)
+    log(f"bench: reduced system {n} nodes, {E} edges, runs of 32 (built in {time.perf_counter() - t0:.1f} s)")
+    dev = vba.Device(0)
+    k1, k2 = 5, 5 + 20 * max(1, args.steps)
+
+    def solve(k):
+        t = time.perf_counter()
+        out = dev.reduced_solve(diag, edges, off, rhs, 0.0, k, clusters=clusters, structure_id=1)
+        if out is None or out[1] != k:
+            raise SystemExit(f"reduced solve failed: {dev.last_error()}")
+        return time.perf_counter() - t
+
+    for _ in range(args.warmup):
+        solve(k1)
+    t1 = min(solve(k1) for _ in range(2))
+    t2 = min(solve(k2) for _ in range(2))
+    per_it = (t2 - t1) / (k2 - k1)
+    dev.close()
+    # algorithmic bytes per iteration: every FP32 block once, its edge ids, the
+    # diagonal blocks, the cluster inverses and the PCG vectors
+    cl_bytes = sum(inv.size for _, inv in clusters) * 8.0
+    algo = E * (144.0 + 8.0) + n * 288.0 + cl_bytes + n * 6 * 8.0 * 10
+    result = {
+        "metric": "reduced-camera PCG ms per iteration at 3.4M camera-pair edges",
+        "value": 1000.0 * per_it,
+        "unit": "ms/iteration",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000.0 * t2,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": 1000.0 * per_it / REDUCED_PUBLISHED_MS,
+        "dtype": "f32 blocks, f64 arithmetic",
+        "data": "synthetic",
+        "config": {"workload": f"reduced camera system of {n} nodes (SmallCity's frame count), {E} camera-pair "
+                               "edges (+-620 capture-order neighbours), FP32 blocks, runs of 32 nodes, "
+                               "fixed iteration counts", "nodes": n, "edges": E},
+        "baseline_note": "vs_baseline divides by the reference's published 3.4 ms per PCG iteration at 3.4M edges "
+                         "on an NVIDIA L4 (docs/gpu_bundle_adjustment.md:97-107)",
+        "roofline": {"kernel": "whole PCG iteration", "bound": "hbm", "achieved": round(algo / per_it / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(algo / per_it / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": None, "algorithmic_bytes_per_iteration": algo},
+    }
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import reduced_pcg
+        offd = off.astype(np.float64)
+        ts = []
+        for k in (1, 3):
+            t = time.perf_counter()
+            reduced_pcg.pcg(diag, edges, offd, rhs, 0.0, k, clusters=clusters)
+            ts.append(time.perf_counter() - t)
+        cpu_it = (ts[1] - ts[0]) / 2
+        result["cpu_baseline"] = {"value": 1000.0 * cpu_it, "unit": "ms/iteration", "cores": 1, "kind": "port",
+                                  "sample": "the oracle's numpy PCG (oracle/reduced_pcg.py) on the same system, "
+                                            f"3 against 1 iterations; {cpu_model()}"}
+    print(json.dumps(result), flush=True)
+
+
 def serve_pools(seed=11, n_pose=48, n_win=32, n_sub=6):
     """Request pools of the served mixed stream: pose-only refinements (1
     camera, 200-2000 fixed points, 1 px noise, 5% outliers), local windows
@@ -312,7 +424,7 @@ def main():
     ap.add_argument("--pcg-policy", choices=["accept", "reject"], default="accept",
                     help="at the PCG iteration cap: use the iterate (GPU backend) or reject the step (CPU "
                          "matrix-free entry)")
-    ap.add_argument("--workload", choices=["loop", "windows", "serve"], default="loop",
+    ap.add_argument("--workload", choices=["loop", "windows", "serve", "reduced"], default="loop",
                     help="loop: the 3.5M-observation single problem (default); windows: a batch of local windows; "
                          "serve: the request loop under a Poisson stream")
     ap.add_argument("--rate", type=float, default=2000.0, help="requests per second (--workload serve)")
@@ -333,6 +445,9 @@ def main():
         return
     if args.workload == "serve":
         bench_serve(args)
+        return
+    if args.workload == "reduced":
+        bench_reduced(args)
         return
 
     t0 = time.perf_counter()

@@ -48,6 +48,8 @@ struct RsDev {
     const double* rhs;         // 6 per node
     const int32_t* in_ptr;     // node i: edges with b == i are in_edge[in_ptr[i] .. in_ptr[i+1]]
     const int32_t* in_edge;
+    const int32_t* in_from;    // the other node (a) of each in_edge entry
+    float* off_in;             // the blocks in in_edge order, row-major (B's rows contiguous)
     const int32_t* out_ptr;    // node i: edges with a == i are out_ptr[i] .. out_ptr[i+1]
     const double* cl_inv;      // cluster inverses, column-major, concatenated
     const int64_t* cl_off;     // offset of cluster c's inverse (doubles)
@@ -233,12 +235,16 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_direction(RsDev D, int k, int
 
 // ap = A p, one wave per node: 48 lanes as 6 rows x 8 edge groups, the
 // diagonal block on group 0; in-edges add B p_a, out-edges B^T p_b; a
-// butterfly over the groups. Per-work-group p.ap partials.
+// butterfly over the groups. A group takes its edges four at a time, their
+// loads issued together. Nodes are spread over the XCDs in contiguous
+// ranges. Per-work-group p.ap partials.
 __global__ __launch_bounds__(kRsThreads) void k_rs_matvec(RsDev D) {
     __shared__ double lds[(kRsThreads / 64) * 2];
     if (wave_flag(&D.st->stop)) return;
+    constexpr int KB = 4;
     const int lane = threadIdx.x & 63;
-    const int node = blockIdx.x * (kRsThreads / 64) + (threadIdx.x >> 6);
+    const int wg = xcd_tile(blockIdx.x, (D.n + kRsThreads / 64 - 1) / (kRsThreads / 64));
+    const int node = wg * (kRsThreads / 64) + (threadIdx.x >> 6);
     double v[2] = {0.0, 0.0};
     if (node < D.n) {
         const int i = lane >> 3, g = lane & 7;
@@ -249,18 +255,43 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_matvec(RsDev D) {
                 for (int c = 0; c < 6; ++c)
                     acc += D.diag[36 * (size_t)node + 6 * c + i] * D.p[6 * (size_t)node + c];
             }
-            for (int q = D.in_ptr[node] + g; q < D.in_ptr[node + 1]; q += 8) {
-                const int e = D.in_edge[q];
-                const int a = (int)D.edges[2 * (size_t)e];
-                const float* B = D.off + 36 * (size_t)e;
+            const int in_end = D.in_ptr[node + 1];
+            for (int q0 = D.in_ptr[node] + g; q0 < in_end; q0 += 8 * KB) {
+                int a[KB];
 #pragma unroll
-                for (int c = 0; c < 6; ++c) acc += (double)B[6 * c + i] * D.p[6 * (size_t)a + c];
+                for (int u = 0; u < KB; ++u) a[u] = D.in_from[q0 + 8 * u < in_end ? q0 + 8 * u : q0];
+                float B[KB][6];
+                double x[KB][6];
+#pragma unroll
+                for (int u = 0; u < KB; ++u) {
+                    const int q = q0 + 8 * u < in_end ? q0 + 8 * u : q0;
+                    load6(D.off_in + 36 * (size_t)q + 6 * i, B[u]);
+                    load6(D.p + 6 * (size_t)a[u], x[u]);
+                }
+#pragma unroll
+                for (int u = 0; u < KB; ++u)
+                    if (q0 + 8 * u < in_end)
+#pragma unroll
+                        for (int c = 0; c < 6; ++c) acc += (double)B[u][c] * x[u][c];
             }
-            for (int e = D.out_ptr[node] + g; e < D.out_ptr[node + 1]; e += 8) {
-                const int b = (int)D.edges[2 * (size_t)e + 1];
-                const float* B = D.off + 36 * (size_t)e + 6 * i;
+            const int out_end = D.out_ptr[node + 1];
+            for (int e0 = D.out_ptr[node] + g; e0 < out_end; e0 += 8 * KB) {
+                int b[KB];
 #pragma unroll
-                for (int r = 0; r < 6; ++r) acc += (double)B[r] * D.p[6 * (size_t)b + r];
+                for (int u = 0; u < KB; ++u) b[u] = (int)D.edges[2 * (size_t)(e0 + 8 * u < out_end ? e0 + 8 * u : e0) + 1];
+                float B[KB][6];
+                double x[KB][6];
+#pragma unroll
+                for (int u = 0; u < KB; ++u) {
+                    const int e = e0 + 8 * u < out_end ? e0 + 8 * u : e0;
+                    load6(D.off + 36 * (size_t)e + 6 * i, B[u]);
+                    load6(D.p + 6 * (size_t)b[u], x[u]);
+                }
+#pragma unroll
+                for (int u = 0; u < KB; ++u)
+                    if (e0 + 8 * u < out_end)
+#pragma unroll
+                        for (int r = 0; r < 6; ++r) acc += (double)B[u][r] * x[u][r];
             }
         }
         acc += __shfl_xor(acc, 1, 64);
@@ -275,6 +306,16 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_matvec(RsDev D) {
         v[0] = wave_sum(v[0]);
     }
     wave_values_to_partial(v, lds, D.part_a + 2 * (size_t)blockIdx.x);
+}
+
+// The blocks in in_edge order, row-major, so that a node reads the blocks of
+// its in-edges as one contiguous stream (its out-edges' blocks already are).
+__global__ __launch_bounds__(kRsThreads) void k_rs_gather_in(RsDev D) {
+    const size_t t = (size_t)blockIdx.x * kRsThreads + threadIdx.x;
+    if (t >= 36 * (size_t)D.num_edges) return;
+    const size_t q = t / 36;
+    const int k = (int)(t % 36), i = k / 6, c = k % 6;
+    D.off_in[t] = D.off[36 * (size_t)D.in_edge[q] + 6 * c + i];
 }
 
 // alpha = r.z / p.ap (the solve gives up when p.ap is not positive and
@@ -321,7 +362,7 @@ struct ReducedState {
 namespace {
 
 enum RsBuf {
-    R_EDGES, R_IN_PTR, R_IN_EDGE, R_OUT_PTR, R_DIAG, R_OFF, R_RHS, R_CL_INV, R_CL_OFF, R_CINV, R_X, R_R, R_Z, R_P,
+    R_EDGES, R_IN_PTR, R_IN_EDGE, R_IN_FROM, R_OUT_PTR, R_DIAG, R_OFF, R_OFF_IN, R_RHS, R_CL_INV, R_CL_OFF, R_CINV, R_X, R_R, R_Z, R_P,
     R_AP, R_MINV, R_RC, R_YC, R_PART_A, R_PART_B, R_STATE, R_COUNT
 };
 
@@ -428,11 +469,16 @@ int32_t reduced_solve(Context& ctx, const vba_reduced_system& sys, double* step,
                 out_ptr[i + 1] += out_ptr[i];
                 in_ptr[i + 1] += in_ptr[i];
             }
-            std::vector<int32_t> cur(in_ptr.begin(), in_ptr.end() - 1);
-            for (uint32_t e = 0; e < E; ++e) in_edge[cur[sys.edges[2 * (size_t)e + 1]]++] = (int32_t)e;
+            std::vector<int32_t> cur(in_ptr.begin(), in_ptr.end() - 1), in_from(E);
+            for (uint32_t e = 0; e < E; ++e) {
+                const int32_t q = cur[sys.edges[2 * (size_t)e + 1]]++;
+                in_edge[q] = (int32_t)e;
+                in_from[q] = (int32_t)sys.edges[2 * (size_t)e];
+            }
             rs_upload(R, R_EDGES, sys.edges, 2 * (size_t)E, s);
             rs_upload(R, R_IN_PTR, in_ptr.data(), in_ptr.size(), s);
             rs_upload(R, R_IN_EDGE, in_edge.data(), in_edge.size(), s);
+            rs_upload(R, R_IN_FROM, in_from.data(), in_from.size(), s);
             rs_upload(R, R_OUT_PTR, out_ptr.data(), out_ptr.size(), s);
             rs_check_hip(hipStreamSynchronize(s), "sync");  // the host vectors go out of scope
             R.structure_id = sys.structure_id;
@@ -442,9 +488,11 @@ int32_t reduced_solve(Context& ctx, const vba_reduced_system& sys, double* step,
         D.edges = (const uint32_t*)R.bufs[R_EDGES];
         D.in_ptr = (const int32_t*)R.bufs[R_IN_PTR];
         D.in_edge = (const int32_t*)R.bufs[R_IN_EDGE];
+        D.in_from = (const int32_t*)R.bufs[R_IN_FROM];
         D.out_ptr = (const int32_t*)R.bufs[R_OUT_PTR];
         D.diag = rs_upload(R, R_DIAG, sys.diag, 36 * (size_t)n, s);
         D.off = rs_upload(R, R_OFF, sys.offdiag, 36 * (size_t)E, s);
+        D.off_in = (float*)rs_buf(R, R_OFF_IN, 36 * (size_t)E * sizeof(float));
         D.rhs = rs_upload(R, R_RHS, sys.rhs, 6 * (size_t)n, s);
         std::vector<int64_t> cl_off;
         if (clusters) {
@@ -473,7 +521,7 @@ int32_t reduced_solve(Context& ctx, const vba_reduced_system& sys, double* step,
         const int wg_node = (int)((n + kRsThreads / 64 - 1) / (kRsThreads / 64));  // wave per node
         const int wg_cl = D.num_clusters;
         const int wg_coarse = (int)((6 * (size_t)cm + kRsThreads - 1) / kRsThreads);
-        const int num_part = std::max({wg_vec, wg_node, wg_cl, 1});
+        const int num_part = std::max({wg_vec, xcd_grid(wg_node), wg_cl, 1});
         D.part_a = (double*)rs_buf(R, R_PART_A, 2 * (size_t)num_part * sizeof(double));
         D.part_b = (double*)rs_buf(R, R_PART_B, 2 * (size_t)num_part * sizeof(double));
         D.st = (RsState*)rs_buf(R, R_STATE, sizeof(RsState));
@@ -494,6 +542,9 @@ int32_t reduced_solve(Context& ctx, const vba_reduced_system& sys, double* step,
             }
             hipLaunchKernelGGL(k_rs_direction, dim3(wg_vec), dim3(kRsThreads), 0, s, D, it, zpart);
         };
+        if (E)
+            hipLaunchKernelGGL(k_rs_gather_in, dim3((unsigned)((36 * (size_t)E + kRsThreads - 1) / kRsThreads)),
+                               dim3(kRsThreads), 0, s, D);
         hipLaunchKernelGGL(k_rs_prepare, dim3(wg_vec), dim3(kRsThreads), 0, s, D);
         hipLaunchKernelGGL(k_rs_start, dim3(1), dim3(64), 0, s, D, wg_vec, sys.tol, (int)sys.max_iterations);
         precond(-1, 0);  // z = M^-1 b, p = z, r.z
@@ -511,8 +562,8 @@ int32_t reduced_solve(Context& ctx, const vba_reduced_system& sys, double* step,
             if (it >= (int)sys.max_iterations) throw RsFail{VBA_ERR_DEVICE, "reduced PCG did not stop"};
             const int end = std::min<int>(it + chunk, (int)sys.max_iterations);
             for (; it < end; ++it) {
-                hipLaunchKernelGGL(k_rs_matvec, dim3(wg_node), dim3(kRsThreads), 0, s, D);
-                hipLaunchKernelGGL(k_rs_update, dim3(wg_vec), dim3(kRsThreads), 0, s, D, it, wg_node);
+                hipLaunchKernelGGL(k_rs_matvec, dim3(xcd_grid(wg_node)), dim3(kRsThreads), 0, s, D);
+                hipLaunchKernelGGL(k_rs_update, dim3(wg_vec), dim3(kRsThreads), 0, s, D, it, xcd_grid(wg_node));
                 precond(it, wg_vec);
             }
             rs_check_hip(hipGetLastError(), "kernel launch");

@@ -301,7 +301,7 @@ class Device:
         edges = np.ascontiguousarray(np.asarray(edges, dtype=np.uint32).reshape(-1, 2))
         col = lambda m: np.ascontiguousarray(np.swapaxes(np.asarray(m), -1, -2))
         d = col(diag).astype(np.float64)
-        off = col(np.asarray(offdiag, dtype=np.float32).reshape(-1, 6, 6)).astype(np.float32)
+        off = col(np.asarray(offdiag, dtype=np.float32).reshape(-1, 6, 6)).astype(np.float32, copy=False)
         b = np.ascontiguousarray(np.asarray(rhs, dtype=np.float64).reshape(n, 6))
         sysv = abi.ReducedSystem()
         sysv.structure_id = structure_id
