@@ -195,8 +195,14 @@ typedef struct vba_options {
     double pcg_relative_tolerance;
     double pcg_absolute_tolerance;
     uint32_t refine_intrinsics;      /* nonzero: declined (joint intrinsics) */
-    uint32_t reserved;
+    uint32_t flags;                  /* enum vba_flags, 0 by default */
 } vba_options;
+
+/* vba_options.flags. A prepared batch of windows (vba_solve_prepared_batch)
+ * records its LM iterations as one HIP graph on its first solve and replays
+ * it on later solves with the same options; VBA_FLAG_NO_GRAPH launches the
+ * kernels one by one instead, with the same results. */
+enum vba_flags { VBA_FLAG_NO_GRAPH = 1 };
 
 /* `BaIterationStats` (mod.rs:754-763) plus the matrix-free diagnostics of
  * `MatrixFreeBaIterationStats` (mod.rs:810-818). */

@@ -111,3 +111,36 @@ def test_prepared_batch_resolves_from_initial_state(device):
                 assert np.array_equal(poses, b.poses) and np.array_equal(lms, b.landmarks)
     finally:
         pb.free()
+
+
+def test_graph_replays_match_eager(device):
+    """A prepared batch records its LM iterations as a HIP graph on the first
+    solve and replays it; every replay, with other device work between them,
+    equals the kernel-by-kernel solve bitwise, and new options re-record."""
+    wins = make_windows(4, seed0=700)
+    o = window_options()
+    eager_o = window_options()
+    eager_o.flags = abi.FLAG_NO_GRAPH
+    pb = device.prepare_batch([w.problem for w in wins], o)
+    try:
+        ref = pb.solve(eager_o)
+        assert ref.status == abi.OK, ref.message
+        ref_state = [pb.state(i) for i in range(len(wins))]
+        other = make_windows(1, seed0=900)[0]
+        for _ in range(3):
+            out = pb.solve(o)
+            assert out.status == abi.OK, out.message
+            for i, (a, b) in enumerate(zip(out.windows, ref.windows)):
+                assert a.result.final_cost == b.result.final_cost
+                assert a.accepted == b.accepted
+                poses, lms = pb.state(i)
+                assert np.array_equal(poses, ref_state[i][0]) and np.array_equal(lms, ref_state[i][1])
+            device.optimize(other.problem, o)  # eager work between replays
+        o.max_iterations = 4
+        eager_o.max_iterations = 4
+        short_ref = pb.solve(eager_o)
+        short = pb.solve(o)
+        for a, b in zip(short.windows, short_ref.windows):
+            assert a.result.final_cost == b.result.final_cost and a.accepted == b.accepted
+    finally:
+        pb.free()

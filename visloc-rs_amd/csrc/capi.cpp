@@ -347,7 +347,7 @@ int32_t vba_solve_prepared_batch(vba_context* ctx, vba_prepared_batch* batch, co
         return VBA_ERR_INVALID_CONFIGURATION;
     }
     try {
-        return vba::run_batch(*ctx, *batch, *options, stats, stats_capacity, results, statuses);
+        return vba::run_batch(*ctx, *batch, *options, stats, stats_capacity, results, statuses, true);
     } catch (const std::bad_alloc&) {
         ctx->last_error = "out of host memory";
         return VBA_ERR_OUT_OF_MEMORY;

@@ -60,6 +60,7 @@ void* ensure(Prepared& pr, int id, size_t bytes) {
         b.bytes = 0;
         hip_check(hipMalloc(&b.ptr, bytes), "hipMalloc");
         b.bytes = bytes;
+        ++pr.alloc_gen;
     }
     return b.ptr;
 }
@@ -330,6 +331,13 @@ void release(Prepared& pr) {
     for (DevBuffer& b : pr.bufs)
         if (b.ptr) (void)hipFree(b.ptr);
     pr.bufs.clear();
+}
+
+void release(PreparedBatch& pb) {
+    if (pb.graph) (void)hipGraphExecDestroy(pb.graph);
+    pb.graph = nullptr;
+    pb.graph_key.clear();
+    release(static_cast<Prepared&>(pb));
 }
 
 void release_context(Context& ctx) {
@@ -1044,7 +1052,7 @@ int32_t prepare_batch(Context& ctx, std::vector<Model>& models, const std::vecto
 }
 
 int32_t run_batch(Context& ctx, PreparedBatch& pb, const vba_options& o, vba_iteration_stats* stats, uint32_t cap,
-                  vba_result* results, int32_t* statuses) {
+                  vba_result* results, int32_t* statuses, bool use_graph) {
     hipStream_t s = ctx.stream;
     const int32_t W = (int32_t)pb.index.size();
     if (W == 0) return VBA_OK;
@@ -1083,28 +1091,58 @@ int32_t run_batch(Context& ctx, PreparedBatch& pb, const vba_options& o, vba_ite
         D.terms = buf<int32_t>(pb, B_DENSE_TERMS);
         D.var_slot = buf<int32_t>(pb, B_DENSE_VAR_SLOT);
 
-        hip_check(hipEventRecord(ctx.ev_start, s), "event");
         // ---- initial cost and state, then max_iterations LM steps with no
-        // host synchronisation: every window decides on the device ----
-        launch_cost(P, false, s);
-        launch_batch_scalars(P, B, s);
-        launch_batch_init(P, B, o, s);
-        for (uint32_t it = 0; it < o.max_iterations; ++it) {
-            launch_linearize(P, false, s);
-            launch_camera_finalize(P, s);
-            launch_landmark_reduce(P, s);
-            launch_landmark_prepare(P, 0.0, s);
-            launch_rhs(P, s);
-            launch_batch_dense_zero(P, B, s);
-            launch_batch_dense_assemble(P, D, B, s);
-            launch_batch_dense_solve(P, D, B, s);
-            launch_backsub(P, s);
-            launch_pose_update(P, s);
-            launch_cost(P, true, s);
+        // host synchronisation: every window decides on the device. Kernel
+        // launches only (no memsets or copies), so the sequence can be
+        // recorded as a graph ----
+        auto launch_all = [&]() {
+            launch_cost(P, false, s);
             launch_batch_scalars(P, B, s);
-            launch_batch_control(P, B, o, (int32_t)it, s);
-            launch_batch_commit(P, B, s);
+            launch_batch_init(P, B, o, s);
+            for (uint32_t it = 0; it < o.max_iterations; ++it) {
+                launch_linearize(P, false, s);
+                launch_camera_finalize(P, s);
+                launch_landmark_reduce(P, s);
+                launch_landmark_prepare(P, 0.0, s);
+                launch_rhs(P, s);
+                launch_batch_dense_zero(P, B, s);
+                launch_batch_dense_assemble(P, D, B, s);
+                launch_batch_dense_solve(P, D, B, s);
+                launch_backsub(P, s);
+                launch_pose_update(P, s);
+                launch_cost(P, true, s);
+                launch_batch_scalars(P, B, s);
+                launch_batch_control(P, B, o, (int32_t)it, s);
+                launch_batch_commit(P, B, s);
+            }
+        };
+        const bool graph = use_graph && !(o.flags & VBA_FLAG_NO_GRAPH);
+        if (graph) {
+            // what the recorded kernels' arguments depend on: the options, the
+            // stats capacity and the device buffers (their generation)
+            std::vector<uint8_t> key(sizeof(vba_options) + sizeof(uint32_t) + sizeof(uint64_t));
+            std::memcpy(key.data(), &o, sizeof(vba_options));
+            const uint32_t capk = (uint32_t)B.stats_capacity;
+            std::memcpy(key.data() + sizeof(vba_options), &capk, sizeof(capk));
+            std::memcpy(key.data() + sizeof(vba_options) + sizeof(capk), &pb.alloc_gen, sizeof(uint64_t));
+            if (!pb.graph || pb.graph_key != key) {
+                if (pb.graph) (void)hipGraphExecDestroy(pb.graph);
+                pb.graph = nullptr;
+                hipGraph_t g = nullptr;
+                hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+                launch_all();
+                const hipError_t launched = hipGetLastError();
+                hip_check(hipStreamEndCapture(s, &g), "hipStreamEndCapture");
+                hip_check(launched, "kernel launch (capture)");
+                const hipError_t inst = hipGraphInstantiate(&pb.graph, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                hip_check(inst, "hipGraphInstantiate");
+                pb.graph_key = key;
+            }
         }
+        hip_check(hipEventRecord(ctx.ev_start, s), "event");
+        if (graph) hip_check(hipGraphLaunch(pb.graph, s), "hipGraphLaunch");
+        else launch_all();
         hip_check(hipGetLastError(), "kernel launch");
         hip_check(hipEventRecord(ctx.ev_stop, s), "event");
         std::vector<WindowState> fin(W);

@@ -65,6 +65,7 @@ struct Context {
 struct Prepared {
     Model m;
     std::vector<DevBuffer> bufs;
+    uint64_t alloc_gen = 0;  // counts (re)allocations of bufs: device pointers change with it
     bool dense_built = false;
     std::vector<int32_t> dense_block_pq, dense_block_ptr, dense_terms, dense_var_slot;
     // explicit reduced camera matrix (over the dense term lists): full block
@@ -102,6 +103,9 @@ struct PreparedBatch : Prepared {
     std::vector<std::vector<uint32_t>> pose_slot_of_input, lm_slot_of_input;  // per batch window
     std::vector<uint64_t> num_obs;                                           // per batch window
     bool solved = false;
+    // the LM iterations recorded as a HIP graph, with what it was recorded for
+    hipGraphExec_t graph = nullptr;
+    std::vector<uint8_t> graph_key;
 };
 
 // Lay out the valid windows (valid[i] = 1) of `models` and upload them.
@@ -111,8 +115,12 @@ int32_t prepare_batch(Context& ctx, std::vector<Model>& models, const std::vecto
                       PreparedBatch* batch, int32_t* statuses, std::string* message);
 // Solve every window of the batch from its prepared initial state; results,
 // stats (stats_capacity per caller index) and statuses by caller index.
+// With `graph` (and o.flags without VBA_FLAG_NO_GRAPH) the LM iterations are
+// recorded as a HIP graph on the first solve and replayed while the options,
+// the stats capacity and the device buffers stay the same.
 int32_t run_batch(Context& ctx, PreparedBatch& batch, const vba_options& o, vba_iteration_stats* stats,
-                  uint32_t stats_capacity, vba_result* results, int32_t* statuses);
+                  uint32_t stats_capacity, vba_result* results, int32_t* statuses, bool graph = false);
+void release(PreparedBatch& batch);
 // Final (before any solve: initial) state of the caller's window i.
 int32_t download_batch_window(Context& ctx, const PreparedBatch& batch, uint32_t i, double* poses_out,
                               double* landmarks_out);

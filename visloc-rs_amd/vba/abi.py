@@ -13,6 +13,7 @@ ABI_VERSION = 1
 
 OK = 0
 DECLINED = 1
+FLAG_NO_GRAPH = 1  # vba_options.flags: launch a prepared batch kernel by kernel
 PENDING = 2
 ERR_NO_POSES = -1
 ERR_NO_LANDMARKS = -2
@@ -132,7 +133,7 @@ class Options(C.Structure):
         ("pcg_relative_tolerance", C.c_double),
         ("pcg_absolute_tolerance", C.c_double),
         ("refine_intrinsics", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("flags", C.c_uint32),
     ]
 
 
