@@ -781,18 +781,30 @@ __global__ __launch_bounds__(kSmallBlock) void k_pcg_init(DevProblem P, double r
 __global__ __launch_bounds__(kBlock) void k_pcg_pose(DevProblem P, const double* __restrict__ part) {
     __shared__ double lds[(kBlock / 64) * 2];
     PcgState* st = P.pcg;
-    if (wave_stop(st)) return;
+    // the flag is loaded with the pose's data and decides after
+    const int stop = __hip_atomic_load(&st->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const double beta = st->beta, lambda = st->lambda;
     const int lane = threadIdx.x & 63;
     const int p = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    const int c = lane < 6 ? lane : 0;
+    const bool in = p < P.num_pose_slots;
+    const size_t k = 6 * (size_t)(in ? p : 0) + c;
+    const bool var = in && P.pose_var[in ? p : 0] >= 0;
+    const double z = P.z[k], d_old = P.d[k];
+    double u[6];
+    const double* U = P.U + 21 * (size_t)(in ? p : 0);
+#pragma unroll
+    for (int j = 0; j < 6; ++j) u[j] = U[c <= j ? up6(c, j) : up6(j, c)];
+    double s[6];
+    pose_run_sum<6>(P, part, in ? p : 0, lane, s);
+    if (__builtin_amdgcn_readfirstlane(stop)) return;
     double v[2] = {0.0, 0.0};
-    if (p < P.num_pose_slots && P.pose_var[p] >= 0) {
-        double s[6];
-        pose_run_sum<6>(P, part, p, lane, s);
-        const int c = lane < 6 ? lane : 0;
-        const size_t k = 6 * (size_t)p + c;
-        const double dc = P.z[k] + beta * P.d[k];
-        const double q = pose_apply_u(P, p, c, dc, lambda, s[c]);
+    if (var) {
+        const double dc = z + beta * d_old;
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) acc += u[j] * __shfl(dc, j, 64);
+        const double q = acc + lambda * dc - s[c];
         if (lane < 6) {
             P.d[k] = dc;
             P.q[k] = q;
@@ -805,33 +817,47 @@ __global__ __launch_bounds__(kBlock) void k_pcg_pose(DevProblem P, const double*
 
 // alpha = rho / d.q, x += alpha d, r -= alpha q, z = M^-1 r; per
 // work-group (r.z, r.r) partials. Lanes 6i..6i+5 of a wave hold pose i.
+// Everything that does not depend on alpha is loaded with the partials.
 __global__ __launch_bounds__(kBlock) void k_pcg_step(DevProblem P, int k) {
     __shared__ double lds[(kBlock / 64) * 2];
     PcgState* st = P.pcg;
-    if (wave_stop(st)) return;
+    const int stop = __hip_atomic_load(&st->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const double rho = st->rho[k & 1];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int pp = lane / 6, c = lane % 6;
+    const int p = (blockIdx.x * (kBlock / 64) + wave) * kPosesPerWave + pp;
+    const bool in = lane < 6 * kPosesPerWave && p < P.num_pose_slots;
+    const int ps = in ? p : 0;
+    const size_t i = 6 * (size_t)ps + c;
+    const bool valid = in && P.pose_var[ps] >= 0;
+    const double x = P.x[i], d = P.d[i], r = P.r[i], q = P.q[i];
+    double m[6];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) m[j] = P.Minv[36 * (size_t)ps + 6 * c + j];
     double curvature, unused;
     wave_pair_total(P.bpart, P.num_bpart, &curvature, &unused);
+    if (__builtin_amdgcn_readfirstlane(stop)) return;
     if (!isfinite(curvature) || !(curvature > 0.0)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) set_stop(st, PCG_NONPOSITIVE);
         return;
     }
-    const double alpha = st->rho[k & 1] / curvature;
+    const double alpha = rho / curvature;
     if (!isfinite(alpha)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) set_stop(st, PCG_NONFINITE);
         return;
     }
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int pp = lane / 6, c = lane % 6;
-    const int p = (blockIdx.x * (kBlock / 64) + wave) * kPosesPerWave + pp;
-    const bool valid = lane < 6 * kPosesPerWave && p < P.num_pose_slots && P.pose_var[p] >= 0;
-    const size_t i = 6 * (size_t)p + c;
     double rc = 0.0;
     if (valid) {
-        P.x[i] = P.x[i] + alpha * P.d[i];
-        rc = P.r[i] - alpha * P.q[i];
+        P.x[i] = x + alpha * d;
+        rc = r - alpha * q;
         P.r[i] = rc;
     }
-    const double zc = pose_minv(P, p, c, 6 * pp, rc, valid);
+    double zc = 0.0;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        const double rj = __shfl(rc, min(6 * pp + j, 63), 64);
+        if (valid) zc += m[j] * rj;
+    }
     double v[2] = {0.0, 0.0};
     if (valid) {
         P.z[i] = zc;
