@@ -488,15 +488,18 @@ __global__ __launch_bounds__(64) void k_schur_apply(DevProblem P, const double* 
     // both are in flight together (a stopped launch drops them) ----
     const int s0 = kApplyPer * lane;
     const int nv = one ? 0 : max(0, min(kApplyPer, n - s0));
-    int key[kApplyPer], jl[kApplyPer];
+    static_assert(kApplyPer <= 4 && kTileLandmarks <= 256, "local landmarks packed in one word");
+    int key[kApplyPer], slot[kApplyPer];
+    uint32_t jl4 = 0;  // the positions' local landmarks, a byte each
     TS G[kApplyPer][6];
 #pragma unroll
     for (int i = 0; i < kApplyPer; ++i) {
         key[i] = -1 - (s0 + i);  // unique for positions past the end
-        jl[i] = 0;
+        slot[i] = -1;
         if (i < nv) {
             key[i] = P.s_pose[e0 + s0 + i];
-            jl[i] = P.s_lm[e0 + s0 + i];
+            jl4 |= (uint32_t)P.s_lm[e0 + s0 + i] << (8 * i);
+            slot[i] = P.run_slot[e0 + s0 + i];  // read here, used by the run's partial at the end
             load6(Gl + 6 * (size_t)(e0 + s0 + i), G[i]);
         }
     }
@@ -531,7 +534,7 @@ __global__ __launch_bounds__(64) void k_schur_apply(DevProblem P, const double* 
                     load_d(v1, v2, beta, key[i], d);
                     pcur = key[i];
                 }
-                const int j = jl[i];
+                const int j = (jl4 >> (8 * i)) & 0xff;
                 const double X[3] = {sX[3 * j], sX[3 * j + 1], sX[3 * j + 2]};
                 double t[3];
                 edge_wtd(G[i], d, X, t);
@@ -584,25 +587,30 @@ __global__ __launch_bounds__(64) void k_schur_apply(DevProblem P, const double* 
             rid[i] = cnt - 1;
         }
     }
-    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double acc[6];
     bool own = false;   // the current segment's run starts in this lane
     int carried = -1;   // position whose run started in an earlier lane and ends here
 #pragma unroll
     for (int i = 0; i < kApplyPer; ++i) {
         if (i < nv) {
-            if (i == 0 || fin[i - 1]) own = head[i];
-            const int j = jl[i];
+            const int j = (jl4 >> (8 * i)) & 0xff;
             const double X[3] = {sX[3 * j], sX[3 * j + 1], sX[3 * j + 2]};
             const double u[3] = {sT[3 * j], sT[3 * j + 1], sT[3 * j + 2]};
             const double Gd[6] = {(double)G[i][0], (double)G[i][1], (double)G[i][2],
                                   (double)G[i][3], (double)G[i][4], (double)G[i][5]};
             double y[6];
             edge_wu(Gd, u, X, y);
+            if (i == 0 || fin[i - 1]) {  // a new segment of this lane
+                own = head[i];
 #pragma unroll
-            for (int c = 0; c < 6; ++c) acc[c] += y[c];
+                for (int c = 0; c < 6; ++c) acc[c] = y[c];
+            } else {
+#pragma unroll
+                for (int c = 0; c < 6; ++c) acc[c] += y[c];
+            }
             if (fin[i]) {
                 if (own) {
-                    double* o = part + 6 * (size_t)P.run_slot[e0 + s0 + i];
+                    double* o = part + 6 * (size_t)slot[i];
 #pragma unroll
                     for (int c = 0; c < 6; ++c) o[c] = acc[c];
                 } else if (rid[i] < kTileRuns) {
@@ -610,8 +618,6 @@ __global__ __launch_bounds__(64) void k_schur_apply(DevProblem P, const double* 
                     for (int c = 0; c < 6; ++c) atomicAdd(&sR[6 * rid[i] + c], acc[c]);
                     carried = i;
                 }
-#pragma unroll
-                for (int c = 0; c < 6; ++c) acc[c] = 0.0;
             }
         }
     }
@@ -624,7 +630,11 @@ __global__ __launch_bounds__(64) void k_schur_apply(DevProblem P, const double* 
     __syncthreads();
     if (carried >= 0) {
         const int r = rid[carried];
-        double* o = part + 6 * (size_t)P.run_slot[e0 + s0 + carried];
+        int sl = slot[0];
+#pragma unroll
+        for (int i = 1; i < kApplyPer; ++i)
+            if (i == carried) sl = slot[i];
+        double* o = part + 6 * (size_t)sl;
 #pragma unroll
         for (int c = 0; c < 6; ++c) o[c] = sR[6 * r + c];
     }
