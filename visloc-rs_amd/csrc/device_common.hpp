@@ -20,6 +20,16 @@ __device__ __forceinline__ int xcd_tile(int bid, int n) {
     return (bid & 7) * per + (bid >> 3);
 }
 inline int xcd_grid(int n) { return ((n + 7) >> 3) << 3; }
+
+// Float offsets of G0..G3 and of G4, G5 of the position with index pp in the
+// lane-major FP32 G copy (its layout is described with kG32Chunk).
+static_assert(kTileEdges == 256 && kG32Chunk == 6 * kTileEdges, "lane-major G chunk of one tile");
+__device__ __forceinline__ size_t g32_off4(int pp) {
+    return (size_t)(pp >> 8) * kG32Chunk + (pp & 3) * 384 + ((pp & 255) >> 2) * 4;
+}
+__device__ __forceinline__ size_t g32_off2(int pp) {
+    return (size_t)(pp >> 8) * kG32Chunk + (pp & 3) * 384 + 256 + ((pp & 255) >> 2) * 2;
+}
 inline int blocks_for(int n, int per) { return (n + per - 1) / per; }
 
 template <typename T>

@@ -93,10 +93,9 @@ __global__ __launch_bounds__(kBlock) void k_linearize(DevProblem P, int store32)
         P.hl64[3 * pl + 1] = h[1];
         P.hl64[3 * pl + 2] = h[2];
         if (store32) {
-            float2* g32 = (float2*)(P.Gl32 + 6 * pl);
-            g32[0] = make_float2((float)G[0], (float)G[1]);
-            g32[1] = make_float2((float)G[2], (float)G[3]);
-            g32[2] = make_float2((float)G[4], (float)G[5]);
+            const int pp = P.g32_pos[pl];
+            *(float4*)(P.Gl32 + g32_off4(pp)) = make_float4((float)G[0], (float)G[1], (float)G[2], (float)G[3]);
+            *(float2*)(P.Gl32 + g32_off2(pp)) = make_float2((float)G[4], (float)G[5]);
         }
         add_schur_block<double>(G, X, 1.0, acc);
         acc[21] += h[0];
@@ -151,7 +150,10 @@ __global__ void k_landmark_reduce(DevProblem P) {
     if (P.Vm) {  // V of the FP32-stored G, the one the mixed operator pairs with them
         double Vm[6] = {0, 0, 0, 0, 0, 0};
         for (int e = P.lm_edge_ptr[l]; e < P.lm_edge_ptr[l + 1]; ++e) {
-            const float* G = P.Gl32 + 6 * (size_t)P.lm_store[e];
+            const int pp = P.g32_pos[P.lm_store[e]];
+            const float4 a = *(const float4*)(P.Gl32 + g32_off4(pp));
+            const float2 b = *(const float2*)(P.Gl32 + g32_off2(pp));
+            const float G[6] = {a.x, a.y, a.z, a.w, b.x, b.y};
 #pragma unroll
             for (int i = 0; i < 6; ++i) Vm[i] += (double)G[i];
         }

@@ -80,6 +80,8 @@ void build_runs(Model* mp) {
     m.s_lm.assign(m.num_edges, 0);
     m.s_pose.assign(m.num_edges, 0);
     m.lm_store.assign(m.num_edges, 0);
+    m.g32_pos.assign(m.num_edges, 0);
+    m.num_g32_chunks = 0;
     m.big_tiles.clear();
     std::vector<int32_t> run_pose, run_end, count;
     std::vector<uint64_t> keys;
@@ -116,12 +118,15 @@ void build_runs(Model* mp) {
         for (int32_t l = m.lm_tiles[t].lm_begin; l < m.lm_tiles[t].lm_end; ++l)
             for (int32_t k = m.lm_edge_ptr[l] - e0; k < m.lm_edge_ptr[l + 1] - e0; ++k)
                 lm_of_k[k] = (uint8_t)(l - m.lm_tiles[t].lm_begin);
+        m.lm_tiles[t].g_chunk = m.num_g32_chunks;
         for (int32_t s = 0; s < n; ++s) {
             const int32_t k = n > kTileEdges ? s : m.tile_perm[e0 + s];
             m.s_pose[e0 + s] = m.l_pose[e0 + k];
             m.s_lm[e0 + s] = lm_of_k[k];
             m.lm_store[e0 + k] = e0 + s;
+            m.g32_pos[e0 + s] = m.num_g32_chunks * kTileEdges + s;
         }
+        m.num_g32_chunks += std::max<int32_t>(1, (n + kTileEdges - 1) / kTileEdges);
         for (int32_t s = 0; s < n; ++s)
             if (s == n - 1 || m.s_pose[e0 + s + 1] != m.s_pose[e0 + s]) {
                 run_pose.push_back(m.s_pose[e0 + s]);
@@ -488,7 +493,7 @@ int32_t build_model(const vba_problem& pb, Model* mp, uint64_t* error_id, std::s
                     ++e;
                 }
             }
-            m.lm_tiles.push_back({(int32_t)b, (int32_t)e, m.lm_edge_ptr[b], m.lm_edge_ptr[e]});
+            m.lm_tiles.push_back({(int32_t)b, (int32_t)e, m.lm_edge_ptr[b], m.lm_edge_ptr[e], 0});
             m.max_lm_tile_edges = std::max(m.max_lm_tile_edges, m.lm_edge_ptr[e] - m.lm_edge_ptr[b]);
             b = e;
         }

@@ -49,7 +49,17 @@ struct LmTile {
     int32_t lm_end;
     int32_t e_begin;   // its l-order edge range (lm_edge_ptr at lm_begin, lm_end)
     int32_t e_end;
+    int32_t g_chunk;   // its first chunk of the lane-major FP32 G copy
 };
+
+// The FP32 copy of G that the mixed-precision operator streams is stored
+// lane-major: a tile's sorted positions in chunks of kTileEdges (one chunk
+// unless the tile is a landmark with more edges), chunk c holding position
+// s = 4 i + k (lane i, slot k) at floats 1536 c + 384 k + 4 i (G0..G3) and
+// 1536 c + 384 k + 256 + 2 i (G4, G5). A wave's load of one slot is then 1 KiB
+// plus 512 bytes of consecutive memory. A position's chunk-relative index
+// pp = 256 c + s is g32_pos.
+constexpr int kG32Chunk = 1536;
 
 // Landmark tiles: runs of whole landmarks, each the work of one wave in the
 // operator kernel: at most kTileEdges edges (four per lane), kTileLandmarks
@@ -104,6 +114,8 @@ struct Model {
     std::vector<int32_t> big_tiles;  // tiles of one landmark with more than kTileEdges edges
     std::vector<int32_t> pose_run_ptr, run_slot;
     uint64_t num_runs = 0;
+    std::vector<int32_t> g32_pos;  // per storage position (lane-major FP32 G)
+    int32_t num_g32_chunks = 0;
 };
 
 // Lens of a vba_camera, resolved as MonoProjection::for_camera does

@@ -24,6 +24,8 @@
 // r.r partials). Scalars never leave the device between iterations.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <climits>
 #include <cmath>
 
@@ -427,17 +429,33 @@ __global__ __launch_bounds__(kTileThreads) void k_schur_tile(DevProblem P, const
 // ---------------------------------------------------------------------------
 constexpr int kApplyPer = kTileEdges / 64;  // sorted positions per lane
 
+// G of the edge at storage position e, index pp in the lane-major FP32 copy:
+// the FP32 operator reads that copy, the FP64 one the per-edge array.
+__device__ __forceinline__ void load_g(const float* __restrict__ Gl, size_t, int pp, float G[6]) {
+    const float4 a = *(const float4*)(Gl + g32_off4(pp));
+    const float2 b = *(const float2*)(Gl + g32_off2(pp));
+    G[0] = a.x;
+    G[1] = a.y;
+    G[2] = a.z;
+    G[3] = a.w;
+    G[4] = b.x;
+    G[5] = b.y;
+}
+__device__ __forceinline__ void load_g(const double* __restrict__ Gl, size_t e, int, double G[6]) {
+    load6(Gl + 6 * e, G);
+}
+
 template <typename TS>
 __device__ void apply_one_landmark(const DevProblem& P, const double* __restrict__ v1,
                                    const double* __restrict__ v2, double beta, const TS* __restrict__ Gl,
                                    const double* __restrict__ Vinv, double* __restrict__ part, int l, int e0,
-                                   int n, int lane) {
+                                   int g0, int n, int lane) {
     const double X[3] = {P.lms[3 * (size_t)l], P.lms[3 * (size_t)l + 1], P.lms[3 * (size_t)l + 2]};
     double t[3] = {0.0, 0.0, 0.0};
     for (int s = lane; s < n; s += 64) {
         TS G[6];
         double d[6], te[3];
-        load6(Gl + 6 * (size_t)(e0 + s), G);
+        load_g(Gl, (size_t)(e0 + s), g0 + s, G);
         load_d(v1, v2, beta, P.s_pose[e0 + s], d);
         edge_wtd(G, d, X, te);
         t[0] += te[0];
@@ -452,7 +470,7 @@ __device__ void apply_one_landmark(const DevProblem& P, const double* __restrict
     sym3_mul(vi, t, u);
     for (int s = lane; s < n; s += 64) {
         TS Gs[6];
-        load6(Gl + 6 * (size_t)(e0 + s), Gs);
+        load_g(Gl, (size_t)(e0 + s), g0 + s, Gs);
         const double G[6] = {(double)Gs[0], (double)Gs[1], (double)Gs[2], (double)Gs[3], (double)Gs[4], (double)Gs[5]};
         double y[6];
         edge_wu(G, u, X, y);
@@ -462,25 +480,47 @@ __device__ void apply_one_landmark(const DevProblem& P, const double* __restrict
     }
 }
 
+// Waves per work-group of the operator: each wave works on a tile of its own
+// and synchronises only with itself, so a work-group is a packing of
+// independent waves (a CU holds a limited number of work-groups).
+constexpr int kApplyWaves = 1;
+
+typedef __attribute__((address_space(3))) void* lds_void_ptr;
+
+// This wave's LDS writes are performed before its later LDS reads (all lanes).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 template <typename TS>
-__global__ __launch_bounds__(64) void k_schur_apply(DevProblem P, const double* __restrict__ v1,
-                                                    const double* __restrict__ v2, const TS* __restrict__ Gl,
-                                                    const double* __restrict__ Vinv, double* __restrict__ part,
-                                                    int pcg_iter, int fresh) {
+__global__ __launch_bounds__(64 * kApplyWaves) void k_schur_apply(DevProblem P, const double* __restrict__ v1,
+                                                                  const double* __restrict__ v2,
+                                                                  const TS* __restrict__ Gl,
+                                                                  const double* __restrict__ Vinv,
+                                                                  double* __restrict__ part, int pcg_iter,
+                                                                  int fresh) {
     static_assert(kTileRuns == 64 && kTileLandmarks == 64, "one landmark and one run accumulator per lane");
-    __shared__ double sX[kTileLandmarks * 3];
-    __shared__ double sT[kTileLandmarks * 3];  // t, then u, per landmark
+    // X per landmark, placed at sX = sXa + (lm_begin & 1) so that the LDS
+    // loads start at a 16-byte boundary in global memory and in LDS
+    __shared__ __align__(16) double sXa[kApplyWaves][kTileLandmarks * 3 + 2];
+    __shared__ double sTa[kApplyWaves][kTileLandmarks * 3];  // t, then u, per landmark
     // V^-1 per landmark (stages 0-2), then the sums of runs that cross lanes
     // (stage 3)
-    __shared__ double sVR[kTileLandmarks * 6];
-    double* const sV = sVR;
-    double* const sR = sVR;
-    const int tile = xcd_tile(blockIdx.x, P.num_lm_tiles);
+    __shared__ __align__(16) double sVRa[kApplyWaves][kTileLandmarks * 6];
+    const int wv = threadIdx.x >> 6;
+    double* sX = sXa[wv];
+    double* const sT = sTa[wv];
+    double* const sV = sVRa[wv];
+    double* const sR = sVRa[wv];
+    const int groups = (P.num_lm_tiles + kApplyWaves - 1) / kApplyWaves;
+    const int tile = xcd_tile(blockIdx.x, groups) * kApplyWaves + wv;
     if (tile >= P.num_lm_tiles) return;
     const LmTile lt = P.lm_tiles[tile];
     if (P.wstate && !P.wstate[P.lm_tile_win[tile]].active) return;
-    const int lane = threadIdx.x;
-    const int e0 = lt.e_begin;
+    const int lane = threadIdx.x & 63;
+    const int e0 = lt.e_begin, g0 = kTileEdges * lt.g_chunk;
     const int n = lt.e_end - lt.e_begin, nl = lt.lm_end - lt.lm_begin;
     const bool one = nl == 1 && n > kTileRuns;
 
@@ -500,28 +540,60 @@ __global__ __launch_bounds__(64) void k_schur_apply(DevProblem P, const double* 
             key[i] = P.s_pose[e0 + s0 + i];
             jl4 |= (uint32_t)P.s_lm[e0 + s0 + i] << (8 * i);
             slot[i] = P.run_slot[e0 + s0 + i];  // read here, used by the run's partial at the end
-            load6(Gl + 6 * (size_t)(e0 + s0 + i), G[i]);
+            if constexpr (std::is_same<TS, float>::value) {
+                // slot i of the tile's lane-major chunk: 1 KiB + 512 B per wave
+                const float* gb = Gl + (size_t)lt.g_chunk * kG32Chunk + 384 * i;
+                const float4 a = *(const float4*)(gb + 4 * lane);
+                const float2 b = *(const float2*)(gb + 256 + 2 * lane);
+                G[i][0] = a.x;
+                G[i][1] = a.y;
+                G[i][2] = a.z;
+                G[i][3] = a.w;
+                G[i][4] = b.x;
+                G[i][5] = b.y;
+            } else {
+                load6(Gl + 6 * (size_t)(e0 + s0 + i), G[i]);
+            }
         }
     }
+    // V^-1 and X of the tile's landmarks go straight from global memory into
+    // LDS (their layout there is the global one), in flight together with
+    // the loads above: 16-byte pieces, lane i's piece at byte 16 i of each
+    // 1 KiB chunk, over the tile's contiguous landmark range. The X range
+    // starts one double early when lm_begin is odd, and its last piece may
+    // end 8 bytes past the range (the landmark buffers carry 16 bytes of
+    // slack).
+    if (!one) {
+        const char* gv = reinterpret_cast<const char*>(Vinv + 6 * (size_t)lt.lm_begin);
+        const int xs = lt.lm_begin & 1;
+        const char* gx = reinterpret_cast<const char*>(P.lms + 3 * (size_t)lt.lm_begin - xs);
+        char* lx = reinterpret_cast<char*>(sX);
+        sX += xs;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            if (1024 * c + 16 * lane < 48 * nl)
+                __builtin_amdgcn_global_load_lds(gv + 1024 * c + 16 * lane, (lds_void_ptr)((char*)sV + 1024 * c), 16,
+                                                 0, 0);
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+            if (1024 * c + 16 * lane < 8 * xs + 24 * nl)
+                __builtin_amdgcn_global_load_lds(gx + 1024 * c + 16 * lane, (lds_void_ptr)(lx + 1024 * c), 16, 0, 0);
+    }
     double beta = 0.0;
-    if (pcg_iter >= 0 && !pcg_prologue(P, pcg_iter, fresh, &beta)) return;
+    if (pcg_iter >= 0 && !pcg_prologue(P, pcg_iter, fresh, &beta)) {
+        __builtin_amdgcn_s_waitcnt(0);  // no LDS write may land after the wave ends
+        return;
+    }
     if (one) {
-        apply_one_landmark<TS>(P, v1, v2, beta, Gl, Vinv, part, lt.lm_begin, e0, n, lane);
+        apply_one_landmark<TS>(P, v1, v2, beta, Gl, Vinv, part, lt.lm_begin, e0, g0, n, lane);
         return;
     }
     if (lane < nl) {
-        const int l = lt.lm_begin + lane;
-        double vi[6];
-        load6(Vinv + 6 * (size_t)l, vi);
 #pragma unroll
-        for (int c = 0; c < 6; ++c) sV[6 * lane + c] = vi[c];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            sX[3 * lane + c] = P.lms[3 * (size_t)l + c];
-            sT[3 * lane + c] = 0.0;
-        }
+        for (int c = 0; c < 3; ++c) sT[3 * lane + c] = 0.0;
     }
-    __syncthreads();
+    __builtin_amdgcn_s_waitcnt(0);  // the LDS loads have landed
+    wave_lds_sync();
 
     // ---- stage 1: t_j += W_e^T d_p, one d per pose run of this lane ----
     {
@@ -544,7 +616,7 @@ __global__ __launch_bounds__(64) void k_schur_apply(DevProblem P, const double* 
             }
         }
     }
-    __syncthreads();
+    wave_lds_sync();
 
     // ---- stage 2: u_j = V_j^-1 t_j ----
     if (lane < nl) {
@@ -561,7 +633,7 @@ __global__ __launch_bounds__(64) void k_schur_apply(DevProblem P, const double* 
     // first in its LDS order
 #pragma unroll
     for (int c = 0; c < 6; ++c) sR[6 * lane + c] = 0.0;
-    __syncthreads();
+    wave_lds_sync();
 
     // ---- stage 3: y_e = W_e u_j summed per camera run ----
     const int prev_key = __shfl_up(key[kApplyPer - 1], 1, 64);  // last key of the lane before
@@ -627,7 +699,7 @@ __global__ __launch_bounds__(64) void k_schur_apply(DevProblem P, const double* 
 #pragma unroll
         for (int c = 0; c < 6; ++c) atomicAdd(&sR[6 * r + c], acc[c]);
     }
-    __syncthreads();
+    wave_lds_sync();
     if (carried >= 0) {
         const int r = rid[carried];
         int sl = slot[0];
@@ -968,12 +1040,13 @@ void launch_rhs(const DevProblem& P, hipStream_t s) {
 
 void launch_schur_apply(const DevProblem& P, bool fp32, const double* v1, const double* v2, int32_t pcg_iter,
                         int32_t fresh, hipStream_t s) {
+    const dim3 grid(xcd_grid(blocks_for(P.num_lm_tiles, kApplyWaves))), block(64 * kApplyWaves);
     if (fp32)
-        hipLaunchKernelGGL(k_schur_apply<float>, dim3(xcd_grid(P.num_lm_tiles)), dim3(64), 0, s, P, v1, v2,
-                           (const float*)P.Gl32, (const double*)P.VinvM, P.part64, pcg_iter, fresh);
+        hipLaunchKernelGGL(k_schur_apply<float>, grid, block, 0, s, P, v1, v2, (const float*)P.Gl32,
+                           (const double*)P.VinvM, P.part64, pcg_iter, fresh);
     else
-        hipLaunchKernelGGL(k_schur_apply<double>, dim3(xcd_grid(P.num_lm_tiles)), dim3(64), 0, s, P, v1, v2,
-                           (const double*)P.Gl64, (const double*)P.Vinv64, P.part64, pcg_iter, fresh);
+        hipLaunchKernelGGL(k_schur_apply<double>, grid, block, 0, s, P, v1, v2, (const double*)P.Gl64,
+                           (const double*)P.Vinv64, P.part64, pcg_iter, fresh);
 }
 
 void launch_pcg_init(const DevProblem& P, double rtol, double atol, int32_t max_iter, double lambda,

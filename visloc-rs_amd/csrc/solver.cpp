@@ -30,7 +30,7 @@ enum BufId {
     B_VQ, B_PART64, B_PART21, B_BPART, B_CPART, B_PCG, B_DELTA_L, B_TILE_SCALAR,
     B_SCALARS, B_DENSE_BLOCK_PQ, B_DENSE_BLOCK_PTR, B_DENSE_TERMS, B_DENSE_VAR_SLOT, B_DENSE_S,
     B_DENSE_WORK, B_DENSE_STATUS, B_X_BLOCK_POS, B_X_ROW_PTR, B_X_COL, B_X_DIAG_POS, B_X_S64, B_X_S32, B_VD2,
-    B_COUNT
+    B_G32_POS, B_COUNT
 };
 
 // Profiled kernel families.
@@ -369,9 +369,11 @@ int32_t prepare(Context& ctx, Prepared& pr, std::string* msg) {
         upload(pr, B_POSES0, m.poses, s);
         upload(pr, B_LMS0, m.lms, s);
         ensure(pr, B_POSES, m.poses.size() * sizeof(double));
-        ensure(pr, B_LMS, m.lms.size() * sizeof(double));
+        // 16 bytes past the last landmark: the operator moves landmark
+        // positions into LDS in whole 16-byte pieces from a 16-byte boundary
+        ensure(pr, B_LMS, m.lms.size() * sizeof(double) + 16);
         ensure(pr, B_POSES_T, m.poses.size() * sizeof(double));
-        ensure(pr, B_LMS_T, m.lms.size() * sizeof(double));
+        ensure(pr, B_LMS_T, m.lms.size() * sizeof(double) + 16);
         upload(pr, B_SENSORS, m.sensors, s);
         upload(pr, B_POSE_VAR, m.pose_var, s);
         upload(pr, B_LM_VAR, m.lm_var, s);
@@ -391,6 +393,7 @@ int32_t prepare(Context& ctx, Prepared& pr, std::string* msg) {
         upload(pr, B_POSE_RUN_PTR, m.pose_run_ptr, s);
         upload(pr, B_RUN_SLOT, m.run_slot, s);
         upload(pr, B_S_LM, m.s_lm, s);
+        upload(pr, B_G32_POS, m.g32_pos, s);
         hip_check(hipStreamSynchronize(s), "sync");
         pr.final_slot = -1;
         return VBA_OK;
@@ -478,8 +481,9 @@ DevProblem bind(const Context& ctx, Prepared& pr, const vba_options& o, bool mix
     P.s_lm = buf<uint8_t>(pr, B_S_LM);
     P.Gl64 = (double*)ensure(pr, B_GL64, 6 * NE * sizeof(double));
     P.hl64 = (double*)ensure(pr, B_HL64, 3 * NE * sizeof(double));
+    P.g32_pos = buf<int32_t>(pr, B_G32_POS);
     if (mixed) {
-        P.Gl32 = (float*)ensure(pr, B_GL32, 6 * NE * sizeof(float));
+        P.Gl32 = (float*)ensure(pr, B_GL32, (size_t)kG32Chunk * std::max(m.num_g32_chunks, 1) * sizeof(float));
         P.Vm = (double*)ensure(pr, B_VM, 6 * NL * sizeof(double));
         P.VinvM = (double*)ensure(pr, B_VINV32, 6 * NL * sizeof(double));
     }
@@ -923,7 +927,7 @@ void merge_models(const std::vector<const Model*>& ms, Model* out, std::vector<W
         for (int32_t t : m.big_tiles) M.big_tiles.push_back((int32_t)(t + M.lm_tiles.size()));
         for (const LmTile& t : m.lm_tiles) {
             M.lm_tiles.push_back({(int32_t)(t.lm_begin + lo), (int32_t)(t.lm_end + lo), (int32_t)(t.e_begin + eo),
-                                  (int32_t)(t.e_end + eo)});
+                                  (int32_t)(t.e_end + eo), t.g_chunk + M.num_g32_chunks});
             lm_tile_win->push_back((int32_t)w);
         }
         // camera runs: tile-local permutations are unchanged; run ends shift
@@ -933,6 +937,8 @@ void merge_models(const std::vector<const Model*>& ms, Model* out, std::vector<W
         for (uint32_t s2 = 0; s2 < m.num_pose_slots; ++s2)
             M.pose_run_ptr.push_back((int32_t)(m.pose_run_ptr[s2] + ro));
         for (int32_t r : m.run_slot) M.run_slot.push_back(r < 0 ? -1 : (int32_t)(r + ro));
+        for (int32_t g : m.g32_pos) M.g32_pos.push_back(g + kTileEdges * M.num_g32_chunks);
+        M.num_g32_chunks += m.num_g32_chunks;
         ro += (int64_t)m.num_runs;
         M.max_lm_tile_edges = std::max(M.max_lm_tile_edges, m.max_lm_tile_edges);
         if (m.any_fixed_pose) M.any_fixed_pose = true;
