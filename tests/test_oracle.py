@@ -192,6 +192,20 @@ def test_projection_closed_forms():
         assert oracle.mono_project(cam, [0.1, 0.1, -1.0]) is None
 
 
+def test_opencv_projection_known_answer():
+    """imported_opencv_camera_applies_tangential_distortion
+    (crates/io/tests/colmap.rs:651-668): the pixel OpenCV 4.10
+    cv2.projectPoints gives for a real COLMAP OPENCV camera, to 1e-8 px, with
+    the tangential terms (a k1/k2-only projection misses it by pixels)."""
+    cam = vba.camera(abi.CAMERA_OPENCV, [1353.09, 1338.03, 962.7, 539.93, 0.0346, -0.0235, 0.0012, -0.0008])
+    uv = oracle.camera_project(cam, [0.5, -0.3, 1.0])
+    assert abs(uv[0] - 1643.9695843529998) < 1e-8 and abs(uv[1] - 136.0453501334) < 1e-8, uv
+    uv2, _ = oracle.mono_project(cam, [0.5, -0.3, 1.0])
+    assert abs(uv2[0] - 1643.9695843529998) < 1e-8 and abs(uv2[1] - 136.0453501334) < 1e-8, uv2
+    radial_only = vba.camera(abi.CAMERA_OPENCV, [1353.09, 1338.03, 962.7, 539.93, 0.0346, -0.0235, 0.0, 0.0])
+    assert np.abs(oracle.camera_project(radial_only, [0.5, -0.3, 1.0]) - uv).max() > 1.0
+
+
 LENSES = [
     vba.camera(abi.CAMERA_PINHOLE, [512.5, 498.25, 321.0, 239.5]),
     vba.camera(abi.CAMERA_SIMPLE_RADIAL, [500.0, 320.0, 240.0, 0.01]),

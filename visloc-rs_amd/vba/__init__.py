@@ -8,9 +8,10 @@ from .abi import library
 from .api import (BatchOutput, BundleProblem, Device, PreparedBatch, PreparedProblem, PrepareError, ServedResult,
                   Server, SolveOutput, Synthetic, camera, camera_centres, check, default_options, run_solver,
                   synth_params, synthesize)
+from .io import IoError, Model
 
 __all__ = [
     "abi", "library", "BatchOutput", "BundleProblem", "Device", "PreparedBatch", "PreparedProblem", "PrepareError",
     "ServedResult", "Server", "SolveOutput", "Synthetic", "camera", "camera_centres", "check", "default_options",
-    "run_solver", "synth_params", "synthesize",
+    "run_solver", "synth_params", "synthesize", "IoError", "Model",
 ]

@@ -30,6 +30,9 @@ ERR_LINEAR_SOLVE = -22
 ERR_INVALID_ARGUMENT = -100
 ERR_DEVICE = -101
 ERR_OUT_OF_MEMORY = -102
+ERR_IO = -110           # vba_io.h: a file could not be read or written
+ERR_FORMAT = -111       # vba_io.h: malformed content
+ERR_EXPORT = -112       # vba_io.h: the model cannot be written in the asked format
 
 CAMERA_PINHOLE = 0
 CAMERA_SIMPLE_PINHOLE = 1
@@ -269,6 +272,18 @@ SIGNATURES = {
     "vba_server_counters": (None, [C.c_void_p, PU64, PU64, PU64, PU64]),
     "vba_server_destroy": (None, [C.c_void_p]),
     "vba_reduced_solve": (C.c_int32, [C.c_void_p, C.POINTER(ReducedSystem), P64, C.POINTER(C.c_uint32)]),
+    # vba_io.h: problems on disk
+    "vba_io_last_error": (C.c_char_p, []),
+    "vba_model_read_ba_problem": (C.c_int32, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    "vba_model_write_ba_problem": (C.c_int32, [C.c_void_p, C.c_char_p]),
+    "vba_model_read_colmap": (C.c_int32, [C.c_char_p, C.c_int32, C.POINTER(C.c_void_p)]),
+    "vba_model_write_colmap": (C.c_int32, [C.c_void_p, C.c_char_p, C.c_int32]),
+    "vba_model_from_problem": (C.c_int32, [C.POINTER(Problem), P64, C.POINTER(C.c_void_p)]),
+    "vba_model_problem": (C.c_int32, [C.c_void_p, C.POINTER(Problem), C.POINTER(P64)]),
+    "vba_model_set_state": (C.c_int32, [C.c_void_p, P64, P64]),
+    "vba_model_set_fixed": (C.c_int32, [C.c_void_p, C.POINTER(C.c_uint8)]),
+    "vba_model_keep_local_window": (C.c_int32, [C.c_void_p, C.c_uint32]),
+    "vba_model_free": (None, [C.c_void_p]),
     "vba_profile_enable": (None, [C.c_void_p, C.c_int32]),
     "vba_profile_reset": (None, [C.c_void_p]),
     "vba_profile_query": (C.c_int32, [C.c_void_p, C.c_char_p, PU64, P64, P64]),
