@@ -432,6 +432,7 @@ def main():
     ap.add_argument("--windows", type=int, default=256, help="windows in the batch (--workload windows)")
     ap.add_argument("--profile-stride", type=int, default=16,
                     help="record kernel times around every n-th launch of each kernel family")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default PCG launch schedule")
     ap.add_argument("--solver", choices=["matrix-free", "explicit"], default="matrix-free",
                     help="reduced camera PCG of the loop workload: implicit operator (the workload's own "
                          "setting) or the explicitly assembled block-sparse matrix")
@@ -472,6 +473,16 @@ def main():
         log(f"bench: warmup {w}: {out.result.num_iterations} LM iterations, {out.result.total_pcg_iterations} PCG, "
             f"cost {out.result.initial_cost:.6e} -> {out.result.final_cost:.6e}, "
             f"device {out.result.device_seconds:.3f} s")
+
+    # The PCG launch schedule picked by timing the candidate schedules on this
+    # problem (untimed; every candidate gives the same result bit for bit).
+    tuning = None
+    if not args.no_autotune:
+        t1 = time.perf_counter()
+        tuning = prep.autotune(opts, lm_iterations=3)
+        log(f"bench: autotuned PCG schedule {tuning['tuning']} ({tuning['seconds'] * 1e3:.1f} ms vs "
+            f"{tuning['default_seconds'] * 1e3:.1f} ms for the default, 3 LM iterations) in "
+            f"{time.perf_counter() - t1:.2f} s")
 
     # HIP events around every 16th launch of each kernel family: enough launches
     # for the averages, few enough that recording them does not slow the solve
@@ -569,6 +580,7 @@ def main():
         "roofline": roofline,
         "kernels": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in kernels.items()},
+        "pcg_schedule": tuning,
     }
     prep.free()
     dev.close()

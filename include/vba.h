@@ -378,6 +378,41 @@ void vba_profile_reset(vba_context* ctx);
 int32_t vba_profile_query(vba_context* ctx, const char* kernel, uint64_t* launches,
                           double* seconds, double* bytes);
 
+/* ---- PCG launch schedule and its autotuner --------------------------------
+ * The reference's GPU backend dispatches PCG in chunks of 10 iterations and
+ * reads its done flag between chunks (crates/ba-gpu/src/solver.rs:493-510;
+ * VISLOC_BA_GPU_PCG_CHUNK); the chunk is the first field below, its default
+ * the reference's. The device decides when PCG stops and launches after the stop do
+ * nothing, so no field here changes a result, only how much host-device
+ * waiting and how many empty launches a solve costs:
+ * - pcg_chunk: iterations enqueued per read of the device PCG state (>= 1;
+ *   default 10);
+ * - pcg_lookahead: chunks enqueued beyond the one whose state is awaited
+ *   (0..2; default 0), so that the device does not idle while the host reads;
+ * - pcg_predict: nonzero: a solve's first chunk runs to the iteration count
+ *   of the previous PCG solve of the same prepared problem (default 0).
+ * The schedule is per context and applies to the matrix-free and explicit
+ * PCG solvers. */
+typedef struct vba_tuning {
+    uint32_t pcg_chunk;
+    uint32_t pcg_lookahead;
+    uint32_t pcg_predict;
+    uint32_t reserved;
+} vba_tuning;
+/* NULL restores the defaults. VBA_ERR_INVALID_ARGUMENT for a field out of
+ * range. */
+int32_t vba_set_tuning(vba_context* ctx, const vba_tuning* tuning);
+int32_t vba_get_tuning(const vba_context* ctx, vba_tuning* tuning);
+/* Times each schedule of a fixed candidate set on `prepared` (a PCG solver in
+ * `options`, max_iterations replaced by lm_iterations when that is nonzero;
+ * each candidate solved twice, the faster kept), checks that every candidate
+ * gives the default schedule's result bit for bit (VBA_ERR_DEVICE
+ * otherwise), sets the fastest on the context and returns it in *best with
+ * its seconds and the default schedule's. */
+int32_t vba_autotune(vba_context* ctx, vba_prepared* prepared, const vba_options* options,
+                     uint32_t lm_iterations, vba_tuning* best, double* best_seconds,
+                     double* default_seconds);
+
 /* ---- reduced camera solve ------------------------------------------------
  * The iterative reduced-camera solve of the COLMAP-port rig solver,
  * `ReducedSolveAccelerator::solve` (pipelines/slam/src/ba_accel.rs:46-157;

@@ -41,6 +41,7 @@ def test_struct_sizes_match_c_layout():
     assert C.sizeof(abi.Sensor) == 168
     assert C.sizeof(abi.IterationStats) == 72
     assert C.sizeof(abi.Result) == 56
+    assert C.sizeof(abi.Tuning) == 16
 
 
 def test_default_options_are_baconfig_default():

@@ -1,0 +1,76 @@
+"""The PCG launch schedule (vba_tuning) and its autotuner. The device decides
+when PCG stops and launches past the stop do nothing, so every schedule must
+give the default schedule's result bit for bit: iteration records, final
+cost and final state."""
+import numpy as np
+import pytest
+
+import vba
+from vba import abi
+
+pytestmark = pytest.mark.gpu
+
+SCHEDULES = [(10, 0, False), (1, 0, False), (3, 2, True), (7, 1, False), (25, 0, True), (4, 2, False)]
+
+
+def loop_options(solver, precision, max_pcg=100):
+    return vba.default_options(linear_solver=solver, max_iterations=6, max_pcg_iterations=max_pcg,
+                               pcg_relative_tolerance=1e-4, robust_kernel=abi.KERNEL_HUBER, robust_scale=1.0,
+                               pcg_maxit_policy=abi.PCG_MAXIT_ACCEPT_ITERATE, precision=precision)
+
+
+def solve_all(device, pp, o):
+    outs = []
+    for chunk, ahead, predict in SCHEDULES:
+        device.set_tuning(chunk, ahead, predict)
+        out = pp.solve(o)
+        assert out.status == abi.OK, out.message
+        outs.append((out, pp.state()))
+    device.set_tuning()
+    return outs
+
+
+@pytest.mark.parametrize("solver", [abi.SOLVER_MATRIX_FREE, abi.SOLVER_PCG_EXPLICIT])
+@pytest.mark.parametrize("precision", [abi.PRECISION_FP64, abi.PRECISION_MIXED])
+@pytest.mark.parametrize("max_pcg", [100, 7])
+def test_schedules_give_identical_results(device, solver, precision, max_pcg):
+    s = vba.synthesize(abi.SYNTH_LOOP, num_cameras=80, num_landmarks=6000, seed=2, scene_scale=16.0)
+    o = loop_options(solver, precision, max_pcg)
+    pp = device.prepare(s.problem, o)
+    outs = solve_all(device, pp, o)
+    (ref, (p0, l0)) = outs[0]
+    assert ref.result.total_pcg_iterations > 0
+    for out, (p, l) in outs[1:]:
+        assert out.iterations == ref.iterations
+        assert out.result.final_cost == ref.result.final_cost
+        assert out.result.total_pcg_iterations == ref.result.total_pcg_iterations
+        np.testing.assert_array_equal(p, p0)
+        np.testing.assert_array_equal(l, l0)
+    pp.free()
+
+
+def test_autotune_keeps_the_result(device):
+    s = vba.synthesize(abi.SYNTH_LOOP, num_cameras=80, num_landmarks=6000, seed=5, scene_scale=16.0)
+    o = loop_options(abi.SOLVER_MATRIX_FREE, abi.PRECISION_MIXED)
+    pp = device.prepare(s.problem, o)
+    device.set_tuning()
+    ref = pp.solve(o)
+    p0, l0 = pp.state()
+    tuned = pp.autotune(o, lm_iterations=2)
+    assert tuned["seconds"] <= tuned["default_seconds"]
+    assert device.tuning() == tuned["tuning"]
+    out = pp.solve(o)
+    assert out.iterations == ref.iterations and out.result.final_cost == ref.result.final_cost
+    p, l = pp.state()
+    np.testing.assert_array_equal(p, p0)
+    np.testing.assert_array_equal(l, l0)
+    device.set_tuning()
+    pp.free()
+
+
+def test_tuning_rejects_out_of_range(device):
+    with pytest.raises(ValueError):
+        device.set_tuning(0, 0, False)
+    with pytest.raises(ValueError):
+        device.set_tuning(10, 3, False)
+    assert device.tuning() == {"pcg_chunk": 10, "pcg_lookahead": 0, "pcg_predict": 0}

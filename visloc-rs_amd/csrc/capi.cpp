@@ -1,6 +1,8 @@
 // capi.cpp — the extern "C" entry points of include/vba.h.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -419,6 +421,106 @@ int32_t vba_profile_query(vba_context* ctx, const char* kernel, uint64_t* launch
     if (launches) *launches = 0;
     if (seconds) *seconds = 0.0;
     if (bytes) *bytes = 0.0;
+    return VBA_OK;
+}
+
+int32_t vba_set_tuning(vba_context* ctx, const vba_tuning* tuning) {
+    if (!ctx) return VBA_ERR_INVALID_ARGUMENT;
+    vba_tuning t{10, 0, 0, 0};
+    if (tuning) {
+        if (tuning->pcg_chunk < 1 || tuning->pcg_chunk > 1000 || tuning->pcg_lookahead > 2) {
+            std::lock_guard<std::mutex> lock(ctx->mu);
+            ctx->last_error = "vba_set_tuning: pcg_chunk must be 1..1000 and pcg_lookahead 0..2";
+            return VBA_ERR_INVALID_ARGUMENT;
+        }
+        t = *tuning;
+        t.pcg_predict = t.pcg_predict ? 1u : 0u;
+        t.reserved = 0;
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->tuning = t;
+    return VBA_OK;
+}
+
+int32_t vba_get_tuning(const vba_context* ctx, vba_tuning* tuning) {
+    if (!ctx || !tuning) return VBA_ERR_INVALID_ARGUMENT;
+    std::lock_guard<std::mutex> lock(const_cast<vba_context*>(ctx)->mu);
+    *tuning = ctx->tuning;
+    return VBA_OK;
+}
+
+// The candidate schedules: the reference's chunk of 10 alone (the default)
+// and with prediction and lookahead, and shorter and longer chunks.
+int32_t vba_autotune(vba_context* ctx, vba_prepared* prepared, const vba_options* options, uint32_t lm_iterations,
+                     vba_tuning* best, double* best_seconds, double* default_seconds) {
+    if (!ctx || !prepared || !options) return VBA_ERR_INVALID_ARGUMENT;
+    if (options->linear_solver == VBA_SOLVER_DENSE) {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        ctx->last_error = "vba_autotune: the schedule applies to the PCG solvers";
+        return VBA_ERR_INVALID_ARGUMENT;
+    }
+    vba_options o = *options;
+    if (lm_iterations) o.max_iterations = lm_iterations;
+    static const vba_tuning kCandidates[] = {
+        {10, 0, 0, 0}, {10, 1, 0, 0}, {10, 1, 1, 0}, {5, 1, 1, 0}, {5, 2, 1, 0},
+        {20, 1, 1, 0}, {20, 0, 1, 0}, {4, 2, 1, 0},
+    };
+    const uint32_t cap = o.max_iterations ? o.max_iterations : 1;
+    std::vector<vba_iteration_stats> ref_stats(cap), stats(cap);
+    vba_result ref{}, res{};
+    double t_best = 0.0, t_default = 0.0;
+    vba_tuning chosen = kCandidates[0];
+    const vba_tuning saved = ctx->tuning;
+    auto same = [&](const vba_result& a, const std::vector<vba_iteration_stats>& sa, const vba_result& b,
+                    const std::vector<vba_iteration_stats>& sb) {
+        if (std::memcmp(&a.final_cost, &b.final_cost, sizeof(double)) || a.num_iterations != b.num_iterations ||
+            a.total_pcg_iterations != b.total_pcg_iterations)
+            return false;
+        for (uint32_t i = 0; i < std::min(a.num_iterations, cap); ++i)
+            if (sa[i].step_accepted != sb[i].step_accepted || sa[i].pcg_iterations != sb[i].pcg_iterations ||
+                std::memcmp(&sa[i].cost_after, &sb[i].cost_after, sizeof(double)))
+                return false;
+        return true;
+    };
+    for (size_t c = 0; c < sizeof(kCandidates) / sizeof(kCandidates[0]); ++c) {
+        {
+            std::lock_guard<std::mutex> lock(ctx->mu);
+            ctx->tuning = kCandidates[c];
+        }
+        double t = 0.0;
+        for (int rep = 0; rep < 2; ++rep) {
+            const auto t0 = std::chrono::steady_clock::now();
+            const int32_t st = vba_solve_prepared(ctx, prepared, &o, stats.data(), cap, &res);
+            const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (st != VBA_OK) {
+                std::lock_guard<std::mutex> lock(ctx->mu);
+                ctx->tuning = saved;
+                return st;
+            }
+            if (c == 0 && rep == 0) {
+                ref = res;
+                ref_stats = stats;
+            } else if (!same(ref, ref_stats, res, stats)) {
+                std::lock_guard<std::mutex> lock(ctx->mu);
+                ctx->tuning = saved;
+                ctx->last_error = "vba_autotune: a PCG schedule changed the result";
+                return VBA_ERR_DEVICE;
+            }
+            t = rep == 0 ? dt : std::min(t, dt);
+        }
+        if (c == 0) t_default = t;
+        if (c == 0 || t < t_best) {
+            t_best = t;
+            chosen = kCandidates[c];
+        }
+    }
+    {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        ctx->tuning = chosen;
+    }
+    if (best) *best = chosen;
+    if (best_seconds) *best_seconds = t_best;
+    if (default_seconds) *default_seconds = t_default;
     return VBA_OK;
 }
 

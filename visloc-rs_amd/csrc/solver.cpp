@@ -229,13 +229,17 @@ struct ProfScope {
         ctx.pending.push_back({a, b, kernel, bytes, pcg_iter});
     }
 };
-// Fold finished event pairs into the per-kernel totals (after a sync).
-void harvest(Context& ctx) {
+// Fold the first n pending event pairs, which must have completed (dropped
+// ones may still be in flight: they are only returned to the pool), into the
+// per-kernel totals. Returns n.
+size_t harvest_upto(Context& ctx, size_t n) {
     if (ctx.kernels.size() < K_COUNT) {
         ctx.kernels.resize(K_COUNT);
         for (int k = 0; k < K_COUNT; ++k) ctx.kernels[k].name = kProfNames[k];
     }
-    for (const Context::Pending& p : ctx.pending) {
+    n = std::min(n, ctx.pending.size());
+    for (size_t i = 0; i < n; ++i) {
+        const Context::Pending& p = ctx.pending[i];
         if (p.kernel < 0) {  // a launch that exited early (PCG already finished)
             ctx.event_pool.push_back(p.a);
             ctx.event_pool.push_back(p.b);
@@ -250,8 +254,11 @@ void harvest(Context& ctx) {
         ctx.event_pool.push_back(p.a);
         ctx.event_pool.push_back(p.b);
     }
-    ctx.pending.clear();
+    ctx.pending.erase(ctx.pending.begin(), ctx.pending.begin() + (ptrdiff_t)n);
+    return n;
 }
+// All of them (after a sync).
+void harvest(Context& ctx) { harvest_upto(ctx, ctx.pending.size()); }
 
 // Drop the pending PCG launches of iterations the device did not run (it
 // had stopped; those launches exited at once).
@@ -344,6 +351,12 @@ void release_context(Context& ctx) {
     if (ctx.host_scalars) (void)hipHostFree(ctx.host_scalars);
     if (ctx.host_pcg) (void)hipHostFree(ctx.host_pcg);
     if (ctx.host_status) (void)hipHostFree(ctx.host_status);
+    if (ctx.host_pcg_ring) (void)hipHostFree(ctx.host_pcg_ring);
+    ctx.host_pcg_ring = nullptr;
+    for (hipEvent_t& e : ctx.pcg_ev) {
+        if (e) (void)hipEventDestroy(e);
+        e = nullptr;
+    }
     ctx.host_scalars = nullptr;
     ctx.host_pcg = nullptr;
     ctx.host_status = nullptr;
@@ -365,6 +378,12 @@ int32_t prepare(Context& ctx, Prepared& pr, std::string* msg) {
             hip_check(hipHostMalloc((void**)&ctx.host_scalars, sizeof(StepScalars), hipHostMallocDefault), "hipHostMalloc");
             hip_check(hipHostMalloc((void**)&ctx.host_pcg, sizeof(PcgState), hipHostMallocDefault), "hipHostMalloc");
             hip_check(hipHostMalloc((void**)&ctx.host_status, sizeof(int32_t), hipHostMallocDefault), "hipHostMalloc");
+        }
+        if (!ctx.host_pcg_ring) {
+            hip_check(hipHostMalloc((void**)&ctx.host_pcg_ring, Context::kPcgRing * sizeof(PcgState),
+                                    hipHostMallocDefault), "hipHostMalloc");
+            for (hipEvent_t& e : ctx.pcg_ev)
+                hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
         }
         upload(pr, B_POSES0, m.poses, s);
         upload(pr, B_LMS0, m.lms, s);
@@ -636,7 +655,6 @@ int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stat
         double lambda = o.has_initial_lambda ? o.initial_lambda : 0.0;
         bool converged = false, need_lin = true;
         uint32_t written = 0, total_pcg = 0, linearizations = 0;
-        const int32_t chunk = 10;
         const int32_t max_pcg = (int32_t)o.max_pcg_iterations;
         auto push = [&](const vba_iteration_stats& st) {
             if (stats && written < cap) stats[written] = st;
@@ -679,49 +697,94 @@ int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stat
                     launch_precondition(P, lambda, s);
                 }
                 launch_pcg_init(P, o.pcg_relative_tolerance, o.pcg_absolute_tolerance, max_pcg, lambda, s);
-                // PCG iterations k .. in chunks; the device stops itself, the
-                // host reads the state once per chunk.
                 int32_t k = 0;
                 int32_t fresh = 1;
                 PcgState ps{};
-                while (true) {
-                    while (true) {
-                        const size_t batch = ctx.pending.size();
-                        const int32_t k_end = std::min(k + chunk, max_pcg);
-                        for (int32_t kk = k; kk < k_end; ++kk) {
-                            if (xplicit) {
-                                // iteration kk reads d from dbuf[kk & 1] and writes the next
-                                double* d_new = dbuf[(kk + 1) & 1];
-                                {
-                                    ProfScope a(ctx, K_EXPLICIT_APPLY, bytes_xapply, kk);
-                                    launch_pcg_explicit(P, X, mixed, dbuf[kk & 1], d_new, kk, fresh && kk == k, s);
-                                }
-                                DevProblem Ps = P;
-                                Ps.d = d_new;
-                                ProfScope c(ctx, K_PCG_STEP, bytes_pcg_step, kk);
-                                launch_pcg_step(Ps, kk, s);
-                                continue;
-                            }
-                            {
-                                ProfScope a(ctx, K_SCHUR_APPLY, bytes_apply, kk);
-                                launch_schur_apply(P, mixed, P.z, P.d, kk,
-                                                   fresh && kk == k, s);
-                            }
-                            {
-                                ProfScope b(ctx, K_PCG_POSE, bytes_pcg_pose, kk);
-                                launch_pcg_pose(P, s);
-                            }
-                            {
-                                ProfScope c(ctx, K_PCG_STEP, bytes_pcg_step, kk);
-                                launch_pcg_step(P, kk, s);
-                            }
+                // one PCG iteration's launches; fresh_kk: the first of a
+                // solve or restart (d starts from z)
+                auto launch_iteration = [&](int32_t kk, int32_t fresh_kk) {
+                    if (xplicit) {
+                        // iteration kk reads d from dbuf[kk & 1] and writes the next
+                        double* d_new = dbuf[(kk + 1) & 1];
+                        {
+                            ProfScope a(ctx, K_EXPLICIT_APPLY, bytes_xapply, kk);
+                            launch_pcg_explicit(P, X, mixed, dbuf[kk & 1], d_new, kk, fresh_kk, s);
                         }
-                        launch_pcg_status(P, k_end, fresh && k_end == k, s);
-                        ps = read_pcg();
-                        drop_after(ctx, batch, ps.iter);
-                        harvest(ctx);
-                        if (ps.stop) break;
-                        if (ps.iter <= k) throw Fail{VBA_ERR_DEVICE, "PCG made no progress"};
+                        DevProblem Ps = P;
+                        Ps.d = d_new;
+                        ProfScope c(ctx, K_PCG_STEP, bytes_pcg_step, kk);
+                        launch_pcg_step(Ps, kk, s);
+                        return;
+                    }
+                    {
+                        ProfScope a(ctx, K_SCHUR_APPLY, bytes_apply, kk);
+                        launch_schur_apply(P, mixed, P.z, P.d, kk, fresh_kk, s);
+                    }
+                    {
+                        ProfScope b(ctx, K_PCG_POSE, bytes_pcg_pose, kk);
+                        launch_pcg_pose(P, s);
+                    }
+                    {
+                        ProfScope c(ctx, K_PCG_STEP, bytes_pcg_step, kk);
+                        launch_pcg_step(P, kk, s);
+                    }
+                };
+                while (true) {
+                    // Iterations k .. in chunks (the context's schedule): each
+                    // chunk ends with a status kernel and a copy of the PCG
+                    // state into a pinned ring slot; the host waits for one
+                    // chunk's slot while `lookahead` later chunks are already
+                    // queued. The device stops itself, so launches past the
+                    // stop do nothing and no schedule changes a result.
+                    {
+                        struct Ticket {
+                            int32_t end;
+                            int slot;
+                            size_t pending_end;
+                        };
+                        Ticket inflight[Context::kPcgRing];
+                        int n_inflight = 0;
+                        const vba_tuning& T = ctx.tuning;
+                        const int32_t k0 = k, fresh0 = fresh;
+                        int32_t next = k0;
+                        auto enqueue = [&](int32_t end) {
+                            for (int32_t kk = next; kk < end; ++kk) launch_iteration(kk, fresh0 && kk == k0);
+                            launch_pcg_status(P, end, fresh0 && end == k0, s);
+                            const int slot = (int)(ctx.pcg_ring_next++ % Context::kPcgRing);
+                            hip_check(hipMemcpyAsync(ctx.host_pcg_ring + slot, P.pcg, sizeof(PcgState),
+                                                     hipMemcpyDeviceToHost, s), "D2H");
+                            hip_check(hipEventRecord(ctx.pcg_ev[slot], s), "hipEventRecord");
+                            inflight[n_inflight++] = Ticket{end, slot, ctx.pending.size()};
+                            next = end;
+                        };
+                        int32_t first = k0 + (int32_t)T.pcg_chunk;
+                        if (T.pcg_predict && k0 == 0 && pr.last_pcg > first) first = pr.last_pcg;
+                        enqueue(std::min(first, max_pcg));
+                        for (uint32_t a = 0; a < T.pcg_lookahead && next < max_pcg; ++a)
+                            enqueue(std::min(next + (int32_t)T.pcg_chunk, max_pcg));
+                        int32_t seen = k0;
+                        while (true) {
+                            if (n_inflight == 0) throw Fail{VBA_ERR_DEVICE, "PCG did not stop at its cap"};
+                            const Ticket t = inflight[0];
+                            for (int i = 1; i < n_inflight; ++i) inflight[i - 1] = inflight[i];
+                            --n_inflight;
+                            hip_check(hipGetLastError(), "kernel launch");
+                            hip_check(hipEventSynchronize(ctx.pcg_ev[t.slot]), "hipEventSynchronize");
+                            ps = ctx.host_pcg_ring[t.slot];
+                            if (ps.stop) {
+                                // every later launch found the stop flag; their
+                                // profile records are dropped (the records still
+                                // pending are this segment's and non-PCG ones)
+                                drop_after(ctx, 0, ps.iter);
+                                harvest_upto(ctx, ctx.pending.size());
+                                break;
+                            }
+                            const size_t done = harvest_upto(ctx, t.pending_end);
+                            for (int i = 0; i < n_inflight; ++i) inflight[i].pending_end -= done;
+                            if (ps.iter <= seen) throw Fail{VBA_ERR_DEVICE, "PCG made no progress"};
+                            seen = ps.iter;
+                            if (next < max_pcg) enqueue(std::min(next + (int32_t)T.pcg_chunk, max_pcg));
+                        }
                         k = ps.iter;
                         fresh = 0;
                     }
@@ -764,6 +827,7 @@ int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stat
                 st.pcg_residual_norm = ps.true_norm;
                 st.pcg_target = ps.target;
                 total_pcg += (uint32_t)ps.iter;
+                pr.last_pcg = ps.iter;
             }
             const double cost_before = current_cost;
             if (!solved) {  // lm.rs:610-662: a failed solve is a rejected step

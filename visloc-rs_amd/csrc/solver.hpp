@@ -57,6 +57,13 @@ struct Context {
     std::vector<hipEvent_t> event_pool;
     // device buffers of the reduced-camera solve (reduced.hip), kept between calls
     ReducedState* reduced = nullptr;
+    // PCG launch schedule (vba_tuning) and the pinned ring its chunk reads
+    // land in, one event per slot
+    vba_tuning tuning{10, 0, 0, 0};
+    static constexpr int kPcgRing = 4;
+    PcgState* host_pcg_ring = nullptr;
+    hipEvent_t pcg_ev[kPcgRing] = {};
+    uint32_t pcg_ring_next = 0;
 };
 
 // One problem laid out on the device: structure, initial state and working
@@ -74,6 +81,7 @@ struct Prepared {
     std::vector<int32_t> x_row_ptr, x_col, x_block_pos, x_diag_pos;
     int final_slot = -1;  // buffer holding the final poses (-1: none yet)
     int final_lm_slot = -1;
+    int32_t last_pcg = 0;  // iterations of the last PCG solve (the schedule's prediction)
 };
 
 // Matrix-free entry validation (optimize.rs:382-626) on a built model.

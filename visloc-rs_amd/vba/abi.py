@@ -209,6 +209,15 @@ class ServerConfig(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    """vba_tuning: the PCG launch schedule (results do not depend on it)."""
+    _fields_ = [("pcg_chunk", C.c_uint32), ("pcg_lookahead", C.c_uint32), ("pcg_predict", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+    def as_dict(self):
+        return {"pcg_chunk": self.pcg_chunk, "pcg_lookahead": self.pcg_lookahead, "pcg_predict": self.pcg_predict}
+
+
 class ReducedSystem(C.Structure):
     _fields_ = [
         ("structure_id", C.c_uint64),
@@ -272,6 +281,10 @@ SIGNATURES = {
     "vba_server_counters": (None, [C.c_void_p, PU64, PU64, PU64, PU64]),
     "vba_server_destroy": (None, [C.c_void_p]),
     "vba_reduced_solve": (C.c_int32, [C.c_void_p, C.POINTER(ReducedSystem), P64, C.POINTER(C.c_uint32)]),
+    "vba_set_tuning": (C.c_int32, [C.c_void_p, C.POINTER(Tuning)]),
+    "vba_get_tuning": (C.c_int32, [C.c_void_p, C.POINTER(Tuning)]),
+    "vba_autotune": (C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(Options), C.c_uint32, C.POINTER(Tuning),
+                                 C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     # vba_io.h: problems on disk
     "vba_io_last_error": (C.c_char_p, []),
     "vba_model_read_ba_problem": (C.c_int32, [C.c_char_p, C.POINTER(C.c_void_p)]),

@@ -269,6 +269,18 @@ class Device:
             self.lib.vba_profile_reset(self.ctx)
         self.lib.vba_profile_enable(self.ctx, max(1, int(stride)) if enable else 0)
 
+    def set_tuning(self, pcg_chunk: int = 10, pcg_lookahead: int = 0, pcg_predict: bool = False) -> None:
+        """The PCG launch schedule of this context (vba_set_tuning)."""
+        t = abi.Tuning(int(pcg_chunk), int(pcg_lookahead), int(bool(pcg_predict)), 0)
+        st = self.lib.vba_set_tuning(self.ctx, C.byref(t))
+        if st != abi.OK:
+            raise ValueError(self.last_error())
+
+    def tuning(self) -> dict:
+        t = abi.Tuning()
+        self.lib.vba_get_tuning(self.ctx, C.byref(t))
+        return t.as_dict()
+
     def profile_query(self, kernel: str) -> dict:
         launches = C.c_uint64()
         secs = C.c_double()
@@ -411,6 +423,18 @@ class PreparedProblem:
         if st != abi.OK:
             out.message = self.device.last_error()
         return out
+
+    def autotune(self, options: abi.Options, lm_iterations: int = 3) -> dict:
+        """Pick the fastest PCG launch schedule for this problem
+        (vba_autotune); it is left set on the device context."""
+        t = abi.Tuning()
+        best = C.c_double()
+        default = C.c_double()
+        st = self.device.lib.vba_autotune(self.device.ctx, self.handle, C.byref(options), int(lm_iterations),
+                                          C.byref(t), C.byref(best), C.byref(default))
+        if st != abi.OK:
+            raise RuntimeError(self.device.last_error())
+        return {"tuning": t.as_dict(), "seconds": best.value, "default_seconds": default.value}
 
     def state(self):
         poses = np.empty_like(self.problem.poses)
