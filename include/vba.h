@@ -52,6 +52,9 @@ enum vba_status {
     VBA_ERR_MISSING_LANDMARK = -6,         /* BaError::MissingLandmark(id) */
     VBA_ERR_UNSUPPORTED_CAMERA_MODEL = -7, /* BaError::UnsupportedCameraModel */
     VBA_ERR_MISSING_STEREO_BASELINE = -8,  /* BaError::MissingStereoBaseline */
+    VBA_ERR_OBSERVATION_WEIGHT_COUNT = -9, /* BaError::ObservationWeightCount */
+    VBA_ERR_INVALID_OBSERVATION_WEIGHT = -10, /* BaError::InvalidObservationWeight(index in error_id) */
+    VBA_ERR_WEIGHTS_WITH_INTRINSICS = -12, /* BaError::ObservationWeightsWithIntrinsicsRefinement */
     VBA_ERR_SINGULAR_SYSTEM = -13,         /* BaError::SingularSystem */
     /* Matrix-free entry errors (MatrixFreeBaError, mod.rs:893-910). */
     VBA_ERR_INELIGIBLE = -20,              /* MatrixFreeBaError::Ineligible */
@@ -263,6 +266,63 @@ int32_t vba_optimize(vba_context* ctx, const vba_problem* problem,
                      const vba_options* options, double* poses_out,
                      double* landmarks_out, vba_iteration_stats* stats,
                      uint32_t stats_capacity, vba_result* result);
+
+/* BundleAdjustment::optimize_with_observation_weights (optimize.rs:43-53):
+ * `weights` holds num_weights values, one per observation in the reference's
+ * flattened order (the monocular observations, then the rig observations).
+ * Each multiplies its observation's robust-kernel weight in the normal
+ * equations (normal_equations.rs:458, 627) and its cost term
+ * (problem.rs:521-523, 583). Checked first, as validate_observation_weights
+ * does (optimize.rs:628-650): VBA_ERR_OBSERVATION_WEIGHT_COUNT, then
+ * VBA_ERR_INVALID_OBSERVATION_WEIGHT for the first negative or non-finite
+ * weight (its index in result.error_id), then
+ * VBA_ERR_WEIGHTS_WITH_INTRINSICS. The reference solves weighted problems
+ * with its normal-equation path (the matrix-free entry does not take weights,
+ * optimize.rs:87-103); here every linear solver takes them. */
+int32_t vba_optimize_weighted(vba_context* ctx, const vba_problem* problem, const vba_options* options,
+                              const double* weights, uint64_t num_weights, double* poses_out,
+                              double* landmarks_out, vba_iteration_stats* stats, uint32_t stats_capacity,
+                              vba_result* result);
+
+/* Graduated non-convexity, BundleAdjustment::optimize_gnc (lm.rs:28-142;
+ * pipelines/slam/src/gnc.rs): weighted LM solves with no M-estimator,
+ * re-weighted between them by the surrogate's Black–Rangarajan weights while
+ * the control parameter anneals. Fields mirror `GncConfig` (gnc.rs:52-100;
+ * defaults gnc.rs:103-115 via vba_default_gnc_options). */
+enum vba_gnc_kernel {
+    VBA_GNC_GEMAN_MCCLURE = 0,
+    VBA_GNC_TRUNCATED_LEAST_SQUARES = 1
+};
+typedef struct vba_gnc_options {
+    uint32_t kernel;             /* enum vba_gnc_kernel */
+    uint32_t max_outer;
+    uint32_t inner_iterations;
+    uint32_t has_auto_scale;     /* auto_scale: Some(k) when nonzero */
+    double c;
+    double anneal_factor;
+    double auto_scale;           /* k of median + k 1.4826 MAD */
+    uint32_t auto_scale_readapt;
+    uint32_t reserved;
+} vba_gnc_options;
+/* BaGncResult (bundle/mod.rs:973-1010). */
+typedef struct vba_gnc_result {
+    double initial_cost;
+    double final_cost;
+    double inlier_cost;
+    double inlier_scale;
+    uint64_t observation_count;
+    uint32_t outer_iterations;
+    uint32_t converged;
+} vba_gnc_result;
+void vba_default_gnc_options(vba_gnc_options* gnc);
+/* `options` is the inner LM's BaConfig (its kernel and iteration count are
+ * replaced as the reference does). On VBA_OK the final state is written as by
+ * vba_optimize, and weights_out (nullable) receives the final weights, one per
+ * observation in the flattened order, NaN where an observation cannot be
+ * evaluated. Errors of an inner solve are returned as they are. */
+int32_t vba_optimize_gnc(vba_context* ctx, const vba_problem* problem, const vba_options* options,
+                         const vba_gnc_options* gnc, double* poses_out, double* landmarks_out,
+                         double* weights_out, vba_gnc_result* result);
 
 /* ---- device-resident problems ------------------------------------------
  * No counterpart in the reference, whose GPU backend uploads on every call

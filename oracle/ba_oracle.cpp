@@ -563,6 +563,7 @@ struct Obs {
     uint32_t lm;    // index into landmarks (id order)
     double x, y;
     int32_t sensor;  // -1 = monocular observation with the shared camera
+    double weight = 1.0;  // external observation weight (gnc_weights[obs_idx]), 1 when none
 };
 struct Problem {
     Cam camera;
@@ -775,7 +776,7 @@ void cost_and_nonprojectable(const Problem& P, const Kernel& K, double* cost, ui
             }
             double rx = uv[0] - o.x, ry = uv[1] - o.y;
             double s = rx * rx + ry * ry;
-            total += 1.0 * K.cost(s);
+            total += o.weight * K.cost(s);
         } else {
             Lin lin;
             if (!linearize_rig(P, o, &lin)) {
@@ -783,7 +784,7 @@ void cost_and_nonprojectable(const Problem& P, const Kernel& K, double* cost, ui
                 continue;
             }
             double s = lin.r[0] * lin.r[0] + lin.r[1] * lin.r[1];
-            total += 1.0 * K.cost(s);
+            total += o.weight * K.cost(s);
         }
     }
     *cost = total;
@@ -830,7 +831,7 @@ void compute_contribution(const Problem& P, const Intr& in, const Proj& pr, cons
     c->valid = linearize_obs(P, in, pr, o, &lin);
     if (!c->valid) return;
     double s = lin.r[0] * lin.r[0] + lin.r[1] * lin.r[1];
-    double w = K.weight(s) * 1.0;
+    double w = K.weight(s) * o.weight;
     c->p = P.pose_var[o.pose];
     c->l = P.lm_var[o.lm];
     if (c->p >= 0) {
@@ -1603,11 +1604,131 @@ void normal_from_view(const oracle_normal_system* v, Normal* ne) {
     }
 }
 
+// reprojection_squared_residuals (problem.rs:756-...): per observation in
+// the flattened order, NaN when it cannot be evaluated (no intrinsics, behind
+// the camera, or no projection). Monocular observations use project_pinhole
+// for a distortion-free camera and Camera::project otherwise.
+std::vector<double> reprojection_squared_residuals(const Problem& P) {
+    std::vector<double> out(P.obs.size(), std::numeric_limits<double>::quiet_NaN());
+    Intr in;
+    if (!intrinsics(P.camera, &in)) return out;
+    const Proj pr = mono_projection_for(P.camera);
+    for (size_t i = 0; i < P.obs.size(); ++i) {
+        const Obs& o = P.obs[i];
+        if (o.sensor < 0) {
+            const V3 xc = se3_apply(P.poses[o.pose], P.lms[o.lm]);
+            if (xc.z <= 0.0) continue;
+            double uv[2];
+            const bool ok = pr.kind == PROJ_PINHOLE ? project_pinhole(in, xc, uv) : camera_project(P.camera, xc, uv);
+            if (!ok) continue;
+            const double rx = uv[0] - o.x, ry = uv[1] - o.y;
+            out[i] = rx * rx + ry * ry;
+        } else {
+            Lin lin;
+            if (!linearize_rig(P, o, &lin)) continue;
+            out[i] = lin.r[0] * lin.r[0] + lin.r[1] * lin.r[1];
+        }
+    }
+    return out;
+}
+
+// GncState (gnc.rs:118-227).
+constexpr double kTlsMuTerminal = 200.0;
+struct GncState {
+    uint32_t kernel;
+    double c2, factor, mu;
+    GncState(const vba_gnc_options& g, double max_residual_sq) : kernel(g.kernel), c2(g.c * g.c),
+                                                                  factor(g.anneal_factor) {
+        const double s_max = std::max(max_residual_sq, 0.0);
+        if (kernel == VBA_GNC_GEMAN_MCCLURE) {
+            mu = std::max(2.0 * s_max / c2, 1.0);
+        } else {
+            const double denom = 2.0 * s_max - c2;
+            mu = denom <= 0.0 ? kTlsMuTerminal
+                              : std::min(std::max(c2 / denom, std::numeric_limits<double>::min()), kTlsMuTerminal);
+        }
+    }
+    void set_inlier_scale(double c) { c2 = c * c; }
+    double weight(double s) const {
+        s = std::max(s, 0.0);
+        if (kernel == VBA_GNC_GEMAN_MCCLURE) {
+            const double mc2 = mu * c2;
+            const double r = mc2 / (s + mc2);
+            return r * r;
+        }
+        const double lo = (mu / (mu + 1.0)) * c2;
+        const double hi = ((mu + 1.0) / mu) * c2;
+        if (s <= lo) return 1.0;
+        if (s >= hi) return 0.0;
+        return std::sqrt(c2 * mu * (mu + 1.0) / s) - mu;
+    }
+    bool is_terminal() const { return kernel == VBA_GNC_GEMAN_MCCLURE ? mu <= 1.0 : mu >= kTlsMuTerminal; }
+    bool anneal() {
+        if (kernel == VBA_GNC_GEMAN_MCCLURE) mu = std::max(mu / factor, 1.0);
+        else mu = std::min(mu * factor, kTlsMuTerminal);
+        return is_terminal();
+    }
+};
+
+// median_in_place / estimate_scale_mad (gnc.rs:252-286).
+double median_in_place(std::vector<double>& v) {
+    std::sort(v.begin(), v.end(), [](double a, double b) {
+        return a < b || (a == b && std::signbit(a) && !std::signbit(b));  // f64::total_cmp on non-NaN values
+    });
+    const size_t n = v.size();
+    return n % 2 == 1 ? v[n / 2] : 0.5 * (v[n / 2 - 1] + v[n / 2]);
+}
+bool estimate_scale_mad(const std::vector<double>& squared, double k, double* out) {
+    std::vector<double> norms;
+    for (double s : squared)
+        if (std::isfinite(s) && s >= 0.0) norms.push_back(std::sqrt(s));
+    if (norms.size() < 2) return false;
+    const double median = median_in_place(norms);
+    std::vector<double> dev;
+    dev.reserve(norms.size());
+    for (double r : norms) dev.push_back(std::fabs(r - median));
+    const double mad = median_in_place(dev);
+    *out = median + k * (1.4826 * mad);
+    return true;
+}
+
 }  // namespace
 
 // ===========================================================================
 // C API
 // ===========================================================================
+extern "C" {
+
+}  // extern "C"
+
+namespace {
+
+void write_state(const vba_problem* problem, const Problem& P, double* poses_out, double* landmarks_out) {
+    for (uint64_t i = 0; i < problem->num_poses; ++i)
+        if (poses_out) pose_to7(P.poses[P.pose_slot_of_input[i]], poses_out + 7 * i);
+    for (uint64_t i = 0; i < problem->num_landmarks; ++i) {
+        if (!landmarks_out) break;
+        const V3& X = P.lms[P.lm_slot_of_input[i]];
+        landmarks_out[3 * i] = X.x;
+        landmarks_out[3 * i + 1] = X.y;
+        landmarks_out[3 * i + 2] = X.z;
+    }
+}
+
+// optimize / optimize_weighted (lm.rs:153-160) on a loaded problem whose
+// observations carry their weights.
+int32_t optimize_loaded(Problem& P, const vba_options& o, int32_t threads, vba_iteration_stats* stats,
+                        uint32_t cap, vba_result* result) {
+    const bool matrix_free = o.linear_solver == VBA_SOLVER_MATRIX_FREE || o.linear_solver == VBA_SOLVER_PCG_EXPLICIT;
+    if (matrix_free) {
+        int32_t st = validate_matrix_free(P, o, Kernel{o.robust_kernel, o.robust_scale});
+        if (st != VBA_OK) return st;
+    }
+    return run_lm(P, o, threads, stats, cap, result);
+}
+
+}  // namespace
+
 extern "C" {
 
 int32_t oracle_optimize(const vba_problem* problem, const vba_options* options, double* poses_out,
@@ -1618,27 +1739,145 @@ int32_t oracle_optimize(const vba_problem* problem, const vba_options* options, 
     if (problem->declined_features) return VBA_DECLINED;
     Problem P;
     uint64_t error_id = 0;
-    const bool matrix_free =
-        options->linear_solver == VBA_SOLVER_MATRIX_FREE || options->linear_solver == VBA_SOLVER_PCG_EXPLICIT;
     int32_t st = load_problem(problem, &P, &error_id);
     result->error_id = error_id;
     if (st != VBA_OK) return st;
-    if (matrix_free) {
-        st = validate_matrix_free(P, *options, Kernel{options->robust_kernel, options->robust_scale});
-        if (st != VBA_OK) return st;
-    }
-    st = run_lm(P, *options, threads, stats, stats_capacity, result);
+    st = optimize_loaded(P, *options, threads, stats, stats_capacity, result);
     if (st != VBA_OK) return st;
-    for (uint64_t i = 0; i < problem->num_poses; ++i)
-        if (poses_out) pose_to7(P.poses[P.pose_slot_of_input[i]], poses_out + 7 * i);
-    for (uint64_t i = 0; i < problem->num_landmarks; ++i) {
-        if (!landmarks_out) break;
-        const V3& X = P.lms[P.lm_slot_of_input[i]];
-        landmarks_out[3 * i] = X.x;
-        landmarks_out[3 * i + 1] = X.y;
-        landmarks_out[3 * i + 2] = X.z;
-    }
+    write_state(problem, P, poses_out, landmarks_out);
     return VBA_OK;
+}
+
+int32_t oracle_optimize_weighted(const vba_problem* problem, const vba_options* options, const double* weights,
+                                 uint64_t num_weights, double* poses_out, double* landmarks_out,
+                                 vba_iteration_stats* stats, uint32_t stats_capacity, vba_result* result,
+                                 int32_t threads) {
+    if (!problem || !options || !result) return VBA_ERR_INVALID_ARGUMENT;
+    std::memset(result, 0, sizeof(*result));
+    if (problem->declined_features & ~(uint32_t)VBA_FEATURE_OBSERVATION_WEIGHTS) return VBA_DECLINED;
+    // validate_observation_weights (optimize.rs:628-650), then the intrinsics
+    // refusal (:49-51)
+    const uint64_t expected = problem->num_observations + problem->num_rig_observations;
+    if (num_weights != expected || (num_weights && !weights)) return VBA_ERR_OBSERVATION_WEIGHT_COUNT;
+    for (uint64_t i = 0; i < num_weights; ++i)
+        if (!std::isfinite(weights[i]) || weights[i] < 0.0) {
+            result->error_id = i;
+            return VBA_ERR_INVALID_OBSERVATION_WEIGHT;
+        }
+    if (options->refine_intrinsics) return VBA_ERR_WEIGHTS_WITH_INTRINSICS;
+    Problem P;
+    uint64_t error_id = 0;
+    int32_t st = load_problem(problem, &P, &error_id);
+    result->error_id = error_id;
+    if (st != VBA_OK) return st;
+    for (size_t i = 0; i < P.obs.size(); ++i) P.obs[i].weight = weights[i];
+    st = optimize_loaded(P, *options, threads, stats, stats_capacity, result);
+    if (st != VBA_OK) return st;
+    write_state(problem, P, poses_out, landmarks_out);
+    return VBA_OK;
+}
+
+void oracle_default_gnc_options(vba_gnc_options* g) {
+    std::memset(g, 0, sizeof(*g));
+    g->kernel = VBA_GNC_GEMAN_MCCLURE;
+    g->c = 1.0;
+    g->anneal_factor = 1.4;
+    g->max_outer = 100;
+    g->inner_iterations = 5;
+}
+
+// BundleAdjustment::optimize_gnc (lm.rs:28-142).
+int32_t oracle_optimize_gnc(const vba_problem* problem, const vba_options* options, const vba_gnc_options* gnc,
+                            double* poses_out, double* landmarks_out, double* weights_out, vba_gnc_result* res,
+                            int32_t threads) {
+    if (!problem || !options || !gnc || !res) return VBA_ERR_INVALID_ARGUMENT;
+    std::memset(res, 0, sizeof(*res));
+    if (problem->declined_features) return VBA_DECLINED;
+    Problem P;
+    uint64_t error_id = 0;
+    int32_t st = load_problem(problem, &P, &error_id);
+    if (st != VBA_OK) return st;
+    const Kernel none{VBA_KERNEL_NONE, 0.0};
+    uint64_t nonproj = 0;
+    double initial_cost = 0.0;
+    cost_and_nonprojectable(P, none, &initial_cost, &nonproj);
+    const size_t n = P.obs.size();
+    std::vector<double> sq = reprojection_squared_residuals(P);
+    double s_max = 0.0;
+    for (double s : sq)
+        if (std::isfinite(s)) s_max = std::max(s_max, s);
+    vba_gnc_options eff = *gnc;
+    if (gnc->has_auto_scale) {
+        double est;
+        if (estimate_scale_mad(sq, gnc->auto_scale, &est)) eff.c = std::max(est, gnc->c);
+    }
+    double inlier_scale = eff.c;
+    GncState state(eff, s_max);
+    vba_options inner = *options;
+    inner.robust_kernel = VBA_KERNEL_NONE;
+    inner.max_iterations = std::max<uint32_t>(gnc->inner_iterations, 1);
+    std::vector<double> w(n, 1.0);
+    bool converged = false;
+    uint32_t outer = 0;
+    for (uint32_t it = 0; it < std::max<uint32_t>(gnc->max_outer, 1); ++it) {
+        ++outer;
+        const bool terminal = state.is_terminal();
+        const std::vector<double> r = reprojection_squared_residuals(P);
+        if (gnc->auto_scale_readapt && gnc->has_auto_scale) {
+            double est;
+            if (estimate_scale_mad(r, gnc->auto_scale, &est)) {
+                const double c = std::max(est, gnc->c);
+                state.set_inlier_scale(c);
+                inlier_scale = c;
+            }
+        }
+        for (size_t i = 0; i < n; ++i) {
+            w[i] = std::isfinite(r[i]) ? state.weight(r[i]) : 1.0;
+            P.obs[i].weight = w[i];
+        }
+        vba_result inner_res{};
+        st = optimize_loaded(P, inner, threads, nullptr, 0, &inner_res);
+        if (st != VBA_OK) return st;
+        if (terminal) {
+            converged = true;
+            break;
+        }
+        state.anneal();
+    }
+    const std::vector<double> r = reprojection_squared_residuals(P);
+    for (size_t i = 0; i < n; ++i) {
+        w[i] = std::isfinite(r[i]) ? state.weight(r[i]) : std::numeric_limits<double>::quiet_NaN();
+        P.obs[i].weight = w[i];
+    }
+    double final_cost = 0.0;
+    cost_and_nonprojectable(P, none, &final_cost, &nonproj);
+    for (size_t i = 0; i < n; ++i)
+        P.obs[i].weight = std::isfinite(w[i]) ? (w[i] >= 0.5 ? 1.0 : 0.0) : std::numeric_limits<double>::quiet_NaN();
+    double inlier_cost = 0.0;
+    cost_and_nonprojectable(P, none, &inlier_cost, &nonproj);
+    res->initial_cost = initial_cost;
+    res->final_cost = final_cost;
+    res->inlier_cost = inlier_cost;
+    res->inlier_scale = inlier_scale;
+    res->observation_count = n;
+    res->outer_iterations = outer;
+    res->converged = converged ? 1 : 0;
+    write_state(problem, P, poses_out, landmarks_out);
+    if (weights_out) std::copy(w.begin(), w.end(), weights_out);
+    return VBA_OK;
+}
+
+int32_t oracle_gnc_state_weights(const vba_gnc_options* gnc, double max_residual_sq, uint32_t anneal_steps,
+                                 const double* s, uint32_t n, double* weights, double* mu) {
+    GncState st(*gnc, max_residual_sq);
+    for (uint32_t i = 0; i < anneal_steps; ++i) st.anneal();
+    for (uint32_t i = 0; i < n; ++i) weights[i] = st.weight(s[i]);
+    if (mu) *mu = st.mu;
+    return st.is_terminal() ? 1 : 0;
+}
+
+int32_t oracle_estimate_scale_mad(const double* squared, uint32_t n, double k, double* out) {
+    return estimate_scale_mad(std::vector<double>(squared, squared + n), k, out) ? 1 : 0;
 }
 
 int32_t oracle_camera_project(const vba_camera* camera, const double xc[3], double uv[2]) {

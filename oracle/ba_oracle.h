@@ -29,6 +29,32 @@ int32_t oracle_optimize(const vba_problem* problem, const vba_options* options,
                         vba_iteration_stats* stats, uint32_t stats_capacity,
                         vba_result* result, int32_t threads);
 
+/* BundleAdjustment::optimize_with_observation_weights (optimize.rs:43-53,
+ * 628-650): the external weight multiplies each observation's robust weight
+ * (normal_equations.rs:458, 627) and its cost term (problem.rs:521-523, 583).
+ * Same contract as vba_optimize_weighted. */
+int32_t oracle_optimize_weighted(const vba_problem* problem, const vba_options* options,
+                                 const double* weights, uint64_t num_weights, double* poses_out,
+                                 double* landmarks_out, vba_iteration_stats* stats,
+                                 uint32_t stats_capacity, vba_result* result, int32_t threads);
+
+/* GncConfig::default (gnc.rs:103-115). */
+void oracle_default_gnc_options(vba_gnc_options* gnc);
+
+/* BundleAdjustment::optimize_gnc (lm.rs:28-142) with GncState and
+ * estimate_scale_mad (gnc.rs:118-286). Same contract as vba_optimize_gnc. */
+int32_t oracle_optimize_gnc(const vba_problem* problem, const vba_options* options,
+                            const vba_gnc_options* gnc, double* poses_out, double* landmarks_out,
+                            double* weights_out, vba_gnc_result* result, int32_t threads);
+
+/* GncState::new(gnc, max_residual_sq), annealed anneal_steps times, then
+ * weight(s[i]) for each i; returns is_terminal() and the final mu. */
+int32_t oracle_gnc_state_weights(const vba_gnc_options* gnc, double max_residual_sq, uint32_t anneal_steps,
+                                 const double* s, uint32_t n, double* weights, double* mu);
+
+/* estimate_scale_mad (gnc.rs:252-273): 1 and *out, or 0 for None. */
+int32_t oracle_estimate_scale_mad(const double* squared, uint32_t n, double k, double* out);
+
 /* Camera::project (crates/core/src/types/camera.rs:351-433). Returns 1 and
  * writes uv, or 0 for None. */
 int32_t oracle_camera_project(const vba_camera* camera, const double xc[3], double uv[2]);

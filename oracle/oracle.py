@@ -36,6 +36,20 @@ def library():
     lib.oracle_optimize.argtypes = [C.POINTER(abi.Problem), C.POINTER(abi.Options), P64, P64,
                                     C.POINTER(abi.IterationStats), C.c_uint32, C.POINTER(abi.Result),
                                     C.c_int32]
+    lib.oracle_optimize_weighted.restype = C.c_int32
+    lib.oracle_optimize_weighted.argtypes = [C.POINTER(abi.Problem), C.POINTER(abi.Options), P64, C.c_uint64, P64,
+                                             P64, C.POINTER(abi.IterationStats), C.c_uint32, C.POINTER(abi.Result),
+                                             C.c_int32]
+    lib.oracle_default_gnc_options.restype = None
+    lib.oracle_default_gnc_options.argtypes = [C.POINTER(abi.GncOptions)]
+    lib.oracle_optimize_gnc.restype = C.c_int32
+    lib.oracle_optimize_gnc.argtypes = [C.POINTER(abi.Problem), C.POINTER(abi.Options), C.POINTER(abi.GncOptions),
+                                        P64, P64, P64, C.POINTER(abi.GncResult), C.c_int32]
+    lib.oracle_gnc_state_weights.restype = C.c_int32
+    lib.oracle_gnc_state_weights.argtypes = [C.POINTER(abi.GncOptions), C.c_double, C.c_uint32, P64, C.c_uint32,
+                                             P64, P64]
+    lib.oracle_estimate_scale_mad.restype = C.c_int32
+    lib.oracle_estimate_scale_mad.argtypes = [P64, C.c_uint32, C.c_double, P64]
     lib.oracle_camera_project.restype = C.c_int32
     lib.oracle_camera_project.argtypes = [C.POINTER(abi.Camera), P64, P64]
     lib.oracle_mono_project.restype = C.c_int32
@@ -98,6 +112,62 @@ def optimize(problem, options, threads: int = 1, cap: int = 1024):
                                    threads)
 
     return vba.run_solver(call, problem, options, cap)
+
+
+def optimize_weighted(problem, options, weights, threads: int = 1, cap: int = 1024):
+    """oracle_optimize_weighted; `weights` one per observation (mono, then rig)."""
+    import vba
+
+    lib = library()
+    w = np.ascontiguousarray(weights, np.float64)
+
+    def call(view, opts, poses, lms, stats, cap_, res):
+        return lib.oracle_optimize_weighted(C.byref(view), C.byref(opts), _p(w), len(w), _p(poses), _p(lms), stats,
+                                            cap_, C.byref(res), threads)
+
+    return vba.run_solver(call, problem, options, cap)
+
+
+def default_gnc_options(**overrides):
+    import vba.abi as abi
+
+    g = abi.GncOptions()
+    library().oracle_default_gnc_options(C.byref(g))
+    for k, v in overrides.items():
+        setattr(g, k, v)
+    return g
+
+
+def optimize_gnc(problem, options, gnc, threads: int = 1):
+    """oracle_optimize_gnc: (status, GncResult, poses, landmarks, weights)."""
+    import vba.abi as abi
+
+    lib = library()
+    view = problem.view()
+    poses = np.empty_like(problem.poses)
+    lms = np.empty_like(problem.landmarks)
+    w = np.empty(problem.num_observations)
+    res = abi.GncResult()
+    st = lib.oracle_optimize_gnc(C.byref(view), C.byref(options), C.byref(gnc), _p(poses), _p(lms), _p(w),
+                                 C.byref(res), threads)
+    return st, res, poses, lms, w
+
+
+def gnc_state_weights(gnc, max_residual_sq, anneal_steps, s):
+    """GncState::new + anneal() x anneal_steps, then weight(s): (weights, mu, terminal)."""
+    s = np.ascontiguousarray(s, np.float64)
+    w = np.empty_like(s)
+    mu = C.c_double()
+    t = library().oracle_gnc_state_weights(C.byref(gnc), max_residual_sq, anneal_steps, _p(s), len(s), _p(w),
+                                           C.byref(mu))
+    return w, mu.value, bool(t)
+
+
+def estimate_scale_mad(squared, k):
+    s = np.ascontiguousarray(squared, np.float64)
+    out = C.c_double()
+    ok = library().oracle_estimate_scale_mad(_p(s), len(s), k, C.byref(out))
+    return out.value if ok else None
 
 
 def camera_project(cam, xc):

@@ -119,3 +119,25 @@ def over_determined_bundle(inject_outlier: bool):
                 uv = uv + 50.0
             obs.append((kid, lid, uv))
     return make_problem(pinhole_camera(), truth, lms, obs, fixed_poses=(10, 20, 30, 40))
+
+
+def gnc_bundle(outliers=(), shift=0.0):
+    """build_with_outliers (pipelines/slam/tests/gnc_robust_ba.rs:18-106): five
+    yawing keyframes 10..14 at truth, a 3x4 landmark grid 4.5-7 m ahead seen
+    by every keyframe, the end poses fixed; observations whose row-major
+    insertion index is in `outliers` are shifted by (+shift, -shift) px."""
+    truth = [(10 + i, pose_at([0.3 * i, 0.0, 0.04 * i], 0.015 * i)) for i in range(5)]
+    lms = []
+    for r in range(3):
+        for c in range(4):
+            lms.append((r * 4 + c + 1, np.array([-1.2 + 0.8 * c, -0.8 + 0.8 * r, 4.5 + 0.4 * (r + c)])))
+    obs = []
+    idx = 0
+    for kid, pose in truth:
+        for lid, X in lms:
+            uv = project_pinhole((500.0, 500.0), (320.0, 240.0), transform(pose, X))
+            if idx in outliers:
+                uv = uv + np.array([shift, -shift])
+            obs.append((kid, lid, uv))
+            idx += 1
+    return make_problem(pinhole_camera(), truth, lms, obs, fixed_poses=(10, 14)), truth

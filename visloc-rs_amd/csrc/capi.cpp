@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -15,6 +16,7 @@
 #include "capi_types.hpp"
 #include "reduced.hpp"
 #include "solver.hpp"
+#include "gnc.hpp"
 
 
 
@@ -163,6 +165,95 @@ int32_t vba_optimize(vba_context* ctx, const vba_problem* problem, const vba_opt
     if (st == VBA_OK) {
         try {
             st = vba::run(*ctx, prep, *options, stats, stats_capacity, result);
+        } catch (const std::bad_alloc&) {
+            ctx->last_error = "out of host memory";
+            st = VBA_ERR_OUT_OF_MEMORY;
+        }
+    }
+    if (st == VBA_OK) st = vba::download_state(*ctx, prep, poses_out, landmarks_out);
+    vba::release(prep);
+    return st;
+}
+
+int32_t vba_optimize_weighted(vba_context* ctx, const vba_problem* problem, const vba_options* options,
+                              const double* weights, uint64_t num_weights, double* poses_out, double* landmarks_out,
+                              vba_iteration_stats* stats, uint32_t stats_capacity, vba_result* result) {
+    if (!ctx || !problem || !options || !result) {
+        vba::set_thread_error("vba_optimize_weighted: null argument");
+        if (ctx) ctx->last_error = "vba_optimize_weighted: null argument";
+        return VBA_ERR_INVALID_ARGUMENT;
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    std::memset(result, 0, sizeof(*result));
+    ctx->last_error.clear();
+    // validate_observation_weights (optimize.rs:628-650), then the intrinsics
+    // refusal (:49-51)
+    const uint64_t expected = problem->num_observations + problem->num_rig_observations;
+    if (num_weights != expected || (num_weights && !weights)) {
+        ctx->last_error = "observation weight count " + std::to_string(num_weights) + ", expected " +
+                          std::to_string(expected);
+        return VBA_ERR_OBSERVATION_WEIGHT_COUNT;
+    }
+    for (uint64_t i = 0; i < num_weights; ++i)
+        if (!std::isfinite(weights[i]) || weights[i] < 0.0) {
+            result->error_id = i;
+            ctx->last_error = "observation weight " + std::to_string(i) + " is negative or not finite";
+            return VBA_ERR_INVALID_OBSERVATION_WEIGHT;
+        }
+    if (options->refine_intrinsics) {
+        ctx->last_error = "observation weights with intrinsics refinement";
+        return VBA_ERR_WEIGHTS_WITH_INTRINSICS;
+    }
+    vba_problem view = *problem;
+    view.declined_features &= ~(uint32_t)VBA_FEATURE_OBSERVATION_WEIGHTS;
+    vba_prepared prep;
+    int32_t st = build_prepared(ctx, view, *options, &prep, result);
+    if (st == VBA_OK) st = vba::set_weights(*ctx, prep, weights);
+    if (st == VBA_OK) {
+        try {
+            st = vba::run(*ctx, prep, *options, stats, stats_capacity, result);
+        } catch (const std::bad_alloc&) {
+            ctx->last_error = "out of host memory";
+            st = VBA_ERR_OUT_OF_MEMORY;
+        }
+    }
+    if (st == VBA_OK) st = vba::download_state(*ctx, prep, poses_out, landmarks_out);
+    vba::release(prep);
+    return st;
+}
+
+void vba_default_gnc_options(vba_gnc_options* gnc) {
+    if (!gnc) return;
+    std::memset(gnc, 0, sizeof(*gnc));
+    gnc->kernel = VBA_GNC_GEMAN_MCCLURE;  // GncConfig::default (gnc.rs:103-115)
+    gnc->c = 1.0;
+    gnc->anneal_factor = 1.4;
+    gnc->max_outer = 100;
+    gnc->inner_iterations = 5;
+}
+
+int32_t vba_optimize_gnc(vba_context* ctx, const vba_problem* problem, const vba_options* options,
+                         const vba_gnc_options* gnc, double* poses_out, double* landmarks_out, double* weights_out,
+                         vba_gnc_result* result) {
+    if (!ctx || !problem || !options || !gnc || !result) {
+        vba::set_thread_error("vba_optimize_gnc: null argument");
+        if (ctx) ctx->last_error = "vba_optimize_gnc: null argument";
+        return VBA_ERR_INVALID_ARGUMENT;
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    std::memset(result, 0, sizeof(*result));
+    ctx->last_error.clear();
+    if (!(gnc->c > 0.0) || !std::isfinite(gnc->c) || !(gnc->anneal_factor > 1.0) ||
+        !std::isfinite(gnc->anneal_factor) || gnc->kernel > VBA_GNC_TRUNCATED_LEAST_SQUARES) {
+        ctx->last_error = "vba_optimize_gnc: c must be positive and anneal_factor above 1";
+        return VBA_ERR_INVALID_ARGUMENT;
+    }
+    vba_prepared prep;
+    vba_result r{};
+    int32_t st = build_prepared(ctx, *problem, *options, &prep, &r);
+    if (st == VBA_OK) {
+        try {
+            st = vba::optimize_gnc(*ctx, prep, *options, *gnc, weights_out, result);
         } catch (const std::bad_alloc&) {
             ctx->last_error = "out of host memory";
             st = VBA_ERR_OUT_OF_MEMORY;

@@ -65,7 +65,10 @@ __global__ __launch_bounds__(kBlock) void k_linearize(DevProblem P, int store32)
             double uv[2], Jp[6];
             if (!project<double>(S.lens, xs[0], xs[1], xs[2], uv, Jp)) continue;
             const double r0 = uv[0] - P.obs_uv[2 * (size_t)o], r1 = uv[1] - P.obs_uv[2 * (size_t)o + 1];
-            const double w = robust_weight<double>((uint32_t)P.kernel_kind, P.kernel_scale, r0 * r0 + r1 * r1);
+            // the robust weight times the observation's external weight
+            // (normal_equations.rs:458, 627), 1 when there are none
+            const double w = robust_weight<double>((uint32_t)P.kernel_kind, P.kernel_scale, r0 * r0 + r1 * r1) *
+                             (P.obs_w ? P.obs_w[o] : 1.0);
             double J[6];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
@@ -271,10 +274,15 @@ __global__ __launch_bounds__(kBlock) void k_cost(DevProblem P, int trial) {
             double uv[2];
             if (!project<double>(S.lens, xs[0], xs[1], xs[2], uv, (double*)nullptr)) {
                 bad += 1.0;
+                if (P.obs_s) P.obs_s[o] = __builtin_nan("");
                 continue;
             }
             const double r0 = uv[0] - P.obs_uv[2 * (size_t)o], r1 = uv[1] - P.obs_uv[2 * (size_t)o + 1];
-            csum_add(cs, robust_cost<double>((uint32_t)P.kernel_kind, P.kernel_scale, r0 * r0 + r1 * r1));
+            const double sq = r0 * r0 + r1 * r1;
+            if (P.obs_s) P.obs_s[o] = sq;
+            const double c = robust_cost<double>((uint32_t)P.kernel_kind, P.kernel_scale, sq);
+            // w rho(s) with the observation's external weight (problem.rs:521-523, 583)
+            csum_add(cs, P.obs_w ? P.obs_w[o] * c : c);
         }
     }
     // compensated tree: wave shuffles of (s, c) pairs, then LDS in fixed order

@@ -78,6 +78,8 @@ struct DevProblem {
     const int32_t* lm_var;
     // structure
     const double* obs_uv;
+    const double* obs_w;  // external weight per observation, c-order (null: all 1)
+    double* obs_s;        // k_cost writes each observation's squared residual here (NaN: not evaluable)
     const uint16_t* obs_sensor;
     const int32_t* edge_lm;
     const int32_t* edge_obs;

@@ -115,6 +115,7 @@ struct Model {
     std::vector<int32_t> pose_run_ptr, run_slot;
     uint64_t num_runs = 0;
     std::vector<int32_t> g32_pos;  // per storage position (lane-major FP32 G)
+    std::vector<double> obs_w;     // external observation weights, c-order (empty: none)
     int32_t num_g32_chunks = 0;
 };
 

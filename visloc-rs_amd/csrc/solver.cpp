@@ -30,7 +30,7 @@ enum BufId {
     B_VQ, B_PART64, B_PART21, B_BPART, B_CPART, B_PCG, B_DELTA_L, B_TILE_SCALAR,
     B_SCALARS, B_DENSE_BLOCK_PQ, B_DENSE_BLOCK_PTR, B_DENSE_TERMS, B_DENSE_VAR_SLOT, B_DENSE_S,
     B_DENSE_WORK, B_DENSE_STATUS, B_X_BLOCK_POS, B_X_ROW_PTR, B_X_COL, B_X_DIAG_POS, B_X_S64, B_X_S32, B_VD2,
-    B_G32_POS, B_COUNT
+    B_G32_POS, B_OBS_W, B_OBS_S, B_COUNT
 };
 
 // Profiled kernel families.
@@ -413,6 +413,7 @@ int32_t prepare(Context& ctx, Prepared& pr, std::string* msg) {
         upload(pr, B_RUN_SLOT, m.run_slot, s);
         upload(pr, B_S_LM, m.s_lm, s);
         upload(pr, B_G32_POS, m.g32_pos, s);
+        if (!m.obs_w.empty()) upload(pr, B_OBS_W, m.obs_w, s);
         hip_check(hipStreamSynchronize(s), "sync");
         pr.final_slot = -1;
         return VBA_OK;
@@ -441,6 +442,63 @@ int32_t download_state(Context& ctx, const Prepared& pr, double* poses_out, doub
         if (landmarks_out)
             for (size_t i = 0; i < m.lm_slot_of_input.size(); ++i)
                 std::memcpy(landmarks_out + 3 * i, lms.data() + 3 * (size_t)m.lm_slot_of_input[i], 3 * sizeof(double));
+        return VBA_OK;
+    } catch (const Fail& f) {
+        ctx.last_error = f.msg;
+        return f.code;
+    }
+}
+
+int32_t set_weights(Context& ctx, Prepared& pr, const double* weights) {
+    Model& m = pr.m;
+    try {
+        if (!weights) {
+            m.obs_w.clear();
+            return VBA_OK;
+        }
+        m.obs_w.resize(m.num_obs);
+        for (uint64_t k = 0; k < m.num_obs; ++k) m.obs_w[k] = weights[m.obs_input[k]];
+        upload(pr, B_OBS_W, m.obs_w, ctx.stream);
+        return VBA_OK;
+    } catch (const Fail& f) {
+        ctx.last_error = f.msg;
+        return f.code;
+    }
+}
+
+int32_t adopt_final_state(Context& ctx, Prepared& pr) {
+    if (pr.final_slot < 0) return VBA_OK;
+    try {
+        const Model& m = pr.m;
+        hip_check(hipMemcpyAsync(pr.bufs[B_POSES0].ptr, pr.bufs[pr.final_slot].ptr, m.poses.size() * sizeof(double),
+                                 hipMemcpyDeviceToDevice, ctx.stream), "D2D");
+        hip_check(hipMemcpyAsync(pr.bufs[B_LMS0].ptr, pr.bufs[pr.final_lm_slot].ptr, m.lms.size() * sizeof(double),
+                                 hipMemcpyDeviceToDevice, ctx.stream), "D2D");
+        hip_check(hipStreamSynchronize(ctx.stream), "sync");
+        pr.final_slot = pr.final_lm_slot = -1;
+        return VBA_OK;
+    } catch (const Fail& f) {
+        ctx.last_error = f.msg;
+        return f.code;
+    }
+}
+
+namespace {
+DevProblem bind(const Context& ctx, Prepared& pr, const vba_options& o, bool mixed, bool matrix_free,
+                hipStream_t s);
+}  // namespace
+
+int32_t squared_residuals(Context& ctx, Prepared& pr, const vba_options& o, double* out) {
+    hipStream_t s = ctx.stream;
+    const Model& m = pr.m;
+    try {
+        DevProblem P = bind(ctx, pr, o, false, o.linear_solver != VBA_SOLVER_DENSE, s);
+        P.obs_s = (double*)ensure(pr, B_OBS_S, std::max<uint64_t>(m.num_obs, 1) * sizeof(double));
+        launch_cost(P, false, s);
+        std::vector<double> sc(m.num_obs);
+        hip_check(hipMemcpyAsync(sc.data(), P.obs_s, m.num_obs * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+        hip_check(hipStreamSynchronize(s), "sync");
+        for (uint64_t k = 0; k < m.num_obs; ++k) out[m.obs_input[k]] = sc[k];
         return VBA_OK;
     } catch (const Fail& f) {
         ctx.last_error = f.msg;
@@ -482,6 +540,8 @@ DevProblem bind(const Context& ctx, Prepared& pr, const vba_options& o, bool mix
     P.pose_var = buf<int32_t>(pr, B_POSE_VAR);
     P.lm_var = buf<int32_t>(pr, B_LM_VAR);
     P.obs_uv = buf<double>(pr, B_OBS_UV);
+    P.obs_w = m.obs_w.empty() ? nullptr : buf<double>(pr, B_OBS_W);
+    P.obs_s = nullptr;
     P.obs_sensor = m.any_rig ? buf<uint16_t>(pr, B_OBS_SENSOR) : nullptr;
     P.edge_lm = buf<int32_t>(pr, B_EDGE_LM);
     P.edge_obs = buf<int32_t>(pr, B_EDGE_OBS);

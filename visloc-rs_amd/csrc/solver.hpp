@@ -98,6 +98,18 @@ int32_t run(Context& ctx, Prepared& prep, const vba_options& o, vba_iteration_st
 // Final (or, before any solve, initial) poses / landmarks in input order.
 int32_t download_state(Context& ctx, const Prepared& prep, double* poses_out, double* landmarks_out);
 
+// External observation weights, one per observation in the flattened input
+// order (monocular, then rig), for the following solves; NULL removes them.
+int32_t set_weights(Context& ctx, Prepared& prep, const double* weights);
+
+// Make the last solve's final state the prepared initial state, so that the
+// next solve continues from it.
+int32_t adopt_final_state(Context& ctx, Prepared& prep);
+
+// Squared reprojection residual of every observation at the prepared initial
+// state, in the flattened input order; NaN where it cannot be evaluated.
+int32_t squared_residuals(Context& ctx, Prepared& prep, const vba_options& o, double* out);
+
 // Independent windows laid out as one device-resident problem with disjoint
 // slots; the LM control runs per window on the device.
 struct PreparedBatch : Prepared {

@@ -23,6 +23,9 @@ ERR_MISSING_POSE = -5
 ERR_MISSING_LANDMARK = -6
 ERR_UNSUPPORTED_CAMERA_MODEL = -7
 ERR_MISSING_STEREO_BASELINE = -8
+ERR_OBSERVATION_WEIGHT_COUNT = -9
+ERR_INVALID_OBSERVATION_WEIGHT = -10
+ERR_WEIGHTS_WITH_INTRINSICS = -12
 ERR_SINGULAR_SYSTEM = -13
 ERR_INELIGIBLE = -20
 ERR_INVALID_CONFIGURATION = -21
@@ -209,6 +212,24 @@ class ServerConfig(C.Structure):
     ]
 
 
+GNC_GEMAN_MCCLURE = 0
+GNC_TRUNCATED_LEAST_SQUARES = 1
+
+
+class GncOptions(C.Structure):
+    """vba_gnc_options (GncConfig)."""
+    _fields_ = [("kernel", C.c_uint32), ("max_outer", C.c_uint32), ("inner_iterations", C.c_uint32),
+                ("has_auto_scale", C.c_uint32), ("c", C.c_double), ("anneal_factor", C.c_double),
+                ("auto_scale", C.c_double), ("auto_scale_readapt", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class GncResult(C.Structure):
+    """vba_gnc_result (BaGncResult)."""
+    _fields_ = [("initial_cost", C.c_double), ("final_cost", C.c_double), ("inlier_cost", C.c_double),
+                ("inlier_scale", C.c_double), ("observation_count", C.c_uint64), ("outer_iterations", C.c_uint32),
+                ("converged", C.c_uint32)]
+
+
 class Tuning(C.Structure):
     """vba_tuning: the PCG launch schedule (results do not depend on it)."""
     _fields_ = [("pcg_chunk", C.c_uint32), ("pcg_lookahead", C.c_uint32), ("pcg_predict", C.c_uint32),
@@ -281,6 +302,17 @@ SIGNATURES = {
     "vba_server_counters": (None, [C.c_void_p, PU64, PU64, PU64, PU64]),
     "vba_server_destroy": (None, [C.c_void_p]),
     "vba_reduced_solve": (C.c_int32, [C.c_void_p, C.POINTER(ReducedSystem), P64, C.POINTER(C.c_uint32)]),
+    "vba_optimize_weighted": (
+        C.c_int32,
+        [C.c_void_p, C.POINTER(Problem), C.POINTER(Options), P64, C.c_uint64, P64, P64,
+         C.POINTER(IterationStats), C.c_uint32, C.POINTER(Result)],
+    ),
+    "vba_default_gnc_options": (None, [C.POINTER(GncOptions)]),
+    "vba_optimize_gnc": (
+        C.c_int32,
+        [C.c_void_p, C.POINTER(Problem), C.POINTER(Options), C.POINTER(GncOptions), P64, P64, P64,
+         C.POINTER(GncResult)],
+    ),
     "vba_set_tuning": (C.c_int32, [C.c_void_p, C.POINTER(Tuning)]),
     "vba_get_tuning": (C.c_int32, [C.c_void_p, C.POINTER(Tuning)]),
     "vba_autotune": (C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(Options), C.c_uint32, C.POINTER(Tuning),

@@ -185,6 +185,15 @@ def synthesize(kind: int, **overrides) -> Synthetic:
     return Synthetic(prob, truth_poses, truth_lms, extent, params)
 
 
+def default_gnc_options(**overrides) -> abi.GncOptions:
+    """GncConfig::default (vba_default_gnc_options) with field overrides."""
+    g = abi.GncOptions()
+    abi.library().vba_default_gnc_options(C.byref(g))
+    for k, v in overrides.items():
+        setattr(g, k, v)
+    return g
+
+
 def default_options(**overrides) -> abi.Options:
     """BaConfig::default() (+ MatrixFreeBaOptions::default()), with overrides."""
     o = abi.Options()
@@ -298,6 +307,35 @@ class Device:
         if out.status != abi.OK:
             out.message = self.last_error()
         return out
+
+    def optimize_weighted(self, problem: BundleProblem, options: abi.Options, weights, cap: int = 1024) -> SolveOutput:
+        """vba_optimize_weighted: `weights`, one per observation (monocular,
+        then rig), multiply the robust-kernel weights
+        (BundleAdjustment::optimize_with_observation_weights)."""
+        w = np.ascontiguousarray(weights, np.float64)
+
+        def call(view, opts, poses, lms, stats, cap_, res):
+            return self.lib.vba_optimize_weighted(
+                self.ctx, C.byref(view), C.byref(opts), w.ctypes.data_as(abi.P64), len(w),
+                poses.ctypes.data_as(abi.P64), lms.ctypes.data_as(abi.P64), stats, cap_, C.byref(res))
+
+        out = run_solver(call, problem, options, cap)
+        if out.status != abi.OK:
+            out.message = self.last_error()
+        return out
+
+    def optimize_gnc(self, problem: BundleProblem, options: abi.Options, gnc: abi.GncOptions):
+        """vba_optimize_gnc (BundleAdjustment::optimize_gnc): (status,
+        GncResult, poses, landmarks, weights)."""
+        view = problem.view()
+        poses = np.empty_like(problem.poses)
+        lms = np.empty_like(problem.landmarks)
+        w = np.empty(problem.num_observations)
+        res = abi.GncResult()
+        st = self.lib.vba_optimize_gnc(self.ctx, C.byref(view), C.byref(options), C.byref(gnc),
+                                       poses.ctypes.data_as(abi.P64), lms.ctypes.data_as(abi.P64),
+                                       w.ctypes.data_as(abi.P64), C.byref(res))
+        return st, res, poses, lms, w
 
     def reduced_solve(self, diag, edges, offdiag, rhs, tol: float, max_iterations: int = 500, clusters=None,
                       coarse=None, structure_id: int = 0):
