@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define VBA_ABI_VERSION 1u
+#define VBA_ABI_VERSION 2u
 
 /* ---------------------------------------------------------------------------
  * Status codes. VBA_DECLINED is the `None` of BaAccelerator::optimize
@@ -120,7 +120,9 @@ enum vba_pcg_maxit_policy {
 
 /* Terms of `BundleAdjustment` that this ABI does not carry. A caller sets the
  * bit when its problem has them; the library then declines (as
- * GpuBundleAdjuster::check does, solver.rs:212-251). */
+ * GpuBundleAdjuster::check does, solver.rs:212-251). Stereo observations
+ * are carried by vba_problem.stereo; the two stereo bits are for callers
+ * that do not pass them. */
 enum vba_declined_feature {
     VBA_FEATURE_STEREO_OBSERVATIONS = 1u << 0,
     VBA_FEATURE_GENERAL_STEREO = 1u << 1,
@@ -144,6 +146,28 @@ typedef struct vba_sensor {
     vba_camera camera;
     double sensor_from_rig[7]; /* qw qx qy qz tx ty tz */
 } vba_sensor;
+
+/* Stereo terms of a `BundleAdjustment`: the rectified stereo observations
+ * with their baseline (BaStereoObservation, bundle/mod.rs:207-221;
+ * stereo_baseline, problem.rs:233-235) and the calibrated general stereo
+ * observations (BaGeneralStereoObservation, bundle/mod.rs:223-237). Both need
+ * a PINHOLE or SIMPLE_PINHOLE `camera` (problem.rs:733-746) and project with
+ * its intrinsics only; a general pair's right camera also contributes its
+ * pinhole intrinsics only (sqrt_qr.rs:465-515). */
+typedef struct vba_stereo_terms {
+    double baseline;                    /* stereo_baseline; NaN or <= 0: none */
+    uint64_t num_stereo;
+    const uint64_t* stereo_pose_ids;
+    const uint64_t* stereo_landmark_ids;
+    const double* stereo_obs;           /* 3 per observation: u_l, v_l, u_r */
+    uint64_t num_general;
+    const uint64_t* general_pose_ids;
+    const uint64_t* general_landmark_ids;
+    const double* general_obs;          /* 4 per observation: left x, y, right x, y */
+    const uint32_t* general_pair;       /* index into pairs */
+    uint64_t num_pairs;
+    const vba_sensor* pairs;            /* right_camera, left_to_right = T_right<-left */
+} vba_stereo_terms;
 
 /* Caller-owned view of a `BundleAdjustment` (bundle/mod.rs:529-622).
  * Poses are T_world->camera (T_rig<-world for rig observations) as a unit
@@ -171,6 +195,9 @@ typedef struct vba_problem {
     const vba_sensor* sensors;
     uint32_t declined_features;      /* enum vba_declined_feature bits */
     uint32_t reserved;
+    const vba_stereo_terms* stereo;  /* nullable; observations are flattened as the
+                                        reference does: mono, rectified stereo,
+                                        general stereo, rig */
 } vba_problem;
 
 /* `BaConfig` (bundle/mod.rs:625-752) plus the PCG settings of
@@ -269,7 +296,7 @@ int32_t vba_optimize(vba_context* ctx, const vba_problem* problem,
 
 /* BundleAdjustment::optimize_with_observation_weights (optimize.rs:43-53):
  * `weights` holds num_weights values, one per observation in the reference's
- * flattened order (the monocular observations, then the rig observations).
+ * flattened order (monocular, rectified stereo, general stereo, rig).
  * Each multiplies its observation's robust-kernel weight in the normal
  * equations (normal_equations.rs:458, 627) and its cost term
  * (problem.rs:521-523, 583). Checked first, as validate_observation_weights
@@ -318,8 +345,8 @@ void vba_default_gnc_options(vba_gnc_options* gnc);
 /* `options` is the inner LM's BaConfig (its kernel and iteration count are
  * replaced as the reference does). On VBA_OK the final state is written as by
  * vba_optimize, and weights_out (nullable) receives the final weights, one per
- * observation in the flattened order, NaN where an observation cannot be
- * evaluated. Errors of an inner solve are returned as they are. */
+ * observation in the flattened order (monocular, rectified stereo, general
+ * stereo, rig), NaN where an observation cannot be evaluated. Errors of an inner solve are returned as they are. */
 int32_t vba_optimize_gnc(vba_context* ctx, const vba_problem* problem, const vba_options* options,
                          const vba_gnc_options* gnc, double* poses_out, double* landmarks_out,
                          double* weights_out, vba_gnc_result* result);

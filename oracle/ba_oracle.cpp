@@ -59,6 +59,8 @@ using M3 = Mat<3, 3>;
 using M23 = Mat<2, 3>;
 using M26 = Mat<2, 6>;
 using M36 = Mat<3, 6>;
+using M46 = Mat<4, 6>;
+using M43 = Mat<4, 3>;
 using M63 = Mat<6, 3>;
 using M66 = Mat<6, 6>;
 
@@ -558,12 +560,15 @@ struct Sensor {
     Cam cam;
     SE3 sensor_from_rig;
 };
+enum ObsKind { OBS_MONO = 0, OBS_RIG = 1, OBS_RECTIFIED = 2, OBS_GENERAL = 3 };
 struct Obs {
     uint32_t pose;  // index into poses (id order)
     uint32_t lm;    // index into landmarks (id order)
-    double x, y;
-    int32_t sensor;  // -1 = monocular observation with the shared camera
+    double x, y;     // pixel (the left one of a stereo observation)
+    int32_t sensor;  // rig: index into sensors; general stereo: index into pairs; else -1
     double weight = 1.0;  // external observation weight (gnc_weights[obs_idx]), 1 when none
+    int kind = OBS_MONO;
+    double x2 = 0.0, y2 = 0.0;  // rectified: u_right; general: the right pixel
 };
 struct Problem {
     Cam camera;
@@ -571,8 +576,10 @@ struct Problem {
     std::vector<uint8_t> pose_fixed;
     std::vector<V3> lms;
     std::vector<uint8_t> lm_fixed;
-    std::vector<Obs> obs;  // mono observations first, then rig, input order
+    std::vector<Obs> obs;  // flattened: mono, rectified stereo, general stereo, rig; input order
     std::vector<Sensor> sensors;
+    std::vector<Sensor> pairs;  // general stereo: right_camera and T_right<-left
+    double baseline = 0.0;      // stereo_baseline
     size_t num_mono = 0;
     // input position ↔ id-sorted slot
     std::vector<uint32_t> pose_slot_of_input, lm_slot_of_input;
@@ -614,8 +621,13 @@ int32_t load_problem(const vba_problem* pb, Problem* out, uint64_t* error_id) {
     P.camera = to_cam(pb->camera);
     Intr in;
     if (!known_model(P.camera.model) || !intrinsics(P.camera, &in)) return VBA_ERR_UNSUPPORTED_CAMERA_MODEL;
+    const vba_stereo_terms* st = pb->stereo;
+    const uint64_t ns = st ? st->num_stereo : 0, ng = st ? st->num_general : 0;
+    // check_stereo_camera_model (problem.rs:733-746)
+    if ((ns || ng) && P.camera.model != VBA_CAMERA_PINHOLE && P.camera.model != VBA_CAMERA_SIMPLE_PINHOLE)
+        return VBA_ERR_UNSUPPORTED_CAMERA_MODEL;
     if (pb->num_poses == 0) return VBA_ERR_NO_POSES;
-    bool has_visual = pb->num_observations > 0 || pb->num_rig_observations > 0;
+    bool has_visual = pb->num_observations > 0 || ns > 0 || ng > 0 || pb->num_rig_observations > 0;
     if (!has_visual) return VBA_ERR_NO_OBSERVATIONS;
     if (pb->num_landmarks == 0) return VBA_ERR_NO_LANDMARKS;
     std::vector<uint64_t> pose_ids, lm_ids;
@@ -639,7 +651,7 @@ int32_t load_problem(const vba_problem* pb, Problem* out, uint64_t* error_id) {
         P.lm_fixed[s] = pb->landmark_fixed ? (pb->landmark_fixed[i] != 0) : 0;
     }
     P.obs.clear();
-    P.obs.reserve(pb->num_observations + pb->num_rig_observations);
+    P.obs.reserve(pb->num_observations + ns + ng + pb->num_rig_observations);
     for (uint64_t i = 0; i < pb->num_observations; ++i) {
         Obs o;
         if (!find_slot(pose_ids, pb->obs_pose_ids[i], &o.pose)) {
@@ -656,6 +668,56 @@ int32_t load_problem(const vba_problem* pb, Problem* out, uint64_t* error_id) {
         P.obs.push_back(o);
     }
     P.num_mono = P.obs.size();
+    auto resolve = [&](uint64_t pid, uint64_t lid, Obs* o) -> int32_t {
+        if (!find_slot(pose_ids, pid, &o->pose)) {
+            *error_id = pid;
+            return VBA_ERR_MISSING_POSE;
+        }
+        if (!find_slot(lm_ids, lid, &o->lm)) {
+            *error_id = lid;
+            return VBA_ERR_MISSING_LANDMARK;
+        }
+        return VBA_OK;
+    };
+    // rectified stereo: the baseline, then the ids (lm.rs:194-206)
+    if (ns) {
+        if (!(std::isfinite(st->baseline) && st->baseline > 0.0)) return VBA_ERR_MISSING_STEREO_BASELINE;
+        P.baseline = st->baseline;
+        for (uint64_t i = 0; i < ns; ++i) {
+            Obs o;
+            if (int32_t e = resolve(st->stereo_pose_ids[i], st->stereo_landmark_ids[i], &o)) return e;
+            o.kind = OBS_RECTIFIED;
+            o.x = st->stereo_obs[3 * i];
+            o.y = st->stereo_obs[3 * i + 1];
+            o.x2 = st->stereo_obs[3 * i + 2];
+            o.sensor = -1;
+            P.obs.push_back(o);
+        }
+    }
+    // general stereo: ids, then the right camera's intrinsics (lm.rs:207-217)
+    if (ng) {
+        P.pairs.resize(st->num_pairs);
+        for (uint64_t q = 0; q < st->num_pairs; ++q) {
+            P.pairs[q].cam = to_cam(st->pairs[q].camera);
+            P.pairs[q].sensor_from_rig = pose_from7(st->pairs[q].sensor_from_rig);
+        }
+        for (uint64_t i = 0; i < ng; ++i) {
+            Obs o;
+            if (int32_t e = resolve(st->general_pose_ids[i], st->general_landmark_ids[i], &o)) return e;
+            const uint32_t q = st->general_pair[i];
+            if (q >= st->num_pairs) return VBA_ERR_INVALID_ARGUMENT;
+            Intr ri;
+            if (!known_model(P.pairs[q].cam.model) || !intrinsics(P.pairs[q].cam, &ri))
+                return VBA_ERR_UNSUPPORTED_CAMERA_MODEL;
+            o.kind = OBS_GENERAL;
+            o.x = st->general_obs[4 * i];
+            o.y = st->general_obs[4 * i + 1];
+            o.x2 = st->general_obs[4 * i + 2];
+            o.y2 = st->general_obs[4 * i + 3];
+            o.sensor = (int32_t)q;
+            P.obs.push_back(o);
+        }
+    }
     P.sensors.resize(pb->num_sensors);
     for (uint64_t s = 0; s < pb->num_sensors; ++s) {
         P.sensors[s].cam = to_cam(pb->sensors[s].camera);
@@ -679,6 +741,7 @@ int32_t load_problem(const vba_problem* pb, Problem* out, uint64_t* error_id) {
         o.x = pb->rig_xy[2 * i];
         o.y = pb->rig_xy[2 * i + 1];
         o.sensor = (int32_t)s;
+        o.kind = OBS_RIG;
         P.obs.push_back(o);
     }
     P.pose_var.assign(P.poses.size(), -1);
@@ -758,7 +821,129 @@ bool linearize_rig(const Problem& P, const Obs& o, Lin* out) {
     return true;
 }
 bool linearize_obs(const Problem& P, const Intr& in, const Proj& pr, const Obs& o, Lin* out) {
-    return o.sensor < 0 ? linearize_mono(P, in, pr, o, out) : linearize_rig(P, o, out);
+    return o.kind == OBS_MONO ? linearize_mono(P, in, pr, o, out) : linearize_rig(P, o, out);
+}
+// d(T X)/d(translation, rotation) = [R | -R [X]x] (normal_equations.rs:533-539).
+M36 pose_point_jacobian(const M3& R, const V3& X) {
+    M36 d = M36::zeros();
+    M3 nRS = mul(neg(R), skew(X));
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            d(r, c) = R(r, c);
+            d(r, 3 + c) = nRS(r, c);
+        }
+    return d;
+}
+// Rectified stereo (normal_equations.rs:486-571): residual (u_l, v_l, u_r)
+// with u_r = u_l - fx b / Z through the plain pinhole.
+struct Lin3 {
+    double r[3];
+    M36 jp;
+    M3 jl;
+};
+bool linearize_rectified(const Problem& P, const Intr& in, const Obs& o, Lin3* out) {
+    const SE3& T = P.poses[o.pose];
+    const V3& X = P.lms[o.lm];
+    const M3 R = qmatrix(T.q);
+    const V3 xc = se3_apply(T, X);
+    if (xc.z <= 0.0) return false;
+    double uv[2];
+    if (!project_pinhole(in, xc, uv)) return false;
+    const double b = P.baseline;
+    const double u_r = uv[0] - in.fx * b / xc.z;
+    out->r[0] = uv[0] - o.x;
+    out->r[1] = uv[1] - o.y;
+    out->r[2] = u_r - o.x2;
+    const double z_inv = 1.0 / xc.z, z_inv2 = z_inv * z_inv;
+    M3 jpi = M3::zeros();
+    jpi(0, 0) = in.fx * z_inv;
+    jpi(0, 2) = -in.fx * xc.x * z_inv2;
+    jpi(1, 1) = in.fy * z_inv;
+    jpi(1, 2) = -in.fy * xc.y * z_inv2;
+    jpi(2, 0) = in.fx * z_inv;
+    jpi(2, 2) = -in.fx * (xc.x - b) * z_inv2;
+    out->jp = mul(jpi, pose_point_jacobian(R, X));
+    out->jl = mul(jpi, R);
+    return true;
+}
+// General stereo: general_stereo_residual_jacobians (sqrt_qr.rs:465-515),
+// residual (u_l, v_l, u_r, v_r), the right camera at T_right<-left.
+struct Lin4 {
+    double r[4];
+    M46 jp;
+    M43 jl;
+};
+bool linearize_general(const Problem& P, const Intr& in, const Obs& o, Lin4* out) {
+    const Sensor& S = P.pairs[o.sensor];
+    Intr ri;
+    if (!intrinsics(S.cam, &ri)) return false;
+    const SE3& T = P.poses[o.pose];
+    const V3& X = P.lms[o.lm];
+    const V3 pl = se3_apply(T, X);
+    const V3 pr = se3_apply(S.sensor_from_rig, pl);
+    double ul[2], ur[2];
+    if (!project_pinhole(in, pl, ul) || !project_pinhole(ri, pr, ur)) return false;
+    out->r[0] = ul[0] - o.x;
+    out->r[1] = ul[1] - o.y;
+    out->r[2] = ur[0] - o.x2;
+    out->r[3] = ur[1] - o.y2;
+    M23 jl, jr;
+    if (!pinhole_projection_jacobian(in, pl, &jl) || !pinhole_projection_jacobian(ri, pr, &jr)) return false;
+    const M3 Rwl = qmatrix(T.q), Rlr = qmatrix(S.sensor_from_rig.q);
+    const M36 d = pose_point_jacobian(Rwl, X);
+    const M23 jr_lr = mul(jr, Rlr);
+    const M26 jpl = mul(jl, d), jpr = mul(jr_lr, d);
+    const M23 jll = mul(jl, Rwl), jlr = mul(jr_lr, Rwl);
+    for (int c = 0; c < 6; ++c)
+        for (int k = 0; k < 2; ++k) {
+            out->jp(k, c) = jpl(k, c);
+            out->jp(2 + k, c) = jpr(k, c);
+        }
+    for (int c = 0; c < 3; ++c)
+        for (int k = 0; k < 2; ++k) {
+            out->jl(k, c) = jll(k, c);
+            out->jl(2 + k, c) = jlr(k, c);
+        }
+    return true;
+}
+// Squared norm of a residual, summed in row order.
+template <int N>
+double norm_sq(const double* r) {
+    double s = 0.0;
+    for (int k = 0; k < N; ++k) s += r[k] * r[k];
+    return s;
+}
+// The squared residual of an observation of any kind and whether it can be
+// evaluated: the projection tests of robust_cost_weighted /
+// nonprojectable_observation_count (problem.rs:252-303, 497-583).
+bool squared_residual(const Problem& P, const Intr& in, const Obs& o, double* s) {
+    if (o.kind == OBS_MONO) {
+        const V3 xc = se3_apply(P.poses[o.pose], P.lms[o.lm]);
+        double uv[2];
+        if (xc.z <= 0.0 || !camera_project(P.camera, xc, uv)) return false;
+        const double rx = uv[0] - o.x, ry = uv[1] - o.y;
+        *s = rx * rx + ry * ry;
+        return true;
+    }
+    if (o.kind == OBS_RECTIFIED) {
+        const V3 xc = se3_apply(P.poses[o.pose], P.lms[o.lm]);
+        double uv[2];
+        if (xc.z <= 0.0 || !project_pinhole(in, xc, uv)) return false;
+        const double u_r = uv[0] - in.fx * P.baseline / xc.z;
+        const double dx = uv[0] - o.x, dy = uv[1] - o.y, dr = u_r - o.x2;
+        *s = dx * dx + dy * dy + dr * dr;
+        return true;
+    }
+    if (o.kind == OBS_GENERAL) {
+        Lin4 lin;
+        if (!linearize_general(P, in, o, &lin)) return false;
+        *s = norm_sq<4>(lin.r);
+        return true;
+    }
+    Lin lin;
+    if (!linearize_rig(P, o, &lin)) return false;
+    *s = lin.r[0] * lin.r[0] + lin.r[1] * lin.r[1];
+    return true;
 }
 
 // robust_cost_weighted (problem.rs:497-583) and
@@ -766,26 +951,15 @@ bool linearize_obs(const Problem& P, const Intr& in, const Proj& pr, const Obs& 
 void cost_and_nonprojectable(const Problem& P, const Kernel& K, double* cost, uint64_t* nonproj) {
     double total = 0.0;
     uint64_t bad = 0;
+    Intr in;
+    intrinsics(P.camera, &in);
     for (const Obs& o : P.obs) {
-        if (o.sensor < 0) {
-            V3 xc = se3_apply(P.poses[o.pose], P.lms[o.lm]);
-            double uv[2];
-            if (xc.z <= 0.0 || !camera_project(P.camera, xc, uv)) {
-                ++bad;
-                continue;
-            }
-            double rx = uv[0] - o.x, ry = uv[1] - o.y;
-            double s = rx * rx + ry * ry;
-            total += o.weight * K.cost(s);
-        } else {
-            Lin lin;
-            if (!linearize_rig(P, o, &lin)) {
-                ++bad;
-                continue;
-            }
-            double s = lin.r[0] * lin.r[0] + lin.r[1] * lin.r[1];
-            total += o.weight * K.cost(s);
+        double s;
+        if (!squared_residual(P, in, o, &s)) {
+            ++bad;
+            continue;
         }
+        total += o.weight * K.cost(s);
     }
     *cost = total;
     *nonproj = bad;
@@ -825,32 +999,50 @@ struct Contribution {
     M63 cross;
 };
 // compute_mono_contribution (normal_equations.rs:203-265) and its rig twin.
-void compute_contribution(const Problem& P, const Intr& in, const Proj& pr, const Kernel& K,
-                          const Obs& o, Contribution* c) {
-    Lin lin;
-    c->valid = linearize_obs(P, in, pr, o, &lin);
-    if (!c->valid) return;
-    double s = lin.r[0] * lin.r[0] + lin.r[1] * lin.r[1];
+// The blocks of one observation's N residual rows: w J_p^T J_p, w J_p^T r,
+// w J_l^T J_l, w J_l^T r and w J_p^T J_l (normal_equations.rs:546-567).
+template <int N>
+void contribution_from(const Problem& P, const Kernel& K, const Obs& o, const double* r,
+                       const Mat<N, 6>& jp, const Mat<N, 3>& jl, Contribution* c) {
+    double s = norm_sq<N>(r);
     double w = K.weight(s) * o.weight;
     c->p = P.pose_var[o.pose];
     c->l = P.lm_var[o.lm];
     if (c->p >= 0) {
-        M66 h = mul(tr(lin.jp), lin.jp);
+        M66 h = mul(tr(jp), jp);
         double b[6];
-        Mat<6, 2> jpt = tr(lin.jp);
-        mulv(jpt, lin.r, b);
+        Mat<6, N> jpt = tr(jp);
+        mulv(jpt, r, b);
         c->hpp = scale(w, h);
         for (int k = 0; k < 6; ++k) c->bp[k] = w * b[k];
     }
     if (c->l >= 0) {
-        M3 h = mul(tr(lin.jl), lin.jl);
+        M3 h = mul(tr(jl), jl);
         double b[3];
-        Mat<3, 2> jlt = tr(lin.jl);
-        mulv(jlt, lin.r, b);
+        Mat<3, N> jlt = tr(jl);
+        mulv(jlt, r, b);
         c->hll = scale(w, h);
         for (int k = 0; k < 3; ++k) c->bl[k] = w * b[k];
     }
-    if (c->p >= 0 && c->l >= 0) c->cross = scale(w, mul(tr(lin.jp), lin.jl));
+    if (c->p >= 0 && c->l >= 0) c->cross = scale(w, mul(tr(jp), jl));
+}
+void compute_contribution(const Problem& P, const Intr& in, const Proj& pr, const Kernel& K,
+                          const Obs& o, Contribution* c) {
+    if (o.kind == OBS_RECTIFIED) {
+        Lin3 lin;
+        c->valid = linearize_rectified(P, in, o, &lin);
+        if (c->valid) contribution_from<3>(P, K, o, lin.r, lin.jp, lin.jl, c);
+        return;
+    }
+    if (o.kind == OBS_GENERAL) {
+        Lin4 lin;
+        c->valid = linearize_general(P, in, o, &lin);
+        if (c->valid) contribution_from<4>(P, K, o, lin.r, lin.jp, lin.jl, c);
+        return;
+    }
+    Lin lin;
+    c->valid = linearize_obs(P, in, pr, o, &lin);
+    if (c->valid) contribution_from<2>(P, K, o, lin.r, lin.jp, lin.jl, c);
 }
 void apply_contribution(const Contribution& c, Normal* ne) {
     if (!c.valid) return;
@@ -1429,11 +1621,21 @@ int32_t validate_matrix_free(const Problem& P, const vba_options& o, const Kerne
     uint64_t np;
     cost_and_nonprojectable(P, K, &c, &np);
     if (!std::isfinite(c)) return VBA_ERR_INELIGIBLE;
+    auto finite_pose = [](const SE3& T) {
+        const double v[7] = {T.q.w, T.q.i, T.q.j, T.q.k, T.t.x, T.t.y, T.t.z};
+        for (double x : v)
+            if (!std::isfinite(x)) return false;
+        return true;
+    };
+    // optimize.rs:546-622: pixels, and the rig / stereo pair calibration
     for (const Obs& ob : P.obs) {
         if (!std::isfinite(ob.x) || !std::isfinite(ob.y)) return VBA_ERR_INELIGIBLE;
-        if (ob.sensor >= 0) {
-            const Sensor& s = P.sensors[ob.sensor];
-            if (!finite_cam(s.cam)) return VBA_ERR_INELIGIBLE;
+        if ((ob.kind == OBS_RECTIFIED || ob.kind == OBS_GENERAL) && !std::isfinite(ob.x2))
+            return VBA_ERR_INELIGIBLE;
+        if (ob.kind == OBS_GENERAL && !std::isfinite(ob.y2)) return VBA_ERR_INELIGIBLE;
+        if (ob.kind == OBS_RIG || ob.kind == OBS_GENERAL) {
+            const Sensor& s = ob.kind == OBS_RIG ? P.sensors[ob.sensor] : P.pairs[ob.sensor];
+            if (!finite_cam(s.cam) || !finite_pose(s.sensor_from_rig)) return VBA_ERR_INELIGIBLE;
         }
     }
     return VBA_OK;
@@ -1615,7 +1817,7 @@ std::vector<double> reprojection_squared_residuals(const Problem& P) {
     const Proj pr = mono_projection_for(P.camera);
     for (size_t i = 0; i < P.obs.size(); ++i) {
         const Obs& o = P.obs[i];
-        if (o.sensor < 0) {
+        if (o.kind == OBS_MONO) {
             const V3 xc = se3_apply(P.poses[o.pose], P.lms[o.lm]);
             if (xc.z <= 0.0) continue;
             double uv[2];
@@ -1624,9 +1826,8 @@ std::vector<double> reprojection_squared_residuals(const Problem& P) {
             const double rx = uv[0] - o.x, ry = uv[1] - o.y;
             out[i] = rx * rx + ry * ry;
         } else {
-            Lin lin;
-            if (!linearize_rig(P, o, &lin)) continue;
-            out[i] = lin.r[0] * lin.r[0] + lin.r[1] * lin.r[1];
+            double s;
+            if (squared_residual(P, in, o, &s)) out[i] = s;
         }
     }
     return out;
@@ -1757,7 +1958,8 @@ int32_t oracle_optimize_weighted(const vba_problem* problem, const vba_options* 
     if (problem->declined_features & ~(uint32_t)VBA_FEATURE_OBSERVATION_WEIGHTS) return VBA_DECLINED;
     // validate_observation_weights (optimize.rs:628-650), then the intrinsics
     // refusal (:49-51)
-    const uint64_t expected = problem->num_observations + problem->num_rig_observations;
+    const uint64_t expected = problem->num_observations + problem->num_rig_observations +
+                              (problem->stereo ? problem->stereo->num_stereo + problem->stereo->num_general : 0);
     if (num_weights != expected || (num_weights && !weights)) return VBA_ERR_OBSERVATION_WEIGHT_COUNT;
     for (uint64_t i = 0; i < num_weights; ++i)
         if (!std::isfinite(weights[i]) || weights[i] < 0.0) {
@@ -1906,6 +2108,8 @@ int32_t oracle_linearize(const vba_problem* problem, uint32_t kernel, double sca
     uint64_t eid;
     int32_t st = load_problem(problem, &P, &eid);
     if (st != VBA_OK) return st;
+    for (const Obs& o : P.obs)
+        if (o.kind == OBS_RECTIFIED || o.kind == OBS_GENERAL) return VBA_ERR_INVALID_ARGUMENT;  // 2-row layout only
     Kernel K{kernel, scale};
     Intr in;
     intrinsics(P.camera, &in);

@@ -71,7 +71,9 @@ void oracle_robust(uint32_t kernel, double scale, double s, double* cost, double
  * observations first, then rig observations (normal_equations.rs:418-484,
  * 617-651; sqrt_qr.rs:960-1001). For each: residual r[2], J_pose 2x6
  * row-major [12], J_landmark 2x3 [6], IRLS weight w and valid (0 when the
- * observation is skipped). Returns VBA_OK or a validation error. */
+ * observation is skipped). Returns VBA_OK or a validation error, and
+ * VBA_ERR_INVALID_ARGUMENT for a problem with stereo observations (3 or 4
+ * rows each). */
 int32_t oracle_linearize(const vba_problem* problem, uint32_t kernel, double scale,
                          double* r, double* j_pose, double* j_landmark, double* w,
                          uint8_t* valid);

@@ -141,3 +141,179 @@ def gnc_bundle(outliers=(), shift=0.0):
             obs.append((kid, lid, uv))
             idx += 1
     return make_problem(pinhole_camera(), truth, lms, obs, fixed_poses=(10, 14)), truth
+
+
+# ---------------------------------------------------------------------------
+# Stereo scenes (pipelines/slam/tests/bundle_adjustment.rs:1519-1853)
+# ---------------------------------------------------------------------------
+def euler_matrix(roll, pitch, yaw):
+    """UnitQuaternion::from_euler_angles: R = Rz(yaw) Ry(pitch) Rx(roll)."""
+    return rodrigues([0.0, 0.0, yaw]) @ rodrigues([0.0, pitch, 0.0]) @ rodrigues([roll, 0.0, 0.0])
+
+
+def rectified_right_u(pose, X, baseline, f=500.0, c=320.0):
+    """The right camera of a rectified pair sits at +baseline along the left
+    camera's x axis: u_r = f (X_c - b) / Z_c + c_x."""
+    xc = transform(pose, X)
+    return f * (xc[0] - baseline) / xc[2] + c
+
+
+def with_stereo(problem, stereo=(), baseline=float("nan"), general=(), pairs=()):
+    """The problem plus rectified observations [(pose id, lm id, (u_l, v_l,
+    u_r))] and general ones [(pose id, lm id, (xl, yl, xr, yr), pair)]."""
+    import dataclasses
+    return dataclasses.replace(
+        problem,
+        stereo_pose_ids=np.array([s[0] for s in stereo], np.uint64),
+        stereo_landmark_ids=np.array([s[1] for s in stereo], np.uint64),
+        stereo_obs=np.array([s[2] for s in stereo], float).reshape(-1, 3), stereo_baseline=baseline,
+        general_pose_ids=np.array([g[0] for g in general], np.uint64),
+        general_landmark_ids=np.array([g[1] for g in general], np.uint64),
+        general_obs=np.array([g[2] for g in general], float).reshape(-1, 4),
+        general_pair=np.array([g[3] for g in general], np.uint32), pairs=list(pairs))
+
+
+STEREO_TRUTH = [(10, [0.0, 0.0, 0.0]), (20, [0.5, 0.0, 0.0]), (30, [1.0, 0.0, 0.0])]
+
+
+def truth_stereo_bundle(baseline, fixed=(), poses=None, landmarks=None):
+    """truth_stereo_bundle (bundle_adjustment.rs:1519-1551): three keyframes
+    seeing the six-point grid through a rectified pair; `poses` / `landmarks`
+    replace initial states by id."""
+    truth = [(k, pose_at(c)) for k, c in STEREO_TRUTH]
+    st = []
+    for kid, pose in truth:
+        for lid, X in world_grid():
+            uv = project_pinhole((500.0, 500.0), (320.0, 240.0), transform(pose, X))
+            st.append((kid, lid, (uv[0], uv[1], rectified_right_u(pose, X, baseline))))
+    init = [(k, (poses or {}).get(k, p)) for k, p in truth]
+    lms = [(l, (landmarks or {}).get(l, X)) for l, X in world_grid()]
+    p = make_problem(pinhole_camera(), init, lms, [], fixed_poses=fixed)
+    return with_stereo(p, stereo=st, baseline=baseline)
+
+
+def right_camera():
+    """The general pair's right camera (bundle_adjustment.rs:1638)."""
+    return vba.camera(abi.CAMERA_PINHOLE, [510.0, 495.0, 318.0, 242.0], 640, 480)
+
+
+def left_to_right():
+    """T_right<-left with rotation, vertical translation and unequal
+    intrinsics (bundle_adjustment.rs:1641-1644)."""
+    return pose_from_rt(euler_matrix(0.01, -0.02, 0.015), np.array([-0.11, 0.002, 0.001]))
+
+
+def general_obs(pose, X, lr=None):
+    lr = left_to_right() if lr is None else lr
+    pl = transform(pose, X)
+    pr = transform(lr, pl)
+    return (*project_pinhole((500.0, 500.0), (320.0, 240.0), pl), *project_pinhole((510.0, 495.0), (318.0, 242.0), pr))
+
+
+def truth_general_stereo_bundle(start=None):
+    """truth_general_stereo_bundle (bundle_adjustment.rs:1636-1668): one fixed
+    identity keyframe, landmark 5 at (0.2, -0.1, 4) seen by the general pair;
+    `start` replaces the landmark's initial position."""
+    pose = np.array([1.0, 0, 0, 0, 0, 0, 0])
+    truth = np.array([0.2, -0.1, 4.0])
+    p = make_problem(pinhole_camera(), [(10, pose)], [(5, truth if start is None else np.asarray(start, float))], [],
+                     fixed_poses=(10,))
+    return with_stereo(p, general=[(10, 5, general_obs(pose, truth), 0)],
+                       pairs=[vba.sensor(right_camera(), left_to_right())]), truth
+
+
+def general_pose_bundle():
+    """bundle_general_stereo_recovers_pose_with_rotated_right_camera
+    (bundle_adjustment.rs:1702-1755): one variable keyframe starting off its
+    truth, the six fixed grid landmarks seen through the general pair."""
+    truth = pose_at([0.25, -0.03, 0.04], 0.035)
+    start = pose_from_rt(euler_matrix(0.025, 0.07, -0.018), np.array([-0.39, 0.12, -0.08]))
+    grid = world_grid()
+    p = make_problem(pinhole_camera(), [(10, start)], grid, [], fixed_landmarks=[l for l, _ in grid])
+    gen = [(10, l, general_obs(truth, X), 0) for l, X in grid]
+    return with_stereo(p, general=gen, pairs=[vba.sensor(right_camera(), left_to_right())]), truth
+
+
+def mixed_mono_stereo_bundle(baseline=0.5):
+    """bundle_mixed_mono_and_stereo_observations_optimize_together
+    (bundle_adjustment.rs:1795-1853): keyframes 10 and 20 stereo, 40
+    monocular and drifted, 10 fixed."""
+    truth = [(10, pose_at([0.0, 0.0, 0.0])), (20, pose_at([0.5, 0.0, 0.0])), (40, pose_at([1.5, 0.0, 0.0]))]
+    mono, st = [], []
+    for kid, pose in truth:
+        for lid, X in world_grid():
+            uv = project_pinhole((500.0, 500.0), (320.0, 240.0), transform(pose, X))
+            if kid == 40:
+                mono.append((kid, lid, uv))
+            else:
+                st.append((kid, lid, (uv[0], uv[1], rectified_right_u(pose, X, baseline))))
+    init = truth[:2] + [(40, pose_at([1.55, 0.04, 0.06]))]
+    p = make_problem(pinhole_camera(), init, world_grid(), mono, fixed_poses=(10,))
+    return with_stereo(p, stereo=st, baseline=baseline)
+
+
+def split_into_stereo(p, rng, rectified=0.3, general=0.2, baseline=0.12, noise=0.5):
+    """A mixed problem from one with monocular observations and a PINHOLE
+    camera: a share of them become rectified stereo observations (u_r from
+    the current state's depth) and another share general stereo
+    observations (the right pixel of the current state through the
+    right_camera() pair), both with pixel noise; the rest stay monocular."""
+    fx, fy, cx, cy = (p.camera.params[i] for i in range(4))
+    pidx = {int(i): k for k, i in enumerate(p.pose_ids)}
+    lidx = {int(i): k for k, i in enumerate(p.landmark_ids)}
+    n = len(p.obs_pose_ids)
+    pk = np.array([pidx[int(i)] for i in p.obs_pose_ids])
+    lk = np.array([lidx[int(i)] for i in p.obs_landmark_ids])
+    Rs = np.array([quat_to_matrix(q) for q in p.poses[:, :4]])
+    xc = np.einsum("nij,nj->ni", Rs[pk], p.landmarks[lk]) + p.poses[pk, 4:7]
+    kind = rng.choice(3, size=n, p=[1 - rectified - general, rectified, general])
+    kind[xc[:, 2] <= 0.1] = 0
+    lr = left_to_right()
+    Rlr = quat_to_matrix(lr[:4])
+    xr = xc @ Rlr.T + lr[4:7]
+    kind[(kind == 2) & (xr[:, 2] <= 0.1)] = 0
+    st = [(int(p.obs_pose_ids[k]), int(p.obs_landmark_ids[k]),
+           (p.obs_xy[k, 0], p.obs_xy[k, 1], p.obs_xy[k, 0] - fx * baseline / xc[k, 2] + rng.normal(0, noise)))
+          for k in np.flatnonzero(kind == 1)]
+    ge = [(int(p.obs_pose_ids[k]), int(p.obs_landmark_ids[k]),
+           (p.obs_xy[k, 0], p.obs_xy[k, 1], 510.0 * xr[k, 0] / xr[k, 2] + 318.0 + rng.normal(0, noise),
+            495.0 * xr[k, 1] / xr[k, 2] + 242.0 + rng.normal(0, noise)), 0)
+          for k in np.flatnonzero(kind == 2)]
+    import dataclasses
+    keep = kind == 0
+    mono = dataclasses.replace(p, obs_pose_ids=p.obs_pose_ids[keep], obs_landmark_ids=p.obs_landmark_ids[keep],
+                               obs_xy=p.obs_xy[keep])
+    return with_stereo(mono, stereo=st, baseline=baseline, general=ge,
+                       pairs=[sensor_pair()])
+
+
+def sensor_pair():
+    return vba.sensor(right_camera(), left_to_right())
+
+
+def add_stereo_from_state(p, rng, n_rect, n_gen, baseline=0.12, noise=0.5):
+    """p plus rectified and general stereo observations of (pose, landmark)
+    pairs the problem already observes, measured at the current state with
+    pixel noise through p's pinhole intrinsics (and right_camera() for the
+    general pair): edges then hold observations of several kinds."""
+    fx, fy, cx, cy = (p.camera.params[i] for i in range(4))
+    pairs = sorted(set(zip(np.concatenate([p.obs_pose_ids, p.rig_pose_ids]).tolist(),
+                           np.concatenate([p.obs_landmark_ids, p.rig_landmark_ids]).tolist())))
+    pick = rng.choice(len(pairs), size=min(len(pairs), n_rect + n_gen), replace=False)
+    P = {int(i): p.poses[k] for k, i in enumerate(p.pose_ids)}
+    X = {int(i): p.landmarks[k] for k, i in enumerate(p.landmark_ids)}
+    lr = left_to_right()
+    st, ge = [], []
+    for j, k in enumerate(pick):
+        pid, lid = pairs[k]
+        xc = transform(P[pid], X[lid])
+        xr = transform(lr, xc)
+        if xc[2] <= 0.1 or xr[2] <= 0.1:
+            continue
+        u, v = fx * xc[0] / xc[2] + cx + rng.normal(0, noise), fy * xc[1] / xc[2] + cy + rng.normal(0, noise)
+        if j < n_rect:
+            st.append((pid, lid, (u, v, fx * (xc[0] - baseline) / xc[2] + cx + rng.normal(0, noise))))
+        else:
+            ge.append((pid, lid, (u, v, 510.0 * xr[0] / xr[2] + 318.0 + rng.normal(0, noise),
+                                  495.0 * xr[1] / xr[2] + 242.0 + rng.normal(0, noise)), 0))
+    return with_stereo(p, stereo=st, baseline=baseline, general=ge, pairs=[sensor_pair()])

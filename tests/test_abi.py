@@ -42,6 +42,8 @@ def test_struct_sizes_match_c_layout():
     assert C.sizeof(abi.IterationStats) == 72
     assert C.sizeof(abi.Result) == 56
     assert C.sizeof(abi.Tuning) == 16
+    assert C.sizeof(abi.StereoTerms) == 96
+    assert C.sizeof(abi.Problem) == 280 and abi.Problem.stereo.offset == 272
 
 
 def test_default_options_are_baconfig_default():

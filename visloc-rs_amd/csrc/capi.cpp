@@ -188,7 +188,7 @@ int32_t vba_optimize_weighted(vba_context* ctx, const vba_problem* problem, cons
     ctx->last_error.clear();
     // validate_observation_weights (optimize.rs:628-650), then the intrinsics
     // refusal (:49-51)
-    const uint64_t expected = problem->num_observations + problem->num_rig_observations;
+    const uint64_t expected = vba::observation_count(*problem);
     if (num_weights != expected || (num_weights && !weights)) {
         ctx->last_error = "observation weight count " + std::to_string(num_weights) + ", expected " +
                           std::to_string(expected);

@@ -19,6 +19,71 @@ namespace vba {
 
 namespace {
 
+// Rows of a 2x3 projection Jacobian times a 3x3 matrix A, into J (3 per row).
+__device__ __forceinline__ void rows_times(const double Jp[6], const double A[9], double* J) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            J[3 * k + c] = Jp[3 * k] * A[c] + Jp[3 * k + 1] * A[3 + c] + Jp[3 * k + 2] * A[6 + c];
+}
+
+// Residual rows r and landmark Jacobian rows J (3 per row, J_lm = J_pi R) of
+// a stereo observation at xr, the point in the pose (left camera) frame, as
+// four rows (a rectified observation's fourth row is zero):
+//   rectified (normal_equations.rs:486-571): (u_l, v_l, u_r) with
+//     u_r = u_l - fx b / Z, J_pi's third row (fx/Z, 0, -fx (X - b)/Z^2);
+//   general (sqrt_qr.rs:465-515): the left pinhole rows, then the right
+//     camera's pinhole rows at T_right<-left xr, J = J_pi_r R_lr R.
+// Both project with the camera's plain pinhole (fx, fy, cx, cy). Returns
+// false where the reference skips the observation (a point at or behind
+// either camera).
+template <bool JAC>
+__device__ __forceinline__ bool stereo_rows(const DevProblem& P, int o, const DevSensor& S, const double xr[3],
+                                            const double R[9], double r[4], double J[12]) {
+    const uint8_t kind = P.obs_kind[o];
+    Lens<double> L = (kind == OBS_GENERAL ? P.sensors[S.left] : S).lens;
+    L.kind = LENS_PINHOLE;
+    const double ex0 = P.obs_extra[2 * (size_t)o], ex1 = P.obs_extra[2 * (size_t)o + 1];
+    double uv[2], Jp[6];
+    if (!project<double>(L, xr[0], xr[1], xr[2], uv, JAC ? Jp : (double*)nullptr)) return false;
+    r[0] = uv[0] - P.obs_uv[2 * (size_t)o];
+    r[1] = uv[1] - P.obs_uv[2 * (size_t)o + 1];
+    if (JAC) rows_times(Jp, R, J);
+    if (kind == OBS_RECTIFIED) {
+        const double b = ex1;
+        r[2] = (uv[0] - L.fx * b / xr[2]) - ex0;
+        r[3] = 0.0;
+        if (JAC) {
+            const double zi = 1.0 / xr[2];
+            const double j0 = L.fx * zi, j2 = -L.fx * (xr[0] - b) * (zi * zi);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                J[6 + c] = j0 * R[c] + j2 * R[6 + c];
+                J[9 + c] = 0.0;
+            }
+        }
+        return true;
+    }
+    double xs[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) xs[i] = S.R[3 * i] * xr[0] + S.R[3 * i + 1] * xr[1] + S.R[3 * i + 2] * xr[2] + S.t[i];
+    double uvr[2];
+    if (!project<double>(S.lens, xs[0], xs[1], xs[2], uvr, JAC ? Jp : (double*)nullptr)) return false;
+    r[2] = uvr[0] - ex0;
+    r[3] = uvr[1] - ex1;
+    if (JAC) {
+        double M[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                M[3 * i + j] = S.R[3 * i] * R[j] + S.R[3 * i + 1] * R[3 + j] + S.R[3 * i + 2] * R[6 + j];
+        rows_times(Jp, M, J + 6);
+    }
+    return true;
+}
+
 // ---------------------------------------------------------------------------
 // Linearization: residuals, Jacobians and IRLS weights in FP64 at the current
 // state, G and h per edge into c-order and l-order records, and the tile's
@@ -48,6 +113,26 @@ __global__ __launch_bounds__(kBlock) void k_linearize(DevProblem P, int store32)
         const int o_end = P.edge_obs[e + 1];
         for (int o = P.edge_obs[e]; o < o_end; ++o) {
             const DevSensor& S = P.sensors[RIG ? P.obs_sensor[o] : 0];
+            if (RIG && P.obs_kind && P.obs_kind[o] != OBS_PIXEL) {
+                double r[4], J[12];
+                if (!stereo_rows<true>(P, o, S, xr, R, r, J)) continue;
+                const double sq = ((r[0] * r[0] + r[1] * r[1]) + r[2] * r[2]) + r[3] * r[3];
+                const double w = robust_weight<double>((uint32_t)P.kernel_kind, P.kernel_scale, sq) *
+                                 (P.obs_w ? P.obs_w[o] : 1.0);
+                double Gs[6] = {0, 0, 0, 0, 0, 0}, hs[3] = {0, 0, 0};
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const double a = J[3 * k], b = J[3 * k + 1], c = J[3 * k + 2];
+                    Gs[0] += a * a; Gs[1] += a * b; Gs[2] += a * c;
+                    Gs[3] += b * b; Gs[4] += b * c; Gs[5] += c * c;
+                    hs[0] += a * r[k]; hs[1] += b * r[k]; hs[2] += c * r[k];
+                }
+#pragma unroll
+                for (int i = 0; i < 6; ++i) G[i] += w * Gs[i];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) h[i] += w * hs[i];
+                continue;
+            }
             double xs[3], M[9];
             if (RIG && !S.identity) {
 #pragma unroll
@@ -266,19 +351,30 @@ __global__ __launch_bounds__(kBlock) void k_cost(DevProblem P, int trial) {
                               R[6] * X[0] + R[7] * X[1] + R[8] * X[2] + pq[6]};
         for (int o = P.edge_obs[e]; o < P.edge_obs[e + 1]; ++o) {
             const DevSensor& S = P.sensors[RIG ? P.obs_sensor[o] : 0];
-            double xs[3] = {xr[0], xr[1], xr[2]};
-            if (RIG && !S.identity) {
-                for (int i = 0; i < 3; ++i)
-                    xs[i] = S.R[3 * i] * xr[0] + S.R[3 * i + 1] * xr[1] + S.R[3 * i + 2] * xr[2] + S.t[i];
+            double sq;
+            if (RIG && P.obs_kind && P.obs_kind[o] != OBS_PIXEL) {
+                double r[4];
+                if (!stereo_rows<false>(P, o, S, xr, R, r, (double*)nullptr)) {
+                    bad += 1.0;
+                    if (P.obs_s) P.obs_s[o] = __builtin_nan("");
+                    continue;
+                }
+                sq = ((r[0] * r[0] + r[1] * r[1]) + r[2] * r[2]) + r[3] * r[3];
+            } else {
+                double xs[3] = {xr[0], xr[1], xr[2]};
+                if (RIG && !S.identity) {
+                    for (int i = 0; i < 3; ++i)
+                        xs[i] = S.R[3 * i] * xr[0] + S.R[3 * i + 1] * xr[1] + S.R[3 * i + 2] * xr[2] + S.t[i];
+                }
+                double uv[2];
+                if (!project<double>(S.lens, xs[0], xs[1], xs[2], uv, (double*)nullptr)) {
+                    bad += 1.0;
+                    if (P.obs_s) P.obs_s[o] = __builtin_nan("");
+                    continue;
+                }
+                const double r0 = uv[0] - P.obs_uv[2 * (size_t)o], r1 = uv[1] - P.obs_uv[2 * (size_t)o + 1];
+                sq = r0 * r0 + r1 * r1;
             }
-            double uv[2];
-            if (!project<double>(S.lens, xs[0], xs[1], xs[2], uv, (double*)nullptr)) {
-                bad += 1.0;
-                if (P.obs_s) P.obs_s[o] = __builtin_nan("");
-                continue;
-            }
-            const double r0 = uv[0] - P.obs_uv[2 * (size_t)o], r1 = uv[1] - P.obs_uv[2 * (size_t)o + 1];
-            const double sq = r0 * r0 + r1 * r1;
             if (P.obs_s) P.obs_s[o] = sq;
             const double c = robust_cost<double>((uint32_t)P.kernel_kind, P.kernel_scale, sq);
             // w rho(s) with the observation's external weight (problem.rs:521-523, 583)

@@ -81,6 +81,8 @@ struct DevProblem {
     const double* obs_w;  // external weight per observation, c-order (null: all 1)
     double* obs_s;        // k_cost writes each observation's squared residual here (NaN: not evaluable)
     const uint16_t* obs_sensor;
+    const uint8_t* obs_kind;    // ObsKind per observation, c-order (null: all pixel residuals)
+    const double* obs_extra;    // 2 per observation, c-order: rectified (u_r, baseline), general (u_r, v_r)
     const int32_t* edge_lm;
     const int32_t* edge_obs;
     const int32_t* perm_cl;   // c-order edge -> storage position

@@ -208,7 +208,7 @@ int32_t check_support(const vba_problem& pb, const vba_options& o, std::string* 
         return VBA_DECLINED;
     }
     if (pb.declined_features & (VBA_FEATURE_STEREO_OBSERVATIONS | VBA_FEATURE_GENERAL_STEREO)) {
-        *reason = "stereo observations";
+        *reason = "stereo observations not passed in vba_problem.stereo";
         return VBA_DECLINED;
     }
     if (pb.declined_features & (VBA_FEATURE_PRIORS | VBA_FEATURE_INERTIAL)) {
@@ -236,11 +236,13 @@ int32_t check_support(const vba_problem& pb, const vba_options& o, std::string* 
             *reason = "rig camera model";
             return VBA_DECLINED;
         }
-    if (pb.num_sensors > 65535) {
-        *reason = "more than 65535 rig sensors";
+    const uint64_t ns = pb.stereo ? pb.stereo->num_stereo : 0, ng = pb.stereo ? pb.stereo->num_general : 0;
+    const uint64_t npairs = ng ? pb.stereo->num_pairs : 0;
+    if (pb.num_sensors + npairs > 65534) {
+        *reason = "more than 65534 rig sensors and stereo pairs";
         return VBA_DECLINED;
     }
-    if (pb.num_observations + pb.num_rig_observations > (uint64_t)INT32_MAX) {
+    if (pb.num_observations + ns + ng + pb.num_rig_observations > (uint64_t)INT32_MAX) {
         *reason = "more than 2^31 observations";
         return VBA_DECLINED;
     }
@@ -256,13 +258,24 @@ int32_t build_model(const vba_problem& pb, Model* mp, uint64_t* error_id, std::s
         *message = "camera model has no intrinsics";
         return VBA_ERR_UNSUPPORTED_CAMERA_MODEL;
     }
+    const vba_stereo_terms* st = pb.stereo;
+    const uint64_t ns = st ? st->num_stereo : 0, ng = st ? st->num_general : 0;
+    // check_stereo_camera_model (problem.rs:733-746)
+    if ((ns || ng) && pb.camera.model != VBA_CAMERA_PINHOLE && pb.camera.model != VBA_CAMERA_SIMPLE_PINHOLE) {
+        *message = "stereo observations need a PINHOLE or SIMPLE_PINHOLE camera";
+        return VBA_ERR_UNSUPPORTED_CAMERA_MODEL;
+    }
     if (pb.num_poses == 0) return VBA_ERR_NO_POSES;
-    if (pb.num_observations == 0 && pb.num_rig_observations == 0) return VBA_ERR_NO_OBSERVATIONS;
+    if (pb.num_observations == 0 && ns == 0 && ng == 0 && pb.num_rig_observations == 0)
+        return VBA_ERR_NO_OBSERVATIONS;
     if (pb.num_landmarks == 0) return VBA_ERR_NO_LANDMARKS;
     if (!pb.pose_ids || !pb.poses || !pb.landmark_ids || !pb.landmarks ||
         (pb.num_observations && (!pb.obs_pose_ids || !pb.obs_landmark_ids || !pb.obs_xy)) ||
         (pb.num_rig_observations &&
-         (!pb.rig_pose_ids || !pb.rig_landmark_ids || !pb.rig_xy || !pb.rig_sensor || !pb.sensors))) {
+         (!pb.rig_pose_ids || !pb.rig_landmark_ids || !pb.rig_xy || !pb.rig_sensor || !pb.sensors)) ||
+        (ns && (!st->stereo_pose_ids || !st->stereo_landmark_ids || !st->stereo_obs)) ||
+        (ng && (!st->general_pose_ids || !st->general_landmark_ids || !st->general_obs || !st->general_pair ||
+                !st->pairs))) {
         *message = "null array in problem";
         return VBA_ERR_INVALID_ARGUMENT;
     }
@@ -309,24 +322,75 @@ int32_t build_model(const vba_problem& pb, Model* mp, uint64_t* error_id, std::s
         d.identity = 0;
         m.sensors.push_back(d);
     }
+    // general stereo pairs: the right camera's pinhole intrinsics
+    // (sqrt_qr.rs:471-476) and T_right<-left; checked per observation
+    if (ng && pb.num_sensors + st->num_pairs > 65534) {
+        *message = "more than 65534 rig sensors and stereo pairs";
+        return VBA_ERR_INVALID_ARGUMENT;
+    }
+    if (ng)
+        for (uint64_t q = 0; q < pb.stereo->num_pairs; ++q) {
+            DevSensor d{};
+            const vba_sensor& vs = pb.stereo->pairs[q];
+            d.lens.kind = LENS_PINHOLE;
+            if (!camera_intrinsics(vs.camera, &d.lens.fx, &d.lens.fy, &d.lens.cx, &d.lens.cy)) d.lens.kind = -1;
+            quat_to_matrix(vs.sensor_from_rig, d.R);
+            d.t[0] = vs.sensor_from_rig[4];
+            d.t[1] = vs.sensor_from_rig[5];
+            d.t[2] = vs.sensor_from_rig[6];
+            d.identity = 0;
+            d.left = 0;
+            m.sensors.push_back(d);
+        }
 
-    // Observations: mono then rig, ids resolved in input order (lm.rs:193-229).
-    const uint64_t nm = pb.num_observations, nr = pb.num_rig_observations, n = nm + nr;
+    // Observations in the reference's flattened order (mono, rectified
+    // stereo, general stereo, rig), ids resolved in that order with the
+    // checks of lm.rs:193-229.
+    const uint64_t nm = pb.num_observations, nr = pb.num_rig_observations;
+    const uint64_t so = nm, go = nm + ns, ro = nm + ns + ng, n = ro + nr;
     std::vector<uint32_t> opose(n), olm(n);
     const IdIndex pose_slot(pose_ids), lm_slot(lm_ids);
     std::vector<uint16_t> osensor(n, 0);
-    for (uint64_t i = 0; i < nm; ++i) {
-        if (!pose_slot(pb.obs_pose_ids[i], &opose[i])) {
-            *error_id = pb.obs_pose_ids[i];
+    auto resolve = [&](const uint64_t* pids, const uint64_t* lids, uint64_t i, uint64_t k) -> int32_t {
+        if (!pose_slot(pids[i], &opose[k])) {
+            *error_id = pids[i];
             return VBA_ERR_MISSING_POSE;
         }
-        if (!lm_slot(pb.obs_landmark_ids[i], &olm[i])) {
-            *error_id = pb.obs_landmark_ids[i];
+        if (!lm_slot(lids[i], &olm[k])) {
+            *error_id = lids[i];
             return VBA_ERR_MISSING_LANDMARK;
+        }
+        return VBA_OK;
+    };
+    for (uint64_t i = 0; i < nm; ++i)
+        if (int32_t e = resolve(pb.obs_pose_ids, pb.obs_landmark_ids, i, i)) return e;
+    double baseline = 0.0;
+    if (ns) {
+        if (!(std::isfinite(st->baseline) && st->baseline > 0.0)) return VBA_ERR_MISSING_STEREO_BASELINE;
+        baseline = st->baseline;
+        for (uint64_t i = 0; i < ns; ++i)
+            if (int32_t e = resolve(st->stereo_pose_ids, st->stereo_landmark_ids, i, so + i)) return e;
+    }
+    // a general pair is a sensor after the rig sensors: its right camera's
+    // pinhole intrinsics and T_right<-left
+    const uint32_t pair0 = 1 + (uint32_t)pb.num_sensors;
+    if (ng) {
+        for (uint64_t i = 0; i < ng; ++i) {
+            if (int32_t e = resolve(st->general_pose_ids, st->general_landmark_ids, i, go + i)) return e;
+            const uint32_t q = st->general_pair[i];
+            if (q >= st->num_pairs) {
+                *message = "stereo pair index out of range";
+                return VBA_ERR_INVALID_ARGUMENT;
+            }
+            if (m.sensors[pair0 + q].lens.kind < 0) {
+                *message = "right camera has no intrinsics";
+                return VBA_ERR_UNSUPPORTED_CAMERA_MODEL;
+            }
+            osensor[go + i] = (uint16_t)(pair0 + q);
         }
     }
     for (uint64_t i = 0; i < nr; ++i) {
-        uint64_t k = nm + i;
+        uint64_t k = ro + i;
         if (!pose_slot(pb.rig_pose_ids[i], &opose[k])) {
             *error_id = pb.rig_pose_ids[i];
             return VBA_ERR_MISSING_POSE;
@@ -347,6 +411,7 @@ int32_t build_model(const vba_problem& pb, Model* mp, uint64_t* error_id, std::s
         osensor[k] = (uint16_t)(s + 1);
     }
     m.any_rig = nr > 0;
+    m.any_stereo = ns + ng > 0;
     // Landmark slots in order of their first, then last, observing pose slot
     // (ties in id order): the landmarks of a tile then span few cameras, so
     // its edges form few, long camera runs. Two stable counting sorts, by the
@@ -411,14 +476,32 @@ int32_t build_model(const vba_problem& pb, Model* mp, uint64_t* error_id, std::s
     }
     m.obs_uv.resize(2 * n);
     m.obs_input = cord;
-    if (m.any_rig) m.obs_sensor.resize(n);
+    const bool per_sensor = m.any_rig || m.any_stereo;
+    if (per_sensor) m.obs_sensor.resize(n);
     else m.obs_sensor.clear();
+    m.obs_kind.assign(m.any_stereo ? n : 0, OBS_PIXEL);
+    m.obs_extra.assign(m.any_stereo ? 2 * n : 0, 0.0);
     for (uint64_t k = 0; k < n; ++k) {
         uint32_t i = cord[k];
-        const double* xy = i < nm ? pb.obs_xy + 2 * i : pb.rig_xy + 2 * (i - nm);
+        const double* xy;
+        if (i < so) {
+            xy = pb.obs_xy + 2 * i;
+        } else if (i < go) {
+            xy = st->stereo_obs + 3 * (i - so);
+            m.obs_kind[k] = OBS_RECTIFIED;
+            m.obs_extra[2 * k] = xy[2];
+            m.obs_extra[2 * k + 1] = baseline;
+        } else if (i < ro) {
+            xy = st->general_obs + 4 * (i - go);
+            m.obs_kind[k] = OBS_GENERAL;
+            m.obs_extra[2 * k] = xy[2];
+            m.obs_extra[2 * k + 1] = xy[3];
+        } else {
+            xy = pb.rig_xy + 2 * (i - ro);
+        }
         m.obs_uv[2 * k] = xy[0];
         m.obs_uv[2 * k + 1] = xy[1];
-        if (m.any_rig) m.obs_sensor[k] = osensor[i];
+        if (per_sensor) m.obs_sensor[k] = osensor[i];
     }
     // Edges: runs of equal (pose, landmark).
     m.edge_pose.clear();

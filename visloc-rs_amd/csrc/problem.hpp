@@ -28,14 +28,16 @@
 namespace vba {
 
 // A lens plus the fixed extrinsic T_sensor<-rig as a rotation matrix and a
-// translation. Sensor 0 is the shared camera of monocular observations with
-// the identity extrinsic.
+// translation. Sensor 0 is the shared camera of monocular and stereo
+// observations with the identity extrinsic; then come the rig sensors, then
+// the right cameras of general stereo pairs (pinhole lens, T_right<-left),
+// whose `left` is the sensor of their left camera.
 struct DevSensor {
     Lens<double> lens;
     double R[9];
     double t[3];
     int32_t identity;
-    int32_t pad;
+    int32_t left;
 };
 
 struct CamTile {
@@ -61,6 +63,10 @@ struct LmTile {
 // pp = 256 c + s is g32_pos.
 constexpr int kG32Chunk = 1536;
 
+// Observation kinds: 2-row pixel residuals (monocular and rig), rectified
+// stereo (u_l, v_l, u_r) and general stereo (u_l, v_l, u_r, v_r).
+enum ObsKind : uint8_t { OBS_PIXEL = 0, OBS_RECTIFIED = 1, OBS_GENERAL = 2 };
+
 // Landmark tiles: runs of whole landmarks, each the work of one wave in the
 // operator kernel: at most kTileEdges edges (four per lane), kTileLandmarks
 // landmarks (one per lane) and kTileRuns camera runs (distinct poses, one
@@ -79,12 +85,17 @@ struct Model {
     std::vector<double> lms;                // 3 per slot
     std::vector<DevSensor> sensors;
     bool any_rig = false;
+    bool any_stereo = false;  // rectified or general stereo observations
     bool any_fixed_pose = false;
 
     uint64_t num_obs = 0;
-    std::vector<double> obs_uv;         // 2 per observation, c-order
-    std::vector<uint16_t> obs_sensor;   // c-order (empty when !any_rig)
-    std::vector<uint32_t> obs_input;    // c-order -> input index (mono, then rig)
+    std::vector<double> obs_uv;         // 2 per observation, c-order (the left pixel of a stereo one)
+    std::vector<uint16_t> obs_sensor;   // c-order (empty unless any_rig or any_stereo)
+    std::vector<uint8_t> obs_kind;      // c-order ObsKind (empty unless any_stereo)
+    std::vector<double> obs_extra;      // 2 per observation, c-order (empty unless any_stereo):
+                                        // rectified (u_r, baseline), general (u_r, v_r)
+    std::vector<uint32_t> obs_input;    // c-order -> flattened input index
+                                        // (mono, rectified stereo, general stereo, rig)
 
     uint64_t num_edges = 0;
     std::vector<int32_t> edge_pose, edge_lm;  // c-order
@@ -118,6 +129,13 @@ struct Model {
     std::vector<double> obs_w;     // external observation weights, c-order (empty: none)
     int32_t num_g32_chunks = 0;
 };
+
+// Observations of a problem, all kinds: the length of its flattened
+// observation order (mono, rectified stereo, general stereo, rig).
+inline uint64_t observation_count(const vba_problem& p) {
+    return p.num_observations + p.num_rig_observations +
+           (p.stereo ? p.stereo->num_stereo + p.stereo->num_general : 0);
+}
 
 // Lens of a vba_camera, resolved as MonoProjection::for_camera does
 // (jacobians.rs:29-58). Returns false for an unsupported model.

@@ -47,6 +47,11 @@ struct OwnedProblem {
     std::vector<uint8_t> pose_fixed, lm_fixed;
     std::vector<uint32_t> rig_sensor;
     std::vector<vba_sensor> sensors;
+    vba_stereo_terms stereo{};
+    std::vector<uint64_t> st_pose, st_lm, ge_pose, ge_lm;
+    std::vector<double> st_obs, ge_obs;
+    std::vector<uint32_t> ge_pair;
+    std::vector<vba_sensor> pairs;
 
     template <typename T>
     static void take(std::vector<T>& dst, const T* src, uint64_t n) {
@@ -84,6 +89,27 @@ struct OwnedProblem {
         view.rig_xy = ptr(rig_xy);
         view.rig_sensor = ptr(rig_sensor);
         view.sensors = ptr(sensors);
+        if (p.stereo) {
+            const vba_stereo_terms& t = *p.stereo;
+            take(st_pose, t.stereo_pose_ids, t.num_stereo);
+            take(st_lm, t.stereo_landmark_ids, t.num_stereo);
+            take(st_obs, t.stereo_obs, 3 * t.num_stereo);
+            take(ge_pose, t.general_pose_ids, t.num_general);
+            take(ge_lm, t.general_landmark_ids, t.num_general);
+            take(ge_obs, t.general_obs, 4 * t.num_general);
+            take(ge_pair, t.general_pair, t.num_general);
+            take(pairs, t.pairs, t.num_pairs);
+            stereo = t;
+            stereo.stereo_pose_ids = ptr(st_pose);
+            stereo.stereo_landmark_ids = ptr(st_lm);
+            stereo.stereo_obs = ptr(st_obs);
+            stereo.general_pose_ids = ptr(ge_pose);
+            stereo.general_landmark_ids = ptr(ge_lm);
+            stereo.general_obs = ptr(ge_obs);
+            stereo.general_pair = ptr(ge_pair);
+            stereo.pairs = ptr(pairs);
+            view.stereo = &stereo;
+        }
     }
 };
 
@@ -103,7 +129,7 @@ struct Request {
 // Device bytes a request holds while it runs (structure, working buffers
 // and, for a window, its dense system), an upper estimate.
 uint64_t estimate_bytes(const vba_problem& p, bool window) {
-    const uint64_t n = p.num_observations + p.num_rig_observations;
+    const uint64_t n = vba::observation_count(p);
     const uint64_t L = p.num_landmarks, P = p.num_poses;
     uint64_t b = n * 40 + n * 190 + L * 260 + P * 1400 + (64u << 10);
     if (window) {

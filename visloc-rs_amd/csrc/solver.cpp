@@ -30,7 +30,7 @@ enum BufId {
     B_VQ, B_PART64, B_PART21, B_BPART, B_CPART, B_PCG, B_DELTA_L, B_TILE_SCALAR,
     B_SCALARS, B_DENSE_BLOCK_PQ, B_DENSE_BLOCK_PTR, B_DENSE_TERMS, B_DENSE_VAR_SLOT, B_DENSE_S,
     B_DENSE_WORK, B_DENSE_STATUS, B_X_BLOCK_POS, B_X_ROW_PTR, B_X_COL, B_X_DIAG_POS, B_X_S64, B_X_S32, B_VD2,
-    B_G32_POS, B_OBS_W, B_OBS_S, B_COUNT
+    B_G32_POS, B_OBS_W, B_OBS_S, B_OBS_KIND, B_OBS_EXTRA, B_COUNT
 };
 
 // Profiled kernel families.
@@ -303,11 +303,19 @@ int32_t validate_matrix_free(const Model& m, const vba_options& o, std::string* 
             if (!std::isfinite(x)) return false;
         return L.fx > 0.0 && L.fy > 0.0;
     };
-    for (const DevSensor& s : m.sensors)
-        if (!finite_lens(s.lens)) {
+    // every sensor an observation uses (one without intrinsics is refused
+    // when the problem is built): lens, and the rig or stereo pair
+    // extrinsic (optimize.rs:572-622)
+    for (const DevSensor& s : m.sensors) {
+        if (s.lens.kind < 0) continue;
+        bool ok = finite_lens(s.lens);
+        for (double x : s.R) ok = ok && std::isfinite(x);
+        for (double x : s.t) ok = ok && std::isfinite(x);
+        if (!ok) {
             *msg = "camera calibration is unsupported or non-finite";
             return VBA_ERR_INELIGIBLE;
         }
+    }
     if (!m.any_fixed_pose) {
         *msg = "at least one existing pose must be fixed as a gauge anchor";
         return VBA_ERR_INELIGIBLE;
@@ -326,11 +334,12 @@ int32_t validate_matrix_free(const Model& m, const vba_options& o, std::string* 
             *msg = "pose or landmark state is non-finite";
             return VBA_ERR_INELIGIBLE;
         }
-    for (double v : m.obs_uv)
-        if (!std::isfinite(v)) {
-            *msg = "observation has an invalid pixel";
-            return VBA_ERR_INELIGIBLE;
-        }
+    for (const std::vector<double>* px : {&m.obs_uv, &m.obs_extra})
+        for (double v : *px)
+            if (!std::isfinite(v)) {
+                *msg = "observation has an invalid pixel";
+                return VBA_ERR_INELIGIBLE;
+            }
     return VBA_OK;
 }
 
@@ -397,7 +406,11 @@ int32_t prepare(Context& ctx, Prepared& pr, std::string* msg) {
         upload(pr, B_POSE_VAR, m.pose_var, s);
         upload(pr, B_LM_VAR, m.lm_var, s);
         upload(pr, B_OBS_UV, m.obs_uv, s);
-        if (m.any_rig) upload(pr, B_OBS_SENSOR, m.obs_sensor, s);
+        if (m.any_rig || m.any_stereo) upload(pr, B_OBS_SENSOR, m.obs_sensor, s);
+        if (m.any_stereo) {
+            upload(pr, B_OBS_KIND, m.obs_kind, s);
+            upload(pr, B_OBS_EXTRA, m.obs_extra, s);
+        }
         upload(pr, B_EDGE_LM, m.edge_lm, s);
         upload(pr, B_EDGE_OBS, m.edge_obs, s);
         upload(pr, B_PERM_CL, m.perm_cl, s);
@@ -522,7 +535,7 @@ DevProblem bind(const Context& ctx, Prepared& pr, const vba_options& o, bool mix
     P.num_obs = (int32_t)m.num_obs;
     P.num_cam_tiles = (int32_t)m.cam_tiles.size();
     P.num_lm_tiles = (int32_t)m.lm_tiles.size();
-    P.any_rig = m.any_rig;
+    P.any_rig = m.any_rig || m.any_stereo;  // per-observation sensors
     P.kernel_kind = (int32_t)o.robust_kernel;
     P.kernel_scale = o.robust_scale;
     const size_t NP = m.num_pose_slots, NL = m.num_lm_slots, NE = m.num_edges, NT = m.cam_tiles.size();
@@ -542,7 +555,9 @@ DevProblem bind(const Context& ctx, Prepared& pr, const vba_options& o, bool mix
     P.obs_uv = buf<double>(pr, B_OBS_UV);
     P.obs_w = m.obs_w.empty() ? nullptr : buf<double>(pr, B_OBS_W);
     P.obs_s = nullptr;
-    P.obs_sensor = m.any_rig ? buf<uint16_t>(pr, B_OBS_SENSOR) : nullptr;
+    P.obs_sensor = m.any_rig || m.any_stereo ? buf<uint16_t>(pr, B_OBS_SENSOR) : nullptr;
+    P.obs_kind = m.any_stereo ? buf<uint8_t>(pr, B_OBS_KIND) : nullptr;
+    P.obs_extra = m.any_stereo ? buf<double>(pr, B_OBS_EXTRA) : nullptr;
     P.edge_lm = buf<int32_t>(pr, B_EDGE_LM);
     P.edge_obs = buf<int32_t>(pr, B_EDGE_OBS);
     P.perm_cl = buf<int32_t>(pr, B_PERM_CL);
@@ -995,6 +1010,8 @@ void merge_models(const std::vector<const Model*>& ms, Model* out, std::vector<W
     Model& M = *out;
     M = Model();
     M.any_rig = true;
+    for (const Model* mw : ms)
+        if (mw->any_stereo) M.any_stereo = true;
     layout->clear();
     pose_win->clear();
     lm_win->clear();
@@ -1026,10 +1043,23 @@ void merge_models(const std::vector<const Model*>& ms, Model* out, std::vector<W
         }
         M.poses.insert(M.poses.end(), m.poses.begin(), m.poses.end());
         M.lms.insert(M.lms.end(), m.lms.begin(), m.lms.end());
-        M.sensors.insert(M.sensors.end(), m.sensors.begin(), m.sensors.end());
+        for (DevSensor d : m.sensors) {
+            d.left += (int32_t)so;
+            M.sensors.push_back(d);
+        }
         M.obs_uv.insert(M.obs_uv.end(), m.obs_uv.begin(), m.obs_uv.end());
+        const bool per_sensor = m.any_rig || m.any_stereo;
         for (uint64_t o = 0; o < m.num_obs; ++o)
-            M.obs_sensor.push_back((uint16_t)(so + (m.any_rig ? m.obs_sensor[o] : 0)));
+            M.obs_sensor.push_back((uint16_t)(so + (per_sensor ? m.obs_sensor[o] : 0)));
+        if (M.any_stereo) {
+            if (m.any_stereo) {
+                M.obs_kind.insert(M.obs_kind.end(), m.obs_kind.begin(), m.obs_kind.end());
+                M.obs_extra.insert(M.obs_extra.end(), m.obs_extra.begin(), m.obs_extra.end());
+            } else {
+                M.obs_kind.insert(M.obs_kind.end(), m.num_obs, OBS_PIXEL);
+                M.obs_extra.insert(M.obs_extra.end(), 2 * m.num_obs, 0.0);
+            }
+        }
         for (uint64_t e = 0; e < m.num_edges; ++e) {
             M.edge_pose.push_back((int32_t)(m.edge_pose[e] + po));
             M.edge_lm.push_back((int32_t)(m.edge_lm[e] + lo));
@@ -1171,6 +1201,8 @@ int32_t prepare_batch(Context& ctx, std::vector<Model>& models, const std::vecto
         // the structure now lives on the device; keep only what outputs need
         pb.m.obs_uv = std::vector<double>();
         pb.m.obs_sensor = std::vector<uint16_t>();
+        pb.m.obs_kind = std::vector<uint8_t>();
+        pb.m.obs_extra = std::vector<double>();
         pb.m.obs_input = std::vector<uint32_t>();
         pb.solved = false;
         return VBA_OK;

@@ -7,13 +7,12 @@ from . import abi
 from .abi import library
 from .api import (BatchOutput, BundleProblem, Device, PreparedBatch, PreparedProblem, PrepareError, ServedResult,
                   Server, SolveOutput, Synthetic, camera, camera_centres, check, default_gnc_options, default_options,
-                  run_solver,
-                  synth_params, synthesize)
+                  run_solver, sensor, synth_params, synthesize)
 from .io import IoError, Model
 
 __all__ = [
     "abi", "library", "BatchOutput", "BundleProblem", "Device", "PreparedBatch", "PreparedProblem", "PrepareError",
     "ServedResult", "Server", "SolveOutput", "Synthetic", "camera", "camera_centres", "check", "default_gnc_options",
     "default_options",
-    "run_solver", "synth_params", "synthesize", "IoError", "Model",
+    "run_solver", "sensor", "synth_params", "synthesize", "IoError", "Model",
 ]

@@ -9,7 +9,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 OK = 0
 DECLINED = 1
@@ -90,6 +90,25 @@ class Sensor(C.Structure):
     _fields_ = [("camera", Camera), ("sensor_from_rig", C.c_double * 7)]
 
 
+class StereoTerms(C.Structure):
+    """vba_stereo_terms: rectified (u_l, v_l, u_r) and general (left x, y,
+    right x, y) stereo observations."""
+    _fields_ = [
+        ("baseline", C.c_double),
+        ("num_stereo", C.c_uint64),
+        ("stereo_pose_ids", PU64),
+        ("stereo_landmark_ids", PU64),
+        ("stereo_obs", P64),
+        ("num_general", C.c_uint64),
+        ("general_pose_ids", PU64),
+        ("general_landmark_ids", PU64),
+        ("general_obs", P64),
+        ("general_pair", PU32),
+        ("num_pairs", C.c_uint64),
+        ("pairs", C.POINTER(Sensor)),
+    ]
+
+
 class Problem(C.Structure):
     _fields_ = [
         ("camera", Camera),
@@ -114,6 +133,7 @@ class Problem(C.Structure):
         ("sensors", C.POINTER(Sensor)),
         ("declined_features", C.c_uint32),
         ("reserved", C.c_uint32),
+        ("stereo", C.POINTER(StereoTerms)),
     ]
 
 

@@ -29,6 +29,16 @@ def camera(model: int, params, width: int = 0, height: int = 0) -> abi.Camera:
     return cam
 
 
+def sensor(cam: abi.Camera, transform) -> abi.Sensor:
+    """A rig sensor (camera, T_sensor<-rig) or a general stereo pair (right
+    camera, T_right<-left); the transform is (qw qx qy qz tx ty tz)."""
+    s = abi.Sensor()
+    s.camera = cam
+    for i, v in enumerate(np.asarray(transform, np.float64).reshape(7)):
+        s.sensor_from_rig[i] = float(v)
+    return s
+
+
 def _ptr(a: Optional[np.ndarray], ctype):
     if a is None or a.size == 0:
         return C.cast(None, C.POINTER(ctype))
@@ -53,6 +63,18 @@ class BundleProblem:
     rig_sensor: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint32))
     sensors: List[abi.Sensor] = field(default_factory=list)
     declined_features: int = 0
+    # rectified stereo (BaStereoObservation) and its baseline
+    stereo_pose_ids: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint64))
+    stereo_landmark_ids: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint64))
+    stereo_obs: np.ndarray = field(default_factory=lambda: np.zeros((0, 3)))      # u_l, v_l, u_r
+    stereo_baseline: float = float("nan")                                         # None in the reference
+    # general stereo (BaGeneralStereoObservation): pair q = (right camera,
+    # T_right<-left) as an abi.Sensor
+    general_pose_ids: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint64))
+    general_landmark_ids: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint64))
+    general_obs: np.ndarray = field(default_factory=lambda: np.zeros((0, 4)))     # left x, y, right x, y
+    general_pair: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint32))
+    pairs: List[abi.Sensor] = field(default_factory=list)
 
     def __post_init__(self):
         self.pose_ids = np.ascontiguousarray(self.pose_ids, np.uint64)
@@ -69,10 +91,25 @@ class BundleProblem:
         self.rig_xy = np.ascontiguousarray(self.rig_xy, np.float64).reshape(-1, 2)
         self.rig_sensor = np.ascontiguousarray(self.rig_sensor, np.uint32)
         self._sensor_array = (abi.Sensor * max(1, len(self.sensors)))(*self.sensors)
+        self.stereo_pose_ids = np.ascontiguousarray(self.stereo_pose_ids, np.uint64)
+        self.stereo_landmark_ids = np.ascontiguousarray(self.stereo_landmark_ids, np.uint64)
+        self.stereo_obs = np.ascontiguousarray(self.stereo_obs, np.float64).reshape(-1, 3)
+        self.general_pose_ids = np.ascontiguousarray(self.general_pose_ids, np.uint64)
+        self.general_landmark_ids = np.ascontiguousarray(self.general_landmark_ids, np.uint64)
+        self.general_obs = np.ascontiguousarray(self.general_obs, np.float64).reshape(-1, 4)
+        self.general_pair = np.ascontiguousarray(self.general_pair, np.uint32)
+        self._pair_array = (abi.Sensor * max(1, len(self.pairs)))(*self.pairs)
+
+    @property
+    def has_stereo(self) -> bool:
+        return len(self.stereo_pose_ids) > 0 or len(self.general_pose_ids) > 0
 
     @property
     def num_observations(self) -> int:
-        return len(self.obs_pose_ids) + len(self.rig_pose_ids)
+        """All observations, in the flattened order mono, rectified stereo,
+        general stereo, rig."""
+        return (len(self.obs_pose_ids) + len(self.stereo_pose_ids) + len(self.general_pose_ids) +
+                len(self.rig_pose_ids))
 
     def view(self) -> abi.Problem:
         """The C view; valid while this object is alive and unmodified."""
@@ -98,6 +135,23 @@ class BundleProblem:
         v.num_sensors = len(self.sensors)
         v.sensors = C.cast(self._sensor_array, C.POINTER(abi.Sensor)) if self.sensors else None
         v.declined_features = self.declined_features
+        if self.has_stereo or np.isfinite(self.stereo_baseline):
+            # one StereoTerms per problem, refreshed in place, so that every
+            # view handed out keeps pointing at live memory
+            t = self.__dict__.setdefault("_stereo_terms", abi.StereoTerms())
+            t.baseline = self.stereo_baseline
+            t.num_stereo = len(self.stereo_pose_ids)
+            t.stereo_pose_ids = _ptr(self.stereo_pose_ids, C.c_uint64)
+            t.stereo_landmark_ids = _ptr(self.stereo_landmark_ids, C.c_uint64)
+            t.stereo_obs = _ptr(self.stereo_obs, C.c_double)
+            t.num_general = len(self.general_pose_ids)
+            t.general_pose_ids = _ptr(self.general_pose_ids, C.c_uint64)
+            t.general_landmark_ids = _ptr(self.general_landmark_ids, C.c_uint64)
+            t.general_obs = _ptr(self.general_obs, C.c_double)
+            t.general_pair = _ptr(self.general_pair, C.c_uint32)
+            t.num_pairs = len(self.pairs)
+            t.pairs = C.cast(self._pair_array, C.POINTER(abi.Sensor)) if self.pairs else None
+            v.stereo = C.pointer(t)
         return v
 
     def copy(self) -> "BundleProblem":
@@ -109,7 +163,11 @@ class BundleProblem:
             obs_xy=self.obs_xy.copy(), rig_pose_ids=self.rig_pose_ids.copy(),
             rig_landmark_ids=self.rig_landmark_ids.copy(), rig_xy=self.rig_xy.copy(),
             rig_sensor=self.rig_sensor.copy(), sensors=list(self.sensors),
-            declined_features=self.declined_features)
+            declined_features=self.declined_features, stereo_pose_ids=self.stereo_pose_ids.copy(),
+            stereo_landmark_ids=self.stereo_landmark_ids.copy(), stereo_obs=self.stereo_obs.copy(),
+            stereo_baseline=self.stereo_baseline, general_pose_ids=self.general_pose_ids.copy(),
+            general_landmark_ids=self.general_landmark_ids.copy(), general_obs=self.general_obs.copy(),
+            general_pair=self.general_pair.copy(), pairs=list(self.pairs))
 
     def with_state(self, poses: np.ndarray, landmarks: np.ndarray) -> "BundleProblem":
         p = self.copy()
