@@ -224,7 +224,8 @@ typedef struct vba_options {
     uint32_t pcg_maxit_policy;       /* enum vba_pcg_maxit_policy */
     double pcg_relative_tolerance;
     double pcg_absolute_tolerance;
-    uint32_t refine_intrinsics;      /* nonzero: declined (joint intrinsics) */
+    uint32_t refine_intrinsics;      /* enum vba_refine bits; vba_optimize declines them,
+                                        vba_optimize_intrinsics runs the joint solve */
     uint32_t flags;                  /* enum vba_flags, 0 by default */
 } vba_options;
 
@@ -293,6 +294,32 @@ int32_t vba_optimize(vba_context* ctx, const vba_problem* problem,
                      const vba_options* options, double* poses_out,
                      double* landmarks_out, vba_iteration_stats* stats,
                      uint32_t stats_capacity, vba_result* result);
+
+/* vba_options.refine_intrinsics: BaConfig::refine_intrinsics,
+ * refine_distortion, refine_tangential_distortion and shared_focal
+ * (bundle/mod.rs:655-700). */
+enum vba_refine {
+    VBA_REFINE_INTRINSICS = 1u << 0,   /* (fx, fy, cx, cy) */
+    VBA_REFINE_DISTORTION = 1u << 1,   /* + (k1, k2), monocular problems only */
+    VBA_REFINE_TANGENTIAL = 1u << 2,   /* + (p1, p2) with DISTORTION; the camera becomes OPENCV */
+    VBA_REFINE_SHARED_FOCAL = 1u << 3  /* fx == fy, starting from their mean */
+};
+
+/* BundleAdjustment::optimize with refine_intrinsics set (optimize.rs:8-28):
+ * for a PINHOLE or OPENCV camera with intrinsics and no general stereo or
+ * rig observations, the joint pose / landmark / camera solve
+ * (optimize_joint_intrinsics, joint_intrinsics.rs:24-353): the 4, 6 or 8
+ * camera parameters border the dense reduced camera system (the Schur
+ * complement also eliminates the landmarks from the camera block), always
+ * with the dense solver, and the camera is refined even when every pose and
+ * landmark is fixed. Otherwise, as the reference does, the pose / structure
+ * solve of vba_optimize with refine_intrinsics cleared. camera_out receives
+ * the refined camera: its model (OPENCV when TANGENTIAL promoted it), its
+ * parameter count (at least 6 with DISTORTION, 8 with TANGENTIAL) and
+ * values; it is the input camera when the camera is not refined. */
+int32_t vba_optimize_intrinsics(vba_context* ctx, const vba_problem* problem, const vba_options* options,
+                                double* poses_out, double* landmarks_out, vba_camera* camera_out,
+                                vba_iteration_stats* stats, uint32_t stats_capacity, vba_result* result);
 
 /* BundleAdjustment::optimize_with_observation_weights (optimize.rs:43-53):
  * `weights` holds num_weights values, one per observation in the reference's

@@ -34,6 +34,7 @@
 #include <limits>
 #include <string>
 #include <thread>
+#include <map>
 #include <vector>
 
 namespace {
@@ -1783,6 +1784,392 @@ int32_t run_lm(Problem& P, const vba_options& o, int threads, vba_iteration_stat
     return VBA_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Joint pose / landmark / intrinsics refinement: optimize_joint_intrinsics
+// (joint_intrinsics.rs:24-353) with its dense camera system
+// (build_joint_intrinsics_system, :593-825; the monocular terms :356-465).
+// ---------------------------------------------------------------------------
+struct JointLm {
+    M3 h_ll;
+    double b_l[3];
+    std::map<size_t, std::vector<double>> cross;  // column start -> rows x 3, row-major
+};
+void joint_add_cross(JointLm& L, size_t cs, int rows, const double* blk) {
+    auto it = L.cross.find(cs);
+    if (it == L.cross.end()) {
+        L.cross.emplace(cs, std::vector<double>(blk, blk + 3 * rows));
+        return;
+    }
+    for (int i = 0; i < 3 * rows; ++i) it->second[i] += blk[i];
+}
+// joint_mono_terms (:356-465): residual, J_pose, J_landmark and the camera
+// columns J_k (2 x k_dim, row-major) of a monocular observation.
+bool joint_mono_terms(const Problem& P, const Obs& o, const Intr& in, int k_dim, bool has_dist, double k1,
+                      double k2, bool has_tan, double r[2], M26* jp, M23* jl, double* jk) {
+    const SE3& T = P.poses[o.pose];
+    const V3& X = P.lms[o.lm];
+    const V3 xc = se3_apply(T, X);
+    if (xc.z <= 0.0) return false;
+    const double fx = in.fx, fy = in.fy;
+    const double z_inv = 1.0 / xc.z;
+    const double x = xc.x * z_inv, y = xc.y * z_inv;
+    const double r2 = x * x + y * y;
+    if (!has_dist) k1 = k2 = 0.0;
+    const double d = 1.0 + k1 * r2 + k2 * r2 * r2;
+    const double g = k1 + 2.0 * k2 * r2;
+    double xd = x * d, yd = y * d;
+    r[0] = (fx * xd + in.cx) - o.x;
+    r[1] = (fy * yd + in.cy) - o.y;
+    const M3 R = qmatrix(T.q);
+    const double d11 = d + 2.0 * x * x * g, d12 = 2.0 * x * y * g, d22 = d + 2.0 * y * y * g;
+    M23 jpi = M23::zeros();
+    jpi(0, 0) = fx * d11 * z_inv;
+    jpi(0, 1) = fx * d12 * z_inv;
+    jpi(0, 2) = -fx * (d11 * x + d12 * y) * z_inv;
+    jpi(1, 0) = fy * d12 * z_inv;
+    jpi(1, 1) = fy * d22 * z_inv;
+    jpi(1, 2) = -fy * (d12 * x + d22 * y) * z_inv;
+    const M36 dx = pose_point_jacobian(R, X);
+    *jp = mul(jpi, dx);
+    *jl = mul(jpi, R);
+    for (int i = 0; i < 2 * k_dim; ++i) jk[i] = 0.0;
+    if (has_tan) {
+        double uv[2];
+        M23 J;
+        if (!lens_project_with_jacobian(P.camera, xc, uv, &J)) return false;
+        r[0] = uv[0] - o.x;
+        r[1] = uv[1] - o.y;
+        *jp = mul(J, dx);
+        *jl = mul(J, R);
+        const double p1 = P.camera.np > 6 ? P.camera.p[6] : 0.0, p2 = P.camera.np > 7 ? P.camera.p[7] : 0.0;
+        const double xy2 = 2.0 * x * y;
+        xd = x * d + p1 * xy2 + p2 * (r2 + 2.0 * x * x);
+        yd = y * d + p1 * (r2 + 2.0 * y * y) + p2 * xy2;
+    }
+    jk[0] = xd;
+    jk[2] = 1.0;
+    jk[k_dim + 1] = yd;
+    jk[k_dim + 3] = 1.0;
+    if (k_dim >= 6) {
+        jk[4] = fx * x * r2;
+        jk[5] = fx * x * r2 * r2;
+        jk[k_dim + 4] = fy * y * r2;
+        jk[k_dim + 5] = fy * y * r2 * r2;
+    }
+    if (has_tan && k_dim == 8) {
+        const double xy2 = 2.0 * x * y;
+        jk[6] = fx * xy2;
+        jk[7] = fx * (r2 + 2.0 * x * x);
+        jk[k_dim + 6] = fy * (r2 + 2.0 * y * y);
+        jk[k_dim + 7] = fy * xy2;
+    }
+    return true;
+}
+// One observation's blocks into the dense camera system (rows x 6 J_p,
+// rows x 3 J_l, rows x kd J_k, row-major).
+void joint_accumulate(const Problem& P, const Obs& o, const Kernel& K, int rows, const double* r, const double* jp,
+                      const double* jl, const double* jk, int kd, size_t k_off, size_t cam_dim,
+                      std::vector<double>& H, std::vector<double>& b, std::vector<JointLm>& lms) {
+    double s = 0.0;
+    for (int i = 0; i < rows; ++i) s += r[i] * r[i];
+    const double w = K.weight(s);
+    auto dot_cols = [&](const double* A, int na, int a, const double* B, int nb, int c) {
+        double acc = 0.0;
+        for (int i = 0; i < rows; ++i) acc += A[i * na + a] * B[i * nb + c];
+        return acc;
+    };
+    for (int a = 0; a < kd; ++a) {
+        for (int c = 0; c < kd; ++c) H[(k_off + a) * cam_dim + k_off + c] += w * dot_cols(jk, kd, a, jk, kd, c);
+        double acc = 0.0;
+        for (int i = 0; i < rows; ++i) acc += jk[i * kd + a] * r[i];
+        b[k_off + a] += w * acc;
+    }
+    const int32_t p = P.pose_var[o.pose], l = P.lm_var[o.lm];
+    if (p >= 0) {
+        const size_t p0 = 6 * (size_t)p;
+        for (int a = 0; a < 6; ++a) {
+            for (int c = 0; c < 6; ++c) H[(p0 + a) * cam_dim + p0 + c] += w * dot_cols(jp, 6, a, jp, 6, c);
+            double acc = 0.0;
+            for (int i = 0; i < rows; ++i) acc += jp[i * 6 + a] * r[i];
+            b[p0 + a] += w * acc;
+            for (int c = 0; c < kd; ++c) {
+                const double v = w * dot_cols(jp, 6, a, jk, kd, c);
+                H[(p0 + a) * cam_dim + k_off + c] += v;
+                H[(k_off + c) * cam_dim + p0 + a] += v;
+            }
+        }
+    }
+    if (l >= 0) {
+        JointLm& L = lms[l];
+        for (int a = 0; a < 3; ++a) {
+            for (int c = 0; c < 3; ++c) L.h_ll(a, c) += w * dot_cols(jl, 3, a, jl, 3, c);
+            double acc = 0.0;
+            for (int i = 0; i < rows; ++i) acc += jl[i * 3 + a] * r[i];
+            L.b_l[a] += w * acc;
+        }
+        if (p >= 0) {
+            double cr[18];
+            for (int a = 0; a < 6; ++a)
+                for (int c = 0; c < 3; ++c) cr[a * 3 + c] = w * dot_cols(jp, 6, a, jl, 3, c);
+            joint_add_cross(L, 6 * (size_t)p, 6, cr);
+        }
+        double ck[24];
+        for (int a = 0; a < kd; ++a)
+            for (int c = 0; c < 3; ++c) ck[a * 3 + c] = w * dot_cols(jk, kd, a, jl, 3, c);
+        joint_add_cross(L, k_off, kd, ck);
+    }
+}
+
+constexpr uint32_t kRefineIntrinsics = 1, kRefineDistortion = 2, kRefineTangential = 4, kSharedFocal = 8;
+
+int32_t run_joint(Problem& P, const vba_options& o, vba_iteration_stats* stats, uint32_t cap, vba_result* res) {
+    const Kernel K{o.robust_kernel, o.robust_scale};
+    const uint32_t flags = o.refine_intrinsics;
+    bool any_stereo = false;
+    for (const Obs& ob : P.obs)
+        if (ob.kind == OBS_RECTIFIED) any_stereo = true;
+    const size_t k_off = 6 * (size_t)P.P;
+    Cam& cam = P.camera;
+    const bool refine_dist = (flags & kRefineDistortion) &&
+                             (cam.model == VBA_CAMERA_PINHOLE || cam.model == VBA_CAMERA_OPENCV) && !any_stereo;
+    if (refine_dist)
+        while (cam.np < 6) cam.p[cam.np++] = 0.0;
+    const bool refine_tan = refine_dist && (flags & kRefineTangential);
+    if (refine_tan) {
+        cam.model = VBA_CAMERA_OPENCV;
+        while (cam.np < 8) cam.p[cam.np++] = 0.0;
+    }
+    const int k_dim = refine_tan ? 8 : refine_dist ? 6 : 4;
+    const size_t cam_dim = k_off + k_dim;
+    const bool shared = flags & kSharedFocal;
+    if (shared) {
+        const double f = 0.5 * (cam.p[0] + cam.p[1]);
+        cam.p[0] = f;
+        cam.p[1] = f;
+    }
+    double initial_cost;
+    uint64_t current_nonproj;
+    cost_and_nonprojectable(P, K, &initial_cost, &current_nonproj);
+    double current_cost = initial_cost;
+    double lambda = o.has_initial_lambda ? o.initial_lambda : 0.0;
+    bool converged = false;
+    uint32_t written = 0, linearizations = 0;
+    auto push = [&](const vba_iteration_stats& s) {
+        if (written < cap && stats) stats[written] = s;
+        ++written;
+    };
+    for (uint32_t iteration = 0; iteration < o.max_iterations; ++iteration) {
+        double k1 = 0.0, k2 = 0.0;
+        const bool has_dist = radial_distortion(cam, &k1, &k2);
+        const bool has_tan = refine_tan || tangential_distortion(cam);
+        Intr in;
+        intrinsics(cam, &in);
+        // build_joint_intrinsics_system, dense
+        std::vector<double> H(cam_dim * cam_dim, 0.0), bc(cam_dim, 0.0);
+        std::vector<JointLm> lms(P.L);
+        for (JointLm& L : lms) {
+            L.h_ll = M3::zeros();
+            L.b_l[0] = L.b_l[1] = L.b_l[2] = 0.0;
+        }
+        for (const Obs& ob : P.obs) {
+            if (ob.kind != OBS_MONO) continue;
+            double r[2], jk[16];
+            M26 jp;
+            M23 jl;
+            if (!joint_mono_terms(P, ob, in, k_dim, has_dist, k1, k2, has_tan, r, &jp, &jl, jk)) continue;
+            joint_accumulate(P, ob, K, 2, r, jp.a, jl.a, jk, k_dim, k_off, cam_dim, H, bc, lms);
+        }
+        for (const Obs& ob : P.obs) {
+            if (ob.kind != OBS_RECTIFIED) continue;
+            const SE3& T = P.poses[ob.pose];
+            const V3& X = P.lms[ob.lm];
+            const V3 xc = se3_apply(T, X);
+            if (xc.z <= 0.0) continue;
+            double uv[2];
+            if (!project_pinhole(in, xc, uv)) continue;
+            const double b = P.baseline;
+            const double z_inv = 1.0 / xc.z, z_inv2 = z_inv * z_inv;
+            const double u_r = uv[0] - in.fx * b * z_inv;
+            const double r[3] = {uv[0] - ob.x, uv[1] - ob.y, u_r - ob.x2};
+            const M3 R = qmatrix(T.q);
+            M3 jpi = M3::zeros();
+            jpi(0, 0) = in.fx * z_inv;
+            jpi(0, 2) = -in.fx * xc.x * z_inv2;
+            jpi(1, 1) = in.fy * z_inv;
+            jpi(1, 2) = -in.fy * xc.y * z_inv2;
+            jpi(2, 0) = in.fx * z_inv;
+            jpi(2, 2) = -in.fx * (xc.x - b) * z_inv2;
+            const M36 jp = mul(jpi, pose_point_jacobian(R, X));
+            const M3 jl = mul(jpi, R);
+            double jk[12] = {xc.x * z_inv, 0, 1, 0, 0, xc.y * z_inv, 0, 1, (xc.x - b) * z_inv, 0, 1, 0};
+            joint_accumulate(P, ob, K, 3, r, jp.a, jl.a, jk, 4, k_off, cam_dim, H, bc, lms);
+        }
+        ++linearizations;
+        // damped Schur reduction (joint_intrinsics.rs:143-185)
+        std::vector<double> S = H;
+        if (lambda > 0.0)
+            for (size_t d = 0; d < cam_dim; ++d) S[d * cam_dim + d] += lambda;
+        std::vector<double> bred(cam_dim);
+        for (size_t i = 0; i < cam_dim; ++i) bred[i] = -bc[i];
+        std::vector<bool> has_inv(P.L, false);
+        std::vector<M3> invs(P.L);
+        for (uint32_t l = 0; l < P.L; ++l) {
+            M3 h = lms[l].h_ll;
+            if (lambda > 0.0) {
+                h(0, 0) += lambda;
+                h(1, 1) += lambda;
+                h(2, 2) += lambda;
+            }
+            has_inv[l] = try_inverse3(h, &invs[l]);
+            if (!has_inv[l]) continue;
+            const M3& inv = invs[l];
+            for (const auto& ea : lms[l].cross) {
+                const int ra = (int)ea.second.size() / 3;
+                std::vector<double> ah(3 * ra);
+                for (int i = 0; i < ra; ++i)
+                    for (int c = 0; c < 3; ++c) {
+                        double acc = 0.0;
+                        for (int t = 0; t < 3; ++t) acc += ea.second[i * 3 + t] * inv(t, c);
+                        ah[i * 3 + c] = acc;
+                    }
+                for (const auto& eb : lms[l].cross) {
+                    const int rb = (int)eb.second.size() / 3;
+                    for (int i = 0; i < ra; ++i)
+                        for (int j = 0; j < rb; ++j) {
+                            double acc = 0.0;
+                            for (int t = 0; t < 3; ++t) acc += ah[i * 3 + t] * eb.second[j * 3 + t];
+                            S[(ea.first + i) * cam_dim + eb.first + j] -= acc;
+                        }
+                }
+                for (int i = 0; i < ra; ++i) {
+                    double acc = 0.0;
+                    for (int t = 0; t < 3; ++t) acc += ah[i * 3 + t] * lms[l].b_l[t];
+                    bred[ea.first + i] += acc;
+                }
+            }
+        }
+        std::vector<double> delta;
+        bool ok;
+        if (shared) {
+            // fy folded into fx: S' = T^T S T, b' = T^T b, delta = T delta'
+            const size_t fyi = k_off + 1, n1 = cam_dim - 1;
+            auto map = [&](size_t i) { return i < fyi ? i : i == fyi ? k_off : i - 1; };
+            std::vector<double> Sr(n1 * n1, 0.0), br(n1, 0.0), dr;
+            for (size_t i = 0; i < cam_dim; ++i) {
+                br[map(i)] += bred[i];
+                for (size_t j = 0; j < cam_dim; ++j) Sr[map(i) * n1 + map(j)] += S[i * cam_dim + j];
+            }
+            ok = solve_normal_equations(Sr, n1, br, &dr);
+            if (ok) {
+                delta.resize(cam_dim);
+                for (size_t i = 0; i < cam_dim; ++i) delta[i] = dr[map(i)];
+            }
+        } else {
+            ok = solve_normal_equations(S, cam_dim, bred, &delta);
+        }
+        vba_iteration_stats st;
+        std::memset(&st, 0, sizeof(st));
+        st.iteration = iteration;
+        st.pcg_iterations = -1;
+        const double cost_before = current_cost;
+        if (!ok) {
+            lambda = std::min(lambda * o.lambda_increase_factor, o.max_lambda);
+            st.cost_before = st.cost_after = cost_before;
+            st.lambda = lambda;
+            st.nonprojectable_after = (uint32_t)current_nonproj;
+            push(st);
+            if (lambda >= o.max_lambda) break;
+            continue;
+        }
+        // back-substitution (:219-233)
+        std::vector<V3> dl(P.L, V3{0, 0, 0});
+        for (uint32_t l = 0; l < P.L; ++l) {
+            if (!has_inv[l]) continue;
+            double acc[3] = {-lms[l].b_l[0], -lms[l].b_l[1], -lms[l].b_l[2]};
+            for (const auto& ea : lms[l].cross) {
+                const int ra = (int)ea.second.size() / 3;
+                for (int c = 0; c < 3; ++c) {
+                    double t = 0.0;
+                    for (int i = 0; i < ra; ++i) t += ea.second[i * 3 + c] * delta[ea.first + i];
+                    acc[c] -= t;
+                }
+            }
+            const M3& inv = invs[l];
+            dl[l] = {inv(0, 0) * acc[0] + inv(0, 1) * acc[1] + inv(0, 2) * acc[2],
+                     inv(1, 0) * acc[0] + inv(1, 1) * acc[1] + inv(1, 2) * acc[2],
+                     inv(2, 0) * acc[0] + inv(2, 1) * acc[1] + inv(2, 2) * acc[2]};
+        }
+        const std::vector<SE3> saved_poses = P.poses;
+        const std::vector<V3> saved_lms = P.lms;
+        const Cam saved_cam = cam;
+        double max_pose_step = 0.0, max_landmark_step = 0.0;
+        for (size_t s = 0; s < P.poses.size(); ++s) {
+            const int32_t i = P.pose_var[s];
+            if (i < 0) continue;
+            const double* xi = delta.data() + 6 * (size_t)i;
+            max_pose_step = std::max(max_pose_step, norm6_seq(xi));
+            P.poses[s] = se3_compose(P.poses[s], se3_exp(xi));
+        }
+        for (size_t s = 0; s < P.lms.size(); ++s) {
+            const int32_t i = P.lm_var[s];
+            if (i < 0 || !has_inv[i]) continue;
+            const V3 v = dl[i];
+            max_landmark_step = std::max(max_landmark_step, std::sqrt(norm_sq3(v)));
+            P.lms[s] = {P.lms[s].x + v.x, P.lms[s].y + v.y, P.lms[s].z + v.z};
+        }
+        for (int j = 0; j < k_dim; ++j) cam.p[j] += delta[k_off + j];
+        double cost_after;
+        uint64_t nonproj_after;
+        cost_and_nonprojectable(P, K, &cost_after, &nonproj_after);
+        const bool cost_accepted = o.has_initial_lambda ? (cost_after < cost_before) : true;
+        const bool accepted = cost_accepted && nonproj_after <= current_nonproj;
+        st.cost_before = cost_before;
+        st.cost_after = cost_after;
+        st.max_pose_step = max_pose_step;
+        st.max_landmark_step = max_landmark_step;
+        st.nonprojectable_after = (uint32_t)nonproj_after;
+        if (!accepted) {
+            P.poses = saved_poses;
+            P.lms = saved_lms;
+            cam = saved_cam;
+            lambda = std::min(lambda * o.lambda_increase_factor, o.max_lambda);
+            st.lambda = lambda;
+            push(st);
+            if (!o.has_initial_lambda) break;
+            if (lambda >= o.max_lambda) break;
+            continue;
+        }
+        st.lambda = lambda;
+        st.step_accepted = 1;
+        push(st);
+        current_cost = cost_after;
+        current_nonproj = nonproj_after;
+        if (o.has_initial_lambda) lambda = std::max(lambda * o.lambda_decrease_factor, o.min_lambda);
+        if (max_pose_step < o.step_tolerance && max_landmark_step < o.step_tolerance) {
+            converged = true;
+            break;
+        }
+        if (std::fabs(cost_before - cost_after) < o.cost_tolerance) {
+            converged = true;
+            break;
+        }
+        if (o.has_relative_cost_tolerance && std::isfinite(o.relative_cost_tolerance) &&
+            o.relative_cost_tolerance >= 0.0 &&
+            (cost_before - cost_after) / std::max(std::fabs(cost_before), std::numeric_limits<double>::epsilon()) <
+                o.relative_cost_tolerance) {
+            converged = true;
+            break;
+        }
+    }
+    res->initial_cost = initial_cost;
+    res->final_cost = current_cost;
+    res->num_iterations = std::min(written, cap);
+    res->converged = converged;
+    res->num_observations_used = P.obs.size();
+    res->linearizations = linearizations;
+    return VBA_OK;
+}
+
 // Build a Normal from the C normal-system view.
 void normal_from_view(const oracle_normal_system* v, Normal* ne) {
     ne->P = v->num_poses;
@@ -1946,6 +2333,48 @@ int32_t oracle_optimize(const vba_problem* problem, const vba_options* options, 
     st = optimize_loaded(P, *options, threads, stats, stats_capacity, result);
     if (st != VBA_OK) return st;
     write_state(problem, P, poses_out, landmarks_out);
+    return VBA_OK;
+}
+
+int32_t oracle_optimize_intrinsics(const vba_problem* problem, const vba_options* options, double* poses_out,
+                                   double* landmarks_out, vba_camera* camera_out, vba_iteration_stats* stats,
+                                   uint32_t stats_capacity, vba_result* result, int32_t threads) {
+    if (!problem || !options || !result || !camera_out) return VBA_ERR_INVALID_ARGUMENT;
+    std::memset(result, 0, sizeof(*result));
+    *camera_out = problem->camera;
+    if (problem->declined_features) return VBA_DECLINED;
+    const vba_stereo_terms* st3 = problem->stereo;
+    const bool general = st3 && st3->num_general;
+    const int model = problem->camera.model;
+    Intr in0;
+    // BundleAdjustment::optimize (optimize.rs:8-28): the joint solve for a
+    // PINHOLE / OPENCV camera with intrinsics and no general stereo or rig
+    // observations, else the pose / structure solve
+    const bool joint = (options->refine_intrinsics & kRefineIntrinsics) && !general &&
+                       problem->num_rig_observations == 0 &&
+                       (model == VBA_CAMERA_PINHOLE || model == VBA_CAMERA_OPENCV) &&
+                       intrinsics(to_cam(problem->camera), &in0);
+    Problem P;
+    uint64_t error_id = 0;
+    int32_t st = load_problem(problem, &P, &error_id);
+    result->error_id = error_id;
+    if (!joint) {
+        if (st != VBA_OK) return st;
+        vba_options plain = *options;
+        plain.refine_intrinsics = 0;
+        st = optimize_loaded(P, plain, threads, stats, stats_capacity, result);
+        if (st != VBA_OK) return st;
+        write_state(problem, P, poses_out, landmarks_out);
+        return VBA_OK;
+    }
+    // the joint solve refines the camera even with every pose and landmark fixed
+    if (st != VBA_OK && st != VBA_ERR_ALL_POSES_FIXED) return st;
+    st = run_joint(P, *options, stats, stats_capacity, result);
+    if (st != VBA_OK) return st;
+    write_state(problem, P, poses_out, landmarks_out);
+    camera_out->model = P.camera.model;
+    camera_out->num_params = (uint32_t)P.camera.np;
+    for (int i = 0; i < 12; ++i) camera_out->params[i] = i < P.camera.np ? P.camera.p[i] : 0.0;
     return VBA_OK;
 }
 

@@ -33,6 +33,14 @@ int32_t oracle_optimize(const vba_problem* problem, const vba_options* options,
  * 628-650): the external weight multiplies each observation's robust weight
  * (normal_equations.rs:458, 627) and its cost term (problem.rs:521-523, 583).
  * Same contract as vba_optimize_weighted. */
+/* BundleAdjustment::optimize with BaConfig::refine_intrinsics
+ * (optimize.rs:8-28): the joint pose / landmark / intrinsics solve
+ * (joint_intrinsics.rs:24-353, dense camera system) where it applies, else
+ * the pose / structure solve. options->refine_intrinsics holds the bits
+ * VBA_REFINE_*; camera_out receives the refined camera. */
+int32_t oracle_optimize_intrinsics(const vba_problem* problem, const vba_options* options, double* poses_out,
+                                   double* landmarks_out, vba_camera* camera_out, vba_iteration_stats* stats,
+                                   uint32_t stats_capacity, vba_result* result, int32_t threads);
 int32_t oracle_optimize_weighted(const vba_problem* problem, const vba_options* options,
                                  const double* weights, uint64_t num_weights, double* poses_out,
                                  double* landmarks_out, vba_iteration_stats* stats,

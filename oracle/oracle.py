@@ -40,6 +40,10 @@ def library():
     lib.oracle_optimize_weighted.argtypes = [C.POINTER(abi.Problem), C.POINTER(abi.Options), P64, C.c_uint64, P64,
                                              P64, C.POINTER(abi.IterationStats), C.c_uint32, C.POINTER(abi.Result),
                                              C.c_int32]
+    lib.oracle_optimize_intrinsics.restype = C.c_int32
+    lib.oracle_optimize_intrinsics.argtypes = [C.POINTER(abi.Problem), C.POINTER(abi.Options), P64, P64,
+                                               C.POINTER(abi.Camera), C.POINTER(abi.IterationStats), C.c_uint32,
+                                               C.POINTER(abi.Result), C.c_int32]
     lib.oracle_default_gnc_options.restype = None
     lib.oracle_default_gnc_options.argtypes = [C.POINTER(abi.GncOptions)]
     lib.oracle_optimize_gnc.restype = C.c_int32
@@ -126,6 +130,25 @@ def optimize_weighted(problem, options, weights, threads: int = 1, cap: int = 10
                                             cap_, C.byref(res), threads)
 
     return vba.run_solver(call, problem, options, cap)
+
+
+def optimize_intrinsics(problem, options, threads: int = 1, cap: int = 1024):
+    """oracle_optimize_intrinsics: returns (vba.SolveOutput, refined abi.Camera)."""
+    import vba
+
+    lib = library()
+    cam = abi_camera()
+
+    def call(view, opts, poses, lms, stats, cap_, res):
+        return lib.oracle_optimize_intrinsics(C.byref(view), C.byref(opts), _p(poses), _p(lms), C.byref(cam), stats,
+                                              cap_, C.byref(res), threads)
+
+    return vba.run_solver(call, problem, options, cap), cam
+
+
+def abi_camera():
+    import vba.abi as abi
+    return abi.Camera()
 
 
 def default_gnc_options(**overrides):

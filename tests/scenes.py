@@ -317,3 +317,79 @@ def add_stereo_from_state(p, rng, n_rect, n_gen, baseline=0.12, noise=0.5):
             ge.append((pid, lid, (u, v, 510.0 * xr[0] / xr[2] + 318.0 + rng.normal(0, noise),
                                   495.0 * xr[1] / xr[2] + 242.0 + rng.normal(0, noise)), 0))
     return with_stereo(p, stereo=st, baseline=baseline, general=ge, pairs=[sensor_pair()])
+
+
+# ---------------------------------------------------------------------------
+# Joint intrinsics scenes (pipelines/slam/tests/bundle_adjustment.rs:78-320,
+# 404-433, 864-873, 3645-3715)
+# ---------------------------------------------------------------------------
+def project_brown(params, Xc):
+    """The textbook Brown-Conrady lens: (fx, fy, cx, cy[, k1, k2[, p1, p2]])."""
+    p = list(params) + [0.0] * (8 - len(params))
+    fx, fy, cx, cy, k1, k2, p1, p2 = p
+    x, y = Xc[0] / Xc[2], Xc[1] / Xc[2]
+    r2 = x * x + y * y
+    rad = 1 + k1 * r2 + k2 * r2 * r2
+    xd = x * rad + 2 * p1 * x * y + p2 * (r2 + 2 * x * x)
+    yd = y * rad + p1 * (r2 + 2 * y * y) + 2 * p2 * x * y
+    return np.array([fx * xd + cx, fy * yd + cy])
+
+
+def drifted_bundle_through(model, params):
+    """drifted_bundle_through (bundle_adjustment.rs:404-433): three views
+    rendered through the camera, 10 and 20 fixed, 30 and landmark 5 drifted."""
+    truth = [(10, pose_at([0.0, 0.0, 0.0])), (20, pose_at([0.5, 0.0, 0.0])), (30, pose_at([1.0, 0.0, 0.0]))]
+    obs = [(k, l, project_brown(params, transform(p, X))) for k, p in truth for l, X in world_grid()]
+    poses = truth[:2] + [(30, pose_at([1.05, 0.02, 0.03]))]
+    lms = [(l, np.array([0.03, -0.02, 6.1]) if l == 5 else X) for l, X in world_grid()]
+    return make_problem(vba.camera(model, params, 640, 480), poses, lms, obs, fixed_poses=(10, 20))
+
+
+def fixed_structure_bundle_through(model, params):
+    """fixed_structure_bundle_through (:864-873): the drifted bundle reset to
+    truth, every landmark fixed."""
+    p = drifted_bundle_through(model, params)
+    p.poses[2] = pose_at([1.0, 0.0, 0.0])
+    p.landmarks[4] = [0.0, 0.0, 6.0]
+    p.landmark_fixed[:] = 1
+    return p
+
+
+def truth_bundle_through(model, params, start, fixed_structure=True):
+    """truth_bundle_distorted / the perturbed-focal scenes (:78-107,
+    :151-191): exact three-view data through (model, params), solved from the
+    `start` camera with poses 10, 20 and every landmark fixed."""
+    truth = [(10, pose_at([0.0, 0.0, 0.0])), (20, pose_at([0.5, 0.0, 0.0])), (30, pose_at([1.0, 0.0, 0.0]))]
+    obs = [(k, l, project_brown(params, transform(p, X))) for k, p in truth for l, X in world_grid()]
+    fixed_l = [l for l, _ in world_grid()] if fixed_structure else []
+    return make_problem(start, truth, world_grid(), obs, fixed_poses=(10, 20), fixed_landmarks=fixed_l)
+
+
+def perturbed_many_view_bundle(truth_params, start):
+    """perturbed_many_view_bundle (:3645-3715): ten yawing views of 80 points
+    rendered through the truth camera, poses 100 and 109 fixed, the others
+    and every point perturbed; the solve starts from `start`."""
+    import oracle
+    poses, obs = [], []
+    for i in range(10):
+        x = i * 0.4 - 1.8
+        truth = pose_at([x, 0.1 * (i % 3), 0.0], 0.04 * x)
+        if i < 2:
+            start_pose = truth
+        else:
+            s = float(i)
+            jit = [0.01 * np.sin(s), -0.01 * np.cos(s), 0.008 * np.sin(2 * s), 0.003 * np.cos(s),
+                   -0.002 * np.sin(s), 0.002 * np.cos(3 * s)]
+            start_pose = oracle.se3_compose(truth, oracle.se3_exp(jit))
+        poses.append((100 + i, truth, start_pose))
+    pts = []
+    for j in range(80):
+        a = float(j)
+        pts.append((1000 + j, np.array([np.sin(a * 0.37) * 2.5, np.cos(a * 0.61) * 1.2, 6.0 + np.sin(a * 0.23) * 1.5])))
+    init_pts = [(i, X + [0.02 * np.sin(float(i)), -0.02 * np.cos(float(i)), 0.03 * np.sin(0.5 * i)]) for i, X in pts]
+    for pid, truth, _ in poses:
+        for lid, X in pts:
+            xc = transform(truth, X)
+            if xc[2] > 0:
+                obs.append((pid, lid, project_brown(truth_params, xc)))
+    return make_problem(start, [(p, s) for p, _, s in poses], init_pts, obs, fixed_poses=(100, 109))

@@ -175,6 +175,98 @@ int32_t vba_optimize(vba_context* ctx, const vba_problem* problem, const vba_opt
     return st;
 }
 
+int32_t vba_optimize_intrinsics(vba_context* ctx, const vba_problem* problem, const vba_options* options,
+                                double* poses_out, double* landmarks_out, vba_camera* camera_out,
+                                vba_iteration_stats* stats, uint32_t stats_capacity, vba_result* result) {
+    if (!ctx || !problem || !options || !result || !camera_out) {
+        vba::set_thread_error("vba_optimize_intrinsics: null argument");
+        if (ctx) ctx->last_error = "vba_optimize_intrinsics: null argument";
+        return VBA_ERR_INVALID_ARGUMENT;
+    }
+    *camera_out = problem->camera;
+    const vba_camera& c = problem->camera;
+    const uint32_t flags = options->refine_intrinsics;
+    double fx, fy, cx, cy;
+    // BundleAdjustment::optimize (optimize.rs:8-28): the joint solve for a
+    // PINHOLE / OPENCV camera with intrinsics and neither general stereo nor
+    // rig observations; the pose / structure solve otherwise
+    const bool joint = (flags & VBA_REFINE_INTRINSICS) && !(problem->stereo && problem->stereo->num_general) &&
+                       problem->num_rig_observations == 0 &&
+                       (c.model == VBA_CAMERA_PINHOLE || c.model == VBA_CAMERA_OPENCV) &&
+                       vba::camera_intrinsics(c, &fx, &fy, &cx, &cy);
+    vba_options plain = *options;
+    plain.refine_intrinsics = 0;
+    if (!joint) return vba_optimize(ctx, problem, &plain, poses_out, landmarks_out, stats, stats_capacity, result);
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    std::memset(result, 0, sizeof(*result));
+    ctx->last_error.clear();
+    std::string reason;
+    int32_t st = vba::check_support(*problem, plain, &reason);
+    if (st != VBA_OK) {
+        ctx->last_error = "declined: " + reason;
+        return st;
+    }
+    if (hipSetDevice(ctx->device) != hipSuccess) {
+        ctx->last_error = "hipSetDevice failed";
+        return VBA_ERR_DEVICE;
+    }
+    vba_prepared prep;
+    uint64_t error_id = 0;
+    std::string msg;
+    try {
+        st = vba::build_model(*problem, &prep.m, &error_id, &msg);
+    } catch (const std::bad_alloc&) {
+        ctx->last_error = "out of host memory";
+        return VBA_ERR_OUT_OF_MEMORY;
+    }
+    result->error_id = error_id;
+    // the joint solve refines the camera even with every pose and landmark fixed
+    if (st != VBA_OK && st != VBA_ERR_ALL_POSES_FIXED) {
+        ctx->last_error = msg.empty() ? "invalid problem" : msg;
+        return st;
+    }
+    // the camera block (joint_intrinsics.rs:49-99): (k1, k2) on a monocular
+    // problem when asked, (p1, p2) on top (the camera then becomes OPENCV),
+    // fx = fy = their mean for a shared focal
+    const bool any_stereo = problem->stereo && problem->stereo->num_stereo;
+    const bool dist = (flags & VBA_REFINE_DISTORTION) && !any_stereo;
+    const bool tan = dist && (flags & VBA_REFINE_TANGENTIAL);
+    vba_camera out = c;
+    if (dist)
+        while (out.num_params < 6) out.params[out.num_params++] = 0.0;
+    if (tan) {
+        out.model = VBA_CAMERA_OPENCV;
+        while (out.num_params < 8) out.params[out.num_params++] = 0.0;
+    }
+    vba::Lens<double>& L = prep.m.sensors[0].lens;
+    vba::lens_from_camera(out, &L);
+    if (flags & VBA_REFINE_SHARED_FOCAL) L.fx = L.fy = 0.5 * (L.fx + L.fy);
+    // the lens family that the refined parameters can reach
+    if (tan) L.kind = vba::LENS_OPENCV;
+    else if (dist && L.kind == vba::LENS_PINHOLE) L.kind = vba::LENS_RADIAL;
+    const vba::JointSpec spec{tan ? 8 : dist ? 6 : 4, (flags & VBA_REFINE_SHARED_FOCAL) ? 1 : 0};
+    plain.linear_solver = VBA_SOLVER_DENSE;
+    st = vba::prepare(*ctx, prep, &msg);
+    if (st != VBA_OK) ctx->last_error = msg;
+    if (st == VBA_OK) {
+        try {
+            st = vba::run(*ctx, prep, plain, stats, stats_capacity, result, &spec);
+        } catch (const std::bad_alloc&) {
+            ctx->last_error = "out of host memory";
+            st = VBA_ERR_OUT_OF_MEMORY;
+        }
+    }
+    if (st == VBA_OK) st = vba::download_state(*ctx, prep, poses_out, landmarks_out);
+    if (st == VBA_OK) {
+        const vba::Lens<double>& F = prep.joint_lens;
+        const double v[8] = {F.fx, F.fy, F.cx, F.cy, F.k1, F.k2, F.p1, F.p2};
+        for (int i = 0; i < spec.k; ++i) out.params[i] = v[i];
+        *camera_out = out;
+    }
+    vba::release(prep);
+    return st;
+}
+
 int32_t vba_optimize_weighted(vba_context* ctx, const vba_problem* problem, const vba_options* options,
                               const double* weights, uint64_t num_weights, double* poses_out, double* landmarks_out,
                               vba_iteration_stats* stats, uint32_t stats_capacity, vba_result* result) {

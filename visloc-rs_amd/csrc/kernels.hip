@@ -39,10 +39,10 @@ __device__ __forceinline__ void rows_times(const double Jp[6], const double A[9]
 // false where the reference skips the observation (a point at or behind
 // either camera).
 template <bool JAC>
-__device__ __forceinline__ bool stereo_rows(const DevProblem& P, int o, const DevSensor& S, const double xr[3],
-                                            const double R[9], double r[4], double J[12]) {
+__device__ __forceinline__ bool stereo_rows(const DevProblem& P, int o, const DevSensor* sensors, const DevSensor& S,
+                                            const double xr[3], const double R[9], double r[4], double J[12]) {
     const uint8_t kind = P.obs_kind[o];
-    Lens<double> L = (kind == OBS_GENERAL ? P.sensors[S.left] : S).lens;
+    Lens<double> L = (kind == OBS_GENERAL ? sensors[S.left] : S).lens;
     L.kind = LENS_PINHOLE;
     const double ex0 = P.obs_extra[2 * (size_t)o], ex1 = P.obs_extra[2 * (size_t)o + 1];
     double uv[2], Jp[6];
@@ -115,7 +115,7 @@ __global__ __launch_bounds__(kBlock) void k_linearize(DevProblem P, int store32)
             const DevSensor& S = P.sensors[RIG ? P.obs_sensor[o] : 0];
             if (RIG && P.obs_kind && P.obs_kind[o] != OBS_PIXEL) {
                 double r[4], J[12];
-                if (!stereo_rows<true>(P, o, S, xr, R, r, J)) continue;
+                if (!stereo_rows<true>(P, o, P.sensors, S, xr, R, r, J)) continue;
                 const double sq = ((r[0] * r[0] + r[1] * r[1]) + r[2] * r[2]) + r[3] * r[3];
                 const double w = robust_weight<double>((uint32_t)P.kernel_kind, P.kernel_scale, sq) *
                                  (P.obs_w ? P.obs_w[o] : 1.0);
@@ -339,6 +339,8 @@ __global__ __launch_bounds__(kBlock) void k_cost(DevProblem P, int trial) {
     const double* poses = trial ? P.poses_trial : P.poses;
     const double* lms = trial ? P.lms_trial : P.lms;
     const double* pq = poses + 7 * (size_t)ct.pose;
+    // the trial camera of the joint camera solve
+    const DevSensor* sensors = trial && P.sensors_trial ? P.sensors_trial : P.sensors;
     double R[9];
     qmatrix<double>(Quat{pq[0], pq[1], pq[2], pq[3]}, R);
     CSum cs{0.0, 0.0};
@@ -350,11 +352,11 @@ __global__ __launch_bounds__(kBlock) void k_cost(DevProblem P, int trial) {
                               R[3] * X[0] + R[4] * X[1] + R[5] * X[2] + pq[5],
                               R[6] * X[0] + R[7] * X[1] + R[8] * X[2] + pq[6]};
         for (int o = P.edge_obs[e]; o < P.edge_obs[e + 1]; ++o) {
-            const DevSensor& S = P.sensors[RIG ? P.obs_sensor[o] : 0];
+            const DevSensor& S = sensors[RIG ? P.obs_sensor[o] : 0];
             double sq;
             if (RIG && P.obs_kind && P.obs_kind[o] != OBS_PIXEL) {
                 double r[4];
-                if (!stereo_rows<false>(P, o, S, xr, R, r, (double*)nullptr)) {
+                if (!stereo_rows<false>(P, o, sensors, S, xr, R, r, (double*)nullptr)) {
                     bad += 1.0;
                     if (P.obs_s) P.obs_s[o] = __builtin_nan("");
                     continue;
@@ -644,8 +646,11 @@ __device__ void dense_solve_body(const DevProblem& P, const int32_t* var_slot, d
                                  int* status, int* fail_sh, double* piv_val, int* piv_idx, double* dblk) {
     double* A2 = work;                 // n*n copy for the LU fallback
     double* y = work + (size_t)n * n;  // n
+    // the last joint_k unknowns are the camera block of the joint solve
+    const int nv = n - P.joint_k;
     for (size_t k = threadIdx.x; k < (size_t)n * n; k += NT) A2[k] = S[k];
-    for (int i = threadIdx.x; i < n; i += NT) y[i] = P.b[6 * (size_t)var_slot[i / 6] + (i % 6)];
+    for (int i = threadIdx.x; i < n; i += NT)
+        y[i] = i < nv ? P.b[6 * (size_t)var_slot[i / 6] + (i % 6)] : P.joint_rhs[i - nv];
     if (threadIdx.x == 0) *fail_sh = 0;
     __syncthreads();
     chol_blocked<NT>(S, n, dblk, fail_sh);
@@ -654,7 +659,8 @@ __device__ void dense_solve_body(const DevProblem& P, const int32_t* var_slot, d
         chol_blocked_solve<NT>(S, n, y);
     } else {
         // LU with partial pivoting on A2; y holds b again.
-        for (int i = threadIdx.x; i < n; i += NT) y[i] = P.b[6 * (size_t)var_slot[i / 6] + (i % 6)];
+        for (int i = threadIdx.x; i < n; i += NT)
+            y[i] = i < nv ? P.b[6 * (size_t)var_slot[i / 6] + (i % 6)] : P.joint_rhs[i - nv];
         __syncthreads();
         for (int k = 0; k < n; ++k) {
             double best = -1.0;
@@ -727,7 +733,11 @@ __device__ void dense_solve_body(const DevProblem& P, const int32_t* var_slot, d
     }
     __syncthreads();
     const int f = *fail_sh;
-    for (int i = threadIdx.x; i < n; i += NT) P.x[6 * (size_t)var_slot[i / 6] + (i % 6)] = (f != 2) ? y[i] : 0.0;
+    for (int i = threadIdx.x; i < n; i += NT) {
+        const double v = (f != 2) ? y[i] : 0.0;
+        if (i < nv) P.x[6 * (size_t)var_slot[i / 6] + (i % 6)] = v;
+        else P.joint_delta[i - nv] = v;
+    }
     if (threadIdx.x == 0) *status = (f == 2) ? 1 : 0;
 }
 
@@ -922,6 +932,341 @@ __global__ void k_batch_commit(DevProblem P, BatchDev B) {
         for (int k = 0; k < 3; ++k) P.lms[3 * (size_t)i + k] = P.lms_trial[3 * (size_t)i + k];
 }
 
+// ---------------------------------------------------------------------------
+// Joint pose / landmark / camera solve (optimize_joint_intrinsics,
+// joint_intrinsics.rs:24-353): the camera parameters (fx, fy, cx, cy[, k1,
+// k2[, p1, p2]]) are K extra unknowns bordering the reduced camera system.
+// Every observation's pose Jacobian is J_l [I | -[X]x], so its pose-camera
+// block is [E; X x E] with E = sum w J_l^T J_k over the edge; the landmark
+// block W_lk is the sum of its edges' E.
+// ---------------------------------------------------------------------------
+__host__ __device__ constexpr int joint_tri(int K) { return K * (K + 1) / 2; }
+__host__ __device__ constexpr int joint_na(int K) { return 6 * K + joint_tri(K) + K; }
+
+// Residual rows r, landmark Jacobian rows Jl (3 per row) and camera columns
+// Jk (K per row) of one observation (joint_mono_terms, :356-465; the
+// rectified stereo terms, :702-790). Returns the row count, 0 where the
+// reference skips the observation.
+template <int K>
+__device__ int joint_rows(const DevProblem& P, int o, const double xr[3], const double R[9], double r[3],
+                          double Jl[9], double Jk[3 * K]) {
+#pragma unroll
+    for (int i = 0; i < 3 * K; ++i) Jk[i] = 0.0;
+    const DevSensor& S = P.sensors[0];
+    if (P.obs_kind && P.obs_kind[o] == OBS_RECTIFIED) {
+        double r4[4], J12[12];
+        if (!stereo_rows<true>(P, o, P.sensors, S, xr, R, r4, J12)) return 0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) r[i] = r4[i];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) Jl[i] = J12[i];
+        // u_r = fx (X - b) / Z + cx: d u_r / d fx = (X - b) / Z, d u_r / d cx = 1
+        const double zi = 1.0 / xr[2], b = P.obs_extra[2 * (size_t)o + 1];
+        Jk[0] = xr[0] * zi;
+        Jk[2] = 1.0;
+        Jk[K + 1] = xr[1] * zi;
+        Jk[K + 3] = 1.0;
+        Jk[2 * K] = (xr[0] - b) * zi;
+        Jk[2 * K + 2] = 1.0;
+        return 3;
+    }
+    const Lens<double>& L = S.lens;
+    double uv[2], Jp[6];
+    if (!project<double>(L, xr[0], xr[1], xr[2], uv, Jp)) return 0;
+    r[0] = uv[0] - P.obs_uv[2 * (size_t)o];
+    r[1] = uv[1] - P.obs_uv[2 * (size_t)o + 1];
+    r[2] = 0.0;
+    rows_times(Jp, R, Jl);
+    Jl[6] = Jl[7] = Jl[8] = 0.0;
+    const double zi = 1.0 / xr[2];
+    const double x = xr[0] * zi, y = xr[1] * zi, r2 = x * x + y * y;
+    const double d = 1.0 + L.k1 * r2 + L.k2 * r2 * r2;
+    double xd = x * d, yd = y * d;
+    const double xy2 = 2.0 * x * y;
+    if (L.kind == LENS_OPENCV) {
+        xd = x * d + L.p1 * xy2 + L.p2 * (r2 + 2.0 * x * x);
+        yd = y * d + L.p1 * (r2 + 2.0 * y * y) + L.p2 * xy2;
+    }
+    Jk[0] = xd;
+    Jk[2] = 1.0;
+    Jk[K + 1] = yd;
+    Jk[K + 3] = 1.0;
+    if (K >= 6) {
+        Jk[4] = L.fx * x * r2;
+        Jk[5] = L.fx * x * r2 * r2;
+        Jk[K + 4] = L.fy * y * r2;
+        Jk[K + 5] = L.fy * y * r2 * r2;
+    }
+    if (K == 8) {
+        Jk[6] = L.fx * xy2;
+        Jk[7] = L.fx * (r2 + 2.0 * x * x);
+        Jk[K + 6] = L.fy * (r2 + 2.0 * y * y);
+        Jk[K + 7] = L.fy * xy2;
+    }
+    return 2;
+}
+
+// Per camera tile: E per edge, and the tile's J_p^T J_k, J_k^T J_k (upper)
+// and J_k^T r (build_joint_intrinsics_system, :593-790).
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_joint_linearize(DevProblem P) {
+    constexpr int NA = joint_na(K), T0 = 6 * K, B0 = 6 * K + joint_tri(K);
+    __shared__ double lds[(kBlock / 64) * NA];
+    const int tile = xcd_tile(blockIdx.x, P.num_cam_tiles);
+    if (tile >= P.num_cam_tiles) return;
+    const CamTile ct = P.cam_tiles[tile];
+    const double* pq = P.poses + 7 * (size_t)ct.pose;
+    double R[9];
+    qmatrix<double>(Quat{pq[0], pq[1], pq[2], pq[3]}, R);
+    double acc[NA];
+#pragma unroll
+    for (int i = 0; i < NA; ++i) acc[i] = 0.0;
+    for (int e = ct.begin + (int)threadIdx.x; e < ct.end; e += kBlock) {
+        const int l = P.edge_lm[e];
+        const double X[3] = {P.lms[3 * (size_t)l], P.lms[3 * (size_t)l + 1], P.lms[3 * (size_t)l + 2]};
+        const double xr[3] = {R[0] * X[0] + R[1] * X[1] + R[2] * X[2] + pq[4],
+                              R[3] * X[0] + R[4] * X[1] + R[5] * X[2] + pq[5],
+                              R[6] * X[0] + R[7] * X[1] + R[8] * X[2] + pq[6]};
+        double E[3 * K];
+#pragma unroll
+        for (int i = 0; i < 3 * K; ++i) E[i] = 0.0;
+        for (int o = P.edge_obs[e]; o < P.edge_obs[e + 1]; ++o) {
+            double r[3], Jl[9], Jk[3 * K];
+            const int rows = joint_rows<K>(P, o, xr, R, r, Jl, Jk);
+            if (!rows) continue;
+            double sq = 0.0;
+            for (int i = 0; i < rows; ++i) sq += r[i] * r[i];
+            const double w = robust_weight<double>((uint32_t)P.kernel_kind, P.kernel_scale, sq);
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int c = 0; c < K; ++c)
+                    E[a * K + c] += w * (Jl[a] * Jk[c] + Jl[3 + a] * Jk[K + c] + Jl[6 + a] * Jk[2 * K + c]);
+            int t = T0;
+#pragma unroll
+            for (int a = 0; a < K; ++a) {
+#pragma unroll
+                for (int c = a; c < K; ++c)
+                    acc[t++] += w * (Jk[a] * Jk[c] + Jk[K + a] * Jk[K + c] + Jk[2 * K + a] * Jk[2 * K + c]);
+                acc[B0 + a] += w * (Jk[a] * r[0] + Jk[K + a] * r[1] + Jk[2 * K + a] * r[2]);
+            }
+        }
+        double* Es = P.joint_E + (size_t)3 * K * P.perm_cl[e];
+#pragma unroll
+        for (int i = 0; i < 3 * K; ++i) Es[i] = E[i];
+#pragma unroll
+        for (int c = 0; c < K; ++c) {
+            const double v0 = E[c], v1 = E[K + c], v2 = E[2 * K + c];
+            acc[c] += v0;
+            acc[K + c] += v1;
+            acc[2 * K + c] += v2;
+            acc[3 * K + c] += X[1] * v2 - X[2] * v1;
+            acc[4 * K + c] += X[2] * v0 - X[0] * v2;
+            acc[5 * K + c] += X[0] * v1 - X[1] * v0;
+        }
+    }
+    block_sum<kBlock, double, NA>(acc, lds);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int i = 0; i < NA; ++i) P.joint_tile[(size_t)NA * tile + i] = acc[i];
+}
+
+// Per pose slot its J_p^T J_k over its tiles in order; per landmark W_lk
+// over its edges in order; thread 0.. of block 0 also sums the camera
+// terms over all tiles in order.
+template <int K>
+__global__ void k_joint_reduce(DevProblem P) {
+    constexpr int NA = joint_na(K);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < P.num_pose_slots)
+        for (int a = 0; a < 6 * K; ++a) {
+            double s = 0.0;
+            for (int t = P.cam_tile_ptr[i]; t < P.cam_tile_ptr[i + 1]; ++t) s += P.joint_tile[(size_t)NA * t + a];
+            P.joint_hpk[(size_t)6 * K * i + a] = s;
+        }
+    if (i < P.num_lm_slots)
+        for (int a = 0; a < 3 * K; ++a) {
+            double s = 0.0;
+            for (int e = P.lm_edge_ptr[i]; e < P.lm_edge_ptr[i + 1]; ++e)
+                s += P.joint_E[(size_t)3 * K * P.lm_store[e] + a];
+            P.joint_Wk[(size_t)3 * K * i + a] = s;
+        }
+    if (i < NA - 6 * K) {
+        double s = 0.0;
+        for (int t = 0; t < P.num_cam_tiles; ++t) s += P.joint_tile[(size_t)NA * t + 6 * K + i];
+        P.joint_kk[i] = s;
+    }
+}
+
+// Border rows of S: S_pk = H_pk - sum over the pose's edges of
+// [G; X x G] V^-1 W_lk, one thread per variable pose (joint_intrinsics.rs:
+// 143-185).
+template <int K>
+__global__ void k_joint_border(DevProblem P, double* S, int n) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P.num_pose_slots) return;
+    const int v = P.pose_var[p];
+    if (v < 0) return;
+    const int nv = n - K;
+    double acc[6 * K];
+#pragma unroll
+    for (int i = 0; i < 6 * K; ++i) acc[i] = 0.0;
+    for (int t = P.cam_tile_ptr[p]; t < P.cam_tile_ptr[p + 1]; ++t) {
+        const CamTile ct = P.cam_tiles[t];
+        for (int e = ct.begin; e < ct.end; ++e) {
+            const int l = P.edge_lm[e];
+            const double* G = P.Gl64 + 6 * (size_t)P.perm_cl[e];
+            const double* Vi = P.Vinv64 + 6 * (size_t)l;
+            const double* W = P.joint_Wk + (size_t)3 * K * l;
+            const double X[3] = {P.lms[3 * (size_t)l], P.lms[3 * (size_t)l + 1], P.lms[3 * (size_t)l + 2]};
+#pragma unroll
+            for (int c = 0; c < K; ++c) {
+                const double w0 = W[c], w1 = W[K + c], w2 = W[2 * K + c];
+                const double t0 = Vi[0] * w0 + Vi[1] * w1 + Vi[2] * w2;
+                const double t1 = Vi[1] * w0 + Vi[3] * w1 + Vi[4] * w2;
+                const double t2 = Vi[2] * w0 + Vi[4] * w1 + Vi[5] * w2;
+                const double y0 = G[0] * t0 + G[1] * t1 + G[2] * t2;
+                const double y1 = G[1] * t0 + G[3] * t1 + G[4] * t2;
+                const double y2 = G[2] * t0 + G[4] * t1 + G[5] * t2;
+                acc[c] += y0;
+                acc[K + c] += y1;
+                acc[2 * K + c] += y2;
+                acc[3 * K + c] += X[1] * y2 - X[2] * y1;
+                acc[4 * K + c] += X[2] * y0 - X[0] * y2;
+                acc[5 * K + c] += X[0] * y1 - X[1] * y0;
+            }
+        }
+    }
+    const double* H = P.joint_hpk + (size_t)6 * K * p;
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+#pragma unroll
+        for (int c = 0; c < K; ++c) {
+            const double s = H[a * K + c] - acc[a * K + c];
+            S[(size_t)(6 * v + a) * n + nv + c] = s;
+            S[(size_t)(nv + c) * n + 6 * v + a] = s;
+        }
+}
+
+// Camera-block Schur terms sum_l W_lk^T V^-1 W_lk (upper) and
+// W_lk^T V^-1 g_l, per work-group over a contiguous landmark range.
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_joint_kk_partial(DevProblem P) {
+    constexpr int NB = joint_tri(K) + K;
+    __shared__ double lds[(kBlock / 64) * NB];
+    double acc[NB];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) acc[i] = 0.0;
+    const int l = blockIdx.x * kBlock + threadIdx.x;
+    if (l < P.num_lm_slots) {
+        const double* Vi = P.Vinv64 + 6 * (size_t)l;
+        const double* W = P.joint_Wk + (size_t)3 * K * l;
+        const double* g = P.gl + 3 * (size_t)l;
+        double T[3 * K];
+#pragma unroll
+        for (int c = 0; c < K; ++c) {
+            const double w0 = W[c], w1 = W[K + c], w2 = W[2 * K + c];
+            T[c] = Vi[0] * w0 + Vi[1] * w1 + Vi[2] * w2;
+            T[K + c] = Vi[1] * w0 + Vi[3] * w1 + Vi[4] * w2;
+            T[2 * K + c] = Vi[2] * w0 + Vi[4] * w1 + Vi[5] * w2;
+        }
+        int t = 0;
+#pragma unroll
+        for (int a = 0; a < K; ++a) {
+#pragma unroll
+            for (int c = a; c < K; ++c) acc[t++] = W[a] * T[c] + W[K + a] * T[K + c] + W[2 * K + a] * T[2 * K + c];
+            acc[joint_tri(K) + a] = T[a] * g[0] + T[K + a] * g[1] + T[2 * K + a] * g[2];
+        }
+    }
+    block_sum<kBlock, double, NB>(acc, lds);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int i = 0; i < NB; ++i) P.joint_part[(size_t)NB * blockIdx.x + i] = acc[i];
+}
+
+// The camera block S_kk = H_kk + lambda I - sum and its right-hand side
+// -b_k + sum, the partials summed in order; then, for a shared focal, fy
+// folded into fx (S' = T^T S T with fy's row and column left as the
+// identity, joint_intrinsics.rs:187-209). One work-group.
+template <int K>
+__global__ __launch_bounds__(kSmallBlock) void k_joint_kk_final(DevProblem P, double* S, int n, double lambda,
+                                                                int num_parts) {
+    constexpr int NB = joint_tri(K) + K;
+    const int nv = n - K;
+    if (threadIdx.x < NB) {
+        const int i = threadIdx.x;
+        double s = 0.0;
+        for (int b = 0; b < num_parts; ++b) s += P.joint_part[(size_t)NB * b + i];
+        if (i < joint_tri(K)) {
+            int a = 0, rem = i;
+            while (rem >= K - a) {
+                rem -= K - a;
+                ++a;
+            }
+            const int c = a + rem;
+            double h = P.joint_kk[i];
+            if (a == c && lambda > 0.0) h += lambda;
+            S[(size_t)(nv + a) * n + nv + c] = h - s;
+            S[(size_t)(nv + c) * n + nv + a] = h - s;
+        } else {
+            P.joint_rhs[i - joint_tri(K)] = -P.joint_kk[i] + s;
+        }
+    }
+    if (!P.joint_shared) return;
+    __syncthreads();
+    const int fx = nv, fy = nv + 1;
+    for (int j = threadIdx.x; j < n; j += kSmallBlock) S[(size_t)fx * n + j] += S[(size_t)fy * n + j];
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += kSmallBlock) S[(size_t)i * n + fx] += S[(size_t)i * n + fy];
+    __syncthreads();
+    for (int j = threadIdx.x; j < n; j += kSmallBlock) {
+        S[(size_t)fy * n + j] = j == fy ? 1.0 : 0.0;
+        if (j != fy) S[(size_t)j * n + fy] = 0.0;
+    }
+    if (threadIdx.x == 0) {
+        P.joint_rhs[0] += P.joint_rhs[1];
+        P.joint_rhs[1] = 0.0;
+    }
+}
+
+// After the solve: a shared focal's step written to fy too; the trial
+// camera; the landmark gradient shifted by the camera step, which
+// back-substitution reads (delta_l = -V^-1 (g_l + W_lk delta_k + W_pl^T dp)).
+template <int K>
+__global__ void k_joint_step(DevProblem P) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    double dk[K];
+#pragma unroll
+    for (int c = 0; c < K; ++c) dk[c] = P.joint_delta[c];
+    if (P.joint_shared) dk[1] = dk[0];
+    if (l == 0) {
+        DevSensor s = P.sensors[0];
+        s.lens.fx += dk[0];
+        s.lens.fy += dk[1];
+        s.lens.cx += dk[2];
+        s.lens.cy += dk[3];
+        if (K >= 6) {
+            s.lens.k1 += dk[4];
+            s.lens.k2 += dk[5];
+        }
+        if (K == 8) {
+            s.lens.p1 += dk[6];
+            s.lens.p2 += dk[7];
+        }
+        P.sensors_trial[0] = s;
+    }
+    if (l < P.num_lm_slots) {
+        const double* W = P.joint_Wk + (size_t)3 * K * l;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            double t = 0.0;
+#pragma unroll
+            for (int c = 0; c < K; ++c) t += W[i * K + c] * dk[c];
+            P.joint_gl[3 * (size_t)l + i] = P.gl[3 * (size_t)l + i] + t;
+        }
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -956,6 +1301,44 @@ void launch_cost(const DevProblem& P, bool trial, hipStream_t s) {
 }
 void launch_scalars_finalize(const DevProblem& P, hipStream_t s) {
     hipLaunchKernelGGL(k_scalars_finalize, dim3(1), dim3(kSmallBlock), 0, s, P, blocks_for(P.num_pose_slots, kBlock));
+}
+template <int K>
+void joint_linearize_k(const DevProblem& P, hipStream_t s) {
+    if (P.num_cam_tiles)
+        hipLaunchKernelGGL(k_joint_linearize<K>, dim3(xcd_grid(P.num_cam_tiles)), dim3(kBlock), 0, s, P);
+    const int n = std::max(std::max(P.num_pose_slots, P.num_lm_slots), joint_na(K));
+    hipLaunchKernelGGL(k_joint_reduce<K>, dim3(blocks_for(n, 256)), dim3(256), 0, s, P);
+}
+template <int K>
+void joint_border_k(const DevProblem& P, double* S, int32_t n, double lambda, hipStream_t s) {
+    if (P.num_pose_slots)
+        hipLaunchKernelGGL(k_joint_border<K>, dim3(blocks_for(P.num_pose_slots, 64)), dim3(64), 0, s, P, S, n);
+    const int parts = blocks_for(P.num_lm_slots, kBlock);
+    if (parts) hipLaunchKernelGGL(k_joint_kk_partial<K>, dim3(parts), dim3(kBlock), 0, s, P);
+    hipLaunchKernelGGL(k_joint_kk_final<K>, dim3(1), dim3(kSmallBlock), 0, s, P, S, n, lambda, parts);
+}
+template <int K>
+void joint_step_k(const DevProblem& P, hipStream_t s) {
+    hipLaunchKernelGGL(k_joint_step<K>, dim3(blocks_for(std::max(P.num_lm_slots, 1), 256)), dim3(256), 0, s, P);
+}
+void launch_joint_linearize(const DevProblem& P, hipStream_t s) {
+    if (P.joint_k == 4) joint_linearize_k<4>(P, s);
+    else if (P.joint_k == 6) joint_linearize_k<6>(P, s);
+    else if (P.joint_k == 8) joint_linearize_k<8>(P, s);
+}
+void launch_joint_border(const DevProblem& P, double* S, int32_t n, double lambda, hipStream_t s) {
+    if (P.joint_k == 4) joint_border_k<4>(P, S, n, lambda, s);
+    else if (P.joint_k == 6) joint_border_k<6>(P, S, n, lambda, s);
+    else if (P.joint_k == 8) joint_border_k<8>(P, S, n, lambda, s);
+}
+void launch_joint_step(const DevProblem& P, hipStream_t s) {
+    if (P.joint_k == 4) joint_step_k<4>(P, s);
+    else if (P.joint_k == 6) joint_step_k<6>(P, s);
+    else if (P.joint_k == 8) joint_step_k<8>(P, s);
+}
+size_t joint_part_size(const DevProblem& P) {
+    const int K = P.joint_k;
+    return (size_t)std::max(blocks_for(P.num_lm_slots, kBlock), 1) * (K * (K + 1) / 2 + K);
 }
 void launch_dense_assemble(const DevProblem& P, const DensePairs& D, double* S, int32_t n, double lambda,
                            hipStream_t s) {

@@ -138,6 +138,18 @@ struct DevProblem {
     const int32_t* cam_tile_win;
     const int32_t* lm_tile_win;
     const WindowState* wstate;  // lambda, active (iterating), relin (state changed)
+    // joint pose / landmark / camera solve (joint_k > 0 camera parameters)
+    int32_t joint_k, joint_shared;
+    DevSensor* sensors_trial;  // the trial camera (null outside the joint solve)
+    double* joint_E;           // per storage position: sum of w J_l^T J_k (3 x k)
+    double* joint_tile;        // per camera tile: J_p^T J_k (6 x k), J_k^T J_k (upper), J_k^T r
+    double* joint_hpk;         // per pose slot: 6 x k
+    double* joint_kk;          // J_k^T J_k (upper) and J_k^T r over all observations
+    double* joint_Wk;          // per landmark slot: 3 x k
+    double* joint_part;        // per work-group partials of the camera-block Schur terms
+    double* joint_rhs;         // k
+    double* joint_delta;       // k
+    double* joint_gl;          // per landmark slot: g_l + W_lk delta_k
 };
 
 struct WindowLayout {
@@ -229,6 +241,17 @@ void launch_pcg_explicit(const DevProblem& P, const ExplicitDev& X, bool fp32, c
 // True residual b - S64 x and its norm (launch_pcg_true_finish); with
 // restart also r = r_t, z = M^-1 r_t and d = 0 (d is P.d).
 void launch_explicit_true_residual(const DevProblem& P, const ExplicitDev& X, int32_t restart, hipStream_t stream);
+
+// Joint camera block (joint_k = 4, 6 or 8): the camera terms after
+// linearization; the border rows of S and the camera right-hand side once
+// the pose blocks are assembled (damping, Schur terms, the shared-focal
+// fold); after the solve, the landmark gradient shifted by the camera step
+// (back-substitution reads joint_gl) and the trial camera.
+void launch_joint_linearize(const DevProblem& P, hipStream_t stream);
+void launch_joint_border(const DevProblem& P, double* S, int32_t n, double lambda, hipStream_t stream);
+void launch_joint_step(const DevProblem& P, hipStream_t stream);
+// Doubles of joint_part a problem needs.
+size_t joint_part_size(const DevProblem& P);
 
 void launch_dense_assemble(const DevProblem& P, const DensePairs& D, double* S, int32_t n,
                            double lambda, hipStream_t stream);

@@ -13,23 +13,6 @@ namespace {
 bool supported_model(int32_t m) { return m >= VBA_CAMERA_PINHOLE && m <= VBA_CAMERA_OPENCV; }
 
 // Camera::intrinsics (crates/core/src/types/camera.rs:241-271).
-bool camera_intrinsics(const vba_camera& c, double* fx, double* fy, double* cx, double* cy) {
-    switch (c.model) {
-        case VBA_CAMERA_PINHOLE:
-        case VBA_CAMERA_OPENCV:
-            if (c.num_params < 4) return false;
-            *fx = c.params[0]; *fy = c.params[1]; *cx = c.params[2]; *cy = c.params[3];
-            return true;
-        case VBA_CAMERA_SIMPLE_PINHOLE:
-        case VBA_CAMERA_SIMPLE_RADIAL:
-        case VBA_CAMERA_RADIAL:
-            if (c.num_params < 3) return false;
-            *fx = *fy = c.params[0]; *cx = c.params[1]; *cy = c.params[2];
-            return true;
-        default:
-            return false;
-    }
-}
 
 void quat_to_matrix(const double* q, double R[9]) {
     Quat u{q[0], q[1], q[2], q[3]};
@@ -151,6 +134,24 @@ uint32_t next_pow2(uint64_t v) {
 }
 
 }  // namespace
+
+bool camera_intrinsics(const vba_camera& c, double* fx, double* fy, double* cx, double* cy) {
+    switch (c.model) {
+        case VBA_CAMERA_PINHOLE:
+        case VBA_CAMERA_OPENCV:
+            if (c.num_params < 4) return false;
+            *fx = c.params[0]; *fy = c.params[1]; *cx = c.params[2]; *cy = c.params[3];
+            return true;
+        case VBA_CAMERA_SIMPLE_PINHOLE:
+        case VBA_CAMERA_SIMPLE_RADIAL:
+        case VBA_CAMERA_RADIAL:
+            if (c.num_params < 3) return false;
+            *fx = *fy = c.params[0]; *cx = c.params[1]; *cy = c.params[2];
+            return true;
+        default:
+            return false;
+    }
+}
 
 bool lens_from_camera(const vba_camera& c, Lens<double>* L) {
     double fx, fy, cx, cy;
@@ -455,7 +456,9 @@ int32_t build_model(const vba_problem& pb, Model* mp, uint64_t* error_id, std::s
     }
     for (uint32_t s = 0; s < m.num_lm_slots; ++s)
         if (!lm_fixed[s]) m.lm_var[s] = (int32_t)m.L++;
-    if (m.P == 0 && m.L == 0) return VBA_ERR_ALL_POSES_FIXED;
+    // reported after the whole layout is built: the joint camera solve
+    // still runs with every pose and landmark fixed
+    const bool all_fixed = m.P == 0 && m.L == 0;
 
     // c-order: (pose, landmark, input index) by two stable counting sorts,
     // by landmark and then by pose.
@@ -582,7 +585,7 @@ int32_t build_model(const vba_problem& pb, Model* mp, uint64_t* error_id, std::s
         }
     }
     build_runs(&m);
-    return VBA_OK;
+    return all_fixed ? VBA_ERR_ALL_POSES_FIXED : VBA_OK;
 }
 
 }  // namespace vba

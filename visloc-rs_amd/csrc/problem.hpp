@@ -137,6 +137,10 @@ inline uint64_t observation_count(const vba_problem& p) {
            (p.stereo ? p.stereo->num_stereo + p.stereo->num_general : 0);
 }
 
+// Camera::intrinsics (camera.rs:241-271): (fx, fy, cx, cy), false for a
+// model without them.
+bool camera_intrinsics(const vba_camera& camera, double* fx, double* fy, double* cx, double* cy);
+
 // Lens of a vba_camera, resolved as MonoProjection::for_camera does
 // (jacobians.rs:29-58). Returns false for an unsupported model.
 bool lens_from_camera(const vba_camera& camera, Lens<double>* lens);

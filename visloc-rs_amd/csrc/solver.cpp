@@ -30,7 +30,8 @@ enum BufId {
     B_VQ, B_PART64, B_PART21, B_BPART, B_CPART, B_PCG, B_DELTA_L, B_TILE_SCALAR,
     B_SCALARS, B_DENSE_BLOCK_PQ, B_DENSE_BLOCK_PTR, B_DENSE_TERMS, B_DENSE_VAR_SLOT, B_DENSE_S,
     B_DENSE_WORK, B_DENSE_STATUS, B_X_BLOCK_POS, B_X_ROW_PTR, B_X_COL, B_X_DIAG_POS, B_X_S64, B_X_S32, B_VD2,
-    B_G32_POS, B_OBS_W, B_OBS_S, B_OBS_KIND, B_OBS_EXTRA, B_COUNT
+    B_G32_POS, B_OBS_W, B_OBS_S, B_OBS_KIND, B_OBS_EXTRA, B_SENSORS_T, B_J_E, B_J_TILE, B_J_HPK, B_J_KK, B_J_WK,
+    B_J_PART, B_J_RHS, B_J_DELTA, B_J_GL, B_COUNT
 };
 
 // Profiled kernel families.
@@ -616,7 +617,7 @@ DevProblem bind(const Context& ctx, Prepared& pr, const vba_options& o, bool mix
 }  // namespace
 
 int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stats* stats, uint32_t cap,
-            vba_result* res) {
+            vba_result* res, const JointSpec* joint) {
     hipStream_t s = ctx.stream;
     const Model& m = pr.m;
     const bool matrix_free = o.linear_solver != VBA_SOLVER_DENSE;  // a PCG solver
@@ -633,7 +634,26 @@ int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stat
         double* dS = nullptr;
         double* dwork = nullptr;
         int32_t* dstatus = nullptr;
-        const int32_t ndense = (int32_t)(6 * m.P);
+        const int32_t JK = joint ? joint->k : 0;
+        if (JK) {
+            if (matrix_free) throw Fail{VBA_ERR_INVALID_ARGUMENT, "the joint camera solve is dense"};
+            const size_t NA = 6 * (size_t)JK + JK * (JK + 1) / 2 + JK;
+            P.joint_k = JK;
+            P.joint_shared = joint->shared;
+            P.sensors_trial = (DevSensor*)ensure(pr, B_SENSORS_T, m.sensors.size() * sizeof(DevSensor));
+            hip_check(hipMemcpyAsync(P.sensors_trial, P.sensors, m.sensors.size() * sizeof(DevSensor),
+                                     hipMemcpyDeviceToDevice, s), "D2D");
+            P.joint_E = (double*)ensure(pr, B_J_E, 3 * JK * std::max<size_t>(NE, 1) * sizeof(double));
+            P.joint_tile = (double*)ensure(pr, B_J_TILE, NA * std::max<size_t>(NT, 1) * sizeof(double));
+            P.joint_hpk = (double*)ensure(pr, B_J_HPK, 6 * JK * std::max<size_t>(NP, 1) * sizeof(double));
+            P.joint_kk = (double*)ensure(pr, B_J_KK, NA * sizeof(double));
+            P.joint_Wk = (double*)ensure(pr, B_J_WK, 3 * JK * std::max<size_t>(NL, 1) * sizeof(double));
+            P.joint_part = (double*)ensure(pr, B_J_PART, joint_part_size(P) * sizeof(double));
+            P.joint_rhs = (double*)ensure(pr, B_J_RHS, JK * sizeof(double));
+            P.joint_delta = (double*)ensure(pr, B_J_DELTA, JK * sizeof(double));
+            P.joint_gl = (double*)ensure(pr, B_J_GL, 3 * std::max<size_t>(NL, 1) * sizeof(double));
+        }
+        const int32_t ndense = (int32_t)(6 * m.P) + JK;
         if (!matrix_free || xplicit) {
             if (!pr.dense_built) {
                 build_dense(pr);
@@ -744,6 +764,7 @@ int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stat
                 }
                 launch_camera_finalize(P, s);
                 launch_landmark_reduce(P, s);
+                if (JK) launch_joint_linearize(P, s);
                 need_lin = false;
                 ++linearizations;
             }
@@ -759,6 +780,7 @@ int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stat
             if (!matrix_free) {
                 hip_check(hipMemsetAsync(dS, 0, (size_t)ndense * ndense * sizeof(double), s), "memset");
                 launch_dense_assemble(P, D, dS, ndense, lambda, s);
+                if (JK) launch_joint_border(P, dS, ndense, lambda, s);
                 launch_dense_solve(P, D, dS, dwork, ndense, dstatus, s);
                 hip_check(hipGetLastError(), "kernel launch");
                 hip_check(hipMemcpyAsync(ctx.host_status, dstatus, sizeof(int32_t), hipMemcpyDeviceToHost, s), "D2H");
@@ -915,7 +937,15 @@ int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stat
                 if (lambda >= o.max_lambda) break;
                 continue;
             }
-            launch_backsub(P, s);
+            if (JK) {
+                // the trial camera, and back-substitution with g_l + W_lk delta_k
+                launch_joint_step(P, s);
+                DevProblem Pb = P;
+                Pb.gl = P.joint_gl;
+                launch_backsub(Pb, s);
+            } else {
+                launch_backsub(P, s);
+            }
             launch_pose_update(P, s);
             {
                 ProfScope ps(ctx, K_COST, bytes_cost);
@@ -945,6 +975,11 @@ int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stat
             push(st);
             std::swap(P.poses, P.poses_trial);
             std::swap(P.lms, P.lms_trial);
+            if (JK) {
+                DevSensor* cur = const_cast<DevSensor*>(P.sensors);
+                P.sensors = P.sensors_trial;
+                P.sensors_trial = cur;
+            }
             need_lin = true;
             current_cost = cost_after;
             current_nonproj = nonproj_after;
@@ -970,6 +1005,11 @@ int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stat
         harvest(ctx);
         float ms = 0.f;
         hip_check(hipEventElapsedTime(&ms, ctx.ev_start, ctx.ev_stop), "event");
+        if (JK) {
+            DevSensor cam;
+            hip_check(hipMemcpy(&cam, P.sensors, sizeof(DevSensor), hipMemcpyDeviceToHost), "D2H");
+            pr.joint_lens = cam.lens;
+        }
         pr.final_slot = P.poses == buf<double>(pr, B_POSES) ? B_POSES : B_POSES_T;
         pr.final_lm_slot = P.lms == buf<double>(pr, B_LMS) ? B_LMS : B_LMS_T;
         res->initial_cost = initial_cost;

@@ -82,6 +82,7 @@ struct Prepared {
     int final_slot = -1;  // buffer holding the final poses (-1: none yet)
     int final_lm_slot = -1;
     int32_t last_pcg = 0;  // iterations of the last PCG solve (the schedule's prediction)
+    Lens<double> joint_lens{};  // the camera after a joint camera solve
 };
 
 // Matrix-free entry validation (optimize.rs:382-626) on a built model.
@@ -92,8 +93,16 @@ int32_t prepare(Context& ctx, Prepared& prep, std::string* message);
 
 // The LM loop of optimize_weighted_backend (lm.rs:162-1017) on the device,
 // from the prepared initial state.
+// A joint pose / landmark / camera solve (optimize_joint_intrinsics,
+// joint_intrinsics.rs:24-353): k camera parameters (4, 6 or 8) border the
+// dense reduced system; shared: fx and fy move together. The model's
+// sensor 0 holds the starting camera.
+struct JointSpec {
+    int32_t k;
+    int32_t shared;
+};
 int32_t run(Context& ctx, Prepared& prep, const vba_options& o, vba_iteration_stats* stats,
-            uint32_t stats_capacity, vba_result* result);
+            uint32_t stats_capacity, vba_result* result, const JointSpec* joint = nullptr);
 
 // Final (or, before any solve, initial) poses / landmarks in input order.
 int32_t download_state(Context& ctx, const Prepared& prep, double* poses_out, double* landmarks_out);

@@ -64,6 +64,13 @@ FEATURE_INERTIAL = 1 << 3
 FEATURE_FIXED_ROTATIONS = 1 << 4
 FEATURE_OBSERVATION_WEIGHTS = 1 << 5
 
+# vba_options.refine_intrinsics bits (BaConfig::refine_intrinsics,
+# refine_distortion, refine_tangential_distortion, shared_focal)
+REFINE_INTRINSICS = 1
+REFINE_DISTORTION = 2
+REFINE_TANGENTIAL = 4
+REFINE_SHARED_FOCAL = 8
+
 SYNTH_CIRCLE = 1
 SYNTH_LOOP = 2
 SYNTH_STEREO_WINDOW = 3
@@ -325,6 +332,11 @@ SIGNATURES = {
     "vba_optimize_weighted": (
         C.c_int32,
         [C.c_void_p, C.POINTER(Problem), C.POINTER(Options), P64, C.c_uint64, P64, P64,
+         C.POINTER(IterationStats), C.c_uint32, C.POINTER(Result)],
+    ),
+    "vba_optimize_intrinsics": (
+        C.c_int32,
+        [C.c_void_p, C.POINTER(Problem), C.POINTER(Options), P64, P64, C.POINTER(Camera),
          C.POINTER(IterationStats), C.c_uint32, C.POINTER(Result)],
     ),
     "vba_default_gnc_options": (None, [C.POINTER(GncOptions)]),

@@ -366,9 +366,25 @@ class Device:
             out.message = self.last_error()
         return out
 
+    def optimize_intrinsics(self, problem: BundleProblem, options: abi.Options, cap: int = 1024):
+        """vba_optimize_intrinsics (BundleAdjustment::optimize with
+        options.refine_intrinsics = abi.REFINE_* bits): returns the solve and
+        the refined abi.Camera."""
+        cam = abi.Camera()
+
+        def call(view, opts, poses, lms, stats, cap_, res):
+            return self.lib.vba_optimize_intrinsics(
+                self.ctx, C.byref(view), C.byref(opts), poses.ctypes.data_as(abi.P64),
+                lms.ctypes.data_as(abi.P64), C.byref(cam), stats, cap_, C.byref(res))
+
+        out = run_solver(call, problem, options, cap)
+        if out.status != abi.OK:
+            out.message = self.last_error()
+        return out, cam
+
     def optimize_weighted(self, problem: BundleProblem, options: abi.Options, weights, cap: int = 1024) -> SolveOutput:
-        """vba_optimize_weighted: `weights`, one per observation (monocular,
-        then rig), multiply the robust-kernel weights
+        """vba_optimize_weighted: `weights`, one per observation in the
+        flattened order (monocular, rectified stereo, general stereo, rig), multiply the robust-kernel weights
         (BundleAdjustment::optimize_with_observation_weights)."""
         w = np.ascontiguousarray(weights, np.float64)
 
