@@ -413,6 +413,121 @@ def bench_serve(args):
     print(json.dumps(result), flush=True)
 
 
+def quat_rows(q):
+    w, x, y, z = q[:, 0], q[:, 1], q[:, 2], q[:, 3]
+    return np.stack([np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)], -1),
+                     np.stack([2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)], -1),
+                     np.stack([2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], -1)], 1)
+
+
+def with_stereo_terms(p, rng, rect=0.3, gen=0.2, baseline=0.12):
+    """The loop problem with 30 % of its observations turned rectified stereo
+    (u_r from the current depth) and 20 % general stereo (a rotated,
+    differently calibrated right camera), pixel noise 0.5 px."""
+    import dataclasses
+    fx, fy, cx, cy = (p.camera.params[i] for i in range(4))
+    cam = vba.camera(abi.CAMERA_PINHOLE, [fx, fy, cx, cy], p.camera.width, p.camera.height)
+    po, lo = np.argsort(p.pose_ids), np.argsort(p.landmark_ids)
+    pk = po[np.searchsorted(p.pose_ids[po], p.obs_pose_ids)]
+    lk = lo[np.searchsorted(p.landmark_ids[lo], p.obs_landmark_ids)]
+    R = quat_rows(p.poses[:, :4])
+    xc = np.einsum("nij,nj->ni", R[pk], p.landmarks[lk]) + p.poses[pk, 4:7]
+    lr_q = np.array([0.99995, 0.005, -0.008, 0.003])
+    lr_q /= np.linalg.norm(lr_q)
+    lr_t = np.array([-baseline, 0.002, 0.001])
+    xr = xc @ quat_rows(lr_q[None])[0].T + lr_t
+    kind = rng.choice(3, size=len(pk), p=[1 - rect - gen, rect, gen])
+    kind[(xc[:, 2] <= 0.1) | ((kind == 2) & (xr[:, 2] <= 0.1))] = 0
+    noise = rng.normal(0.0, 0.5, (len(pk), 2))
+    r, g = kind == 1, kind == 2
+    st = np.column_stack([p.obs_xy[r], p.obs_xy[r, 0] - fx * baseline / xc[r, 2] + noise[r, 0]])
+    gx = np.column_stack([p.obs_xy[g], 1.02 * fx * xr[g, 0] / xr[g, 2] + cx - 2 + noise[g, 0],
+                          0.99 * fy * xr[g, 1] / xr[g, 2] + cy + 2 + noise[g, 1]])
+    pair = vba.sensor(vba.camera(abi.CAMERA_PINHOLE, [1.02 * fx, 0.99 * fy, cx - 2, cy + 2]),
+                      np.concatenate([lr_q, lr_t]))
+    m = kind == 0
+    return dataclasses.replace(
+        p, camera=cam, obs_pose_ids=p.obs_pose_ids[m], obs_landmark_ids=p.obs_landmark_ids[m], obs_xy=p.obs_xy[m],
+        stereo_pose_ids=p.obs_pose_ids[r], stereo_landmark_ids=p.obs_landmark_ids[r], stereo_obs=st,
+        stereo_baseline=baseline, general_pose_ids=p.obs_pose_ids[g], general_landmark_ids=p.obs_landmark_ids[g],
+        general_obs=gx, general_pair=np.zeros(int(g.sum()), np.uint32), pairs=[pair])
+
+
+def bench_rows(args):
+    """Throughput of the entry points beside the default path, each on a
+    loop problem, one JSON line each: stereo observation kinds (mixed
+    matrix-free), observation weights, GNC, the joint camera solve (dense)
+    and the COLMAP binary round trip. Times are the caller's wall clock
+    around one-shot entry points (layout, upload and solve) and the device
+    time of the solve."""
+    import tempfile
+    rng = np.random.default_rng(0)
+    dev = vba.Device(0)
+    s = vba.synthesize(abi.SYNTH_LOOP, num_landmarks=args.landmarks // 4)
+    p = s.problem
+    mf = loop_options(abi.PRECISION_MIXED, abi.PCG_MAXIT_ACCEPT_ITERATE, abi.SOLVER_MATRIX_FREE)
+
+    def line(name, problem, out, wall, extra=None):
+        if out.status != abi.OK:
+            raise SystemExit(f"{name}: status {out.status} {out.message}")
+        it = out.result.num_iterations
+        rec = {"metric": f"{name} observation-iterations/s", "value": problem.num_observations * it / wall,
+               "unit": "obs*iter/s", "higher_is_better": True, "wall_s": wall,
+               "device_s": out.result.device_seconds, "lm_iterations": it,
+               "final_cost": out.result.final_cost, "observations": problem.num_observations,
+               "config": {"workload": f"loop, {len(problem.pose_ids)} cameras, {len(problem.landmark_ids)} landmarks"}}
+        rec.update(extra or {})
+        print(json.dumps(rec), flush=True)
+
+    st = with_stereo_terms(p, rng)
+    dev.optimize(st, mf)  # warm
+    t0 = time.perf_counter()
+    out = dev.optimize(st, mf)
+    line("stereo kinds (mixed, matrix-free)", st, out, time.perf_counter() - t0,
+         {"rectified": len(st.stereo_pose_ids), "general": len(st.general_pose_ids)})
+
+    w = rng.uniform(0.5, 1.0, p.num_observations)
+    t0 = time.perf_counter()
+    out = dev.optimize_weighted(p, mf, w)
+    line("weighted (mixed, matrix-free)", p, out, time.perf_counter() - t0)
+
+    q = p.copy()
+    bad = rng.choice(len(q.obs_xy), size=len(q.obs_xy) // 50, replace=False)
+    q.obs_xy[bad] += rng.uniform(-60.0, 60.0, (len(bad), 2))
+    g = vba.default_gnc_options(kernel=abi.GNC_TRUNCATED_LEAST_SQUARES, c=3.0, max_outer=8, inner_iterations=3)
+    t0 = time.perf_counter()
+    st_, res, _, _, _ = dev.optimize_gnc(q, mf, g)
+    wall = time.perf_counter() - t0
+    if st_ != abi.OK:
+        raise SystemExit(f"gnc: status {st_}")
+    print(json.dumps({"metric": "GNC solve (TLS, 8 levels x 3 LM iterations, mixed matrix-free)", "value": wall,
+                      "unit": "s", "higher_is_better": False, "outer_iterations": res.outer_iterations,
+                      "observations": q.num_observations, "final_cost": res.final_cost,
+                      "config": {"workload": f"loop with 2 % wrong matches, {len(q.pose_ids)} cameras"}}), flush=True)
+
+    sj = vba.synthesize(abi.SYNTH_LOOP, num_cameras=400, num_landmarks=args.landmarks // 8)
+    j = sj.problem.copy()
+    fx, fy, cx, cy = (j.camera.params[i] for i in range(4))
+    j.camera = vba.camera(abi.CAMERA_PINHOLE, [fx * 1.01, fy * 0.99, cx + 3, cy - 2], j.camera.width,
+                          j.camera.height)
+    oj = vba.default_options(refine_intrinsics=abi.REFINE_INTRINSICS | abi.REFINE_DISTORTION, max_iterations=10)
+    t0 = time.perf_counter()
+    out, cam = dev.optimize_intrinsics(j, oj)
+    line("joint camera solve (dense, k = 6)", j, out, time.perf_counter() - t0,
+         {"camera": list(cam.params[:cam.num_params])})
+
+    with tempfile.TemporaryDirectory() as d:
+        t0 = time.perf_counter()
+        vba.Model.from_problem(p).write_colmap(os.path.join(d, "m"), binary=True)
+        t1 = time.perf_counter()
+        model = vba.Model.read_colmap(os.path.join(d, "m"), binary=True)
+        prob, _ = model.problem()
+        t2 = time.perf_counter()
+    print(json.dumps({"metric": "COLMAP binary write + read into a problem", "value": p.num_observations / (t2 - t0),
+                      "unit": "obs/s", "higher_is_better": True, "write_s": t1 - t0, "read_s": t2 - t1,
+                      "observations": int(prob.num_observations)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -424,7 +539,7 @@ def main():
     ap.add_argument("--pcg-policy", choices=["accept", "reject"], default="accept",
                     help="at the PCG iteration cap: use the iterate (GPU backend) or reject the step (CPU "
                          "matrix-free entry)")
-    ap.add_argument("--workload", choices=["loop", "windows", "serve", "reduced"], default="loop",
+    ap.add_argument("--workload", choices=["loop", "windows", "serve", "reduced", "rows"], default="loop",
                     help="loop: the 3.5M-observation single problem (default); windows: a batch of local windows; "
                          "serve: the request loop under a Poisson stream")
     ap.add_argument("--rate", type=float, default=2000.0, help="requests per second (--workload serve)")
@@ -449,6 +564,9 @@ def main():
         return
     if args.workload == "reduced":
         bench_reduced(args)
+        return
+    if args.workload == "rows":
+        bench_rows(args)
         return
 
     t0 = time.perf_counter()
