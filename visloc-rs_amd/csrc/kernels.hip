@@ -643,12 +643,16 @@ __device__ void chol_blocked_solve(const double* S, int n, double* y) {
 // pose slots are var_slot[0 .. n/6). One work-group of NT threads.
 template <int NT>
 __device__ void dense_solve_body(const DevProblem& P, const int32_t* var_slot, double* S, double* work, int n,
-                                 int* status, int* fail_sh, double* piv_val, int* piv_idx, double* dblk) {
+                                 int* status, int* fail_sh, double* piv_val, int* piv_idx, double* dblk,
+                                 bool copied = false) {
     double* A2 = work;                 // n*n copy for the LU fallback
     double* y = work + (size_t)n * n;  // n
     // the last joint_k unknowns are the camera block of the joint solve
     const int nv = n - P.joint_k;
-    for (size_t k = threadIdx.x; k < (size_t)n * n; k += NT) A2[k] = S[k];
+    // the saved copy for the LU fallback, unless the whole device made it
+    // before this work-group started
+    if (!copied)
+        for (size_t k = threadIdx.x; k < (size_t)n * n; k += NT) A2[k] = S[k];
     for (int i = threadIdx.x; i < n; i += NT)
         y[i] = i < nv ? P.b[6 * (size_t)var_slot[i / 6] + (i % 6)] : P.joint_rhs[i - nv];
     if (threadIdx.x == 0) *fail_sh = 0;
@@ -747,7 +751,7 @@ __global__ __launch_bounds__(kSmallBlock) void k_dense_solve(DevProblem P, Dense
     __shared__ double piv_val[kSmallBlock];
     __shared__ int piv_idx[kSmallBlock];
     __shared__ double dblk[16 * 17];
-    dense_solve_body<kSmallBlock>(P, D.var_slot, S, work, n, status, &fail, piv_val, piv_idx, dblk);
+    dense_solve_body<kSmallBlock>(P, D.var_slot, S, work, n, status, &fail, piv_val, piv_idx, dblk, true);
 }
 
 // ---------------------------------------------------------------------------
@@ -1348,6 +1352,9 @@ void launch_dense_assemble(const DevProblem& P, const DensePairs& D, double* S, 
 }
 void launch_dense_solve(const DevProblem& P, const DensePairs& D, double* S, double* work, int32_t n,
                         int32_t* status, hipStream_t s) {
+    // the LU fallback's copy of S at full device bandwidth; the solve itself
+    // is one work-group
+    (void)hipMemcpyAsync(work, S, (size_t)n * n * sizeof(double), hipMemcpyDeviceToDevice, s);
     hipLaunchKernelGGL(k_dense_solve, dim3(1), dim3(kSmallBlock), 0, s, P, D, S, work, n, status);
 }
 void launch_batch_dense_assemble(const DevProblem& P, const DensePairs& D, const BatchDev& B, hipStream_t s) {
