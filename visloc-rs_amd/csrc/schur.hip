@@ -556,6 +556,13 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_schur_apply(DevProblem P, 
             }
         }
     }
+    // the z and d of this lane's first pose, in flight with the tile data
+    // (most lanes' positions belong to one camera run)
+    double z0[6], w0[6];
+    if (nv > 0) {
+        load6(v1 + 6 * (size_t)key[0], z0);
+        if (v2) load6(v2 + 6 * (size_t)key[0], w0);
+    }
     // V^-1 and X of the tile's landmarks go straight from global memory into
     // LDS (their layout there is the global one), in flight together with
     // the loads above: 16-byte pieces, lane i's piece at byte 16 i of each
@@ -603,7 +610,12 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_schur_apply(DevProblem P, 
         for (int i = 0; i < kApplyPer; ++i) {
             if (i < nv) {
                 if (key[i] != pcur) {
-                    load_d(v1, v2, beta, key[i], d);
+                    if (i == 0) {
+#pragma unroll
+                        for (int c = 0; c < 6; ++c) d[c] = v2 ? z0[c] + beta * w0[c] : z0[c];
+                    } else {
+                        load_d(v1, v2, beta, key[i], d);
+                    }
                     pcur = key[i];
                 }
                 const int j = (jl4 >> (8 * i)) & 0xff;
