@@ -475,6 +475,13 @@ int32_t vba_server_wait(vba_server* server, uint64_t ticket, double timeout_seco
                         double* landmarks_out, vba_result* result, double* latency_seconds);
 void vba_server_counters(vba_server* server, uint64_t* completed, uint64_t* batches,
                          uint64_t* batched_windows, uint64_t* queued);
+/* The device bytes the server counts against hbm_budget_bytes while it runs
+ * this request (an upper estimate of its buffers); 0 for a null argument.
+ * A batch holds the sum of its windows'. Work is admitted while the running
+ * total plus the request stays within the budget; a request larger than the
+ * budget runs only when nothing else does. */
+uint64_t vba_server_request_bytes(const vba_server* server, const vba_problem* problem,
+                                  const vba_options* options);
 /* Finishes the queued requests, then stops. */
 void vba_server_destroy(vba_server* server);
 
@@ -526,6 +533,17 @@ int32_t vba_get_tuning(const vba_context* ctx, vba_tuning* tuning);
 int32_t vba_autotune(vba_context* ctx, vba_prepared* prepared, const vba_options* options,
                      uint32_t lm_iterations, vba_tuning* best, double* best_seconds,
                      double* default_seconds);
+
+/* ---- dense normal equations ----------------------------------------------
+ * `solve_normal_equations` (pipelines/slam/src/pose_graph.rs:2259-2270), the
+ * dense solve behind LinearSolver::Dense (normal_equations.rs:1478), the
+ * joint-intrinsics solve (joint_intrinsics.rs:205-208) and the pose graphs
+ * (pose_graph.rs:619, sim3_pose_graph.rs:286): H x = b by Cholesky, and by
+ * LU with partial pivoting when H is not positive definite. H is n x n
+ * column-major (nalgebra's DMatrix layout); Cholesky reads its lower
+ * triangle. VBA_OK with x, or VBA_ERR_SINGULAR_SYSTEM
+ * (PoseGraphError::SingularSystem) when LU finds H singular. */
+int32_t vba_dense_solve(vba_context* ctx, uint32_t n, const double* h, const double* b, double* x);
 
 /* ---- reduced camera solve ------------------------------------------------
  * The iterative reduced-camera solve of the COLMAP-port rig solver,

@@ -71,6 +71,18 @@ def test_check_declines_what_the_library_does_not_implement():
     assert "declined" in abi.library().vba_last_error(None).decode()
 
 
+def test_check_declines_out_of_range_option_enums():
+    """Every enum of vba_options out of its range declines (the options half
+    of the support check, which the prepared solves also apply)."""
+    ba = scenes.truth_bundle()
+    for field, bad in (("robust_kernel", 3), ("linear_solver", 1), ("linear_solver", 4), ("precision", 2),
+                       ("pcg_maxit_policy", 2)):
+        assert vba.check(ba, vba.default_options(**{field: bad})) == abi.DECLINED, field
+    for field, good in (("linear_solver", abi.SOLVER_MATRIX_FREE), ("linear_solver", abi.SOLVER_PCG_EXPLICIT),
+                        ("precision", abi.PRECISION_MIXED), ("pcg_maxit_policy", abi.PCG_MAXIT_ACCEPT_ITERATE)):
+        assert vba.check(ba, vba.default_options(**{field: good})) == abi.OK, field
+
+
 def test_generators_are_deterministic_and_sized():
     a = vba.synthesize(abi.SYNTH_LOOP, num_cameras=50, num_landmarks=3000, seed=9, scene_scale=15.0)
     b = vba.synthesize(abi.SYNTH_LOOP, num_cameras=50, num_landmarks=3000, seed=9, scene_scale=15.0)

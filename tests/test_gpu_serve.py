@@ -65,3 +65,28 @@ def test_served_errors_and_pending(device):
     assert r_bad.status == abi.ERR_MISSING_LANDMARK
     assert r_bad.result.error_id == 10 ** 9
     assert r_good.status == abi.OK
+
+
+def test_admission_keeps_batches_within_the_hbm_budget(device):
+    """With a budget that holds two windows, no batch (nor two batches in
+    flight together) holds more; a request larger than the budget still
+    runs, by itself. Results stay those of the solo solves."""
+    cauchy = dict(robust_kernel=abi.KERNEL_CAUCHY, robust_scale=1.0)
+    p = vba.synthesize(abi.SYNTH_STEREO_WINDOW, num_cameras=8, num_landmarks=600, seed=70).problem
+    o = vba.default_options(max_iterations=6, **cauchy)
+    alone = device.optimize(p, o)
+    with vba.Server(window_streams=2, submap_streams=1, batch_wait_seconds=0.01) as probe:
+        est = probe.request_bytes(p, o)
+    assert est > 0
+    for budget, per_batch in ((int(2.5 * est), 2), (est // 2, 1)):
+        with vba.Server(window_streams=2, submap_streams=1, batch_wait_seconds=0.01,
+                        hbm_budget_bytes=budget) as srv:
+            tickets = [srv.submit(p, o) for _ in range(8)]
+            served = [srv.wait(t) for t in tickets]
+            c = srv.counters()
+        assert c["completed"] == 8 and c["batched_windows"] == 8
+        assert c["batched_windows"] <= per_batch * c["batches"], (budget, c)
+        for got in served:
+            assert got.status == abi.OK
+            assert got.result.final_cost == alone.result.final_cost
+            assert np.array_equal(got.poses, alone.poses)

@@ -74,3 +74,30 @@ def test_tuning_rejects_out_of_range(device):
     with pytest.raises(ValueError):
         device.set_tuning(10, 3, False)
     assert device.tuning() == {"pcg_chunk": 10, "pcg_lookahead": 0, "pcg_predict": 0}
+
+
+def test_prepared_solves_decline_out_of_range_options(device):
+    """A prepared problem or batch applies the option checks of vba_check to
+    every solve: out-of-range enums and intrinsics refinement decline
+    instead of running with a default."""
+    s = vba.synthesize(abi.SYNTH_LOOP, num_cameras=20, num_landmarks=600, seed=5, scene_scale=12.0)
+    o = loop_options(abi.SOLVER_MATRIX_FREE, abi.PRECISION_FP64)
+    pp = device.prepare(s.problem, o)
+    bad = [dict(linear_solver=1), dict(linear_solver=9), dict(robust_kernel=5), dict(precision=3),
+           dict(pcg_maxit_policy=4), dict(refine_intrinsics=1)]
+    for fields in bad:
+        ob = loop_options(abi.SOLVER_MATRIX_FREE, abi.PRECISION_FP64)
+        for k, v in fields.items():
+            setattr(ob, k, v)
+        out = pp.solve(ob)
+        assert out.status == abi.DECLINED, fields
+        assert "declined" in out.message
+        with pytest.raises(RuntimeError):
+            pp.autotune(ob, lm_iterations=1)
+    assert pp.solve(o).status == abi.OK
+    w = vba.synthesize(abi.SYNTH_STEREO_WINDOW, num_cameras=6, num_landmarks=300, seed=3).problem
+    od = vba.default_options(max_iterations=3)
+    pb = device.prepare_batch([w], od)
+    ob = vba.default_options(max_iterations=3, robust_kernel=5)
+    assert pb.solve(ob).status == abi.DECLINED
+    assert pb.solve(od).status == abi.OK

@@ -459,3 +459,33 @@ def test_circle_lm_converges_and_recovers_cameras():
     before = np.linalg.norm(vba.camera_centres(s.problem.poses) - vba.camera_centres(s.truth_poses), axis=1)
     after = np.linalg.norm(vba.camera_centres(out.poses) - vba.camera_centres(s.truth_poses), axis=1)
     assert after.max() < before.max()
+
+
+def test_solve_normal_equations_cholesky_lu_and_singular():
+    """solve_normal_equations (pose_graph.rs:2259-2270): Cholesky on the
+    lower triangle when H is positive definite, LU with partial pivoting when
+    it is not, None when LU meets a zero pivot."""
+    rng = np.random.default_rng(11)
+    a = rng.standard_normal((30, 30))
+    h = a @ a.T + 30 * np.eye(30)
+    b = rng.standard_normal(30)
+    x = oracle.solve_normal_equations(h, b)
+    assert np.abs(x - np.linalg.solve(h, b)).max() <= 1e-12 * np.abs(x).max()
+    # Cholesky reads the lower triangle only
+    hu = h.copy()
+    hu[np.triu_indices(30, 1)] = 1e3
+    assert np.array_equal(oracle.solve_normal_equations(hu, b), x)
+    # a zero first pivot: Cholesky fails, LU swaps rows; exact answer
+    h3 = np.array([[0.0, 1.0, 0.0], [1.0, 0.0, 0.0], [0.0, 0.0, -1.0]])
+    assert np.array_equal(oracle.solve_normal_equations(h3, np.array([1.0, 2.0, 3.0])), [2.0, 1.0, -3.0])
+    # symmetric indefinite: LU against numpy (LAPACK getrf, the same pivots)
+    q, _ = np.linalg.qr(rng.standard_normal((25, 25)))
+    hi = (q * np.linspace(-2.0, 2.5, 25)) @ q.T
+    bi = rng.standard_normal(25)
+    xi = oracle.solve_normal_equations(hi, bi)
+    assert np.abs(xi - np.linalg.solve(hi, bi)).max() <= 1e-9 * np.abs(xi).max()
+    # singular: a zero row and column
+    hs = h.copy()
+    hs[4, :] = 0.0
+    hs[:, 4] = 0.0
+    assert oracle.solve_normal_equations(hs, b) is None

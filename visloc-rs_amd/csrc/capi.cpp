@@ -390,6 +390,10 @@ int32_t vba_solve_prepared(vba_context* ctx, vba_prepared* prepared, const vba_o
     std::memset(result, 0, sizeof(*result));
     ctx->last_error.clear();
     std::string msg;
+    if (vba::check_options(*options, &msg) != VBA_OK) {
+        ctx->last_error = "declined: " + msg;
+        return VBA_DECLINED;
+    }
     if (options->linear_solver != VBA_SOLVER_DENSE) {
         int32_t st = vba::validate_matrix_free(prepared->m, *options, &msg);
         if (st != VBA_OK) {
@@ -527,6 +531,11 @@ int32_t vba_solve_prepared_batch(vba_context* ctx, vba_prepared_batch* batch, co
     }
     std::lock_guard<std::mutex> lock(ctx->mu);
     ctx->last_error.clear();
+    std::string reason;
+    if (vba::check_options(*options, &reason) != VBA_OK) {
+        ctx->last_error = "declined: " + reason;
+        return VBA_DECLINED;
+    }
     if (options->linear_solver != VBA_SOLVER_DENSE) {
         ctx->last_error = "batched windows use the dense reduced solve (linear_solver = VBA_SOLVER_DENSE)";
         return VBA_ERR_INVALID_CONFIGURATION;
@@ -570,6 +579,26 @@ void vba_profile_reset(vba_context* ctx) {
         k.launches = 0;
         k.seconds = 0.0;
         k.bytes = 0.0;
+    }
+}
+
+int32_t vba_dense_solve(vba_context* ctx, uint32_t n, const double* h, const double* b, double* x) {
+    if (!ctx || (n && (!h || !b || !x))) {
+        vba::set_thread_error("vba_dense_solve: null argument");
+        if (ctx) ctx->last_error = "vba_dense_solve: null argument";
+        return VBA_ERR_INVALID_ARGUMENT;
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->last_error.clear();
+    if (hipSetDevice(ctx->device) != hipSuccess) {
+        ctx->last_error = "hipSetDevice failed";
+        return VBA_ERR_DEVICE;
+    }
+    try {
+        return vba::dense_solve(*ctx, n, h, b, x, &ctx->last_error);
+    } catch (const std::bad_alloc&) {
+        ctx->last_error = "out of host memory";
+        return VBA_ERR_OUT_OF_MEMORY;
     }
 }
 
@@ -637,6 +666,14 @@ int32_t vba_get_tuning(const vba_context* ctx, vba_tuning* tuning) {
 int32_t vba_autotune(vba_context* ctx, vba_prepared* prepared, const vba_options* options, uint32_t lm_iterations,
                      vba_tuning* best, double* best_seconds, double* default_seconds) {
     if (!ctx || !prepared || !options) return VBA_ERR_INVALID_ARGUMENT;
+    {
+        std::string reason;
+        if (vba::check_options(*options, &reason) != VBA_OK) {
+            std::lock_guard<std::mutex> lock(ctx->mu);
+            ctx->last_error = "declined: " + reason;
+            return VBA_DECLINED;
+        }
+    }
     if (options->linear_solver == VBA_SOLVER_DENSE) {
         std::lock_guard<std::mutex> lock(ctx->mu);
         ctx->last_error = "vba_autotune: the schedule applies to the PCG solvers";

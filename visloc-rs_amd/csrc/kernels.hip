@@ -637,24 +637,45 @@ __device__ void chol_blocked_solve(const double* S, int n, double* y) {
     }
 }
 
+// Where a dense solve's right-hand side comes from and its solution goes:
+// unknown i < nv is row var_slot[i / 6] * 6 + i % 6 of b / x (row i when
+// var_slot is null), unknown i >= nv is row i - nv of tail_b / tail_x (the
+// camera block of the joint solve).
+struct DenseRows {
+    const double* b;
+    double* x;
+    const int32_t* var_slot;
+    int nv;
+    const double* tail_b;
+    double* tail_x;
+    __device__ __forceinline__ size_t row(int i) const {
+        return var_slot ? 6 * (size_t)var_slot[i / 6] + (i % 6) : (size_t)i;
+    }
+    __device__ __forceinline__ double rhs(int i) const { return i < nv ? b[row(i)] : tail_b[i - nv]; }
+    __device__ __forceinline__ void put(int i, double v) const {
+        if (i < nv) x[row(i)] = v;
+        else tail_x[i - nv] = v;
+    }
+};
+__device__ __forceinline__ DenseRows pose_rows(const DevProblem& P, const int32_t* var_slot, int n) {
+    // the last joint_k unknowns are the camera block of the joint solve
+    return DenseRows{P.b, P.x, var_slot, n - P.joint_k, P.joint_rhs, P.joint_delta};
+}
+
 // Cholesky (blocked, MFMA trailing updates), then forward/back
 // substitution; on a non-positive pivot an LU with partial pivoting on the
-// saved copy. Solves S x = b for the n = 6 * (variable poses) unknowns whose
-// pose slots are var_slot[0 .. n/6). One work-group of NT threads.
+// saved copy. Solves S x = b for the n unknowns of `io`. One work-group of
+// NT threads.
 template <int NT>
-__device__ void dense_solve_body(const DevProblem& P, const int32_t* var_slot, double* S, double* work, int n,
-                                 int* status, int* fail_sh, double* piv_val, int* piv_idx, double* dblk,
-                                 bool copied = false) {
+__device__ void dense_solve_body(const DenseRows& io, double* S, double* work, int n, int* status, int* fail_sh,
+                                 double* piv_val, int* piv_idx, double* dblk, bool copied = false) {
     double* A2 = work;                 // n*n copy for the LU fallback
     double* y = work + (size_t)n * n;  // n
-    // the last joint_k unknowns are the camera block of the joint solve
-    const int nv = n - P.joint_k;
     // the saved copy for the LU fallback, unless the whole device made it
     // before this work-group started
     if (!copied)
         for (size_t k = threadIdx.x; k < (size_t)n * n; k += NT) A2[k] = S[k];
-    for (int i = threadIdx.x; i < n; i += NT)
-        y[i] = i < nv ? P.b[6 * (size_t)var_slot[i / 6] + (i % 6)] : P.joint_rhs[i - nv];
+    for (int i = threadIdx.x; i < n; i += NT) y[i] = io.rhs(i);
     if (threadIdx.x == 0) *fail_sh = 0;
     __syncthreads();
     chol_blocked<NT>(S, n, dblk, fail_sh);
@@ -663,8 +684,7 @@ __device__ void dense_solve_body(const DevProblem& P, const int32_t* var_slot, d
         chol_blocked_solve<NT>(S, n, y);
     } else {
         // LU with partial pivoting on A2; y holds b again.
-        for (int i = threadIdx.x; i < n; i += NT)
-            y[i] = i < nv ? P.b[6 * (size_t)var_slot[i / 6] + (i % 6)] : P.joint_rhs[i - nv];
+        for (int i = threadIdx.x; i < n; i += NT) y[i] = io.rhs(i);
         __syncthreads();
         for (int k = 0; k < n; ++k) {
             double best = -1.0;
@@ -737,11 +757,7 @@ __device__ void dense_solve_body(const DevProblem& P, const int32_t* var_slot, d
     }
     __syncthreads();
     const int f = *fail_sh;
-    for (int i = threadIdx.x; i < n; i += NT) {
-        const double v = (f != 2) ? y[i] : 0.0;
-        if (i < nv) P.x[6 * (size_t)var_slot[i / 6] + (i % 6)] = v;
-        else P.joint_delta[i - nv] = v;
-    }
+    for (int i = threadIdx.x; i < n; i += NT) io.put(i, (f != 2) ? y[i] : 0.0);
     if (threadIdx.x == 0) *status = (f == 2) ? 1 : 0;
 }
 
@@ -751,7 +767,19 @@ __global__ __launch_bounds__(kSmallBlock) void k_dense_solve(DevProblem P, Dense
     __shared__ double piv_val[kSmallBlock];
     __shared__ int piv_idx[kSmallBlock];
     __shared__ double dblk[16 * 17];
-    dense_solve_body<kSmallBlock>(P, D.var_slot, S, work, n, status, &fail, piv_val, piv_idx, dblk, true);
+    dense_solve_body<kSmallBlock>(pose_rows(P, D.var_slot, n), S, work, n, status, &fail, piv_val, piv_idx, dblk,
+                                  true);
+}
+
+// S x = b for a caller's row-major S (overwritten); work holds n * n + n.
+__global__ __launch_bounds__(kSmallBlock) void k_dense_solve_plain(double* S, double* work, const double* b,
+                                                                   double* x, int n, int* status) {
+    __shared__ int fail;
+    __shared__ double piv_val[kSmallBlock];
+    __shared__ int piv_idx[kSmallBlock];
+    __shared__ double dblk[16 * 17];
+    const DenseRows io{b, x, nullptr, n, nullptr, nullptr};
+    dense_solve_body<kSmallBlock>(io, S, work, n, status, &fail, piv_val, piv_idx, dblk, true);
 }
 
 // ---------------------------------------------------------------------------
@@ -793,8 +821,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_dense_solve(DevProblem P, Dens
         return;
     }
     __shared__ double dblk[16 * 17];
-    dense_solve_body<kBlock>(P, D.var_slot + L.var_begin, B.S + L.s_offset, B.work + B.work_offset[w], n,
-                             &st.solve_status, &fail, piv_val, piv_idx, dblk);
+    dense_solve_body<kBlock>(pose_rows(P, D.var_slot + L.var_begin, n), B.S + L.s_offset, B.work + B.work_offset[w],
+                             n, &st.solve_status, &fail, piv_val, piv_idx, dblk);
 }
 
 // Per window: compensated cost and non-projectable count over its camera
@@ -1350,12 +1378,21 @@ void launch_dense_assemble(const DevProblem& P, const DensePairs& D, double* S, 
         hipLaunchKernelGGL(k_dense_assemble, dim3(blocks_for(D.num_blocks, kBlock / kBlockLanes)), dim3(kBlock), 0, s, P,
                            D, S, n, lambda);
 }
-void launch_dense_solve(const DevProblem& P, const DensePairs& D, double* S, double* work, int32_t n,
-                        int32_t* status, hipStream_t s) {
+hipError_t launch_dense_solve(const DevProblem& P, const DensePairs& D, double* S, double* work, int32_t n,
+                              int32_t* status, hipStream_t s) {
     // the LU fallback's copy of S at full device bandwidth; the solve itself
-    // is one work-group
-    (void)hipMemcpyAsync(work, S, (size_t)n * n * sizeof(double), hipMemcpyDeviceToDevice, s);
+    // is one work-group and is not launched when the copy fails
+    const hipError_t e = hipMemcpyAsync(work, S, (size_t)n * n * sizeof(double), hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_dense_solve, dim3(1), dim3(kSmallBlock), 0, s, P, D, S, work, n, status);
+    return hipGetLastError();
+}
+hipError_t launch_dense_solve_plain(double* S, double* work, const double* b, double* x, int32_t n, int32_t* status,
+                                    hipStream_t s) {
+    const hipError_t e = hipMemcpyAsync(work, S, (size_t)n * n * sizeof(double), hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_dense_solve_plain, dim3(1), dim3(kSmallBlock), 0, s, S, work, b, x, n, status);
+    return hipGetLastError();
 }
 void launch_batch_dense_assemble(const DevProblem& P, const DensePairs& D, const BatchDev& B, hipStream_t s) {
     if (D.num_blocks)

@@ -256,8 +256,13 @@ size_t joint_part_size(const DevProblem& P);
 void launch_dense_assemble(const DevProblem& P, const DensePairs& D, double* S, int32_t n,
                            double lambda, hipStream_t stream);
 // Solve S x = b (Cholesky, LU fallback) for the variable poses; writes x and
-// sets *status (0 ok, 1 singular). S is overwritten.
-void launch_dense_solve(const DevProblem& P, const DensePairs& D, double* S, double* work,
+// sets *status (0 ok, 1 singular). S is overwritten; `work` receives a copy of
+// S for the LU fallback. Returns the error of the copy or of the launch.
+// The same solve for a plain system: S row-major n x n (overwritten), b and x
+// n long, work n * n + n.
+hipError_t launch_dense_solve_plain(double* S, double* work, const double* b, double* x, int32_t n,
+                                    int32_t* status, hipStream_t stream);
+hipError_t launch_dense_solve(const DevProblem& P, const DensePairs& D, double* S, double* work,
                         int32_t n, int32_t* status, hipStream_t stream);
 
 // ---- batches of windows (one work-group per window where noted) ----

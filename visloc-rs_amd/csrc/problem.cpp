@@ -188,9 +188,9 @@ bool lens_from_camera(const vba_camera& c, Lens<double>* L) {
     return true;
 }
 
-int32_t check_support(const vba_problem& pb, const vba_options& o, std::string* reason) {
-    // GpuBundleAdjuster::check (crates/ba-gpu/src/solver.rs:212-251) widened to
-    // the lens models, rig observations and kernels implemented here.
+int32_t check_options(const vba_options& o, std::string* reason) {
+    // The option half of GpuBundleAdjuster::check (crates/ba-gpu/src/solver.rs:212-251),
+    // also applied to the options of every solve of a prepared problem or batch.
     if (o.refine_intrinsics) {
         *reason = "intrinsics refinement";
         return VBA_DECLINED;
@@ -208,6 +208,18 @@ int32_t check_support(const vba_problem& pb, const vba_options& o, std::string* 
         *reason = "precision";
         return VBA_DECLINED;
     }
+    if (o.pcg_maxit_policy > VBA_PCG_MAXIT_ACCEPT_ITERATE) {
+        *reason = "PCG iteration-cap policy";
+        return VBA_DECLINED;
+    }
+    return VBA_OK;
+}
+
+int32_t check_support(const vba_problem& pb, const vba_options& o, std::string* reason) {
+    // GpuBundleAdjuster::check (crates/ba-gpu/src/solver.rs:212-251) widened to
+    // the lens models, rig observations and kernels implemented here.
+    const int32_t st = check_options(o, reason);
+    if (st != VBA_OK) return st;
     if (pb.declined_features & (VBA_FEATURE_STEREO_OBSERVATIONS | VBA_FEATURE_GENERAL_STEREO)) {
         *reason = "stereo observations not passed in vba_problem.stereo";
         return VBA_DECLINED;

@@ -152,6 +152,9 @@ int32_t build_model(const vba_problem& problem, Model* model, uint64_t* error_id
                     std::string* message);
 
 // vba_check: VBA_OK or VBA_DECLINED with a reason.
+// Options only: kernel, solver, precision and PCG policy enums in range and no
+// intrinsics refinement. VBA_OK or VBA_DECLINED with the reason.
+int32_t check_options(const vba_options& options, std::string* reason);
 int32_t check_support(const vba_problem& problem, const vba_options& options, std::string* reason);
 
 }  // namespace vba

@@ -156,7 +156,11 @@ struct vba_server {
     std::vector<std::thread> threads;
     std::vector<vba_context*> contexts;
 
-    bool fits(uint64_t bytes) const { return hbm_in_use == 0 || hbm_in_use + bytes <= cfg.hbm_budget_bytes; }
+    // `bytes` more of resident work stays within the budget.
+    bool fits(uint64_t bytes) const { return hbm_in_use + bytes <= cfg.hbm_budget_bytes; }
+    // A lane may start a request of `bytes`: within the budget, or alone on an
+    // idle device (a request larger than the budget still runs, by itself).
+    bool admits(uint64_t bytes) const { return hbm_in_use == 0 || fits(bytes); }
 
     void finish(const std::shared_ptr<Request>& r) {
         r->finished = Clock::now();
@@ -171,7 +175,7 @@ struct vba_server {
             uint64_t bytes = 0;
             {
                 std::unique_lock<std::mutex> lock(mu);
-                cv_work.wait(lock, [&] { return stopping || (!window_q.empty() && fits(window_q.front()->est_bytes)); });
+                cv_work.wait(lock, [&] { return stopping || (!window_q.empty() && admits(window_q.front()->est_bytes)); });
                 if (stopping && window_q.empty()) return;
                 if (window_q.size() < cfg.max_window_batch && cfg.batch_wait_seconds > 0.0) {
                     const auto until =
@@ -179,12 +183,16 @@ struct vba_server {
                                            std::chrono::duration<double>(cfg.batch_wait_seconds));
                     cv_work.wait_until(lock, until,
                                        [&] { return stopping || window_q.size() >= cfg.max_window_batch; });
-                    if (window_q.empty()) continue;
+                    // the lock was released: another lane may have taken the
+                    // queue's head or admitted work meanwhile
+                    if (window_q.empty() || !admits(window_q.front()->est_bytes)) continue;
                 }
+                // the head is admitted as admits() allows; every later window
+                // only within the budget, counting the batch gathered so far
                 const vba_options key = window_q.front()->opts;
                 for (auto it = window_q.begin(); it != window_q.end() && batch.size() < cfg.max_window_batch;) {
                     if (std::memcmp(&(*it)->opts, &key, sizeof(key)) == 0 &&
-                        (batch.empty() || fits(bytes + (*it)->est_bytes))) {
+                        (batch.empty() ? admits((*it)->est_bytes) : fits(bytes + (*it)->est_bytes))) {
                         bytes += (*it)->est_bytes;
                         batch.push_back(*it);
                         it = window_q.erase(it);
@@ -276,7 +284,7 @@ struct vba_server {
             std::shared_ptr<Request> r;
             {
                 std::unique_lock<std::mutex> lock(mu);
-                cv_work.wait(lock, [&] { return stopping || (!submap_q.empty() && fits(submap_q.front()->est_bytes)); });
+                cv_work.wait(lock, [&] { return stopping || (!submap_q.empty() && admits(submap_q.front()->est_bytes)); });
                 if (stopping && submap_q.empty()) return;
                 r = submap_q.front();
                 submap_q.pop_front();
@@ -345,6 +353,13 @@ vba_server* vba_server_create(const vba_server_config* config) {
 
 namespace {
 
+// Dense solves of at most 256 variable poses go to the window lane.
+bool window_lane(const vba_server& s, const vba_problem& problem, const vba_options& options) {
+    uint64_t var = 0;
+    for (uint64_t i = 0; i < problem.num_poses; ++i) var += problem.pose_fixed ? (problem.pose_fixed[i] == 0) : 1;
+    return options.linear_solver == VBA_SOLVER_DENSE && var <= 256 && s.cfg.window_streams > 0;
+}
+
 uint64_t submit(vba_server* s, const vba_problem* problem, const vba_options* options, bool borrow) {
     if (!s || !problem || !options) {
         vba::set_thread_error("vba_server_submit: null argument");
@@ -359,9 +374,7 @@ uint64_t submit(vba_server* s, const vba_problem* problem, const vba_options* op
         return 0;
     }
     r->opts = *options;
-    uint64_t var = 0;
-    for (uint64_t i = 0; i < problem->num_poses; ++i) var += problem->pose_fixed ? (problem->pose_fixed[i] == 0) : 1;
-    r->window_lane = options->linear_solver == VBA_SOLVER_DENSE && var <= 256 && s->cfg.window_streams > 0;
+    r->window_lane = window_lane(*s, *problem, *options);
     if (!r->window_lane && s->cfg.submap_streams == 0) {
         vba::set_thread_error("vba_server_submit: no lane serves this request");
         return 0;
@@ -416,6 +429,11 @@ int32_t vba_server_wait(vba_server* s, uint64_t ticket, double timeout_seconds, 
         if (landmarks_out) std::memcpy(landmarks_out, r->lms_out.data(), r->lms_out.size() * sizeof(double));
     }
     return r->status;
+}
+
+uint64_t vba_server_request_bytes(const vba_server* s, const vba_problem* problem, const vba_options* options) {
+    if (!s || !problem || !options) return 0;
+    return estimate_bytes(*problem, window_lane(*s, *problem, *options));
 }
 
 void vba_server_counters(vba_server* s, uint64_t* completed, uint64_t* batches, uint64_t* batched_windows,

@@ -438,6 +438,51 @@ int32_t prepare(Context& ctx, Prepared& pr, std::string* msg) {
     }
 }
 
+int32_t dense_solve(Context& ctx, uint32_t n, const double* h, const double* b, double* x, std::string* msg) {
+    if (n == 0) return VBA_OK;
+    hipStream_t s = ctx.stream;
+    const size_t nn = (size_t)n * n;
+    // row-major copy of the column-major H: the kernels' lower triangle is
+    // the caller's lower triangle, which is all Cholesky reads
+    std::vector<double> rows(nn);
+    for (size_t c = 0; c < n; ++c)
+        for (size_t r = 0; r < n; ++r) rows[r * n + c] = h[c * n + r];
+    DevBuffer bS, bwork, bb, bx, bst;
+    auto alloc = [](DevBuffer& d, size_t bytes) {
+        hip_check(hipMalloc(&d.ptr, bytes), "hipMalloc");
+        d.bytes = bytes;
+    };
+    int32_t code = VBA_OK;
+    try {
+        if (!ctx.host_status)
+            hip_check(hipHostMalloc((void**)&ctx.host_status, sizeof(int32_t), hipHostMallocDefault), "hipHostMalloc");
+        alloc(bS, nn * sizeof(double));
+        alloc(bwork, (nn + n) * sizeof(double));
+        alloc(bb, n * sizeof(double));
+        alloc(bx, n * sizeof(double));
+        alloc(bst, sizeof(int32_t));
+        hip_check(hipMemcpyAsync(bS.ptr, rows.data(), nn * sizeof(double), hipMemcpyHostToDevice, s), "upload");
+        hip_check(hipMemcpyAsync(bb.ptr, b, n * sizeof(double), hipMemcpyHostToDevice, s), "upload");
+        hip_check(launch_dense_solve_plain((double*)bS.ptr, (double*)bwork.ptr, (const double*)bb.ptr,
+                                           (double*)bx.ptr, (int32_t)n, (int32_t*)bst.ptr, s),
+                  "dense solve");
+        hip_check(hipMemcpyAsync(ctx.host_status, bst.ptr, sizeof(int32_t), hipMemcpyDeviceToHost, s), "D2H");
+        hip_check(hipMemcpyAsync(x, bx.ptr, n * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+        hip_check(hipStreamSynchronize(s), "sync");
+        if (*ctx.host_status != 0) {
+            *msg = "singular system: neither Cholesky nor LU with partial pivoting solves it";
+            code = VBA_ERR_SINGULAR_SYSTEM;
+        }
+    } catch (const Fail& f) {
+        *msg = f.msg;
+        (void)hipStreamSynchronize(s);
+        code = f.code;
+    }
+    for (DevBuffer* d : {&bS, &bwork, &bb, &bx, &bst})
+        if (d->ptr) (void)hipFree(d->ptr);
+    return code;
+}
+
 int32_t download_state(Context& ctx, const Prepared& pr, double* poses_out, double* landmarks_out) {
     hipStream_t s = ctx.stream;
     const Model& m = pr.m;
@@ -781,8 +826,7 @@ int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stat
                 hip_check(hipMemsetAsync(dS, 0, (size_t)ndense * ndense * sizeof(double), s), "memset");
                 launch_dense_assemble(P, D, dS, ndense, lambda, s);
                 if (JK) launch_joint_border(P, dS, ndense, lambda, s);
-                launch_dense_solve(P, D, dS, dwork, ndense, dstatus, s);
-                hip_check(hipGetLastError(), "kernel launch");
+                hip_check(launch_dense_solve(P, D, dS, dwork, ndense, dstatus, s), "dense solve");
                 hip_check(hipMemcpyAsync(ctx.host_status, dstatus, sizeof(int32_t), hipMemcpyDeviceToHost, s), "D2H");
                 hip_check(hipStreamSynchronize(s), "sync");
                 solved = *ctx.host_status == 0;

@@ -104,6 +104,11 @@ struct JointSpec {
 int32_t run(Context& ctx, Prepared& prep, const vba_options& o, vba_iteration_stats* stats,
             uint32_t stats_capacity, vba_result* result, const JointSpec* joint = nullptr);
 
+// solve_normal_equations (pose_graph.rs:2259-2270) on the device: H x = b
+// for a column-major n x n H, Cholesky with the LU fallback.
+// VBA_ERR_SINGULAR_SYSTEM when both fail; x is then all zero.
+int32_t dense_solve(Context& ctx, uint32_t n, const double* h, const double* b, double* x, std::string* message);
+
 // Final (or, before any solve, initial) poses / landmarks in input order.
 int32_t download_state(Context& ctx, const Prepared& prep, double* poses_out, double* landmarks_out);
 

@@ -327,7 +327,9 @@ SIGNATURES = {
     "vba_server_wait": (C.c_int32, [C.c_void_p, C.c_uint64, C.c_double, P64, P64, C.POINTER(Result),
                                     C.POINTER(C.c_double)]),
     "vba_server_counters": (None, [C.c_void_p, PU64, PU64, PU64, PU64]),
+    "vba_server_request_bytes": (C.c_uint64, [C.c_void_p, C.POINTER(Problem), C.POINTER(Options)]),
     "vba_server_destroy": (None, [C.c_void_p]),
+    "vba_dense_solve": (C.c_int32, [C.c_void_p, C.c_uint32, P64, P64, P64]),
     "vba_reduced_solve": (C.c_int32, [C.c_void_p, C.POINTER(ReducedSystem), P64, C.POINTER(C.c_uint32)]),
     "vba_optimize_weighted": (
         C.c_int32,

@@ -411,6 +411,18 @@ class Device:
                                        w.ctypes.data_as(abi.P64), C.byref(res))
         return st, res, poses, lms, w
 
+    def dense_solve(self, h, b):
+        """solve_normal_equations on the device (vba_dense_solve): returns
+        (status, x); x is None unless the status is OK."""
+        h = np.asfortranarray(h, np.float64)
+        b = np.ascontiguousarray(b, np.float64)
+        n = len(b)
+        assert h.shape == (n, n)
+        x = np.zeros(n)
+        st = self.lib.vba_dense_solve(self.ctx, n, h.ravel(order="F").ctypes.data_as(abi.P64),
+                                      b.ctypes.data_as(abi.P64), x.ctypes.data_as(abi.P64))
+        return st, (x if st == abi.OK else None)
+
     def reduced_solve(self, diag, edges, offdiag, rhs, tol: float, max_iterations: int = 500, clusters=None,
                       coarse=None, structure_id: int = 0):
         """vba_reduced_solve, `ReducedSolveAccelerator::solve`
@@ -684,6 +696,11 @@ class Server:
         self.borrowed.pop(ticket, None)
         ok = st == abi.OK
         return ServedResult(st, res, poses if ok else None, lms if ok else None, lat.value)
+
+    def request_bytes(self, problem: BundleProblem, options: abi.Options) -> int:
+        """Device bytes the server counts against its HBM budget for this request."""
+        view = problem.view()
+        return int(self.lib.vba_server_request_bytes(self.handle, C.byref(view), C.byref(options)))
 
     def counters(self) -> dict:
         v = [C.c_uint64() for _ in range(4)]
