@@ -545,6 +545,20 @@ int32_t vba_autotune(vba_context* ctx, vba_prepared* prepared, const vba_options
  * (PoseGraphError::SingularSystem) when LU finds H singular. */
 int32_t vba_dense_solve(vba_context* ctx, uint32_t n, const double* h, const double* b, double* x);
 
+/* vba_dense_solve without device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments, layout, results and errors as
+ * vba_dense_solve, for n <= VBA_DENSE_NOALLOC_MAX_N unknowns
+ * (VBA_ERR_INVALID_ARGUMENT beyond). Every device buffer of the solve lives
+ * in one fixed device array of the library (about 16 MiB for the maximum n,
+ * held on each GPU the library runs on until the process ends) and the
+ * context's pinned status word, both set up by vba_create; the call
+ * allocates nothing. All calls on one GPU share that array: calls that
+ * overlap on one GPU, from any context or thread, race. */
+#define VBA_DENSE_NOALLOC_MAX_N 1024u
+int32_t vba_dense_solve_noalloc(vba_context* ctx, uint32_t n, const double* h, const double* b, double* x);
+
 /* ---- reduced camera solve ------------------------------------------------
  * The iterative reduced-camera solve of the COLMAP-port rig solver,
  * `ReducedSolveAccelerator::solve` (pipelines/slam/src/ba_accel.rs:46-157;

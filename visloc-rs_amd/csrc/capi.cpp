@@ -93,6 +93,17 @@ vba_context* vba_create(int32_t device) {
         vba::set_thread_error("vba_create: stream/event creation failed");
         return nullptr;
     }
+    // the pinned status word of the dense solves, and the device address of
+    // the no-allocation solve's array (looked up once per device)
+    if (hipHostMalloc((void**)&ctx->host_status, sizeof(int32_t), hipHostMallocDefault) != hipSuccess) {
+        ctx->host_status = nullptr;
+        vba::set_thread_error("vba_create: pinned status allocation failed");
+        (void)hipEventDestroy(ctx->ev_start);
+        (void)hipEventDestroy(ctx->ev_stop);
+        (void)hipStreamDestroy(ctx->stream);
+        return nullptr;
+    }
+    (void)vba::scratch_setup(device);
     return ctx.release();
 }
 
@@ -600,6 +611,21 @@ int32_t vba_dense_solve(vba_context* ctx, uint32_t n, const double* h, const dou
         ctx->last_error = "out of host memory";
         return VBA_ERR_OUT_OF_MEMORY;
     }
+}
+
+int32_t vba_dense_solve_noalloc(vba_context* ctx, uint32_t n, const double* h, const double* b, double* x) {
+    if (!ctx || (n && (!h || !b || !x))) {
+        vba::set_thread_error("vba_dense_solve_noalloc: null argument");
+        if (ctx) ctx->last_error = "vba_dense_solve_noalloc: null argument";
+        return VBA_ERR_INVALID_ARGUMENT;
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->last_error.clear();
+    if (hipSetDevice(ctx->device) != hipSuccess) {
+        ctx->last_error = "hipSetDevice failed";
+        return VBA_ERR_DEVICE;
+    }
+    return vba::dense_solve_noalloc(*ctx, n, h, b, x, &ctx->last_error);
 }
 
 int32_t vba_reduced_solve(vba_context* ctx, const vba_reduced_system* system, double* step, uint32_t* iterations) {

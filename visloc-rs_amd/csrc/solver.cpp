@@ -387,8 +387,9 @@ int32_t prepare(Context& ctx, Prepared& pr, std::string* msg) {
         if (!ctx.host_scalars) {
             hip_check(hipHostMalloc((void**)&ctx.host_scalars, sizeof(StepScalars), hipHostMallocDefault), "hipHostMalloc");
             hip_check(hipHostMalloc((void**)&ctx.host_pcg, sizeof(PcgState), hipHostMallocDefault), "hipHostMalloc");
-            hip_check(hipHostMalloc((void**)&ctx.host_status, sizeof(int32_t), hipHostMallocDefault), "hipHostMalloc");
         }
+        if (!ctx.host_status)
+            hip_check(hipHostMalloc((void**)&ctx.host_status, sizeof(int32_t), hipHostMallocDefault), "hipHostMalloc");
         if (!ctx.host_pcg_ring) {
             hip_check(hipHostMalloc((void**)&ctx.host_pcg_ring, Context::kPcgRing * sizeof(PcgState),
                                     hipHostMallocDefault), "hipHostMalloc");
@@ -1306,8 +1307,9 @@ int32_t run_batch(Context& ctx, PreparedBatch& pb, const vba_options& o, vba_ite
         if (!ctx.host_scalars) {
             hip_check(hipHostMalloc((void**)&ctx.host_scalars, sizeof(StepScalars), hipHostMallocDefault), "hipHostMalloc");
             hip_check(hipHostMalloc((void**)&ctx.host_pcg, sizeof(PcgState), hipHostMallocDefault), "hipHostMalloc");
-            hip_check(hipHostMalloc((void**)&ctx.host_status, sizeof(int32_t), hipHostMallocDefault), "hipHostMalloc");
         }
+        if (!ctx.host_status)
+            hip_check(hipHostMalloc((void**)&ctx.host_status, sizeof(int32_t), hipHostMallocDefault), "hipHostMalloc");
         DevProblem P = bind(ctx, pb, o, false, false, s);
         P.any_rig = 1;
         P.obs_sensor = buf<uint16_t>(pb, B_OBS_SENSOR);

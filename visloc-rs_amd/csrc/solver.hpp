@@ -109,6 +109,13 @@ int32_t run(Context& ctx, Prepared& prep, const vba_options& o, vba_iteration_st
 // VBA_ERR_SINGULAR_SYSTEM when both fail; x is then all zero.
 int32_t dense_solve(Context& ctx, uint32_t n, const double* h, const double* b, double* x, std::string* message);
 
+// The same solve with every device buffer in the library's fixed device
+// array (scratch.hip): no device allocation. n <= VBA_DENSE_NOALLOC_MAX_N.
+int32_t dense_solve_noalloc(Context& ctx, uint32_t n, const double* h, const double* b, double* x,
+                            std::string* message);
+// Looks up the device address of that array on `device` (current) once.
+hipError_t scratch_setup(int device);
+
 // Final (or, before any solve, initial) poses / landmarks in input order.
 int32_t download_state(Context& ctx, const Prepared& prep, double* poses_out, double* landmarks_out);
 

@@ -330,6 +330,7 @@ SIGNATURES = {
     "vba_server_request_bytes": (C.c_uint64, [C.c_void_p, C.POINTER(Problem), C.POINTER(Options)]),
     "vba_server_destroy": (None, [C.c_void_p]),
     "vba_dense_solve": (C.c_int32, [C.c_void_p, C.c_uint32, P64, P64, P64]),
+    "vba_dense_solve_noalloc": (C.c_int32, [C.c_void_p, C.c_uint32, P64, P64, P64]),
     "vba_reduced_solve": (C.c_int32, [C.c_void_p, C.POINTER(ReducedSystem), P64, C.POINTER(C.c_uint32)]),
     "vba_optimize_weighted": (
         C.c_int32,

@@ -411,16 +411,18 @@ class Device:
                                        w.ctypes.data_as(abi.P64), C.byref(res))
         return st, res, poses, lms, w
 
-    def dense_solve(self, h, b):
-        """solve_normal_equations on the device (vba_dense_solve): returns
-        (status, x); x is None unless the status is OK."""
+    def dense_solve(self, h, b, noalloc: bool = False):
+        """solve_normal_equations on the device (vba_dense_solve, or
+        vba_dense_solve_noalloc with `noalloc`): returns (status, x); x is
+        None unless the status is OK."""
         h = np.asfortranarray(h, np.float64)
         b = np.ascontiguousarray(b, np.float64)
         n = len(b)
         assert h.shape == (n, n)
         x = np.zeros(n)
-        st = self.lib.vba_dense_solve(self.ctx, n, h.ravel(order="F").ctypes.data_as(abi.P64),
-                                      b.ctypes.data_as(abi.P64), x.ctypes.data_as(abi.P64))
+        fn = self.lib.vba_dense_solve_noalloc if noalloc else self.lib.vba_dense_solve
+        st = fn(self.ctx, n, h.ravel(order="F").ctypes.data_as(abi.P64), b.ctypes.data_as(abi.P64),
+                x.ctypes.data_as(abi.P64))
         return st, (x if st == abi.OK else None)
 
     def reduced_solve(self, diag, edges, offdiag, rhs, tol: float, max_iterations: int = 500, clusters=None,
