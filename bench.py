@@ -28,7 +28,8 @@ sys.path.insert(0, os.path.join(ROOT, "visloc-rs_amd"))
 import numpy as np  # noqa: E402
 
 import vba  # noqa: E402
-from vba import abi  # noqa: E402
+from vba import abi, workloads  # noqa: E402
+from vba.workloads import loop_options, window_options, window_problems  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "BA obs/s + LM iters/s at ≤1e-6 rel cost vs FP64; served p50/p99 solve latency"
@@ -36,15 +37,6 @@ METRIC = "BA obs/s + LM iters/s at ≤1e-6 rel cost vs FP64; served p50/p99 solv
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
-
-
-def loop_options(precision, policy=abi.PCG_MAXIT_ACCEPT_ITERATE, solver=abi.SOLVER_MATRIX_FREE):
-    """PCG rtol 1e-4 / max 100 are the reference GPU backend's GpuBaSettings
-    defaults (crates/ba-gpu/src/solver.rs:36-48); a solve that reaches the cap
-    uses its last iterate, as that backend does (solver.rs:497-508)."""
-    return vba.default_options(linear_solver=solver, max_iterations=30, max_pcg_iterations=100,
-                               pcg_relative_tolerance=1e-4, robust_kernel=abi.KERNEL_HUBER, robust_scale=1.0,
-                               precision=precision, pcg_maxit_policy=policy)
 
 
 def cpu_model():
@@ -58,50 +50,70 @@ def cpu_model():
     return platform.processor()
 
 
-def cpu_baseline(problem, options):
-    """The CPU FP64 oracle (a restatement of the reference's Rust solver) on a
-    bounded sample of the same workload: its first LM iteration (assembly,
-    Schur operator, PCG up to 100 iterations, back-substitution, trial cost)
-    on one core."""
+def cpu_threads():
+    """Host threads for the threaded CPU baseline: the cores this process may
+    use, at most 16 (the host share of one GPU)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def as_fp64(options):
+    o = vba.default_options(**{f: getattr(options, f) for f, _ in abi.Options._fields_})
+    o.precision = abi.PRECISION_FP64
+    return o
+
+
+def cpu_sample(problems, options, label):
+    """The CPU FP64 oracle (a restatement of the reference's Rust solver,
+    bitwise equal serial and threaded) on a bounded sample: `problems` with
+    `options`, timed serially (the reference default, parallel = false) and
+    on cpu_threads() threads (parallel = true). Returns the timing record and
+    the serial outputs."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
-    o = vba.default_options(**{f: getattr(options, f) for f, _ in abi.Options._fields_})
-    o.max_iterations = 1
-    o.precision = abi.PRECISION_FP64
+    o = as_fp64(options)
+    nthr = cpu_threads()
     t0 = time.perf_counter()
-    out = oracle.optimize(problem, o, threads=1)
-    dt = time.perf_counter() - t0
-    iters = max(1, len(out.iterations))
-    obs = problem.num_observations * iters
+    outs = [oracle.optimize(p, o, threads=1) for p in problems]
+    serial = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    outs_t = [oracle.optimize(p, o, threads=nthr) for p in problems]
+    threaded = time.perf_counter() - t0
+    same = all(a.result.final_cost == b.result.final_cost and a.iterations == b.iterations
+               for a, b in zip(outs, outs_t))
+    obs = sum(p.num_observations * len(r.iterations) for p, r in zip(problems, outs))
+    lm = sum(len(r.iterations) for r in outs)
+    pcg = sum(max(0, it["pcg_iterations"]) for r in outs for it in r.iterations)
     return {
-        "value": obs / dt,
+        "value": obs / serial,
         "unit": "obs/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"1 LM iteration ({out.iterations[0]['pcg_iterations'] if out.iterations else 0} PCG iterations) "
-                  f"of the full {problem.num_observations}-observation loop workload, FP64, serial; "
-                  f"{dt:.1f} s on {cpu_model()} ({os.cpu_count()} host threads visible)",
-        "seconds": dt,
-    }
+        "sample": f"{label}: {lm} LM iterations ({pcg} PCG iterations), FP64, "
+                  f"{serial:.1f} s serial, {threaded:.1f} s on {nthr} threads (bitwise equal: {same}); "
+                  f"{cpu_model()}, nproc {os.cpu_count()}",
+        "seconds": serial,
+        "threaded": {"value": obs / threaded, "cores": nthr, "seconds": threaded, "bitwise_equal_to_serial": same},
+        "lm_iterations": lm,
+        "pcg_iterations": pcg,
+    }, outs
 
 
-def window_problems(count=256, seed=7):
-    """Local-BA windows: keyframes uniform in [5, 40], landmarks uniform in
-    [300, 8000], stereo rig (0.11 m baseline), pinhole f=460 at 752x480,
-    tracks of 2..min(10, K) keyframes, 0.5 px noise, 2% outliers."""
-    rng = np.random.default_rng(seed)
-    out = []
-    for i in range(count):
-        k = int(rng.integers(5, 41))
-        l = int(rng.integers(300, 8001))
-        out.append(vba.synthesize(abi.SYNTH_STEREO_WINDOW, num_cameras=k, num_landmarks=l, seed=1000 + i).problem)
-    return out
-
-
-def window_options():
-    return vba.default_options(linear_solver=abi.SOLVER_DENSE, max_iterations=10, robust_kernel=abi.KERNEL_CAUCHY,
-                               robust_scale=1.0)
+def device_sample(dev, problems, options, reps=3):
+    """The device on the same sample as cpu_sample (same options, so the
+    same LM iterations), for a like-for-like ratio: best of `reps` solves."""
+    best = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        for p in problems:
+            dev.optimize(p, options)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    return best
 
 
 def bench_windows(args):
@@ -528,6 +540,226 @@ def bench_rows(args):
                       "observations": int(prob.num_observations)}), flush=True)
 
 
+TRACES = os.path.join(ROOT, "tests", "golden", "traces")
+FP64_MFMA_PEAK_TFLOPS = 78.6
+
+
+def load_trace(name):
+    path = os.path.join(TRACES, f"{name}.npz")
+    if not os.path.exists(path):
+        return None
+    with np.load(path, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def parity_record(out, tr, extent, poses, idx=None):
+    """Device solve against an oracle trace: same accept vector, relative
+    final cost, largest camera-centre distance over the scene extent."""
+    if idx is None:
+        ref = [bool(a) for a in tr["accepted"]]
+        ref_cost = float(tr["final_cost"])
+        centres = tr["centres"]
+    else:
+        ip, cp = tr["iteration_ptr"], tr["camera_ptr"]
+        ref = [bool(a) for a in tr["accepted"][ip[idx]:ip[idx + 1]]]
+        ref_cost = float(tr["final_cost"][idx])
+        centres = tr["centres"][cp[idx]:cp[idx + 1]]
+    acc = [bool(it["step_accepted"]) for it in out.iterations]
+    return {
+        "same_accept_vector": acc == ref,
+        "rel_final_cost": abs(out.result.final_cost - ref_cost) / ref_cost,
+        "centre_err_over_extent": float(np.linalg.norm(vba.camera_centres(poses) - centres, axis=1).max()) / extent,
+    }
+
+
+def trace_parity(name, problem, out, extent, poses):
+    tr = load_trace(name)
+    if tr is None:
+        return None
+    if workloads.problem_digest(problem) != str(tr["digest"]):
+        return {"error": "the problem's bytes differ from the trace's"}
+    r = parity_record(out, tr, extent, poses)
+    r["trace"] = name
+    return r
+
+
+def kernel_table(dev, names):
+    out = {}
+    for name in names:
+        q = dev.profile_query(name)
+        if q["launches"]:
+            q["avg_us"] = 1e6 * q["seconds"] / q["launches"]
+            out[name] = q
+    return out
+
+
+def dense_records(kern, assemble, factor):
+    """Roofline of the dense assembly (HBM, algorithmic bytes) and FP64-MFMA
+    utilisation of the dense factor (n^3 / 3 FLOPs per system)."""
+    rec = {}
+    if assemble in kern and kern[assemble]["seconds"] > 0:
+        a = kern[assemble]
+        gbs = a["bytes"] / a["seconds"] / 1e9
+        rec["roofline"] = {"kernel": assemble, "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                           "algorithmic_bytes_per_launch": a["bytes"] / a["launches"],
+                           "avg_launch_us": round(a["avg_us"], 2)}
+    if factor in kern and kern[factor]["seconds"] > 0:
+        f = kern[factor]
+        tf = f["bytes"] / f["seconds"] / 1e12
+        rec["dense_factor_mfma"] = {"kernel": factor, "achieved_tflops": round(tf, 4), "peak": FP64_MFMA_PEAK_TFLOPS,
+                                    "unit": "TFLOP/s (FP64)", "frac": round(tf / FP64_MFMA_PEAK_TFLOPS, 5),
+                                    "flops_per_launch": f["bytes"] / f["launches"],
+                                    "avg_launch_us": round(f["avg_us"], 2)}
+    return rec
+
+
+def extra_circle(dev, args, steps=5):
+    """50 cameras on a circle, 5,000 landmarks, SIMPLE_RADIAL, every in-frame
+    observation (250k), squared loss, dense Schur + Cholesky, 20 LM
+    iterations (the solve converges earlier; iterations executed count)."""
+    s = workloads.circle_problem()
+    p = s.problem
+    N = p.num_observations
+    o = workloads.circle_options()
+    prep = dev.prepare(p, o)
+    prep.solve(o)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = prep.solve(o)
+    wall = time.perf_counter() - t0
+    poses = prep.state()[0]
+    dev.profile(True, stride=1)
+    prep.solve(o)
+    dev.profile(False, reset=False)
+    kern = kernel_table(dev, ["linearize", "cost", "dense_assemble", "dense_solve"])
+    prep.free()
+    lm = out.result.num_iterations * steps
+    rec = {
+        "workload": f"circle: 50 cameras, 5,000 landmarks, SIMPLE_RADIAL, {N} observations, squared loss, dense, "
+                    "20 LM iterations max",
+        "obs_per_s": N * lm / wall, "lm_iters_per_s": lm / wall, "ms_per_solve": 1e3 * wall / steps,
+        "lm_iterations": out.result.num_iterations, "dtype": "f64",
+        "vs_oracle_trace": trace_parity("circle", p, out, s.scene_extent, poses),
+    }
+    rec.update(dense_records(kern, "dense_assemble", "dense_solve"))
+    if not args.no_cpu_baseline:
+        cpu, _ = cpu_sample([p], o, "the full circle workload")
+        cpu["device_seconds_same_sample"] = wall / steps
+        rec["cpu_baseline"] = cpu
+    return rec
+
+
+def extra_windows(dev, args, steps=3):
+    """The 256 local stereo windows (K in [5, 40], L in [300, 8000], Cauchy
+    1 px, dense, 10 LM iterations) as one prepared batch replayed as a graph."""
+    specs = workloads.window_specs()
+    synths = [workloads.window_problem(sp) for sp in specs]
+    probs = [x.problem for x in synths]
+    n_obs = [x.num_observations for x in probs]
+    o = workloads.window_options()
+    pb = dev.prepare_batch(probs, o)
+    pb.solve(o, cap=16)
+    t0 = time.perf_counter()
+    obs_iters = lm = 0
+    for _ in range(steps):
+        out = pb.solve(o, cap=16)
+        for n, w in zip(n_obs, out.windows):
+            obs_iters += n * w.result.num_iterations
+            lm += w.result.num_iterations
+    wall = time.perf_counter() - t0
+    tr = load_trace("windows")
+    parity = None
+    if tr is not None:
+        if [tuple(int(v) for v in r) for r in tr["specs"]] == specs and all(
+                workloads.problem_digest(p) == str(d) for p, d in zip(probs, tr["digest"])):
+            recs = [parity_record(w, tr, tr["extent"][i], pb.state(i)[0], idx=i) for i, w in enumerate(out.windows)]
+            parity = {"windows": len(recs), "same_accept_vectors": sum(r["same_accept_vector"] for r in recs),
+                      "max_rel_final_cost": max(r["rel_final_cost"] for r in recs),
+                      "max_centre_err_over_extent": max(r["centre_err_over_extent"] for r in recs),
+                      "trace": "windows"}
+        else:
+            parity = {"error": "the problems' bytes differ from the trace's"}
+    # kernel times from one solve launched kernel by kernel (no graph)
+    eager = vba.default_options(**{f: getattr(o, f) for f, _ in abi.Options._fields_})
+    eager.flags = abi.FLAG_NO_GRAPH
+    dev.profile(True, stride=1)
+    pb.solve(eager, cap=16)
+    dev.profile(False, reset=False)
+    kern = kernel_table(dev, ["batch_dense_assemble", "batch_dense_solve"])
+    pb.free()
+    rec = {
+        "workload": f"{len(probs)} local stereo windows, K in [5,40], L in [300,8000], {sum(n_obs)} observations, "
+                    "Cauchy 1 px, dense, 10 LM iterations, one batch (HIP graph)",
+        "obs_per_s": obs_iters / wall, "lm_iters_per_s": lm / wall, "ms_per_step": 1e3 * wall / steps,
+        "dtype": "f64", "vs_oracle_trace": parity,
+    }
+    rec.update(dense_records(kern, "batch_dense_assemble", "batch_dense_solve"))
+    if not args.no_cpu_baseline:
+        sample = probs[:3]
+        cpu, _ = cpu_sample(sample, o, f"the first {len(sample)} windows, 10 LM iterations each")
+        rec["cpu_baseline"] = cpu
+    return rec
+
+
+def extra_city(dev, args, steps=2):
+    """The city scene at full size: 5,500 cameras with per-camera OPENCV
+    intrinsics, 2.5M landmarks (~20M observations), 10 % outliers, Huber
+    2 px, matrix-free PCG (rtol 1e-4, max 100), 25 LM iterations, mixed
+    precision, checked against the FP64 device solve."""
+    t0 = time.perf_counter()
+    s = workloads.city_problem()
+    p = s.problem
+    N = p.num_observations
+    gen = time.perf_counter() - t0
+    o = workloads.city_options(abi.PRECISION_MIXED)
+    t0 = time.perf_counter()
+    prep = dev.prepare(p, o)
+    prep_s = time.perf_counter() - t0
+    prep.solve(o)
+    dev.profile(True, stride=args.profile_stride)
+    t0 = time.perf_counter()
+    lm = pcg = 0
+    for _ in range(steps):
+        out = prep.solve(o)
+        lm += out.result.num_iterations
+        pcg += out.result.total_pcg_iterations
+    wall = time.perf_counter() - t0
+    dev.profile(False, reset=False)
+    kern = kernel_table(dev, ["linearize", "schur_apply", "pcg_pose", "pcg_step", "cost"])
+    t1 = time.perf_counter()
+    ref = prep.solve(workloads.city_options(abi.PRECISION_FP64))
+    t_ref = time.perf_counter() - t1
+    prep.free()
+    roof = None
+    if "schur_apply" in kern and kern["schur_apply"]["seconds"] > 0:
+        k = kern["schur_apply"]
+        gbs = k["bytes"] / k["seconds"] / 1e9
+        roof = {"kernel": "schur_apply", "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                "algorithmic_bytes_per_launch": k["bytes"] / k["launches"], "avg_launch_us": round(k["avg_us"], 2)}
+    rec = {
+        "workload": f"city: 5,500 cameras, per-camera OPENCV, 2.5M landmarks, {N} observations, 10 % outliers, "
+                    "Huber 2 px, matrix-free PCG rtol 1e-4 max 100, 25 LM iterations",
+        "obs_per_s": N * lm / wall, "lm_iters_per_s": lm / wall, "ms_per_solve": 1e3 * wall / steps,
+        "pcg_iters_per_solve": pcg / steps, "dtype": "f32+f64",
+        "rel_cost_vs_fp64": abs(out.result.final_cost - ref.result.final_cost) / ref.result.final_cost,
+        "same_accept_vector_as_fp64": out.accepted == ref.accepted,
+        "fp64_obs_per_s": N * ref.result.num_iterations / t_ref,
+        "generate_s": round(gen, 2), "prepare_s": round(prep_s, 2),
+        "roofline": roof,
+        "kernels": {k: {kk: (round(vv, 6) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                    for k, v in kern.items()},
+    }
+    if not args.no_cpu_baseline:
+        sub = workloads.city_problem(500).problem
+        one = workloads.city_options(abi.PRECISION_FP64, max_iterations=1)
+        cpu, _ = cpu_sample([sub], one, "the first LM iteration of the 500-camera city sub-problem "
+                                        f"({sub.num_observations} observations)")
+        rec["cpu_baseline"] = cpu
+    return rec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -535,11 +767,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="leave out the circle, windows and city results of the default line")
     ap.add_argument("--landmarks", type=int, default=400_000, help="landmark count (default: full size)")
     ap.add_argument("--pcg-policy", choices=["accept", "reject"], default="accept",
                     help="at the PCG iteration cap: use the iterate (GPU backend) or reject the step (CPU "
                          "matrix-free entry)")
-    ap.add_argument("--workload", choices=["loop", "windows", "serve", "reduced", "rows"], default="loop",
+    ap.add_argument("--workload", choices=["loop", "circle", "windows", "city", "serve", "reduced", "rows"],
+                    default="loop",
                     help="loop: the 3.5M-observation single problem (default); windows: a batch of local windows; "
                          "serve: the request loop under a Poisson stream")
     ap.add_argument("--rate", type=float, default=2000.0, help="requests per second (--workload serve)")
@@ -558,6 +793,12 @@ def main():
         sys.exit(2)
     if args.workload == "windows":
         bench_windows(args)
+        return
+    if args.workload in ("circle", "city"):
+        dev = vba.Device(0)
+        r = (extra_circle if args.workload == "circle" else extra_city)(dev, args)
+        dev.close()
+        print(json.dumps(r), flush=True)
         return
     if args.workload == "serve":
         bench_serve(args)
@@ -618,6 +859,7 @@ def main():
         final_cost = out.result.final_cost
     wall = time.perf_counter() - t0
     dev.profile(False, reset=False)
+    timed_poses = prep.state()[0]
     accepted = sum(it["step_accepted"] for it in out.iterations)
     log(f"bench: {args.steps} steps in {wall:.3f} s; last solve {out.result.num_iterations} LM iterations "
         f"({accepted} accepted), {out.result.total_pcg_iterations} PCG iterations")
@@ -656,6 +898,7 @@ def main():
     # Parity of the measured solve against the plain FP64 solve (untimed).
     ref_rel = None
     fp64_obs_per_s = None
+    ref = None
     if precision == abi.PRECISION_MIXED:
         f64 = loop_options(abi.PRECISION_FP64, policy, solver)
         t1 = time.perf_counter()
@@ -700,13 +943,38 @@ def main():
                     for k, v in kernels.items()},
         "pcg_schedule": tuning,
     }
-    prep.free()
-    dev.close()
+    # Parity of the measured solve against the oracle's stored trace of the
+    # same problem (tests/golden/traces, FP64, same at-cap rule), when present.
+    if args.landmarks == 400_000:
+        result["vs_oracle_trace"] = trace_parity(f"loop_{args.pcg_policy}", problem, out, s.scene_extent,
+                                                 timed_poses)
+        if ref is not None and ref.status == abi.OK:
+            result["fp64_vs_oracle_trace"] = trace_parity(f"loop_{args.pcg_policy}", problem, ref, s.scene_extent,
+                                                          prep.state()[0])
+
     if not args.no_cpu_baseline:
-        log("bench: CPU baseline (oracle, 1 LM iteration) ...")
-        result["cpu_baseline"] = cpu_baseline(problem, opts)
+        log("bench: CPU baseline (oracle, first LM iteration, serial and threaded) ...")
+        one = loop_options(precision, policy, solver, max_iterations=1)
+        cpu, _ = cpu_sample([problem], one, "the first LM iteration of the full loop workload")
+        t1 = time.perf_counter()
+        prep.solve(one)
+        gpu_s = time.perf_counter() - t1
+        cpu["device_seconds_same_sample"] = gpu_s
+        cpu["device_speedup_same_sample"] = cpu["seconds"] / gpu_s
+        result["cpu_baseline"] = cpu
     else:
         result["cpu_baseline"] = None
+    prep.free()
+    if not args.no_extra:
+        result["extra"] = {}
+        for name, fn in (("circle", extra_circle), ("windows", extra_windows), ("city", extra_city)):
+            t1 = time.perf_counter()
+            try:
+                result["extra"][name] = fn(dev, args)
+            except Exception as e:  # a failure is recorded, the headline line still prints
+                result["extra"][name] = {"error": f"{type(e).__name__}: {e}"}
+            log(f"bench: extra {name} in {time.perf_counter() - t1:.1f} s: {result['extra'][name]}")
+    dev.close()
     print(json.dumps(result), flush=True)
 
 

@@ -65,6 +65,27 @@ using M43 = Mat<4, 3>;
 using M63 = Mat<6, 3>;
 using M66 = Mat<6, 6>;
 
+// Runs fn(begin, end) over [0, n) in `threads` contiguous ranges, one thread
+// each. Every index is handled by exactly one thread with the serial code's
+// operations in the serial order, so a result whose entries do not depend
+// on one another has the serial bits (the parallel == serial property of
+// parallel_ba_tests.rs:113-160).
+template <class F>
+void parallel_ranges(int threads, size_t n, F&& fn) {
+    if (threads <= 1 || n < 2) {
+        fn((size_t)0, n);
+        return;
+    }
+    const size_t t = std::min<size_t>((size_t)threads, n);
+    std::vector<std::thread> pool;
+    pool.reserve(t);
+    for (size_t k = 0; k < t; ++k) {
+        const size_t b = n * k / t, e = n * (k + 1) / t;
+        pool.emplace_back([&fn, b, e]() { fn(b, e); });
+    }
+    for (auto& th : pool) th.join();
+}
+
 template <int R, int K, int C>
 Mat<R, C> mul(const Mat<R, K>& A, const Mat<K, C>& B) {
     Mat<R, C> out;
@@ -949,18 +970,41 @@ bool squared_residual(const Problem& P, const Intr& in, const Obs& o, double* s)
 
 // robust_cost_weighted (problem.rs:497-583) and
 // nonprojectable_observation_count (problem.rs:252-303).
-void cost_and_nonprojectable(const Problem& P, const Kernel& K, double* cost, uint64_t* nonproj) {
+// threads > 1 evaluates the per-observation terms in parallel and sums them
+// in observation order, as the serial loop does.
+void cost_and_nonprojectable(const Problem& P, const Kernel& K, double* cost, uint64_t* nonproj,
+                             int threads = 1) {
     double total = 0.0;
     uint64_t bad = 0;
     Intr in;
     intrinsics(P.camera, &in);
-    for (const Obs& o : P.obs) {
-        double s;
-        if (!squared_residual(P, in, o, &s)) {
-            ++bad;
-            continue;
+    const size_t n = P.obs.size();
+    if (threads <= 1 || n < 65536) {
+        for (const Obs& o : P.obs) {
+            double s;
+            if (!squared_residual(P, in, o, &s)) {
+                ++bad;
+                continue;
+            }
+            total += o.weight * K.cost(s);
         }
-        total += o.weight * K.cost(s);
+    } else {
+        std::vector<double> term(n);
+        std::vector<uint8_t> ok(n);
+        parallel_ranges(threads, n, [&](size_t b, size_t e) {
+            for (size_t i = b; i < e; ++i) {
+                double s;
+                ok[i] = squared_residual(P, in, P.obs[i], &s);
+                if (ok[i]) term[i] = P.obs[i].weight * K.cost(s);
+            }
+        });
+        for (size_t i = 0; i < n; ++i) {
+            if (!ok[i]) {
+                ++bad;
+                continue;
+            }
+            total += term[i];
+        }
     }
     *cost = total;
     *nonproj = bad;
@@ -1255,8 +1299,12 @@ bool cholesky_inverse6(const M66& block, M66* inv) {
 // ---------------------------------------------------------------------------
 // Dense Schur step: solve_step_with_debug (normal_equations.rs:1314-1592).
 // ---------------------------------------------------------------------------
+// threads > 1: the landmark inverses and the back-substitution run per
+// landmark in parallel; the Schur reduction runs per range of pose rows, each
+// range visiting the landmarks in order, so every entry of S and b_S takes
+// the serial sequence of updates.
 int32_t solve_step_dense(Normal& ne, uint32_t L, double lambda, std::vector<double>* dp,
-                         std::vector<double>* dl) {
+                         std::vector<double>* dl, int threads = 1) {
     const uint32_t P = ne.P;
     if (P == 0) {  // landmark-only branch (:1332-1371)
         dl->assign((size_t)3 * L, 0.0);
@@ -1284,40 +1332,47 @@ int32_t solve_step_dense(Normal& ne, uint32_t L, double lambda, std::vector<doub
     for (size_t k = 0; k < dim; ++k) b[k] = -ne.b_p[k];
     std::vector<M3> inv(L);
     std::vector<uint8_t> has_inv(L, 0);
-    for (uint32_t l = 0; l < L; ++l) {
-        const LandmarkBlock& lb = ne.lm[l];
-        has_inv[l] = try_inverse3(damped3(lb.h_ll, lambda), &inv[l]);
-        if (!has_inv[l]) continue;
-        for (const CrossEntry& a : lb.cross) {
-            M63 a_h = mul(a.w, inv[l]);
-            for (const CrossEntry& bq : lb.cross) {
-                M66 blk = mul(a_h, tr(bq.w));
-                for (int r = 0; r < 6; ++r)
-                    for (int c = 0; c < 6; ++c) {
-                        double& dst = S[(a.pose * 6 + r) * dim + bq.pose * 6 + c];
-                        dst -= blk(r, c);
-                    }
+    parallel_ranges(threads, L, [&](size_t l0, size_t l1) {
+        for (size_t l = l0; l < l1; ++l) has_inv[l] = try_inverse3(damped3(ne.lm[l].h_ll, lambda), &inv[l]);
+    });
+    parallel_ranges(threads, P, [&](size_t p0, size_t p1) {
+        for (uint32_t l = 0; l < L; ++l) {
+            if (!has_inv[l]) continue;
+            const LandmarkBlock& lb = ne.lm[l];
+            for (const CrossEntry& a : lb.cross) {
+                if (a.pose < p0 || a.pose >= p1) continue;
+                M63 a_h = mul(a.w, inv[l]);
+                for (const CrossEntry& bq : lb.cross) {
+                    M66 blk = mul(a_h, tr(bq.w));
+                    for (int r = 0; r < 6; ++r)
+                        for (int c = 0; c < 6; ++c) {
+                            double& dst = S[(a.pose * 6 + r) * dim + bq.pose * 6 + c];
+                            dst -= blk(r, c);
+                        }
+                }
+                double upd[6];
+                mulv(a_h, lb.b_l, upd);
+                for (int k = 0; k < 6; ++k) b[a.pose * 6 + k] += upd[k];
             }
-            double upd[6];
-            mulv(a_h, lb.b_l, upd);
-            for (int k = 0; k < 6; ++k) b[a.pose * 6 + k] += upd[k];
         }
-    }
+    });
     if (!solve_normal_equations(S, dim, b, dp)) return VBA_ERR_SINGULAR_SYSTEM;
     dl->assign((size_t)3 * L, 0.0);
-    for (uint32_t l = 0; l < L; ++l) {
-        if (!has_inv[l]) continue;
-        const LandmarkBlock& lb = ne.lm[l];
-        double acc[3] = {-lb.b_l[0], -lb.b_l[1], -lb.b_l[2]};
-        for (const CrossEntry& a : lb.cross) {
-            double sub[3];
-            mulv(tr(a.w), dp->data() + a.pose * 6, sub);
-            for (int k = 0; k < 3; ++k) acc[k] -= sub[k];
+    parallel_ranges(threads, L, [&](size_t l0, size_t l1) {
+        for (size_t l = l0; l < l1; ++l) {
+            if (!has_inv[l]) continue;
+            const LandmarkBlock& lb = ne.lm[l];
+            double acc[3] = {-lb.b_l[0], -lb.b_l[1], -lb.b_l[2]};
+            for (const CrossEntry& a : lb.cross) {
+                double sub[3];
+                mulv(tr(a.w), dp->data() + a.pose * 6, sub);
+                for (int k = 0; k < 3; ++k) acc[k] -= sub[k];
+            }
+            double d[3];
+            mulv(inv[l], acc, d);
+            for (int k = 0; k < 3; ++k) (*dl)[l * 3 + k] = d[k];
         }
-        double d[3];
-        mulv(inv[l], acc, d);
-        for (int k = 0; k < 3; ++k) (*dl)[l * 3 + k] = d[k];
-    }
+    });
     return VBA_OK;
 }
 
@@ -1342,10 +1397,15 @@ struct Implicit {
     std::vector<uint8_t> has_inv;
     std::vector<M66> precond_inv;
     std::vector<double> rhs;
+    // threads > 1: each pose's cross entries in (landmark, entry) order, the
+    // order in which the serial operator subtracts them
+    int threads = 1;
+    std::vector<size_t> pose_ptr;
+    std::vector<std::pair<uint32_t, const CrossEntry*>> pose_entries;
     size_t dim() const { return diagonal.size() * 6; }
 };
 // ImplicitSchurOperator::new (implicit_schur.rs:133-297).
-int build_implicit(const Normal& ne, double lambda, Implicit* op) {
+int build_implicit_serial(const Normal& ne, double lambda, Implicit* op) {
     if (!std::isfinite(lambda) || lambda < 0.0) return IE_INVALID_LAMBDA;
     if (ne.diag.empty()) return IE_EMPTY;
     op->ne = &ne;
@@ -1417,10 +1477,150 @@ int build_implicit(const Normal& ne, double lambda, Implicit* op) {
     }
     return IE_OK;
 }
-// ImplicitSchurOperator::apply (implicit_schur.rs:390-431).
+// The same operator with the landmark inverses per landmark in parallel and
+// the right-hand side and preconditioner sums per range of poses, each range
+// visiting the landmarks in order (so every sum has the serial order).
+int build_implicit_threaded(const Normal& ne, double lambda, int threads, Implicit* op) {
+    if (!std::isfinite(lambda) || lambda < 0.0) return IE_INVALID_LAMBDA;
+    if (ne.diag.empty()) return IE_EMPTY;
+    op->ne = &ne;
+    op->threads = threads;
+    const uint32_t P = (uint32_t)ne.diag.size();
+    if (!all_finite(ne.b_p)) return IE_NONFINITE;
+    for (const M66& b : ne.diag)
+        if (!all_finite(b)) return IE_NONFINITE;
+    op->diagonal = ne.diag;
+    if (lambda > 0.0)
+        for (M66& b : op->diagonal)
+            for (int k = 0; k < 6; ++k) b(k, k) += lambda;
+    for (const M66& b : op->diagonal)
+        if (!all_finite(b)) return IE_NONFINITE;
+    const size_t L = ne.lm.size();
+    op->inv.assign(L, M3::zeros());
+    op->has_inv.assign(L, 0);
+    std::vector<uint8_t> bad(L, 0);
+    parallel_ranges(threads, L, [&](size_t l0, size_t l1) {
+        for (size_t l = l0; l < l1; ++l) {
+            const LandmarkBlock& lb = ne.lm[l];
+            if (!all_finite(lb.h_ll) || !std::isfinite(lb.b_l[0]) || !std::isfinite(lb.b_l[1]) ||
+                !std::isfinite(lb.b_l[2])) {
+                bad[l] = 1;
+                continue;
+            }
+            for (const CrossEntry& c : lb.cross)
+                if (!all_finite(c.w)) bad[l] = 1;
+            if (bad[l]) continue;
+            M3 h = lb.h_ll;
+            if (lambda > 0.0)
+                for (int k = 0; k < 3; ++k) h(k, k) += lambda;
+            if (!all_finite(h)) {
+                bad[l] = 1;
+                continue;
+            }
+            M3 inv;
+            bool ok = try_inverse3(h, &inv);
+            if (ok && !all_finite(inv)) {
+                bad[l] = 1;
+                continue;
+            }
+            op->has_inv[l] = ok;
+            if (ok) op->inv[l] = inv;
+        }
+    });
+    for (size_t l = 0; l < L; ++l)
+        if (bad[l]) return IE_NONFINITE;
+    // per-pose entry lists in (landmark, entry) order
+    op->pose_ptr.assign((size_t)P + 1, 0);
+    for (size_t l = 0; l < L; ++l)
+        for (const CrossEntry& c : ne.lm[l].cross) ++op->pose_ptr[c.pose + 1];
+    for (uint32_t p = 0; p < P; ++p) op->pose_ptr[p + 1] += op->pose_ptr[p];
+    op->pose_entries.resize(op->pose_ptr[P]);
+    {
+        std::vector<size_t> fill(op->pose_ptr.begin(), op->pose_ptr.end() - 1);
+        for (size_t l = 0; l < L; ++l)
+            for (const CrossEntry& c : ne.lm[l].cross) op->pose_entries[fill[c.pose]++] = {(uint32_t)l, &c};
+    }
+    op->rhs.resize((size_t)6 * P);
+    for (size_t k = 0; k < op->rhs.size(); ++k) op->rhs[k] = -ne.b_p[k];
+    std::vector<M66> pre = op->diagonal;
+    parallel_ranges(threads, P, [&](size_t p0, size_t p1) {
+        std::vector<std::pair<uint32_t, M63>> grouped;
+        for (size_t l = 0; l < L; ++l) {
+            if (!op->has_inv[l]) continue;
+            const LandmarkBlock& lb = ne.lm[l];
+            grouped.clear();
+            for (const CrossEntry& c : lb.cross) {
+                if (c.pose < p0 || c.pose >= p1) continue;
+                M63 ci = mul(c.w, op->inv[l]);
+                double upd[6];
+                mulv(ci, lb.b_l, upd);
+                for (int k = 0; k < 6; ++k) op->rhs[c.pose * 6 + k] += upd[k];
+                auto it = std::find_if(grouped.begin(), grouped.end(),
+                                       [&](const std::pair<uint32_t, M63>& g) { return g.first == c.pose; });
+                if (it == grouped.end())
+                    grouped.push_back({c.pose, add(M63::zeros(), c.w)});
+                else
+                    add_assign(it->second, c.w);
+            }
+            for (auto& g : grouped) sub_assign(pre[g.first], mul(mul(g.second, op->inv[l]), tr(g.second)));
+        }
+    });
+    if (!all_finite(op->rhs)) return IE_NONFINITE;
+    for (const M66& b : pre)
+        if (!all_finite(b)) return IE_NONFINITE;
+    op->precond_inv.resize(P);
+    std::vector<int> code(P, IE_OK);
+    parallel_ranges(threads, P, [&](size_t p0, size_t p1) {
+        for (size_t p = p0; p < p1; ++p) {
+            if (!cholesky_inverse6(pre[p], &op->precond_inv[p])) code[p] = IE_NON_SPD_PRECONDITIONER;
+            else if (!all_finite(op->precond_inv[p])) code[p] = IE_NONFINITE;
+        }
+    });
+    for (uint32_t p = 0; p < P; ++p)
+        if (code[p] != IE_OK) return code[p];
+    return IE_OK;
+}
+int build_implicit(const Normal& ne, double lambda, Implicit* op, int threads = 1) {
+    if (threads > 1) return build_implicit_threaded(ne, lambda, threads, op);
+    return build_implicit_serial(ne, lambda, op);
+}
+// ImplicitSchurOperator::apply (implicit_schur.rs:390-431). With threads > 1
+// the landmark reductions run per landmark and the pose sums per pose, each
+// pose subtracting its entries in the serial (landmark, entry) order.
 int implicit_apply(const Implicit& op, const std::vector<double>& x, std::vector<double>* out) {
     if (!all_finite(x)) return IE_NONFINITE;
     out->assign(op.dim(), 0.0);
+    if (op.threads > 1) {
+        const Normal& ne = *op.ne;
+        const size_t L = ne.lm.size();
+        std::vector<double> reduced(3 * L);
+        parallel_ranges(op.threads, L, [&](size_t l0, size_t l1) {
+            for (size_t l = l0; l < l1; ++l) {
+                if (!op.has_inv[l]) continue;
+                double projected[3] = {0.0, 0.0, 0.0};
+                for (const CrossEntry& c : ne.lm[l].cross) {
+                    double v[3];
+                    mulv(tr(c.w), x.data() + c.pose * 6, v);
+                    for (int k = 0; k < 3; ++k) projected[k] += v[k];
+                }
+                mulv(op.inv[l], projected, reduced.data() + 3 * l);
+            }
+        });
+        parallel_ranges(op.threads, op.diagonal.size(), [&](size_t p0, size_t p1) {
+            for (size_t p = p0; p < p1; ++p) {
+                mulv(op.diagonal[p], x.data() + p * 6, out->data() + p * 6);
+                for (size_t e = op.pose_ptr[p]; e < op.pose_ptr[p + 1]; ++e) {
+                    const uint32_t l = op.pose_entries[e].first;
+                    if (!op.has_inv[l]) continue;
+                    double v[6];
+                    mulv(op.pose_entries[e].second->w, reduced.data() + 3 * l, v);
+                    for (int k = 0; k < 6; ++k) (*out)[p * 6 + k] -= v[k];
+                }
+            }
+        });
+        if (!all_finite(*out)) return IE_NONFINITE;
+        return IE_OK;
+    }
     for (size_t p = 0; p < op.diagonal.size(); ++p) mulv(op.diagonal[p], x.data() + p * 6, out->data() + p * 6);
     const Normal& ne = *op.ne;
     for (size_t l = 0; l < ne.lm.size(); ++l) {
@@ -1557,18 +1757,20 @@ int complete_delta(const Implicit& op, const std::vector<double>& dp, std::vecto
     if (!all_finite(dp)) return IE_NONFINITE;
     const Normal& ne = *op.ne;
     dl->assign(ne.lm.size() * 3, 0.0);
-    for (size_t l = 0; l < ne.lm.size(); ++l) {
-        if (!op.has_inv[l]) continue;
-        double acc[3] = {-ne.lm[l].b_l[0], -ne.lm[l].b_l[1], -ne.lm[l].b_l[2]};
-        for (const CrossEntry& c : ne.lm[l].cross) {
-            double v[3];
-            mulv(tr(c.w), dp.data() + c.pose * 6, v);
-            for (int k = 0; k < 3; ++k) acc[k] -= v[k];
+    parallel_ranges(op.threads, ne.lm.size(), [&](size_t l0, size_t l1) {
+        for (size_t l = l0; l < l1; ++l) {
+            if (!op.has_inv[l]) continue;
+            double acc[3] = {-ne.lm[l].b_l[0], -ne.lm[l].b_l[1], -ne.lm[l].b_l[2]};
+            for (const CrossEntry& c : ne.lm[l].cross) {
+                double v[3];
+                mulv(tr(c.w), dp.data() + c.pose * 6, v);
+                for (int k = 0; k < 3; ++k) acc[k] -= v[k];
+            }
+            double d[3];
+            mulv(op.inv[l], acc, d);
+            for (int k = 0; k < 3; ++k) (*dl)[l * 3 + k] = d[k];
         }
-        double d[3];
-        mulv(op.inv[l], acc, d);
-        for (int k = 0; k < 3; ++k) (*dl)[l * 3 + k] = d[k];
-    }
+    });
     if (!all_finite(*dl)) return IE_NONFINITE;
     return IE_OK;
 }
@@ -1658,7 +1860,7 @@ int32_t run_lm(Problem& P, const vba_options& o, int threads, vba_iteration_stat
     const bool accept_iterate = o.pcg_maxit_policy == VBA_PCG_MAXIT_ACCEPT_ITERATE;
     double initial_cost;
     uint64_t current_nonproj;
-    cost_and_nonprojectable(P, K, &initial_cost, &current_nonproj);
+    cost_and_nonprojectable(P, K, &initial_cost, &current_nonproj, threads);
     double current_cost = initial_cost;
     double lambda = o.has_initial_lambda ? o.initial_lambda : 0.0;
     bool converged = false;
@@ -1681,10 +1883,10 @@ int32_t run_lm(Problem& P, const vba_options& o, int threads, vba_iteration_stat
         st.iteration = iteration;
         st.pcg_iterations = -1;
         if (!matrix_free) {
-            solve_status = solve_step_dense(ne, P.L, lambda, &dp, &dl);
+            solve_status = solve_step_dense(ne, P.L, lambda, &dp, &dl, threads);
         } else {
             Implicit op;
-            int e = build_implicit(ne, lambda, &op);
+            int e = build_implicit(ne, lambda, &op, threads);
             PcgOut pcg;
             if (e == IE_OK) e = solve_pcg(op, o.max_pcg_iterations, o.pcg_relative_tolerance,
                                           o.pcg_absolute_tolerance, accept_iterate, &pcg);
@@ -1732,7 +1934,7 @@ int32_t run_lm(Problem& P, const vba_options& o, int threads, vba_iteration_stat
         }
         double cost_after;
         uint64_t nonproj_after;
-        cost_and_nonprojectable(P, K, &cost_after, &nonproj_after);
+        cost_and_nonprojectable(P, K, &cost_after, &nonproj_after, threads);
         bool cost_accepted = o.has_initial_lambda ? (cost_after < cost_before) : true;
         bool feasible = nonproj_after <= current_nonproj;
         bool accepted = cost_accepted && feasible;

@@ -391,16 +391,33 @@ def test_first_lm_step_equals_brute_force_full_solve():
     assert np.allclose(out.landmarks, ba.landmarks + delta[6 * Pv:].reshape(-1, 3), atol=1e-9)
 
 
-def test_parallel_assembly_is_bitwise_equal_to_serial():
-    """parallel_ba_tests.rs:113-160."""
-    s = vba.synthesize(abi.SYNTH_LOOP, num_cameras=40, num_landmarks=2500, seed=5, scene_scale=12.0)
-    o = vba.default_options(max_iterations=4)
+@pytest.mark.parametrize("case", ["loop_dense", "loop_pcg_accept", "loop_pcg_reject", "circle_dense"])
+def test_parallel_oracle_is_bitwise_equal_to_serial(case):
+    """parallel_ba_tests.rs:113-160: the threaded solve equals the serial one
+    bit for bit. The problems exceed 65,536 observations so that the threaded
+    assembly, cost, Schur reduction, implicit operator, preconditioner and
+    back-substitution all run."""
+    if case == "circle_dense":
+        s = vba.synthesize(abi.SYNTH_CIRCLE, num_cameras=12, num_landmarks=6000, seed=5)
+        o = vba.default_options(max_iterations=4)
+    else:
+        s = vba.synthesize(abi.SYNTH_LOOP, num_cameras=90, num_landmarks=10000, seed=5, scene_scale=30.0)
+        if case == "loop_dense":
+            o = vba.default_options(max_iterations=4)
+        else:
+            o = vba.default_options(max_iterations=4, linear_solver=abi.SOLVER_MATRIX_FREE, max_pcg_iterations=25,
+                                    pcg_relative_tolerance=1e-6, robust_kernel=abi.KERNEL_HUBER, robust_scale=1.0,
+                                    pcg_maxit_policy=(abi.PCG_MAXIT_ACCEPT_ITERATE if case.endswith("accept")
+                                                      else abi.PCG_MAXIT_REJECT))
+    assert s.problem.num_observations > 65536
     a = oracle.optimize(s.problem, o, threads=1)
-    b = oracle.optimize(s.problem, o, threads=4)
+    b = oracle.optimize(s.problem, o, threads=5)
     assert a.status == b.status == abi.OK
     assert a.result.final_cost == b.result.final_cost
     assert np.array_equal(a.poses, b.poses) and np.array_equal(a.landmarks, b.landmarks)
     assert a.iterations == b.iterations
+    if case != "circle_dense" and case != "loop_dense":
+        assert any(it["pcg_iterations"] > 0 for it in a.iterations)
 
 
 def test_kernel_none_matches_gauss_newton_weights():

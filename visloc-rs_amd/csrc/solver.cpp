@@ -10,6 +10,7 @@
 #include "solver.hpp"
 
 #include <algorithm>
+#include <optional>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -36,9 +37,12 @@ enum BufId {
 
 // Profiled kernel families.
 enum ProfId { K_LINEARIZE, K_SCHUR_APPLY, K_PCG_POSE, K_PCG_STEP, K_COST, K_EXPLICIT_ASSEMBLE, K_EXPLICIT_APPLY,
-              K_COUNT };
-const char* kProfNames[K_COUNT] = {"linearize", "schur_apply",       "pcg_pose",      "pcg_step",
-                                   "cost",      "explicit_assemble", "explicit_apply"};
+              K_DENSE_ASSEMBLE, K_DENSE_SOLVE, K_BATCH_DENSE_ASSEMBLE, K_BATCH_DENSE_SOLVE, K_COUNT };
+// dense_solve / batch_dense_solve record the factor's FLOPs, n^3 / 3 per
+// system, in the bytes slot (their bound is FP64 MFMA, not HBM)
+const char* kProfNames[K_COUNT] = {"linearize",      "schur_apply",          "pcg_pose",         "pcg_step",
+                                   "cost",           "explicit_assemble",    "explicit_apply",   "dense_assemble",
+                                   "dense_solve",    "batch_dense_assemble", "batch_dense_solve"};
 
 struct Fail {
     int32_t code;
@@ -762,6 +766,13 @@ int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stat
         const double bytes_xassemble = nterms * 12.0 + NE * 48.0 + NL * 72.0 +
                                        X.num_full * 36.0 * (8.0 + (mixed ? 4.0 : 0.0)) + NP * (168.0 + 288.0 + 288.0);
         const double bytes_xapply = X.num_full * (36.0 * real + 4.0) + NP * (4.0 + 4.0 * 48.0);
+        // dense assembly: per term its indices (12); per edge G, per landmark X
+        // and V^-1, per pose U once; per block its pair and term pointer (12);
+        // the whole n x n S written. Dense factor: n^3 / 3 FLOPs.
+        const double dterms = !matrix_free ? (double)pr.dense_terms.size() / 3.0 : 0.0;
+        const double bytes_dassemble = dterms * 12.0 + NE * 48.0 + NL * 72.0 + NP * 168.0 + D.num_blocks * 12.0 +
+                                       8.0 * (double)ndense * ndense;
+        const double flops_dfactor = (double)ndense * ndense * ndense / 3.0;
 
         hip_check(hipEventRecord(ctx.ev_start, s), "event");
 
@@ -825,9 +836,15 @@ int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stat
             bool solved = true;
             if (!matrix_free) {
                 hip_check(hipMemsetAsync(dS, 0, (size_t)ndense * ndense * sizeof(double), s), "memset");
-                launch_dense_assemble(P, D, dS, ndense, lambda, s);
+                {
+                    ProfScope a(ctx, K_DENSE_ASSEMBLE, bytes_dassemble);
+                    launch_dense_assemble(P, D, dS, ndense, lambda, s);
+                }
                 if (JK) launch_joint_border(P, dS, ndense, lambda, s);
-                hip_check(launch_dense_solve(P, D, dS, dwork, ndense, dstatus, s), "dense solve");
+                {
+                    ProfScope f(ctx, K_DENSE_SOLVE, flops_dfactor);
+                    hip_check(launch_dense_solve(P, D, dS, dwork, ndense, dstatus, s), "dense solve");
+                }
                 hip_check(hipMemcpyAsync(ctx.host_status, dstatus, sizeof(int32_t), hipMemcpyDeviceToHost, s), "D2H");
                 hip_check(hipStreamSynchronize(s), "sync");
                 solved = *ctx.host_status == 0;
@@ -1343,6 +1360,18 @@ int32_t run_batch(Context& ctx, PreparedBatch& pb, const vba_options& o, vba_ite
         // host synchronisation: every window decides on the device. Kernel
         // launches only (no memsets or copies), so the sequence can be
         // recorded as a graph ----
+        // algorithmic bytes of the batch's dense assembly and FLOPs of its
+        // factors, summed over the windows (as for the single problem)
+        const Model& bm = pb.m;
+        double flops_bfactor = 0.0;
+        for (const WindowLayout& L : pb.layout) {
+            const double n = 6.0 * (L.var_end - L.var_begin);
+            flops_bfactor += n * n * n / 3.0;
+        }
+        const double bytes_bassemble = (double)pb.dense_terms.size() / 3.0 * 12.0 + bm.num_edges * 48.0 +
+                                       bm.num_lm_slots * 72.0 + bm.num_pose_slots * 168.0 +
+                                       pb.num_dense_blocks * 12.0 + 8.0 * (double)pb.s_total;
+        bool capturing = false;
         auto launch_all = [&]() {
             launch_cost(P, false, s);
             launch_batch_scalars(P, B, s);
@@ -1354,8 +1383,18 @@ int32_t run_batch(Context& ctx, PreparedBatch& pb, const vba_options& o, vba_ite
                 launch_landmark_prepare(P, 0.0, s);
                 launch_rhs(P, s);
                 launch_batch_dense_zero(P, B, s);
-                launch_batch_dense_assemble(P, D, B, s);
-                launch_batch_dense_solve(P, D, B, s);
+                {
+                    // events only when the launches are not being recorded
+                    // into the batch's graph
+                    std::optional<ProfScope> a;
+                    if (!capturing) a.emplace(ctx, K_BATCH_DENSE_ASSEMBLE, bytes_bassemble);
+                    launch_batch_dense_assemble(P, D, B, s);
+                }
+                {
+                    std::optional<ProfScope> f;
+                    if (!capturing) f.emplace(ctx, K_BATCH_DENSE_SOLVE, flops_bfactor);
+                    launch_batch_dense_solve(P, D, B, s);
+                }
                 launch_backsub(P, s);
                 launch_pose_update(P, s);
                 launch_cost(P, true, s);
@@ -1378,7 +1417,9 @@ int32_t run_batch(Context& ctx, PreparedBatch& pb, const vba_options& o, vba_ite
                 pb.graph = nullptr;
                 hipGraph_t g = nullptr;
                 hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+                capturing = true;
                 launch_all();
+                capturing = false;
                 const hipError_t launched = hipGetLastError();
                 hip_check(hipStreamEndCapture(s, &g), "hipStreamEndCapture");
                 hip_check(launched, "kernel launch (capture)");
@@ -1399,6 +1440,7 @@ int32_t run_batch(Context& ctx, PreparedBatch& pb, const vba_options& o, vba_ite
         hip_check(hipMemcpyAsync(st_host.data(), B.stats, st_host.size() * sizeof(vba_iteration_stats),
                                  hipMemcpyDeviceToHost, s), "D2H");
         hip_check(hipStreamSynchronize(s), "sync");
+        harvest(ctx);
         float ms_elapsed = 0.f;
         hip_check(hipEventElapsedTime(&ms_elapsed, ctx.ev_start, ctx.ev_stop), "event");
         pb.final_slot = B_POSES;
