@@ -607,9 +607,73 @@ typedef struct vba_reduced_system {
  * clusters longer than 256 nodes, more than 512 coarse aggregates, non-finite
  * input) or when the iteration breaks down as pcg_core returns None (a
  * diagonal block not positive definite, p^T A p not positive and finite).
- * vba_last_error(ctx) says which. */
+ * offdiag == NULL with edges: the blocks the last vba_reduced_eliminate of
+ * the same (nonzero) structure_id formed on this context (declined when
+ * there are none). vba_last_error(ctx) says which. */
 int32_t vba_reduced_solve(vba_context* ctx, const vba_reduced_system* system, double* step,
                           uint32_t* iterations);
+
+/* ---- point elimination ---------------------------------------------------
+ * `ReducedSolveAccelerator::eliminate` (pipelines/slam/src/ba_accel.rs:71-157;
+ * caller colmap_incremental/rig_ba_solver.rs:2733-2855): the off-diagonal
+ * blocks of the reduced camera system formed on the device,
+ *   block(e = (s, t)) = -sum_p hcp[p, j] V_p^-1 hcp[p, k]^T + extra(e)
+ * over the points p whose k-th and j-th nodes are s and t (ascending p), the
+ * (row t, column s) block as in vba_reduced_system. Fields mirror
+ * `PointElimination`:
+ * - edges / row_off: the system's sorted edges (s, t), s < t, and per node
+ *   the first of its edges (s, _); num_nodes + 1 offsets;
+ * - slot_off / slots: per node, the (point, k) pairs of the points observing
+ *   it, ascending by point (2 values per slot);
+ * - point_off / point_nodes: point p's nodes, ascending, at
+ *   point_nodes[point_off[p] .. point_off[p + 1]];
+ * - hcp: J_node^T J_point per (point, k), 6x3 column-major (18 values);
+ * - point_inverse: the damped point inverses, 3x3 column-major (9 values),
+ *   has_inverse[p] == 0 for `None` (the point is left out);
+ * - scale: the per-node Cauchy-Schwarz scales (6 per node); the device sums
+ *   in FP64 and does not need them, they are accepted for the interface;
+ * - extra_edge / extra_blocks: direct couplings added to the eliminated
+ *   blocks (36 values each, column-major);
+ * - run_edges: edges whose final blocks are returned in run_blocks (36 each,
+ *   column-major, FP64), in that order;
+ * - coarse_group > 0: also return coarse_sums[36 (g m + h)], g <= h, the sum
+ *   of the blocks of edges from aggregate g to h (m = ceil(coarse_nodes /
+ *   coarse_group) aggregates of the leading coarse_nodes nodes), summed in
+ *   edge order; entries g > h are zero.
+ * structure_id (nonzero) covers every field but hcp, point_inverse,
+ * has_inverse, scale and the extra couplings; while it stays the same the
+ * device keeps its pair index. The blocks stay on the device for the next
+ * vba_reduced_solve of the same structure with offdiag == NULL.
+ * VBA_OK, or VBA_DECLINED (`None`: the caller eliminates on the CPU) for
+ * inconsistent input; vba_last_error(ctx) says why. */
+typedef struct vba_point_elimination {
+    uint64_t structure_id;
+    uint64_t linearization_id;
+    uint32_t num_nodes;
+    uint32_t num_edges;
+    const uint32_t* edges;
+    const uint64_t* row_off;
+    const uint64_t* slot_off;
+    const uint32_t* slots;
+    uint32_t num_points;
+    uint32_t num_extra;
+    const uint64_t* point_off;
+    const uint32_t* point_nodes;
+    const double* hcp;
+    const double* point_inverse;
+    const uint8_t* has_inverse;
+    const double* scale;
+    const uint32_t* extra_edge;
+    const double* extra_blocks;
+    uint32_t num_run_edges;
+    uint32_t coarse_group;
+    const uint32_t* run_edges;
+    uint32_t coarse_nodes;
+    uint32_t reserved;
+} vba_point_elimination;
+
+int32_t vba_reduced_eliminate(vba_context* ctx, const vba_point_elimination* elimination, double* run_blocks,
+                              double* coarse_sums);
 
 #ifdef __cplusplus
 } /* extern "C" */

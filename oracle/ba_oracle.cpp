@@ -2884,4 +2884,115 @@ int32_t oracle_solve_normal_equations(uint32_t n, const double* a, const double*
     return 1;
 }
 
+// ---------------------------------------------------------------------------
+// Point elimination of the COLMAP-port rig solver's reduced camera system
+// (pipelines/slam/src/colmap_incremental/rig_ba_solver.rs): the off-diagonal
+// blocks of eliminate_by_output (:2551-2660), with the shard partial sums of
+// elimination_shard_count (:1903-1928) and balanced_shard_boundaries
+// (:2902-2925), sub_outer_6x3's fused multiply-adds (:2531-2539) and the
+// product hcp[k] * inv (nalgebra's column sweep); then the direct couplings,
+// the run blocks and the aggregate-pair sums in edge order that
+// ReducedSolveAccelerator::eliminate returns (ba_accel.rs:71-157).
+// ---------------------------------------------------------------------------
+int32_t oracle_reduced_eliminate(const vba_point_elimination* el, double* blocks, double* run_blocks,
+                                 double* coarse_sums) {
+    const uint32_t n = el->num_nodes, E = el->num_edges, NP = el->num_points;
+    // shards: a pure function of the edge count and the points' track lengths
+    const size_t per_shard = (size_t)std::max<uint32_t>(E, 1) * 288;
+    const size_t shards = std::min<size_t>(8, std::max<size_t>(1, (size_t)1536 * 1024 * 1024 / per_shard));
+    std::vector<uint64_t> prefix(NP + 1, 0);
+    for (uint32_t p = 0; p < NP; ++p) {
+        const uint64_t k = el->point_off[p + 1] - el->point_off[p];
+        prefix[p + 1] = prefix[p] + k * (k > 0 ? k - 1 : 0) / 2;
+    }
+    std::vector<size_t> bounds{0};
+    for (size_t sh = 1; sh < shards; ++sh) {
+        const uint64_t target = prefix[NP] * sh / shards;
+        size_t idx = std::partition_point(prefix.begin(), prefix.end(), [&](uint64_t v) { return v < target; }) -
+                     prefix.begin();
+        idx = std::min<size_t>(std::max(idx, bounds.back()), NP);
+        bounds.push_back(idx);
+    }
+    bounds.push_back(NP);
+    std::vector<uint32_t> shard_of(NP, 0);
+    for (size_t sh = 0; sh + 1 < bounds.size(); ++sh)
+        for (size_t p = bounds[sh]; p < bounds[sh + 1]; ++p) shard_of[p] = (uint32_t)sh;
+    // block(e) in column-major order, rows t = later node, columns s
+    std::fill(blocks, blocks + 36 * (size_t)E, 0.0);
+    std::vector<double> acc, scaled(18);
+    std::vector<uint32_t> position(n, 0);
+    for (uint32_t s = 0; s < n; ++s) {
+        const uint64_t r0 = el->row_off[s], r1 = el->row_off[s + 1];
+        for (uint64_t e = r0; e < r1; ++e) position[el->edges[2 * e + 1]] = (uint32_t)(e - r0);
+        acc.assign(36 * (r1 - r0), 0.0);
+        double* total = blocks + 36 * r0;
+        int64_t current = -1;
+        auto flush = [&]() {
+            for (size_t i = 0; i < acc.size(); ++i) {
+                total[i] += acc[i];
+                acc[i] = 0.0;
+            }
+        };
+        for (uint64_t q = el->slot_off[s]; q < el->slot_off[s + 1]; ++q) {
+            const uint32_t p = el->slots[2 * q], k = el->slots[2 * q + 1];
+            if (!el->has_inverse[p]) continue;
+            if (current != (int64_t)shard_of[p]) {
+                if (current >= 0) flush();
+                current = shard_of[p];
+            }
+            const uint64_t po = el->point_off[p];
+            const double* hk = el->hcp + 18 * (po + k);
+            const double* inv = el->point_inverse + 9 * (size_t)p;
+            // scaled = hcp[k] * inv: column c = sum_i hcp.col(i) inv(i, c), i ascending
+            for (int c = 0; c < 3; ++c)
+                for (int r = 0; r < 6; ++r) {
+                    double v = hk[r] * inv[3 * c];
+                    v = v + hk[6 + r] * inv[3 * c + 1];
+                    v = v + hk[12 + r] * inv[3 * c + 2];
+                    scaled[6 * c + r] = v;
+                }
+            const uint64_t pn = el->point_off[p + 1] - po;
+            for (uint64_t j = k + 1; j < pn; ++j) {
+                const uint32_t t = el->point_nodes[po + j];
+                double* a = acc.data() + 36 * (size_t)position[t];
+                const double* hj = el->hcp + 18 * (po + j);
+                for (int c = 0; c < 6; ++c) {
+                    const double b0 = -scaled[c], b1 = -scaled[6 + c], b2 = -scaled[12 + c];
+                    for (int r = 0; r < 6; ++r) {
+                        double& v = a[6 * c + r];
+                        v = std::fma(hj[12 + r], b2, std::fma(hj[6 + r], b1, std::fma(hj[r], b0, v)));
+                    }
+                }
+            }
+        }
+        if (current >= 0) flush();
+    }
+    for (uint32_t x = 0; x < el->num_extra; ++x) {
+        double* b = blocks + 36 * (size_t)el->extra_edge[x];
+        for (int i = 0; i < 36; ++i) b[i] = b[i] + el->extra_blocks[36 * (size_t)x + i];
+    }
+    if (run_blocks)
+        for (uint32_t i = 0; i < el->num_run_edges; ++i)
+            std::copy(blocks + 36 * (size_t)el->run_edges[i], blocks + 36 * (size_t)el->run_edges[i] + 36,
+                      run_blocks + 36 * (size_t)i);
+    if (coarse_sums && el->coarse_group > 0) {
+        const uint32_t grp = el->coarse_group, nodes = el->coarse_nodes;
+        const uint32_t m = (nodes + grp - 1) / grp;
+        std::fill(coarse_sums, coarse_sums + 36 * (size_t)m * m, 0.0);
+        for (uint32_t g = 0; g < m; ++g)
+            for (uint32_t h = g; h < m; ++h) {
+                double sum[36] = {0.0};
+                const uint32_t t0 = h * grp, t1 = std::min((h + 1) * grp, nodes);
+                for (uint32_t a = g * grp; a < std::min((g + 1) * grp, nodes); ++a)
+                    for (uint64_t e = el->row_off[a]; e < el->row_off[a + 1]; ++e) {
+                        const uint32_t b = el->edges[2 * e + 1];
+                        if (b < t0 || b >= t1) continue;
+                        for (int i = 0; i < 36; ++i) sum[i] = sum[i] + blocks[36 * e + i];
+                    }
+                std::copy(sum, sum + 36, coarse_sums + 36 * ((size_t)g * m + h));
+            }
+    }
+    return VBA_OK;
+}
+
 }  // extern "C"

@@ -136,6 +136,13 @@ int32_t oracle_ns_solve_pcg(const oracle_normal_system* ns, double lambda,
  * A is n x n row-major. Returns 1 on success. */
 int32_t oracle_solve_normal_equations(uint32_t n, const double* a, const double* b, double* x);
 
+/* ReducedSolveAccelerator::eliminate's blocks as the CPU rig solver forms
+ * them (eliminate_by_output, rig_ba_solver.rs:2551-2660): blocks[36 E] the
+ * final (row t, column s) blocks, column-major; run_blocks and coarse_sums
+ * as vba_reduced_eliminate returns them (either may be NULL). */
+int32_t oracle_reduced_eliminate(const vba_point_elimination* elimination, double* blocks, double* run_blocks,
+                                 double* coarse_sums);
+
 #ifdef __cplusplus
 }
 #endif

@@ -80,6 +80,8 @@ def library():
     lib.oracle_ns_solve_pcg.restype = C.c_int32
     lib.oracle_ns_solve_pcg.argtypes = [ns, C.c_double, C.c_uint32, C.c_double, C.c_double, P64, P64,
                                         C.POINTER(C.c_uint32), P64, P64]
+    lib.oracle_reduced_eliminate.restype = C.c_int32
+    lib.oracle_reduced_eliminate.argtypes = [C.POINTER(abi.PointElimination), P64, P64, P64]
     lib.oracle_solve_normal_equations.restype = C.c_int32
     lib.oracle_solve_normal_equations.argtypes = [C.c_uint32, P64, P64, P64]
     _lib = lib
@@ -352,3 +354,23 @@ def solve_normal_equations(a, b):
     x = np.zeros(len(b))
     ok = library().oracle_solve_normal_equations(len(b), _p(a), _p(b), _p(x))
     return x if ok else None
+
+
+def reduced_eliminate(elim):
+    """oracle_reduced_eliminate on a vba.PointEliminationInput: (blocks
+    (E, 6, 6), run_blocks (R, 6, 6), coarse_sums (m, m, 6, 6) or None), all
+    row-major 6x6 here; the CPU rig solver's eliminate_by_output
+    (rig_ba_solver.rs:2551-2660) plus the direct couplings, run blocks and
+    aggregate sums of ReducedSolveAccelerator::eliminate."""
+    lib = library()
+    view, keep = elim.view()
+    E = len(keep["edges"])
+    blocks = np.zeros((max(E, 1), 36))
+    runs = np.zeros((max(1, len(keep["run_edges"])), 36))
+    m = elim.coarse_aggregates()
+    coarse = np.zeros((max(1, m * m), 36))
+    st = lib.oracle_reduced_eliminate(C.byref(view), _p(blocks), _p(runs), _p(coarse))
+    assert st == 0, st
+    rm = lambda a: np.swapaxes(a.reshape(-1, 6, 6), -1, -2).copy()
+    return (rm(blocks[:E]), rm(runs[:len(keep["run_edges"])]),
+            np.swapaxes(coarse[:m * m].reshape(m, m, 6, 6), -1, -2).copy() if m else None)

@@ -393,3 +393,85 @@ def perturbed_many_view_bundle(truth_params, start):
             if xc[2] > 0:
                 obs.append((pid, lid, project_brown(truth_params, xc)))
     return make_problem(start, [(p, s) for p, _, s in poses], init_pts, obs, fixed_poses=(100, 109))
+
+
+def elimination_problem(n=120, n_points=2500, seed=3, extras=2, cluster=8, coarse_group=16,
+                        no_inverse=0.1):
+    """A point elimination of the rig solver's shape (vba.PointEliminationInput):
+    n nodes in capture order, the last `extras` of them extra nodes (rig
+    sensors / shared cameras) joined to earlier nodes only by direct
+    couplings; each point seen by a run of 2-12 consecutive nodes with
+    random gaps; random 6x3 camera-point blocks and SPD point inverses, a
+    fraction of the points without one (None). Edges are every node pair
+    sharing a point plus the extra couplings, sorted."""
+    rng = np.random.default_rng(seed)
+    n_cam = n - extras
+    point_nodes, point_off = [], [0]
+    for _ in range(n_points):
+        length = int(rng.integers(2, 13))
+        start = int(rng.integers(0, n_cam - 1))
+        nodes = sorted(set(min(n_cam - 1, start + int(v)) for v in np.cumsum(rng.integers(1, 3, length)) - 1))
+        if len(nodes) < 2:
+            nodes = [start, start + 1]
+        point_nodes.extend(nodes)
+        point_off.append(len(point_nodes))
+    point_nodes = np.array(point_nodes, np.uint32)
+    point_off = np.array(point_off, np.uint64)
+    pairs = set()
+    for p in range(n_points):
+        nd = point_nodes[point_off[p]:point_off[p + 1]]
+        for i in range(len(nd)):
+            for j in range(i + 1, len(nd)):
+                pairs.add((int(nd[i]), int(nd[j])))
+    extra_pairs = []
+    for x in range(extras):
+        for a in sorted(rng.choice(n_cam, size=min(n_cam, 6), replace=False)):
+            extra_pairs.append((int(a), n_cam + x))
+    pairs |= set(extra_pairs)
+    edges = np.array(sorted(pairs), np.uint32)
+    row_off = np.searchsorted(edges[:, 0], np.arange(n + 1)).astype(np.uint64)
+    slots_by_node = [[] for _ in range(n)]
+    for p in range(n_points):
+        for k, s in enumerate(point_nodes[point_off[p]:point_off[p + 1]]):
+            slots_by_node[int(s)].append((p, k))
+    slot_off = np.cumsum([0] + [len(v) for v in slots_by_node]).astype(np.uint64)
+    slots = np.array([sl for v in slots_by_node for sl in v], np.uint32).reshape(-1, 2)
+    hcp = rng.standard_normal((len(point_nodes), 6, 3))
+    a = rng.standard_normal((n_points, 3, 3))
+    inv = np.linalg.inv(a @ np.swapaxes(a, 1, 2) + 0.5 * np.eye(3))
+    inv = 0.5 * (inv + np.swapaxes(inv, 1, 2))
+    has = (rng.uniform(size=n_points) >= no_inverse).astype(np.uint8)
+    eid = {tuple(e): i for i, e in enumerate(edges.tolist())}
+    extra_edge = np.array([eid[e] for e in extra_pairs], np.uint32)
+    extra_blocks = rng.standard_normal((len(extra_pairs), 6, 6))
+    run_edges = np.array([i for i, (a_, b_) in enumerate(edges.tolist()) if a_ // cluster == b_ // cluster],
+                         np.uint32)
+    return vba.PointEliminationInput(
+        num_nodes=n, edges=edges, row_off=row_off, slot_off=slot_off, slots=slots, point_off=point_off,
+        point_nodes=point_nodes, hcp=hcp, point_inverse=inv, has_inverse=has,
+        scale=np.abs(rng.standard_normal((n, 6))) + 1.0, extra_edge=extra_edge, extra_blocks=extra_blocks,
+        run_edges=run_edges, coarse=(coarse_group, n_cam) if coarse_group else None,
+        structure_id=1000 + seed, linearization_id=1)
+
+
+def elimination_brute_force(el):
+    """The eliminated blocks from the dense Schur complement: with Hcp the
+    (6n x 3P) camera-point matrix and Vinv the block-diagonal point
+    inverses (zero for points without one), block (t, s) of
+    -Hcp Vinv Hcp^T, plus the direct couplings."""
+    n = el.num_nodes
+    P = len(el.point_off) - 1
+    H = np.zeros((6 * n, 3 * P))
+    for p in range(P):
+        for k in range(int(el.point_off[p]), int(el.point_off[p + 1])):
+            s = int(el.point_nodes[k])
+            H[6 * s:6 * s + 6, 3 * p:3 * p + 3] = el.hcp[k]
+    V = np.zeros((3 * P, 3 * P))
+    for p in range(P):
+        if el.has_inverse[p]:
+            V[3 * p:3 * p + 3, 3 * p:3 * p + 3] = el.point_inverse[p]
+    S = -(H @ V @ H.T)
+    out = np.array([S[6 * t:6 * t + 6, 6 * s:6 * s + 6] for s, t in el.edges.tolist()])
+    for e, blk in zip(el.extra_edge.tolist(), el.extra_blocks):
+        out[e] = out[e] + blk
+    return out

@@ -645,6 +645,27 @@ int32_t vba_reduced_solve(vba_context* ctx, const vba_reduced_system* system, do
     }
 }
 
+int32_t vba_reduced_eliminate(vba_context* ctx, const vba_point_elimination* elimination, double* run_blocks,
+                              double* coarse_sums) {
+    if (!ctx || !elimination) {
+        vba::set_thread_error("vba_reduced_eliminate: null argument");
+        if (ctx) ctx->last_error = "vba_reduced_eliminate: null argument";
+        return VBA_ERR_INVALID_ARGUMENT;
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->last_error.clear();
+    if (hipSetDevice(ctx->device) != hipSuccess) {
+        ctx->last_error = "hipSetDevice failed";
+        return VBA_ERR_DEVICE;
+    }
+    try {
+        return vba::reduced_eliminate(*ctx, *elimination, run_blocks, coarse_sums, &ctx->last_error);
+    } catch (const std::bad_alloc&) {
+        ctx->last_error = "out of host memory";
+        return VBA_ERR_OUT_OF_MEMORY;
+    }
+}
+
 int32_t vba_profile_query(vba_context* ctx, const char* kernel, uint64_t* launches, double* seconds,
                           double* bytes) {
     if (!ctx || !kernel) return VBA_ERR_INVALID_ARGUMENT;

@@ -17,6 +17,7 @@
 // host reads the state once per ten iterations.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -346,6 +347,132 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_update(RsDev D, int k, int nu
     wave_values_to_partial(v, lds, D.part_b + 2 * (size_t)blockIdx.x);
 }
 
+// ---------------------------------------------------------------------------
+// Point elimination (ReducedSolveAccelerator::eliminate, ba_accel.rs:71-157):
+// the blocks -sum_p hcp[p, j] V_p^-1 hcp[p, k]^T of the CPU rig solver's
+// eliminate_by_output (rig_ba_solver.rs:2551-2660), in its arithmetic: the
+// product hcp[k] V^-1 with separately rounded multiplies and adds (a column
+// sweep), each entry a chain of three fused multiply-adds per point over the
+// points in ascending order, with a fresh partial sum per point shard added
+// to the total when the shard ends. The pair index lists every (point, k < j)
+// term under its edge, points ascending.
+// ---------------------------------------------------------------------------
+struct ElDev {
+    int32_t num_edges, num_slots, num_extra, num_run_edges;
+    int32_t coarse_group, coarse_nodes, coarse_m, pad;
+    const int64_t* term_ptr;     // per edge: its terms
+    const uint32_t* terms;       // 3 per term: slot index of k, slot index of j, point
+    const uint8_t* shard;        // per point
+    const uint32_t* point_of;    // per slot index: its point
+    const double* hcp;           // 18 per slot, 6x3 column-major
+    const double* vinv;          // 9 per point, column-major
+    const uint8_t* has;          // per point
+    double* scaled;              // 18 per slot: hcp V^-1
+    double* b64;                 // 36 per edge, column-major
+    float* off;                  // the solve's FP32 blocks
+    const uint32_t* extra_edge;
+    const double* extra_blocks;
+    const uint32_t* run_edges;
+    double* run_out;
+    const uint32_t* edges;
+    const int32_t* out_ptr;      // per node: its edges (s, _)
+    double* coarse_out;
+};
+
+__global__ __launch_bounds__(kRsThreads) void k_el_scaled(ElDev D) {
+    // every product and sum rounded on its own, as the CPU's (no fusion)
+#pragma clang fp contract(off)
+    const int q = blockIdx.x * kRsThreads + threadIdx.x;
+    if (q >= D.num_slots) return;
+    const uint32_t p = D.point_of[q];
+    if (!D.has[p]) return;
+    const double* h = D.hcp + 18 * (size_t)q;
+    const double* v = D.vinv + 9 * (size_t)p;
+    double* o = D.scaled + 18 * (size_t)q;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            double x = h[r] * v[3 * c];
+            x = x + h[6 + r] * v[3 * c + 1];
+            x = x + h[12 + r] * v[3 * c + 2];
+            o[6 * c + r] = x;
+        }
+}
+
+// One wave per edge block; lane q < 36 forms entry q (column q / 6, row
+// q % 6) over the edge's terms in point order.
+__global__ __launch_bounds__(kRsThreads) void k_el_blocks(ElDev D) {
+    const int lane = threadIdx.x & 63;
+    const int wg = xcd_tile(blockIdx.x, (D.num_edges + kRsThreads / 64 - 1) / (kRsThreads / 64));
+    const int e = wg * (kRsThreads / 64) + (threadIdx.x >> 6);
+    if (e >= D.num_edges || lane >= 36) return;
+    const int c = lane / 6, r = lane % 6;
+    double total = 0.0, acc = 0.0;
+    int cur = -1;
+    const int64_t t1 = D.term_ptr[e + 1];
+    for (int64_t t = D.term_ptr[e]; t < t1; ++t) {
+        const uint32_t sk = D.terms[3 * t], sj = D.terms[3 * t + 1], p = D.terms[3 * t + 2];
+        if (!D.has[p]) continue;
+        const int sh = D.shard[p];
+        if (sh != cur) {
+            if (cur >= 0) {
+                total = total + acc;
+                acc = 0.0;
+            }
+            cur = sh;
+        }
+        const double* b = D.scaled + 18 * (size_t)sk;
+        const double* a = D.hcp + 18 * (size_t)sj;
+        acc = fma(a[12 + r], -b[12 + c], fma(a[6 + r], -b[6 + c], fma(a[r], -b[c], acc)));
+    }
+    if (cur >= 0) total = total + acc;
+    D.b64[36 * (size_t)e + lane] = total;
+}
+
+// Direct couplings (one per edge at most), then the FP32 blocks of the solve.
+__global__ __launch_bounds__(kRsThreads) void k_el_extra(ElDev D) {
+    const size_t i = (size_t)blockIdx.x * kRsThreads + threadIdx.x;
+    if (i >= 36 * (size_t)D.num_extra) return;
+    const size_t k = 36 * (size_t)D.extra_edge[i / 36] + i % 36;
+    D.b64[k] = D.b64[k] + D.extra_blocks[i];
+}
+__global__ __launch_bounds__(kRsThreads) void k_el_store(ElDev D) {
+    const size_t i = (size_t)blockIdx.x * kRsThreads + threadIdx.x;
+    if (i < 36 * (size_t)D.num_edges) D.off[i] = __double2float_rn(D.b64[i]);
+    if (i < 36 * (size_t)D.num_run_edges) D.run_out[i] = D.b64[36 * (size_t)D.run_edges[i / 36] + i % 36];
+}
+
+// Aggregate pair (g, h), g <= h: the sum of the blocks of edges (a, b), a in
+// g and b in h (both below coarse_nodes), in edge order. One thread per pair.
+__global__ __launch_bounds__(kRsThreads) void k_el_coarse(ElDev D) {
+    const int m = D.coarse_m;
+    const int idx = blockIdx.x * kRsThreads + threadIdx.x;
+    if (idx >= m * m) return;
+    const int g = idx / m, h = idx % m;
+    double* o = D.coarse_out + 36 * (size_t)idx;
+    double sum[36];
+#pragma unroll
+    for (int q = 0; q < 36; ++q) sum[q] = 0.0;
+    if (g <= h) {
+        const int grp = D.coarse_group, nodes = D.coarse_nodes;
+        const uint32_t t0 = (uint32_t)(h * grp), t1 = (uint32_t)min((h + 1) * grp, nodes);
+        for (int a = g * grp; a < min((g + 1) * grp, nodes); ++a) {
+            int lo = D.out_ptr[a], hi = D.out_ptr[a + 1];
+            while (lo < hi) {  // first edge of row a with b >= t0
+                const int mid = (lo + hi) >> 1;
+                if (D.edges[2 * (size_t)mid + 1] < t0) lo = mid + 1;
+                else hi = mid;
+            }
+            for (int e = lo; e < D.out_ptr[a + 1] && D.edges[2 * (size_t)e + 1] < t1; ++e)
+#pragma unroll
+                for (int q = 0; q < 36; ++q) sum[q] = sum[q] + D.b64[36 * (size_t)e + q];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 36; ++q) o[q] = sum[q];
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -355,6 +482,9 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_update(RsDev D, int k, int nu
 struct ReducedState {
     uint64_t structure_id = 0;
     int32_t n = -1, num_edges = -1;
+    // the point elimination's pair index (for elim_structure) and the
+    // structure whose blocks the last elimination left in the solve's buffer
+    uint64_t elim_structure = 0, blocks_of = 0;
     std::vector<void*> bufs;
     std::vector<size_t> sizes;
 };
@@ -363,7 +493,9 @@ namespace {
 
 enum RsBuf {
     R_EDGES, R_IN_PTR, R_IN_EDGE, R_IN_FROM, R_OUT_PTR, R_DIAG, R_OFF, R_OFF_IN, R_RHS, R_CL_INV, R_CL_OFF, R_CINV, R_X, R_R, R_Z, R_P,
-    R_AP, R_MINV, R_RC, R_YC, R_PART_A, R_PART_B, R_STATE, R_COUNT
+    R_AP, R_MINV, R_RC, R_YC, R_PART_A, R_PART_B, R_STATE, R_EL_TERM_PTR, R_EL_TERMS, R_EL_SHARD, R_EL_POINT_OF,
+    R_EL_HCP, R_EL_VINV, R_EL_HAS, R_EL_SCALED, R_EL_B64, R_EL_EXTRA_E, R_EL_EXTRA_B, R_EL_RUN_E, R_EL_RUN_OUT,
+    R_EL_COARSE, R_COUNT
 };
 
 struct RsFail {
@@ -400,6 +532,40 @@ T* rs_upload(ReducedState& R, int id, const T* src, size_t count, hipStream_t s)
     return p;
 }
 
+// The solve's edge structure (in/out lists of every node), rebuilt unless the
+// caller's structure id (nonzero) and the sizes are those it was built for.
+// A rebuild forgets the blocks of an earlier elimination.
+void ensure_structure(ReducedState& R, uint64_t structure_id, uint32_t n, uint32_t E, const uint32_t* edges,
+                      hipStream_t s) {
+    if (structure_id != 0 && structure_id == R.structure_id && R.n == (int32_t)n && R.num_edges == (int32_t)E)
+        return;
+    std::vector<int32_t> in_ptr(n + 1, 0), in_edge(E), out_ptr(n + 1, 0);
+    for (uint32_t e = 0; e < E; ++e) {
+        ++out_ptr[edges[2 * (size_t)e] + 1];
+        ++in_ptr[edges[2 * (size_t)e + 1] + 1];
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        out_ptr[i + 1] += out_ptr[i];
+        in_ptr[i + 1] += in_ptr[i];
+    }
+    std::vector<int32_t> cur(in_ptr.begin(), in_ptr.end() - 1), in_from(E);
+    for (uint32_t e = 0; e < E; ++e) {
+        const int32_t q = cur[edges[2 * (size_t)e + 1]]++;
+        in_edge[q] = (int32_t)e;
+        in_from[q] = (int32_t)edges[2 * (size_t)e];
+    }
+    rs_upload(R, R_EDGES, edges, 2 * (size_t)E, s);
+    rs_upload(R, R_IN_PTR, in_ptr.data(), in_ptr.size(), s);
+    rs_upload(R, R_IN_EDGE, in_edge.data(), in_edge.size(), s);
+    rs_upload(R, R_IN_FROM, in_from.data(), in_from.size(), s);
+    rs_upload(R, R_OUT_PTR, out_ptr.data(), out_ptr.size(), s);
+    rs_check_hip(hipStreamSynchronize(s), "sync");  // the host vectors go out of scope
+    R.structure_id = structure_id;
+    R.n = (int32_t)n;
+    R.num_edges = (int32_t)E;
+    R.blocks_of = 0;
+}
+
 int32_t decline(std::string* msg, const char* why) {
     *msg = why;
     return VBA_DECLINED;
@@ -420,9 +586,13 @@ int32_t reduced_solve(Context& ctx, const vba_reduced_system& sys, double* step,
     const uint32_t n = sys.num_nodes, E = sys.num_edges;
     // Shapes the solve supports (as the reference GPU backend's checks,
     // crates/ba-gpu/src/reduced.rs:654-760, decline the rest).
-    if (n == 0 || !sys.diag || !sys.rhs || (E && (!sys.edges || !sys.offdiag)) || E >= (1u << 31) ||
-        n > (1u << 24))
+    if (n == 0 || !sys.diag || !sys.rhs || (E && !sys.edges) || E >= (1u << 31) || n > (1u << 24))
         return decline(msg, "reduced system: empty, missing arrays or too large");
+    if (E && !sys.offdiag &&
+        !(ctx.reduced && sys.structure_id != 0 && ctx.reduced->blocks_of == sys.structure_id &&
+          ctx.reduced->structure_id == sys.structure_id && ctx.reduced->n == (int32_t)n &&
+          ctx.reduced->num_edges == (int32_t)E))
+        return decline(msg, "reduced system: no offdiag blocks, and no elimination of this structure on the device");
     for (uint32_t e = 0; e < E; ++e) {
         const uint32_t a = sys.edges[2 * (size_t)e], b = sys.edges[2 * (size_t)e + 1];
         if (!(a < b && b < n)) return decline(msg, "reduced system: an edge is not (a, b) with a < b < n");
@@ -457,41 +627,19 @@ int32_t reduced_solve(Context& ctx, const vba_reduced_system& sys, double* step,
         D.coarse_group = coarse ? (int32_t)sys.coarse_group : 1;
         D.coarse_nodes = coarse ? (int32_t)sys.coarse_nodes : 0;
         D.coarse_m = cm;
-        // edge structure: reused while the caller's structure id (nonzero) stays the same
-        if (sys.structure_id == 0 || sys.structure_id != R.structure_id || R.n != (int32_t)n ||
-            R.num_edges != (int32_t)E) {
-            std::vector<int32_t> in_ptr(n + 1, 0), in_edge(E), out_ptr(n + 1, 0);
-            for (uint32_t e = 0; e < E; ++e) {
-                ++out_ptr[sys.edges[2 * (size_t)e] + 1];
-                ++in_ptr[sys.edges[2 * (size_t)e + 1] + 1];
-            }
-            for (uint32_t i = 0; i < n; ++i) {
-                out_ptr[i + 1] += out_ptr[i];
-                in_ptr[i + 1] += in_ptr[i];
-            }
-            std::vector<int32_t> cur(in_ptr.begin(), in_ptr.end() - 1), in_from(E);
-            for (uint32_t e = 0; e < E; ++e) {
-                const int32_t q = cur[sys.edges[2 * (size_t)e + 1]]++;
-                in_edge[q] = (int32_t)e;
-                in_from[q] = (int32_t)sys.edges[2 * (size_t)e];
-            }
-            rs_upload(R, R_EDGES, sys.edges, 2 * (size_t)E, s);
-            rs_upload(R, R_IN_PTR, in_ptr.data(), in_ptr.size(), s);
-            rs_upload(R, R_IN_EDGE, in_edge.data(), in_edge.size(), s);
-            rs_upload(R, R_IN_FROM, in_from.data(), in_from.size(), s);
-            rs_upload(R, R_OUT_PTR, out_ptr.data(), out_ptr.size(), s);
-            rs_check_hip(hipStreamSynchronize(s), "sync");  // the host vectors go out of scope
-            R.structure_id = sys.structure_id;
-            R.n = (int32_t)n;
-            R.num_edges = (int32_t)E;
-        }
+        ensure_structure(R, sys.structure_id, n, E, sys.edges, s);
         D.edges = (const uint32_t*)R.bufs[R_EDGES];
         D.in_ptr = (const int32_t*)R.bufs[R_IN_PTR];
         D.in_edge = (const int32_t*)R.bufs[R_IN_EDGE];
         D.in_from = (const int32_t*)R.bufs[R_IN_FROM];
         D.out_ptr = (const int32_t*)R.bufs[R_OUT_PTR];
         D.diag = rs_upload(R, R_DIAG, sys.diag, 36 * (size_t)n, s);
-        D.off = rs_upload(R, R_OFF, sys.offdiag, 36 * (size_t)E, s);
+        if (sys.offdiag || !E) {
+            D.off = rs_upload(R, R_OFF, sys.offdiag, 36 * (size_t)E, s);
+            R.blocks_of = 0;
+        } else {
+            D.off = (const float*)R.bufs[R_OFF];  // formed by reduced_eliminate for this structure
+        }
         D.off_in = (float*)rs_buf(R, R_OFF_IN, 36 * (size_t)E * sizeof(float));
         D.rhs = rs_upload(R, R_RHS, sys.rhs, 6 * (size_t)n, s);
         std::vector<int64_t> cl_off;
@@ -578,6 +726,178 @@ int32_t reduced_solve(Context& ctx, const vba_reduced_system& sys, double* step,
         rs_check_hip(hipMemcpyAsync(step, D.x, n6 * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
         rs_check_hip(hipStreamSynchronize(s), "sync");
         *iterations = (uint32_t)hs->iter;
+        return VBA_OK;
+    } catch (const RsFail& f) {
+        *msg = f.msg;
+        return f.code;
+    }
+}
+
+int32_t reduced_eliminate(Context& ctx, const vba_point_elimination& el, double* run_blocks, double* coarse_sums,
+                          std::string* msg) {
+    const uint32_t n = el.num_nodes, E = el.num_edges, NP = el.num_points;
+    if (n == 0 || n > (1u << 24) || E >= (1u << 31) || el.structure_id == 0)
+        return decline(msg, "point elimination: no nodes, too large, or structure_id 0");
+    if ((E && (!el.edges || !el.row_off)) || !el.slot_off || (NP && (!el.point_off || !el.has_inverse)) ||
+        (el.num_extra && (!el.extra_edge || !el.extra_blocks)) || (el.num_run_edges && (!el.run_edges || !run_blocks)) ||
+        (el.coarse_group && !coarse_sums))
+        return decline(msg, "point elimination: missing arrays");
+    const uint64_t H = NP ? el.point_off[NP] : 0;
+    if (H && (!el.point_nodes || !el.hcp || !el.point_inverse))
+        return decline(msg, "point elimination: missing point blocks");
+    if (!ctx.reduced) ctx.reduced = new ReducedState();
+    ReducedState& R = *ctx.reduced;
+    hipStream_t s = ctx.stream;
+    const int32_t m = el.coarse_group ? (int32_t)((el.coarse_nodes + el.coarse_group - 1) / el.coarse_group) : 0;
+    if (el.coarse_group && (el.coarse_nodes == 0 || el.coarse_nodes > n || m > 4096))
+        return decline(msg, "point elimination: invalid coarse aggregates");
+    for (uint32_t i = 0; i < el.num_run_edges; ++i)
+        if (el.run_edges[i] >= E) return decline(msg, "point elimination: a run edge is out of range");
+    {
+        std::vector<uint8_t> seen(E, 0);
+        for (uint32_t x = 0; x < el.num_extra; ++x) {
+            if (el.extra_edge[x] >= E || seen[el.extra_edge[x]]) return decline(msg, "point elimination: extra edges must be distinct edges");
+            seen[el.extra_edge[x]] = 1;
+        }
+    }
+    try {
+        ElDev D{};
+        D.num_edges = (int32_t)E;
+        D.num_slots = (int32_t)H;
+        D.num_extra = (int32_t)el.num_extra;
+        D.num_run_edges = (int32_t)el.num_run_edges;
+        D.coarse_group = (int32_t)el.coarse_group;
+        D.coarse_nodes = (int32_t)el.coarse_nodes;
+        D.coarse_m = m;
+        if (el.structure_id != R.elim_structure || R.structure_id != el.structure_id || R.n != (int32_t)n ||
+            R.num_edges != (int32_t)E) {
+            // validate the structure and build the pair index
+            if (el.row_off[0] != 0 || el.row_off[n] != E) return decline(msg, "point elimination: row_off");
+            for (uint32_t a = 0; a < n; ++a) {
+                if (el.row_off[a] > el.row_off[a + 1]) return decline(msg, "point elimination: row_off");
+                for (uint64_t e = el.row_off[a]; e < el.row_off[a + 1]; ++e) {
+                    const uint32_t x = el.edges[2 * e], y = el.edges[2 * e + 1];
+                    if (x != a || y <= a || y >= n || (e > el.row_off[a] && el.edges[2 * e - 1] >= y))
+                        return decline(msg, "point elimination: edges are not sorted (s, t), s < t < n, by row");
+                }
+            }
+            if (el.point_off[0] != 0 || H >= (1ull << 32)) return decline(msg, "point elimination: point_off");
+            std::vector<int64_t> term_ptr((size_t)E + 1, 0);
+            std::vector<uint32_t> pair_edge;
+            std::vector<uint32_t> point_of(H);
+            for (uint32_t p = 0; p < NP; ++p) {
+                const uint64_t po = el.point_off[p], pe = el.point_off[p + 1];
+                if (pe < po || pe > H) return decline(msg, "point elimination: point_off");
+                for (uint64_t k = po; k < pe; ++k) {
+                    point_of[k] = p;
+                    if (el.point_nodes[k] >= n || (k > po && el.point_nodes[k - 1] >= el.point_nodes[k]))
+                        return decline(msg, "point elimination: a point's nodes are not ascending below num_nodes");
+                }
+                for (uint64_t k = po; k < pe; ++k) {
+                    const uint32_t a = el.point_nodes[k];
+                    uint64_t lo = el.row_off[a];
+                    const uint64_t hi = el.row_off[a + 1];
+                    for (uint64_t j = k + 1; j < pe; ++j) {
+                        const uint32_t b = el.point_nodes[j];
+                        // partners ascend, so the search continues from the last edge
+                        uint64_t l2 = lo, h2 = hi;
+                        while (l2 < h2) {
+                            const uint64_t mid = (l2 + h2) / 2;
+                            if (el.edges[2 * mid + 1] < b) l2 = mid + 1;
+                            else h2 = mid;
+                        }
+                        if (l2 >= hi || el.edges[2 * l2 + 1] != b)
+                            return decline(msg, "point elimination: a pair of a point's nodes is not an edge");
+                        lo = l2;
+                        pair_edge.push_back((uint32_t)l2);
+                        ++term_ptr[l2 + 1];
+                    }
+                }
+            }
+            for (uint32_t e = 0; e < E; ++e) term_ptr[e + 1] += term_ptr[e];
+            std::vector<uint32_t> terms(3 * pair_edge.size());
+            {
+                std::vector<int64_t> cur(term_ptr.begin(), term_ptr.end() - 1);
+                size_t q = 0;
+                for (uint32_t p = 0; p < NP; ++p) {
+                    const uint64_t po = el.point_off[p], pe = el.point_off[p + 1];
+                    for (uint64_t k = po; k < pe; ++k)
+                        for (uint64_t j = k + 1; j < pe; ++j) {
+                            const int64_t t = cur[pair_edge[q++]]++;
+                            terms[3 * t] = (uint32_t)k;
+                            terms[3 * t + 1] = (uint32_t)j;
+                            terms[3 * t + 2] = p;
+                        }
+                }
+            }
+            // the CPU elimination's point shards (elimination_shard_count,
+            // balanced_shard_boundaries, rig_ba_solver.rs:1903-1928, 2902-2925):
+            // a pure function of the edge count and the track lengths
+            const uint64_t per_shard = (uint64_t)std::max<uint32_t>(E, 1) * 288;
+            const uint64_t shards = std::min<uint64_t>(8, std::max<uint64_t>(1, (1536ull << 20) / per_shard));
+            std::vector<uint64_t> prefix((size_t)NP + 1, 0);
+            for (uint32_t p = 0; p < NP; ++p) {
+                const uint64_t k = el.point_off[p + 1] - el.point_off[p];
+                prefix[p + 1] = prefix[p] + k * (k > 0 ? k - 1 : 0) / 2;
+            }
+            std::vector<uint8_t> shard(std::max<uint32_t>(NP, 1), 0);
+            size_t prev = 0;
+            for (uint64_t sh = 1; sh <= shards; ++sh) {
+                size_t idx = NP;
+                if (sh < shards) {
+                    const uint64_t target = prefix[NP] * sh / shards;
+                    idx = (size_t)(std::partition_point(prefix.begin(), prefix.end(),
+                                                        [&](uint64_t v) { return v < target; }) -
+                                   prefix.begin());
+                    idx = std::min<size_t>(std::max(idx, prev), NP);
+                }
+                for (size_t p = prev; p < idx; ++p) shard[p] = (uint8_t)(sh - 1);
+                prev = idx;
+            }
+            ensure_structure(R, el.structure_id, n, E, el.edges, s);
+            rs_upload(R, R_EL_TERM_PTR, term_ptr.data(), term_ptr.size(), s);
+            rs_upload(R, R_EL_TERMS, terms.data(), terms.size(), s);
+            rs_upload(R, R_EL_SHARD, shard.data(), shard.size(), s);
+            rs_upload(R, R_EL_POINT_OF, point_of.data(), point_of.size(), s);
+            rs_check_hip(hipStreamSynchronize(s), "sync");  // the host vectors go out of scope
+            R.elim_structure = el.structure_id;
+        }
+        D.term_ptr = (const int64_t*)R.bufs[R_EL_TERM_PTR];
+        D.terms = (const uint32_t*)R.bufs[R_EL_TERMS];
+        D.shard = (const uint8_t*)R.bufs[R_EL_SHARD];
+        D.point_of = (const uint32_t*)R.bufs[R_EL_POINT_OF];
+        D.edges = (const uint32_t*)R.bufs[R_EDGES];
+        D.out_ptr = (const int32_t*)R.bufs[R_OUT_PTR];
+        D.hcp = rs_upload(R, R_EL_HCP, el.hcp, 18 * (size_t)H, s);
+        D.vinv = rs_upload(R, R_EL_VINV, el.point_inverse, 9 * (size_t)NP, s);
+        D.has = rs_upload(R, R_EL_HAS, el.has_inverse, (size_t)NP, s);
+        D.scaled = (double*)rs_buf(R, R_EL_SCALED, 18 * (size_t)H * sizeof(double));
+        D.b64 = (double*)rs_buf(R, R_EL_B64, 36 * (size_t)E * sizeof(double));
+        D.off = (float*)rs_buf(R, R_OFF, 36 * (size_t)E * sizeof(float));
+        D.extra_edge = rs_upload(R, R_EL_EXTRA_E, el.extra_edge, el.num_extra, s);
+        D.extra_blocks = rs_upload(R, R_EL_EXTRA_B, el.extra_blocks, 36 * (size_t)el.num_extra, s);
+        D.run_edges = rs_upload(R, R_EL_RUN_E, el.run_edges, el.num_run_edges, s);
+        D.run_out = (double*)rs_buf(R, R_EL_RUN_OUT, 36 * (size_t)el.num_run_edges * sizeof(double));
+        D.coarse_out = (double*)rs_buf(R, R_EL_COARSE, 36 * (size_t)m * m * sizeof(double));
+        R.blocks_of = 0;
+        auto grid = [](size_t work) { return dim3((unsigned)std::max<size_t>(1, (work + kRsThreads - 1) / kRsThreads)); };
+        if (H) hipLaunchKernelGGL(k_el_scaled, grid(H), dim3(kRsThreads), 0, s, D);
+        if (E) {
+            const int wgs = (int)((E + kRsThreads / 64 - 1) / (kRsThreads / 64));
+            hipLaunchKernelGGL(k_el_blocks, dim3(xcd_grid(wgs)), dim3(kRsThreads), 0, s, D);
+        }
+        if (el.num_extra) hipLaunchKernelGGL(k_el_extra, grid(36 * (size_t)el.num_extra), dim3(kRsThreads), 0, s, D);
+        hipLaunchKernelGGL(k_el_store, grid(36 * std::max<size_t>(E, el.num_run_edges)), dim3(kRsThreads), 0, s, D);
+        if (m) hipLaunchKernelGGL(k_el_coarse, grid((size_t)m * m), dim3(kRsThreads), 0, s, D);
+        rs_check_hip(hipGetLastError(), "kernel launch");
+        if (el.num_run_edges)
+            rs_check_hip(hipMemcpyAsync(run_blocks, D.run_out, 36 * (size_t)el.num_run_edges * sizeof(double),
+                                        hipMemcpyDeviceToHost, s), "D2H");
+        if (m)
+            rs_check_hip(hipMemcpyAsync(coarse_sums, D.coarse_out, 36 * (size_t)m * m * sizeof(double),
+                                        hipMemcpyDeviceToHost, s), "D2H");
+        rs_check_hip(hipStreamSynchronize(s), "sync");
+        R.blocks_of = el.structure_id;
         return VBA_OK;
     } catch (const RsFail& f) {
         *msg = f.msg;

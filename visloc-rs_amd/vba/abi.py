@@ -285,6 +285,34 @@ class ReducedSystem(C.Structure):
     ]
 
 
+class PointElimination(C.Structure):
+    _fields_ = [
+        ("structure_id", C.c_uint64),
+        ("linearization_id", C.c_uint64),
+        ("num_nodes", C.c_uint32),
+        ("num_edges", C.c_uint32),
+        ("edges", PU32),
+        ("row_off", PU64),
+        ("slot_off", PU64),
+        ("slots", PU32),
+        ("num_points", C.c_uint32),
+        ("num_extra", C.c_uint32),
+        ("point_off", PU64),
+        ("point_nodes", PU32),
+        ("hcp", P64),
+        ("point_inverse", P64),
+        ("has_inverse", PU8),
+        ("scale", P64),
+        ("extra_edge", PU32),
+        ("extra_blocks", P64),
+        ("num_run_edges", C.c_uint32),
+        ("coarse_group", C.c_uint32),
+        ("run_edges", PU32),
+        ("coarse_nodes", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
+
+
 SIGNATURES = {
     "vba_abi_version": (C.c_uint32, []),
     "vba_create": (C.c_void_p, [C.c_int32]),
@@ -332,6 +360,7 @@ SIGNATURES = {
     "vba_dense_solve": (C.c_int32, [C.c_void_p, C.c_uint32, P64, P64, P64]),
     "vba_dense_solve_noalloc": (C.c_int32, [C.c_void_p, C.c_uint32, P64, P64, P64]),
     "vba_reduced_solve": (C.c_int32, [C.c_void_p, C.POINTER(ReducedSystem), P64, C.POINTER(C.c_uint32)]),
+    "vba_reduced_eliminate": (C.c_int32, [C.c_void_p, C.POINTER(PointElimination), P64, P64]),
     "vba_optimize_weighted": (
         C.c_int32,
         [C.c_void_p, C.POINTER(Problem), C.POINTER(Options), P64, C.c_uint64, P64, P64,

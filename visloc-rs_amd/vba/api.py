@@ -297,6 +297,74 @@ def run_solver(fn, problem: BundleProblem, options: abi.Options, cap: int = 1024
     return SolveOutput(st, res, stats_list(stats, n) if ok else [], poses if ok else None, lms if ok else None)
 
 
+@dataclass
+class PointEliminationInput:
+    """`PointElimination` (pipelines/slam/src/ba_accel.rs:71-121) as numpy
+    arrays: edges (E, 2) sorted (s, t), s < t; row_off (n + 1); slot_off
+    (n + 1) and slots (S, 2) = (point, k), ascending point per node;
+    point_off (P + 1) and point_nodes (ascending per point); hcp (H, 6, 3)
+    per (point, k), row-major here; point_inverse (P, 3, 3) with
+    has_inverse (P) for Some / None; scale (n, 6); extra as (edge ids,
+    (X, 6, 6) blocks); run_edges; coarse as (group, nodes) or None."""
+    num_nodes: int
+    edges: np.ndarray
+    row_off: np.ndarray
+    slot_off: np.ndarray
+    slots: np.ndarray
+    point_off: np.ndarray
+    point_nodes: np.ndarray
+    hcp: np.ndarray
+    point_inverse: np.ndarray
+    has_inverse: np.ndarray
+    scale: np.ndarray
+    extra_edge: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint32))
+    extra_blocks: np.ndarray = field(default_factory=lambda: np.zeros((0, 6, 6)))
+    run_edges: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint32))
+    coarse: Optional[tuple] = None
+    structure_id: int = 1
+    linearization_id: int = 1
+
+    def coarse_aggregates(self) -> int:
+        if self.coarse is None:
+            return 0
+        group, nodes = self.coarse
+        return -(-nodes // group)
+
+    def view(self):
+        """(abi.PointElimination, arrays to keep alive while it is used)."""
+        col = lambda m: np.ascontiguousarray(np.swapaxes(np.asarray(m, np.float64), -1, -2))
+        arrs = dict(
+            edges=np.ascontiguousarray(np.asarray(self.edges, np.uint32).reshape(-1, 2)),
+            row_off=np.ascontiguousarray(self.row_off, np.uint64),
+            slot_off=np.ascontiguousarray(self.slot_off, np.uint64),
+            slots=np.ascontiguousarray(np.asarray(self.slots, np.uint32).reshape(-1, 2)),
+            point_off=np.ascontiguousarray(self.point_off, np.uint64),
+            point_nodes=np.ascontiguousarray(self.point_nodes, np.uint32),
+            hcp=col(np.asarray(self.hcp).reshape(-1, 6, 3)),
+            point_inverse=col(np.asarray(self.point_inverse).reshape(-1, 3, 3)),
+            has_inverse=np.ascontiguousarray(self.has_inverse, np.uint8),
+            scale=np.ascontiguousarray(np.asarray(self.scale, np.float64).reshape(-1, 6)),
+            extra_edge=np.ascontiguousarray(self.extra_edge, np.uint32),
+            extra_blocks=col(np.asarray(self.extra_blocks).reshape(-1, 6, 6)),
+            run_edges=np.ascontiguousarray(self.run_edges, np.uint32),
+        )
+        v = abi.PointElimination()
+        v.structure_id = self.structure_id
+        v.linearization_id = self.linearization_id
+        v.num_nodes = self.num_nodes
+        v.num_edges = len(arrs["edges"])
+        v.num_points = len(arrs["point_off"]) - 1
+        v.num_extra = len(arrs["extra_edge"])
+        v.num_run_edges = len(arrs["run_edges"])
+        if self.coarse is not None:
+            v.coarse_group, v.coarse_nodes = int(self.coarse[0]), int(self.coarse[1])
+        ptr = {np.dtype(np.uint32): abi.PU32, np.dtype(np.uint64): abi.PU64, np.dtype(np.uint8): abi.PU8,
+               np.dtype(np.float64): abi.P64}
+        for k, a in arrs.items():
+            setattr(v, k, a.ctypes.data_as(ptr[a.dtype]))
+        return v, arrs
+
+
 class Device:
     """A libvba context on one HIP device (vba_create / vba_destroy)."""
 
@@ -439,7 +507,8 @@ class Device:
         edges = np.ascontiguousarray(np.asarray(edges, dtype=np.uint32).reshape(-1, 2))
         col = lambda m: np.ascontiguousarray(np.swapaxes(np.asarray(m), -1, -2))
         d = col(diag).astype(np.float64)
-        off = col(np.asarray(offdiag, dtype=np.float32).reshape(-1, 6, 6)).astype(np.float32, copy=False)
+        off = None if offdiag is None else col(np.asarray(offdiag, dtype=np.float32).reshape(-1, 6, 6)).astype(
+            np.float32, copy=False)
         b = np.ascontiguousarray(np.asarray(rhs, dtype=np.float64).reshape(n, 6))
         sysv = abi.ReducedSystem()
         sysv.structure_id = structure_id
@@ -447,7 +516,7 @@ class Device:
         sysv.num_edges = len(edges)
         sysv.diag = d.ctypes.data_as(abi.P64)
         sysv.edges = edges.ctypes.data_as(C.POINTER(C.c_uint32))
-        sysv.offdiag = off.ctypes.data_as(C.POINTER(C.c_float))
+        sysv.offdiag = None if off is None else off.ctypes.data_as(C.POINTER(C.c_float))
         sysv.rhs = b.ctypes.data_as(abi.P64)
         sysv.tol = tol
         sysv.max_iterations = max_iterations
@@ -484,6 +553,28 @@ class Device:
         if st != abi.OK:
             raise RuntimeError(f"vba_reduced_solve: {st} {self.last_error()}")
         return step, int(its.value)
+
+    def reduced_eliminate(self, elim: "PointEliminationInput"):
+        """vba_reduced_eliminate, `ReducedSolveAccelerator::eliminate`
+        (pipelines/slam/src/ba_accel.rs:150-156): returns (run_blocks
+        (len(run_edges), 6, 6), coarse_sums (m, m, 6, 6) or None), row-major
+        6x6 here, or None where the trait returns None. The blocks stay on
+        the device for the next reduced_solve(offdiag=None) of the
+        structure."""
+        view, keep = elim.view()
+        runs = np.zeros((max(1, len(elim.run_edges)), 36))
+        m = elim.coarse_aggregates()
+        coarse = np.zeros((max(1, m * m), 36))
+        st = self.lib.vba_reduced_eliminate(self.ctx, C.byref(view), runs.ctypes.data_as(abi.P64),
+                                            coarse.ctypes.data_as(abi.P64))
+        del keep
+        if st == abi.DECLINED:
+            return None
+        if st != abi.OK:
+            raise RuntimeError(f"vba_reduced_eliminate: {st} {self.last_error()}")
+        rb = np.swapaxes(runs[:len(elim.run_edges)].reshape(-1, 6, 6), -1, -2).copy()
+        cs = np.swapaxes(coarse[:m * m].reshape(m, m, 6, 6), -1, -2).copy() if m else None
+        return rb, cs
 
     def optimize_batch(self, problems: List[BundleProblem], options: abi.Options,
                        cap: int = 64) -> "BatchOutput":
