@@ -95,6 +95,10 @@ enum vba_robust_kernel {
  * assembled block-sparse Schur matrix. */
 enum vba_linear_solver {
     VBA_SOLVER_DENSE = 0,
+    /* LinearSolver::Sparse (normal_equations.rs:1483-1535): the reduced
+     * system by Cholesky only, a failed factor is SingularSystem (no LU
+     * fallback); factored densely on the device */
+    VBA_SOLVER_SPARSE = 1,
     VBA_SOLVER_MATRIX_FREE = 2,
     VBA_SOLVER_PCG_EXPLICIT = 3
 };
@@ -518,7 +522,10 @@ typedef struct vba_tuning {
     uint32_t pcg_chunk;
     uint32_t pcg_lookahead;
     uint32_t pcg_predict;
-    uint32_t reserved;
+    /* 0: the PCG product launches one wave per landmark tile; n (1..32): a
+     * persistent product of n waves per compute unit looping over the tiles
+     * (the same results bit for bit) */
+    uint32_t apply_waves_per_cu;
 } vba_tuning;
 /* NULL restores the defaults. VBA_ERR_INVALID_ARGUMENT for a field out of
  * range. */

@@ -1186,13 +1186,55 @@ M3 damped3(const M3& h, double lambda) {
 
 // nalgebra Cholesky::new on a dense n x n row-major matrix (column-oriented,
 // left-looking). Returns false for a non-positive or non-finite pivot.
-bool cholesky_dense(std::vector<double>& m, size_t n) {
+// nalgebra's column Cholesky: column j gets -L(j, k) L(i, k) added for every
+// k < j in ascending order, then is scaled by the square root of its
+// diagonal. Each entry (i, j) is accumulated on its own, row by row (the same
+// operations in the same order as the column sweep, read along the rows);
+// threads > 1 splits the rows of a column.
+// Columns are taken in blocks of kCholBlock: first every entry (i, j) of the
+// block (i >= j) gets the terms k below the block, k ascending, read against
+// a contiguous copy of the block's rows; then the block's columns finish one
+// by one with the terms k inside the block. Each entry sees exactly the
+// column sweep's sequence of operations.
+constexpr size_t kCholBlock = 64;
+bool cholesky_dense(std::vector<double>& m, size_t n, int threads = 1) {
     auto at = [&](size_t r, size_t c) -> double& { return m[r * n + c]; };
+    std::vector<double> panel;
     for (size_t j = 0; j < n; ++j) {
-        for (size_t k = 0; k < j; ++k) {
-            double factor = -at(j, k);
-            for (size_t i = j; i < n; ++i) at(i, j) = factor * at(i, k) + at(i, j);
+        if (j % kCholBlock == 0 && j > 0) {
+            const size_t j0 = j, j1 = std::min(n, j0 + kCholBlock), B = j1 - j0;
+            panel.resize(j0 * B);  // panel[k * B + jj] = L(j0 + jj, k), k < j0
+            for (size_t jj = 0; jj < B; ++jj)
+                for (size_t k = 0; k < j0; ++k) panel[k * B + jj] = m[(j0 + jj) * n + k];
+            auto rows = [&](size_t i0, size_t i1) {
+                double acc[kCholBlock];
+                for (size_t i = j0 + i0; i < j0 + i1; ++i) {
+                    double* ri = m.data() + i * n;
+                    const size_t cols = std::min(B, i - j0 + 1);  // entries j <= i
+                    for (size_t jj = 0; jj < cols; ++jj) acc[jj] = ri[j0 + jj];
+                    for (size_t k = 0; k < j0; ++k) {
+                        const double aik = ri[k];
+                        const double* pk = panel.data() + k * B;
+                        for (size_t jj = 0; jj < cols; ++jj) acc[jj] = -pk[jj] * aik + acc[jj];
+                    }
+                    for (size_t jj = 0; jj < cols; ++jj) ri[j0 + jj] = acc[jj];
+                }
+            };
+            if (threads > 1 && (n - j0) * j0 >= (1u << 14)) parallel_ranges(threads, n - j0, rows);
+            else rows(0, n - j0);
         }
+        const size_t kb = j - j % kCholBlock;  // terms k in [kb, j) are still to come
+        auto rows = [&](size_t i0, size_t i1) {
+            const double* rj = m.data() + j * n;
+            for (size_t i = j + i0; i < j + i1; ++i) {
+                const double* ri = m.data() + i * n;
+                double v = ri[j];
+                for (size_t k = kb; k < j; ++k) v = -rj[k] * ri[k] + v;
+                m[i * n + j] = v;
+            }
+        };
+        if (threads > 1 && (n - j) * (j - kb) >= (1u << 13)) parallel_ranges(threads, n - j, rows);
+        else rows(0, n - j);
         double diag = at(j, j);
         if (diag != 0.0 && diag >= 0.0) {
             double denom = std::sqrt(diag);
@@ -1271,14 +1313,18 @@ bool lu_solve(std::vector<double> m, size_t n, std::vector<double>& b) {
     return true;
 }
 // solve_normal_equations (pose_graph.rs:2259-2270).
+// allow_lu = false: LinearSolver::Sparse, whose block Cholesky has no
+// fallback (normal_equations.rs:1483-1535; its factor equals the dense one
+// to rounding, tests/bundle_adjustment.rs:1160-1200).
 bool solve_normal_equations(const std::vector<double>& A, size_t n, const std::vector<double>& b,
-                            std::vector<double>* x) {
+                            std::vector<double>* x, int threads = 1, bool allow_lu = true) {
     std::vector<double> L = A;
-    if (cholesky_dense(L, n)) {
+    if (cholesky_dense(L, n, threads)) {
         *x = b;
         cholesky_solve(L, n, *x);
         return true;
     }
+    if (!allow_lu) return false;
     *x = b;
     return lu_solve(A, n, *x);
 }
@@ -1304,7 +1350,7 @@ bool cholesky_inverse6(const M66& block, M66* inv) {
 // range visiting the landmarks in order, so every entry of S and b_S takes
 // the serial sequence of updates.
 int32_t solve_step_dense(Normal& ne, uint32_t L, double lambda, std::vector<double>* dp,
-                         std::vector<double>* dl, int threads = 1) {
+                         std::vector<double>* dl, int threads = 1, bool allow_lu = true) {
     const uint32_t P = ne.P;
     if (P == 0) {  // landmark-only branch (:1332-1371)
         dl->assign((size_t)3 * L, 0.0);
@@ -1356,7 +1402,7 @@ int32_t solve_step_dense(Normal& ne, uint32_t L, double lambda, std::vector<doub
             }
         }
     });
-    if (!solve_normal_equations(S, dim, b, dp)) return VBA_ERR_SINGULAR_SYSTEM;
+    if (!solve_normal_equations(S, dim, b, dp, threads, allow_lu)) return VBA_ERR_SINGULAR_SYSTEM;
     dl->assign((size_t)3 * L, 0.0);
     parallel_ranges(threads, L, [&](size_t l0, size_t l1) {
         for (size_t l = l0; l < l1; ++l) {
@@ -1883,7 +1929,8 @@ int32_t run_lm(Problem& P, const vba_options& o, int threads, vba_iteration_stat
         st.iteration = iteration;
         st.pcg_iterations = -1;
         if (!matrix_free) {
-            solve_status = solve_step_dense(ne, P.L, lambda, &dp, &dl, threads);
+            solve_status = solve_step_dense(ne, P.L, lambda, &dp, &dl, threads,
+                                            o.linear_solver != VBA_SOLVER_SPARSE);
         } else {
             Implicit op;
             int e = build_implicit(ne, lambda, &op, threads);
@@ -2875,6 +2922,13 @@ int32_t oracle_ns_solve_pcg(const oracle_normal_system* ns, double lambda, uint3
     std::copy(out.x.begin(), out.x.end(), delta_p);
     std::copy(dl.begin(), dl.end(), delta_l);
     return 0;
+}
+
+int32_t oracle_cholesky(uint32_t n, const double* a, double* l, int32_t threads) {
+    std::vector<double> L(a, a + (size_t)n * n);
+    if (!cholesky_dense(L, n, threads)) return 0;
+    std::copy(L.begin(), L.end(), l);
+    return 1;
 }
 
 int32_t oracle_solve_normal_equations(uint32_t n, const double* a, const double* b, double* x) {

@@ -135,6 +135,9 @@ int32_t oracle_ns_solve_pcg(const oracle_normal_system* ns, double lambda,
 /* solve_normal_equations (pose_graph.rs:2259-2270): Cholesky, LU fallback.
  * A is n x n row-major. Returns 1 on success. */
 int32_t oracle_solve_normal_equations(uint32_t n, const double* a, const double* b, double* x);
+/* nalgebra's dense Cholesky (row-major in and out; the lower triangle of l is
+ * the factor, the rest is what the factor leaves there); 1 on success. */
+int32_t oracle_cholesky(uint32_t n, const double* a, double* l, int32_t threads);
 
 /* ReducedSolveAccelerator::eliminate's blocks as the CPU rig solver forms
  * them (eliminate_by_output, rig_ba_solver.rs:2551-2660): blocks[36 E] the

@@ -82,6 +82,8 @@ def library():
                                         C.POINTER(C.c_uint32), P64, P64]
     lib.oracle_reduced_eliminate.restype = C.c_int32
     lib.oracle_reduced_eliminate.argtypes = [C.POINTER(abi.PointElimination), P64, P64, P64]
+    lib.oracle_cholesky.restype = C.c_int32
+    lib.oracle_cholesky.argtypes = [C.c_uint32, P64, P64, C.c_int32]
     lib.oracle_solve_normal_equations.restype = C.c_int32
     lib.oracle_solve_normal_equations.argtypes = [C.c_uint32, P64, P64, P64]
     _lib = lib
@@ -374,3 +376,13 @@ def reduced_eliminate(elim):
     rm = lambda a: np.swapaxes(a.reshape(-1, 6, 6), -1, -2).copy()
     return (rm(blocks[:E]), rm(runs[:len(keep["run_edges"])]),
             np.swapaxes(coarse[:m * m].reshape(m, m, 6, 6), -1, -2).copy() if m else None)
+
+
+def cholesky(a, threads: int = 1):
+    """oracle_cholesky: the factor L (lower triangle) of a row-major SPD
+    matrix, or None when the factor fails."""
+    a = np.ascontiguousarray(a, np.float64)
+    n = a.shape[0]
+    out = np.zeros_like(a)
+    ok = library().oracle_cholesky(n, _p(a), _p(out), threads)
+    return np.tril(out) if ok else None

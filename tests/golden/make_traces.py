@@ -18,6 +18,8 @@ Traces:
   windows       the 256 local stereo windows, dense, Cauchy 1 px, 10 iterations
   city500       a 500-camera sub-problem of the city scene (per-camera OPENCV,
                 10 % outliers, Huber 2 px), matrix-free FP64, 25 iterations
+  dense1500     a 1,500-camera loop (250 m, 60k landmarks), Huber 1 px, the
+                dense reduced solve (9,000 unknowns), 8 iterations
 
 Usage: python tests/golden/make_traces.py [--threads N] [name ...]
 """
@@ -106,7 +108,8 @@ def windows(threads):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--threads", type=int, default=os.cpu_count() or 1)
-    ap.add_argument("names", nargs="*", default=["circle", "windows", "city500", "loop_accept", "loop_reject"])
+    ap.add_argument("names", nargs="*",
+                    default=["circle", "windows", "city500", "dense1500", "loop_accept", "loop_reject"])
     a = ap.parse_args()
     os.makedirs(OUT, exist_ok=True)
     for name in a.names:
@@ -116,6 +119,8 @@ def main():
             windows(a.threads)
         elif name == "city500":
             single(name, workloads.city_problem(500), workloads.city_options(), a.threads)
+        elif name == "dense1500":
+            single(name, workloads.dense_loop_problem(), workloads.dense_loop_options(), a.threads)
         elif name in ("loop_accept", "loop_reject"):
             pol = abi.PCG_MAXIT_ACCEPT_ITERATE if name == "loop_accept" else abi.PCG_MAXIT_REJECT
             single(name, workloads.loop_problem(), workloads.loop_options(policy=pol), a.threads)

@@ -75,10 +75,11 @@ def test_check_declines_out_of_range_option_enums():
     """Every enum of vba_options out of its range declines (the options half
     of the support check, which the prepared solves also apply)."""
     ba = scenes.truth_bundle()
-    for field, bad in (("robust_kernel", 3), ("linear_solver", 1), ("linear_solver", 4), ("precision", 2),
+    for field, bad in (("robust_kernel", 3), ("linear_solver", 4), ("linear_solver", 7), ("precision", 2),
                        ("pcg_maxit_policy", 2)):
         assert vba.check(ba, vba.default_options(**{field: bad})) == abi.DECLINED, field
-    for field, good in (("linear_solver", abi.SOLVER_MATRIX_FREE), ("linear_solver", abi.SOLVER_PCG_EXPLICIT),
+    for field, good in (("linear_solver", abi.SOLVER_SPARSE), ("linear_solver", abi.SOLVER_MATRIX_FREE),
+                        ("linear_solver", abi.SOLVER_PCG_EXPLICIT),
                         ("precision", abi.PRECISION_MIXED), ("pcg_maxit_policy", abi.PCG_MAXIT_ACCEPT_ITERATE)):
         assert vba.check(ba, vba.default_options(**{field: good})) == abi.OK, field
 

@@ -73,6 +73,38 @@ def test_dense_solve_lu_reads_the_whole_matrix(device):
     assert np.abs(x - np.linalg.solve(h, b)).max() <= 1e-10 * np.abs(x).max()
 
 
+@pytest.mark.parametrize("n", [257, 700, 1500])
+def test_dense_solve_device_wide_cholesky(device, n):
+    """Beyond 256 unknowns the factor runs across the device (64-column
+    panels, MFMA trailing updates)."""
+    h, b = spd(n, n)
+    st, x = device.dense_solve(h, b)
+    ref = oracle.solve_normal_equations(h, b)
+    assert st == abi.OK and ref is not None
+    assert np.abs(x - ref).max() <= SOLVE_RTOL * np.abs(ref).max()
+    assert np.abs(h @ x - b).max() <= 1e-9 * np.abs(b).max()
+
+
+@pytest.mark.parametrize("n", [300, 900])
+def test_dense_solve_device_wide_lu_fallback(device, n):
+    """The device-wide factor meets a negative pivot; the LU fallback runs on
+    the copy saved before it."""
+    h, b = indefinite(n, 300 + n)
+    st, x = device.dense_solve(h, b)
+    ref = oracle.solve_normal_equations(h, b)
+    assert st == abi.OK and ref is not None
+    assert np.abs(x - ref).max() <= SOLVE_RTOL * np.abs(ref).max()
+
+
+def test_dense_solve_device_wide_singular(device):
+    h, b = spd(400, 4)
+    h[123, :] = 0.0
+    h[:, 123] = 0.0
+    assert oracle.solve_normal_equations(h, b) is None
+    st, x = device.dense_solve(h, b)
+    assert st == abi.ERR_SINGULAR_SYSTEM and x is None
+
+
 def test_dense_solve_singular(device):
     h, b = spd(24, 3)
     h[7, :] = 0.0
@@ -125,3 +157,43 @@ def test_singular_window_in_a_batch(device):
     assert w0.result.final_cost == alone.result.final_cost
     assert w1.accepted == [False] * 6
     assert w1.result.final_cost == w1.result.initial_cost
+
+
+def dense_loop(cams, lms, seed):
+    return vba.synthesize(abi.SYNTH_LOOP, num_cameras=cams, num_landmarks=lms, seed=seed,
+                          scene_scale=200.0 * cams / 1200.0)
+
+
+@pytest.mark.parametrize("solver", ["dense", "sparse"])
+def test_lm_with_the_device_wide_factor(device, solver):
+    """A 200-camera loop (1,194 unknowns in the reduced system) through the
+    dense LM, whose factor now runs across the device; the Sparse solver
+    (Cholesky only) takes the same steps while the factor succeeds."""
+    s = dense_loop(200, 12000, 31)
+    ls = abi.SOLVER_DENSE if solver == "dense" else abi.SOLVER_SPARSE
+    o = vba.default_options(linear_solver=ls, max_iterations=6, robust_kernel=abi.KERNEL_HUBER, robust_scale=1.0)
+    ref = oracle.optimize(s.problem, o, threads=8)
+    gpu = device.optimize(s.problem, o)
+    assert ref.status == gpu.status == abi.OK, gpu.message
+    assert gpu.accepted == ref.accepted
+    rel = abs(gpu.result.final_cost - ref.result.final_cost) / ref.result.final_cost
+    assert rel <= 1e-9, rel
+    d = np.linalg.norm(vba.camera_centres(gpu.poses) - vba.camera_centres(ref.poses), axis=1).max()
+    assert d <= 1e-6 * s.scene_extent
+    if solver == "sparse":
+        dense = device.optimize(s.problem, vba.default_options(
+            linear_solver=abi.SOLVER_DENSE, max_iterations=6, robust_kernel=abi.KERNEL_HUBER, robust_scale=1.0))
+        assert dense.result.final_cost == gpu.result.final_cost
+
+
+def test_sparse_solver_has_no_lu_fallback(device):
+    """LinearSolver::Sparse (normal_equations.rs:1483-1535): a failed block
+    Cholesky is SingularSystem, which the LM takes as a rejected step; the
+    singular problem is rejected step after step as with Dense, where LU
+    fails too."""
+    ba = singular_bundle()
+    o = vba.default_options(has_initial_lambda=0, max_iterations=4, linear_solver=abi.SOLVER_SPARSE)
+    ref = oracle.optimize(ba, o)
+    gpu = device.optimize(ba, o)
+    assert ref.status == gpu.status == abi.OK
+    assert ref.accepted == gpu.accepted == [False] * 4

@@ -16,6 +16,8 @@ landmarks' bounding box). Each test prints the levels it actually reached.
   (300-8,000), solved as one device batch.
 - city500: 500 cameras of the city scene with per-camera OPENCV intrinsics,
   10 % outliers, Huber 2 px, matrix-free, 25 iterations, FP64 and mixed.
+- dense1500: a 1,500-camera loop through the dense reduced solve (9,000
+  unknowns), 8 iterations.
 
 A problem is regenerated from its seed and must hash to the digest stored
 with its trace, so generator drift fails here instead of comparing
@@ -111,6 +113,16 @@ def test_city_subproblem(device, precision):
     prec = abi.PRECISION_FP64 if precision == "fp64" else abi.PRECISION_MIXED
     out = device.optimize(s.problem, workloads.city_options(prec))
     check_single(f"city500 {precision}", out, tr, s.scene_extent)
+
+
+def test_dense_1500_cameras(device):
+    """The dense reduced solve at 9,000 unknowns (a 1,500-camera loop),
+    factored across the device."""
+    tr = load("dense1500")
+    s = workloads.dense_loop_problem()
+    regenerate(s, tr)
+    out = device.optimize(s.problem, workloads.dense_loop_options())
+    check_single("dense 1500 cameras", out, tr, s.scene_extent)
 
 
 def test_all_windows_full_size_in_one_batch(device):

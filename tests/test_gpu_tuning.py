@@ -10,7 +10,10 @@ from vba import abi
 
 pytestmark = pytest.mark.gpu
 
-SCHEDULES = [(10, 0, False), (1, 0, False), (3, 2, True), (7, 1, False), (25, 0, True), (4, 2, False)]
+# (chunk, lookahead, predict, persistent product waves per CU); the persistent
+# product at several residencies, including more waves than resident
+SCHEDULES = [(10, 0, False, 0), (1, 0, False, 0), (3, 2, True, 0), (7, 1, False, 0), (25, 0, True, 0),
+             (4, 2, False, 0), (10, 0, False, 16), (3, 2, True, 4), (7, 1, False, 1), (20, 1, True, 32)]
 
 
 def loop_options(solver, precision, max_pcg=100):
@@ -21,8 +24,8 @@ def loop_options(solver, precision, max_pcg=100):
 
 def solve_all(device, pp, o):
     outs = []
-    for chunk, ahead, predict in SCHEDULES:
-        device.set_tuning(chunk, ahead, predict)
+    for chunk, ahead, predict, waves in SCHEDULES:
+        device.set_tuning(chunk, ahead, predict, waves)
         out = pp.solve(o)
         assert out.status == abi.OK, out.message
         outs.append((out, pp.state()))
@@ -73,7 +76,9 @@ def test_tuning_rejects_out_of_range(device):
         device.set_tuning(0, 0, False)
     with pytest.raises(ValueError):
         device.set_tuning(10, 3, False)
-    assert device.tuning() == {"pcg_chunk": 10, "pcg_lookahead": 0, "pcg_predict": 0}
+    with pytest.raises(ValueError):
+        device.set_tuning(10, 0, False, 33)
+    assert device.tuning() == {"pcg_chunk": 10, "pcg_lookahead": 0, "pcg_predict": 0, "apply_waves_per_cu": 0}
 
 
 def test_prepared_solves_decline_out_of_range_options(device):
@@ -83,7 +88,7 @@ def test_prepared_solves_decline_out_of_range_options(device):
     s = vba.synthesize(abi.SYNTH_LOOP, num_cameras=20, num_landmarks=600, seed=5, scene_scale=12.0)
     o = loop_options(abi.SOLVER_MATRIX_FREE, abi.PRECISION_FP64)
     pp = device.prepare(s.problem, o)
-    bad = [dict(linear_solver=1), dict(linear_solver=9), dict(robust_kernel=5), dict(precision=3),
+    bad = [dict(linear_solver=4), dict(linear_solver=9), dict(robust_kernel=5), dict(precision=3),
            dict(pcg_maxit_policy=4), dict(refine_intrinsics=1)]
     for fields in bad:
         ob = loop_options(abi.SOLVER_MATRIX_FREE, abi.PRECISION_FP64)

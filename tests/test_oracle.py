@@ -1,6 +1,8 @@
 """Pins the CPU FP64 oracle (oracle/ba_oracle.cpp) before anything is checked
 against it: the reference's known-answer tests, closed forms, finite
 differences and a brute-force dense solve of the full normal system."""
+import math
+
 import numpy as np
 import pytest
 
@@ -506,3 +508,40 @@ def test_solve_normal_equations_cholesky_lu_and_singular():
     hs[4, :] = 0.0
     hs[:, 4] = 0.0
     assert oracle.solve_normal_equations(hs, b) is None
+
+
+def column_sweep_cholesky(a):
+    """nalgebra's Cholesky::new as a plain column sweep, in Python floats
+    (IEEE doubles, every product and sum rounded on its own): column j gets
+    -L(j, k) L(i, k) added for k ascending, then its diagonal's square root
+    divides it."""
+    n = len(a)
+    m = [list(map(float, row)) for row in a]
+    for j in range(n):
+        for k in range(j):
+            f = -m[j][k]
+            for i in range(j, n):
+                m[i][j] = f * m[i][k] + m[i][j]
+        d = m[j][j]
+        if not (d != 0.0 and d >= 0.0):
+            return None
+        r = math.sqrt(d)
+        m[j][j] = r
+        for i in range(j + 1, n):
+            m[i][j] = m[i][j] / r
+    return np.tril(np.array(m))
+
+
+@pytest.mark.parametrize("n,threads", [(5, 1), (63, 1), (64, 1), (65, 1), (150, 1), (300, 4)])
+def test_blocked_cholesky_is_the_column_sweep(n, threads):
+    """The oracle's blocked, threaded Cholesky does each entry's operations
+    in the column sweep's order: the factors are bitwise equal."""
+    rng = np.random.default_rng(n)
+    a = rng.standard_normal((n, n))
+    h = a @ a.T + n * np.eye(n)
+    ref = column_sweep_cholesky(h)
+    got = oracle.cholesky(h, threads)
+    assert np.array_equal(got, ref)
+    bad = h.copy()
+    bad[n // 2, n // 2] = -1.0
+    assert oracle.cholesky(bad, threads) is None

@@ -48,7 +48,7 @@ int32_t build_prepared(vba_context* ctx, const vba_problem& problem, const vba_o
         ctx->last_error = msg.empty() ? "invalid problem" : msg;
         return st;
     }
-    if (options.linear_solver != VBA_SOLVER_DENSE) {
+    if (!vba::dense_family(options.linear_solver)) {
         st = vba::validate_matrix_free(prep->m, options, &msg);
         if (st != VBA_OK) {
             ctx->last_error = msg;
@@ -405,7 +405,7 @@ int32_t vba_solve_prepared(vba_context* ctx, vba_prepared* prepared, const vba_o
         ctx->last_error = "declined: " + msg;
         return VBA_DECLINED;
     }
-    if (options->linear_solver != VBA_SOLVER_DENSE) {
+    if (!vba::dense_family(options->linear_solver)) {
         int32_t st = vba::validate_matrix_free(prepared->m, *options, &msg);
         if (st != VBA_OK) {
             ctx->last_error = msg;
@@ -687,14 +687,15 @@ int32_t vba_set_tuning(vba_context* ctx, const vba_tuning* tuning) {
     if (!ctx) return VBA_ERR_INVALID_ARGUMENT;
     vba_tuning t{10, 0, 0, 0};
     if (tuning) {
-        if (tuning->pcg_chunk < 1 || tuning->pcg_chunk > 1000 || tuning->pcg_lookahead > 2) {
+        if (tuning->pcg_chunk < 1 || tuning->pcg_chunk > 1000 || tuning->pcg_lookahead > 2 ||
+            tuning->apply_waves_per_cu > 32) {
             std::lock_guard<std::mutex> lock(ctx->mu);
-            ctx->last_error = "vba_set_tuning: pcg_chunk must be 1..1000 and pcg_lookahead 0..2";
+            ctx->last_error =
+                "vba_set_tuning: pcg_chunk must be 1..1000, pcg_lookahead 0..2 and apply_waves_per_cu 0..32";
             return VBA_ERR_INVALID_ARGUMENT;
         }
         t = *tuning;
         t.pcg_predict = t.pcg_predict ? 1u : 0u;
-        t.reserved = 0;
     }
     std::lock_guard<std::mutex> lock(ctx->mu);
     ctx->tuning = t;
@@ -721,16 +722,19 @@ int32_t vba_autotune(vba_context* ctx, vba_prepared* prepared, const vba_options
             return VBA_DECLINED;
         }
     }
-    if (options->linear_solver == VBA_SOLVER_DENSE) {
+    if (vba::dense_family(options->linear_solver)) {
         std::lock_guard<std::mutex> lock(ctx->mu);
         ctx->last_error = "vba_autotune: the schedule applies to the PCG solvers";
         return VBA_ERR_INVALID_ARGUMENT;
     }
     vba_options o = *options;
     if (lm_iterations) o.max_iterations = lm_iterations;
+    // launch schedules, each with the one-wave-per-tile product and with the
+    // persistent one at its residency (16 waves per compute unit)
     static const vba_tuning kCandidates[] = {
-        {10, 0, 0, 0}, {10, 1, 0, 0}, {10, 1, 1, 0}, {5, 1, 1, 0}, {5, 2, 1, 0},
-        {20, 1, 1, 0}, {20, 0, 1, 0}, {4, 2, 1, 0},
+        {10, 0, 0, 0},  {10, 1, 0, 0},  {10, 1, 1, 0},  {5, 1, 1, 0},   {5, 2, 1, 0},  {20, 1, 1, 0},
+        {20, 0, 1, 0},  {4, 2, 1, 0},   {10, 0, 0, 16}, {10, 1, 1, 16}, {5, 1, 1, 16}, {20, 1, 1, 16},
+        {20, 0, 1, 16}, {4, 2, 1, 16},
     };
     const uint32_t cap = o.max_iterations ? o.max_iterations : 1;
     std::vector<vba_iteration_stats> ref_stats(cap), stats(cap);

@@ -657,23 +657,100 @@ struct DenseRows {
         else tail_x[i - nv] = v;
     }
 };
-__device__ __forceinline__ DenseRows pose_rows(const DevProblem& P, const int32_t* var_slot, int n) {
+__host__ __device__ __forceinline__ DenseRows pose_rows(const DevProblem& P, const int32_t* var_slot, int n) {
     // the last joint_k unknowns are the camera block of the joint solve
     return DenseRows{P.b, P.x, var_slot, n - P.joint_k, P.joint_rhs, P.joint_delta};
 }
 
+// LU with partial pivoting of the n x n row-major A2 (overwritten), solving
+// A2 y = y in place; *fail_sh = 2 when A2 is singular (a zero pivot column
+// survives to the back substitution). One work-group of NT threads.
+template <int NT>
+__device__ void lu_fallback_body(double* A2, double* y, int n, int* fail_sh, double* piv_val, int* piv_idx) {
+    for (int k = 0; k < n; ++k) {
+        double best = -1.0;
+        int bi = k;
+        for (int i = k + threadIdx.x; i < n; i += NT) {
+            const double v = fabs(A2[(size_t)i * n + k]);
+            if (v > best) {
+                best = v;
+                bi = i;
+            }
+        }
+        piv_val[threadIdx.x] = best;
+        piv_idx[threadIdx.x] = bi;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double bv = piv_val[0];
+            int bidx = piv_idx[0];
+            for (int t = 1; t < NT; ++t)
+                if (piv_val[t] > bv || (piv_val[t] == bv && piv_idx[t] < bidx)) {
+                    bv = piv_val[t];
+                    bidx = piv_idx[t];
+                }
+            piv_idx[0] = bidx;
+            piv_val[0] = bv;
+        }
+        __syncthreads();
+        const int pr = piv_idx[0];
+        const double pv = A2[(size_t)pr * n + k];
+        __syncthreads();
+        if (pv == 0.0) continue;
+        if (pr != k) {
+            for (int c = threadIdx.x; c < n; c += NT) {
+                const double t = A2[(size_t)k * n + c];
+                A2[(size_t)k * n + c] = A2[(size_t)pr * n + c];
+                A2[(size_t)pr * n + c] = t;
+            }
+            if (threadIdx.x == 0) {
+                const double t = y[k];
+                y[k] = y[pr];
+                y[pr] = t;
+            }
+        }
+        __syncthreads();
+        const double inv = 1.0 / A2[(size_t)k * n + k];
+        for (int i = k + 1 + threadIdx.x; i < n; i += NT) A2[(size_t)i * n + k] *= inv;
+        __syncthreads();
+        const int rows = n - k - 1;
+        for (int idx = threadIdx.x; idx < rows * rows; idx += NT) {
+            const int i = k + 1 + idx / rows, c = k + 1 + idx % rows;
+            A2[(size_t)i * n + c] -= A2[(size_t)i * n + k] * A2[(size_t)k * n + c];
+        }
+        if (threadIdx.x == 0) {
+            const double yk = y[k];
+            for (int i = k + 1; i < n; ++i) y[i] -= A2[(size_t)i * n + k] * yk;
+        }
+        __syncthreads();
+    }
+    for (int j = n - 1; j >= 0; --j) {
+        const double dj = A2[(size_t)j * n + j];
+        if (dj == 0.0) {
+            if (threadIdx.x == 0) *fail_sh = 2;
+            break;
+        }
+        if (threadIdx.x == 0) y[j] = y[j] / dj;
+        __syncthreads();
+        const double xj = y[j];
+        for (int i = threadIdx.x; i < j; i += NT) y[i] -= A2[(size_t)i * n + j] * xj;
+        __syncthreads();
+    }
+}
+
 // Cholesky (blocked, MFMA trailing updates), then forward/back
 // substitution; on a non-positive pivot an LU with partial pivoting on the
-// saved copy. Solves S x = b for the n unknowns of `io`. One work-group of
-// NT threads.
+// saved copy (allow_lu, solve_normal_equations), or singular (the Sparse
+// solver's block Cholesky has no fallback). Solves S x = b for the n
+// unknowns of `io`. One work-group of NT threads.
 template <int NT>
 __device__ void dense_solve_body(const DenseRows& io, double* S, double* work, int n, int* status, int* fail_sh,
-                                 double* piv_val, int* piv_idx, double* dblk, bool copied = false) {
+                                 double* piv_val, int* piv_idx, double* dblk, bool copied = false,
+                                 bool allow_lu = true) {
     double* A2 = work;                 // n*n copy for the LU fallback
     double* y = work + (size_t)n * n;  // n
     // the saved copy for the LU fallback, unless the whole device made it
     // before this work-group started
-    if (!copied)
+    if (!copied && allow_lu)
         for (size_t k = threadIdx.x; k < (size_t)n * n; k += NT) A2[k] = S[k];
     for (int i = threadIdx.x; i < n; i += NT) y[i] = io.rhs(i);
     if (threadIdx.x == 0) *fail_sh = 0;
@@ -682,78 +759,13 @@ __device__ void dense_solve_body(const DenseRows& io, double* S, double* work, i
     __syncthreads();
     if (!*fail_sh) {
         chol_blocked_solve<NT>(S, n, y);
-    } else {
+    } else if (allow_lu) {
         // LU with partial pivoting on A2; y holds b again.
         for (int i = threadIdx.x; i < n; i += NT) y[i] = io.rhs(i);
         __syncthreads();
-        for (int k = 0; k < n; ++k) {
-            double best = -1.0;
-            int bi = k;
-            for (int i = k + threadIdx.x; i < n; i += NT) {
-                const double v = fabs(A2[(size_t)i * n + k]);
-                if (v > best) {
-                    best = v;
-                    bi = i;
-                }
-            }
-            piv_val[threadIdx.x] = best;
-            piv_idx[threadIdx.x] = bi;
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                double bv = piv_val[0];
-                int bidx = piv_idx[0];
-                for (int t = 1; t < NT; ++t)
-                    if (piv_val[t] > bv || (piv_val[t] == bv && piv_idx[t] < bidx)) {
-                        bv = piv_val[t];
-                        bidx = piv_idx[t];
-                    }
-                piv_idx[0] = bidx;
-                piv_val[0] = bv;
-            }
-            __syncthreads();
-            const int pr = piv_idx[0];
-            const double pv = A2[(size_t)pr * n + k];
-            __syncthreads();
-            if (pv == 0.0) continue;
-            if (pr != k) {
-                for (int c = threadIdx.x; c < n; c += NT) {
-                    const double t = A2[(size_t)k * n + c];
-                    A2[(size_t)k * n + c] = A2[(size_t)pr * n + c];
-                    A2[(size_t)pr * n + c] = t;
-                }
-                if (threadIdx.x == 0) {
-                    const double t = y[k];
-                    y[k] = y[pr];
-                    y[pr] = t;
-                }
-            }
-            __syncthreads();
-            const double inv = 1.0 / A2[(size_t)k * n + k];
-            for (int i = k + 1 + threadIdx.x; i < n; i += NT) A2[(size_t)i * n + k] *= inv;
-            __syncthreads();
-            const int rows = n - k - 1;
-            for (int idx = threadIdx.x; idx < rows * rows; idx += NT) {
-                const int i = k + 1 + idx / rows, c = k + 1 + idx % rows;
-                A2[(size_t)i * n + c] -= A2[(size_t)i * n + k] * A2[(size_t)k * n + c];
-            }
-            if (threadIdx.x == 0) {
-                const double yk = y[k];
-                for (int i = k + 1; i < n; ++i) y[i] -= A2[(size_t)i * n + k] * yk;
-            }
-            __syncthreads();
-        }
-        for (int j = n - 1; j >= 0; --j) {
-            const double dj = A2[(size_t)j * n + j];
-            if (dj == 0.0) {
-                if (threadIdx.x == 0) *fail_sh = 2;
-                break;
-            }
-            if (threadIdx.x == 0) y[j] = y[j] / dj;
-            __syncthreads();
-            const double xj = y[j];
-            for (int i = threadIdx.x; i < j; i += NT) y[i] -= A2[(size_t)i * n + j] * xj;
-            __syncthreads();
-        }
+        lu_fallback_body<NT>(A2, y, n, fail_sh, piv_val, piv_idx);
+    } else {
+        if (threadIdx.x == 0) *fail_sh = 2;
     }
     __syncthreads();
     const int f = *fail_sh;
@@ -761,14 +773,132 @@ __device__ void dense_solve_body(const DenseRows& io, double* S, double* work, i
     if (threadIdx.x == 0) *status = (f == 2) ? 1 : 0;
 }
 
+// ---------------------------------------------------------------------------
+// The same solve across the whole device, for systems larger than one
+// work-group factors quickly: a right-looking Cholesky over 64-column
+// panels, two launches per panel. k_chol_panel: every work-group factors the
+// 64 x 64 diagonal block in LDS (the same arithmetic in each, so every one
+// sees the same pivots) and solves its 256 rows of the panel below,
+// L_R = A_R L_kk^-T; work-group 0 writes the diagonal block back.
+// k_chol_update: the trailing lower triangle A_ij -= L_i L_j^T in 16 x 16
+// tiles, one wave each, on FP64 MFMA. flags[1] records a failed pivot (set
+// by work-group 0; later launches exit at once); k_dense_finish then solves
+// with the factor, or runs the LU fallback on the saved copy, in one
+// work-group.
+// ---------------------------------------------------------------------------
+constexpr int kPanel = 64;
+
+__global__ void k_dense_flags_clear(int* flags) {
+    if (threadIdx.x < 2) flags[threadIdx.x] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_chol_panel(double* S, int n, int k0, int* flags) {
+    if (__hip_atomic_load(&flags[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    __shared__ double L[kPanel][kPanel + 1];
+    const int w = min(kPanel, n - k0);
+    for (int t = threadIdx.x; t < kPanel * kPanel; t += 256) {
+        const int r = t / kPanel, c = t % kPanel;
+        L[r][c] = (r < w && c < w) ? S[(size_t)(k0 + r) * n + k0 + c] : (r == c ? 1.0 : 0.0);
+    }
+    __syncthreads();
+    for (int j = 0; j < kPanel; ++j) {
+        const double d = L[j][j];
+        if (!(d > 0.0) || !isfinite(d)) {  // the same in every thread and work-group
+            if (blockIdx.x == 0 && threadIdx.x == 0)
+                __hip_atomic_store(&flags[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return;
+        }
+        const double ljj = sqrt(d);
+        __syncthreads();  // every thread has read L[j][j]
+        if (threadIdx.x == 0) L[j][j] = ljj;
+        for (int i = j + 1 + (int)threadIdx.x; i < kPanel; i += 256) L[i][j] = L[i][j] / ljj;
+        __syncthreads();
+        for (int t = threadIdx.x; t < kPanel * kPanel; t += 256) {
+            const int i = t / kPanel, c = t % kPanel;
+            if (c > j && c <= i) L[i][c] -= L[i][j] * L[c][j];
+        }
+        __syncthreads();
+    }
+    if (blockIdx.x == 0)
+        for (int t = threadIdx.x; t < kPanel * kPanel; t += 256) {
+            const int r = t / kPanel, c = t % kPanel;
+            if (c <= r && r < w) S[(size_t)(k0 + r) * n + k0 + c] = L[r][c];
+        }
+    const int R = k0 + kPanel + (int)(blockIdx.x * 256 + threadIdx.x);
+    if (R < n) {  // a panel row below exists only when the block is whole (w == 64)
+        double x[kPanel];
+        double* row = S + (size_t)R * n + k0;
+#pragma unroll
+        for (int c = 0; c < kPanel; ++c) x[c] = row[c];
+#pragma unroll
+        for (int c = 0; c < kPanel; ++c) {
+            double t = x[c];
+#pragma unroll
+            for (int jj = 0; jj < c; ++jj) t -= x[jj] * L[c][jj];
+            x[c] = t / L[c][c];
+        }
+#pragma unroll
+        for (int c = 0; c < kPanel; ++c) row[c] = x[c];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_chol_update(double* S, int n, int k0, const int* flags) {
+    if (__hip_atomic_load(&flags[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    const int k1 = k0 + kPanel;
+    const int m = (n - k1 + 15) >> 4;
+    const int lane = threadIdx.x & 63;
+    const long long bb = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (m <= 0 || bb >= (long long)m * (m + 1) / 2) return;
+    long long i = (long long)((sqrt(8.0 * (double)bb + 1.0) - 1.0) * 0.5);
+    while ((i + 1) * (i + 2) / 2 <= bb) ++i;
+    while (i * (i + 1) / 2 > bb) --i;
+    const int j = (int)(bb - i * (i + 1) / 2);
+    const int bi = k1 + 16 * (int)i, bj = k1 + 16 * j;
+    v4d acc;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] = padded_at(S, n, bi + 4 * q + (lane >> 4), bj + (lane & 15));
+#pragma unroll
+    for (int kk = 0; kk < kPanel / 4; ++kk) {
+        const double a = -padded_at(S, n, bi + (lane & 15), k0 + 4 * kk + (lane >> 4));
+        const double bv = padded_at(S, n, bj + (lane & 15), k0 + 4 * kk + (lane >> 4));
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int r = bi + 4 * q + (lane >> 4), c = bj + (lane & 15);
+        if (r < n && c < n) S[(size_t)r * n + c] = acc[q];
+    }
+}
+
+// After the device-wide factor: forward / back substitution with it, or the
+// LU fallback on the saved copy in work, or singular without a fallback.
+// flags[0] = 1 when singular.
+__global__ __launch_bounds__(kSmallBlock) void k_dense_finish(DenseRows io, double* S, double* work, int n,
+                                                              int* flags, int allow_lu) {
+    __shared__ int fail;
+    __shared__ double piv_val[kSmallBlock];
+    __shared__ int piv_idx[kSmallBlock];
+    double* y = work + (size_t)n * n;
+    for (int i = threadIdx.x; i < n; i += kSmallBlock) y[i] = io.rhs(i);
+    if (threadIdx.x == 0) fail = __hip_atomic_load(&flags[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    if (!fail) chol_blocked_solve<kSmallBlock>(S, n, y);
+    else if (allow_lu) lu_fallback_body<kSmallBlock>(work, y, n, &fail, piv_val, piv_idx);
+    else if (threadIdx.x == 0) fail = 2;
+    __syncthreads();
+    const int f = fail;
+    for (int i = threadIdx.x; i < n; i += kSmallBlock) io.put(i, (f != 2) ? y[i] : 0.0);
+    if (threadIdx.x == 0) flags[0] = (f == 2) ? 1 : 0;
+}
+
 __global__ __launch_bounds__(kSmallBlock) void k_dense_solve(DevProblem P, DensePairs D, double* S, double* work,
-                                                             int n, int* status) {
+                                                             int n, int* status, int allow_lu) {
     __shared__ int fail;
     __shared__ double piv_val[kSmallBlock];
     __shared__ int piv_idx[kSmallBlock];
     __shared__ double dblk[16 * 17];
     dense_solve_body<kSmallBlock>(pose_rows(P, D.var_slot, n), S, work, n, status, &fail, piv_val, piv_idx, dblk,
-                                  true);
+                                  true, allow_lu != 0);
 }
 
 // S x = b for a caller's row-major S (overwritten); work holds n * n + n.
@@ -1378,19 +1508,45 @@ void launch_dense_assemble(const DevProblem& P, const DensePairs& D, double* S, 
         hipLaunchKernelGGL(k_dense_assemble, dim3(blocks_for(D.num_blocks, kBlock / kBlockLanes)), dim3(kBlock), 0, s, P,
                            D, S, n, lambda);
 }
+namespace {
+// The device-wide factor and the finishing solve (n > kDenseOneGroupMax).
+hipError_t dense_solve_device_wide(const DenseRows& io, double* S, double* work, int32_t n, int32_t* status,
+                                   bool allow_lu, hipStream_t s) {
+    hipLaunchKernelGGL(k_dense_flags_clear, dim3(1), dim3(64), 0, s, status);
+    for (int k0 = 0; k0 < n; k0 += kPanel) {
+        const int below = n - k0 - kPanel;
+        hipLaunchKernelGGL(k_chol_panel, dim3(below > 0 ? (below + 255) / 256 : 1), dim3(256), 0, s, S, n, k0,
+                           status);
+        if (below > 0) {
+            const long long m = (below + 15) / 16, tiles = m * (m + 1) / 2;
+            hipLaunchKernelGGL(k_chol_update, dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0, s, S, n, k0,
+                               (const int*)status);
+        }
+    }
+    hipLaunchKernelGGL(k_dense_finish, dim3(1), dim3(kSmallBlock), 0, s, io, S, work, n, status, (int)allow_lu);
+    return hipGetLastError();
+}
+}  // namespace
+
 hipError_t launch_dense_solve(const DevProblem& P, const DensePairs& D, double* S, double* work, int32_t n,
-                              int32_t* status, hipStream_t s) {
-    // the LU fallback's copy of S at full device bandwidth; the solve itself
-    // is one work-group and is not launched when the copy fails
-    const hipError_t e = hipMemcpyAsync(work, S, (size_t)n * n * sizeof(double), hipMemcpyDeviceToDevice, s);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_dense_solve, dim3(1), dim3(kSmallBlock), 0, s, P, D, S, work, n, status);
+                              int32_t* status, hipStream_t s, bool allow_lu) {
+    allow_lu = allow_lu && n <= kLuFallbackMax;
+    // the LU fallback's copy of S at full device bandwidth
+    if (allow_lu) {
+        const hipError_t e = hipMemcpyAsync(work, S, (size_t)n * n * sizeof(double), hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return e;
+    }
+    if (n > kDenseOneGroupMax) return dense_solve_device_wide(pose_rows(P, D.var_slot, n), S, work, n, status, allow_lu, s);
+    hipLaunchKernelGGL(k_dense_solve, dim3(1), dim3(kSmallBlock), 0, s, P, D, S, work, n, status, (int)allow_lu);
     return hipGetLastError();
 }
 hipError_t launch_dense_solve_plain(double* S, double* work, const double* b, double* x, int32_t n, int32_t* status,
                                     hipStream_t s) {
     const hipError_t e = hipMemcpyAsync(work, S, (size_t)n * n * sizeof(double), hipMemcpyDeviceToDevice, s);
     if (e != hipSuccess) return e;
+    if (n > kDenseOneGroupMax)
+        return dense_solve_device_wide(DenseRows{b, x, nullptr, n, nullptr, nullptr}, S, work, n, status,
+                                       n <= kLuFallbackMax, s);
     hipLaunchKernelGGL(k_dense_solve_plain, dim3(1), dim3(kSmallBlock), 0, s, S, work, b, x, n, status);
     return hipGetLastError();
 }

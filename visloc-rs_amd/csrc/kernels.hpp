@@ -181,9 +181,11 @@ void launch_rhs(const DevProblem& P, hipStream_t stream);
 // partials part64; fp32 reads the FP32 copies of G and V^-1 (arithmetic
 // stays FP64). With pcg_iter >= 0 the launch is PCG
 // iteration pcg_iter: each work-group first derives the iteration's
-// scalars (beta, stop) from the step kernel's partials.
+// scalars (beta, stop) from the step kernel's partials. resident_waves > 0
+// runs the persistent form (that many waves loop over the tiles) with the
+// same results.
 void launch_schur_apply(const DevProblem& P, bool fp32, const double* v1, const double* v2, int32_t pcg_iter,
-                        int32_t fresh, hipStream_t stream);
+                        int32_t fresh, hipStream_t stream, int32_t resident_waves = 0);
 // PCG: r = b, z = M^-1 r, x = d = 0 and the state (implicit_schur.rs:522-560).
 void launch_pcg_init(const DevProblem& P, double rtol, double atol, int32_t max_iter, double lambda,
                      hipStream_t stream);
@@ -255,15 +257,24 @@ size_t joint_part_size(const DevProblem& P);
 
 void launch_dense_assemble(const DevProblem& P, const DensePairs& D, double* S, int32_t n,
                            double lambda, hipStream_t stream);
-// Solve S x = b (Cholesky, LU fallback) for the variable poses; writes x and
-// sets *status (0 ok, 1 singular). S is overwritten; `work` receives a copy of
-// S for the LU fallback. Returns the error of the copy or of the launch.
+// Solve S x = b (Cholesky, then an LU fallback with allow_lu, else singular)
+// for the variable poses; writes x and sets status[0] (0 ok, 1 singular);
+// status holds two words (the second records a failed pivot). S is
+// overwritten; `work` (n * n + n) receives a copy of S for the LU fallback.
+// Systems of more than kDenseOneGroupMax unknowns are factored across the
+// device (two launches per 64 columns), smaller ones by one work-group.
+// Returns the error of the copy or of a launch.
 // The same solve for a plain system: S row-major n x n (overwritten), b and x
-// n long, work n * n + n.
+// n long, always with the LU fallback.
+constexpr int32_t kDenseOneGroupMax = 256;
+// The LU fallback runs in one work-group, reading and writing the trailing
+// matrix once per pivot (n^3 / 3 x 16 bytes through one CU); beyond this size
+// it would run for minutes, so a failed Cholesky is reported singular.
+constexpr int32_t kLuFallbackMax = 3072;
 hipError_t launch_dense_solve_plain(double* S, double* work, const double* b, double* x, int32_t n,
                                     int32_t* status, hipStream_t stream);
 hipError_t launch_dense_solve(const DevProblem& P, const DensePairs& D, double* S, double* work,
-                        int32_t n, int32_t* status, hipStream_t stream);
+                              int32_t n, int32_t* status, hipStream_t stream, bool allow_lu = true);
 
 // ---- batches of windows (one work-group per window where noted) ----
 struct BatchDev {

@@ -199,8 +199,7 @@ int32_t check_options(const vba_options& o, std::string* reason) {
         *reason = "robust kernel";
         return VBA_DECLINED;
     }
-    if (o.linear_solver != VBA_SOLVER_DENSE && o.linear_solver != VBA_SOLVER_MATRIX_FREE &&
-        o.linear_solver != VBA_SOLVER_PCG_EXPLICIT) {
+    if (o.linear_solver > VBA_SOLVER_PCG_EXPLICIT) {
         *reason = "linear solver";
         return VBA_DECLINED;
     }

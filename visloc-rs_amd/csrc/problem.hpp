@@ -27,6 +27,15 @@
 
 namespace vba {
 
+// The linear solvers that form the reduced camera matrix densely (Dense, and
+// Sparse, whose block Cholesky the device runs as a dense factor).
+inline bool dense_family(uint32_t linear_solver) {
+    return linear_solver == VBA_SOLVER_DENSE || linear_solver == VBA_SOLVER_SPARSE;
+}
+// Variable poses of a single dense reduced solve: its two n x n FP64 buffers
+// (S and the LU copy) stay below 40 GB.
+constexpr uint32_t kDenseMaxPoses = 8192;
+
 // A lens plus the fixed extrinsic T_sensor<-rig as a rotation matrix and a
 // translation. Sensor 0 is the shared camera of monocular and stereo
 // observations with the identity extrinsic; then come the rig sensors, then

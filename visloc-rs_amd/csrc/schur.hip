@@ -494,6 +494,226 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
+// One landmark tile of the operator, by one wave. sX0 (3 * 64 + 2 doubles,
+// 16-byte aligned), sT (3 * 64) and sVR (6 * 64, 16-byte aligned) are the
+// wave's LDS. With `prologue` the PCG scalars of iteration pcg_iter are
+// derived here, after the tile's loads are issued (false: the solve has
+// stopped); without it `beta` comes in.
+template <typename TS>
+__device__ __forceinline__ bool apply_tile(const DevProblem& P, const double* __restrict__ v1,
+                                           const double* __restrict__ v2, const TS* __restrict__ Gl,
+                                           const double* __restrict__ Vinv, double* __restrict__ part,
+                                           int tile, const LmTile lt, int pcg_iter, int fresh, bool prologue,
+                                           double beta, double* sX0, double* const sT, double* const sVR) {
+    double* sX = sX0;
+    double* const sV = sVR;
+    double* const sR = sVR;
+    if (P.wstate && !P.wstate[P.lm_tile_win[tile]].active) return true;
+    const int lane = threadIdx.x & 63;
+    const int e0 = lt.e_begin, g0 = kTileEdges * lt.g_chunk;
+    const int n = lt.e_end - lt.e_begin, nl = lt.lm_end - lt.lm_begin;
+    const bool one = nl == 1 && n > kTileRuns;
+
+    // ---- stage 0: the tile's data, loaded before the PCG scalars so that
+    // both are in flight together (a stopped launch drops them) ----
+    const int s0 = kApplyPer * lane;
+    const int nv = one ? 0 : max(0, min(kApplyPer, n - s0));
+    static_assert(kApplyPer <= 4 && kTileLandmarks <= 256, "local landmarks packed in one word");
+    int key[kApplyPer], slot[kApplyPer];
+    uint32_t jl4 = 0;  // the positions' local landmarks, a byte each
+    TS G[kApplyPer][6];
+#pragma unroll
+    for (int i = 0; i < kApplyPer; ++i) {
+        key[i] = -1 - (s0 + i);  // unique for positions past the end
+        slot[i] = -1;
+        if (i < nv) {
+            key[i] = P.s_pose[e0 + s0 + i];
+            jl4 |= (uint32_t)P.s_lm[e0 + s0 + i] << (8 * i);
+            slot[i] = P.run_slot[e0 + s0 + i];  // read here, used by the run's partial at the end
+            if constexpr (std::is_same<TS, float>::value) {
+                // slot i of the tile's lane-major chunk: 1 KiB + 512 B per wave
+                const float* gb = Gl + (size_t)lt.g_chunk * kG32Chunk + 384 * i;
+                const float4 a = *(const float4*)(gb + 4 * lane);
+                const float2 b = *(const float2*)(gb + 256 + 2 * lane);
+                G[i][0] = a.x;
+                G[i][1] = a.y;
+                G[i][2] = a.z;
+                G[i][3] = a.w;
+                G[i][4] = b.x;
+                G[i][5] = b.y;
+            } else {
+                load6(Gl + 6 * (size_t)(e0 + s0 + i), G[i]);
+            }
+        }
+    }
+    // the z and d of this lane's first pose, in flight with the tile data
+    // (most lanes' positions belong to one camera run)
+    double z0[6], w0[6];
+    if (nv > 0) {
+        load6(v1 + 6 * (size_t)key[0], z0);
+        if (v2) load6(v2 + 6 * (size_t)key[0], w0);
+    }
+    // V^-1 and X of the tile's landmarks go straight from global memory into
+    // LDS (their layout there is the global one), in flight together with
+    // the loads above: 16-byte pieces, lane i's piece at byte 16 i of each
+    // 1 KiB chunk, over the tile's contiguous landmark range. The X range
+    // starts one double early when lm_begin is odd, and its last piece may
+    // end 8 bytes past the range (the landmark buffers carry 16 bytes of
+    // slack).
+    if (!one) {
+        const char* gv = reinterpret_cast<const char*>(Vinv + 6 * (size_t)lt.lm_begin);
+        const int xs = lt.lm_begin & 1;
+        const char* gx = reinterpret_cast<const char*>(P.lms + 3 * (size_t)lt.lm_begin - xs);
+        char* lx = reinterpret_cast<char*>(sX);
+        sX += xs;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            if (1024 * c + 16 * lane < 48 * nl)
+                __builtin_amdgcn_global_load_lds(gv + 1024 * c + 16 * lane, (lds_void_ptr)((char*)sV + 1024 * c), 16,
+                                                 0, 0);
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+            if (1024 * c + 16 * lane < 8 * xs + 24 * nl)
+                __builtin_amdgcn_global_load_lds(gx + 1024 * c + 16 * lane, (lds_void_ptr)(lx + 1024 * c), 16, 0, 0);
+    }
+    if (prologue && pcg_iter >= 0 && !pcg_prologue(P, pcg_iter, fresh, &beta)) {
+        __builtin_amdgcn_s_waitcnt(0);  // no LDS write may land after the wave ends
+        return false;
+    }
+    if (one) {
+        apply_one_landmark<TS>(P, v1, v2, beta, Gl, Vinv, part, lt.lm_begin, e0, g0, n, lane);
+        return true;
+    }
+    if (lane < nl) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) sT[3 * lane + c] = 0.0;
+    }
+    __builtin_amdgcn_s_waitcnt(0);  // the LDS loads have landed
+    wave_lds_sync();
+
+    // ---- stage 1: t_j += W_e^T d_p, one d per pose run of this lane ----
+    {
+        int pcur = INT_MIN;
+        double d[6];
+#pragma unroll
+        for (int i = 0; i < kApplyPer; ++i) {
+            if (i < nv) {
+                if (key[i] != pcur) {
+                    if (i == 0) {
+#pragma unroll
+                        for (int c = 0; c < 6; ++c) d[c] = v2 ? z0[c] + beta * w0[c] : z0[c];
+                    } else {
+                        load_d(v1, v2, beta, key[i], d);
+                    }
+                    pcur = key[i];
+                }
+                const int j = (jl4 >> (8 * i)) & 0xff;
+                const double X[3] = {sX[3 * j], sX[3 * j + 1], sX[3 * j + 2]};
+                double t[3];
+                edge_wtd(G[i], d, X, t);
+                atomicAdd(&sT[3 * j], t[0]);
+                atomicAdd(&sT[3 * j + 1], t[1]);
+                atomicAdd(&sT[3 * j + 2], t[2]);
+            }
+        }
+    }
+    wave_lds_sync();
+
+    // ---- stage 2: u_j = V_j^-1 t_j ----
+    if (lane < nl) {
+        const double t[3] = {sT[3 * lane], sT[3 * lane + 1], sT[3 * lane + 2]};
+        double vi[6], u[3];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) vi[c] = sV[6 * lane + c];
+        sym3_mul(vi, t, u);
+        sT[3 * lane] = u[0];
+        sT[3 * lane + 1] = u[1];
+        sT[3 * lane + 2] = u[2];
+    }
+    // the run accumulators replace V^-1: this wave's reads of it above come
+    // first in its LDS order
+#pragma unroll
+    for (int c = 0; c < 6; ++c) sR[6 * lane + c] = 0.0;
+    wave_lds_sync();
+
+    // ---- stage 3: y_e = W_e u_j summed per camera run ----
+    const int prev_key = __shfl_up(key[kApplyPer - 1], 1, 64);  // last key of the lane before
+    const int next_key = __shfl_down(key[0], 1, 64);             // first key of the lane after
+    bool head[kApplyPer], fin[kApplyPer];
+#pragma unroll
+    for (int i = 0; i < kApplyPer; ++i) {
+        const int pk = i == 0 ? (lane == 0 ? INT_MIN : prev_key) : key[i - 1];
+        const int nk = i + 1 < kApplyPer ? key[i + 1] : (lane == 63 ? INT_MIN : next_key);
+        head[i] = i < nv && key[i] != pk;
+        fin[i] = i < nv && (s0 + i == n - 1 || key[i] != nk);
+    }
+    // run index of each position within the tile: run starts before it
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    int rid[kApplyPer];
+    {
+        int cnt = 0;
+#pragma unroll
+        for (int i = 0; i < kApplyPer; ++i) cnt += __popcll(__ballot(head[i]) & below);
+#pragma unroll
+        for (int i = 0; i < kApplyPer; ++i) {
+            cnt += head[i] ? 1 : 0;
+            rid[i] = cnt - 1;
+        }
+    }
+    double acc[6];
+    bool own = false;   // the current segment's run starts in this lane
+    int carried = -1;   // position whose run started in an earlier lane and ends here
+#pragma unroll
+    for (int i = 0; i < kApplyPer; ++i) {
+        if (i < nv) {
+            const int j = (jl4 >> (8 * i)) & 0xff;
+            const double X[3] = {sX[3 * j], sX[3 * j + 1], sX[3 * j + 2]};
+            const double u[3] = {sT[3 * j], sT[3 * j + 1], sT[3 * j + 2]};
+            const double Gd[6] = {(double)G[i][0], (double)G[i][1], (double)G[i][2],
+                                  (double)G[i][3], (double)G[i][4], (double)G[i][5]};
+            double y[6];
+            edge_wu(Gd, u, X, y);
+            if (i == 0 || fin[i - 1]) {  // a new segment of this lane
+                own = head[i];
+#pragma unroll
+                for (int c = 0; c < 6; ++c) acc[c] = y[c];
+            } else {
+#pragma unroll
+                for (int c = 0; c < 6; ++c) acc[c] += y[c];
+            }
+            if (fin[i]) {
+                if (own) {
+                    double* o = part + 6 * (size_t)slot[i];
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) o[c] = acc[c];
+                } else if (rid[i] < kTileRuns) {
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) atomicAdd(&sR[6 * rid[i] + c], acc[c]);
+                    carried = i;
+                }
+            }
+        }
+    }
+    // a segment open at the lane's end continues in the next lane
+    if (nv > 0 && !fin[nv - 1] && rid[nv - 1] < kTileRuns) {
+        const int r = rid[nv - 1];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) atomicAdd(&sR[6 * r + c], acc[c]);
+    }
+    wave_lds_sync();
+    if (carried >= 0) {
+        const int r = rid[carried];
+        int sl = slot[0];
+#pragma unroll
+        for (int i = 1; i < kApplyPer; ++i)
+            if (i == carried) sl = slot[i];
+        double* o = part + 6 * (size_t)sl;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) o[c] = sR[6 * r + c];
+    }
+    return true;
+}
+
 template <typename TS>
 __global__ __launch_bounds__(64 * kApplyWaves) void k_schur_apply(DevProblem P, const double* __restrict__ v1,
                                                                   const double* __restrict__ v2,
@@ -723,6 +943,45 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_schur_apply(DevProblem P, 
         for (int c = 0; c < 6; ++c) o[c] = sR[6 * r + c];
     }
 }
+
+// The same operator with a fixed grid of resident waves: XCD x takes the
+// x-th eighth of the tiles, its waves take them in turn, each wave loading
+// its next tile's bounds while it works on the current one. The PCG scalars
+// are derived once per wave. Tiles are those of k_schur_apply and each is
+// worked the same way, so the results are the same bit for bit. gridDim.x is
+// a multiple of 8.
+template <typename TS>
+__global__ __launch_bounds__(64, 4) void k_schur_apply_loop(DevProblem P, const double* __restrict__ v1,
+                                                         const double* __restrict__ v2,
+                                                         const TS* __restrict__ Gl,
+                                                         const double* __restrict__ Vinv,
+                                                         double* __restrict__ part, int pcg_iter, int fresh) {
+    __shared__ __align__(16) double sXa[kTileLandmarks * 3 + 2];
+    __shared__ double sTa[kTileLandmarks * 3];
+    __shared__ __align__(16) double sVRa[kTileLandmarks * 6];
+    const int T = P.num_lm_tiles;
+    const int per = (T + 7) >> 3;
+    const int x = blockIdx.x & 7, stride = gridDim.x >> 3;
+    const int t1 = min(T, (x + 1) * per);
+    int tile = x * per + (int)(blockIdx.x >> 3);
+    double beta = 0.0;
+    if (pcg_iter >= 0 && !pcg_prologue(P, pcg_iter, fresh, &beta)) return;
+    if (tile >= t1) return;
+    LmTile lt = P.lm_tiles[tile];
+    while (true) {
+        const int next = tile + stride;
+        LmTile ln{};
+        if (next < t1) ln = P.lm_tiles[next];
+        (void)apply_tile<TS>(P, v1, v2, Gl, Vinv, part, tile, lt, -1, 0, false, beta, sXa, sTa, sVRa);
+        if (next >= t1) break;
+        __builtin_amdgcn_s_waitcnt(0);  // this tile's LDS traffic is done before the next tile's loads land
+        wave_lds_sync();
+        tile = next;
+        lt = ln;
+    }
+}
+
+
 
 // ---------------------------------------------------------------------------
 // Pose side: one wave per pose sums its runs' partials (lanes stride the
@@ -1051,7 +1310,17 @@ void launch_rhs(const DevProblem& P, hipStream_t s) {
 }
 
 void launch_schur_apply(const DevProblem& P, bool fp32, const double* v1, const double* v2, int32_t pcg_iter,
-                        int32_t fresh, hipStream_t s) {
+                        int32_t fresh, hipStream_t s, int32_t resident_waves) {
+    if (resident_waves > 0 && resident_waves < P.num_lm_tiles) {
+        const dim3 grid(xcd_grid(resident_waves)), block(64);
+        if (fp32)
+            hipLaunchKernelGGL(k_schur_apply_loop<float>, grid, block, 0, s, P, v1, v2, (const float*)P.Gl32,
+                               (const double*)P.VinvM, P.part64, pcg_iter, fresh);
+        else
+            hipLaunchKernelGGL(k_schur_apply_loop<double>, grid, block, 0, s, P, v1, v2, (const double*)P.Gl64,
+                               (const double*)P.Vinv64, P.part64, pcg_iter, fresh);
+        return;
+    }
     const dim3 grid(xcd_grid(blocks_for(P.num_lm_tiles, kApplyWaves))), block(64 * kApplyWaves);
     if (fp32)
         hipLaunchKernelGGL(k_schur_apply<float>, grid, block, 0, s, P, v1, v2, (const float*)P.Gl32,

@@ -126,7 +126,9 @@ int32_t dense_solve_noalloc(Context& ctx, uint32_t n, const double* h, const dou
         return fail(e, "download of x");
     if ((e = hipStreamSynchronize(s)) != hipSuccess) return fail(e, "sync");
     if (*ctx.host_status != 0) {
-        *msg = "singular system: neither Cholesky nor LU with partial pivoting solves it";
+        *msg = n <= (uint32_t)kLuFallbackMax
+                   ? "singular system: neither Cholesky nor LU with partial pivoting solves it"
+                   : "singular system: Cholesky fails (no LU fallback beyond 3072 unknowns on the device)";
         return VBA_ERR_SINGULAR_SYSTEM;
     }
     return VBA_OK;

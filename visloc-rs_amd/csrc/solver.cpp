@@ -465,7 +465,7 @@ int32_t dense_solve(Context& ctx, uint32_t n, const double* h, const double* b, 
         alloc(bwork, (nn + n) * sizeof(double));
         alloc(bb, n * sizeof(double));
         alloc(bx, n * sizeof(double));
-        alloc(bst, sizeof(int32_t));
+        alloc(bst, 2 * sizeof(int32_t));
         hip_check(hipMemcpyAsync(bS.ptr, rows.data(), nn * sizeof(double), hipMemcpyHostToDevice, s), "upload");
         hip_check(hipMemcpyAsync(bb.ptr, b, n * sizeof(double), hipMemcpyHostToDevice, s), "upload");
         hip_check(launch_dense_solve_plain((double*)bS.ptr, (double*)bwork.ptr, (const double*)bb.ptr,
@@ -475,7 +475,9 @@ int32_t dense_solve(Context& ctx, uint32_t n, const double* h, const double* b, 
         hip_check(hipMemcpyAsync(x, bx.ptr, n * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
         hip_check(hipStreamSynchronize(s), "sync");
         if (*ctx.host_status != 0) {
-            *msg = "singular system: neither Cholesky nor LU with partial pivoting solves it";
+            *msg = n <= (uint32_t)kLuFallbackMax
+                       ? "singular system: neither Cholesky nor LU with partial pivoting solves it"
+                       : "singular system: Cholesky fails (no LU fallback beyond 3072 unknowns on the device)";
             code = VBA_ERR_SINGULAR_SYSTEM;
         }
     } catch (const Fail& f) {
@@ -556,7 +558,7 @@ int32_t squared_residuals(Context& ctx, Prepared& pr, const vba_options& o, doub
     hipStream_t s = ctx.stream;
     const Model& m = pr.m;
     try {
-        DevProblem P = bind(ctx, pr, o, false, o.linear_solver != VBA_SOLVER_DENSE, s);
+        DevProblem P = bind(ctx, pr, o, false, !dense_family(o.linear_solver), s);
         P.obs_s = (double*)ensure(pr, B_OBS_S, std::max<uint64_t>(m.num_obs, 1) * sizeof(double));
         launch_cost(P, false, s);
         std::vector<double> sc(m.num_obs);
@@ -670,13 +672,13 @@ int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stat
             vba_result* res, const JointSpec* joint) {
     hipStream_t s = ctx.stream;
     const Model& m = pr.m;
-    const bool matrix_free = o.linear_solver != VBA_SOLVER_DENSE;  // a PCG solver
+    const bool matrix_free = !dense_family(o.linear_solver);  // a PCG solver
     const bool xplicit = o.linear_solver == VBA_SOLVER_PCG_EXPLICIT;
     const bool mixed = matrix_free && o.precision == VBA_PRECISION_MIXED;
     const bool accept_iterate = o.pcg_maxit_policy == VBA_PCG_MAXIT_ACCEPT_ITERATE;
     try {
-        if (!matrix_free && m.P > 1024)
-            throw Fail{VBA_ERR_INVALID_ARGUMENT, "dense reduced solve limited to 1024 variable poses"};
+        if (!matrix_free && m.P > kDenseMaxPoses)
+            throw Fail{VBA_ERR_INVALID_ARGUMENT, "dense reduced solve limited to 8192 variable poses"};
         DevProblem P = bind(ctx, pr, o, mixed, matrix_free, s);
         const size_t NP = m.num_pose_slots, NL = m.num_lm_slots, NE = m.num_edges, NT = m.cam_tiles.size();
         const size_t NR = m.num_runs;
@@ -742,7 +744,7 @@ int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stat
         if (!matrix_free) {
             dS = (double*)ensure(pr, B_DENSE_S, (size_t)ndense * ndense * sizeof(double));
             dwork = (double*)ensure(pr, B_DENSE_WORK, ((size_t)ndense * ndense + ndense) * sizeof(double));
-            dstatus = (int32_t*)ensure(pr, B_DENSE_STATUS, sizeof(int32_t));
+            dstatus = (int32_t*)ensure(pr, B_DENSE_STATUS, 2 * sizeof(int32_t));
         }
         // Algorithmic bytes per launch of the profiled kernels: what the data
         // layout must move at least once (include/vba.h, vba_profile_query).
@@ -843,7 +845,9 @@ int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stat
                 if (JK) launch_joint_border(P, dS, ndense, lambda, s);
                 {
                     ProfScope f(ctx, K_DENSE_SOLVE, flops_dfactor);
-                    hip_check(launch_dense_solve(P, D, dS, dwork, ndense, dstatus, s), "dense solve");
+                    hip_check(launch_dense_solve(P, D, dS, dwork, ndense, dstatus, s,
+                                                 o.linear_solver != VBA_SOLVER_SPARSE),
+                              "dense solve");
                 }
                 hip_check(hipMemcpyAsync(ctx.host_status, dstatus, sizeof(int32_t), hipMemcpyDeviceToHost, s), "D2H");
                 hip_check(hipStreamSynchronize(s), "sync");
@@ -877,7 +881,8 @@ int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stat
                     }
                     {
                         ProfScope a(ctx, K_SCHUR_APPLY, bytes_apply, kk);
-                        launch_schur_apply(P, mixed, P.z, P.d, kk, fresh_kk, s);
+                        launch_schur_apply(P, mixed, P.z, P.d, kk, fresh_kk, s,
+                                           (int32_t)ctx.tuning.apply_waves_per_cu * ctx.num_cus);
                     }
                     {
                         ProfScope b(ctx, K_PCG_POSE, bytes_pcg_pose, kk);

@@ -48,6 +48,7 @@ KERNEL_HUBER = 1
 KERNEL_CAUCHY = 2
 
 SOLVER_DENSE = 0
+SOLVER_SPARSE = 1  # LinearSolver::Sparse: Cholesky only, no LU fallback
 SOLVER_MATRIX_FREE = 2
 SOLVER_PCG_EXPLICIT = 3
 
@@ -260,10 +261,11 @@ class GncResult(C.Structure):
 class Tuning(C.Structure):
     """vba_tuning: the PCG launch schedule (results do not depend on it)."""
     _fields_ = [("pcg_chunk", C.c_uint32), ("pcg_lookahead", C.c_uint32), ("pcg_predict", C.c_uint32),
-                ("reserved", C.c_uint32)]
+                ("apply_waves_per_cu", C.c_uint32)]
 
     def as_dict(self):
-        return {"pcg_chunk": self.pcg_chunk, "pcg_lookahead": self.pcg_lookahead, "pcg_predict": self.pcg_predict}
+        return {"pcg_chunk": self.pcg_chunk, "pcg_lookahead": self.pcg_lookahead, "pcg_predict": self.pcg_predict,
+                "apply_waves_per_cu": self.apply_waves_per_cu}
 
 
 class ReducedSystem(C.Structure):

@@ -404,9 +404,10 @@ class Device:
             self.lib.vba_profile_reset(self.ctx)
         self.lib.vba_profile_enable(self.ctx, max(1, int(stride)) if enable else 0)
 
-    def set_tuning(self, pcg_chunk: int = 10, pcg_lookahead: int = 0, pcg_predict: bool = False) -> None:
+    def set_tuning(self, pcg_chunk: int = 10, pcg_lookahead: int = 0, pcg_predict: bool = False,
+                   apply_waves_per_cu: int = 0) -> None:
         """The PCG launch schedule of this context (vba_set_tuning)."""
-        t = abi.Tuning(int(pcg_chunk), int(pcg_lookahead), int(bool(pcg_predict)), 0)
+        t = abi.Tuning(int(pcg_chunk), int(pcg_lookahead), int(bool(pcg_predict)), int(apply_waves_per_cu))
         st = self.lib.vba_set_tuning(self.ctx, C.byref(t))
         if st != abi.OK:
             raise ValueError(self.last_error())

@@ -60,6 +60,20 @@ def loop_problem(num_landmarks: int | None = None):
     return synthesize(abi.SYNTH_LOOP, **kw)
 
 
+def dense_loop_options(linear_solver=abi.SOLVER_DENSE, max_iterations: int = 8):
+    """A large dense reduced solve: the loop scene's Huber 1 px with the
+    dense Schur complement and Cholesky (LinearSolver::Dense)."""
+    return default_options(linear_solver=linear_solver, max_iterations=max_iterations,
+                           robust_kernel=abi.KERNEL_HUBER, robust_scale=1.0)
+
+
+def dense_loop_problem(num_cameras: int = 1500, num_landmarks: int = 60000):
+    """Loop cameras at the full scene's spacing (a 250 m loop for 1,500
+    cameras), for the dense reduced solve of 6 x (cameras - 1) unknowns."""
+    return synthesize(abi.SYNTH_LOOP, num_cameras=num_cameras, num_landmarks=num_landmarks,
+                      scene_scale=200.0 * num_cameras / 1200.0)
+
+
 def window_options(precision=abi.PRECISION_FP64):
     return default_options(linear_solver=abi.SOLVER_DENSE, max_iterations=10, robust_kernel=abi.KERNEL_CAUCHY,
                            robust_scale=1.0, precision=precision)
