@@ -587,22 +587,39 @@ __device__ void chol_blocked(double* S, int n, double* dblk, int* fail) {
     }
 }
 
+// Lane l's value of v in every lane; l must be the same in every lane (a
+// scalar read of one lane, where a shuffle would go through LDS).
+__device__ __forceinline__ double lane_value(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // y = L^-1 y then y = L^-T y for the factor of chol_blocked, by 16-row
-// blocks: the diagonal block by wave 0 (one row per lane, shuffles), the
-// rest as one short dot product per row.
+// blocks: the diagonal block, staged in LDS (dblk, 16 x 17), by wave 0 (one
+// row per lane, shuffles), the rest as one short dot product per row.
 template <int NT>
-__device__ void chol_blocked_solve(const double* S, int n, double* y) {
+__device__ void chol_blocked_solve(const double* S, int n, double* y, double* dblk) {
+    static_assert(NT >= 256, "one thread per entry of the diagonal block");
     const int nb = (n + 15) >> 4;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int ti = threadIdx.x >> 4, tc = threadIdx.x & 15;
+    auto stage = [&](int k0) {
+        if (threadIdx.x < 256)
+            dblk[ti * 17 + tc] = (k0 + ti < n && tc <= ti) ? S[(size_t)(k0 + ti) * n + k0 + tc] : 0.0;
+        __syncthreads();
+    };
     for (int k = 0; k < nb; ++k) {  // forward: L y = b
         const int k0 = 16 * k;
+        stage(k0);
         if (wave == 0) {
             const int r = k0 + lane;
             double v = lane < 16 && r < n ? y[r] : 0.0;
             for (int j = 0; j < 16 && k0 + j < n; ++j) {
-                if (lane == j) v = v / S[(size_t)r * n + r];
-                const double yj = __shfl(v, j, 64);
-                if (lane > j && lane < 16 && r < n) v -= S[(size_t)r * n + k0 + j] * yj;
+                if (lane == j) v = v / dblk[j * 17 + j];
+                const double yj = lane_value(v, j);
+                if (lane > j && lane < 16 && r < n) v -= dblk[lane * 17 + j] * yj;
             }
             if (lane < 16 && r < n) y[r] = v;
         }
@@ -616,14 +633,15 @@ __device__ void chol_blocked_solve(const double* S, int n, double* y) {
     }
     for (int k = nb - 1; k >= 0; --k) {  // backward: L^T x = y
         const int k0 = 16 * k;
+        stage(k0);
         if (wave == 0) {
             const int r = k0 + lane;
             double v = lane < 16 && r < n ? y[r] : 0.0;
             for (int j = 15; j >= 0; --j) {
                 if (k0 + j >= n) continue;
-                if (lane == j) v = v / S[(size_t)(k0 + j) * n + k0 + j];
-                const double xj = __shfl(v, j, 64);
-                if (lane < j && r < n) v -= S[(size_t)(k0 + j) * n + r] * xj;
+                if (lane == j) v = v / dblk[j * 17 + j];
+                const double xj = lane_value(v, j);
+                if (lane < j && r < n) v -= dblk[j * 17 + lane] * xj;
             }
             if (lane < 16 && r < n) y[r] = v;
         }
@@ -758,7 +776,7 @@ __device__ void dense_solve_body(const DenseRows& io, double* S, double* work, i
     chol_blocked<NT>(S, n, dblk, fail_sh);
     __syncthreads();
     if (!*fail_sh) {
-        chol_blocked_solve<NT>(S, n, y);
+        chol_blocked_solve<NT>(S, n, y, dblk);
     } else if (allow_lu) {
         // LU with partial pivoting on A2; y holds b again.
         for (int i = threadIdx.x; i < n; i += NT) y[i] = io.rhs(i);
@@ -788,6 +806,7 @@ __device__ void dense_solve_body(const DenseRows& io, double* S, double* work, i
 // ---------------------------------------------------------------------------
 constexpr int kPanel = 64;
 
+
 __global__ void k_dense_flags_clear(int* flags) {
     if (threadIdx.x < 2) flags[threadIdx.x] = 0;
 }
@@ -801,44 +820,104 @@ __global__ __launch_bounds__(256) void k_chol_panel(double* S, int n, int k0, in
         L[r][c] = (r < w && c < w) ? S[(size_t)(k0 + r) * n + k0 + c] : (r == c ? 1.0 : 0.0);
     }
     __syncthreads();
-    for (int j = 0; j < kPanel; ++j) {
-        const double d = L[j][j];
-        if (!(d > 0.0) || !isfinite(d)) {  // the same in every thread and work-group
-            if (blockIdx.x == 0 && threadIdx.x == 0)
-                __hip_atomic_store(&flags[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            return;
+    // the diagonal block's factor by wave 0, lane i holding row i in registers:
+    // column j's pivot and entries come from lane j and lane c by lane reads.
+    // Then lane c forms column c of L^-1 the same way, into LDS (Li).
+    __shared__ int failed;
+    __shared__ double Li[kPanel][kPanel + 1];
+    if (threadIdx.x < kPanel) {
+        const int i = threadIdx.x;
+        double a[kPanel];
+#pragma unroll
+        for (int c = 0; c < kPanel; ++c) a[c] = L[i][c];
+        int bad = 0;
+#pragma unroll
+        for (int j = 0; j < kPanel; ++j) {
+            const double d = lane_value(a[j], j);
+            if (!(d > 0.0) || !isfinite(d)) {  // the same in every lane and work-group
+                bad = 1;
+                break;
+            }
+            const double ljj = sqrt(d);
+            if (i == j) a[j] = ljj;
+            else if (i > j) a[j] = a[j] / ljj;
+            const double lij = a[j];
+#pragma unroll
+            for (int c = j + 1; c < kPanel; ++c) {
+                const double lcj = lane_value(a[j], c);
+                if (c <= i) a[c] -= lij * lcj;
+            }
         }
-        const double ljj = sqrt(d);
-        __syncthreads();  // every thread has read L[j][j]
-        if (threadIdx.x == 0) L[j][j] = ljj;
-        for (int i = j + 1 + (int)threadIdx.x; i < kPanel; i += 256) L[i][j] = L[i][j] / ljj;
-        __syncthreads();
-        for (int t = threadIdx.x; t < kPanel * kPanel; t += 256) {
-            const int i = t / kPanel, c = t % kPanel;
-            if (c > j && c <= i) L[i][c] -= L[i][j] * L[c][j];
+        if (i == 0) failed = bad;
+        if (!bad) {
+            if (blockIdx.x == 0 && i < w)
+#pragma unroll
+                for (int c = 0; c < kPanel; ++c)
+                    if (c <= i) S[(size_t)(k0 + i) * n + k0 + c] = a[c];
+            if (k0 + kPanel < n) {
+                // column c = i of L^-1: y_r = (e_c - sum_{k<r} L[r][k] y_k) / L[r][r]
+                const int c = i;
+                double yv[kPanel];
+#pragma unroll
+                for (int r = 0; r < kPanel; ++r) {
+                    double t0 = 0.0, t1 = 0.0;
+#pragma unroll
+                    for (int k = 0; k < r; ++k) {
+                        const double lrk = lane_value(a[k], r);
+                        if (k & 1) t1 += lrk * yv[k];
+                        else t0 += lrk * yv[k];
+                    }
+                    const double lrr = lane_value(a[r], r);
+                    yv[r] = r < c ? 0.0 : ((r == c ? 1.0 : 0.0) - (t0 + t1)) / lrr;
+                }
+#pragma unroll
+                for (int r = 0; r < kPanel; ++r) Li[r][c] = yv[r];
+            }
         }
-        __syncthreads();
     }
-    if (blockIdx.x == 0)
-        for (int t = threadIdx.x; t < kPanel * kPanel; t += 256) {
-            const int r = t / kPanel, c = t % kPanel;
-            if (c <= r && r < w) S[(size_t)(k0 + r) * n + k0 + c] = L[r][c];
+    __syncthreads();
+    if (failed) {
+        if (blockIdx.x == 0 && threadIdx.x == 0)
+            __hip_atomic_store(&flags[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
+    if (k0 + kPanel >= n) return;  // no panel rows below
+    // panel rows: X = A_R L_kk^-T on FP64 MFMA, 16 x 16 output tiles; each
+    // wave holds the B operands (L^-1 transposed) of all 4 column tiles and
+    // takes the rows of its 64-row slice of this work-group's 256 rows.
+    // Operand lane l holds A[l % 16][l / 16] and B[l / 16][l % 16]; result
+    // register q holds C[4q + l / 16][l % 16].
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double bop[4][kPanel / 4];
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+        for (int kk = 0; kk < kPanel / 4; ++kk) bop[ct][kk] = Li[16 * ct + (lane & 15)][4 * kk + (lane >> 4)];
+    const int rbase = k0 + kPanel + (int)blockIdx.x * 256 + 64 * wave;
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) {
+        const int r0 = rbase + 16 * rt;
+        if (r0 >= n) break;
+        const int ra = r0 + (lane & 15);
+        double aop[kPanel / 4];
+#pragma unroll
+        for (int kk = 0; kk < kPanel / 4; ++kk)
+            aop[kk] = ra < n ? S[(size_t)ra * n + k0 + 4 * kk + (lane >> 4)] : 0.0;
+        v4d acc[4];
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) {
+            acc[ct] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int kk = 0; kk < kPanel / 4; ++kk)
+                acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(aop[kk], bop[ct][kk], acc[ct], 0, 0, 0);
         }
-    const int R = k0 + kPanel + (int)(blockIdx.x * 256 + threadIdx.x);
-    if (R < n) {  // a panel row below exists only when the block is whole (w == 64)
-        double x[kPanel];
-        double* row = S + (size_t)R * n + k0;
 #pragma unroll
-        for (int c = 0; c < kPanel; ++c) x[c] = row[c];
+        for (int ct = 0; ct < 4; ++ct)
 #pragma unroll
-        for (int c = 0; c < kPanel; ++c) {
-            double t = x[c];
-#pragma unroll
-            for (int jj = 0; jj < c; ++jj) t -= x[jj] * L[c][jj];
-            x[c] = t / L[c][c];
-        }
-#pragma unroll
-        for (int c = 0; c < kPanel; ++c) row[c] = x[c];
+            for (int q = 0; q < 4; ++q) {
+                const int r = r0 + 4 * q + (lane >> 4);
+                if (r < n) S[(size_t)r * n + k0 + 16 * ct + (lane & 15)] = acc[ct][q];
+            }
     }
 }
 
@@ -878,11 +957,12 @@ __global__ __launch_bounds__(kSmallBlock) void k_dense_finish(DenseRows io, doub
     __shared__ int fail;
     __shared__ double piv_val[kSmallBlock];
     __shared__ int piv_idx[kSmallBlock];
+    __shared__ double dblk[16 * 17];
     double* y = work + (size_t)n * n;
     for (int i = threadIdx.x; i < n; i += kSmallBlock) y[i] = io.rhs(i);
     if (threadIdx.x == 0) fail = __hip_atomic_load(&flags[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
-    if (!fail) chol_blocked_solve<kSmallBlock>(S, n, y);
+    if (!fail) chol_blocked_solve<kSmallBlock>(S, n, y, dblk);
     else if (allow_lu) lu_fallback_body<kSmallBlock>(work, y, n, &fail, piv_val, piv_idx);
     else if (threadIdx.x == 0) fail = 2;
     __syncthreads();
