@@ -455,7 +455,7 @@ typedef struct vba_server_config {
     int32_t device;
     uint32_t max_window_batch;    /* windows per device batch */
     uint32_t window_streams;      /* window batches in flight at once */
-    uint32_t submap_streams;      /* single-problem solves in flight at once */
+    uint32_t submap_streams;      /* single-problem solves in flight at once (with window_streams, 16 lanes at most) */
     uint64_t hbm_budget_bytes;    /* admission limit on resident work */
     double batch_wait_seconds;    /* how long a window batch waits to fill */
     uint32_t build_threads;       /* host threads laying out a window batch */

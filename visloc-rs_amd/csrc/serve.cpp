@@ -314,7 +314,7 @@ void vba_server_default_config(vba_server_config* c) {
     c->device = 0;
     c->max_window_batch = 256;
     c->window_streams = 2;
-    c->submap_streams = 2;
+    c->submap_streams = 6;
     c->hbm_budget_bytes = 200ull << 30;
     c->batch_wait_seconds = 0.002;
     c->build_threads = 8;
@@ -325,7 +325,7 @@ vba_server* vba_server_create(const vba_server_config* config) {
     vba_server_default_config(&c);
     if (config) c = *config;
     if (c.max_window_batch == 0 || (c.window_streams == 0 && c.submap_streams == 0) ||
-        c.window_streams + c.submap_streams > 8 || !(c.batch_wait_seconds >= 0.0) || c.build_threads > 64) {
+        c.window_streams + c.submap_streams > 16 || !(c.batch_wait_seconds >= 0.0) || c.build_threads > 64) {
         vba::set_thread_error("vba_server_create: invalid configuration");
         return nullptr;
     }
