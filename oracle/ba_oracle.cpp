@@ -2903,15 +2903,16 @@ int32_t oracle_ns_operator(const oracle_normal_system* ns, double lambda, const 
 }
 
 int32_t oracle_ns_solve_pcg(const oracle_normal_system* ns, double lambda, uint32_t max_iterations,
-                            double rtol, double atol, double* delta_p, double* delta_l,
-                            uint32_t* iterations, double* residual_norm, double* target) {
+                            double rtol, double atol, int32_t accept_at_cap, double* delta_p,
+                            double* delta_l, uint32_t* iterations, double* residual_norm,
+                            double* target) {
     Normal ne;
     normal_from_view(ns, &ne);
     Implicit op;
     int e = build_implicit(ne, lambda, &op);
     if (e) return e;
     PcgOut out;
-    e = solve_pcg(op, max_iterations, rtol, atol, false, &out);
+    e = solve_pcg(op, max_iterations, rtol, atol, accept_at_cap != 0, &out);
     if (iterations) *iterations = out.iterations;
     if (residual_norm) *residual_norm = out.residual_norm;
     if (target) *target = out.target;

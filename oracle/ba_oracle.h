@@ -127,11 +127,13 @@ int32_t oracle_ns_solve_dense(const oracle_normal_system* ns, double lambda, dou
  * negative ImplicitSchurError code (see ba_oracle.cpp). */
 int32_t oracle_ns_operator(const oracle_normal_system* ns, double lambda, const double* x,
                            double* y, const double* r, double* z, double* rhs);
-/* solve_matrix_free_step: PCG + complete_delta (matrix_free.rs:1106-1240). */
+/* solve_matrix_free_step: PCG + complete_delta (matrix_free.rs:1106-1240).
+ * accept_at_cap != 0 takes the iterate at the iteration cap (the GPU
+ * backend's rule, ba-gpu/src/solver.rs:497-508) instead of MaxIterations. */
 int32_t oracle_ns_solve_pcg(const oracle_normal_system* ns, double lambda,
-                            uint32_t max_iterations, double rtol, double atol, double* delta_p,
-                            double* delta_l, uint32_t* iterations, double* residual_norm,
-                            double* target);
+                            uint32_t max_iterations, double rtol, double atol, int32_t accept_at_cap,
+                            double* delta_p, double* delta_l, uint32_t* iterations,
+                            double* residual_norm, double* target);
 /* solve_normal_equations (pose_graph.rs:2259-2270): Cholesky, LU fallback.
  * A is n x n row-major. Returns 1 on success. */
 int32_t oracle_solve_normal_equations(uint32_t n, const double* a, const double* b, double* x);

@@ -78,8 +78,8 @@ def library():
     lib.oracle_ns_operator.restype = C.c_int32
     lib.oracle_ns_operator.argtypes = [ns, C.c_double, P64, P64, P64, P64, P64]
     lib.oracle_ns_solve_pcg.restype = C.c_int32
-    lib.oracle_ns_solve_pcg.argtypes = [ns, C.c_double, C.c_uint32, C.c_double, C.c_double, P64, P64,
-                                        C.POINTER(C.c_uint32), P64, P64]
+    lib.oracle_ns_solve_pcg.argtypes = [ns, C.c_double, C.c_uint32, C.c_double, C.c_double, C.c_int32, P64,
+                                        P64, C.POINTER(C.c_uint32), P64, P64]
     lib.oracle_reduced_eliminate.restype = C.c_int32
     lib.oracle_reduced_eliminate.argtypes = [C.POINTER(abi.PointElimination), P64, P64, P64]
     lib.oracle_cholesky.restype = C.c_int32
@@ -338,15 +338,15 @@ class System:
                                           _p(z) if ra is not None else None, _p(rhs))
         return st, y, z, rhs
 
-    def solve_pcg(self, lam, max_iterations=128, rtol=1e-12, atol=1e-12):
+    def solve_pcg(self, lam, max_iterations=128, rtol=1e-12, atol=1e-12, accept_at_cap=False):
         dp = np.zeros(6 * self.P)
         dl = np.zeros(3 * self.L)
         it = C.c_uint32()
         rn = np.zeros(1)
         tg = np.zeros(1)
         v = self.view()
-        st = library().oracle_ns_solve_pcg(C.byref(v), lam, max_iterations, rtol, atol, _p(dp), _p(dl),
-                                           C.byref(it), _p(rn), _p(tg))
+        st = library().oracle_ns_solve_pcg(C.byref(v), lam, max_iterations, rtol, atol, int(accept_at_cap),
+                                           _p(dp), _p(dl), C.byref(it), _p(rn), _p(tg))
         return st, dp, dl, int(it.value), float(rn[0]), float(tg[0])
 
 

@@ -2,6 +2,7 @@
 against it: the reference's known-answer tests, closed forms, finite
 differences and a brute-force dense solve of the full normal system."""
 import math
+import os
 
 import numpy as np
 import pytest
@@ -9,7 +10,7 @@ import pytest
 import oracle
 import scenes
 import vba
-from vba import abi
+from vba import abi, workloads
 
 
 # ---------------------------------------------------------------------------
@@ -128,6 +129,164 @@ def test_pcg_limits():
     assert st == -7  # MaxIterations: a zero budget cannot solve a nonzero rhs
     st, dp, dl, it, rn, tg = ns.solve_pcg(0.25, max_iterations=1, rtol=1e-12, atol=1e-12)
     assert st == -7 and it == 1
+
+
+def random_system(seed, P=6, L=40):
+    """A BA-shaped normal system J^T J (plus a small pose prior) from random
+    2x6 / 2x3 Jacobian blocks, each landmark seen from 2-3 poses, so that PCG
+    needs many iterations."""
+    rng = np.random.default_rng(seed)
+    diag = [0.5 * np.eye(6) for _ in range(P)]
+    h_ll, b_l, cross = [], [], []
+    for _ in range(L):
+        hl = np.zeros((3, 3))
+        entries = []
+        for p in rng.choice(P, size=int(rng.integers(2, 4)), replace=False):
+            jp, jl = rng.normal(size=(2, 6)), rng.normal(size=(2, 3))
+            diag[p] = diag[p] + jp.T @ jp
+            hl += jl.T @ jl
+            entries.append((int(p), jp.T @ jl))
+        h_ll.append(hl)
+        b_l.append(rng.normal(size=3))
+        cross.append(entries)
+    return oracle.System(diag, rng.normal(size=6 * P), h_ll, b_l, cross)
+
+
+def textbook_pcg(S, rhs, k):
+    """k steps of preconditioned conjugate gradients from x = 0 with the
+    6x6 block-diagonal of S as the preconditioner (Saad, Iterative Methods
+    for Sparse Linear Systems, Algorithm 9.1)."""
+    n = len(rhs)
+    Minv = np.zeros_like(S)
+    for p in range(0, n, 6):
+        Minv[p:p + 6, p:p + 6] = np.linalg.inv(S[p:p + 6, p:p + 6])
+    x = np.zeros(n)
+    r = rhs.copy()
+    z = Minv @ r
+    d = z.copy()
+    rho = r @ z
+    for _ in range(k):
+        q = S @ d
+        alpha = rho / (d @ q)
+        x = x + alpha * d
+        r = r - alpha * q
+        z = Minv @ r
+        rho_next = r @ z
+        d = z + (rho_next / rho) * d
+        rho = rho_next
+    return x
+
+
+@pytest.mark.parametrize("k", [1, 3, 8])
+def test_pcg_iterate_at_the_cap(k):
+    """The at-cap rules of the PCG step: keeping the iterate (the GPU
+    backend, ba-gpu/src/solver.rs:497-508) returns exactly the k-th textbook
+    PCG iterate and its landmark completion; the CPU matrix-free rule
+    (implicit_schur.rs:715-736) fails the step with MaxIterations."""
+    lam = 0.1
+    ns = random_system(11)
+    H, b = full_normal(ns, lam)
+    assert np.linalg.eigvalsh(H).min() > 0.0
+    n = 6 * ns.P
+    A, B, D = H[:n, :n], H[:n, n:], H[n:, n:]
+    S = A - B @ np.linalg.solve(D, B.T)
+    rhs = b[:n] - B @ np.linalg.solve(D, b[n:])
+    xk = textbook_pcg(S, rhs, k)
+    exact = np.linalg.solve(S, rhs)
+    assert np.linalg.norm(xk - exact) > 1e-6 * np.linalg.norm(exact)  # the cap really cuts the solve
+    st, dp, dl, it, rn, tg = ns.solve_pcg(lam, max_iterations=k, rtol=0.0, atol=0.0, accept_at_cap=True)
+    assert st == 0 and it == k
+    assert np.linalg.norm(dp - xk) <= 1e-10 * np.linalg.norm(xk)
+    assert np.linalg.norm(dl - np.linalg.solve(D, b[n:] - B.T @ dp)) <= 1e-10 * np.linalg.norm(dl)
+    assert abs(rn - np.linalg.norm(rhs - S @ dp)) <= 1e-9 * np.linalg.norm(rhs)
+    if k > 1:  # not the previous iterate
+        assert np.linalg.norm(dp - textbook_pcg(S, rhs, k - 1)) > 1e-8 * np.linalg.norm(xk)
+    st, *_ , it, _, _ = ns.solve_pcg(lam, max_iterations=k, rtol=0.0, atol=0.0, accept_at_cap=False)
+    assert st == -7 and it == k
+    # converging before the cap is the same under both rules
+    st_a, dp_a, *_ = ns.solve_pcg(lam, max_iterations=500, rtol=1e-13, atol=0.0, accept_at_cap=True)
+    st_r, dp_r, *_ = ns.solve_pcg(lam, max_iterations=500, rtol=1e-13, atol=0.0, accept_at_cap=False)
+    assert st_a == st_r == 0 and np.array_equal(dp_a, dp_r)
+    assert np.linalg.norm(dp_a - exact) <= 1e-10 * np.linalg.norm(exact)
+
+
+def lambda_rule_violations(iterations, o):
+    """The damping bookkeeping of the LM loop, lm.rs:610-662 and 890-991:
+    a rejected or singular step raises lambda to min(lambda * increase,
+    max_lambda) and reports the raised value, and the loop stops once it
+    reaches max_lambda (after a cost/feasibility rejection also when there is
+    no initial lambda); an accepted step reports the lambda it was solved
+    with and then lowers it to max(lambda * decrease, min_lambda) (held at 0
+    without an initial lambda)."""
+    bad = []
+    lam = o.initial_lambda if o.has_initial_lambda else 0.0
+    for i, it in enumerate(iterations):
+        last = i == len(iterations) - 1
+        if it["step_accepted"]:
+            if it["lambda"] != lam:
+                bad.append(f"iteration {i}: accepted with lambda {it['lambda']}, solved with {lam}")
+            if o.has_initial_lambda:
+                lam = max(lam * o.lambda_decrease_factor, o.min_lambda)
+        else:
+            lam = min(lam * o.lambda_increase_factor, o.max_lambda)
+            if it["lambda"] != lam:
+                bad.append(f"iteration {i}: rejected, reports {it['lambda']}, expected {lam}")
+            if lam >= o.max_lambda and not last:
+                bad.append(f"iteration {i}: lambda reached max_lambda but the loop went on")
+    return bad
+
+
+def test_lambda_bookkeeping():
+    """Reported lambda per iteration, with clamping at both bounds and the
+    loop's stops, on scenes that accept, reject on feasibility and fail the
+    solve (singular at lambda = 0)."""
+    far = scenes.make_problem(scenes.pinhole_camera(), [(10, scenes.pose_at([0, 0, 0]))],
+                              [(1, np.array([1.0, 0.0, 0.1]))], [(10, 1, np.array([105320.0, 240.0]))],
+                              fixed_poses=(10,))
+    o = vba.default_options(max_iterations=10, initial_lambda=1e-4, lambda_increase_factor=10.0, max_lambda=5e-3)
+    out = oracle.optimize(far, o)
+    assert out.status == abi.OK and not lambda_rule_violations(out.iterations, o)
+    assert [it["lambda"] for it in out.iterations] == [min(1e-4 * 10.0, 5e-3), 5e-3]  # stops at the bound
+    # singular at lambda = 0 (one observation of a free landmark): the solve
+    # fails every time, lambda stays 0 and the loop runs to the budget
+    o = vba.default_options(max_iterations=10, has_initial_lambda=0)
+    out = oracle.optimize(far, o)
+    assert out.status == abi.OK and len(out.iterations) == 10
+    assert not lambda_rule_violations(out.iterations, o)
+    # a feasibility rejection without an initial lambda stops at once
+    two = scenes.make_problem(scenes.pinhole_camera(),
+                              [(10, scenes.pose_at([0, 0, 0])), (11, scenes.pose_at([0.3, 0, 0]))],
+                              [(1, np.array([1.0, 0.0, 0.1]))],
+                              [(10, 1, np.array([105320.0, 240.0])), (11, 1, np.array([-90000.0, 300.0]))],
+                              fixed_poses=(10, 11))
+    out = oracle.optimize(two, o)
+    assert out.status == abi.OK and len(out.iterations) == 1 and not out.iterations[0]["step_accepted"]
+    assert out.iterations[0]["cost_after"] != out.iterations[0]["cost_before"]  # a solved, rejected step
+    # accepted steps down to the lower bound
+    drift = scenes.truth_bundle(drift30=[1.3, 0.3, 0.2])
+    o = vba.default_options(max_iterations=20, initial_lambda=1e-4, lambda_decrease_factor=0.5, min_lambda=3e-5)
+    out = oracle.optimize(drift, o)
+    assert out.status == abi.OK and all(it["step_accepted"] for it in out.iterations)
+    assert not lambda_rule_violations(out.iterations, o)
+    assert [it["lambda"] for it in out.iterations][:4] == [1e-4, 5e-5, 3e-5, 3e-5]
+    # without an initial lambda accepted steps keep lambda at 0
+    o = vba.default_options(max_iterations=20, has_initial_lambda=0)
+    out = oracle.optimize(drift, o)
+    assert out.status == abi.OK and out.iterations and all(it["lambda"] == 0.0 for it in out.iterations)
+
+
+@pytest.mark.parametrize("name,policy", [("loop_accept", abi.PCG_MAXIT_ACCEPT_ITERATE),
+                                         ("loop_reject", abi.PCG_MAXIT_REJECT)])
+def test_lambda_bookkeeping_of_the_loop_traces(name, policy):
+    """The stored 1,200-camera loop traces (accepts, cost rejections and, under
+    the reject-at-cap rule, failed solves) follow the same damping rule."""
+    with np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "traces", f"{name}.npz"),
+                 allow_pickle=False) as z:
+        acc, lam = z["accepted"], z["lam"]
+    o = workloads.loop_options(policy=policy)
+    its = [{"step_accepted": bool(a), "lambda": float(l)} for a, l in zip(acc, lam)]
+    assert 0 < int(acc.sum()) < len(acc)
+    assert not lambda_rule_violations(its, o)
 
 
 def test_non_spd_preconditioner_is_rejected():
