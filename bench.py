@@ -786,6 +786,110 @@ def extra_city(dev, args, steps=2):
     return rec
 
 
+def quat_matrix(q):
+    w, x, y, z = q[:, 0], q[:, 1], q[:, 2], q[:, 3]
+    return np.stack([
+        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)], -1),
+        np.stack([2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)], -1),
+        np.stack([2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], -1)], -2)
+
+
+def rms_reprojection(problem, poses, landmarks):
+    """sqrt(robust_cost(None) / observations), the RMS reprojection error
+    ba_gpu_replay.rs:19-29 reports: squared pixel residuals of the
+    observations in front of their camera (problem.rs:497-526), for the
+    PINHOLE camera a problem dump holds."""
+    cam = problem.camera
+    if cam.model != abi.CAMERA_PINHOLE:
+        return None
+    fx, fy, cx, cy = (cam.params[i] for i in range(4))
+    pose_row = {int(i): k for k, i in enumerate(problem.pose_ids)}
+    lm_row = {int(i): k for k, i in enumerate(problem.landmark_ids)}
+    pi = np.fromiter((pose_row[int(i)] for i in problem.obs_pose_ids), np.int64, len(problem.obs_pose_ids))
+    li = np.fromiter((lm_row[int(i)] for i in problem.obs_landmark_ids), np.int64, len(problem.obs_landmark_ids))
+    R = quat_matrix(poses[:, :4])
+    xc = np.einsum("nij,nj->ni", R[pi], landmarks[li]) + poses[pi, 4:7]
+    front = xc[:, 2] > 0.0
+    z = np.where(front, xc[:, 2], 1.0)
+    r = np.stack([fx * xc[:, 0] / z + cx, fy * xc[:, 1] / z + cy], -1) - problem.obs_xy
+    s = np.where(front, (r * r).sum(-1), 0.0)
+    return float(np.sqrt(s.sum() / max(1, len(s))))
+
+
+def replay_record(name, seconds, out, problem):
+    acc = sum(1 for it in out.iterations if it["step_accepted"])
+    rms = rms_reprojection(problem, out.poses, out.landmarks) if out.status == abi.OK else None
+    log(f"{name}: {seconds:.2f}s, {len(out.iterations)} iterations ({acc} accepted), robust cost "
+        f"{out.result.initial_cost:.6e} -> {out.result.final_cost:.6e}, rms reproj "
+        f"{rms if rms is None else round(rms, 4)} px")
+    return {"status": out.status, "seconds": seconds, "iterations": len(out.iterations), "accepted": acc,
+            "initial_cost": out.result.initial_cost, "final_cost": out.result.final_cost, "rms_px": rms,
+            "accept_vector": [bool(it["step_accepted"]) for it in out.iterations]}
+
+
+def bench_replay(args):
+    """crates/ba-gpu/examples/ba_gpu_replay.rs:31-141 on this library: read
+    a VISLOC_SFM_BA_DUMP problem (vba_model_read_ba_problem), keep the local
+    window of its last N poses with --local N, solve it on the device after a
+    one-iteration warm-up, and, unless --no-cpu-baseline, with the CPU FP64
+    oracle (the reference's `ba.optimize`) on the same options; print the
+    reference's report lines and one JSON line. Options are the SfM's BA
+    config (IncrementalSfmConfig::default().ba_config: BaConfig::default()
+    with Huber delta = 3 px, incremental_sfm/mod.rs:648-651), with
+    --iterations as the reference's flag; --replay-solver picks the reduced
+    solve (dense is what BaConfig::default() asks for)."""
+    model = vba.Model.read_ba_problem(args.replay)
+    if args.local:
+        model.keep_local_window(args.local)
+    problem, weights = model.problem()
+    model.close()
+    fixed = int(np.count_nonzero(problem.pose_fixed))
+    log(f"problem: {len(problem.pose_ids)} poses ({fixed} fixed), {len(problem.landmark_ids)} landmarks, "
+        f"{problem.num_observations} observations (local window: {args.local or None})")
+    kw = dict(robust_kernel=abi.KERNEL_HUBER, robust_scale=3.0)
+    if args.iterations:
+        kw["max_iterations"] = args.iterations
+    if args.replay_solver == "matrix-free":
+        # the GPU backend's PCG settings (crates/ba-gpu/src/solver.rs:36-48, 497-508)
+        kw.update(linear_solver=abi.SOLVER_MATRIX_FREE, max_pcg_iterations=100, pcg_relative_tolerance=1e-4,
+                  pcg_maxit_policy=abi.PCG_MAXIT_ACCEPT_ITERATE)
+    elif args.replay_solver == "sparse":
+        kw["linear_solver"] = abi.SOLVER_SPARSE
+    options = vba.default_options(**kw)
+    t0 = time.perf_counter()
+    dev = vba.Device(0)
+    log(f"device init {time.perf_counter() - t0:.2f}s")
+    solve = (lambda o: dev.optimize_weighted(problem, o, weights)) if weights is not None else \
+        (lambda o: dev.optimize(problem, o))
+    warm = vba.default_options(**{f: getattr(options, f) for f, _ in abi.Options._fields_})
+    warm.max_iterations = 1
+    solve(warm)
+    t0 = time.perf_counter()
+    out = solve(options)
+    device = replay_record("mi355x", time.perf_counter() - t0, out, problem)
+    dev.close()
+    rec = {"metric": "replay", "dump": os.path.basename(args.replay), "local": args.local or None,
+           "poses": len(problem.pose_ids), "fixed_poses": fixed, "landmarks": len(problem.landmark_ids),
+           "observations": problem.num_observations, "solver": args.replay_solver,
+           "max_iterations": options.max_iterations, "device": device}
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+
+        nthr = cpu_threads()
+        t0 = time.perf_counter()
+        cpu = (oracle.optimize_weighted(problem, options, weights, threads=nthr) if weights is not None
+               else oracle.optimize(problem, options, threads=nthr))
+        rec["cpu"] = replay_record(f"cpu ({nthr} threads)", time.perf_counter() - t0, cpu, problem)
+        rec["cpu"]["threads"] = nthr
+        rec["same_accept_vector"] = rec["cpu"]["accept_vector"] == device["accept_vector"]
+        if cpu.status == abi.OK and out.status == abi.OK:
+            rec["rel_cost_vs_cpu"] = abs(out.result.final_cost - cpu.result.final_cost) / max(
+                abs(cpu.result.final_cost), 1e-300)
+            rec["speedup"] = rec["cpu"]["seconds"] / max(device["seconds"], 1e-12)
+    print(json.dumps(rec), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -812,6 +916,12 @@ def main():
     ap.add_argument("--profile-stride", type=int, default=16,
                     help="record kernel times around every n-th launch of each kernel family")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default PCG launch schedule")
+    ap.add_argument("--replay", default="",
+                    help="a problem dump (VISLOC_SFM_BA_DUMP) to solve on the device and the CPU, as ba_gpu_replay")
+    ap.add_argument("--local", type=int, default=0, help="--replay: keep the last N poses variable")
+    ap.add_argument("--iterations", type=int, default=0, help="--replay: LM iterations (default: BaConfig's)")
+    ap.add_argument("--replay-solver", choices=["dense", "sparse", "matrix-free"], default="dense",
+                    help="--replay: the reduced solve")
     ap.add_argument("--solver", choices=["matrix-free", "explicit"], default="matrix-free",
                     help="reduced camera PCG of the loop workload: implicit operator (the workload's own "
                          "setting) or the explicitly assembled block-sparse matrix")
@@ -820,6 +930,9 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (run with --gpus 1).",
               file=sys.stderr)
         sys.exit(2)
+    if args.replay:
+        bench_replay(args)
+        return
     if args.workload == "windows":
         bench_windows(args)
         return
