@@ -100,7 +100,12 @@ def test_gnc_auto_scale_parity(device):
 
 def test_gnc_with_outliers_on_a_loop(device):
     """GNC around the matrix-free solver on a loop scene with 2 % of the
-    observations replaced by wrong matches."""
+    observations replaced by wrong matches. The inner solves stop at PCG
+    rtol 1e-6, so the device and the oracle (with different rounding) end
+    within that tolerance of each other, and a truncated-least-squares weight
+    in the transition band moves with the residual at the band's slope
+    (sqrt(mu (mu + 1)) c / r^2, several hundred per pixel at the last level):
+    the bar is the GNC cost to 1e-6 relative and the weights to 1e-3."""
     s = vba.synthesize(abi.SYNTH_LOOP, num_cameras=40, num_landmarks=2000, seed=6, scene_scale=16.0)
     p = s.problem.copy()
     rng = np.random.default_rng(2)
@@ -115,4 +120,5 @@ def test_gnc_with_outliers_on_a_loop(device):
     assert st == abi.OK
     assert res.outer_iterations == ref[1].outer_iterations
     assert np.mean(w[bad] < 0.5) > 0.9
-    np.testing.assert_allclose(w, ref[4], atol=1e-4, rtol=0)
+    assert abs(res.final_cost - ref[1].final_cost) <= 1e-6 * ref[1].final_cost
+    np.testing.assert_allclose(w, ref[4], atol=1e-3, rtol=0)

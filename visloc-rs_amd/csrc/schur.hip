@@ -422,8 +422,10 @@ __global__ __launch_bounds__(kTileThreads) void k_schur_tile(DevProblem P, const
 //   2. owner lanes: u_j = V_j^-1 t_j;
 //   3. y_e = W_e u_j = [G u; X x (G u)] summed over each lane's positions
 //      run by run; a run that starts and ends inside one lane writes its
-//      partial at once, the others add into their run's LDS accumulator,
-//      which the lane holding the run's last position writes out.
+//      partial at once; a segment open at a lane's end leaves its sum in
+//      the lane's LDS slot, and the lane holding the run's last position
+//      adds the slots of the lanes back to the run's first one and writes
+//      the partial.
 // A tile of one landmark with more than kTileRuns edges (each its own run)
 // reduces t over the wave and writes one partial per edge.
 // ---------------------------------------------------------------------------
@@ -630,13 +632,15 @@ __device__ __forceinline__ bool apply_tile(const DevProblem& P, const double* __
         sT[3 * lane + 1] = u[1];
         sT[3 * lane + 2] = u[2];
     }
-    // the run accumulators replace V^-1: this wave's reads of it above come
-    // first in its LDS order
-#pragma unroll
-    for (int c = 0; c < 6; ++c) sR[6 * lane + c] = 0.0;
     wave_lds_sync();
 
     // ---- stage 3: y_e = W_e u_j summed per camera run ----
+    // A run that starts and ends inside the lane writes its partial at once.
+    // A segment open at the lane's end (the head of a run, or a lane inside
+    // one) leaves its sum in the lane's LDS slot (V^-1's place, now free);
+    // the lane that finishes the run adds the slots of the lanes before it,
+    // back to the run's first lane, to its own leading sum: plain LDS
+    // traffic in a fixed order, no atomics.
     const int prev_key = __shfl_up(key[kApplyPer - 1], 1, 64);  // last key of the lane before
     const int next_key = __shfl_down(key[0], 1, 64);             // first key of the lane after
     bool head[kApplyPer], fin[kApplyPer];
@@ -647,22 +651,9 @@ __device__ __forceinline__ bool apply_tile(const DevProblem& P, const double* __
         head[i] = i < nv && key[i] != pk;
         fin[i] = i < nv && (s0 + i == n - 1 || key[i] != nk);
     }
-    // run index of each position within the tile: run starts before it
-    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    int rid[kApplyPer];
-    {
-        int cnt = 0;
-#pragma unroll
-        for (int i = 0; i < kApplyPer; ++i) cnt += __popcll(__ballot(head[i]) & below);
-#pragma unroll
-        for (int i = 0; i < kApplyPer; ++i) {
-            cnt += head[i] ? 1 : 0;
-            rid[i] = cnt - 1;
-        }
-    }
-    double acc[6];
+    double acc[6], lead_acc[6];
     bool own = false;   // the current segment's run starts in this lane
-    int carried = -1;   // position whose run started in an earlier lane and ends here
+    int lead = -1;      // position whose run started in an earlier lane and ends here
 #pragma unroll
     for (int i = 0; i < kApplyPer; ++i) {
         if (i < nv) {
@@ -686,30 +677,38 @@ __device__ __forceinline__ bool apply_tile(const DevProblem& P, const double* __
                     double* o = part + 6 * (size_t)slot[i];
 #pragma unroll
                     for (int c = 0; c < 6; ++c) o[c] = acc[c];
-                } else if (rid[i] < kTileRuns) {
+                } else {  // only a lane's first segment can have started earlier
+                    lead = i;
 #pragma unroll
-                    for (int c = 0; c < 6; ++c) atomicAdd(&sR[6 * rid[i] + c], acc[c]);
-                    carried = i;
+                    for (int c = 0; c < 6; ++c) lead_acc[c] = acc[c];
                 }
             }
         }
     }
-    // a segment open at the lane's end continues in the next lane
-    if (nv > 0 && !fin[nv - 1] && rid[nv - 1] < kTileRuns) {
-        const int r = rid[nv - 1];
+    const bool open = nv > 0 && !fin[nv - 1];
+    if (open) {
 #pragma unroll
-        for (int c = 0; c < 6; ++c) atomicAdd(&sR[6 * r + c], acc[c]);
+        for (int c = 0; c < 6; ++c) sR[6 * lane + c] = acc[c];
     }
+    // lanes whose open segment is the start of its run
+    const uint64_t starts = __ballot(open && own);
     wave_lds_sync();
-    if (carried >= 0) {
-        const int r = rid[carried];
+    if (lead >= 0) {
+        const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+        const int h = 63 - __builtin_clzll(starts & below);  // the run's first lane
+        double t6[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) t6[c] = lead_acc[c];
+        for (int l = lane - 1; l >= h; --l)
+#pragma unroll
+            for (int c = 0; c < 6; ++c) t6[c] += sR[6 * l + c];
         int sl = slot[0];
 #pragma unroll
         for (int i = 1; i < kApplyPer; ++i)
-            if (i == carried) sl = slot[i];
+            if (i == lead) sl = slot[i];
         double* o = part + 6 * (size_t)sl;
 #pragma unroll
-        for (int c = 0; c < 6; ++c) o[c] = sR[6 * r + c];
+        for (int c = 0; c < 6; ++c) o[c] = t6[c];
     }
     return true;
 }
@@ -861,13 +860,15 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_schur_apply(DevProblem P, 
         sT[3 * lane + 1] = u[1];
         sT[3 * lane + 2] = u[2];
     }
-    // the run accumulators replace V^-1: this wave's reads of it above come
-    // first in its LDS order
-#pragma unroll
-    for (int c = 0; c < 6; ++c) sR[6 * lane + c] = 0.0;
     wave_lds_sync();
 
     // ---- stage 3: y_e = W_e u_j summed per camera run ----
+    // A run that starts and ends inside the lane writes its partial at once.
+    // A segment open at the lane's end (the head of a run, or a lane inside
+    // one) leaves its sum in the lane's LDS slot (V^-1's place, now free);
+    // the lane that finishes the run adds the slots of the lanes before it,
+    // back to the run's first lane, to its own leading sum: plain LDS
+    // traffic in a fixed order, no atomics.
     const int prev_key = __shfl_up(key[kApplyPer - 1], 1, 64);  // last key of the lane before
     const int next_key = __shfl_down(key[0], 1, 64);             // first key of the lane after
     bool head[kApplyPer], fin[kApplyPer];
@@ -878,22 +879,9 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_schur_apply(DevProblem P, 
         head[i] = i < nv && key[i] != pk;
         fin[i] = i < nv && (s0 + i == n - 1 || key[i] != nk);
     }
-    // run index of each position within the tile: run starts before it
-    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    int rid[kApplyPer];
-    {
-        int cnt = 0;
-#pragma unroll
-        for (int i = 0; i < kApplyPer; ++i) cnt += __popcll(__ballot(head[i]) & below);
-#pragma unroll
-        for (int i = 0; i < kApplyPer; ++i) {
-            cnt += head[i] ? 1 : 0;
-            rid[i] = cnt - 1;
-        }
-    }
-    double acc[6];
+    double acc[6], lead_acc[6];
     bool own = false;   // the current segment's run starts in this lane
-    int carried = -1;   // position whose run started in an earlier lane and ends here
+    int lead = -1;      // position whose run started in an earlier lane and ends here
 #pragma unroll
     for (int i = 0; i < kApplyPer; ++i) {
         if (i < nv) {
@@ -917,30 +905,38 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_schur_apply(DevProblem P, 
                     double* o = part + 6 * (size_t)slot[i];
 #pragma unroll
                     for (int c = 0; c < 6; ++c) o[c] = acc[c];
-                } else if (rid[i] < kTileRuns) {
+                } else {  // only a lane's first segment can have started earlier
+                    lead = i;
 #pragma unroll
-                    for (int c = 0; c < 6; ++c) atomicAdd(&sR[6 * rid[i] + c], acc[c]);
-                    carried = i;
+                    for (int c = 0; c < 6; ++c) lead_acc[c] = acc[c];
                 }
             }
         }
     }
-    // a segment open at the lane's end continues in the next lane
-    if (nv > 0 && !fin[nv - 1] && rid[nv - 1] < kTileRuns) {
-        const int r = rid[nv - 1];
+    const bool open = nv > 0 && !fin[nv - 1];
+    if (open) {
 #pragma unroll
-        for (int c = 0; c < 6; ++c) atomicAdd(&sR[6 * r + c], acc[c]);
+        for (int c = 0; c < 6; ++c) sR[6 * lane + c] = acc[c];
     }
+    // lanes whose open segment is the start of its run
+    const uint64_t starts = __ballot(open && own);
     wave_lds_sync();
-    if (carried >= 0) {
-        const int r = rid[carried];
+    if (lead >= 0) {
+        const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+        const int h = 63 - __builtin_clzll(starts & below);  // the run's first lane
+        double t6[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) t6[c] = lead_acc[c];
+        for (int l = lane - 1; l >= h; --l)
+#pragma unroll
+            for (int c = 0; c < 6; ++c) t6[c] += sR[6 * l + c];
         int sl = slot[0];
 #pragma unroll
         for (int i = 1; i < kApplyPer; ++i)
-            if (i == carried) sl = slot[i];
+            if (i == lead) sl = slot[i];
         double* o = part + 6 * (size_t)sl;
 #pragma unroll
-        for (int c = 0; c < 6; ++c) o[c] = sR[6 * r + c];
+        for (int c = 0; c < 6; ++c) o[c] = t6[c];
     }
 }
 
