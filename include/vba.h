@@ -524,7 +524,8 @@ typedef struct vba_tuning {
     uint32_t pcg_predict;
     /* 0: the PCG product launches one wave per landmark tile; n (1..32): a
      * persistent product of n waves per compute unit looping over the tiles
-     * (the same results bit for bit) */
+     * (the same results bit for bit); mixed precision only, the FP64
+     * product always takes one wave per tile */
     uint32_t apply_waves_per_cu;
 } vba_tuning;
 /* NULL restores the defaults. VBA_ERR_INVALID_ARGUMENT for a field out of

@@ -412,18 +412,28 @@ __global__ __launch_bounds__(kBlock) void k_cost(DevProblem P, int trial) {
 
 __global__ __launch_bounds__(kSmallBlock) void k_scalars_finalize(DevProblem P, int num_pose_blocks) {
     __shared__ double lds[(kSmallBlock / 64) * 4];
-    // cost partials in tile order: each thread merges a contiguous run
+    // cost partials in tile order: thread t < parts merges a contiguous run
+    // of per tiles, and thread 0 folds the parts in order. Up to
+    // kSmallBlock tiles a part is one tile (the fold is then the tile-order
+    // fold that k_batch_scalars does for a window); beyond, 128 parts keep
+    // the serial fold short.
     const int n = P.num_cam_tiles;
-    const int per = (n + kSmallBlock - 1) / kSmallBlock;
+    const int parts = n <= kSmallBlock ? n : 128;
+    const int per = parts > 0 ? (n + parts - 1) / parts : 0;
     CSum cs{0.0, 0.0};
     double bad = 0.0, mp = 0.0, ml = 0.0;
+    // (unrolled so that each thread's loads are in flight together; the
+    // merge order is the loop order)
+#pragma unroll 8
     for (int i = threadIdx.x * per; i < min(n, (int)(threadIdx.x + 1) * per); ++i) {
         const double* t = P.tile_scalar + 4 * (size_t)i;
         cs = csum_merge(cs, CSum{t[0], t[1]});
         bad += t[2];
     }
+#pragma unroll 8
     for (int i = threadIdx.x; i < P.num_lm_tiles; i += kSmallBlock)
         ml = fmax(ml, P.tile_scalar[4 * (size_t)(n + i) + 3]);
+#pragma unroll 8
     for (int i = threadIdx.x; i < num_pose_blocks; i += kSmallBlock)
         mp = fmax(mp, P.tile_scalar[4 * (size_t)(n + P.num_lm_tiles + i) + 3]);
     // ordered merge across threads through LDS (thread order = tile order)
@@ -437,7 +447,7 @@ __global__ __launch_bounds__(kSmallBlock) void k_scalars_finalize(DevProblem P, 
     __syncthreads();
     if (threadIdx.x == 0) {
         CSum a{ss[0], sc[0]};
-        for (int i = 1; i < kSmallBlock; ++i) a = csum_merge(a, CSum{ss[i], sc[i]});
+        for (int i = 1; i < parts; ++i) a = csum_merge(a, CSum{ss[i], sc[i]});
         StepScalars* o = P.scalars;
         o->cost_sum = a.s;
         o->cost_comp = a.c;
