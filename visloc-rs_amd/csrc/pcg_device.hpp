@@ -83,31 +83,61 @@ __device__ __forceinline__ void wave_pair_total(const double* __restrict__ a, in
     *s1 = wave_allsum(v1);
 }
 
+// Inputs of the PCG scalars of iteration k, loaded with no branch in
+// between so that a kernel can issue them together with its own loads: the
+// stop flag, the step kernel's (r.z, r.r) partial `lane` and the state fields
+// the stop test reads.
+struct PrologueIn {
+    int stop, max_iter;
+    double rz0, rr0;  // partial min(lane, num_cpart - 1); lanes past the end drop it
+    double rnorm, target, rho_prev;
+};
+
+// No value is used here: a use next to its load would wait for it.
+__device__ __forceinline__ PrologueIn pcg_prologue_load(const DevProblem& P, int k) {
+    const PcgState* st = P.pcg;
+    const int i = max(0, min((int)(threadIdx.x & 63), P.num_cpart - 1));
+    PrologueIn in;
+    in.stop = __hip_atomic_load(&P.pcg->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    in.rz0 = P.cpart[2 * (size_t)i];
+    in.rr0 = P.cpart[2 * (size_t)i + 1];
+    in.rnorm = st->rnorm;
+    in.target = st->target;
+    in.rho_prev = st->rho[(k - 1) & 1];
+    in.max_iter = st->max_iter;
+    return in;
+}
+
 // Scalars of PCG iteration k, derived by every wave on its own without
 // synchronisation: r.z and r.r of the current residual from the step
 // kernel's partials (or the totals the init / restart kernels left in entry
 // 0), the stop test on the recursive residual (implicit_schur.rs:640-700)
 // and beta = rho_k / rho_{k-1}. Thread 0 of work-group 0 records them.
 // Returns false when the solve has stopped; the result is the same in every
-// wave of the launch.
-__device__ bool pcg_prologue(const DevProblem& P, int k, int fresh, double* beta_out) {
+// wave of the launch. The partials are summed as wave_pair_total sums them.
+__device__ __forceinline__ bool pcg_prologue_finish(const DevProblem& P, int k, int fresh, const PrologueIn& in,
+                                                    double* beta_out) {
     PcgState* st = P.pcg;
-    // the flag and the partials are loaded together; the flag decides after
-    const int stop = __hip_atomic_load(&st->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    double rz, rr;
-    wave_pair_total(P.cpart, P.num_cpart, &rz, &rr);
-    if (__builtin_amdgcn_readfirstlane(stop)) return false;
+    const int lane = threadIdx.x & 63;
+    const bool own = lane < P.num_cpart;
+    double v0 = 0.0 + (own ? in.rz0 : 0.0), v1 = 0.0 + (own ? in.rr0 : 0.0);
+    for (int i = lane + 64; i < P.num_cpart; i += 64) {
+        v0 += P.cpart[2 * (size_t)i];
+        v1 += P.cpart[2 * (size_t)i + 1];
+    }
+    const double rz = wave_allsum(v0), rr = wave_allsum(v1);
+    if (__builtin_amdgcn_readfirstlane(in.stop)) return false;
     double rn, beta = 0.0;
     int status = PCG_RUNNING;
     if (fresh) {
-        rn = st->rnorm;
+        rn = in.rnorm;
     } else {
         rn = sqrt(rr);
         if (!isfinite(rn)) status = PCG_NONFINITE;
-        else if (rn <= st->target) status = PCG_CONVERGED;
-        else if (k >= st->max_iter) status = PCG_MAXIT;
+        else if (rn <= in.target) status = PCG_CONVERGED;
+        else if (k >= in.max_iter) status = PCG_MAXIT;
         else if (!isfinite(rz) || !(rz > 0.0)) status = PCG_NONPOSITIVE;
-        beta = rz / st->rho[(k - 1) & 1];
+        beta = rz / in.rho_prev;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         st->rho[k & 1] = rz;
@@ -118,6 +148,10 @@ __device__ bool pcg_prologue(const DevProblem& P, int k, int fresh, double* beta
     }
     *beta_out = beta;
     return status == PCG_RUNNING;
+}
+
+__device__ bool pcg_prologue(const DevProblem& P, int k, int fresh, double* beta_out) {
+    return pcg_prologue_finish(P, k, fresh, pcg_prologue_load(P, k), beta_out);
 }
 
 // Sum of one value per wave into a per-work-group partial, in wave order.

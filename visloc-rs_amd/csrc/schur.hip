@@ -3,19 +3,18 @@
 // tiles, its right-hand side, block-Jacobi blocks and back-substitution,
 // and the PCG vector work over poses (implicit_schur.rs:116-824).
 //
-// Landmark-tile pass: one work-group per tile (<= 256 whole landmarks,
-// <= 1024 l-order edges; a landmark with more edges is a tile of its own),
-// each edge record read once, coalesced:
-//   1. lane per edge: t_e = W_e^T d_p = G (rho + omega x X) into LDS;
-//   2. each landmark's owner lane sums its edges in order: u_l = V_l^-1 t;
-//   3. the tile's edges in pose-sorted order (tile_perm) give
-//      y_e = W_e u_l = [G u; X x (G u)]; a wave-level segmented scan keyed
-//      by pose sums each camera run, runs crossing 64-edge chunks are joined
-//      through LDS in chunk order, and every run writes one partial.
-// A pose then sums its runs' partials in ascending order. There are no
-// atomics: every sum has a fixed order, so results do not depend on
-// scheduling, and every edge's G is read from HBM once per operator
-// application instead of once per landmark pass and once per camera pass.
+// Landmark tiles: runs of whole landmarks with at most 256 edges, 64
+// landmarks and 64 distinct poses (a landmark with more than 64 edges is a
+// tile of its own), edges sorted by pose inside the tile. Every pass reads
+// each edge's G once:
+//   1. t_l = sum over the landmark's edges of W_e^T d_p = G (rho + omega x X);
+//   2. u_l = V_l^-1 t_l;
+//   3. y_e = W_e u_l = [G u; X x (G u)], summed per camera run (the tile's
+//      edges of one pose), each run writing one partial to its slot.
+// A pose then sums its runs' partials in ascending slot order. Every sum has
+// a fixed order, so results do not depend on scheduling. k_schur_tile serves
+// the right-hand side, the block-Jacobi blocks and back-substitution;
+// k_schur_apply is the PCG product, one wave per tile.
 //
 // PCG (implicit_schur.rs:522-736) runs as three kernels per iteration:
 // the tile pass (whose work-groups first derive beta and the stop test
@@ -496,64 +495,57 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-// One landmark tile of the operator, by one wave. sX0 (3 * 64 + 2 doubles,
-// 16-byte aligned), sT (3 * 64) and sVR (6 * 64, 16-byte aligned) are the
-// wave's LDS. With `prologue` the PCG scalars of iteration pcg_iter are
-// derived here, after the tile's loads are issued (false: the solve has
-// stopped); without it `beta` comes in.
+// One landmark tile of the operator, by one wave, with the PCG scalar beta
+// given. sX0 (3 * 64 + 2 doubles, 16-byte aligned), sT (3 * 64) and sVR
+// (6 * 64, 16-byte aligned) are the wave's LDS.
 template <typename TS>
-__device__ __forceinline__ bool apply_tile(const DevProblem& P, const double* __restrict__ v1,
+__device__ __forceinline__ void apply_tile(const DevProblem& P, const double* __restrict__ v1,
                                            const double* __restrict__ v2, const TS* __restrict__ Gl,
                                            const double* __restrict__ Vinv, double* __restrict__ part,
-                                           int tile, const LmTile lt, int pcg_iter, int fresh, bool prologue,
-                                           double beta, double* sX0, double* const sT, double* const sVR) {
+                                           int tile, const LmTile lt, double beta, double* sX0,
+                                           double* const sT, double* const sVR) {
     double* sX = sX0;
     double* const sV = sVR;
     double* const sR = sVR;
-    if (P.wstate && !P.wstate[P.lm_tile_win[tile]].active) return true;
+    if (P.wstate && !P.wstate[P.lm_tile_win[tile]].active) return;
     const int lane = threadIdx.x & 63;
     const int e0 = lt.e_begin, g0 = kTileEdges * lt.g_chunk;
     const int n = lt.e_end - lt.e_begin, nl = lt.lm_end - lt.lm_begin;
     const bool one = nl == 1 && n > kTileRuns;
 
-    // ---- stage 0: the tile's data, loaded before the PCG scalars so that
-    // both are in flight together (a stopped launch drops them) ----
+    // ---- stage 0: the tile's data, the inputs of the PCG scalars, V^-1 and
+    // X, and the first pose's vectors, all issued before the wave waits on
+    // any of them (a stopped launch drops them) ----
+    // Every lane loads four positions, clamped to the tile's last one (to the
+    // problem's first edge for a tile without edges), with no branch around
+    // the loads: a value used inside the branch of its own load is waited for
+    // there, before the loads after it are issued. Positions from nv on are
+    // never read.
     const int s0 = kApplyPer * lane;
     const int nv = one ? 0 : max(0, min(kApplyPer, n - s0));
-    static_assert(kApplyPer <= 4 && kTileLandmarks <= 256, "local landmarks packed in one word");
-    int key[kApplyPer], slot[kApplyPer];
-    uint32_t jl4 = 0;  // the positions' local landmarks, a byte each
+    static_assert(kApplyPer <= 4, "four positions per lane");
+    int key[kApplyPer], slot[kApplyPer], lmb[kApplyPer];  // pose, run slot, tile-local landmark
     TS G[kApplyPer][6];
 #pragma unroll
     for (int i = 0; i < kApplyPer; ++i) {
-        key[i] = -1 - (s0 + i);  // unique for positions past the end
-        slot[i] = -1;
-        if (i < nv) {
-            key[i] = P.s_pose[e0 + s0 + i];
-            jl4 |= (uint32_t)P.s_lm[e0 + s0 + i] << (8 * i);
-            slot[i] = P.run_slot[e0 + s0 + i];  // read here, used by the run's partial at the end
-            if constexpr (std::is_same<TS, float>::value) {
-                // slot i of the tile's lane-major chunk: 1 KiB + 512 B per wave
-                const float* gb = Gl + (size_t)lt.g_chunk * kG32Chunk + 384 * i;
-                const float4 a = *(const float4*)(gb + 4 * lane);
-                const float2 b = *(const float2*)(gb + 256 + 2 * lane);
-                G[i][0] = a.x;
-                G[i][1] = a.y;
-                G[i][2] = a.z;
-                G[i][3] = a.w;
-                G[i][4] = b.x;
-                G[i][5] = b.y;
-            } else {
-                load6(Gl + 6 * (size_t)(e0 + s0 + i), G[i]);
-            }
+        const int e = n > 0 ? e0 + min(s0 + i, n - 1) : 0;
+        key[i] = P.s_pose[e];
+        lmb[i] = P.s_lm[e];
+        slot[i] = P.run_slot[e];  // used by the run's partial at the end
+        if constexpr (std::is_same<TS, float>::value) {
+            // slot i of the tile's lane-major chunk: 1 KiB + 512 B per wave
+            const float* gb = Gl + (size_t)lt.g_chunk * kG32Chunk + 384 * i;
+            const float4 a = *(const float4*)(gb + 4 * lane);
+            const float2 b = *(const float2*)(gb + 256 + 2 * lane);
+            G[i][0] = a.x;
+            G[i][1] = a.y;
+            G[i][2] = a.z;
+            G[i][3] = a.w;
+            G[i][4] = b.x;
+            G[i][5] = b.y;
+        } else {
+            load6(Gl + 6 * (size_t)e, G[i]);
         }
-    }
-    // the z and d of this lane's first pose, in flight with the tile data
-    // (most lanes' positions belong to one camera run)
-    double z0[6], w0[6];
-    if (nv > 0) {
-        load6(v1 + 6 * (size_t)key[0], z0);
-        if (v2) load6(v2 + 6 * (size_t)key[0], w0);
     }
     // V^-1 and X of the tile's landmarks go straight from global memory into
     // LDS (their layout there is the global one), in flight together with
@@ -578,13 +570,15 @@ __device__ __forceinline__ bool apply_tile(const DevProblem& P, const double* __
             if (1024 * c + 16 * lane < 8 * xs + 24 * nl)
                 __builtin_amdgcn_global_load_lds(gx + 1024 * c + 16 * lane, (lds_void_ptr)(lx + 1024 * c), 16, 0, 0);
     }
-    if (prologue && pcg_iter >= 0 && !pcg_prologue(P, pcg_iter, fresh, &beta)) {
-        __builtin_amdgcn_s_waitcnt(0);  // no LDS write may land after the wave ends
-        return false;
-    }
+    // the z and d of this lane's first pose (most lanes' positions belong to
+    // one camera run): the wave's first wait is the one for key[0]
+    const int kp = nv > 0 ? key[0] : 0;
+    double z0[6], w0[6];
+    load6(v1 + 6 * (size_t)kp, z0);
+    if (v2) load6(v2 + 6 * (size_t)kp, w0);
     if (one) {
         apply_one_landmark<TS>(P, v1, v2, beta, Gl, Vinv, part, lt.lm_begin, e0, g0, n, lane);
-        return true;
+        return;
     }
     if (lane < nl) {
 #pragma unroll
@@ -609,7 +603,7 @@ __device__ __forceinline__ bool apply_tile(const DevProblem& P, const double* __
                     }
                     pcur = key[i];
                 }
-                const int j = (jl4 >> (8 * i)) & 0xff;
+                const int j = lmb[i];
                 const double X[3] = {sX[3 * j], sX[3 * j + 1], sX[3 * j + 2]};
                 double t[3];
                 edge_wtd(G[i], d, X, t);
@@ -657,7 +651,7 @@ __device__ __forceinline__ bool apply_tile(const DevProblem& P, const double* __
 #pragma unroll
     for (int i = 0; i < kApplyPer; ++i) {
         if (i < nv) {
-            const int j = (jl4 >> (8 * i)) & 0xff;
+            const int j = lmb[i];
             const double X[3] = {sX[3 * j], sX[3 * j + 1], sX[3 * j + 2]};
             const double u[3] = {sT[3 * j], sT[3 * j + 1], sT[3 * j + 2]};
             const double Gd[6] = {(double)G[i][0], (double)G[i][1], (double)G[i][2],
@@ -710,7 +704,6 @@ __device__ __forceinline__ bool apply_tile(const DevProblem& P, const double* __
 #pragma unroll
         for (int c = 0; c < 6; ++c) o[c] = t6[c];
     }
-    return true;
 }
 
 template <typename TS>
@@ -743,45 +736,42 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_schur_apply(DevProblem P, 
     const int n = lt.e_end - lt.e_begin, nl = lt.lm_end - lt.lm_begin;
     const bool one = nl == 1 && n > kTileRuns;
 
-    // ---- stage 0: the tile's data, loaded before the PCG scalars so that
-    // both are in flight together (a stopped launch drops them) ----
+    // ---- stage 0: the tile's data, the inputs of the PCG scalars, V^-1 and
+    // X, and the first pose's vectors, all issued before the wave waits on
+    // any of them (a stopped launch drops them) ----
+    // Every lane loads four positions, clamped to the tile's last one (to the
+    // problem's first edge for a tile without edges), with no branch around
+    // the loads: a value used inside the branch of its own load is waited for
+    // there, before the loads after it are issued. Positions from nv on are
+    // never read.
     const int s0 = kApplyPer * lane;
     const int nv = one ? 0 : max(0, min(kApplyPer, n - s0));
-    static_assert(kApplyPer <= 4 && kTileLandmarks <= 256, "local landmarks packed in one word");
-    int key[kApplyPer], slot[kApplyPer];
-    uint32_t jl4 = 0;  // the positions' local landmarks, a byte each
+    static_assert(kApplyPer <= 4, "four positions per lane");
+    int key[kApplyPer], slot[kApplyPer], lmb[kApplyPer];  // pose, run slot, tile-local landmark
     TS G[kApplyPer][6];
 #pragma unroll
     for (int i = 0; i < kApplyPer; ++i) {
-        key[i] = -1 - (s0 + i);  // unique for positions past the end
-        slot[i] = -1;
-        if (i < nv) {
-            key[i] = P.s_pose[e0 + s0 + i];
-            jl4 |= (uint32_t)P.s_lm[e0 + s0 + i] << (8 * i);
-            slot[i] = P.run_slot[e0 + s0 + i];  // read here, used by the run's partial at the end
-            if constexpr (std::is_same<TS, float>::value) {
-                // slot i of the tile's lane-major chunk: 1 KiB + 512 B per wave
-                const float* gb = Gl + (size_t)lt.g_chunk * kG32Chunk + 384 * i;
-                const float4 a = *(const float4*)(gb + 4 * lane);
-                const float2 b = *(const float2*)(gb + 256 + 2 * lane);
-                G[i][0] = a.x;
-                G[i][1] = a.y;
-                G[i][2] = a.z;
-                G[i][3] = a.w;
-                G[i][4] = b.x;
-                G[i][5] = b.y;
-            } else {
-                load6(Gl + 6 * (size_t)(e0 + s0 + i), G[i]);
-            }
+        const int e = n > 0 ? e0 + min(s0 + i, n - 1) : 0;
+        key[i] = P.s_pose[e];
+        lmb[i] = P.s_lm[e];
+        slot[i] = P.run_slot[e];  // used by the run's partial at the end
+        if constexpr (std::is_same<TS, float>::value) {
+            // slot i of the tile's lane-major chunk: 1 KiB + 512 B per wave
+            const float* gb = Gl + (size_t)lt.g_chunk * kG32Chunk + 384 * i;
+            const float4 a = *(const float4*)(gb + 4 * lane);
+            const float2 b = *(const float2*)(gb + 256 + 2 * lane);
+            G[i][0] = a.x;
+            G[i][1] = a.y;
+            G[i][2] = a.z;
+            G[i][3] = a.w;
+            G[i][4] = b.x;
+            G[i][5] = b.y;
+        } else {
+            load6(Gl + 6 * (size_t)e, G[i]);
         }
     }
-    // the z and d of this lane's first pose, in flight with the tile data
-    // (most lanes' positions belong to one camera run)
-    double z0[6], w0[6];
-    if (nv > 0) {
-        load6(v1 + 6 * (size_t)key[0], z0);
-        if (v2) load6(v2 + 6 * (size_t)key[0], w0);
-    }
+    // the PCG state and partials are bound whenever the operator runs
+    const PrologueIn pin = pcg_prologue_load(P, pcg_iter);
     // V^-1 and X of the tile's landmarks go straight from global memory into
     // LDS (their layout there is the global one), in flight together with
     // the loads above: 16-byte pieces, lane i's piece at byte 16 i of each
@@ -805,8 +795,14 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_schur_apply(DevProblem P, 
             if (1024 * c + 16 * lane < 8 * xs + 24 * nl)
                 __builtin_amdgcn_global_load_lds(gx + 1024 * c + 16 * lane, (lds_void_ptr)(lx + 1024 * c), 16, 0, 0);
     }
+    // the z and d of this lane's first pose (most lanes' positions belong to
+    // one camera run): the wave's first wait is the one for key[0]
+    const int kp = nv > 0 ? key[0] : 0;
+    double z0[6], w0[6];
+    load6(v1 + 6 * (size_t)kp, z0);
+    if (v2) load6(v2 + 6 * (size_t)kp, w0);
     double beta = 0.0;
-    if (pcg_iter >= 0 && !pcg_prologue(P, pcg_iter, fresh, &beta)) {
+    if (pcg_iter >= 0 && !pcg_prologue_finish(P, pcg_iter, fresh, pin, &beta)) {
         __builtin_amdgcn_s_waitcnt(0);  // no LDS write may land after the wave ends
         return;
     }
@@ -837,7 +833,7 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_schur_apply(DevProblem P, 
                     }
                     pcur = key[i];
                 }
-                const int j = (jl4 >> (8 * i)) & 0xff;
+                const int j = lmb[i];
                 const double X[3] = {sX[3 * j], sX[3 * j + 1], sX[3 * j + 2]};
                 double t[3];
                 edge_wtd(G[i], d, X, t);
@@ -885,7 +881,7 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_schur_apply(DevProblem P, 
 #pragma unroll
     for (int i = 0; i < kApplyPer; ++i) {
         if (i < nv) {
-            const int j = (jl4 >> (8 * i)) & 0xff;
+            const int j = lmb[i];
             const double X[3] = {sX[3 * j], sX[3 * j + 1], sX[3 * j + 2]};
             const double u[3] = {sT[3 * j], sT[3 * j + 1], sT[3 * j + 2]};
             const double Gd[6] = {(double)G[i][0], (double)G[i][1], (double)G[i][2],
@@ -968,7 +964,7 @@ __global__ __launch_bounds__(64, 4) void k_schur_apply_loop(DevProblem P, const 
         const int next = tile + stride;
         LmTile ln{};
         if (next < t1) ln = P.lm_tiles[next];
-        (void)apply_tile<TS>(P, v1, v2, Gl, Vinv, part, tile, lt, -1, 0, false, beta, sXa, sTa, sVRa);
+        apply_tile<TS>(P, v1, v2, Gl, Vinv, part, tile, lt, beta, sXa, sTa, sVRa);
         if (next >= t1) break;
         __builtin_amdgcn_s_waitcnt(0);  // this tile's LDS traffic is done before the next tile's loads land
         wave_lds_sync();
