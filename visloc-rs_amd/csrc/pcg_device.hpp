@@ -63,17 +63,17 @@ __device__ __forceinline__ double wave_allsum(double v) {
 __device__ __forceinline__ void wave_pair_total(const double* __restrict__ a, int n, double* s0, double* s1) {
     const int lane = threadIdx.x & 63;
     double v0 = 0.0, v1 = 0.0;
-    // lane k adds pairs k, k + 64, ... in that order, four loads at a time
-    for (int i0 = lane; i0 < n; i0 += 256) {
-        double a0[4], a1[4];
+    // lane k adds pairs k, k + 64, ... in that order, eight loads at a time
+    for (int i0 = lane; i0 < n; i0 += 512) {
+        double a0[8], a1[8];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < 8; ++u) {
             const int i = i0 + 64 * u;
             a0[u] = i < n ? a[2 * (size_t)i] : 0.0;
             a1[u] = i < n ? a[2 * (size_t)i + 1] : 0.0;
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < 8; ++u)
             if (i0 + 64 * u < n) {
                 v0 += a0[u];
                 v1 += a1[u];

@@ -1134,7 +1134,9 @@ __global__ __launch_bounds__(kBlock) void k_pcg_pose(DevProblem P, const double*
     const int c = lane < 6 ? lane : 0;
     const bool in = p < P.num_pose_slots;
     const size_t k = 6 * (size_t)(in ? p : 0) + c;
-    const bool var = in && P.pose_var[in ? p : 0] >= 0;
+    // every load is issued before any loaded value is used (a use next to
+    // its load would wait for it there)
+    const int pv = P.pose_var[in ? p : 0];
     const double z = P.z[k], d_old = P.d[k];
     double u[6];
     const double* U = P.U + 21 * (size_t)(in ? p : 0);
@@ -1144,7 +1146,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_pose(DevProblem P, const double*
     pose_run_sum<6>(P, part, in ? p : 0, lane, s);
     if (__builtin_amdgcn_readfirstlane(stop)) return;
     double v[2] = {0.0, 0.0};
-    if (var) {
+    if (in && pv >= 0) {
         const double dc = z + beta * d_old;
         double acc = 0.0;
 #pragma unroll
@@ -1166,22 +1168,24 @@ __global__ __launch_bounds__(kBlock) void k_pcg_pose(DevProblem P, const double*
 __global__ __launch_bounds__(kBlock) void k_pcg_step(DevProblem P, int k) {
     __shared__ double lds[(kBlock / 64) * 2];
     PcgState* st = P.pcg;
-    const int stop = __hip_atomic_load(&st->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const double rho = st->rho[k & 1];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int pp = lane / 6, c = lane % 6;
     const int p = (blockIdx.x * (kBlock / 64) + wave) * kPosesPerWave + pp;
     const bool in = lane < 6 * kPosesPerWave && p < P.num_pose_slots;
     const int ps = in ? p : 0;
     const size_t i = 6 * (size_t)ps + c;
-    const bool valid = in && P.pose_var[ps] >= 0;
+    // every load is issued before any loaded value is used
+    const int pv = P.pose_var[ps];
     const double x = P.x[i], d = P.d[i], r = P.r[i], q = P.q[i];
     double m[6];
 #pragma unroll
     for (int j = 0; j < 6; ++j) m[j] = P.Minv[36 * (size_t)ps + 6 * c + j];
+    const int stop = __hip_atomic_load(&st->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const double rho = st->rho[k & 1];
     double curvature, unused;
     wave_pair_total(P.bpart, P.num_bpart, &curvature, &unused);
     if (__builtin_amdgcn_readfirstlane(stop)) return;
+    const bool valid = in && pv >= 0;
     if (!isfinite(curvature) || !(curvature > 0.0)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) set_stop(st, PCG_NONPOSITIVE);
         return;
