@@ -125,6 +125,19 @@ void build_runs(Model* mp) {
     m.run_slot.assign(m.num_edges, -1);
     std::vector<int32_t> cursor(m.pose_run_ptr.begin(), m.pose_run_ptr.end() - 1);
     for (size_t r = 0; r < run_pose.size(); ++r) m.run_slot[run_end[r]] = cursor[run_pose[r]]++;
+    const size_t tiles = m.lm_tiles.size();
+    m.t_pose.assign(tiles * kTileEdges, 0);
+    m.t_slot.assign(tiles * kTileEdges, 0);
+    m.t_lm.assign(tiles * kTileEdges, 0);
+    for (size_t t = 0; t < tiles; ++t) {
+        const int32_t e0 = m.lm_tiles[t].e_begin, n = m.lm_tiles[t].e_end - e0;
+        if (n > kTileEdges) continue;
+        for (int32_t s = 0; s < n; ++s) {
+            m.t_pose[t * kTileEdges + s] = m.s_pose[e0 + s];
+            m.t_slot[t * kTileEdges + s] = m.run_slot[e0 + s];
+            m.t_lm[t * kTileEdges + s] = m.s_lm[e0 + s];
+        }
+    }
 }
 
 uint32_t next_pow2(uint64_t v) {

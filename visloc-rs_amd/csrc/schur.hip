@@ -516,22 +516,25 @@ __device__ __forceinline__ void apply_tile(const DevProblem& P, const double* __
     // ---- stage 0: the tile's data, the inputs of the PCG scalars, V^-1 and
     // X, and the first pose's vectors, all issued before the wave waits on
     // any of them (a stopped launch drops them) ----
-    // Every lane loads four positions, clamped to the tile's last one (to the
-    // problem's first edge for a tile without edges), with no branch around
-    // the loads: a value used inside the branch of its own load is waited for
-    // there, before the loads after it are issued. Positions from nv on are
-    // never read.
+    // The lane's four positions (pose, run slot, tile-local landmark) come
+    // from the tile-major copies by tile index alone, one aligned load each,
+    // in flight with the tile's bounds. Past the tile's edges they hold 0;
+    // positions from nv on are never read.
+    static_assert(kApplyPer == 4, "four positions per lane, one aligned load each");
+    const size_t tp = (size_t)kTileEdges * tile + kApplyPer * lane;
+    const int4 key4 = *(const int4*)(P.t_pose + tp);
+    const int4 slot4 = *(const int4*)(P.t_slot + tp);
+    const uint32_t lm4 = *(const uint32_t*)(P.t_lm + tp);
     const int s0 = kApplyPer * lane;
     const int nv = one ? 0 : max(0, min(kApplyPer, n - s0));
-    static_assert(kApplyPer <= 4, "four positions per lane");
-    int key[kApplyPer], slot[kApplyPer], lmb[kApplyPer];  // pose, run slot, tile-local landmark
+    const int key[kApplyPer] = {key4.x, key4.y, key4.z, key4.w};
+    const int slot[kApplyPer] = {slot4.x, slot4.y, slot4.z, slot4.w};  // used by the runs' partials at the end
+    // G of the four positions (the FP64 copy by edge, clamped to the tile's
+    // last one, or the problem's first edge for a tile without edges); no
+    // branch around the loads, so none waits for another
     TS G[kApplyPer][6];
 #pragma unroll
     for (int i = 0; i < kApplyPer; ++i) {
-        const int e = n > 0 ? e0 + min(s0 + i, n - 1) : 0;
-        key[i] = P.s_pose[e];
-        lmb[i] = P.s_lm[e];
-        slot[i] = P.run_slot[e];  // used by the run's partial at the end
         if constexpr (std::is_same<TS, float>::value) {
             // slot i of the tile's lane-major chunk: 1 KiB + 512 B per wave
             const float* gb = Gl + (size_t)lt.g_chunk * kG32Chunk + 384 * i;
@@ -544,7 +547,7 @@ __device__ __forceinline__ void apply_tile(const DevProblem& P, const double* __
             G[i][4] = b.x;
             G[i][5] = b.y;
         } else {
-            load6(Gl + 6 * (size_t)e, G[i]);
+            load6(Gl + 6 * (size_t)(n > 0 ? e0 + min(s0 + i, n - 1) : 0), G[i]);
         }
     }
     // V^-1 and X of the tile's landmarks go straight from global memory into
@@ -603,7 +606,7 @@ __device__ __forceinline__ void apply_tile(const DevProblem& P, const double* __
                     }
                     pcur = key[i];
                 }
-                const int j = lmb[i];
+                const int j = (lm4 >> (8 * i)) & 0xff;
                 const double X[3] = {sX[3 * j], sX[3 * j + 1], sX[3 * j + 2]};
                 double t[3];
                 edge_wtd(G[i], d, X, t);
@@ -651,7 +654,7 @@ __device__ __forceinline__ void apply_tile(const DevProblem& P, const double* __
 #pragma unroll
     for (int i = 0; i < kApplyPer; ++i) {
         if (i < nv) {
-            const int j = lmb[i];
+            const int j = (lm4 >> (8 * i)) & 0xff;
             const double X[3] = {sX[3 * j], sX[3 * j + 1], sX[3 * j + 2]};
             const double u[3] = {sT[3 * j], sT[3 * j + 1], sT[3 * j + 2]};
             const double Gd[6] = {(double)G[i][0], (double)G[i][1], (double)G[i][2],
@@ -729,32 +732,38 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_schur_apply(DevProblem P, 
     const int groups = (P.num_lm_tiles + kApplyWaves - 1) / kApplyWaves;
     const int tile = xcd_tile(blockIdx.x, groups) * kApplyWaves + wv;
     if (tile >= P.num_lm_tiles) return;
-    const LmTile lt = P.lm_tiles[tile];
-    if (P.wstate && !P.wstate[P.lm_tile_win[tile]].active) return;
     const int lane = threadIdx.x & 63;
-    const int e0 = lt.e_begin, g0 = kTileEdges * lt.g_chunk;
-    const int n = lt.e_end - lt.e_begin, nl = lt.lm_end - lt.lm_begin;
-    const bool one = nl == 1 && n > kTileRuns;
 
     // ---- stage 0: the tile's data, the inputs of the PCG scalars, V^-1 and
     // X, and the first pose's vectors, all issued before the wave waits on
     // any of them (a stopped launch drops them) ----
-    // Every lane loads four positions, clamped to the tile's last one (to the
-    // problem's first edge for a tile without edges), with no branch around
-    // the loads: a value used inside the branch of its own load is waited for
-    // there, before the loads after it are issued. Positions from nv on are
-    // never read.
+    // The lane's four positions (pose, run slot, tile-local landmark) come
+    // from the tile-major copies by tile index alone, one aligned load each,
+    // in flight with the tile's bounds. Past the tile's edges they hold 0;
+    // positions from nv on are never read.
+    static_assert(kApplyPer == 4, "four positions per lane, one aligned load each");
+    const size_t tp = (size_t)kTileEdges * tile + kApplyPer * lane;
+    const int4 key4 = *(const int4*)(P.t_pose + tp);
+    const int4 slot4 = *(const int4*)(P.t_slot + tp);
+    const uint32_t lm4 = *(const uint32_t*)(P.t_lm + tp);
+    const LmTile lt = P.lm_tiles[tile];
+    // the PCG state and partials are bound whenever the operator runs
+    const PrologueIn pin = pcg_prologue_load(P, pcg_iter);
+    if (P.wstate && !P.wstate[P.lm_tile_win[tile]].active) return;
+    const int e0 = lt.e_begin, g0 = kTileEdges * lt.g_chunk;
+    const int n = lt.e_end - lt.e_begin, nl = lt.lm_end - lt.lm_begin;
+    const bool one = nl == 1 && n > kTileRuns;
+
     const int s0 = kApplyPer * lane;
     const int nv = one ? 0 : max(0, min(kApplyPer, n - s0));
-    static_assert(kApplyPer <= 4, "four positions per lane");
-    int key[kApplyPer], slot[kApplyPer], lmb[kApplyPer];  // pose, run slot, tile-local landmark
+    const int key[kApplyPer] = {key4.x, key4.y, key4.z, key4.w};
+    const int slot[kApplyPer] = {slot4.x, slot4.y, slot4.z, slot4.w};  // used by the runs' partials at the end
+    // G of the four positions (the FP64 copy by edge, clamped to the tile's
+    // last one, or the problem's first edge for a tile without edges); no
+    // branch around the loads, so none waits for another
     TS G[kApplyPer][6];
 #pragma unroll
     for (int i = 0; i < kApplyPer; ++i) {
-        const int e = n > 0 ? e0 + min(s0 + i, n - 1) : 0;
-        key[i] = P.s_pose[e];
-        lmb[i] = P.s_lm[e];
-        slot[i] = P.run_slot[e];  // used by the run's partial at the end
         if constexpr (std::is_same<TS, float>::value) {
             // slot i of the tile's lane-major chunk: 1 KiB + 512 B per wave
             const float* gb = Gl + (size_t)lt.g_chunk * kG32Chunk + 384 * i;
@@ -767,11 +776,9 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_schur_apply(DevProblem P, 
             G[i][4] = b.x;
             G[i][5] = b.y;
         } else {
-            load6(Gl + 6 * (size_t)e, G[i]);
+            load6(Gl + 6 * (size_t)(n > 0 ? e0 + min(s0 + i, n - 1) : 0), G[i]);
         }
     }
-    // the PCG state and partials are bound whenever the operator runs
-    const PrologueIn pin = pcg_prologue_load(P, pcg_iter);
     // V^-1 and X of the tile's landmarks go straight from global memory into
     // LDS (their layout there is the global one), in flight together with
     // the loads above: 16-byte pieces, lane i's piece at byte 16 i of each
@@ -833,7 +840,7 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_schur_apply(DevProblem P, 
                     }
                     pcur = key[i];
                 }
-                const int j = lmb[i];
+                const int j = (lm4 >> (8 * i)) & 0xff;
                 const double X[3] = {sX[3 * j], sX[3 * j + 1], sX[3 * j + 2]};
                 double t[3];
                 edge_wtd(G[i], d, X, t);
@@ -881,7 +888,7 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_schur_apply(DevProblem P, 
 #pragma unroll
     for (int i = 0; i < kApplyPer; ++i) {
         if (i < nv) {
-            const int j = lmb[i];
+            const int j = (lm4 >> (8 * i)) & 0xff;
             const double X[3] = {sX[3 * j], sX[3 * j + 1], sX[3 * j + 2]};
             const double u[3] = {sT[3 * j], sT[3 * j + 1], sT[3 * j + 2]};
             const double Gd[6] = {(double)G[i][0], (double)G[i][1], (double)G[i][2],

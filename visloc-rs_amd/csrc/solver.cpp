@@ -32,7 +32,7 @@ enum BufId {
     B_SCALARS, B_DENSE_BLOCK_PQ, B_DENSE_BLOCK_PTR, B_DENSE_TERMS, B_DENSE_VAR_SLOT, B_DENSE_S,
     B_DENSE_WORK, B_DENSE_STATUS, B_X_BLOCK_POS, B_X_ROW_PTR, B_X_COL, B_X_DIAG_POS, B_X_S64, B_X_S32, B_VD2,
     B_G32_POS, B_OBS_W, B_OBS_S, B_OBS_KIND, B_OBS_EXTRA, B_SENSORS_T, B_J_E, B_J_TILE, B_J_HPK, B_J_KK, B_J_WK,
-    B_J_PART, B_J_RHS, B_J_DELTA, B_J_GL, B_COUNT
+    B_J_PART, B_J_RHS, B_J_DELTA, B_J_GL, B_T_POSE, B_T_SLOT, B_T_LM, B_COUNT
 };
 
 // Profiled kernel families.
@@ -432,6 +432,9 @@ int32_t prepare(Context& ctx, Prepared& pr, std::string* msg) {
         upload(pr, B_RUN_SLOT, m.run_slot, s);
         upload(pr, B_S_LM, m.s_lm, s);
         upload(pr, B_G32_POS, m.g32_pos, s);
+        upload(pr, B_T_POSE, m.t_pose, s);
+        upload(pr, B_T_SLOT, m.t_slot, s);
+        upload(pr, B_T_LM, m.t_lm, s);
         if (!m.obs_w.empty()) upload(pr, B_OBS_W, m.obs_w, s);
         hip_check(hipStreamSynchronize(s), "sync");
         pr.final_slot = -1;
@@ -629,6 +632,9 @@ DevProblem bind(const Context& ctx, Prepared& pr, const vba_options& o, bool mix
     P.Gl64 = (double*)ensure(pr, B_GL64, 6 * NE * sizeof(double));
     P.hl64 = (double*)ensure(pr, B_HL64, 3 * NE * sizeof(double));
     P.g32_pos = buf<int32_t>(pr, B_G32_POS);
+    P.t_pose = buf<int32_t>(pr, B_T_POSE);
+    P.t_slot = buf<int32_t>(pr, B_T_SLOT);
+    P.t_lm = buf<uint8_t>(pr, B_T_LM);
     if (mixed) {
         P.Gl32 = (float*)ensure(pr, B_GL32, (size_t)kG32Chunk * std::max(m.num_g32_chunks, 1) * sizeof(float));
         P.Vm = (double*)ensure(pr, B_VM, 6 * NL * sizeof(double));
@@ -1199,6 +1205,9 @@ void merge_models(const std::vector<const Model*>& ms, Model* out, std::vector<W
             M.pose_run_ptr.push_back((int32_t)(m.pose_run_ptr[s2] + ro));
         for (int32_t r : m.run_slot) M.run_slot.push_back(r < 0 ? -1 : (int32_t)(r + ro));
         for (int32_t g : m.g32_pos) M.g32_pos.push_back(g + kTileEdges * M.num_g32_chunks);
+        for (int32_t p : m.t_pose) M.t_pose.push_back((int32_t)(p + po));
+        for (int32_t r : m.t_slot) M.t_slot.push_back(r < 0 ? -1 : (int32_t)(r + ro));
+        M.t_lm.insert(M.t_lm.end(), m.t_lm.begin(), m.t_lm.end());
         M.num_g32_chunks += m.num_g32_chunks;
         ro += (int64_t)m.num_runs;
         M.max_lm_tile_edges = std::max(M.max_lm_tile_edges, m.max_lm_tile_edges);
