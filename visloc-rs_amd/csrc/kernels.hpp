@@ -118,11 +118,12 @@ struct DevProblem {
     const int32_t* pose_run_ptr;  // num_pose_slots + 1
     const int32_t* run_slot;      // per storage position: slot of the run ending there, else -1
     const int32_t* g32_pos;       // per storage position: its index in the lane-major FP32 G
-    // pose, run slot and local landmark by tile-major position (kTileEdges
-    // per landmark tile; see Model::t_pose)
-    const int32_t* t_pose;
-    const int32_t* t_slot;
+    // the landmark tiles' run tables (see Model::t_lm): local landmark by
+    // position, run-start mask, each run's pose and slot
     const uint8_t* t_lm;
+    const uint64_t* t_head;
+    const int32_t* t_rpose;
+    const int32_t* t_rslot;
     double* part64;  // 6 per run slot
     double* part21;  // 21 per run slot (block-Jacobi blocks)
     // PCG vectors, 6 per pose slot

@@ -126,16 +126,22 @@ void build_runs(Model* mp) {
     std::vector<int32_t> cursor(m.pose_run_ptr.begin(), m.pose_run_ptr.end() - 1);
     for (size_t r = 0; r < run_pose.size(); ++r) m.run_slot[run_end[r]] = cursor[run_pose[r]]++;
     const size_t tiles = m.lm_tiles.size();
-    m.t_pose.assign(tiles * kTileEdges, 0);
-    m.t_slot.assign(tiles * kTileEdges, 0);
     m.t_lm.assign(tiles * kTileEdges, 0);
+    m.t_head.assign(tiles * (kTileEdges / 64), 0);
+    m.t_rpose.assign(tiles * kTileRuns, 0);
+    m.t_rslot.assign(tiles * kTileRuns, 0);
     for (size_t t = 0; t < tiles; ++t) {
-        const int32_t e0 = m.lm_tiles[t].e_begin, n = m.lm_tiles[t].e_end - e0;
-        if (n > kTileEdges) continue;
+        const LmTile& lt = m.lm_tiles[t];
+        const int32_t e0 = lt.e_begin, n = lt.e_end - e0;
+        if (n > kTileEdges || (lt.lm_end - lt.lm_begin == 1 && n > kTileRuns)) continue;
+        int32_t r = -1;
         for (int32_t s = 0; s < n; ++s) {
-            m.t_pose[t * kTileEdges + s] = m.s_pose[e0 + s];
-            m.t_slot[t * kTileEdges + s] = m.run_slot[e0 + s];
             m.t_lm[t * kTileEdges + s] = m.s_lm[e0 + s];
+            if (s == 0 || m.s_pose[e0 + s] != m.s_pose[e0 + s - 1]) {
+                m.t_head[t * (kTileEdges / 64) + s / 64] |= 1ull << (s % 64);
+                m.t_rpose[t * kTileRuns + ++r] = m.s_pose[e0 + s];
+            }
+            if (m.run_slot[e0 + s] >= 0) m.t_rslot[t * kTileRuns + r] = m.run_slot[e0 + s];
         }
     }
 }

@@ -135,13 +135,15 @@ struct Model {
     std::vector<int32_t> pose_run_ptr, run_slot;
     uint64_t num_runs = 0;
     std::vector<int32_t> g32_pos;  // per storage position (lane-major FP32 G)
-    // The same positions tile-major, kTileEdges per landmark tile (position s
-    // of tile t at kTileEdges t + s, 0 past the tile's edges and for tiles of
-    // more than kTileEdges edges): pose, run slot at a run's last position
-    // (-1 elsewhere), tile-local landmark. A wave reads its lane's four
-    // positions of each with one aligned load, by tile index alone.
-    std::vector<int32_t> t_pose, t_slot;
+    // The operator's view of a landmark tile t, read by tile index alone
+    // (zero for tiles of one landmark with more than kTileRuns edges):
+    // t_lm[kTileEdges t + s] the tile-local landmark at sorted position s;
+    // t_head[4 t + w] bit b set when position 64 w + b starts a camera run;
+    // t_rpose / t_rslot[kTileRuns t + r] the pose and the partial slot of the
+    // tile's run r, runs numbered in position order.
     std::vector<uint8_t> t_lm;
+    std::vector<uint64_t> t_head;
+    std::vector<int32_t> t_rpose, t_rslot;
     std::vector<double> obs_w;     // external observation weights, c-order (empty: none)
     int32_t num_g32_chunks = 0;
 };

@@ -516,19 +516,33 @@ __device__ __forceinline__ void apply_tile(const DevProblem& P, const double* __
     // ---- stage 0: the tile's data, the inputs of the PCG scalars, V^-1 and
     // X, and the first pose's vectors, all issued before the wave waits on
     // any of them (a stopped launch drops them) ----
-    // The lane's four positions (pose, run slot, tile-local landmark) come
-    // from the tile-major copies by tile index alone, one aligned load each,
-    // in flight with the tile's bounds. Past the tile's edges they hold 0;
-    // positions from nv on are never read.
-    static_assert(kApplyPer == 4, "four positions per lane, one aligned load each");
-    const size_t tp = (size_t)kTileEdges * tile + kApplyPer * lane;
-    const int4 key4 = *(const int4*)(P.t_pose + tp);
-    const int4 slot4 = *(const int4*)(P.t_slot + tp);
-    const uint32_t lm4 = *(const uint32_t*)(P.t_lm + tp);
+    // The lane's four positions come from the tile's run tables by tile
+    // index alone, in flight with the tile's bounds: the mask of positions
+    // that start a camera run, run r's pose and slot in lane r, and the
+    // positions' tile-local landmarks (one aligned load). Positions from nv
+    // on are never read.
+    static_assert(kApplyPer == 4 && kTileRuns == 64 && kTileEdges == 256, "four positions and one run per lane");
+    const uint64_t* hm = P.t_head + 4 * (size_t)tile;
+    const uint64_t h0 = hm[0], h1 = hm[1], h2 = hm[2], h3 = hm[3];
+    const int rpose = P.t_rpose[(size_t)kTileRuns * tile + lane];
+    const int rslot = P.t_rslot[(size_t)kTileRuns * tile + lane];
+    const uint32_t lm4 = *(const uint32_t*)(P.t_lm + (size_t)kTileEdges * tile + kApplyPer * lane);
     const int s0 = kApplyPer * lane;
     const int nv = one ? 0 : max(0, min(kApplyPer, n - s0));
-    const int key[kApplyPer] = {key4.x, key4.y, key4.z, key4.w};
-    const int slot[kApplyPer] = {slot4.x, slot4.y, slot4.z, slot4.w};  // used by the runs' partials at the end
+    // this lane's run-start bits and the run of each position: the runs
+    // started before the lane's first position, then the lane's own starts
+    const uint64_t hw = lane < 16 ? h0 : lane < 32 ? h1 : lane < 48 ? h2 : h3;
+    const int sh = kApplyPer * (lane & 15);
+    const uint32_t hb = (uint32_t)(hw >> sh) & 0xf;
+    int run = __popcll(hw & ((1ull << sh) - 1)) - 1 + (lane >= 16 ? __popcll(h0) : 0) +
+              (lane >= 32 ? __popcll(h1) : 0) + (lane >= 48 ? __popcll(h2) : 0);
+    int rix[kApplyPer], key[kApplyPer];
+#pragma unroll
+    for (int i = 0; i < kApplyPer; ++i) {
+        run += (hb >> i) & 1;
+        rix[i] = run & (kTileRuns - 1);
+        key[i] = __shfl(rpose, rix[i], 64);
+    }
     // G of the four positions (the FP64 copy by edge, clamped to the tile's
     // last one, or the problem's first edge for a tile without edges); no
     // branch around the loads, so none waits for another
@@ -638,15 +652,18 @@ __device__ __forceinline__ void apply_tile(const DevProblem& P, const double* __
     // the lane that finishes the run adds the slots of the lanes before it,
     // back to the run's first lane, to its own leading sum: plain LDS
     // traffic in a fixed order, no atomics.
-    const int prev_key = __shfl_up(key[kApplyPer - 1], 1, 64);  // last key of the lane before
-    const int next_key = __shfl_down(key[0], 1, 64);             // first key of the lane after
+    // A position starts a run when its bit is set and ends one when the next
+    // position starts one or it is the tile's last; each run's slot comes
+    // from the lane holding the run.
+    const uint32_t hb_next = __shfl_down(hb, 1, 64);  // the next lane's bits
     bool head[kApplyPer], fin[kApplyPer];
+    int slot[kApplyPer];
 #pragma unroll
     for (int i = 0; i < kApplyPer; ++i) {
-        const int pk = i == 0 ? (lane == 0 ? INT_MIN : prev_key) : key[i - 1];
-        const int nk = i + 1 < kApplyPer ? key[i + 1] : (lane == 63 ? INT_MIN : next_key);
-        head[i] = i < nv && key[i] != pk;
-        fin[i] = i < nv && (s0 + i == n - 1 || key[i] != nk);
+        const bool next_head = i + 1 < kApplyPer ? (hb >> (i + 1)) & 1 : hb_next & 1;
+        head[i] = i < nv && ((hb >> i) & 1);
+        fin[i] = i < nv && (s0 + i == n - 1 || next_head);
+        slot[i] = __shfl(rslot, rix[i], 64);
     }
     double acc[6], lead_acc[6];
     bool own = false;   // the current segment's run starts in this lane
@@ -737,15 +754,17 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_schur_apply(DevProblem P, 
     // ---- stage 0: the tile's data, the inputs of the PCG scalars, V^-1 and
     // X, and the first pose's vectors, all issued before the wave waits on
     // any of them (a stopped launch drops them) ----
-    // The lane's four positions (pose, run slot, tile-local landmark) come
-    // from the tile-major copies by tile index alone, one aligned load each,
-    // in flight with the tile's bounds. Past the tile's edges they hold 0;
-    // positions from nv on are never read.
-    static_assert(kApplyPer == 4, "four positions per lane, one aligned load each");
-    const size_t tp = (size_t)kTileEdges * tile + kApplyPer * lane;
-    const int4 key4 = *(const int4*)(P.t_pose + tp);
-    const int4 slot4 = *(const int4*)(P.t_slot + tp);
-    const uint32_t lm4 = *(const uint32_t*)(P.t_lm + tp);
+    // The lane's four positions come from the tile's run tables by tile
+    // index alone, in flight with the tile's bounds: the mask of positions
+    // that start a camera run, run r's pose and slot in lane r, and the
+    // positions' tile-local landmarks (one aligned load). Positions from nv
+    // on are never read.
+    static_assert(kApplyPer == 4 && kTileRuns == 64 && kTileEdges == 256, "four positions and one run per lane");
+    const uint64_t* hm = P.t_head + 4 * (size_t)tile;
+    const uint64_t h0 = hm[0], h1 = hm[1], h2 = hm[2], h3 = hm[3];
+    const int rpose = P.t_rpose[(size_t)kTileRuns * tile + lane];
+    const int rslot = P.t_rslot[(size_t)kTileRuns * tile + lane];
+    const uint32_t lm4 = *(const uint32_t*)(P.t_lm + (size_t)kTileEdges * tile + kApplyPer * lane);
     const LmTile lt = P.lm_tiles[tile];
     // the PCG state and partials are bound whenever the operator runs
     const PrologueIn pin = pcg_prologue_load(P, pcg_iter);
@@ -756,8 +775,20 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_schur_apply(DevProblem P, 
 
     const int s0 = kApplyPer * lane;
     const int nv = one ? 0 : max(0, min(kApplyPer, n - s0));
-    const int key[kApplyPer] = {key4.x, key4.y, key4.z, key4.w};
-    const int slot[kApplyPer] = {slot4.x, slot4.y, slot4.z, slot4.w};  // used by the runs' partials at the end
+    // this lane's run-start bits and the run of each position: the runs
+    // started before the lane's first position, then the lane's own starts
+    const uint64_t hw = lane < 16 ? h0 : lane < 32 ? h1 : lane < 48 ? h2 : h3;
+    const int sh = kApplyPer * (lane & 15);
+    const uint32_t hb = (uint32_t)(hw >> sh) & 0xf;
+    int run = __popcll(hw & ((1ull << sh) - 1)) - 1 + (lane >= 16 ? __popcll(h0) : 0) +
+              (lane >= 32 ? __popcll(h1) : 0) + (lane >= 48 ? __popcll(h2) : 0);
+    int rix[kApplyPer], key[kApplyPer];
+#pragma unroll
+    for (int i = 0; i < kApplyPer; ++i) {
+        run += (hb >> i) & 1;
+        rix[i] = run & (kTileRuns - 1);
+        key[i] = __shfl(rpose, rix[i], 64);
+    }
     // G of the four positions (the FP64 copy by edge, clamped to the tile's
     // last one, or the problem's first edge for a tile without edges); no
     // branch around the loads, so none waits for another
@@ -872,15 +903,18 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_schur_apply(DevProblem P, 
     // the lane that finishes the run adds the slots of the lanes before it,
     // back to the run's first lane, to its own leading sum: plain LDS
     // traffic in a fixed order, no atomics.
-    const int prev_key = __shfl_up(key[kApplyPer - 1], 1, 64);  // last key of the lane before
-    const int next_key = __shfl_down(key[0], 1, 64);             // first key of the lane after
+    // A position starts a run when its bit is set and ends one when the next
+    // position starts one or it is the tile's last; each run's slot comes
+    // from the lane holding the run.
+    const uint32_t hb_next = __shfl_down(hb, 1, 64);  // the next lane's bits
     bool head[kApplyPer], fin[kApplyPer];
+    int slot[kApplyPer];
 #pragma unroll
     for (int i = 0; i < kApplyPer; ++i) {
-        const int pk = i == 0 ? (lane == 0 ? INT_MIN : prev_key) : key[i - 1];
-        const int nk = i + 1 < kApplyPer ? key[i + 1] : (lane == 63 ? INT_MIN : next_key);
-        head[i] = i < nv && key[i] != pk;
-        fin[i] = i < nv && (s0 + i == n - 1 || key[i] != nk);
+        const bool next_head = i + 1 < kApplyPer ? (hb >> (i + 1)) & 1 : hb_next & 1;
+        head[i] = i < nv && ((hb >> i) & 1);
+        fin[i] = i < nv && (s0 + i == n - 1 || next_head);
+        slot[i] = __shfl(rslot, rix[i], 64);
     }
     double acc[6], lead_acc[6];
     bool own = false;   // the current segment's run starts in this lane
@@ -1320,13 +1354,8 @@ void launch_schur_apply(const DevProblem& P, bool fp32, const double* v1, const 
     // in the last bit; the FP32-G pair is checked bit for bit (autotuner,
     // tests).
     if (fp32 && resident_waves > 0 && resident_waves < P.num_lm_tiles) {
-        const dim3 grid(xcd_grid(resident_waves)), block(64);
-        if (fp32)
-            hipLaunchKernelGGL(k_schur_apply_loop<float>, grid, block, 0, s, P, v1, v2, (const float*)P.Gl32,
-                               (const double*)P.VinvM, P.part64, pcg_iter, fresh);
-        else
-            hipLaunchKernelGGL(k_schur_apply_loop<double>, grid, block, 0, s, P, v1, v2, (const double*)P.Gl64,
-                               (const double*)P.Vinv64, P.part64, pcg_iter, fresh);
+        hipLaunchKernelGGL(k_schur_apply_loop<float>, dim3(xcd_grid(resident_waves)), dim3(64), 0, s, P, v1, v2,
+                           (const float*)P.Gl32, (const double*)P.VinvM, P.part64, pcg_iter, fresh);
         return;
     }
     const dim3 grid(xcd_grid(blocks_for(P.num_lm_tiles, kApplyWaves))), block(64 * kApplyWaves);

@@ -131,7 +131,7 @@ struct Request {
 uint64_t estimate_bytes(const vba_problem& p, bool window) {
     const uint64_t n = vba::observation_count(p);
     const uint64_t L = p.num_landmarks, P = p.num_poses;
-    uint64_t b = n * 40 + n * 202 + L * 260 + P * 1400 + (64u << 10);
+    uint64_t b = n * 40 + n * 196 + L * 260 + P * 1400 + (64u << 10);
     if (window) {
         uint64_t var = 0;
         for (uint64_t i = 0; i < P; ++i) var += p.pose_fixed ? (p.pose_fixed[i] == 0) : 1;

@@ -32,7 +32,7 @@ enum BufId {
     B_SCALARS, B_DENSE_BLOCK_PQ, B_DENSE_BLOCK_PTR, B_DENSE_TERMS, B_DENSE_VAR_SLOT, B_DENSE_S,
     B_DENSE_WORK, B_DENSE_STATUS, B_X_BLOCK_POS, B_X_ROW_PTR, B_X_COL, B_X_DIAG_POS, B_X_S64, B_X_S32, B_VD2,
     B_G32_POS, B_OBS_W, B_OBS_S, B_OBS_KIND, B_OBS_EXTRA, B_SENSORS_T, B_J_E, B_J_TILE, B_J_HPK, B_J_KK, B_J_WK,
-    B_J_PART, B_J_RHS, B_J_DELTA, B_J_GL, B_T_POSE, B_T_SLOT, B_T_LM, B_COUNT
+    B_J_PART, B_J_RHS, B_J_DELTA, B_J_GL, B_T_LM, B_T_HEAD, B_T_RPOSE, B_T_RSLOT, B_COUNT
 };
 
 // Profiled kernel families.
@@ -432,9 +432,10 @@ int32_t prepare(Context& ctx, Prepared& pr, std::string* msg) {
         upload(pr, B_RUN_SLOT, m.run_slot, s);
         upload(pr, B_S_LM, m.s_lm, s);
         upload(pr, B_G32_POS, m.g32_pos, s);
-        upload(pr, B_T_POSE, m.t_pose, s);
-        upload(pr, B_T_SLOT, m.t_slot, s);
         upload(pr, B_T_LM, m.t_lm, s);
+        upload(pr, B_T_HEAD, m.t_head, s);
+        upload(pr, B_T_RPOSE, m.t_rpose, s);
+        upload(pr, B_T_RSLOT, m.t_rslot, s);
         if (!m.obs_w.empty()) upload(pr, B_OBS_W, m.obs_w, s);
         hip_check(hipStreamSynchronize(s), "sync");
         pr.final_slot = -1;
@@ -632,9 +633,10 @@ DevProblem bind(const Context& ctx, Prepared& pr, const vba_options& o, bool mix
     P.Gl64 = (double*)ensure(pr, B_GL64, 6 * NE * sizeof(double));
     P.hl64 = (double*)ensure(pr, B_HL64, 3 * NE * sizeof(double));
     P.g32_pos = buf<int32_t>(pr, B_G32_POS);
-    P.t_pose = buf<int32_t>(pr, B_T_POSE);
-    P.t_slot = buf<int32_t>(pr, B_T_SLOT);
     P.t_lm = buf<uint8_t>(pr, B_T_LM);
+    P.t_head = buf<uint64_t>(pr, B_T_HEAD);
+    P.t_rpose = buf<int32_t>(pr, B_T_RPOSE);
+    P.t_rslot = buf<int32_t>(pr, B_T_RSLOT);
     if (mixed) {
         P.Gl32 = (float*)ensure(pr, B_GL32, (size_t)kG32Chunk * std::max(m.num_g32_chunks, 1) * sizeof(float));
         P.Vm = (double*)ensure(pr, B_VM, 6 * NL * sizeof(double));
@@ -756,11 +758,12 @@ int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stat
         // layout must move at least once (include/vba.h, vba_profile_query).
         // Pose vectors gathered per edge are counted once per pose.
         const double real = mixed ? 4.0 : 8.0;
-        // tile pass: per edge G (6 reals), its pose (4) and sorted position (2);
-        // per landmark X (24), V^-1 (48), edge pointer (4); per run the
-        // FP64 partial (48); per pose z and d (96); per tile its bounds (8)
-        const double bytes_apply = NE * (6.0 * real + 4.0 + 1.0) + NL * (24.0 + 48.0 + 4.0) + NR * (48.0 + 4.0) +
-                                   NP * 96.0 + m.lm_tiles.size() * 8.0;
+        // tile pass: per edge G (6 reals) and its local landmark (1); per
+        // landmark X (24) and V^-1 (48); per run the FP64 partial (48); per
+        // pose z and d (96); per tile its bounds (20), run-start mask (32)
+        // and the pose and slot tables of its 64 runs (512)
+        const double bytes_apply = NE * (6.0 * real + 1.0) + NL * (24.0 + 48.0) + NR * 48.0 + NP * 96.0 +
+                                   m.lm_tiles.size() * (20.0 + 32.0 + 512.0);
         const double bytes_pcg_pose = NR * 48.0 + NP * (8.0 + 96.0 + 168.0 + 96.0);
         const double bytes_pcg_step = NP * (4.0 * 48.0 + 288.0 + 3.0 * 48.0);
         const double bytes_linearize = m.num_obs * 16.0 + NE * (4.0 + 4.0 + 4.0 + 48.0 + 24.0 + (mixed ? 24.0 : 0.0)) +
@@ -1205,9 +1208,10 @@ void merge_models(const std::vector<const Model*>& ms, Model* out, std::vector<W
             M.pose_run_ptr.push_back((int32_t)(m.pose_run_ptr[s2] + ro));
         for (int32_t r : m.run_slot) M.run_slot.push_back(r < 0 ? -1 : (int32_t)(r + ro));
         for (int32_t g : m.g32_pos) M.g32_pos.push_back(g + kTileEdges * M.num_g32_chunks);
-        for (int32_t p : m.t_pose) M.t_pose.push_back((int32_t)(p + po));
-        for (int32_t r : m.t_slot) M.t_slot.push_back(r < 0 ? -1 : (int32_t)(r + ro));
         M.t_lm.insert(M.t_lm.end(), m.t_lm.begin(), m.t_lm.end());
+        M.t_head.insert(M.t_head.end(), m.t_head.begin(), m.t_head.end());
+        for (int32_t p : m.t_rpose) M.t_rpose.push_back((int32_t)(p + po));
+        for (int32_t r : m.t_rslot) M.t_rslot.push_back((int32_t)(r + ro));
         M.num_g32_chunks += m.num_g32_chunks;
         ro += (int64_t)m.num_runs;
         M.max_lm_tile_edges = std::max(M.max_lm_tile_edges, m.max_lm_tile_edges);
