@@ -495,15 +495,36 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
+// A landmark tile's run tables as one wave holds them: the run-start mask
+// (uniform), run `lane`'s pose and slot, and the tile-local landmarks of the
+// lane's four positions. Loaded by tile index alone.
+struct TileRuns {
+    uint64_t h[4];
+    int rpose, rslot;
+    uint32_t lm4;
+};
+
+__device__ __forceinline__ TileRuns load_tile_runs(const DevProblem& P, int tile, int lane) {
+    static_assert(kApplyPer == 4 && kTileRuns == 64 && kTileEdges == 256, "four positions and one run per lane");
+    TileRuns r;
+    const uint64_t* hm = P.t_head + 4 * (size_t)tile;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) r.h[w] = hm[w];
+    r.rpose = P.t_rpose[(size_t)kTileRuns * tile + lane];
+    r.rslot = P.t_rslot[(size_t)kTileRuns * tile + lane];
+    r.lm4 = *(const uint32_t*)(P.t_lm + (size_t)kTileEdges * tile + kApplyPer * lane);
+    return r;
+}
+
 // One landmark tile of the operator, by one wave, with the PCG scalar beta
-// given. sX0 (3 * 64 + 2 doubles, 16-byte aligned), sT (3 * 64) and sVR
+// and the tile's run tables given. sX0 (3 * 64 + 2 doubles, 16-byte aligned), sT (3 * 64) and sVR
 // (6 * 64, 16-byte aligned) are the wave's LDS.
 template <typename TS>
 __device__ __forceinline__ void apply_tile(const DevProblem& P, const double* __restrict__ v1,
                                            const double* __restrict__ v2, const TS* __restrict__ Gl,
                                            const double* __restrict__ Vinv, double* __restrict__ part,
-                                           int tile, const LmTile lt, double beta, double* sX0,
-                                           double* const sT, double* const sVR) {
+                                           int tile, const LmTile lt, const TileRuns tr, double beta,
+                                           double* sX0, double* const sT, double* const sVR) {
     double* sX = sX0;
     double* const sV = sVR;
     double* const sR = sVR;
@@ -516,17 +537,13 @@ __device__ __forceinline__ void apply_tile(const DevProblem& P, const double* __
     // ---- stage 0: the tile's data, the inputs of the PCG scalars, V^-1 and
     // X, and the first pose's vectors, all issued before the wave waits on
     // any of them (a stopped launch drops them) ----
-    // The lane's four positions come from the tile's run tables by tile
-    // index alone, in flight with the tile's bounds: the mask of positions
-    // that start a camera run, run r's pose and slot in lane r, and the
-    // positions' tile-local landmarks (one aligned load). Positions from nv
+    // The lane's four positions come from the tile's run tables (given):
+    // the mask of positions that start a camera run, run r's pose and slot
+    // in lane r, and the positions' tile-local landmarks. Positions from nv
     // on are never read.
-    static_assert(kApplyPer == 4 && kTileRuns == 64 && kTileEdges == 256, "four positions and one run per lane");
-    const uint64_t* hm = P.t_head + 4 * (size_t)tile;
-    const uint64_t h0 = hm[0], h1 = hm[1], h2 = hm[2], h3 = hm[3];
-    const int rpose = P.t_rpose[(size_t)kTileRuns * tile + lane];
-    const int rslot = P.t_rslot[(size_t)kTileRuns * tile + lane];
-    const uint32_t lm4 = *(const uint32_t*)(P.t_lm + (size_t)kTileEdges * tile + kApplyPer * lane);
+    const uint64_t h0 = tr.h[0], h1 = tr.h[1], h2 = tr.h[2], h3 = tr.h[3];
+    const int rpose = tr.rpose, rslot = tr.rslot;
+    const uint32_t lm4 = tr.lm4;
     const int s0 = kApplyPer * lane;
     const int nv = one ? 0 : max(0, min(kApplyPer, n - s0));
     // this lane's run-start bits and the run of each position: the runs
@@ -757,14 +774,11 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_schur_apply(DevProblem P, 
     // The lane's four positions come from the tile's run tables by tile
     // index alone, in flight with the tile's bounds: the mask of positions
     // that start a camera run, run r's pose and slot in lane r, and the
-    // positions' tile-local landmarks (one aligned load). Positions from nv
-    // on are never read.
-    static_assert(kApplyPer == 4 && kTileRuns == 64 && kTileEdges == 256, "four positions and one run per lane");
-    const uint64_t* hm = P.t_head + 4 * (size_t)tile;
-    const uint64_t h0 = hm[0], h1 = hm[1], h2 = hm[2], h3 = hm[3];
-    const int rpose = P.t_rpose[(size_t)kTileRuns * tile + lane];
-    const int rslot = P.t_rslot[(size_t)kTileRuns * tile + lane];
-    const uint32_t lm4 = *(const uint32_t*)(P.t_lm + (size_t)kTileEdges * tile + kApplyPer * lane);
+    // positions' tile-local landmarks. Positions from nv on are never read.
+    const TileRuns tr = load_tile_runs(P, tile, lane);
+    const uint64_t h0 = tr.h[0], h1 = tr.h[1], h2 = tr.h[2], h3 = tr.h[3];
+    const int rpose = tr.rpose, rslot = tr.rslot;
+    const uint32_t lm4 = tr.lm4;
     const LmTile lt = P.lm_tiles[tile];
     // the PCG state and partials are bound whenever the operator runs
     const PrologueIn pin = pcg_prologue_load(P, pcg_iter);
@@ -979,7 +993,7 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_schur_apply(DevProblem P, 
 
 // The same operator with a fixed grid of resident waves: XCD x takes the
 // x-th eighth of the tiles, its waves take them in turn, each wave loading
-// its next tile's bounds while it works on the current one. The PCG scalars
+// its next tile's bounds and run tables while it works on the current one. The PCG scalars
 // are derived once per wave. Tiles are those of k_schur_apply and each is
 // worked the same way, so the results are the same bit for bit. gridDim.x is
 // a multiple of 8.
@@ -1000,17 +1014,23 @@ __global__ __launch_bounds__(64, 4) void k_schur_apply_loop(DevProblem P, const 
     double beta = 0.0;
     if (pcg_iter >= 0 && !pcg_prologue(P, pcg_iter, fresh, &beta)) return;
     if (tile >= t1) return;
+    const int lane = threadIdx.x & 63;
     LmTile lt = P.lm_tiles[tile];
+    TileRuns tr = load_tile_runs(P, tile, lane);
     while (true) {
+        // the next tile's bounds and run tables, loaded with no branch (the
+        // last tile's again at the end) so that nothing waits for them here
         const int next = tile + stride;
-        LmTile ln{};
-        if (next < t1) ln = P.lm_tiles[next];
-        apply_tile<TS>(P, v1, v2, Gl, Vinv, part, tile, lt, beta, sXa, sTa, sVRa);
+        const int nt = min(next, t1 - 1);
+        const LmTile ln = P.lm_tiles[nt];
+        const TileRuns trn = load_tile_runs(P, nt, lane);
+        apply_tile<TS>(P, v1, v2, Gl, Vinv, part, tile, lt, tr, beta, sXa, sTa, sVRa);
         if (next >= t1) break;
         __builtin_amdgcn_s_waitcnt(0);  // this tile's LDS traffic is done before the next tile's loads land
         wave_lds_sync();
         tile = next;
         lt = ln;
+        tr = trn;
     }
 }
 
