@@ -517,17 +517,19 @@ __device__ __forceinline__ TileRuns load_tile_runs(const DevProblem& P, int tile
 }
 
 // One landmark tile of the operator, by one wave, with the PCG scalar beta
-// and the tile's run tables given. sX0 (3 * 64 + 2 doubles, 16-byte aligned), sT (3 * 64) and sVR
-// (6 * 64, 16-byte aligned) are the wave's LDS.
+// and the tile's run tables given. sXT (6 * 64 + 2 doubles) and sVU
+// (6 * 64), both 16-byte aligned, are the wave's LDS, laid out as
+// k_schur_apply lays out its own.
 template <typename TS>
 __device__ __forceinline__ void apply_tile(const DevProblem& P, const double* __restrict__ v1,
                                            const double* __restrict__ v2, const TS* __restrict__ Gl,
                                            const double* __restrict__ Vinv, double* __restrict__ part,
                                            int tile, const LmTile lt, const TileRuns tr, double beta,
-                                           double* sX0, double* const sT, double* const sVR) {
-    double* sX = sX0;
-    double* const sV = sVR;
-    double* const sR = sVR;
+                                           double* const sXT, double* const sVU) {
+    double* sX = sXT;
+    double* const sT = sXT + kTileLandmarks * 3 + 2;
+    double* const sV = sVU;
+    double* const sR = sXT;
     if (P.wstate && !P.wstate[P.lm_tile_win[tile]].active) return;
     const int lane = threadIdx.x & 63;
     const int e0 = lt.e_begin, g0 = kTileEdges * lt.g_chunk;
@@ -649,23 +651,25 @@ __device__ __forceinline__ void apply_tile(const DevProblem& P, const double* __
     }
     wave_lds_sync();
 
-    // ---- stage 2: u_j = V_j^-1 t_j ----
+    // ---- stage 2: u_j = V_j^-1 t_j, stored with X_j in V_j^-1's place ----
     if (lane < nl) {
         const double t[3] = {sT[3 * lane], sT[3 * lane + 1], sT[3 * lane + 2]};
-        double vi[6], u[3];
-#pragma unroll
-        for (int c = 0; c < 6; ++c) vi[c] = sV[6 * lane + c];
+        const double X[3] = {sX[3 * lane], sX[3 * lane + 1], sX[3 * lane + 2]};
+        double2* const vu = reinterpret_cast<double2*>(sV + 6 * lane);
+        const double2 a = vu[0], b = vu[1], c = vu[2];
+        const double vi[6] = {a.x, a.y, b.x, b.y, c.x, c.y};
+        double u[3];
         sym3_mul(vi, t, u);
-        sT[3 * lane] = u[0];
-        sT[3 * lane + 1] = u[1];
-        sT[3 * lane + 2] = u[2];
+        vu[0] = make_double2(u[0], u[1]);
+        vu[1] = make_double2(u[2], X[0]);
+        vu[2] = make_double2(X[1], X[2]);
     }
     wave_lds_sync();
 
     // ---- stage 3: y_e = W_e u_j summed per camera run ----
     // A run that starts and ends inside the lane writes its partial at once.
     // A segment open at the lane's end (the head of a run, or a lane inside
-    // one) leaves its sum in the lane's LDS slot (V^-1's place, now free);
+    // one) leaves its sum in the lane's LDS slot (X's and t's place, now free);
     // the lane that finishes the run adds the slots of the lanes before it,
     // back to the run's first lane, to its own leading sum: plain LDS
     // traffic in a fixed order, no atomics.
@@ -689,8 +693,10 @@ __device__ __forceinline__ void apply_tile(const DevProblem& P, const double* __
     for (int i = 0; i < kApplyPer; ++i) {
         if (i < nv) {
             const int j = (lm4 >> (8 * i)) & 0xff;
-            const double X[3] = {sX[3 * j], sX[3 * j + 1], sX[3 * j + 2]};
-            const double u[3] = {sT[3 * j], sT[3 * j + 1], sT[3 * j + 2]};
+            const double2* const vu = reinterpret_cast<const double2*>(sV + 6 * j);
+            const double2 a = vu[0], b = vu[1], c = vu[2];
+            const double u[3] = {a.x, a.y, b.x};
+            const double X[3] = {b.y, c.x, c.y};
             const double Gd[6] = {(double)G[i][0], (double)G[i][1], (double)G[i][2],
                                   (double)G[i][3], (double)G[i][4], (double)G[i][5]};
             double y[6];
@@ -718,8 +724,10 @@ __device__ __forceinline__ void apply_tile(const DevProblem& P, const double* __
     }
     const bool open = nv > 0 && !fin[nv - 1];
     if (open) {
-#pragma unroll
-        for (int c = 0; c < 6; ++c) sR[6 * lane + c] = acc[c];
+        double2* const o = reinterpret_cast<double2*>(sR + 6 * lane);
+        o[0] = make_double2(acc[0], acc[1]);
+        o[1] = make_double2(acc[2], acc[3]);
+        o[2] = make_double2(acc[4], acc[5]);
     }
     // lanes whose open segment is the start of its run
     const uint64_t starts = __ballot(open && own);
@@ -730,9 +738,16 @@ __device__ __forceinline__ void apply_tile(const DevProblem& P, const double* __
         double t6[6];
 #pragma unroll
         for (int c = 0; c < 6; ++c) t6[c] = lead_acc[c];
-        for (int l = lane - 1; l >= h; --l)
-#pragma unroll
-            for (int c = 0; c < 6; ++c) t6[c] += sR[6 * l + c];
+        for (int l = lane - 1; l >= h; --l) {
+            const double2* const q = reinterpret_cast<const double2*>(sR + 6 * l);
+            const double2 a = q[0], b = q[1], c = q[2];
+            t6[0] += a.x;
+            t6[1] += a.y;
+            t6[2] += b.x;
+            t6[3] += b.y;
+            t6[4] += c.x;
+            t6[5] += c.y;
+        }
         int sl = slot[0];
 #pragma unroll
         for (int i = 1; i < kApplyPer; ++i)
@@ -751,18 +766,19 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_schur_apply(DevProblem P, 
                                                                   double* __restrict__ part, int pcg_iter,
                                                                   int fresh) {
     static_assert(kTileRuns == 64 && kTileLandmarks == 64, "one landmark and one run accumulator per lane");
-    // X per landmark, placed at sX = sXa + (lm_begin & 1) so that the LDS
-    // loads start at a 16-byte boundary in global memory and in LDS
-    __shared__ __align__(16) double sXa[kApplyWaves][kTileLandmarks * 3 + 2];
-    __shared__ double sTa[kApplyWaves][kTileLandmarks * 3];  // t, then u, per landmark
-    // V^-1 per landmark (stages 0-2), then the sums of runs that cross lanes
-    // (stage 3)
-    __shared__ __align__(16) double sVRa[kApplyWaves][kTileLandmarks * 6];
+    // The wave's LDS. sXT: X per landmark at sX = sXT + (lm_begin & 1), so
+    // that the LDS loads start at a 16-byte boundary in global memory and in
+    // LDS, then t per landmark at sT (stages 0-2); the sums of runs that
+    // cross lanes, sR = sXT (stage 3). sVU: V^-1 per landmark (stages 0-2),
+    // which the landmark's owner lane replaces by [u, X] (stages 2-3), so
+    // that stage 3 reads both with three 16-byte loads.
+    __shared__ __align__(16) double sXTa[kApplyWaves][kTileLandmarks * 6 + 2];
+    __shared__ __align__(16) double sVUa[kApplyWaves][kTileLandmarks * 6];
     const int wv = threadIdx.x >> 6;
-    double* sX = sXa[wv];
-    double* const sT = sTa[wv];
-    double* const sV = sVRa[wv];
-    double* const sR = sVRa[wv];
+    double* sX = sXTa[wv];
+    double* const sT = sXTa[wv] + kTileLandmarks * 3 + 2;
+    double* const sV = sVUa[wv];
+    double* const sR = sXTa[wv];
     const int groups = (P.num_lm_tiles + kApplyWaves - 1) / kApplyWaves;
     const int tile = xcd_tile(blockIdx.x, groups) * kApplyWaves + wv;
     if (tile >= P.num_lm_tiles) return;
@@ -897,23 +913,25 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_schur_apply(DevProblem P, 
     }
     wave_lds_sync();
 
-    // ---- stage 2: u_j = V_j^-1 t_j ----
+    // ---- stage 2: u_j = V_j^-1 t_j, stored with X_j in V_j^-1's place ----
     if (lane < nl) {
         const double t[3] = {sT[3 * lane], sT[3 * lane + 1], sT[3 * lane + 2]};
-        double vi[6], u[3];
-#pragma unroll
-        for (int c = 0; c < 6; ++c) vi[c] = sV[6 * lane + c];
+        const double X[3] = {sX[3 * lane], sX[3 * lane + 1], sX[3 * lane + 2]};
+        double2* const vu = reinterpret_cast<double2*>(sV + 6 * lane);
+        const double2 a = vu[0], b = vu[1], c = vu[2];
+        const double vi[6] = {a.x, a.y, b.x, b.y, c.x, c.y};
+        double u[3];
         sym3_mul(vi, t, u);
-        sT[3 * lane] = u[0];
-        sT[3 * lane + 1] = u[1];
-        sT[3 * lane + 2] = u[2];
+        vu[0] = make_double2(u[0], u[1]);
+        vu[1] = make_double2(u[2], X[0]);
+        vu[2] = make_double2(X[1], X[2]);
     }
     wave_lds_sync();
 
     // ---- stage 3: y_e = W_e u_j summed per camera run ----
     // A run that starts and ends inside the lane writes its partial at once.
     // A segment open at the lane's end (the head of a run, or a lane inside
-    // one) leaves its sum in the lane's LDS slot (V^-1's place, now free);
+    // one) leaves its sum in the lane's LDS slot (X's and t's place, now free);
     // the lane that finishes the run adds the slots of the lanes before it,
     // back to the run's first lane, to its own leading sum: plain LDS
     // traffic in a fixed order, no atomics.
@@ -937,8 +955,10 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_schur_apply(DevProblem P, 
     for (int i = 0; i < kApplyPer; ++i) {
         if (i < nv) {
             const int j = (lm4 >> (8 * i)) & 0xff;
-            const double X[3] = {sX[3 * j], sX[3 * j + 1], sX[3 * j + 2]};
-            const double u[3] = {sT[3 * j], sT[3 * j + 1], sT[3 * j + 2]};
+            const double2* const vu = reinterpret_cast<const double2*>(sV + 6 * j);
+            const double2 a = vu[0], b = vu[1], c = vu[2];
+            const double u[3] = {a.x, a.y, b.x};
+            const double X[3] = {b.y, c.x, c.y};
             const double Gd[6] = {(double)G[i][0], (double)G[i][1], (double)G[i][2],
                                   (double)G[i][3], (double)G[i][4], (double)G[i][5]};
             double y[6];
@@ -966,8 +986,10 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_schur_apply(DevProblem P, 
     }
     const bool open = nv > 0 && !fin[nv - 1];
     if (open) {
-#pragma unroll
-        for (int c = 0; c < 6; ++c) sR[6 * lane + c] = acc[c];
+        double2* const o = reinterpret_cast<double2*>(sR + 6 * lane);
+        o[0] = make_double2(acc[0], acc[1]);
+        o[1] = make_double2(acc[2], acc[3]);
+        o[2] = make_double2(acc[4], acc[5]);
     }
     // lanes whose open segment is the start of its run
     const uint64_t starts = __ballot(open && own);
@@ -978,9 +1000,16 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_schur_apply(DevProblem P, 
         double t6[6];
 #pragma unroll
         for (int c = 0; c < 6; ++c) t6[c] = lead_acc[c];
-        for (int l = lane - 1; l >= h; --l)
-#pragma unroll
-            for (int c = 0; c < 6; ++c) t6[c] += sR[6 * l + c];
+        for (int l = lane - 1; l >= h; --l) {
+            const double2* const q = reinterpret_cast<const double2*>(sR + 6 * l);
+            const double2 a = q[0], b = q[1], c = q[2];
+            t6[0] += a.x;
+            t6[1] += a.y;
+            t6[2] += b.x;
+            t6[3] += b.y;
+            t6[4] += c.x;
+            t6[5] += c.y;
+        }
         int sl = slot[0];
 #pragma unroll
         for (int i = 1; i < kApplyPer; ++i)
@@ -1003,9 +1032,8 @@ __global__ __launch_bounds__(64, 4) void k_schur_apply_loop(DevProblem P, const 
                                                          const TS* __restrict__ Gl,
                                                          const double* __restrict__ Vinv,
                                                          double* __restrict__ part, int pcg_iter, int fresh) {
-    __shared__ __align__(16) double sXa[kTileLandmarks * 3 + 2];
-    __shared__ double sTa[kTileLandmarks * 3];
-    __shared__ __align__(16) double sVRa[kTileLandmarks * 6];
+    __shared__ __align__(16) double sXTa[kTileLandmarks * 6 + 2];
+    __shared__ __align__(16) double sVUa[kTileLandmarks * 6];
     const int T = P.num_lm_tiles;
     const int per = (T + 7) >> 3;
     const int x = blockIdx.x & 7, stride = gridDim.x >> 3;
@@ -1024,7 +1052,7 @@ __global__ __launch_bounds__(64, 4) void k_schur_apply_loop(DevProblem P, const 
         const int nt = min(next, t1 - 1);
         const LmTile ln = P.lm_tiles[nt];
         const TileRuns trn = load_tile_runs(P, nt, lane);
-        apply_tile<TS>(P, v1, v2, Gl, Vinv, part, tile, lt, tr, beta, sXa, sTa, sVRa);
+        apply_tile<TS>(P, v1, v2, Gl, Vinv, part, tile, lt, tr, beta, sXTa, sVUa);
         if (next >= t1) break;
         __builtin_amdgcn_s_waitcnt(0);  // this tile's LDS traffic is done before the next tile's loads land
         wave_lds_sync();
