@@ -50,12 +50,40 @@ __device__ __forceinline__ int wave_flag(int32_t* flag) {
 }
 __device__ __forceinline__ int wave_stop(PcgState* st) { return wave_flag(&st->stop); }
 
-// Sum over a wave with the same result in every lane (xor butterfly:
-// each step adds the same two values in every lane).
+__device__ __forceinline__ double from_halves(unsigned lo, unsigned hi) {
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// The value of the partner lane under DPP control CTRL (within a row).
+template <int CTRL>
+__device__ __forceinline__ double dpp_partner(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)b, CTRL, 0xf, 0xf, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, false);
+    return from_halves(lo, hi);
+}
+
+// Sum over a wave with the same result in every lane, in VALU lane moves
+// only (no LDS): each step adds a lane's value and its partner's, and the
+// two lanes of a pair add the same two values. Partners: lane ^ 1, lane ^ 2
+// (quad permutes), the mirror lane within 8 and within 16 (row mirrors),
+// then the two rows of a row pair and the two halves of the wave (gfx950
+// permlane swaps, which leave the lower row's or half's value in the first
+// result and the upper one's in the second, in both lanes of a pair). All
+// lanes must be active.
 __device__ __forceinline__ double wave_allsum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    v = v + dpp_partner<0xB1>(v);   // quad_perm [1,0,3,2]
+    v = v + dpp_partner<0x4E>(v);   // quad_perm [2,3,0,1]
+    v = v + dpp_partner<0x141>(v);  // row_half_mirror
+    v = v + dpp_partner<0x140>(v);  // row_mirror
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const auto l16 = __builtin_amdgcn_permlane16_swap((unsigned)b, (unsigned)b, false, false);
+    const auto h16 = __builtin_amdgcn_permlane16_swap((unsigned)(b >> 32), (unsigned)(b >> 32), false, false);
+    v = from_halves(l16[0], h16[0]) + from_halves(l16[1], h16[1]);
+    const unsigned long long c = (unsigned long long)__double_as_longlong(v);
+    const auto l32 = __builtin_amdgcn_permlane32_swap((unsigned)c, (unsigned)c, false, false);
+    const auto h32 = __builtin_amdgcn_permlane32_swap((unsigned)(c >> 32), (unsigned)(c >> 32), false, false);
+    return from_halves(l32[0], h32[0]) + from_halves(l32[1], h32[1]);
 }
 
 // Sum of pairs (a[2i], a[2i+1]) over i < n, the same in every lane of every
