@@ -54,6 +54,14 @@ __device__ __forceinline__ double from_halves(unsigned lo, unsigned hi) {
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+// Lane `src`'s value (src the same in every lane), without LDS.
+__device__ __forceinline__ double lane_value(double v, int src) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, src);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), src);
+    return from_halves(lo, hi);
+}
+
 // The value of the partner lane under DPP control CTRL (within a row).
 template <int CTRL>
 __device__ __forceinline__ double dpp_partner(double v) {

@@ -555,12 +555,24 @@ __device__ __forceinline__ void apply_tile(const DevProblem& P, const double* __
     const uint32_t hb = (uint32_t)(hw >> sh) & 0xf;
     int run = __popcll(hw & ((1ull << sh) - 1)) - 1 + (lane >= 16 ? __popcll(h0) : 0) +
               (lane >= 32 ? __popcll(h1) : 0) + (lane >= 48 ? __popcll(h2) : 0);
-    int rix[kApplyPer], key[kApplyPer];
+    int rix[kApplyPer];
 #pragma unroll
     for (int i = 0; i < kApplyPer; ++i) {
         run += (hb >> i) & 1;
         rix[i] = run & (kTileRuns - 1);
-        key[i] = __shfl(rpose, rix[i], 64);
+    }
+    // the poses of the lane's first and last runs by lane shuffles; a run
+    // wholly inside the lane's middle positions takes two more, only in the
+    // waves that have one
+    const bool mid = (rix[1] != rix[0] && rix[1] != rix[3]) || (rix[2] != rix[0] && rix[2] != rix[3]);
+    int key[kApplyPer];
+    key[0] = __shfl(rpose, rix[0], 64);
+    key[3] = __shfl(rpose, rix[3], 64);
+    key[1] = rix[1] == rix[0] ? key[0] : key[3];
+    key[2] = rix[2] == rix[3] ? key[3] : key[0];
+    if (__any(mid)) {
+        key[1] = __shfl(rpose, rix[1], 64);
+        key[2] = __shfl(rpose, rix[2], 64);
     }
     // G of the four positions (the FP64 copy by edge, clamped to the tile's
     // last one, or the problem's first edge for a tile without edges); no
@@ -678,13 +690,20 @@ __device__ __forceinline__ void apply_tile(const DevProblem& P, const double* __
     // from the lane holding the run.
     const uint32_t hb_next = __shfl_down(hb, 1, 64);  // the next lane's bits
     bool head[kApplyPer], fin[kApplyPer];
-    int slot[kApplyPer];
 #pragma unroll
     for (int i = 0; i < kApplyPer; ++i) {
         const bool next_head = i + 1 < kApplyPer ? (hb >> (i + 1)) & 1 : hb_next & 1;
         head[i] = i < nv && ((hb >> i) & 1);
         fin[i] = i < nv && (s0 + i == n - 1 || next_head);
-        slot[i] = __shfl(rslot, rix[i], 64);
+    }
+    int slot[kApplyPer];  // as the poses above
+    slot[0] = __shfl(rslot, rix[0], 64);
+    slot[3] = __shfl(rslot, rix[3], 64);
+    slot[1] = rix[1] == rix[0] ? slot[0] : slot[3];
+    slot[2] = rix[2] == rix[3] ? slot[3] : slot[0];
+    if (__any(mid)) {
+        slot[1] = __shfl(rslot, rix[1], 64);
+        slot[2] = __shfl(rslot, rix[2], 64);
     }
     double acc[6], lead_acc[6];
     bool own = false;   // the current segment's run starts in this lane
@@ -812,12 +831,24 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_schur_apply(DevProblem P, 
     const uint32_t hb = (uint32_t)(hw >> sh) & 0xf;
     int run = __popcll(hw & ((1ull << sh) - 1)) - 1 + (lane >= 16 ? __popcll(h0) : 0) +
               (lane >= 32 ? __popcll(h1) : 0) + (lane >= 48 ? __popcll(h2) : 0);
-    int rix[kApplyPer], key[kApplyPer];
+    int rix[kApplyPer];
 #pragma unroll
     for (int i = 0; i < kApplyPer; ++i) {
         run += (hb >> i) & 1;
         rix[i] = run & (kTileRuns - 1);
-        key[i] = __shfl(rpose, rix[i], 64);
+    }
+    // the poses of the lane's first and last runs by lane shuffles; a run
+    // wholly inside the lane's middle positions takes two more, only in the
+    // waves that have one
+    const bool mid = (rix[1] != rix[0] && rix[1] != rix[3]) || (rix[2] != rix[0] && rix[2] != rix[3]);
+    int key[kApplyPer];
+    key[0] = __shfl(rpose, rix[0], 64);
+    key[3] = __shfl(rpose, rix[3], 64);
+    key[1] = rix[1] == rix[0] ? key[0] : key[3];
+    key[2] = rix[2] == rix[3] ? key[3] : key[0];
+    if (__any(mid)) {
+        key[1] = __shfl(rpose, rix[1], 64);
+        key[2] = __shfl(rpose, rix[2], 64);
     }
     // G of the four positions (the FP64 copy by edge, clamped to the tile's
     // last one, or the problem's first edge for a tile without edges); no
@@ -940,13 +971,20 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_schur_apply(DevProblem P, 
     // from the lane holding the run.
     const uint32_t hb_next = __shfl_down(hb, 1, 64);  // the next lane's bits
     bool head[kApplyPer], fin[kApplyPer];
-    int slot[kApplyPer];
 #pragma unroll
     for (int i = 0; i < kApplyPer; ++i) {
         const bool next_head = i + 1 < kApplyPer ? (hb >> (i + 1)) & 1 : hb_next & 1;
         head[i] = i < nv && ((hb >> i) & 1);
         fin[i] = i < nv && (s0 + i == n - 1 || next_head);
-        slot[i] = __shfl(rslot, rix[i], 64);
+    }
+    int slot[kApplyPer];  // as the poses above
+    slot[0] = __shfl(rslot, rix[0], 64);
+    slot[3] = __shfl(rslot, rix[3], 64);
+    slot[1] = rix[1] == rix[0] ? slot[0] : slot[3];
+    slot[2] = rix[2] == rix[3] ? slot[3] : slot[0];
+    if (__any(mid)) {
+        slot[1] = __shfl(rslot, rix[1], 64);
+        slot[2] = __shfl(rslot, rix[2], 64);
     }
     double acc[6], lead_acc[6];
     bool own = false;   // the current segment's run starts in this lane
@@ -1066,9 +1104,12 @@ __global__ __launch_bounds__(64, 4) void k_schur_apply_loop(DevProblem P, const 
 
 // ---------------------------------------------------------------------------
 // Pose side: one wave per pose sums its runs' partials (lanes stride the
-// runs, then a butterfly that leaves the same sum in every lane).
+// runs, then a wave sum that leaves the same value in every lane).
 // ---------------------------------------------------------------------------
-template <int NV>
+// kLaneMoves: the wave sum in lane moves (the PCG pose kernel); otherwise
+// an xor butterfly of lane shuffles (the once-per-LM-iteration kernels,
+// whose right-hand side the dense solves share).
+template <int NV, bool kLaneMoves = false>
 __device__ __forceinline__ void pose_run_sum(const DevProblem& P, const double* __restrict__ part, int p, int lane,
                                              double s[NV]) {
 #pragma unroll
@@ -1091,10 +1132,15 @@ __device__ __forceinline__ void pose_run_sum(const DevProblem& P, const double* 
 #pragma unroll
                 for (int i = 0; i < NV; ++i) s[i] += q[u][i];
     }
+    if constexpr (kLaneMoves) {
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
+        for (int i = 0; i < NV; ++i) s[i] = wave_allsum(s[i]);
+    } else {
 #pragma unroll
-        for (int i = 0; i < NV; ++i) s[i] += __shfl_xor(s[i], off, 64);
+        for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+            for (int i = 0; i < NV; ++i) s[i] += __shfl_xor(s[i], off, 64);
+    }
 }
 
 // q_c = (sum_j U_cj d_j) + lambda d_c - s_c on lane c < 6, d_j shuffled in.
@@ -1232,14 +1278,14 @@ __global__ __launch_bounds__(kBlock) void k_pcg_pose(DevProblem P, const double*
 #pragma unroll
     for (int j = 0; j < 6; ++j) u[j] = U[c <= j ? up6(c, j) : up6(j, c)];
     double s[6];
-    pose_run_sum<6>(P, part, in ? p : 0, lane, s);
+    pose_run_sum<6, true>(P, part, in ? p : 0, lane, s);
     if (__builtin_amdgcn_readfirstlane(stop)) return;
     double v[2] = {0.0, 0.0};
     if (in && pv >= 0) {
         const double dc = z + beta * d_old;
         double acc = 0.0;
 #pragma unroll
-        for (int j = 0; j < 6; ++j) acc += u[j] * __shfl(dc, j, 64);
+        for (int j = 0; j < 6; ++j) acc += u[j] * lane_value(dc, j);
         const double q = acc + lambda * dc - s[c];
         if (lane < 6) {
             P.d[k] = dc;
