@@ -913,7 +913,7 @@ def main():
     ap.add_argument("--window-streams", type=int, default=2, help="window lanes of the server (--workload serve)")
     ap.add_argument("--submap-streams", type=int, default=6, help="submap lanes of the server (--workload serve)")
     ap.add_argument("--windows", type=int, default=256, help="windows in the batch (--workload windows)")
-    ap.add_argument("--profile-stride", type=int, default=16,
+    ap.add_argument("--profile-stride", type=int, default=32,
                     help="record kernel times around every n-th launch of each kernel family")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default PCG launch schedule")
     ap.add_argument("--replay", default="",
@@ -985,7 +985,7 @@ def main():
             f"{tuning['default_seconds'] * 1e3:.1f} ms for the default, 3 LM iterations) in "
             f"{time.perf_counter() - t1:.2f} s")
 
-    # HIP events around every 16th launch of each kernel family: enough launches
+    # HIP events around every 32nd launch of each kernel family: enough launches
     # for the averages, few enough that recording them does not slow the solve
     dev.profile(True, stride=args.profile_stride)
     lm_iters = 0
