@@ -980,9 +980,9 @@ def main():
     tuning = None
     if not args.no_autotune:
         t1 = time.perf_counter()
-        tuning = prep.autotune(opts, lm_iterations=3)
+        tuning = prep.autotune(opts, lm_iterations=6)
         log(f"bench: autotuned PCG schedule {tuning['tuning']} ({tuning['seconds'] * 1e3:.1f} ms vs "
-            f"{tuning['default_seconds'] * 1e3:.1f} ms for the default, 3 LM iterations) in "
+            f"{tuning['default_seconds'] * 1e3:.1f} ms for the default, 6 LM iterations) in "
             f"{time.perf_counter() - t1:.2f} s")
 
     # HIP events around every 32nd launch of each kernel family: enough launches

@@ -534,7 +534,7 @@ int32_t vba_set_tuning(vba_context* ctx, const vba_tuning* tuning);
 int32_t vba_get_tuning(const vba_context* ctx, vba_tuning* tuning);
 /* Times each schedule of a fixed candidate set on `prepared` (a PCG solver in
  * `options`, max_iterations replaced by lm_iterations when that is nonzero;
- * each candidate solved twice, the faster kept), checks that every candidate
+ * each candidate solved three times, the fastest kept), checks that every candidate
  * gives the default schedule's result bit for bit (VBA_ERR_DEVICE
  * otherwise), sets the fastest on the context and returns it in *best with
  * its seconds and the default schedule's. */

@@ -759,7 +759,7 @@ int32_t vba_autotune(vba_context* ctx, vba_prepared* prepared, const vba_options
             ctx->tuning = kCandidates[c];
         }
         double t = 0.0;
-        for (int rep = 0; rep < 2; ++rep) {
+        for (int rep = 0; rep < 3; ++rep) {
             const auto t0 = std::chrono::steady_clock::now();
             const int32_t st = vba_solve_prepared(ctx, prepared, &o, stats.data(), cap, &res);
             const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
