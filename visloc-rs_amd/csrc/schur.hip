@@ -516,20 +516,30 @@ __device__ __forceinline__ TileRuns load_tile_runs(const DevProblem& P, int tile
     return r;
 }
 
-// One landmark tile of the operator, by one wave, with the PCG scalar beta
-// and the tile's run tables given. sXT (6 * 64 + 2 doubles) and sVU
-// (6 * 64), both 16-byte aligned, are the wave's LDS, laid out as
-// k_schur_apply lays out its own.
-template <typename TS>
+// One landmark tile of the operator, by one wave, given the tile's bounds
+// and run tables. With kPrologue the PCG scalars of iteration pcg_iter are
+// derived here, their inputs loaded with the tile's data (a stopped solve
+// returns before stage 1); without it `beta` is given. sXT (6 * 64 + 2
+// doubles) and sVU (6 * 64), both 16-byte aligned, are the wave's LDS: sXT
+// holds X per landmark at sX = sXT + (lm_begin & 1), so that the LDS loads
+// start at a 16-byte boundary in global memory and in LDS, then t per
+// landmark at sT (stages 0-2), then the sums of runs that cross lanes, sR =
+// sXT (stage 3); sVU holds V^-1 per landmark (stages 0-2), which the
+// landmark's owner lane replaces by [u, X] (stages 2-3), so that stage 3
+// reads both with three 16-byte loads.
+template <typename TS, bool kPrologue>
 __device__ __forceinline__ void apply_tile(const DevProblem& P, const double* __restrict__ v1,
                                            const double* __restrict__ v2, const TS* __restrict__ Gl,
                                            const double* __restrict__ Vinv, double* __restrict__ part,
-                                           int tile, const LmTile lt, const TileRuns tr, double beta,
-                                           double* const sXT, double* const sVU) {
+                                           int tile, const LmTile lt, const TileRuns tr, int pcg_iter, int fresh,
+                                           double beta, double* const sXT, double* const sVU) {
     double* sX = sXT;
     double* const sT = sXT + kTileLandmarks * 3 + 2;
     double* const sV = sVU;
     double* const sR = sXT;
+    // the PCG state and partials are bound whenever the operator runs
+    PrologueIn pin;
+    if constexpr (kPrologue) pin = pcg_prologue_load(P, pcg_iter);
     if (P.wstate && !P.wstate[P.lm_tile_win[tile]].active) return;
     const int lane = threadIdx.x & 63;
     const int e0 = lt.e_begin, g0 = kTileEdges * lt.g_chunk;
@@ -624,6 +634,12 @@ __device__ __forceinline__ void apply_tile(const DevProblem& P, const double* __
     double z0[6], w0[6];
     load6(v1 + 6 * (size_t)kp, z0);
     if (v2) load6(v2 + 6 * (size_t)kp, w0);
+    if constexpr (kPrologue) {
+        if (pcg_iter >= 0 && !pcg_prologue_finish(P, pcg_iter, fresh, pin, &beta)) {
+            __builtin_amdgcn_s_waitcnt(0);  // no LDS write may land after the wave ends
+            return;
+        }
+    }
     if (one) {
         apply_one_landmark<TS>(P, v1, v2, beta, Gl, Vinv, part, lt.lm_begin, e0, g0, n, lane);
         return;
@@ -785,277 +801,16 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_schur_apply(DevProblem P, 
                                                                   double* __restrict__ part, int pcg_iter,
                                                                   int fresh) {
     static_assert(kTileRuns == 64 && kTileLandmarks == 64, "one landmark and one run accumulator per lane");
-    // The wave's LDS. sXT: X per landmark at sX = sXT + (lm_begin & 1), so
-    // that the LDS loads start at a 16-byte boundary in global memory and in
-    // LDS, then t per landmark at sT (stages 0-2); the sums of runs that
-    // cross lanes, sR = sXT (stage 3). sVU: V^-1 per landmark (stages 0-2),
-    // which the landmark's owner lane replaces by [u, X] (stages 2-3), so
-    // that stage 3 reads both with three 16-byte loads.
     __shared__ __align__(16) double sXTa[kApplyWaves][kTileLandmarks * 6 + 2];
     __shared__ __align__(16) double sVUa[kApplyWaves][kTileLandmarks * 6];
     const int wv = threadIdx.x >> 6;
-    double* sX = sXTa[wv];
-    double* const sT = sXTa[wv] + kTileLandmarks * 3 + 2;
-    double* const sV = sVUa[wv];
-    double* const sR = sXTa[wv];
     const int groups = (P.num_lm_tiles + kApplyWaves - 1) / kApplyWaves;
     const int tile = xcd_tile(blockIdx.x, groups) * kApplyWaves + wv;
     if (tile >= P.num_lm_tiles) return;
-    const int lane = threadIdx.x & 63;
-
-    // ---- stage 0: the tile's data, the inputs of the PCG scalars, V^-1 and
-    // X, and the first pose's vectors, all issued before the wave waits on
-    // any of them (a stopped launch drops them) ----
-    // The lane's four positions come from the tile's run tables by tile
-    // index alone, in flight with the tile's bounds: the mask of positions
-    // that start a camera run, run r's pose and slot in lane r, and the
-    // positions' tile-local landmarks. Positions from nv on are never read.
-    const TileRuns tr = load_tile_runs(P, tile, lane);
-    const uint64_t h0 = tr.h[0], h1 = tr.h[1], h2 = tr.h[2], h3 = tr.h[3];
-    const int rpose = tr.rpose, rslot = tr.rslot;
-    const uint32_t lm4 = tr.lm4;
+    // the run tables by tile index alone, in flight with the tile's bounds
+    const TileRuns tr = load_tile_runs(P, tile, threadIdx.x & 63);
     const LmTile lt = P.lm_tiles[tile];
-    // the PCG state and partials are bound whenever the operator runs
-    const PrologueIn pin = pcg_prologue_load(P, pcg_iter);
-    if (P.wstate && !P.wstate[P.lm_tile_win[tile]].active) return;
-    const int e0 = lt.e_begin, g0 = kTileEdges * lt.g_chunk;
-    const int n = lt.e_end - lt.e_begin, nl = lt.lm_end - lt.lm_begin;
-    const bool one = nl == 1 && n > kTileRuns;
-
-    const int s0 = kApplyPer * lane;
-    const int nv = one ? 0 : max(0, min(kApplyPer, n - s0));
-    // this lane's run-start bits and the run of each position: the runs
-    // started before the lane's first position, then the lane's own starts
-    const uint64_t hw = lane < 16 ? h0 : lane < 32 ? h1 : lane < 48 ? h2 : h3;
-    const int sh = kApplyPer * (lane & 15);
-    const uint32_t hb = (uint32_t)(hw >> sh) & 0xf;
-    int run = __popcll(hw & ((1ull << sh) - 1)) - 1 + (lane >= 16 ? __popcll(h0) : 0) +
-              (lane >= 32 ? __popcll(h1) : 0) + (lane >= 48 ? __popcll(h2) : 0);
-    int rix[kApplyPer];
-#pragma unroll
-    for (int i = 0; i < kApplyPer; ++i) {
-        run += (hb >> i) & 1;
-        rix[i] = run & (kTileRuns - 1);
-    }
-    // the poses of the lane's first and last runs by lane shuffles; a run
-    // wholly inside the lane's middle positions takes two more, only in the
-    // waves that have one
-    const bool mid = (rix[1] != rix[0] && rix[1] != rix[3]) || (rix[2] != rix[0] && rix[2] != rix[3]);
-    int key[kApplyPer];
-    key[0] = __shfl(rpose, rix[0], 64);
-    key[3] = __shfl(rpose, rix[3], 64);
-    key[1] = rix[1] == rix[0] ? key[0] : key[3];
-    key[2] = rix[2] == rix[3] ? key[3] : key[0];
-    if (__any(mid)) {
-        key[1] = __shfl(rpose, rix[1], 64);
-        key[2] = __shfl(rpose, rix[2], 64);
-    }
-    // G of the four positions (the FP64 copy by edge, clamped to the tile's
-    // last one, or the problem's first edge for a tile without edges); no
-    // branch around the loads, so none waits for another
-    TS G[kApplyPer][6];
-#pragma unroll
-    for (int i = 0; i < kApplyPer; ++i) {
-        if constexpr (std::is_same<TS, float>::value) {
-            // slot i of the tile's lane-major chunk: 1 KiB + 512 B per wave
-            const float* gb = Gl + (size_t)lt.g_chunk * kG32Chunk + 384 * i;
-            const float4 a = *(const float4*)(gb + 4 * lane);
-            const float2 b = *(const float2*)(gb + 256 + 2 * lane);
-            G[i][0] = a.x;
-            G[i][1] = a.y;
-            G[i][2] = a.z;
-            G[i][3] = a.w;
-            G[i][4] = b.x;
-            G[i][5] = b.y;
-        } else {
-            load6(Gl + 6 * (size_t)(n > 0 ? e0 + min(s0 + i, n - 1) : 0), G[i]);
-        }
-    }
-    // V^-1 and X of the tile's landmarks go straight from global memory into
-    // LDS (their layout there is the global one), in flight together with
-    // the loads above: 16-byte pieces, lane i's piece at byte 16 i of each
-    // 1 KiB chunk, over the tile's contiguous landmark range. The X range
-    // starts one double early when lm_begin is odd, and its last piece may
-    // end 8 bytes past the range (the landmark buffers carry 16 bytes of
-    // slack).
-    if (!one) {
-        const char* gv = reinterpret_cast<const char*>(Vinv + 6 * (size_t)lt.lm_begin);
-        const int xs = lt.lm_begin & 1;
-        const char* gx = reinterpret_cast<const char*>(P.lms + 3 * (size_t)lt.lm_begin - xs);
-        char* lx = reinterpret_cast<char*>(sX);
-        sX += xs;
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-            if (1024 * c + 16 * lane < 48 * nl)
-                __builtin_amdgcn_global_load_lds(gv + 1024 * c + 16 * lane, (lds_void_ptr)((char*)sV + 1024 * c), 16,
-                                                 0, 0);
-#pragma unroll
-        for (int c = 0; c < 2; ++c)
-            if (1024 * c + 16 * lane < 8 * xs + 24 * nl)
-                __builtin_amdgcn_global_load_lds(gx + 1024 * c + 16 * lane, (lds_void_ptr)(lx + 1024 * c), 16, 0, 0);
-    }
-    // the z and d of this lane's first pose (most lanes' positions belong to
-    // one camera run): the wave's first wait is the one for key[0]
-    const int kp = nv > 0 ? key[0] : 0;
-    double z0[6], w0[6];
-    load6(v1 + 6 * (size_t)kp, z0);
-    if (v2) load6(v2 + 6 * (size_t)kp, w0);
-    double beta = 0.0;
-    if (pcg_iter >= 0 && !pcg_prologue_finish(P, pcg_iter, fresh, pin, &beta)) {
-        __builtin_amdgcn_s_waitcnt(0);  // no LDS write may land after the wave ends
-        return;
-    }
-    if (one) {
-        apply_one_landmark<TS>(P, v1, v2, beta, Gl, Vinv, part, lt.lm_begin, e0, g0, n, lane);
-        return;
-    }
-    if (lane < nl) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) sT[3 * lane + c] = 0.0;
-    }
-    __builtin_amdgcn_s_waitcnt(0);  // the LDS loads have landed
-    wave_lds_sync();
-
-    // ---- stage 1: t_j += W_e^T d_p, one d per pose run of this lane ----
-    {
-        int pcur = INT_MIN;
-        double d[6];
-#pragma unroll
-        for (int i = 0; i < kApplyPer; ++i) {
-            if (i < nv) {
-                if (key[i] != pcur) {
-                    if (i == 0) {
-#pragma unroll
-                        for (int c = 0; c < 6; ++c) d[c] = v2 ? z0[c] + beta * w0[c] : z0[c];
-                    } else {
-                        load_d(v1, v2, beta, key[i], d);
-                    }
-                    pcur = key[i];
-                }
-                const int j = (lm4 >> (8 * i)) & 0xff;
-                const double X[3] = {sX[3 * j], sX[3 * j + 1], sX[3 * j + 2]};
-                double t[3];
-                edge_wtd(G[i], d, X, t);
-                atomicAdd(&sT[3 * j], t[0]);
-                atomicAdd(&sT[3 * j + 1], t[1]);
-                atomicAdd(&sT[3 * j + 2], t[2]);
-            }
-        }
-    }
-    wave_lds_sync();
-
-    // ---- stage 2: u_j = V_j^-1 t_j, stored with X_j in V_j^-1's place ----
-    if (lane < nl) {
-        const double t[3] = {sT[3 * lane], sT[3 * lane + 1], sT[3 * lane + 2]};
-        const double X[3] = {sX[3 * lane], sX[3 * lane + 1], sX[3 * lane + 2]};
-        double2* const vu = reinterpret_cast<double2*>(sV + 6 * lane);
-        const double2 a = vu[0], b = vu[1], c = vu[2];
-        const double vi[6] = {a.x, a.y, b.x, b.y, c.x, c.y};
-        double u[3];
-        sym3_mul(vi, t, u);
-        vu[0] = make_double2(u[0], u[1]);
-        vu[1] = make_double2(u[2], X[0]);
-        vu[2] = make_double2(X[1], X[2]);
-    }
-    wave_lds_sync();
-
-    // ---- stage 3: y_e = W_e u_j summed per camera run ----
-    // A run that starts and ends inside the lane writes its partial at once.
-    // A segment open at the lane's end (the head of a run, or a lane inside
-    // one) leaves its sum in the lane's LDS slot (X's and t's place, now free);
-    // the lane that finishes the run adds the slots of the lanes before it,
-    // back to the run's first lane, to its own leading sum: plain LDS
-    // traffic in a fixed order, no atomics.
-    // A position starts a run when its bit is set and ends one when the next
-    // position starts one or it is the tile's last; each run's slot comes
-    // from the lane holding the run.
-    const uint32_t hb_next = __shfl_down(hb, 1, 64);  // the next lane's bits
-    bool head[kApplyPer], fin[kApplyPer];
-#pragma unroll
-    for (int i = 0; i < kApplyPer; ++i) {
-        const bool next_head = i + 1 < kApplyPer ? (hb >> (i + 1)) & 1 : hb_next & 1;
-        head[i] = i < nv && ((hb >> i) & 1);
-        fin[i] = i < nv && (s0 + i == n - 1 || next_head);
-    }
-    int slot[kApplyPer];  // as the poses above
-    slot[0] = __shfl(rslot, rix[0], 64);
-    slot[3] = __shfl(rslot, rix[3], 64);
-    slot[1] = rix[1] == rix[0] ? slot[0] : slot[3];
-    slot[2] = rix[2] == rix[3] ? slot[3] : slot[0];
-    if (__any(mid)) {
-        slot[1] = __shfl(rslot, rix[1], 64);
-        slot[2] = __shfl(rslot, rix[2], 64);
-    }
-    double acc[6], lead_acc[6];
-    bool own = false;   // the current segment's run starts in this lane
-    int lead = -1;      // position whose run started in an earlier lane and ends here
-#pragma unroll
-    for (int i = 0; i < kApplyPer; ++i) {
-        if (i < nv) {
-            const int j = (lm4 >> (8 * i)) & 0xff;
-            const double2* const vu = reinterpret_cast<const double2*>(sV + 6 * j);
-            const double2 a = vu[0], b = vu[1], c = vu[2];
-            const double u[3] = {a.x, a.y, b.x};
-            const double X[3] = {b.y, c.x, c.y};
-            const double Gd[6] = {(double)G[i][0], (double)G[i][1], (double)G[i][2],
-                                  (double)G[i][3], (double)G[i][4], (double)G[i][5]};
-            double y[6];
-            edge_wu(Gd, u, X, y);
-            if (i == 0 || fin[i - 1]) {  // a new segment of this lane
-                own = head[i];
-#pragma unroll
-                for (int c = 0; c < 6; ++c) acc[c] = y[c];
-            } else {
-#pragma unroll
-                for (int c = 0; c < 6; ++c) acc[c] += y[c];
-            }
-            if (fin[i]) {
-                if (own) {
-                    double* o = part + 6 * (size_t)slot[i];
-#pragma unroll
-                    for (int c = 0; c < 6; ++c) o[c] = acc[c];
-                } else {  // only a lane's first segment can have started earlier
-                    lead = i;
-#pragma unroll
-                    for (int c = 0; c < 6; ++c) lead_acc[c] = acc[c];
-                }
-            }
-        }
-    }
-    const bool open = nv > 0 && !fin[nv - 1];
-    if (open) {
-        double2* const o = reinterpret_cast<double2*>(sR + 6 * lane);
-        o[0] = make_double2(acc[0], acc[1]);
-        o[1] = make_double2(acc[2], acc[3]);
-        o[2] = make_double2(acc[4], acc[5]);
-    }
-    // lanes whose open segment is the start of its run
-    const uint64_t starts = __ballot(open && own);
-    wave_lds_sync();
-    if (lead >= 0) {
-        const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-        const int h = 63 - __builtin_clzll(starts & below);  // the run's first lane
-        double t6[6];
-#pragma unroll
-        for (int c = 0; c < 6; ++c) t6[c] = lead_acc[c];
-        for (int l = lane - 1; l >= h; --l) {
-            const double2* const q = reinterpret_cast<const double2*>(sR + 6 * l);
-            const double2 a = q[0], b = q[1], c = q[2];
-            t6[0] += a.x;
-            t6[1] += a.y;
-            t6[2] += b.x;
-            t6[3] += b.y;
-            t6[4] += c.x;
-            t6[5] += c.y;
-        }
-        int sl = slot[0];
-#pragma unroll
-        for (int i = 1; i < kApplyPer; ++i)
-            if (i == lead) sl = slot[i];
-        double* o = part + 6 * (size_t)sl;
-#pragma unroll
-        for (int c = 0; c < 6; ++c) o[c] = t6[c];
-    }
+    apply_tile<TS, true>(P, v1, v2, Gl, Vinv, part, tile, lt, tr, pcg_iter, fresh, 0.0, sXTa[wv], sVUa[wv]);
 }
 
 // The same operator with a fixed grid of resident waves: XCD x takes the
@@ -1090,7 +845,7 @@ __global__ __launch_bounds__(64, 4) void k_schur_apply_loop(DevProblem P, const 
         const int nt = min(next, t1 - 1);
         const LmTile ln = P.lm_tiles[nt];
         const TileRuns trn = load_tile_runs(P, nt, lane);
-        apply_tile<TS>(P, v1, v2, Gl, Vinv, part, tile, lt, tr, beta, sXTa, sVUa);
+        apply_tile<TS, false>(P, v1, v2, Gl, Vinv, part, tile, lt, tr, -1, 0, beta, sXTa, sVUa);
         if (next >= t1) break;
         __builtin_amdgcn_s_waitcnt(0);  // this tile's LDS traffic is done before the next tile's loads land
         wave_lds_sync();
