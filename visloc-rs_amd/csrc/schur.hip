@@ -146,7 +146,7 @@ __global__ __launch_bounds__(kTileThreads) void k_schur_tile(DevProblem P, const
     }
     const int tid = threadIdx.x;
     const LmTile lt = P.lm_tiles[tile];
-    const int e0 = P.lm_edge_ptr[lt.lm_begin], e1 = P.lm_edge_ptr[lt.lm_end];
+    const int e0 = lt.e_begin, e1 = lt.e_end;  // lm_edge_ptr at the tile's landmark bounds
     const int n = e1 - e0, nl = lt.lm_end - lt.lm_begin;
 
     if (n > kTileEdges) {
@@ -210,7 +210,22 @@ __global__ __launch_bounds__(kTileThreads) void k_schur_tile(DevProblem P, const
         return;
     }
 
-    // ---- stage 0: landmark positions, edge owners, this lane's positions ----
+    // ---- stage 0: this lane's positions, landmark positions, edge owners ----
+    // The positions load first, with no branch around a load (clamped to the
+    // tile's last edge; the edge-less tile reads edge 0), so that no load
+    // waits for another; past-the-end values are replaced after all loads
+    // are issued.
+    const int s0 = kPer * tid;
+    const int nv = max(0, min(kPer, n - s0));
+    int kk[kPer], key[kPer];
+    TS G[kPer][6];
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+        const int e = n > 0 ? e0 + min(s0 + i, n - 1) : 0;
+        kk[i] = P.tile_perm[e];
+        key[i] = P.s_pose[e];
+        load6(Gl + 6 * (size_t)e, G[i]);
+    }
     double vpre[6];  // this owner's V^-1, loaded early
     if (tid < nl) {
         const int l = lt.lm_begin + tid;
@@ -225,19 +240,13 @@ __global__ __launch_bounds__(kTileThreads) void k_schur_tile(DevProblem P, const
         if (OP == OP_RHS) sym3_mul(P.Vinv64 + 6 * (size_t)l, P.gl + 3 * (size_t)l, sU + 3 * tid);
     }
     if (tid < kWaves) sEndPos[tid] = -1;
-    const int s0 = kPer * tid;
-    const int nv = max(0, min(kPer, n - s0));
-    int kk[kPer], key[kPer];
-    TS G[kPer][6];
 #pragma unroll
     for (int i = 0; i < kPer; ++i) {
-        kk[i] = 0;
-        key[i] = -1 - (s0 + i);  // unique for positions past the end
         if (i < nv) {
-            kk[i] = P.tile_perm[e0 + s0 + i];
-            key[i] = P.s_pose[e0 + s0 + i];
             sKey[s0 + i] = key[i];
-            load6(Gl + 6 * (size_t)(e0 + s0 + i), G[i]);
+        } else {
+            kk[i] = 0;
+            key[i] = -1 - (s0 + i);  // unique for positions past the end
         }
     }
     __syncthreads();
