@@ -638,7 +638,7 @@ __device__ __forceinline__ void apply_tile(const DevProblem& P, const double* __
                 __builtin_amdgcn_global_load_lds(gx + 1024 * c + 16 * lane, (lds_void_ptr)(lx + 1024 * c), 16, 0, 0);
     }
     // the z and d of this lane's first pose (most lanes' positions belong to
-    // one camera run): the wave's first wait is the one for key[0]
+    // one camera run), in flight with G, V^-1 and X
     const int kp = nv > 0 ? key[0] : 0;
     double z0[6], w0[6];
     load6(v1 + 6 * (size_t)kp, z0);
