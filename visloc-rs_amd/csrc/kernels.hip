@@ -218,33 +218,63 @@ __global__ void k_camera_finalize(DevProblem P) {
 }
 
 // V_l, g_l: in-order sum over the landmark's l-order edges.
-__global__ void k_landmark_reduce(DevProblem P) {
+__global__ __launch_bounds__(256) void k_landmark_reduce(DevProblem P) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= P.num_lm_slots) return;
     if (P.wstate && !P.wstate[P.lm_win[l]].relin) return;
-    double V[6] = {0, 0, 0, 0, 0, 0}, g[3] = {0, 0, 0};
-    for (int e = P.lm_edge_ptr[l]; e < P.lm_edge_ptr[l + 1]; ++e) {
-        const size_t st = (size_t)P.lm_store[e];
-        const double* G = P.Gl64 + 6 * st;
+    double V[6] = {0, 0, 0, 0, 0, 0}, g[3] = {0, 0, 0}, Vm[6] = {0, 0, 0, 0, 0, 0};
+    const bool mixed = P.Vm != nullptr;  // V of the FP32-stored G, the one the mixed operator pairs with them
+    // The landmark's edges in batches of KB: every index of a batch is loaded,
+    // then every block it points to, then the sums run in edge order (the
+    // order of one edge at a time), so that a batch waits for two or three
+    // memory round trips instead of each edge.
+    constexpr int KB = 4;
+    const int eb = P.lm_edge_ptr[l], ee = P.lm_edge_ptr[l + 1];
+    for (int e0 = eb; e0 < ee; e0 += KB) {
+        size_t st[KB];
 #pragma unroll
-        for (int i = 0; i < 6; ++i) V[i] += G[i];
+        for (int u = 0; u < KB; ++u) st[u] = (size_t)P.lm_store[min(e0 + u, ee - 1)];
+        double G[KB][6], h[KB][3];
+        int pp[KB];
 #pragma unroll
-        for (int i = 0; i < 3; ++i) g[i] += P.hl64[3 * st + i];
+        for (int u = 0; u < KB; ++u) {
+#pragma unroll
+            for (int i = 0; i < 6; ++i) G[u][i] = P.Gl64[6 * st[u] + i];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) h[u][i] = P.hl64[3 * st[u] + i];
+            pp[u] = mixed ? P.g32_pos[st[u]] : 0;
+        }
+        float G32[KB][6];
+        if (mixed) {
+#pragma unroll
+            for (int u = 0; u < KB; ++u) {
+                const float4 a = *(const float4*)(P.Gl32 + g32_off4(pp[u]));
+                const float2 b = *(const float2*)(P.Gl32 + g32_off2(pp[u]));
+                G32[u][0] = a.x;
+                G32[u][1] = a.y;
+                G32[u][2] = a.z;
+                G32[u][3] = a.w;
+                G32[u][4] = b.x;
+                G32[u][5] = b.y;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < KB; ++u)
+            if (e0 + u < ee) {
+#pragma unroll
+                for (int i = 0; i < 6; ++i) V[i] += G[u][i];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) g[i] += h[u][i];
+                if (mixed)
+#pragma unroll
+                    for (int i = 0; i < 6; ++i) Vm[i] += (double)G32[u][i];
+            }
     }
 #pragma unroll
     for (int i = 0; i < 6; ++i) P.V[6 * (size_t)l + i] = V[i];
 #pragma unroll
     for (int i = 0; i < 3; ++i) P.gl[3 * (size_t)l + i] = g[i];
-    if (P.Vm) {  // V of the FP32-stored G, the one the mixed operator pairs with them
-        double Vm[6] = {0, 0, 0, 0, 0, 0};
-        for (int e = P.lm_edge_ptr[l]; e < P.lm_edge_ptr[l + 1]; ++e) {
-            const int pp = P.g32_pos[P.lm_store[e]];
-            const float4 a = *(const float4*)(P.Gl32 + g32_off4(pp));
-            const float2 b = *(const float2*)(P.Gl32 + g32_off2(pp));
-            const float G[6] = {a.x, a.y, a.z, a.w, b.x, b.y};
-#pragma unroll
-            for (int i = 0; i < 6; ++i) Vm[i] += (double)G[i];
-        }
+    if (mixed) {
 #pragma unroll
         for (int i = 0; i < 6; ++i) P.Vm[6 * (size_t)l + i] = Vm[i];
     }
