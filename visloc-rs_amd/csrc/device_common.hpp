@@ -220,39 +220,52 @@ __device__ __forceinline__ void schur_block_terms(const DevProblem& P, const Den
 }
 
 // 6x6 Cholesky inverse; false for a non-positive or non-finite pivot.
+// Every loop is unrolled, so that L and y stay in registers (dynamic
+// indexing would put them in scratch memory).
 __device__ inline bool cholesky_inverse6(double A[36], double out[36]) {
     double L[36];
 #pragma unroll
     for (int i = 0; i < 36; ++i) L[i] = 0.0;
+#pragma unroll
     for (int j = 0; j < 6; ++j) {
         double d = A[6 * j + j];
+#pragma unroll
         for (int k = 0; k < j; ++k) d -= L[6 * j + k] * L[6 * j + k];
         if (!(d > 0.0) || !isfinite(d)) return false;
         const double ljj = sqrt(d);
         L[6 * j + j] = ljj;
+#pragma unroll
         for (int i = j + 1; i < 6; ++i) {
             double s = A[6 * i + j];
+#pragma unroll
             for (int k = 0; k < j; ++k) s -= L[6 * i + k] * L[6 * j + k];
             L[6 * i + j] = s / ljj;
         }
     }
+#pragma unroll
     for (int c = 0; c < 6; ++c) {
         double y[6];
+#pragma unroll
         for (int i = 0; i < 6; ++i) {
             double s = (i == c) ? 1.0 : 0.0;
+#pragma unroll
             for (int k = 0; k < i; ++k) s -= L[6 * i + k] * y[k];
             y[i] = s / L[6 * i + i];
         }
+#pragma unroll
         for (int i = 5; i >= 0; --i) {
             double s = y[i];
+#pragma unroll
             for (int k = i + 1; k < 6; ++k) s -= L[6 * k + i] * y[k];
             y[i] = s / L[6 * i + i];
         }
+#pragma unroll
         for (int i = 0; i < 6; ++i) out[6 * i + c] = y[i];
     }
-    for (int i = 0; i < 36; ++i)
-        if (!isfinite(out[i])) return false;
-    return true;
+    bool finite = true;
+#pragma unroll
+    for (int i = 0; i < 36; ++i) finite = finite && isfinite(out[i]);
+    return finite;
 }
 
 }  // namespace
