@@ -971,13 +971,24 @@ __global__ __launch_bounds__(kSmallBlock) void k_pcg_init(DevProblem P, double r
     __shared__ double lds[(kSmallBlock / 64) * 2];
     double acc[2] = {0.0, 0.0};  // r.z, b.b
     for (int p = threadIdx.x; p < P.num_pose_slots; p += kSmallBlock) {
+        // the pose's b, its block and its flag load together, with no
+        // branch around a load; a fixed pose's r and z are 0
+        double bv[6], m[36];
+        const int pv = P.pose_var[p];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) bv[i] = P.b[6 * (size_t)p + i];
+#pragma unroll
+        for (int i = 0; i < 36; ++i) m[i] = P.Minv[36 * (size_t)p + i];
+        const bool var = pv >= 0;
         double r[6], z[6];
-        const bool var = P.pose_var[p] >= 0;
-        for (int i = 0; i < 6; ++i) r[i] = var ? P.b[6 * (size_t)p + i] : 0.0;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) r[i] = var ? bv[i] : 0.0;
+#pragma unroll
         for (int i = 0; i < 6; ++i) {
             double sz = 0.0;
             if (var)
-                for (int j = 0; j < 6; ++j) sz += P.Minv[36 * (size_t)p + 6 * i + j] * r[j];
+#pragma unroll
+                for (int j = 0; j < 6; ++j) sz += m[6 * i + j] * r[j];
             z[i] = sz;
         }
         for (int i = 0; i < 6; ++i) {
