@@ -375,13 +375,24 @@ __global__ __launch_bounds__(kBlock) void k_cost(DevProblem P, int trial) {
     qmatrix<double>(Quat{pq[0], pq[1], pq[2], pq[3]}, R);
     CSum cs{0.0, 0.0};
     double bad = 0.0;
-    for (int e = ct.begin + (int)threadIdx.x; e < ct.end; e += kBlock) {
-        const int l = P.edge_lm[e];
+    // the next edge's landmark and observation range load with the current
+    // edge's work (clamped to the tile's last edge, no branch), so that an
+    // edge waits for one memory round trip, not for its indices first
+    int e = ct.begin + (int)threadIdx.x;
+    const int elast = ct.end - 1;
+    int l_next = P.edge_lm[min(e, elast)], ob_next = P.edge_obs[min(e, elast)],
+        oe_next = P.edge_obs[min(e, elast) + 1];
+    for (; e < ct.end; e += kBlock) {
+        const int l = l_next, ob = ob_next, oe = oe_next;
+        const int en = min(e + kBlock, elast);
+        l_next = P.edge_lm[en];
+        ob_next = P.edge_obs[en];
+        oe_next = P.edge_obs[en + 1];
         const double X[3] = {lms[3 * (size_t)l], lms[3 * (size_t)l + 1], lms[3 * (size_t)l + 2]};
         const double xr[3] = {R[0] * X[0] + R[1] * X[1] + R[2] * X[2] + pq[4],
                               R[3] * X[0] + R[4] * X[1] + R[5] * X[2] + pq[5],
                               R[6] * X[0] + R[7] * X[1] + R[8] * X[2] + pq[6]};
-        for (int o = P.edge_obs[e]; o < P.edge_obs[e + 1]; ++o) {
+        for (int o = ob; o < oe; ++o) {
             const DevSensor& S = sensors[RIG ? P.obs_sensor[o] : 0];
             double sq;
             if (RIG && P.obs_kind && P.obs_kind[o] != OBS_PIXEL) {
