@@ -985,8 +985,11 @@ def main():
             f"{tuning['default_seconds'] * 1e3:.1f} ms for the default, 6 LM iterations) in "
             f"{time.perf_counter() - t1:.2f} s")
 
-    # HIP events around every 32nd launch of each kernel family: enough launches
-    # for the averages, few enough that recording them does not slow the solve
+    # HIP events on every 32nd launch of each kernel family: enough launches
+    # for the averages, few enough that recording them does not slow the solve.
+    # The PCG loop's kernels (product, pose, step) get the pair bound to their
+    # dispatch, so it times the kernel alone; the others are bracketed on the
+    # stream.
     dev.profile(True, stride=args.profile_stride)
     lm_iters = 0
     pcg_iters = 0

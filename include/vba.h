@@ -490,12 +490,15 @@ uint64_t vba_server_request_bytes(const vba_server* server, const vba_problem* p
 void vba_server_destroy(vba_server* server);
 
 /* ---- per-kernel device time ----------------------------------------------
- * While enabled, HIP events are recorded on the context's stream around the
- * launches of the kernel families "linearize", "schur_apply", "pcg_pose",
- * "pcg_step", "cost", "explicit_assemble" and "explicit_apply": around every
- * launch for enable = 1, around every enable-th launch of each family for
- * enable > 1 (sampling keeps the events' host cost out of the solve), never
- * for enable = 0. vba_profile_query returns the recorded launches that did
+ * While enabled, HIP events time the launches of the kernel families
+ * "linearize", "schur_apply", "pcg_pose", "pcg_step", "cost",
+ * "explicit_assemble" and "explicit_apply": every launch for enable = 1,
+ * every enable-th launch of each family for enable > 1 (sampling keeps the
+ * events' host cost out of the solve), none for enable = 0. The matrix-free
+ * PCG loop's "schur_apply", "pcg_pose" and "pcg_step" launches carry the
+ * events on their dispatch (the kernel's own start and end); the others are
+ * bracketed by events on the context's stream, which also count the wait
+ * before the dispatch. vba_profile_query returns the recorded launches that did
  * work, their summed device seconds and the algorithmic bytes they moved
  * (the minimum HBM traffic of the data layout, summed over launches). */
 void vba_profile_enable(vba_context* ctx, int32_t enable);

@@ -190,15 +190,22 @@ void launch_rhs(const DevProblem& P, hipStream_t stream);
 // scalars (beta, stop) from the step kernel's partials. resident_waves > 0
 // runs the persistent form (that many waves loop over the tiles) with the
 // same results.
+// Events bound to one kernel dispatch (hipExtLaunchKernel): they take the
+// kernel's own start and end times, so their interval leaves out the time
+// the stream spends getting to the dispatch. Null events time nothing.
+struct LaunchEvents {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
 void launch_schur_apply(const DevProblem& P, bool fp32, const double* v1, const double* v2, int32_t pcg_iter,
-                        int32_t fresh, hipStream_t stream, int32_t resident_waves = 0);
+                        int32_t fresh, hipStream_t stream, int32_t resident_waves = 0,
+                        const LaunchEvents& ev = {});
 // PCG: r = b, z = M^-1 r, x = d = 0 and the state (implicit_schur.rs:522-560).
 void launch_pcg_init(const DevProblem& P, double rtol, double atol, int32_t max_iter, double lambda,
                      hipStream_t stream);
 // d = z + beta d, q = S d from the run partials, and d.q partials.
-void launch_pcg_pose(const DevProblem& P, hipStream_t stream);
+void launch_pcg_pose(const DevProblem& P, hipStream_t stream, const LaunchEvents& ev = {});
 // alpha, x += alpha d, r -= alpha q, z = M^-1 r and (r.z, r.r) partials.
-void launch_pcg_step(const DevProblem& P, int32_t pcg_iter, hipStream_t stream);
+void launch_pcg_step(const DevProblem& P, int32_t pcg_iter, hipStream_t stream, const LaunchEvents& ev = {});
 // The scalars of iteration pcg_iter (status, residual norm) without its work.
 void launch_pcg_status(const DevProblem& P, int32_t pcg_iter, int32_t fresh, hipStream_t stream);
 // True residual b - S x in FP64 (state: true_norm).

@@ -22,6 +22,7 @@
 // q = S d, d.q partials) and a step kernel (alpha, x, r, z = M^-1 r, r.z and
 // r.r partials). Scalars never leave the device between iterations.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <type_traits>
 
@@ -1216,24 +1217,25 @@ void launch_rhs(const DevProblem& P, hipStream_t s) {
 }
 
 void launch_schur_apply(const DevProblem& P, bool fp32, const double* v1, const double* v2, int32_t pcg_iter,
-                        int32_t fresh, hipStream_t s, int32_t resident_waves) {
+                        int32_t fresh, hipStream_t s, int32_t resident_waves, const LaunchEvents& ev) {
     // The persistent form for the FP32-G operator only. The two forms do the
     // same arithmetic in the same order, but the compiler contracts a * b + c
     // into FMAs by context, and for the FP64 operator the two builds differ
     // in the last bit; the FP32-G pair is checked bit for bit (autotuner,
     // tests).
     if (fp32 && resident_waves > 0 && resident_waves < P.num_lm_tiles) {
-        hipLaunchKernelGGL(k_schur_apply_loop<float>, dim3(xcd_grid(resident_waves)), dim3(64), 0, s, P, v1, v2,
-                           (const float*)P.Gl32, (const double*)P.VinvM, P.part64, pcg_iter, fresh);
+        hipExtLaunchKernelGGL(k_schur_apply_loop<float>, dim3(xcd_grid(resident_waves)), dim3(64), 0, s, ev.start,
+                              ev.stop, 0, P, v1, v2, (const float*)P.Gl32, (const double*)P.VinvM, P.part64,
+                              pcg_iter, fresh);
         return;
     }
     const dim3 grid(xcd_grid(blocks_for(P.num_lm_tiles, kApplyWaves))), block(64 * kApplyWaves);
     if (fp32)
-        hipLaunchKernelGGL(k_schur_apply<float>, grid, block, 0, s, P, v1, v2, (const float*)P.Gl32,
-                           (const double*)P.VinvM, P.part64, pcg_iter, fresh);
+        hipExtLaunchKernelGGL(k_schur_apply<float>, grid, block, 0, s, ev.start, ev.stop, 0, P, v1, v2,
+                              (const float*)P.Gl32, (const double*)P.VinvM, P.part64, pcg_iter, fresh);
     else
-        hipLaunchKernelGGL(k_schur_apply<double>, grid, block, 0, s, P, v1, v2, (const double*)P.Gl64,
-                           (const double*)P.Vinv64, P.part64, pcg_iter, fresh);
+        hipExtLaunchKernelGGL(k_schur_apply<double>, grid, block, 0, s, ev.start, ev.stop, 0, P, v1, v2,
+                              (const double*)P.Gl64, (const double*)P.Vinv64, P.part64, pcg_iter, fresh);
 }
 
 void launch_pcg_init(const DevProblem& P, double rtol, double atol, int32_t max_iter, double lambda,
@@ -1241,13 +1243,14 @@ void launch_pcg_init(const DevProblem& P, double rtol, double atol, int32_t max_
     hipLaunchKernelGGL(k_pcg_init, dim3(1), dim3(kSmallBlock), 0, s, P, rtol, atol, max_iter, lambda);
 }
 
-void launch_pcg_pose(const DevProblem& P, hipStream_t s) {
-    hipLaunchKernelGGL(k_pcg_pose, dim3(gather_blocks(P)), dim3(kBlock), 0, s, P, (const double*)P.part64);
+void launch_pcg_pose(const DevProblem& P, hipStream_t s, const LaunchEvents& ev) {
+    hipExtLaunchKernelGGL(k_pcg_pose, dim3(gather_blocks(P)), dim3(kBlock), 0, s, ev.start, ev.stop, 0, P,
+                          (const double*)P.part64);
 }
 
-void launch_pcg_step(const DevProblem& P, int32_t pcg_iter, hipStream_t s) {
-    hipLaunchKernelGGL(k_pcg_step, dim3(blocks_for(P.num_pose_slots, (kBlock / 64) * kPosesPerWave)), dim3(kBlock),
-                       0, s, P, pcg_iter);
+void launch_pcg_step(const DevProblem& P, int32_t pcg_iter, hipStream_t s, const LaunchEvents& ev) {
+    hipExtLaunchKernelGGL(k_pcg_step, dim3(blocks_for(P.num_pose_slots, (kBlock / 64) * kPosesPerWave)),
+                          dim3(kBlock), 0, s, ev.start, ev.stop, 0, P, pcg_iter);
 }
 
 void launch_pcg_status(const DevProblem& P, int32_t pcg_iter, int32_t fresh, hipStream_t s) {

@@ -202,22 +202,43 @@ hipEvent_t take_event(Context& ctx) {
     hip_check(hipEventCreate(&e), "hipEventCreate");
     return e;
 }
+// Times every profile_stride-th launch of a kernel. Stream mode records an
+// event before and after the launch on the stream; bound mode (one kernel
+// per scope, launched with events()) has the launch bind the pair to its
+// dispatch, so the interval is the kernel's own.
 struct ProfScope {
     Context& ctx;
     int kernel;
     double bytes;
     int32_t pcg_iter;
-    hipEvent_t a = nullptr;
-    ProfScope(Context& c, int k, double b, int32_t it = -1) : ctx(c), kernel(k), bytes(b), pcg_iter(it) {
+    bool bound;
+    hipEvent_t a = nullptr, bb = nullptr;
+    ProfScope(Context& c, int k, double b, int32_t it = -1, bool bind = false)
+        : ctx(c), kernel(k), bytes(b), pcg_iter(it), bound(bind) {
         if (ctx.profile_stride > 0) {
             if (ctx.profile_seen.size() < K_COUNT) ctx.profile_seen.resize(K_COUNT, 0);
             if (ctx.profile_seen[kernel]++ % (uint64_t)ctx.profile_stride != 0) return;
             a = take_event(ctx);
-            hip_check(hipEventRecord(a, ctx.stream), "hipEventRecord");
+            if (bound) {
+                try {
+                    bb = take_event(ctx);
+                } catch (...) {
+                    ctx.event_pool.push_back(a);
+                    throw;
+                }
+            } else {
+                hip_check(hipEventRecord(a, ctx.stream), "hipEventRecord");
+            }
         }
     }
+    // the events a bound scope's launch takes (none when not sampled)
+    LaunchEvents events() const { return bound && a ? LaunchEvents{a, bb} : LaunchEvents{}; }
     ~ProfScope() {
         if (!a) return;
+        if (bound) {
+            ctx.pending.push_back({a, bb, kernel, bytes, pcg_iter});
+            return;
+        }
         hipEvent_t b = nullptr;
         if (!ctx.event_pool.empty()) {
             b = ctx.event_pool.back();
@@ -889,17 +910,17 @@ int32_t run(Context& ctx, Prepared& pr, const vba_options& o, vba_iteration_stat
                         return;
                     }
                     {
-                        ProfScope a(ctx, K_SCHUR_APPLY, bytes_apply, kk);
+                        ProfScope a(ctx, K_SCHUR_APPLY, bytes_apply, kk, true);
                         launch_schur_apply(P, mixed, P.z, P.d, kk, fresh_kk, s,
-                                           (int32_t)ctx.tuning.apply_waves_per_cu * ctx.num_cus);
+                                           (int32_t)ctx.tuning.apply_waves_per_cu * ctx.num_cus, a.events());
                     }
                     {
-                        ProfScope b(ctx, K_PCG_POSE, bytes_pcg_pose, kk);
-                        launch_pcg_pose(P, s);
+                        ProfScope b(ctx, K_PCG_POSE, bytes_pcg_pose, kk, true);
+                        launch_pcg_pose(P, s, b.events());
                     }
                     {
-                        ProfScope c(ctx, K_PCG_STEP, bytes_pcg_step, kk);
-                        launch_pcg_step(P, kk, s);
+                        ProfScope c(ctx, K_PCG_STEP, bytes_pcg_step, kk, true);
+                        launch_pcg_step(P, kk, s, c.events());
                     }
                 };
                 while (true) {

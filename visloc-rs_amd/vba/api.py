@@ -398,7 +398,7 @@ class Device:
         return PreparedProblem(self, problem, options)
 
     def profile(self, enable: bool = True, reset: bool = True, stride: int = 1):
-        """Record kernel times around every `stride`-th launch of each kernel
+        """Record the device time of every `stride`-th launch of each kernel
         family (vba_profile_enable)."""
         if reset:
             self.lib.vba_profile_reset(self.ctx)
