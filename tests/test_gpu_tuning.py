@@ -106,3 +106,39 @@ def test_prepared_solves_decline_out_of_range_options(device):
     ob = vba.default_options(max_iterations=3, robust_kernel=5)
     assert pb.solve(ob).status == abi.DECLINED
     assert pb.solve(od).status == abi.OK
+
+
+@pytest.mark.parametrize("stride", [1, 3])
+def test_profiling_keeps_the_result_and_times_each_family(device, stride):
+    """Per-kernel timing (vba_profile_enable): the PCG loop's product, pose
+    and step launches carry their events on the dispatch, the other kernels
+    are bracketed on the stream. Timing changes no result, and each family
+    reports sampled launches with positive, plausible times; no more product
+    launches than PCG iterations run (launches past the device's stop are
+    not counted)."""
+    s = vba.synthesize(abi.SYNTH_LOOP, num_cameras=80, num_landmarks=6000, seed=3, scene_scale=16.0)
+    o = loop_options(abi.SOLVER_MATRIX_FREE, abi.PRECISION_MIXED)
+    pp = device.prepare(s.problem, o)
+    device.set_tuning()
+    device.profile(False)
+    ref = pp.solve(o)
+    assert ref.status == abi.OK, ref.message
+    p0, l0 = pp.state()
+    device.profile(True, stride=stride)
+    out = pp.solve(o)
+    device.profile(False, reset=False)
+    assert out.status == abi.OK, out.message
+    p1, l1 = pp.state()
+    assert out.iterations == ref.iterations
+    assert out.result.final_cost == ref.result.final_cost
+    np.testing.assert_array_equal(p1, p0)
+    np.testing.assert_array_equal(l1, l0)
+    pcg = out.result.total_pcg_iterations
+    assert pcg > 0
+    for name in ("schur_apply", "pcg_pose", "pcg_step", "linearize", "cost"):
+        q = device.profile_query(name)
+        assert q["launches"] > 0, name
+        assert q["seconds"] > 0.0, name
+        assert 1e-7 < q["seconds"] / q["launches"] < 1e-2, (name, q)
+    assert device.profile_query("schur_apply")["launches"] <= pcg
+    pp.free()
